@@ -1,0 +1,378 @@
+"""FP64 CPU oracle for PURIFY's measurement operator -- TEST INFRASTRUCTURE ONLY.
+
+This package wraps ``oracle/liboracle.so`` (built from ``purify_oracle.c`` by ``oracle/Makefile``)
+with ctypes and numpy.  It is the checker: only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import it.  The product library under ``purify-1_amd/`` never
+imports, links or calls it.
+
+Every function restates a reference function; the C file cites reference file:line for each.
+Complex arrays are numpy complex128 (interleaved re/im doubles, as the C side expects).
+
+Parity pins (what each function is checked against) live in ``tests/test_oracle_*.py``.  The SOPT
+power-method iteration is restated from the published power-iteration algorithm; its stopping
+rule is "parity unpinned" (SOPT is not in the reference tree) and only its converged value is
+pinned, against the largest singular value of the dense operator.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+KB, GAUSS, BOX, PSWF, KBMIN, GAUSS_ALT = range(6)
+KERNELS = {"kb": KB, "gauss": GAUSS, "box": BOX, "pswf": PSWF, "kbmin": KBMIN, "gauss_alt": GAUSS_ALT}
+W_APPROX, W_BOX, W_SPHERE = range(3)
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    d, i, i64, vp = C.c_double, C.c_int, C.c_int64, C.c_void_p
+    sig = {
+        "orc_bessel_i0": (d, [d]),
+        "orc_kaiser_bessel": (d, [d, i]),
+        "orc_ft_kaiser_bessel": (d, [d, i]),
+        "orc_gaussian": (d, [d, i]),
+        "orc_ft_gaussian": (d, [d, i]),
+        "orc_pswf": (d, [d, i]),
+        "orc_ft_pswf": (d, [d, i]),
+        "orc_kernel_eval": (d, [i, i, d, i, i, d]),
+        "orc_correction2d": (i, [d, i, i, i, i, i, _dp]),
+        "orc_zero_pad": (None, [_dp, i, i, d, _cp, _cp]),
+        "orc_zero_pad_adjoint": (None, [_dp, i, i, d, _cp, _cp]),
+        "orc_fft2d": (None, [i, i, _cp, _cp, i]),
+        "orc_fft2d_raw": (None, [i, i, _cp, _cp, i]),
+        "orc_gridding_matrix_nnz": (i64, [i64, i, i, d, i, i, i, i]),
+        "orc_gridding_matrix": (i64, [_dp, _dp, _cp, i64, i, i, d, i, i, i, i, i, i, _i64p, _i32p, _cp]),
+        "orc_gridding_matrix_w": (i64, [_dp, _dp, _dp, _cp, i64, i, i, d, i, i, i, i, i, i, i, d, d,
+                                        _i64p, _i32p, _cp]),
+        "orc_projection": (None, [i, i, i, i, i, i, d, d, i, i, d, d, d, d, _cp]),
+        "orc_csr_adjoint": (None, [i64, i64, _i64p, _i32p, _cp, _i64p, _i32p, _cp]),
+        "orc_spmv": (None, [i64, _i64p, _i32p, _cp, _cp, _cp]),
+        "orc_linear_convol_1d": (None, [_cp, i, _cp, i, _cp]),
+        "orc_linear_convol_2d": (None, [_cp, i, _cp, i, _cp, i, i, _cp]),
+        "orc_generate_chirp": (None, [d, d, d, i, i, _cp]),
+        "orc_sparsify_row_dense_thres": (d, [_cp, i64, d]),
+        "orc_sparsify_row_thres_real": (d, [_dp, i64, d]),
+        "orc_row_wise_convolution": (None, [_i32p, _cp, i, _cp, i, i, _cp]),
+        "orc_wprojection_matrix_dense": (i, [i64, _i64p, _i32p, _cp, i, i, _dp, d, d, d, d, _cp, _u8p]),
+        "orc_uv_to_pixels": (None, [_dp, _dp, i64, i, d, d, i, i, d, _dp, _dp]),
+        "orc_calculate_l2_radius": (d, [i64, d, d, i]),
+        "orc_snr_to_standard_deviation": (d, [_cp, i64, d]),
+        "orc_op_create": (vp, [_dp, _dp, _dp, _cp, i64, i, i, d, i, i, i, i, i, d, d, i]),
+        "orc_op_destroy": (None, [vp]),
+        "orc_op_set_norm": (None, [vp, d]),
+        "orc_op_get_norm": (d, [vp]),
+        "orc_op_nnz": (i64, [vp]),
+        "orc_op_forward": (None, [vp, _cp, _cp]),
+        "orc_op_adjoint": (None, [vp, _cp, _cp]),
+        "orc_op_power_method": (d, [vp, i, d, vp]),
+        "orc_op_apply_G": (None, [vp, _cp, _cp]),
+        "orc_op_apply_GH": (None, [vp, _cp, _cp]),
+        "orc_omp_threads": (i, []),
+        "orc_set_omp_threads": (None, [i]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- scalar kernels
+def kernel_eval(kind, which, x, J, ftsize=1, os_=2.0):
+    L = lib()
+    return np.array([L.orc_kernel_eval(int(kind), int(which), float(t), int(J), int(ftsize), float(os_))
+                     for t in np.ravel(x)])
+
+
+def kaiser_bessel(x, J):
+    return np.array([lib().orc_kaiser_bessel(float(t), int(J)) for t in np.ravel(x)])
+
+
+def ft_kaiser_bessel(x, J):
+    return np.array([lib().orc_ft_kaiser_bessel(float(t), int(J)) for t in np.ravel(x)])
+
+
+def gaussian(x, J):
+    return np.array([lib().orc_gaussian(float(t), int(J)) for t in np.ravel(x)])
+
+
+def ft_gaussian(x, J):
+    return np.array([lib().orc_ft_gaussian(float(t), int(J)) for t in np.ravel(x)])
+
+
+def pswf(x, J):
+    return np.array([lib().orc_pswf(float(t), int(J)) for t in np.ravel(x)])
+
+
+def ft_pswf(x, J):
+    return np.array([lib().orc_ft_pswf(float(t), int(J)) for t in np.ravel(x)])
+
+
+def bessel_i0(x):
+    return np.array([lib().orc_bessel_i0(float(t)) for t in np.ravel(x)])
+
+
+# ---------------------------------------------------------------- operator pieces
+def grid_dims(ny, nx, os_):
+    return int(np.floor(ny * os_)), int(np.floor(nx * os_))
+
+
+def correction2d(os_, ny, nx, kind=KB, Ju=4, Jv=4):
+    S = np.zeros(ny * nx)
+    if lib().orc_correction2d(float(os_), ny, nx, kind, Ju, Jv, S) != 0:
+        raise RuntimeError("Incorrect input: PSWF requires a support of 6")
+    return S
+
+
+def zero_pad(S, ny, nx, os_, x):
+    Nv, Nu = grid_dims(ny, nx, os_)
+    g = np.zeros(Nv * Nu, np.complex128)
+    lib().orc_zero_pad(_d(S), ny, nx, float(os_), _c(x), g)
+    return g
+
+
+def zero_pad_adjoint(S, ny, nx, os_, g):
+    x = np.zeros(ny * nx, np.complex128)
+    lib().orc_zero_pad_adjoint(_d(S), ny, nx, float(os_), _c(g), x)
+    return x
+
+
+def fft2d(Nv, Nu, x, sign=-1, unitary=True):
+    out = np.zeros(Nv * Nu, np.complex128)
+    (lib().orc_fft2d if unitary else lib().orc_fft2d_raw)(Nv, Nu, _c(x), out, int(sign))
+    return out
+
+
+class CSR:
+    def __init__(self, rows, cols, rowptr, colidx, vals):
+        self.rows, self.cols = rows, cols
+        self.rowptr, self.colidx, self.vals = rowptr, colidx, vals
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+    def dot(self, x):
+        y = np.zeros(self.rows, np.complex128)
+        lib().orc_spmv(self.rows, self.rowptr, self.colidx, self.vals, _c(x), y)
+        return y
+
+    def adjoint(self):
+        tr = np.zeros(self.cols + 1, np.int64)
+        tc = np.zeros(max(self.nnz, 1), np.int32)
+        tv = np.zeros(max(self.nnz, 1), np.complex128)
+        lib().orc_csr_adjoint(self.rows, self.cols, self.rowptr, self.colidx, self.vals, tr, tc, tv)
+        return CSR(self.cols, self.rows, tr, tc[: self.nnz], tv[: self.nnz])
+
+    def todense(self):
+        D = np.zeros((self.rows, self.cols), np.complex128)
+        for r in range(self.rows):
+            for k in range(self.rowptr[r], self.rowptr[r + 1]):
+                D[r, self.colidx[k]] += self.vals[k]
+        return D
+
+
+def gridding_matrix(u, v, weights, ny, nx, os_, kind=KB, Jeval_u=4, Jeval_v=4, Ju=4, Jv=4, literal_phase=False):
+    M = len(u)
+    nnz = lib().orc_gridding_matrix_nnz(M, ny, nx, float(os_), Ju, Jv, 0, 0)
+    rp = np.zeros(M + 1, np.int64)
+    ci = np.zeros(max(nnz, 1), np.int32)
+    vv = np.zeros(max(nnz, 1), np.complex128)
+    lib().orc_gridding_matrix(_d(u), _d(v), _c(weights), M, ny, nx, float(os_), kind, Jeval_u, Jeval_v, Ju, Jv,
+                              int(literal_phase), rp, ci, vv)
+    Nv, Nu = grid_dims(ny, nx, os_)
+    return CSR(M, Nv * Nu, rp, ci[:nnz], vv[:nnz])
+
+
+def gridding_matrix_w(u, v, w, weights, ny, nx, os_, kind=KB, Jeval_u=4, Jeval_v=4, Ju=4, Jv=4, Jw=6,
+                      wkind=W_APPROX, cellx=1.0, celly=1.0):
+    M = len(u)
+    nnz = lib().orc_gridding_matrix_nnz(M, ny, nx, float(os_), Ju, Jv, 1, Jw)
+    rp = np.zeros(M + 1, np.int64)
+    ci = np.zeros(max(nnz, 1), np.int32)
+    vv = np.zeros(max(nnz, 1), np.complex128)
+    lib().orc_gridding_matrix_w(_d(u), _d(v), _d(w), _c(weights), M, ny, nx, float(os_), kind, Jeval_u, Jeval_v,
+                                Ju, Jv, Jw, wkind, float(cellx), float(celly), rp, ci, vv)
+    Nv, Nu = grid_dims(ny, nx, os_)
+    return CSR(M, Nv * Nu, rp, ci[:nnz], vv[:nnz])
+
+
+def projection(u, v, w, kind=KB, Jeval_u=4, Ju=4, Jv=4, Jw=6, wkind=W_APPROX, cellx=1.0, celly=1.0, nx=16, ny=16,
+               os_=2.0):
+    out = np.zeros((Jv + Jw - 1) * (Ju + Jw - 1), np.complex128)
+    lib().orc_projection(kind, Jeval_u, Ju, Jv, Jw, wkind, float(cellx), float(celly), nx, ny, float(os_),
+                         float(u), float(v), float(w), out)
+    return out.reshape(Jv + Jw - 1, Ju + Jw - 1)
+
+
+def linear_convol_1d(f, g_padded):
+    f, g = _c(f), _c(g_padded)
+    out = np.zeros(len(g) - len(f) + 1, np.complex128)
+    lib().orc_linear_convol_1d(f, len(f), g, len(g), out)
+    return out
+
+
+def linear_convol_2d(fu, fv, g):
+    fu, fv, g = _c(fu), _c(fv), _c(g)
+    Jgv, Jgu = g.shape
+    out = np.zeros((len(fv) + Jgv - 1, len(fu) + Jgu - 1), np.complex128)
+    lib().orc_linear_convol_2d(fu, len(fu), fv, len(fv), np.ascontiguousarray(g).ravel(), Jgv, Jgu, out.ravel())
+    return out
+
+
+def generate_chirp(w_rate, cell_x, cell_y, x_size, y_size):
+    out = np.zeros(x_size * y_size, np.complex128)
+    lib().orc_generate_chirp(float(w_rate), float(cell_x), float(cell_y), x_size, y_size, out)
+    return out
+
+
+def sparsify_row_dense_thres(row, energy):
+    r = _c(row)
+    return lib().orc_sparsify_row_dense_thres(r, len(r), float(energy))
+
+
+def sparsify_row_thres(absvals, energy):
+    a = _d(absvals)
+    return lib().orc_sparsify_row_thres_real(a, len(a), float(energy))
+
+
+def row_wise_convolution(g_idx, g_val, chirp_dense, x_size, y_size):
+    out = np.zeros(x_size * y_size, np.complex128)
+    lib().orc_row_wise_convolution(np.ascontiguousarray(g_idx, np.int32), _c(g_val), len(g_idx), _c(chirp_dense),
+                                   x_size, y_size, out)
+    return out
+
+
+def wprojection_matrix_dense(G: CSR, x_size, y_size, w, cell_x, cell_y, e_chirp, e_wproj):
+    N = x_size * y_size
+    out = np.zeros(G.rows * N, np.complex128)
+    mask = np.zeros(G.rows * N, np.uint8)
+    lib().orc_wprojection_matrix_dense(G.rows, G.rowptr, G.colidx, _c(G.vals), x_size, y_size, _d(w),
+                                       float(cell_x), float(cell_y), float(e_chirp), float(e_wproj), out, mask)
+    return out.reshape(G.rows, N), mask.reshape(G.rows, N).astype(bool)
+
+
+def uv_to_pixels(u, v, units, cell_x, cell_y, nx, ny, os_):
+    u, v = _d(u), _d(v)
+    up, vp = np.zeros_like(u), np.zeros_like(v)
+    lib().orc_uv_to_pixels(u, v, len(u), int(units), float(cell_x), float(cell_y), nx, ny, float(os_), up, vp)
+    return up, vp
+
+
+def calculate_l2_radius(y_size, sigma=0.0, n_sigma=2.0, distribution="chi"):
+    return lib().orc_calculate_l2_radius(int(y_size), float(sigma), float(n_sigma), int(distribution == "chi^2"))
+
+
+def snr_to_standard_deviation(y0, snr=30.0):
+    y = _c(y0)
+    return lib().orc_snr_to_standard_deviation(y, len(y), float(snr))
+
+
+class Operator:
+    """Phi = G F Z S / norm with the explicit CSR G and G^H (operators.h:298-414)."""
+
+    def __init__(self, u_px, v_px, w, weights, ny, nx, os_=2.0, kernel=KB, Ju=4, Jv=4, w_term=False, Jw=6,
+                 cellx=1.0, celly=1.0, literal_phase=False, power_iters=0, power_tol=1e-4):
+        self.ny, self.nx, self.os = ny, nx, float(os_)
+        self.M = len(u_px)
+        w = np.zeros(self.M) if w is None else w
+        weights = np.ones(self.M, np.complex128) if weights is None else weights
+        self._h = lib().orc_op_create(_d(u_px), _d(v_px), _d(w), _c(weights), self.M, ny, nx, self.os, int(kernel),
+                                      Ju, Jv, int(w_term), Jw, float(cellx), float(celly), int(literal_phase))
+        if not self._h:
+            raise RuntimeError("Incorrect input: PSWF requires a support of 6")
+        self.norm = 1.0
+        if power_iters > 0:
+            self.norm = self.power_method(power_iters, power_tol)
+            lib().orc_op_set_norm(self._h, self.norm)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_op_destroy(h)
+            self._h = None
+
+    @property
+    def nnz(self):
+        return lib().orc_op_nnz(self._h)
+
+    def set_norm(self, norm):
+        self.norm = float(norm)
+        lib().orc_op_set_norm(self._h, self.norm)
+
+    def forward(self, x):
+        y = np.zeros(self.M, np.complex128)
+        lib().orc_op_forward(self._h, _c(x), y)
+        return y
+
+    def adjoint(self, y):
+        x = np.zeros(self.nx * self.ny, np.complex128)
+        lib().orc_op_adjoint(self._h, _c(y), x)
+        return x
+
+    def apply_G(self, grid):
+        y = np.zeros(self.M, np.complex128)
+        lib().orc_op_apply_G(self._h, _c(grid), y)
+        return y
+
+    def apply_GH(self, y):
+        Nv, Nu = grid_dims(self.ny, self.nx, self.os)
+        g = np.zeros(Nv * Nu, np.complex128)
+        lib().orc_op_apply_GH(self._h, _c(y), g)
+        return g
+
+    def power_method(self, iters, tol, start=None, normalised=False):
+        """||Phi||_2 of the raw operator, or of Phi/norm when normalised=True."""
+        old = self.norm
+        if not normalised:
+            lib().orc_op_set_norm(self._h, 1.0)
+        if start is None:
+            r = lib().orc_op_power_method(self._h, int(iters), float(tol), None)
+        else:
+            s = _c(start)
+            r = lib().orc_op_power_method(self._h, int(iters), float(tol), s.ctypes.data_as(C.c_void_p))
+        lib().orc_op_set_norm(self._h, old)
+        return r
+
+
+def omp_threads():
+    return lib().orc_omp_threads()
+
+
+def set_omp_threads(n):
+    lib().orc_set_omp_threads(int(n))
