@@ -1,0 +1,226 @@
+#!/usr/bin/env python
+"""Throughput of the measurement-operator pair Phi x, Phi^dagger y on one MI355X.
+
+Workload (default): 2048 x 2048 image, oversample 2 (4096 x 4096 grid), Kaiser-Bessel 6 x 6,
+M = 4e6 visibilities with (u, v) ~ N(0, pi/3) rad clipped to [-pi, pi], no w-term, unit weights;
+synthetic inputs from the library's seeded generator (seed 1002).  A step is one Phi x followed by
+one Phi^dagger y on inputs resident in HBM.  value = M * steps / elapsed (visibilities per second)
+for the complex64 operator; the complex128 operator is timed on the same inputs for the FP32/FP64
+speed ratio.  Accuracy is the relative L2 error against the FP64 oracle on the same inputs.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--workload north_star|c2|c1]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "purify-1_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+WORKLOADS = {
+    # name: (image side, J, M, seed)
+    "north_star": (2048, 6, 4_000_000, 1002),
+    "c2": (1024, 6, 2_000_000, 1001),
+    "c1": (256, 4, 10_000, 1000),
+}
+
+METRIC = "Φ+Φ† vis/s FP32 vs FP64 (rel.err≤1e-5, frac HBM peak); served p50/p99 latency"
+
+
+def model_bytes_pair(n, Ng, M, e, r):
+    """Stage-boundary compulsory traffic of one Phi + Phi^dagger pair."""
+    return 3 * e * n * n + 11 * e * Ng + (4 * r + 4 * e) * M
+
+
+def kernel_alg_bytes(name, n, Nu, Nv, M, e, r):
+    """Algorithmic bytes of one launch of each kernel: its inputs read once and outputs written once."""
+    Ng = Nu * Nv
+    win = n * Nu  # grid rows that hold the image (forward) or are kept by the crop (adjoint)
+    vis = M * (2 * r + 2 * e)  # u, v, weight and value per visibility
+    table = {
+        "fft_rows_fwd_sz": n * n * e + win * e,
+        "fft_rows_adj_crop": win * e + n * n * e,
+        "degrid": Ng * e + vis,
+        "grid_tiles": vis + Ng * e,
+        "fft_cols_stepB": 2 * Ng * e,
+        "fft_cols": 2 * Ng * e,
+    }
+    if name == "fft_cols_stepA":
+        return None  # depends on direction (window read forward, full read adjoint): averaged below
+    return table.get(name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=5)
+    ap.add_argument("--workload", default="north_star", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-pairs", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one operator per GPU, no "
+              "collectives); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from purify_amd import measurementoperator, utilities
+
+    n, J, M, seed = WORKLOADS[args.workload]
+    os_ = 2.0
+    Nu = Nv = int(n * os_)
+    Ng = Nu * Nv
+    dev = torch.device("cuda:0")
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+
+    u_rad, v_rad = utilities.synth_uv(seed, M, np.pi / 3)
+    u = utilities.radians_to_pixels(u_rad, n, os_)
+    v = utilities.radians_to_pixels(v_rad, n, os_)
+    x_h = utilities.synth_image(seed, n, n, 20)
+    y_h = utilities.synth_cnormal(seed + 1, M)
+    log(f"[bench] workload {args.workload}: n={n} grid={Nu} J={J} M={M}")
+
+    res = {}
+    outs = {}
+    for prec, steps in (("c64", args.steps), ("c128", args.fp64_steps)):
+        t0 = time.perf_counter()
+        lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, os_, 0, 1e-4, "kb", J, J,
+                                                         precision=prec)
+        setup = time.perf_counter() - t0
+        op = lt.op
+        dt = torch.complex64 if prec == "c64" else torch.complex128
+        x = torch.from_numpy(x_h).to(dt).to(dev)
+        yin = torch.from_numpy(y_h).to(dt).to(dev)
+        y = torch.empty(M, dtype=dt, device=dev)
+        xo = torch.empty(n * n, dtype=dt, device=dev)
+        for _ in range(args.warmup):
+            op.forward(x, out=y)
+            op.adjoint_apply(yin, out=xo)
+        torch.cuda.synchronize()
+        # per-step latencies with events on torch's current stream (the stream the kernels run on)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev[0].record()
+        for k in range(steps):
+            op.forward(x, out=y)
+            op.adjoint_apply(yin, out=xo)
+            ev[k + 1].record()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        lat = [ev[k].elapsed_time(ev[k + 1]) for k in range(steps)]
+        # per-kernel durations (HIP events around each launch, same stream)
+        op.profile(True)
+        for _ in range(max(3, min(steps, 10))):
+            op.forward(x, out=y)
+            op.adjoint_apply(yin, out=xo)
+        torch.cuda.synchronize()
+        table = op.profile_table()
+        op.profile(False)
+        outs[prec] = (y.cpu().numpy().astype(np.complex128), xo.cpu().numpy().astype(np.complex128))
+        res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
+                         value=M * steps / elapsed, lat=lat, table=table)
+        log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s")
+        del lt, op, x, yin, y, xo
+        torch.cuda.empty_cache()
+
+    r64 = res["c64"]
+    e, r = 8, 4
+    # dominant kernel of the complex64 pair
+    tab = r64["table"]
+    dom = max(tab, key=lambda k: tab[k][0])
+    dom_ms = tab[dom][0] / tab[dom][1]
+    b = kernel_alg_bytes(dom, n, Nu, Nv, M, e, r)
+    if b is None:  # step A: forward reads the image window, adjoint the full grid
+        b = ((n * Nu * e + Ng * e) + (2 * Ng * e)) / 2
+    achieved = b / (dom_ms * 1e-3)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom, "kernel_ms": dom_ms,
+                "alg_bytes_per_launch": b}
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr.get(dom)
+
+    pair_bytes = model_bytes_pair(n, Ng, M, e, r)
+    rel_fp32_fp64 = {
+        "forward": float(np.linalg.norm(outs["c64"][0] - outs["c128"][0]) / np.linalg.norm(outs["c128"][0])),
+        "adjoint": float(np.linalg.norm(outs["c64"][1] - outs["c128"][1]) / np.linalg.norm(outs["c128"][1])),
+    }
+
+    cpu = None
+    acc_oracle = None
+    if not args.no_cpu:
+        import oracle
+        threads = oracle.omp_threads()
+        log(f"[bench] cpu baseline: FP64 oracle, explicit CSR G/G^H + OpenMP SpMV, {threads} threads")
+        t0 = time.perf_counter()
+        orc = oracle.Operator(u, v, None, None, n, n, os_, oracle.KB, J, J)
+        build_s = time.perf_counter() - t0
+        times = []
+        for _ in range(args.cpu_pairs):
+            t0 = time.perf_counter()
+            yo = orc.forward(x_h)
+            xo = orc.adjoint(y_h)
+            times.append(time.perf_counter() - t0)
+        t_pair = statistics.median(times)
+        cpu = {"value": M / t_pair, "unit": "vis/s", "cores": threads, "kind": "port",
+               "sample": f"full workload, {args.cpu_pairs} pairs after a {build_s:.1f} s CSR build; "
+                         f"median pair {t_pair:.3f} s"}
+        acc_oracle = {
+            "c64_forward": float(np.linalg.norm(outs["c64"][0] - yo) / np.linalg.norm(yo)),
+            "c64_adjoint": float(np.linalg.norm(outs["c64"][1] - xo) / np.linalg.norm(xo)),
+            "c128_forward": float(np.linalg.norm(outs["c128"][0] - yo) / np.linalg.norm(yo)),
+            "c128_adjoint": float(np.linalg.norm(outs["c128"][1] - xo) / np.linalg.norm(xo)),
+        }
+        log(f"[bench] cpu: {t_pair:.3f} s/pair; rel err vs oracle {acc_oracle}")
+        del orc
+
+    lat = sorted(r64["lat"])
+    out = {
+        "metric": METRIC,
+        "value": r64["value"],
+        "unit": "vis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"phi_pair_{args.workload}", "image": n, "grid": Nu, "oversample": os_, "kernel": "kb",
+                   "J": J, "M": M, "w_term": False, "uv_sigma_rad": round(np.pi / 3, 6), "seed": seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "pair_frac_hbm_peak": pair_bytes / (r64["ms"] * 1e-3) / HBM_PEAK,
+        "pair_model_bytes": pair_bytes,
+        "fp64": {"value": res["c128"]["value"], "ms_per_step": res["c128"]["ms"],
+                 "fp32_speedup": res["c128"]["ms"] / r64["ms"],
+                 "pair_frac_hbm_peak": model_bytes_pair(n, Ng, M, 16, 8) / (res["c128"]["ms"] * 1e-3) / HBM_PEAK},
+        "rel_err_fp32_vs_fp64_gpu": rel_fp32_fp64,
+        "rel_err_vs_oracle": acc_oracle,
+        "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
+        "setup_s": {"c64": r64["setup_s"], "c128": res["c128"]["setup_s"]},
+        "kernels_ms_per_launch": {k: tab[k][0] / tab[k][1] for k in tab},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/*
+ * purify_amd.h -- C-ABI of the MI355X-native PURIFY measurement operator Phi = W.G.F.Z.S.
+ *
+ * Drop-in boundary: the entry points below are what the reference's operator factory would bind
+ * to build and apply Phi.  Plain pointers and sizes only; no C++ or torch types cross this line.
+ * References are file:line in lao19881213/purify-1 (PURIFY 2.0.0).
+ *
+ * Conventions
+ *   - Complex data is interleaved (re, im).  A PF_C64 operator works on float pairs on the
+ *     device, a PF_C128 operator on double pairs.  Host-side buffers (PF_HOST) are always
+ *     complex128, the type of the reference's Vector<t_complex>; the library converts.
+ *   - Image vectors have imsizey*imsizex entries, index j*imsizex + i (operators.h:163).
+ *     Visibility vectors have M entries in the caller's order.
+ *   - Every function returns PF_OK (0) or a negative pf_status; pf_last_error() gives the
+ *     message of the last failure on the calling thread.  Construction errors carry the
+ *     reference's runtime_error texts.
+ *   - An operator is immutable after creation except for its norm.  Applies on one handle must
+ *     be externally serialised (one workspace per handle); different handles may be used
+ *     concurrently from different threads/streams.
+ */
+#ifndef PURIFY_AMD_H
+#define PURIFY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pf_op pf_op;
+typedef void *pf_stream; /* a hipStream_t; NULL = the null stream */
+
+enum pf_status {
+  PF_OK = 0,
+  PF_ERR_ARG = -1,      /* invalid argument */
+  PF_ERR_HIP = -2,      /* HIP runtime failure */
+  PF_ERR_UNSUPPORTED = -3,
+  PF_ERR_NOMEM = -4,
+  PF_ERR_STATE = -5
+};
+
+enum pf_precision { PF_C64 = 0, PF_C128 = 1 };
+enum pf_location { PF_HOST = 0, PF_DEVICE = 1 };
+/* kernels::kernel, cpp/purify/kernels.h:15 */
+enum pf_kernel { PF_KB = 0, PF_GAUSS = 1, PF_BOX = 2, PF_PSWF = 3, PF_KBMIN = 4, PF_GAUSS_ALT = 5 };
+/* utilities::vis_units, cpp/purify/utilities.h:14 */
+enum pf_units { PF_LAMBDA = 0, PF_RADIANS = 1, PF_PIXELS = 2 };
+
+/* Operator description.  Replaces the argument list of
+ *   measurementoperator::init_degrid_operator_2d(u, v, w, weights, imsizey, imsizex,
+ *       oversample_ratio, power_iters, power_tol, kernel, Ju, Jv, w_term, Jw, cellx, celly)
+ *   (cpp/purify/operators.h:373-396) and of its vis_params overload (operators.h:398-414),
+ * selected by `units` (PF_PIXELS = the raw-vector overload). */
+typedef struct pf_op_desc {
+  int32_t imsizey, imsizex;
+  double oversample_ratio; /* grid is floor(os*imsizey) x floor(os*imsizex) */
+  int32_t kernel;          /* enum pf_kernel */
+  int32_t Ju, Jv;
+  int32_t w_term; /* 0: plain KB gridding (operators.cc:8-46); 1: projection rows (operators.cc:49-99) */
+  int32_t Jw;
+  double cellx, celly; /* arcsec; used by unit conversion and the w-kernel */
+  int32_t units;       /* enum pf_units of u, v (w is always in wavelengths) */
+  int32_t precision;   /* enum pf_precision */
+  int64_t M;
+  const double *u, *v, *w; /* host, length M (w may be NULL when w_term == 0) */
+  const double *weights;   /* host complex128, length M; NULL = unit weights */
+  int32_t power_iters;     /* power-method iterations for the normalisation (operators.h:390) */
+  double power_tol;
+  int32_t tile_u, tile_v; /* gridding tile shape; 0 = library default */
+} pf_op_desc;
+
+/* Fills a descriptor with the reference's defaults: os 2, kb, Ju = Jv = 4, w_term 0, Jw 6,
+ * cells 1, units pixels, power_iters 100, power_tol 1e-4, complex64. */
+void pf_op_desc_init(pf_op_desc *d);
+
+/* Build Phi: copies and sorts the coordinates, uploads them, builds twiddles and the correction
+ * vectors, then (power_iters > 0) normalises by the power-method norm computed on the GPU with
+ * FP64 reductions.  Replaces measurementoperator::init_degrid_operator_2d (operators.h:373-414)
+ * and factory::measurement_operator_factory's gpu_serial branch (measurement_operator_factory.h:
+ * 38-96). */
+int pf_op_create(const pf_op_desc *desc, pf_op **out);
+int pf_op_destroy(pf_op *op);
+
+/* y = Phi x (operators.h:393, the LinearTransform's direct operator). */
+int pf_op_forward(pf_op *op, const void *x, void *y, int x_loc, int y_loc, pf_stream stream);
+/* x = Phi^dagger y (operators.h:394, the adjoint operator). */
+int pf_op_adjoint(pf_op *op, const void *y, void *x, int y_loc, int x_loc, pf_stream stream);
+
+/* Power method on Phi^dagger Phi (SOPT power_method as called at operators.h:390-391) over the
+ * operator as it currently is (including its norm).  start: host complex128 of imsizey*imsizex,
+ * or NULL for all ones.  Iterates in the operator's precision, reduces norms in FP64. */
+int pf_op_power_method(pf_op *op, int32_t iters, double tol, const double *start, double *norm_out);
+/* init_normalise (operators.h:185-189): throws (PF_ERR_ARG) unless norm > 0. */
+int pf_op_set_norm(pf_op *op, double norm);
+int pf_op_get_norm(const pf_op *op, double *norm);
+
+typedef struct pf_op_info {
+  int32_t imsizey, imsizex, Nv, Nu, Ju, Jv, precision, w_term;
+  int64_t M, nnz_w;   /* nnz_w: stored projection-row entries when w_term */
+  int32_t tile_u, tile_v, n_tiles;
+  int64_t tile_list_len; /* total entries of the gridding tile lists */
+  double norm;
+  int64_t device_bytes;  /* HBM the handle holds */
+} pf_op_info;
+int pf_op_get_info(const pf_op *op, pf_op_info *info);
+
+/* ---- stage access (device pointers, op precision), for tests and per-kernel timing ---- */
+/* grid = F Z S x without the norm (operators.h:263-296, base_padding_and_FFT_2d direct). */
+int pf_op_apply_FZ(pf_op *op, const void *x_dev, void *grid_dev, pf_stream stream);
+/* x = S^H Z^H F^H grid without the norm. */
+int pf_op_apply_FZ_adjoint(pf_op *op, const void *grid_dev, void *x_dev, pf_stream stream);
+/* y = G grid and grid = G^H y without the norm (operators.h:123-145). */
+int pf_op_apply_G(pf_op *op, const void *grid_dev, void *y_dev, pf_stream stream);
+int pf_op_apply_GH(pf_op *op, const void *y_dev, void *grid_dev, pf_stream stream);
+
+/* Unitary 2-D FFT of an Nv x Nu grid (operators.h:209-261): sign -1 is FFTW_FORWARD / sqrt(N),
+ * +1 is FFTW_BACKWARD / sqrt(N).  Powers of two from 2 to 16384 per axis.  Device pointers of the
+ * given precision; in == out is allowed.  workspace: NULL or a device buffer of Nv*Nu complex. */
+int pf_fft2d(int32_t Nv, int32_t Nu, int32_t precision, int32_t sign, const void *in_dev, void *out_dev,
+             void *workspace_dev, pf_stream stream);
+
+/* Evaluates a gridding kernel (which = 0) or its correction FT (which = 1) on the GPU exactly as the
+ * operator's kernels do (kernels.cc:233-323), for n host points; results to host.  For which = 1
+ * the argument is the pixel position r and the function is FT(r / ftsize - 0.5). */
+int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ftsize, int32_t precision,
+                   const double *x_host, double *out_host, int64_t n);
+
+/* ---- per-kernel timing (no counterpart in the reference, whose operators log at construction only,
+ *      operators.h:274-290) ---- */
+/* on = 1 enables (and clears) HIP-event timing of every kernel the operator launches, recorded on the
+ * launching stream; on = 0 disables it. */
+int pf_op_profile_enable(pf_op *op, int32_t on);
+/* Entry idx of the timing table (kernel name, summed milliseconds, launch count).  Synchronises on the
+ * recorded events.  Returns PF_ERR_ARG past the last entry. */
+int pf_op_profile_get(pf_op *op, int32_t idx, char *name, int32_t name_len, double *total_ms, int64_t *count);
+
+/* ---- synthetic inputs (the seeded generator of the benchmark and tests) ---- */
+/* mt19937_64 seeded with `seed`, Box-Muller normals.  uv ~ N(0, sigma) rad clipped to [-pi, pi]. */
+int pf_synth_uv(uint64_t seed, int64_t M, double sigma, double *u, double *v);
+/* Sum of nsrc Gaussian sources: centres U[0,n)^2, amplitude U[0.1,1], sigma U[1,8] px. */
+int pf_synth_image(uint64_t seed, int32_t ny, int32_t nx, int32_t nsrc, double *img_complex128);
+/* Complex standard normal vector (re, im each N(0, 1/2) * scale). */
+int pf_synth_cnormal(uint64_t seed, int64_t n, double scale, double *out_complex128);
+
+/* ---- errors ---- */
+const char *pf_last_error(void);
+/* Library version string. */
+const char *pf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

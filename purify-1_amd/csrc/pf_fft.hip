@@ -1,0 +1,498 @@
+// pf_fft.hip -- 2-D FFT F on the oversampled grid (replaces FFTW in cpp/purify/operators.h:209-261).
+//
+// Each 1-D line is a Stockham auto-sort transform (Govindaraju et al., "High Performance Discrete
+// Fourier Transforms on Graphics Processors", SC 2008): stage s with radix R and sub-transform
+// length Ns reads x[j + r N/R], twiddles by W^(r (j mod Ns) N / (Ns R)), does an R-point DFT in
+// registers and writes x[(j - j mod Ns) R + j mod Ns + r Ns].  Every thread holds 16 elements; the
+// first stage loads straight from HBM and the last stores straight to HBM, the stages in between
+// exchange through LDS.  Radices are 16 (plus one 2/4/8 stage for odd log2 N).
+//
+// Row pass: one or more grid rows per block (contiguous lines).  The forward operator's row pass
+// reads an image row, multiplies the separable correction S, zero-pads (Z) in registers and writes
+// only the ny non-zero grid rows; the adjoint's row pass reads those rows back and writes the
+// cropped, corrected image.
+// Column pass: a block owns C adjacent columns (512 B row segments) and the lines are columns.
+// Up to 512 rows it is one pass; beyond, Nv = N1*N2 is split into an N1-point pass over rows of
+// stride N2 (with the W_Nv^(r2 k1) twiddle) and an N2-point pass over contiguous row blocks.
+#include "pf_internal.h"
+
+namespace pf {
+
+LinePlan make_line_plan(int N) {
+  LinePlan p;
+  p.N = N;
+  int m = 0;
+  while ((1 << m) < N) ++m;
+  if ((1 << m) != N || N < 2) throw Error(-3, "FFT length must be a power of two >= 2");
+  p.log2N = m;
+  if (N <= FFT_E) {
+    p.nst = 1;
+    p.rad[0] = N;
+    return p;
+  }
+  const int n16 = m / 4, rem = m % 4;
+  int k = 0;
+  if (rem) p.rad[k++] = 1 << rem;
+  for (int i = 0; i < n16; ++i) p.rad[k++] = 16;
+  p.nst = k;
+  return p;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Line transform core.
+//   v: the thread's E elements; t: thread index within the line; T = N/E threads per line.
+//   lds: this line's LDS base, es: LDS element stride.  tw: exp(-2 pi i m / Lmax), twm = Lmax/N.
+//   load(idx) / store(idx, value) access the line's element idx in HBM.
+// ---------------------------------------------------------------------------------------------
+template <typename R, int E, int RAD, bool INV>
+__device__ inline void stage_bfly(cpx<R> (&v)[E], int t, int T, int N, int Ns, const cpx<R>* __restrict__ tw,
+                                  int twm) {
+#pragma unroll
+  for (int b = 0; b < E / RAD; ++b) {
+    const int j = t + b * T;
+    const int k = j & (Ns - 1);
+    if (Ns > 1) {
+      const int step = k * (N / (Ns * RAD)) * twm;
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) {
+        const cpx<R> w = tw[step * r];
+        v[b * RAD + r] = INV ? cmulc(v[b * RAD + r], w) : cmul(v[b * RAD + r], w);
+      }
+    }
+    DFT<R, RAD, INV>::run(&v[b * RAD]);
+  }
+}
+
+template <typename R, int E, int RAD>
+__device__ inline void stage_read_lds(cpx<R> (&v)[E], int t, int T, int N, const cpx<R>* lds, int es) {
+#pragma unroll
+  for (int b = 0; b < E / RAD; ++b) {
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[(j + r * (N / RAD)) * es];
+  }
+}
+
+template <typename R, int E, int RAD>
+__device__ inline void stage_write_lds(const cpx<R> (&v)[E], int t, int T, int Ns, cpx<R>* lds, int es) {
+#pragma unroll
+  for (int b = 0; b < E / RAD; ++b) {
+    const int j = t + b * T;
+    const int k = j & (Ns - 1);
+    const int o = (j - k) * RAD + k;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) lds[(o + r * Ns) * es] = v[b * RAD + r];
+  }
+}
+
+template <typename R, int E, int RAD, class Load>
+__device__ inline void stage_load(cpx<R> (&v)[E], int t, int T, int N, Load& load) {
+#pragma unroll
+  for (int b = 0; b < E / RAD; ++b) {
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = load(j + r * (N / RAD));
+  }
+}
+
+template <typename R, int E, int RAD, class Store>
+__device__ inline void stage_store(const cpx<R> (&v)[E], int t, int T, int Ns, Store& store) {
+#pragma unroll
+  for (int b = 0; b < E / RAD; ++b) {
+    const int j = t + b * T;
+    const int k = j & (Ns - 1);
+    const int o = (j - k) * RAD + k;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) store(o + r * Ns, v[b * RAD + r]);
+  }
+}
+
+// One stage: (load from HBM | read LDS) -> butterflies -> (store to HBM | write LDS).
+template <typename R, int E, int RAD, bool INV, class Load, class Store>
+__device__ inline void run_stage(cpx<R> (&v)[E], int s, int nst, int t, int T, int N, int Ns, cpx<R>* lds, int es,
+                                 const cpx<R>* tw, int twm, Load& load, Store& store) {
+  if (s == 0) {
+    stage_load<R, E, RAD>(v, t, T, N, load);
+  } else {
+    __syncthreads();  // previous stage's LDS writes are visible
+    stage_read_lds<R, E, RAD>(v, t, T, N, lds, es);
+  }
+  stage_bfly<R, E, RAD, INV>(v, t, T, N, Ns, tw, twm);
+  if (s == nst - 1) {
+    stage_store<R, E, RAD>(v, t, T, Ns, store);
+  } else {
+    __syncthreads();  // every thread has read this stage's inputs before anyone overwrites them
+    stage_write_lds<R, E, RAD>(v, t, T, Ns, lds, es);
+  }
+}
+
+template <typename R, int E, bool INV, class Load, class Store>
+__device__ inline void fft_line(const LinePlan& pl, int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm, Load& load,
+                                Store& store) {
+  const int N = pl.N, T = N / E;
+  cpx<R> v[E];
+  int Ns = 1;
+  for (int s = 0; s < pl.nst; ++s) {
+    const int rad = pl.rad[s];
+    if constexpr (E >= 16) {
+      if (rad == 16) {
+        run_stage<R, E, 16, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
+        Ns *= 16;
+        continue;
+      }
+    }
+    if constexpr (E >= 8) {
+      if (rad == 8) {
+        run_stage<R, E, 8, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
+        Ns *= 8;
+        continue;
+      }
+    }
+    if constexpr (E >= 4) {
+      if (rad == 4) {
+        run_stage<R, E, 4, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
+        Ns *= 4;
+        continue;
+      }
+    }
+    run_stage<R, E, 2, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
+    Ns *= 2;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row pass
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+struct RowArgs {
+  LinePlan pl;
+  const cpx<R>* in;
+  cpx<R>* out;
+  const cpx<R>* tw;
+  int twm, Nu, nlines, lpb;
+  SZ<R> sz;
+};
+
+template <typename R, int E, bool INV, int MODE>
+__global__ void __launch_bounds__(512) k_fft_rows(RowArgs<R> a) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int N = a.pl.N, T = N / E;
+  const int l = threadIdx.x / T, t = threadIdx.x - l * T;
+  const int line = blockIdx.x * a.lpb + l;
+  const bool valid = line < a.nlines;
+  cpx<R>* lds = smem + l * N;
+  const SZ<R>& sz = a.sz;
+  auto load = [&](int idx) -> cpx<R> {
+    if (!valid) return mk<R>(R(0), R(0));
+    if constexpr (MODE == ROW_FWD_SZ) {
+      const int i = idx - sz.x0;
+      if ((unsigned)i >= (unsigned)sz.nx) return mk<R>(R(0), R(0));
+      const R s = sz.sy[line] * sz.sx[i] * sz.scale;
+      return scale(sz.img_in[(int64_t)line * sz.nx + i], s);
+    } else if constexpr (MODE == ROW_ADJ_CROP) {
+      return a.in[(int64_t)(sz.y0 + line) * N + idx];
+    } else {
+      return a.in[(int64_t)line * N + idx];
+    }
+  };
+  auto store = [&](int idx, cpx<R> val) {
+    if (!valid) return;
+    if constexpr (MODE == ROW_FWD_SZ) {
+      a.out[(int64_t)(sz.y0 + line) * N + idx] = val;
+    } else if constexpr (MODE == ROW_ADJ_CROP) {
+      const int i = idx - sz.x0;
+      if ((unsigned)i >= (unsigned)sz.nx) return;
+      const R s = sz.sy[line] * sz.sx[i] * sz.scale;
+      sz.img_out[(int64_t)line * sz.nx + i] = scale(val, s);
+    } else {
+      a.out[(int64_t)line * N + idx] = val;
+    }
+  };
+  fft_line<R, E, INV>(a.pl, t, lds, 1, a.tw, a.twm, load, store);
+}
+
+template <typename R, int E, bool INV>
+static void launch_rows_e(const Fft2dPlan& p, int mode, const RowArgs<R>& a, int threads, size_t shm, int blocks,
+                          hipStream_t s) {
+  ProfScope prof(mode == ROW_FWD_SZ ? "fft_rows_fwd_sz" : mode == ROW_ADJ_CROP ? "fft_rows_adj_crop" : "fft_rows", s);
+  switch (mode) {
+    case ROW_FWD_SZ:
+      hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_FWD_SZ>), dim3(blocks), dim3(threads), shm, s, a);
+      break;
+    case ROW_ADJ_CROP:
+      hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_ADJ_CROP>), dim3(blocks), dim3(threads), shm, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_PLAIN>), dim3(blocks), dim3(threads), shm, s, a);
+  }
+}
+
+template <typename R>
+void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* out, int nlines, const SZ<R>& sz,
+              hipStream_t s) {
+  RowArgs<R> a;
+  a.pl = p.row;
+  a.in = in;
+  a.out = out;
+  a.tw = static_cast<const cpx<R>*>(p.tw);
+  a.twm = p.Lmax / p.Nu;
+  a.Nu = p.Nu;
+  a.nlines = nlines;
+  a.sz = sz;
+  const int N = p.Nu;
+  const int E = N < FFT_E ? N : FFT_E;
+  const int T = N / E;
+  int lpb = 256 / T;
+  if (lpb < 1) lpb = 1;
+  if (lpb > nlines) lpb = nlines;
+  a.lpb = lpb;
+  const int threads = lpb * T;
+  const size_t shm = (size_t)lpb * N * sizeof(cpx<R>);
+  const int blocks = (nlines + lpb - 1) / lpb;
+  if (blocks == 0) return;
+#define PF_ROWS(EE)                                                      \
+  if (inv) launch_rows_e<R, EE, true>(p, mode, a, threads, shm, blocks, s); \
+  else launch_rows_e<R, EE, false>(p, mode, a, threads, shm, blocks, s);
+  switch (E) {
+    case 2: PF_ROWS(2) break;
+    case 4: PF_ROWS(4) break;
+    case 8: PF_ROWS(8) break;
+    default: PF_ROWS(16)
+  }
+#undef PF_ROWS
+  PF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+// Column passes
+// ---------------------------------------------------------------------------------------------
+enum ColKind { COL_SINGLE = 0, COL_STEP_A = 1, COL_STEP_B = 2 };
+
+template <typename R>
+struct ColArgs {
+  LinePlan pl;
+  const cpx<R>* in;
+  cpx<R>* out;
+  const cpx<R>* tw;
+  int twm;       // Lmax / line length
+  int twm_full;  // Lmax / Nv (inter-step twiddle)
+  int Nu, Nv, C, N1, N2;
+  int y0, ny;    // row window
+  int rin, rout; // restrict input / output rows to the window
+};
+
+template <typename R, int E, bool INV, int KIND>
+__global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int C = a.C;
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const int by = blockIdx.y;  // r2 for step A, k1 for step B
+  const int64_t Nu = a.Nu;
+  auto in_window = [&](int row) { return (unsigned)(row - a.y0) < (unsigned)a.ny; };
+  auto load = [&](int idx) -> cpx<R> {
+    int row;
+    if constexpr (KIND == COL_SINGLE) row = idx;
+    else if constexpr (KIND == COL_STEP_A) row = a.N2 * idx + by;
+    else row = a.N2 * by + idx;
+    if (a.rin && !in_window(row)) return mk<R>(R(0), R(0));
+    return a.in[row * Nu + col];
+  };
+  auto store = [&](int idx, cpx<R> val) {
+    int row;
+    if constexpr (KIND == COL_SINGLE) {
+      row = idx;
+    } else if constexpr (KIND == COL_STEP_A) {
+      row = a.N2 * idx + by;
+      const cpx<R> w = a.tw[(by * idx) * a.twm_full];
+      val = INV ? cmulc(val, w) : cmul(val, w);
+    } else {
+      row = by + a.N1 * idx;
+    }
+    if (a.rout && !in_window(row)) return;
+    a.out[row * Nu + col] = val;
+  };
+  fft_line<R, E, INV>(a.pl, t, smem + c, C, a.tw, a.twm, load, store);
+}
+
+template <typename R, int E, bool INV>
+static void launch_cols_e(int kind, const ColArgs<R>& a, dim3 grid, int threads, size_t shm, hipStream_t s) {
+  ProfScope prof(kind == COL_STEP_A ? "fft_cols_stepA" : kind == COL_STEP_B ? "fft_cols_stepB" : "fft_cols", s);
+  switch (kind) {
+    case COL_STEP_A:
+      hipLaunchKernelGGL((k_fft_cols<R, E, INV, COL_STEP_A>), grid, dim3(threads), shm, s, a);
+      break;
+    case COL_STEP_B:
+      hipLaunchKernelGGL((k_fft_cols<R, E, INV, COL_STEP_B>), grid, dim3(threads), shm, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fft_cols<R, E, INV, COL_SINGLE>), grid, dim3(threads), shm, s, a);
+  }
+}
+
+template <typename R>
+static void launch_cols(int kind, bool inv, const ColArgs<R>& a, dim3 grid, hipStream_t s) {
+  const int N = a.pl.N;
+  const int E = N < FFT_E ? N : FFT_E;
+  const int threads = a.C * (N / E);
+  const size_t shm = (size_t)N * a.C * sizeof(cpx<R>);
+#define PF_COLS(EE)                                                    \
+  if (inv) launch_cols_e<R, EE, true>(kind, a, grid, threads, shm, s); \
+  else launch_cols_e<R, EE, false>(kind, a, grid, threads, shm, s);
+  switch (E) {
+    case 2: PF_COLS(2) break;
+    case 4: PF_COLS(4) break;
+    case 8: PF_COLS(8) break;
+    default: PF_COLS(16)
+  }
+#undef PF_COLS
+  PF_HIP(hipGetLastError());
+}
+
+template <typename R>
+void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow win, bool restrict_in,
+              bool restrict_out, hipStream_t s) {
+  ColArgs<R> a;
+  a.in = in;
+  a.out = out;
+  a.tw = static_cast<const cpx<R>*>(p.tw);
+  a.Nu = p.Nu;
+  a.Nv = p.Nv;
+  a.twm_full = p.Lmax / p.Nv;
+  a.y0 = win.y0;
+  a.ny = win.ny ? win.ny : p.Nv;
+  a.N1 = p.N1;
+  a.N2 = p.N2;
+  if (!p.col_two_step) {
+    a.pl = p.col;
+    a.twm = p.Lmax / p.Nv;
+    a.C = p.col_C;
+    a.rin = restrict_in;
+    a.rout = restrict_out;
+    launch_cols<R>(COL_SINGLE, inv, a, dim3(p.Nu / a.C), s);
+    return;
+  }
+  a.C = p.two_C;
+  // step A in place on `in`: N1-point transforms over rows of stride N2
+  a.pl = p.colA;
+  a.twm = p.Lmax / p.N1;
+  a.out = in;
+  a.rin = restrict_in;
+  a.rout = 0;
+  launch_cols<R>(COL_STEP_A, inv, a, dim3(p.Nu / a.C, p.N2), s);
+  // step B `in` -> `out`: N2-point transforms over contiguous row blocks
+  a.pl = p.colB;
+  a.twm = p.Lmax / p.N2;
+  a.in = in;
+  a.out = out;
+  a.rin = 0;
+  a.rout = restrict_out;
+  launch_cols<R>(COL_STEP_B, inv, a, dim3(p.Nu / a.C, p.N1), s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Plans and twiddles
+// ---------------------------------------------------------------------------------------------
+static void twiddle_table(int L, std::vector<double>& re, std::vector<double>& im) {
+  // exp(-2 pi i m / L) from the first octant by symmetry, so W^m and its mirror images agree
+  // bit for bit; sin/cos of arguments <= pi/4 are evaluated in long double.
+  re.resize(L);
+  im.resize(L);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int m = 0; m < L; ++m) {
+    // angle = 2 pi m / L = (pi/4) * (8m / L): octant o and residual
+    const int64_t num = 8LL * m;  // angle in units of pi/4 is num / L
+    const int o = (int)(num / L);
+    const int64_t rmd = num - (int64_t)o * L;  // in [0, L)
+    long double a;
+    bool swap = false;
+    if (o % 2 == 0) {
+      a = (pi / 4) * (long double)rmd / (long double)L;
+    } else {
+      a = (pi / 4) * (long double)(L - rmd) / (long double)L;
+      swap = true;
+    }
+    long double c = cosl(a), sn = sinl(a);
+    if (swap) {
+      long double tmp = c;
+      c = sn;
+      sn = tmp;
+    }
+    // rotate the first-quadrant value (c, sn) of angle (o/2)*pi/2 + ... into place
+    long double cr, sr;
+    switch ((o / 2) % 4) {
+      case 0: cr = c; sr = sn; break;
+      case 1: cr = -sn; sr = c; break;
+      case 2: cr = -c; sr = -sn; break;
+      default: cr = sn; sr = -c; break;
+    }
+    re[m] = (double)cr;
+    im[m] = (double)(-sr);
+  }
+}
+
+void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec) {
+  if (Nv > MAX_AXIS || Nu > MAX_AXIS)
+    throw Error(-3, "grid axis larger than " + std::to_string(MAX_AXIS) + " is not supported");
+  p.Nv = Nv;
+  p.Nu = Nu;
+  p.prec = prec;
+  p.Lmax = Nv > Nu ? Nv : Nu;
+  p.row = make_line_plan(Nu);
+  const size_t esz = prec ? 16 : 8;
+  // single pass while N * C * esz <= 64 KiB with C >= 16 (fp32) / 8 (fp64) columns
+  const int Cmin = prec ? 8 : 16;
+  if ((size_t)Nv * Cmin * esz <= 65536 || Nv <= 32) {
+    p.col_two_step = false;
+    p.col = make_line_plan(Nv);
+    int C = (int)(65536 / ((size_t)Nv * esz));
+    const int E = Nv < FFT_E ? Nv : FFT_E;
+    while ((size_t)C * (Nv / E) > 512) C /= 2;  // threads per block
+    if (C > 64) C = 64;
+    if (C > Nu) C = Nu;
+    if (C < 1) C = 1;
+    p.col_C = C;
+  } else {
+    p.col_two_step = true;
+    int m = 0;
+    while ((1 << m) < Nv) ++m;
+    p.N2 = 1 << (m / 2);
+    p.N1 = Nv / p.N2;
+    p.colA = make_line_plan(p.N1);
+    p.colB = make_line_plan(p.N2);
+    int C = prec ? 32 : 64;
+    while ((size_t)p.N1 * C * esz > 65536 && C > 8) C /= 2;
+    if (C > Nu) C = Nu;
+    p.two_C = C;
+  }
+  std::vector<double> re, im;
+  twiddle_table(p.Lmax, re, im);
+  if (prec) {
+    std::vector<cpx<double>> h(p.Lmax);
+    for (int m = 0; m < p.Lmax; ++m) h[m] = mk<double>(re[m], im[m]);
+    PF_HIP(hipMalloc(&p.tw, sizeof(cpx<double>) * p.Lmax));
+    PF_HIP(hipMemcpy(p.tw, h.data(), sizeof(cpx<double>) * p.Lmax, hipMemcpyHostToDevice));
+  } else {
+    std::vector<cpx<float>> h(p.Lmax);
+    for (int m = 0; m < p.Lmax; ++m) h[m] = mk<float>((float)re[m], (float)im[m]);
+    PF_HIP(hipMalloc(&p.tw, sizeof(cpx<float>) * p.Lmax));
+    PF_HIP(hipMemcpy(p.tw, h.data(), sizeof(cpx<float>) * p.Lmax, hipMemcpyHostToDevice));
+  }
+}
+
+void fft_plan_free(Fft2dPlan& p) {
+  if (p.tw) (void)hipFree(p.tw);
+  p.tw = nullptr;
+}
+
+template void fft_rows<float>(const Fft2dPlan&, int, bool, const cpx<float>*, cpx<float>*, int, const SZ<float>&,
+                              hipStream_t);
+template void fft_rows<double>(const Fft2dPlan&, int, bool, const cpx<double>*, cpx<double>*, int,
+                               const SZ<double>&, hipStream_t);
+template void fft_cols<float>(const Fft2dPlan&, bool, cpx<float>*, cpx<float>*, RowWindow, bool, bool, hipStream_t);
+template void fft_cols<double>(const Fft2dPlan&, bool, cpx<double>*, cpx<double>*, RowWindow, bool, bool,
+                               hipStream_t);
+
+}  // namespace pf

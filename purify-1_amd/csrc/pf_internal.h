@@ -1,0 +1,166 @@
+// pf_internal.h -- host-side structures and kernel launchers shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pf_device.h"
+
+namespace pf {
+
+// ---- errors -------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void hip_check(hipError_t e, const char* what);
+#define PF_HIP(call) ::pf::hip_check((call), #call)
+
+// ---- per-kernel timing --------------------------------------------------------------------
+// When an operator has profiling enabled it installs its Profiler as the calling thread's current
+// one; every launch site then brackets its kernel with HIP events on the launching stream.
+struct Profiler {
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> names;
+  std::vector<double> total_ms;
+  std::vector<int64_t> count;
+  hipEvent_t get_event();
+  void flush();  // synchronises on the recorded events and accumulates
+  void clear();
+  ~Profiler();
+};
+Profiler*& current_profiler();
+struct ProfScope {
+  Profiler* p;
+  const char* name;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  ProfScope(const char* n, hipStream_t st);
+  ~ProfScope();
+};
+
+// ---- FFT plans ----------------------------------------------------------------------------
+constexpr int FFT_E = 16;      // complex elements each thread holds per line
+constexpr int MAX_AXIS = 8192; // largest supported grid axis
+
+struct LinePlan {
+  int N = 0, log2N = 0, nst = 0;
+  int rad[8] = {0};
+};
+LinePlan make_line_plan(int N);
+
+struct Fft2dPlan {
+  int Nv = 0, Nu = 0, prec = 0;
+  int Lmax = 0;        // twiddle table length (max(Nv, Nu))
+  void* tw = nullptr;  // device table exp(-2 pi i m / Lmax), m < Lmax, in the plan's precision
+  LinePlan row;        // N = Nu
+  int row_lpb = 1;     // lines per block of the row pass
+  bool col_two_step = false;
+  LinePlan col;        // single-pass column plan, N = Nv
+  int col_C = 16;      // columns per block of the single-pass column kernel
+  int N1 = 0, N2 = 0;  // Nv = N1 * N2 for the two-step column pass
+  LinePlan colA, colB; // N1- and N2-point plans
+  int two_C = 64;      // columns per block of the two-step kernels
+};
+void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec);
+void fft_plan_free(Fft2dPlan& p);
+
+// Which rows of the grid hold data (forward after the row pass) or are wanted (adjoint crop).
+struct RowWindow {
+  int y0 = 0, ny = 0;  // rows [y0, y0+ny); ny = 0 means all rows
+};
+
+// Fused row-pass variants.
+enum RowMode { ROW_PLAIN = 0, ROW_FWD_SZ = 1, ROW_ADJ_CROP = 2 };
+
+template <typename R>
+struct SZ {
+  const cpx<R>* img_in = nullptr;  // ROW_FWD_SZ input image
+  cpx<R>* img_out = nullptr;       // ROW_ADJ_CROP output image
+  const R* sx = nullptr;           // length nx
+  const R* sy = nullptr;           // length ny
+  R scale = R(1);
+  int nx = 0, ny = 0, x0 = 0, y0 = 0;
+};
+
+template <typename R>
+void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* out, int nlines, const SZ<R>& sz,
+              hipStream_t s);
+// Column pass.  restrict_in: rows outside win are zero and not read; restrict_out: only rows in
+// win are written.  Two-step plans overwrite `in` with the intermediate result.
+template <typename R>
+void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow win, bool restrict_in,
+              bool restrict_out, hipStream_t s);
+
+// ---- gridding -----------------------------------------------------------------------------
+template <typename R>
+struct VisData {
+  // complex64 path: kk = (k_u, k_v) = floor(u - Ju/2), floor(v - Jv/2); ff = (u - k_u, v - k_v)
+  // complex128 path: uv = (u, v) in pixels (k computed on the device exactly as the reference)
+  const int2* kk = nullptr;
+  const cpx<float>* ff = nullptr;
+  const cpx<double>* uv = nullptr;
+  const cpx<R>* weights = nullptr;  // nullptr: unit weights
+  const int32_t* perm = nullptr;    // sorted position -> caller's index
+  int64_t M = 0;
+};
+
+template <typename R>
+struct GridGeom {
+  int Nu = 0, Nv = 0, Ju = 0, Jv = 0;
+  KParams<R> ku, kv;  // u-axis and v-axis kernels
+};
+
+struct TileLists {
+  int TU = 32, TV = 32, ntu = 0, ntv = 0;
+  const uint32_t* ptr = nullptr;   // n_tiles + 1
+  const uint32_t* list = nullptr;  // sorted vis indices per tile
+};
+
+template <typename R>
+void degrid(const GridGeom<R>& g, const VisData<R>& v, const cpx<R>* grid, cpx<R>* y, R scale, hipStream_t s);
+template <typename R>
+void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* y, cpx<R>* grid,
+                R scale, hipStream_t s);
+
+// ---- projection rows (w_term) -------------------------------------------------------------
+template <typename R>
+struct WRows {
+  int Kv = 0, Ku = 0;              // (Jv+Jw-1) x (Ju+Jw-1) taps per row
+  const int2* k0 = nullptr;        // row start (kw_u, kw_v) = floor(u - Ku/2), floor(v - Kv/2)
+  const cpx<R>* vals = nullptr;    // M * Kv * Ku values, row-major [jv][ju], weight and phase folded
+};
+template <typename R>
+void degrid_wrows(const GridGeom<R>& g, const VisData<R>& v, const WRows<R>& w, const cpx<R>* grid, cpx<R>* y,
+                  R scale, hipStream_t s);
+template <typename R>
+void grid_wrows_tiles(const GridGeom<R>& g, const VisData<R>& v, const WRows<R>& w, const TileLists& t,
+                      const cpx<R>* y, cpx<R>* grid, R scale, hipStream_t s);
+
+// ---- vector helpers -----------------------------------------------------------------------
+// sum |x|^2 in FP64 over n complex entries, deterministic two-level reduction into *out (device)
+template <typename R>
+void norm2_fp64(const cpx<R>* x, int64_t n, double* partials, double* out, hipStream_t s);
+template <typename R>
+void scale_vec(cpx<R>* x, int64_t n, double s, hipStream_t st);
+void c128_to_c64(const cpx<double>* in, cpx<float>* out, int64_t n, hipStream_t s);
+void c64_to_c128(const cpx<float>* in, cpx<double>* out, int64_t n, hipStream_t s);
+
+// ---- host-side kernel functions (kernels.cc) ------------------------------------------------
+double host_bessel_i0(double x);
+double host_kernel(int kind, double x, int J, double os);       // gridding kernel value
+double host_ft_kernel(int kind, double x, int J, double os);    // correction FT value at x
+double kb_alpha(int kind, int J, double os);
+template <typename R>
+KParams<R> make_kparams(int kind, int J, double os);
+
+}  // namespace pf

@@ -1,0 +1,982 @@
+// pf_op.hip -- the measurement operator object and the C-ABI declared in include/purify_amd.h.
+//
+// Construction (host, once): unit conversion, tap origins, a counting sort of the visibilities by
+// the grid tile holding their first tap, the per-tile gridding lists, the separable correction
+// vectors and the FFT plan; everything is uploaded once and stays resident in HBM.
+// Apply (device, per call):
+//   forward  y = G F Z S x / norm : [row FFT fused with S, Z] -> [column FFT] -> [degrid gather]
+//   adjoint  x = S Z^T F^H G^H y / norm : [tile gridding] -> [column IFFT] -> [row IFFT fused with crop, S]
+// The unitary 1/sqrt(N) and 1/norm scalings are folded into the S multiply, so no pass is spent on them.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <random>
+
+#include "../../include/purify_amd.h"
+#include "pf_internal.h"
+
+namespace pf {
+
+static thread_local std::string g_last_error;
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(PF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- per-kernel timing ----
+Profiler*& current_profiler() {
+  static thread_local Profiler* p = nullptr;
+  return p;
+}
+
+hipEvent_t Profiler::get_event() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  PF_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Profiler::flush() {
+  for (Rec& r : pending) {
+    PF_HIP(hipEventSynchronize(r.b));
+    float ms = 0;
+    PF_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    size_t k = 0;
+    while (k < names.size() && names[k] != r.name) ++k;
+    if (k == names.size()) {
+      names.push_back(r.name);
+      total_ms.push_back(0);
+      count.push_back(0);
+    }
+    total_ms[k] += ms;
+    count[k] += 1;
+    pool.push_back(r.a);
+    pool.push_back(r.b);
+  }
+  pending.clear();
+}
+
+void Profiler::clear() {
+  flush();
+  names.clear();
+  total_ms.clear();
+  count.clear();
+}
+
+Profiler::~Profiler() {
+  for (Rec& r : pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+ProfScope::ProfScope(const char* n, hipStream_t st) : p(current_profiler()), name(n), s(st) {
+  if (p) {
+    a = p->get_event();
+    PF_HIP(hipEventRecord(a, s));
+  }
+}
+
+ProfScope::~ProfScope() {
+  if (!p) return;
+  hipEvent_t b = p->get_event();
+  if (hipEventRecord(b, s) == hipSuccess) p->pending.push_back(Profiler::Rec{name, a, b});
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host-side kernel functions (cpp/purify/kernels.cc).  Used for the correction vectors and for the
+// Kaiser-Bessel Horner coefficients; the device evaluates kernels through KParams.
+// ---------------------------------------------------------------------------------------------
+static const double PI = 3.14159265358979323846;
+
+double host_bessel_i0(double x) {
+  // power series sum_k (x^2/4)^k / (k!)^2 (all terms positive)
+  const double q = 0.25 * x * x;
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 500; ++k) {
+    term *= q / ((double)k * (double)k);
+    sum += term;
+    if (term < 1e-17 * sum) break;
+  }
+  return sum;
+}
+
+double kb_alpha(int kind, int J, double os) {
+  if (kind == K_KBMIN) return PI * std::sqrt((double)J * J / (os * os) * (os - 0.5) * (os - 0.5) - 0.8);  // kernels.cc:259
+  return 2.34 * J;  // kernels.cc:11
+}
+
+static double gauss_sigma(int kind, int J) { return kind == K_GAUSS_ALT ? 1.0 : 0.31 * std::pow(J, 0.52); }
+
+static double pswf_calc(double eta0, int J) {
+  // kernels.cc:71-105 (alpha = 1)
+  if (J != 6) return 0;
+  const double a = std::fabs(eta0);
+  double e, num, den;
+  if (a <= 0.75) {
+    e = eta0 * eta0 - 0.5625;
+    num = (((2.312756e-1 * e - 5.335581e-1) * e + 6.278660e-1) * e - 3.644705e-1) * e + 8.203343e-2;
+    den = (2.078043e-1 * e + 8.212018e-1) * e + 1.0;
+  } else if (a <= 1) {
+    e = eta0 * eta0 - 1.0;
+    num = (((6.412774e-2 * e - 1.201436e-1) * e + 1.021332e-1) * e - 3.697768e-2) * e + 4.028559e-3;
+    den = (2.918724e-1 * e + 9.599102e-1) * e + 1.0;
+  } else {
+    return 0;
+  }
+  return num / den;
+}
+
+double host_kernel(int kind, double x, int J, double os) {
+  switch (kind) {
+    case K_KB:
+    case K_KBMIN: {
+      const double al = kb_alpha(kind, J, os), a = 2 * x / J;
+      return host_bessel_i0(al * std::sqrt(1 - a * a)) / host_bessel_i0(al);
+    }
+    case K_GAUSS:
+    case K_GAUSS_ALT: {
+      const double a = x / gauss_sigma(kind, J);
+      return std::exp(-a * a * 0.5);
+    }
+    case K_PSWF: {
+      const double eta = 2 * x / J;
+      return pswf_calc(eta, J) * (1 - eta * eta);
+    }
+    case K_BOX:
+      return 1.0 / J;
+  }
+  return NAN;
+}
+
+double host_ft_kernel(int kind, double x, int J, double os) {
+  switch (kind) {
+    case K_KB:
+    case K_KBMIN: {
+      // kernels.cc:24-38: Re[sin(eta)/eta] / 38828.11016883, eta = sqrt((pi x J)^2 - alpha^2)
+      const double al = kb_alpha(kind, J, os);
+      const double t = (PI * x * J) * (PI * x * J) - al * al;
+      const std::complex<double> eta = std::sqrt(std::complex<double>(t, 0.0));
+      if (std::abs(eta) == 0.0) return 1.0 / 38828.11016883;
+      return std::real(std::sin(eta) / eta) / 38828.11016883;
+    }
+    case K_GAUSS:
+    case K_GAUSS_ALT: {
+      const double sg = gauss_sigma(kind, J), a = x * sg * PI;
+      return std::sqrt(PI / 2) / sg * std::exp(-a * a * 2);  // kernels.cc:219-230
+    }
+    case K_PSWF:
+      return pswf_calc(2 * x, J);  // kernels.cc:124-142
+    case K_BOX: {
+      const double t = J * x * PI;  // kernels.cc:197-205
+      return t == 0.0 ? 1.0 : std::sin(t) / t;
+    }
+  }
+  return NAN;
+}
+
+template <typename R>
+KParams<R> make_kparams(int kind, int J, double os) {
+  KParams<R> p;
+  std::memset(&p, 0, sizeof(p));
+  p.kind = kind;
+  p.J = J;
+  p.half = (R)(0.5 * J);
+  p.four_over_J2 = (R)(4.0 / ((double)J * J));
+  p.two_over_J = (R)(2.0 / J);
+  const double sg = gauss_sigma(kind, J);
+  p.inv_two_sigma2 = (R)(1.0 / (2 * sg * sg));
+  p.box = (R)(1.0 / J);
+  p.ncoef = 1;
+  if (kind == K_KB || kind == K_KBMIN) {
+    const long double al = kb_alpha(kind, J, os);
+    const long double q = al * al / 4.0L;
+    const long double i0 = (long double)host_bessel_i0((double)al);
+    const long double cut = sizeof(R) == 8 ? 1e-19L : 1e-10L;
+    long double c = 1.0L;  // (q^k / (k!)^2)
+    int k = 0;
+    for (; k < KB_MAX_COEF; ++k) {
+      p.coef[k] = (R)(c / i0);
+      if (k > 2 && c / i0 < cut) {
+        ++k;
+        break;
+      }
+      c *= q / ((long double)(k + 1) * (k + 1));
+    }
+    p.ncoef = k;
+  }
+  return p;
+}
+template KParams<float> make_kparams<float>(int, int, double);
+template KParams<double> make_kparams<double>(int, int, double);
+
+}  // namespace pf
+
+using namespace pf;
+
+// ---------------------------------------------------------------------------------------------
+// The operator object
+// ---------------------------------------------------------------------------------------------
+struct pf_op {
+  int prec = PF_C64;
+  int ny = 0, nx = 0, Nv = 0, Nu = 0, Ju = 0, Jv = 0, Jw = 0, w_term = 0, kind = 0;
+  int x0 = 0, y0 = 0;
+  double os = 2, norm = 1, cellx = 1, celly = 1;
+  int64_t M = 0;
+  bool unit_weights = true;
+  Fft2dPlan fft;
+  TileLists tl;
+  int n_tiles = 0;
+  int64_t tile_list_len = 0;
+  // device buffers
+  int2* d_kk = nullptr;
+  cpx<float>* d_ff = nullptr;
+  cpx<double>* d_uv = nullptr;
+  void* d_weights = nullptr;
+  int32_t* d_perm = nullptr;
+  uint32_t* d_tptr = nullptr;
+  uint32_t* d_tlist = nullptr;
+  void* d_sx = nullptr;
+  void* d_sy = nullptr;
+  void* d_gridA = nullptr;
+  void* d_gridB = nullptr;
+  void* d_stage128 = nullptr;  // complex128 staging for host-pointer applies
+  void* d_vecx = nullptr;      // image-sized scratch (prec) for host applies / power method
+  void* d_vecy = nullptr;      // vis-sized scratch (prec)
+  double* d_red = nullptr;     // reduction partials + result
+  GridGeom<float> gf;
+  GridGeom<double> gd;
+  std::unique_ptr<Profiler> prof;  // per-kernel timing when enabled
+  int64_t device_bytes = 0;
+  std::vector<void*> allocs;
+
+  void* dalloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 16;
+    PF_HIP(hipMalloc(&p, bytes));
+    allocs.push_back(p);
+    device_bytes += (int64_t)bytes;
+    return p;
+  }
+  ~pf_op() {
+    for (void* p : allocs) (void)hipFree(p);
+    fft_plan_free(fft);
+  }
+  size_t esz() const { return prec == PF_C128 ? 16 : 8; }
+  int64_t nimg() const { return (int64_t)nx * ny; }
+  int64_t ngrid() const { return (int64_t)Nu * Nv; }
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return PF_OK;
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(PF_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(PF_ERR_ARG, e.what());
+  }
+}
+
+bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+int wrapi(int64_t a, int n) {
+  int64_t r = a % n;
+  return (int)(r < 0 ? r + n : r);
+}
+
+// set_cell_size + uv_scale (cpp/purify/utilities.cc:208-278, operators.h:405-410)
+void to_pixels(const pf_op_desc& d, int Nu, int Nv, std::vector<double>& u, std::vector<double>& v) {
+  const int64_t M = d.M;
+  u.assign(d.u, d.u + M);
+  v.assign(d.v, d.v + M);
+  if (d.units == PF_PIXELS) return;
+  double cu = d.cellx, cv = d.celly;
+  double su = 180 * 3600 / PI, sv = 180 * 3600 / PI;
+  if (d.units == PF_LAMBDA) {
+    if (cu == 0 && cv == 0) {
+      double mu = 0, mv = 0;
+      for (int64_t m = 0; m < M; ++m) {
+        mu = std::max(mu, u[m] * u[m]);
+        mv = std::max(mv, v[m] * v[m]);
+      }
+      cu = (180 * 3600) / std::sqrt(mu) / PI / 3;
+      cv = (180 * 3600) / std::sqrt(mv) / PI / 3;
+    }
+    if (cv == 0) cv = cu;
+    su = 180 * 3600 / cu / PI;
+    sv = 180 * 3600 / cv / PI;
+  }
+  for (int64_t m = 0; m < M; ++m) {
+    const double ur = u[m] / su * 2 * PI, vr = v[m] / sv * 2 * PI;
+    u[m] = ur / (2 * PI) * Nu;
+    v[m] = vr / (2 * PI) * Nv;
+  }
+}
+
+void build(pf_op* op, const pf_op_desc& d) {
+  if (d.imsizex <= 0 || d.imsizey <= 0) throw Error(PF_ERR_ARG, "image size must be positive");
+  if (d.M < 0 || (d.M > 0 && (!d.u || !d.v))) throw Error(PF_ERR_ARG, "u and v must be given for M > 0");
+  if (d.M > 0x3fffffffLL) throw Error(PF_ERR_UNSUPPORTED, "more than 2^30 visibilities in one operator");
+  if (d.kernel < 0 || d.kernel > 5) throw Error(PF_ERR_ARG, "Did not choose valid kernel.");
+  if (d.kernel == PF_PSWF && (d.Ju != 6 || d.Jv != 6))
+    throw Error(PF_ERR_ARG, "Incorrect input: PSWF requires a support of 6");  // kernels.cc:241-244
+  if (d.Ju < 1 || d.Jv < 1 || d.Ju > 16 || d.Jv > 16) throw Error(PF_ERR_UNSUPPORTED, "kernel support must be 1..16");
+  if (d.precision != PF_C64 && d.precision != PF_C128) throw Error(PF_ERR_ARG, "precision must be PF_C64 or PF_C128");
+  if (d.w_term) throw Error(PF_ERR_UNSUPPORTED, "w_term operators are built by pf_op_create_w");
+  op->prec = d.precision;
+  op->nx = d.imsizex;
+  op->ny = d.imsizey;
+  op->os = d.oversample_ratio;
+  op->Nu = (int)std::floor(d.imsizex * d.oversample_ratio);
+  op->Nv = (int)std::floor(d.imsizey * d.oversample_ratio);
+  if (!is_pow2(op->Nu) || !is_pow2(op->Nv))
+    throw Error(PF_ERR_UNSUPPORTED, "oversampled grid dimensions must be powers of two (got " + std::to_string(op->Nv) +
+                                        " x " + std::to_string(op->Nu) + ")");
+  op->x0 = (int)std::floor(op->Nu * 0.5 - op->nx * 0.5);
+  op->y0 = (int)std::floor(op->Nv * 0.5 - op->ny * 0.5);
+  op->kind = d.kernel;
+  op->Ju = std::min(d.Ju, op->Nu);  // operators.cc:26-27
+  op->Jv = std::min(d.Jv, op->Nv);
+  op->Jw = d.Jw;
+  op->cellx = d.cellx;
+  op->celly = d.celly;
+  op->M = d.M;
+  op->unit_weights = (d.weights == nullptr);
+  if (d.weights) {
+    op->unit_weights = true;
+    for (int64_t m = 0; m < d.M; ++m)
+      if (d.weights[2 * m] != 1.0 || d.weights[2 * m + 1] != 0.0) {
+        op->unit_weights = false;
+        break;
+      }
+  }
+  fft_plan_init(op->fft, op->Nv, op->Nu, op->prec);
+
+  // Kernel slots: the reference passes (kernelv, kernelu) into the (kernelu, kernelv) slots of the
+  // gridding matrix (operators.h:322-324), so the u-axis taps use the v-support kernel and vice versa.
+  op->gf.Nu = op->gd.Nu = op->Nu;
+  op->gf.Nv = op->gd.Nv = op->Nv;
+  op->gf.Ju = op->gd.Ju = op->Ju;
+  op->gf.Jv = op->gd.Jv = op->Jv;
+  op->gf.ku = make_kparams<float>(d.kernel, d.Jv, op->os);
+  op->gf.kv = make_kparams<float>(d.kernel, d.Ju, op->os);
+  op->gd.ku = make_kparams<double>(d.kernel, d.Jv, op->os);
+  op->gd.kv = make_kparams<double>(d.kernel, d.Ju, op->os);
+
+  // ---- coordinates in pixels, tap origins, sort by owner tile ----
+  std::vector<double> u, v;
+  to_pixels(d, op->Nu, op->Nv, u, v);
+  const int64_t M = d.M;
+  const int TU = std::min(d.tile_u > 0 ? d.tile_u : 32, op->Nu);
+  const int TV = std::min(d.tile_v > 0 ? d.tile_v : 32, op->Nv);
+  if (!is_pow2(TU) && TU != op->Nu) throw Error(PF_ERR_ARG, "tile_u must be a power of two");
+  if (!is_pow2(TV) && TV != op->Nv) throw Error(PF_ERR_ARG, "tile_v must be a power of two");
+  const int ntu = op->Nu / TU, ntv = op->Nv / TV;
+  op->n_tiles = ntu * ntv;
+  std::vector<int64_t> ku(M), kv(M);
+  std::vector<uint32_t> owner(M);
+  std::vector<uint32_t> count(op->n_tiles + 1, 0);
+  for (int64_t m = 0; m < M; ++m) {
+    if (!std::isfinite(u[m]) || !std::isfinite(v[m])) throw Error(PF_ERR_ARG, "non-finite u/v coordinate");
+    ku[m] = (int64_t)std::floor(u[m] - op->Ju * 0.5);  // operators.cc:34-35
+    kv[m] = (int64_t)std::floor(v[m] - op->Jv * 0.5);
+    const int q0 = wrapi(ku[m] + 1, op->Nu), p0 = wrapi(kv[m] + 1, op->Nv);
+    owner[m] = (uint32_t)((p0 / TV) * ntu + q0 / TU);
+    count[owner[m] + 1]++;
+  }
+  for (int t = 0; t < op->n_tiles; ++t) count[t + 1] += count[t];
+  std::vector<int32_t> perm(M);
+  {
+    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+    for (int64_t m = 0; m < M; ++m) perm[fill[owner[m]]++] = (int32_t)m;
+  }
+  // ---- per-tile gridding lists (every visibility with a tap in the tile) ----
+  std::vector<uint32_t> tcount(op->n_tiles + 1, 0);
+  auto tile_sets = [&](int64_t m, int* tcols, int& nc, int* trows, int& nr) {
+    const int q0 = wrapi(ku[m] + 1, op->Nu), p0 = wrapi(kv[m] + 1, op->Nv);
+    nc = nr = 0;
+    for (int a = 0; a < op->Ju; ++a) {
+      const int tc = ((q0 + a) % op->Nu) / TU;
+      bool seen = false;
+      for (int k = 0; k < nc; ++k) seen |= (tcols[k] == tc);
+      if (!seen) tcols[nc++] = tc;
+    }
+    for (int b = 0; b < op->Jv; ++b) {
+      const int tr = ((p0 + b) % op->Nv) / TV;
+      bool seen = false;
+      for (int k = 0; k < nr; ++k) seen |= (trows[k] == tr);
+      if (!seen) trows[nr++] = tr;
+    }
+  };
+  int tc[16], tr[16], nc, nr;
+  for (int64_t i = 0; i < M; ++i) {
+    tile_sets(perm[i], tc, nc, tr, nr);
+    for (int r = 0; r < nr; ++r)
+      for (int c = 0; c < nc; ++c) tcount[tr[r] * ntu + tc[c] + 1]++;
+  }
+  for (int t = 0; t < op->n_tiles; ++t) tcount[t + 1] += tcount[t];
+  op->tile_list_len = tcount[op->n_tiles];
+  if (op->tile_list_len > 0xffffffffLL) throw Error(PF_ERR_UNSUPPORTED, "gridding tile lists exceed 2^32 entries");
+  std::vector<uint32_t> tlist(op->tile_list_len);
+  {
+    std::vector<uint32_t> fill(tcount.begin(), tcount.end() - 1);
+    for (int64_t i = 0; i < M; ++i) {
+      tile_sets(perm[i], tc, nc, tr, nr);
+      for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nc; ++c) tlist[fill[tr[r] * ntu + tc[c]]++] = (uint32_t)i;
+    }
+  }
+  op->tl.TU = TU;
+  op->tl.TV = TV;
+  op->tl.ntu = ntu;
+  op->tl.ntv = ntv;
+
+  // ---- upload ----
+  op->d_perm = (int32_t*)op->dalloc(sizeof(int32_t) * M);
+  PF_HIP(hipMemcpy(op->d_perm, perm.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
+  op->d_tptr = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
+  PF_HIP(hipMemcpy(op->d_tptr, tcount.data(), sizeof(uint32_t) * (op->n_tiles + 1), hipMemcpyHostToDevice));
+  op->d_tlist = (uint32_t*)op->dalloc(sizeof(uint32_t) * op->tile_list_len);
+  if (op->tile_list_len)
+    PF_HIP(hipMemcpy(op->d_tlist, tlist.data(), sizeof(uint32_t) * op->tile_list_len, hipMemcpyHostToDevice));
+  op->tl.ptr = op->d_tptr;
+  op->tl.list = op->d_tlist;
+  if (op->prec == PF_C64) {
+    std::vector<int2> kk(M);
+    std::vector<cpx<float>> ff(M);
+    for (int64_t i = 0; i < M; ++i) {
+      const int64_t m = perm[i];
+      kk[i] = make_int2((int)ku[m], (int)kv[m]);
+      ff[i] = mk<float>((float)(u[m] - (double)ku[m]), (float)(v[m] - (double)kv[m]));
+    }
+    op->d_kk = (int2*)op->dalloc(sizeof(int2) * M);
+    op->d_ff = (cpx<float>*)op->dalloc(sizeof(cpx<float>) * M);
+    PF_HIP(hipMemcpy(op->d_kk, kk.data(), sizeof(int2) * M, hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(op->d_ff, ff.data(), sizeof(cpx<float>) * M, hipMemcpyHostToDevice));
+    if (!op->unit_weights) {
+      std::vector<cpx<float>> w(M);
+      for (int64_t i = 0; i < M; ++i)
+        w[i] = mk<float>((float)d.weights[2 * perm[i]], (float)d.weights[2 * perm[i] + 1]);
+      op->d_weights = op->dalloc(sizeof(cpx<float>) * M);
+      PF_HIP(hipMemcpy(op->d_weights, w.data(), sizeof(cpx<float>) * M, hipMemcpyHostToDevice));
+    }
+  } else {
+    std::vector<cpx<double>> uv(M);
+    for (int64_t i = 0; i < M; ++i) uv[i] = mk<double>(u[perm[i]], v[perm[i]]);
+    op->d_uv = (cpx<double>*)op->dalloc(sizeof(cpx<double>) * M);
+    PF_HIP(hipMemcpy(op->d_uv, uv.data(), sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
+    if (!op->unit_weights) {
+      std::vector<cpx<double>> w(M);
+      for (int64_t i = 0; i < M; ++i) w[i] = mk<double>(d.weights[2 * perm[i]], d.weights[2 * perm[i] + 1]);
+      op->d_weights = op->dalloc(sizeof(cpx<double>) * M);
+      PF_HIP(hipMemcpy(op->d_weights, w.data(), sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
+    }
+  }
+
+  // ---- correction S(j, i) = sy[j] * sx[i] (operators.cc:101-121) ----
+  std::vector<double> sx(op->nx), sy(op->ny);
+  for (int i = 0; i < op->nx; ++i)
+    sx[i] = 1.0 / host_ft_kernel(d.kernel, (op->x0 + i + 0.5) / op->Nu - 0.5, d.Ju, op->os);
+  for (int j = 0; j < op->ny; ++j)
+    sy[j] = 1.0 / host_ft_kernel(d.kernel, (op->y0 + j + 0.5) / op->Nv - 0.5, d.Jv, op->os);
+  if (op->prec == PF_C64) {
+    std::vector<float> fx(sx.begin(), sx.end()), fy(sy.begin(), sy.end());
+    op->d_sx = op->dalloc(sizeof(float) * op->nx);
+    op->d_sy = op->dalloc(sizeof(float) * op->ny);
+    PF_HIP(hipMemcpy(op->d_sx, fx.data(), sizeof(float) * op->nx, hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(op->d_sy, fy.data(), sizeof(float) * op->ny, hipMemcpyHostToDevice));
+  } else {
+    op->d_sx = op->dalloc(sizeof(double) * op->nx);
+    op->d_sy = op->dalloc(sizeof(double) * op->ny);
+    PF_HIP(hipMemcpy(op->d_sx, sx.data(), sizeof(double) * op->nx, hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(op->d_sy, sy.data(), sizeof(double) * op->ny, hipMemcpyHostToDevice));
+  }
+
+  // ---- workspaces ----
+  op->d_gridA = op->dalloc(op->esz() * op->ngrid());
+  op->d_gridB = op->dalloc(op->esz() * op->ngrid());
+  op->d_red = (double*)op->dalloc(sizeof(double) * 1040);
+}
+
+template <typename R>
+VisData<R> visdata(const pf_op* op) {
+  VisData<R> v;
+  v.kk = op->d_kk;
+  v.ff = op->d_ff;
+  v.uv = op->d_uv;
+  v.weights = static_cast<const cpx<R>*>(op->d_weights);
+  v.perm = op->d_perm;
+  v.M = op->M;
+  return v;
+}
+template <typename R>
+const GridGeom<R>& geom(const pf_op* op);
+template <>
+const GridGeom<float>& geom<float>(const pf_op* op) {
+  return op->gf;
+}
+template <>
+const GridGeom<double>& geom<double>(const pf_op* op) {
+  return op->gd;
+}
+
+template <typename R>
+SZ<R> make_sz(const pf_op* op, double scl) {
+  SZ<R> sz;
+  sz.sx = static_cast<const R*>(op->d_sx);
+  sz.sy = static_cast<const R*>(op->d_sy);
+  sz.scale = (R)scl;
+  sz.nx = op->nx;
+  sz.ny = op->ny;
+  sz.x0 = op->x0;
+  sz.y0 = op->y0;
+  return sz;
+}
+
+template <typename R>
+void fz_forward(pf_op* op, const cpx<R>* x, cpx<R>* grid_out, double scl, hipStream_t s) {
+  SZ<R> sz = make_sz<R>(op, scl);
+  sz.img_in = x;
+  cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
+  fft_rows<R>(op->fft, ROW_FWD_SZ, false, nullptr, A, op->ny, sz, s);
+  fft_cols<R>(op->fft, false, A, grid_out, RowWindow{op->y0, op->ny}, true, false, s);
+}
+
+template <typename R>
+void fz_adjoint(pf_op* op, cpx<R>* grid_in, cpx<R>* x, double scl, hipStream_t s) {
+  SZ<R> sz = make_sz<R>(op, scl);
+  sz.img_out = x;
+  cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
+  fft_cols<R>(op->fft, true, grid_in, B, RowWindow{op->y0, op->ny}, false, true, s);
+  fft_rows<R>(op->fft, ROW_ADJ_CROP, true, B, nullptr, op->ny, sz, s);
+}
+
+template <typename R>
+void forward_dev(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s) {
+  const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
+  cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
+  fz_forward<R>(op, x, B, scl, s);
+  degrid<R>(geom<R>(op), visdata<R>(op), B, y, R(1), s);
+}
+
+template <typename R>
+void adjoint_dev(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
+  const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
+  cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
+  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, y, A, R(1), s);
+  fz_adjoint<R>(op, A, x, scl, s);
+}
+
+void ensure_stage(pf_op* op) {
+  if (op->d_stage128) return;
+  const int64_t n = std::max(op->nimg(), op->M);
+  op->d_stage128 = op->dalloc(16 * (size_t)std::max<int64_t>(n, 1));
+  op->d_vecx = op->dalloc(op->esz() * (size_t)std::max<int64_t>(op->nimg(), 1));
+  op->d_vecy = op->dalloc(op->esz() * (size_t)std::max<int64_t>(op->M, 1));
+}
+
+// Bring a complex128 host vector (or an op-precision device vector) to an op-precision device vector.
+const void* stage_in(pf_op* op, const void* p, int loc, int64_t n, void* dev_scratch, hipStream_t s) {
+  if (loc == PF_DEVICE) return p;
+  if (loc != PF_HOST) throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+  ensure_stage(op);
+  if (op->prec == PF_C128) {
+    PF_HIP(hipMemcpyAsync(dev_scratch, p, 16 * n, hipMemcpyHostToDevice, s));
+  } else {
+    PF_HIP(hipMemcpyAsync(op->d_stage128, p, 16 * n, hipMemcpyHostToDevice, s));
+    c128_to_c64(static_cast<const cpx<double>*>(op->d_stage128), static_cast<cpx<float>*>(dev_scratch), n, s);
+  }
+  return dev_scratch;
+}
+
+void stage_out(pf_op* op, const void* dev, void* p, int64_t n, hipStream_t s) {
+  if (op->prec == PF_C128) {
+    PF_HIP(hipMemcpyAsync(p, dev, 16 * n, hipMemcpyDeviceToHost, s));
+  } else {
+    c64_to_c128(static_cast<const cpx<float>*>(dev), static_cast<cpx<double>*>(op->d_stage128), n, s);
+    PF_HIP(hipMemcpyAsync(p, op->d_stage128, 16 * n, hipMemcpyDeviceToHost, s));
+  }
+  PF_HIP(hipStreamSynchronize(s));
+}
+
+void apply(pf_op* op, bool adjoint, const void* in, void* out, int in_loc, int out_loc, hipStream_t s) {
+  if (!op) throw Error(PF_ERR_ARG, "null operator");
+  if ((out_loc != PF_HOST && out_loc != PF_DEVICE) || !in || !out) throw Error(PF_ERR_ARG, "bad buffer arguments");
+  const int64_t nin = adjoint ? op->M : op->nimg(), nout = adjoint ? op->nimg() : op->M;
+  if (in_loc == PF_HOST || out_loc == PF_HOST) ensure_stage(op);
+  const void* din = stage_in(op, in, in_loc, nin, adjoint ? op->d_vecy : op->d_vecx, s);
+  void* dout = out_loc == PF_DEVICE ? out : (adjoint ? op->d_vecx : op->d_vecy);
+  if (op->prec == PF_C64) {
+    if (adjoint) adjoint_dev<float>(op, (const cpx<float>*)din, (cpx<float>*)dout, s);
+    else forward_dev<float>(op, (const cpx<float>*)din, (cpx<float>*)dout, s);
+  } else {
+    if (adjoint) adjoint_dev<double>(op, (const cpx<double>*)din, (cpx<double>*)dout, s);
+    else forward_dev<double>(op, (const cpx<double>*)din, (cpx<double>*)dout, s);
+  }
+  if (out_loc == PF_HOST) stage_out(op, dout, out, nout, s);
+}
+
+double read_scalar(pf_op* op, hipStream_t s) {
+  double h = 0;
+  PF_HIP(hipMemcpyAsync(&h, op->d_red + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
+  PF_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+template <typename R>
+double power_method_t(pf_op* op, int iters, double tol, const double* start) {
+  hipStream_t s = nullptr;
+  ensure_stage(op);
+  const int64_t n = op->nimg();
+  cpx<R>* x = static_cast<cpx<R>*>(op->d_vecx);
+  cpx<R>* y = static_cast<cpx<R>*>(op->d_vecy);
+  if (start) {
+    stage_in(op, start, PF_HOST, n, x, s);
+  } else {
+    std::vector<cpx<R>> ones(n, mk<R>(R(1), R(0)));
+    PF_HIP(hipMemcpyAsync(x, ones.data(), sizeof(cpx<R>) * n, hipMemcpyHostToDevice, s));
+    PF_HIP(hipStreamSynchronize(s));
+  }
+  norm2_fp64<R>(x, n, op->d_red, op->d_red + 1024, s);
+  const double n0 = std::sqrt(read_scalar(op, s));
+  if (!(n0 > 0)) throw Error(PF_ERR_ARG, "power method start vector is zero");
+  scale_vec<R>(x, n, 1.0 / n0, s);
+  double old = 0, val = 1;
+  for (int it = 0; it < iters; ++it) {
+    forward_dev<R>(op, x, y, s);
+    adjoint_dev<R>(op, y, x, s);
+    norm2_fp64<R>(x, n, op->d_red, op->d_red + 1024, s);
+    val = std::sqrt(read_scalar(op, s));
+    if (!(val > 0)) throw Error(PF_ERR_STATE, "Error in operator.");
+    if (val != val) throw Error(PF_ERR_STATE, "Error in operator or data corrupted.");
+    scale_vec<R>(x, n, 1.0 / val, s);
+    if (tol * tol > std::fabs(old - val) / old) {
+      old = val;
+      break;
+    }
+    old = val;
+  }
+  return std::sqrt(old);
+}
+
+// Installs the operator's profiler (if any) as the thread's current one for the scope of a call.
+struct ProfInstall {
+  Profiler* prev;
+  explicit ProfInstall(pf_op* op) : prev(current_profiler()) { current_profiler() = op && op->prof ? op->prof.get() : nullptr; }
+  ~ProfInstall() { current_profiler() = prev; }
+};
+
+template <typename R>
+__global__ void k_kernel_eval(KParams<R> p, const double* x, double* out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)kern1d<R>(p, (R)x[i]);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+void pf_op_desc_init(pf_op_desc* d) {
+  std::memset(d, 0, sizeof(*d));
+  d->oversample_ratio = 2.0;
+  d->kernel = PF_KB;
+  d->Ju = d->Jv = 4;
+  d->Jw = 6;
+  d->cellx = d->celly = 1.0;
+  d->units = PF_PIXELS;
+  d->precision = PF_C64;
+  d->power_iters = 100;
+  d->power_tol = 1e-4;
+}
+
+int pf_op_create(const pf_op_desc* desc, pf_op** out) {
+  if (!desc || !out) return fail(PF_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<pf_op> op(new pf_op());
+  int rc = guarded([&] {
+    build(op.get(), *desc);
+    if (desc->power_iters > 0) {
+      const double nrm = op->prec == PF_C64 ? power_method_t<float>(op.get(), desc->power_iters, desc->power_tol, nullptr)
+                                            : power_method_t<double>(op.get(), desc->power_iters, desc->power_tol, nullptr);
+      if (!(nrm > 0)) throw Error(PF_ERR_ARG, "Operator norm is not greater than zero.");  // operators.h:187
+      op->norm = nrm;
+    }
+  });
+  if (rc == PF_OK) *out = op.release();
+  return rc;
+}
+
+int pf_op_destroy(pf_op* op) {
+  delete op;
+  return PF_OK;
+}
+
+int pf_op_forward(pf_op* op, const void* x, void* y, int x_loc, int y_loc, pf_stream stream) {
+  return guarded([&] {
+    ProfInstall pi(op);
+    apply(op, false, x, y, x_loc, y_loc, (hipStream_t)stream);
+  });
+}
+
+int pf_op_adjoint(pf_op* op, const void* y, void* x, int y_loc, int x_loc, pf_stream stream) {
+  return guarded([&] {
+    ProfInstall pi(op);
+    apply(op, true, y, x, y_loc, x_loc, (hipStream_t)stream);
+  });
+}
+
+int pf_op_profile_enable(pf_op* op, int on) {
+  if (!op) return fail(PF_ERR_ARG, "null operator");
+  return guarded([&] {
+    if (on) {
+      if (!op->prof) op->prof.reset(new Profiler());
+      op->prof->clear();
+    } else {
+      op->prof.reset();
+    }
+  });
+}
+
+int pf_op_profile_get(pf_op* op, int32_t idx, char* name, int32_t name_len, double* total_ms, int64_t* count) {
+  if (!op || !op->prof) return fail(PF_ERR_STATE, "profiling is not enabled on this operator");
+  return guarded([&] {
+    op->prof->flush();
+    if (idx < 0 || idx >= (int32_t)op->prof->names.size()) throw Error(PF_ERR_ARG, "profile index out of range");
+    if (name && name_len > 0) {
+      std::strncpy(name, op->prof->names[idx].c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (total_ms) *total_ms = op->prof->total_ms[idx];
+    if (count) *count = op->prof->count[idx];
+  });
+}
+
+int pf_op_power_method(pf_op* op, int32_t iters, double tol, const double* start, double* norm_out) {
+  if (!op || !norm_out) return fail(PF_ERR_ARG, "null argument");
+  return guarded([&] {
+    if (iters <= 0) {
+      *norm_out = 1.0;
+      return;
+    }
+    *norm_out = op->prec == PF_C64 ? power_method_t<float>(op, iters, tol, start)
+                                   : power_method_t<double>(op, iters, tol, start);
+  });
+}
+
+int pf_op_set_norm(pf_op* op, double norm) {
+  if (!op) return fail(PF_ERR_ARG, "null operator");
+  if (!(norm > 0)) return fail(PF_ERR_ARG, "Operator norm is not greater than zero.");
+  op->norm = norm;
+  return PF_OK;
+}
+
+int pf_op_get_norm(const pf_op* op, double* norm) {
+  if (!op || !norm) return fail(PF_ERR_ARG, "null argument");
+  *norm = op->norm;
+  return PF_OK;
+}
+
+int pf_op_get_info(const pf_op* op, pf_op_info* info) {
+  if (!op || !info) return fail(PF_ERR_ARG, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  info->imsizey = op->ny;
+  info->imsizex = op->nx;
+  info->Nv = op->Nv;
+  info->Nu = op->Nu;
+  info->Ju = op->Ju;
+  info->Jv = op->Jv;
+  info->precision = op->prec;
+  info->w_term = op->w_term;
+  info->M = op->M;
+  info->nnz_w = 0;
+  info->tile_u = op->tl.TU;
+  info->tile_v = op->tl.TV;
+  info->n_tiles = op->n_tiles;
+  info->tile_list_len = op->tile_list_len;
+  info->norm = op->norm;
+  info->device_bytes = op->device_bytes;
+  return PF_OK;
+}
+
+int pf_op_apply_FZ(pf_op* op, const void* x_dev, void* grid_dev, pf_stream stream) {
+  return guarded([&] {
+    ProfInstall pi(op);
+    const double scl = 1.0 / std::sqrt((double)op->ngrid());
+    if (op->prec == PF_C64) fz_forward<float>(op, (const cpx<float>*)x_dev, (cpx<float>*)grid_dev, scl, (hipStream_t)stream);
+    else fz_forward<double>(op, (const cpx<double>*)x_dev, (cpx<double>*)grid_dev, scl, (hipStream_t)stream);
+  });
+}
+
+int pf_op_apply_FZ_adjoint(pf_op* op, const void* grid_dev, void* x_dev, pf_stream stream) {
+  return guarded([&] {
+    ProfInstall pi(op);
+    const double scl = 1.0 / std::sqrt((double)op->ngrid());
+    // the column pass works in place on its input, so copy the caller's grid first
+    const size_t bytes = op->esz() * op->ngrid();
+    PF_HIP(hipMemcpyAsync(op->d_gridA, grid_dev, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    if (op->prec == PF_C64) fz_adjoint<float>(op, (cpx<float>*)op->d_gridA, (cpx<float>*)x_dev, scl, (hipStream_t)stream);
+    else fz_adjoint<double>(op, (cpx<double>*)op->d_gridA, (cpx<double>*)x_dev, scl, (hipStream_t)stream);
+  });
+}
+
+int pf_op_apply_G(pf_op* op, const void* grid_dev, void* y_dev, pf_stream stream) {
+  return guarded([&] {
+    ProfInstall pi(op);
+    if (op->prec == PF_C64)
+      degrid<float>(op->gf, visdata<float>(op), (const cpx<float>*)grid_dev, (cpx<float>*)y_dev, 1.0f, (hipStream_t)stream);
+    else
+      degrid<double>(op->gd, visdata<double>(op), (const cpx<double>*)grid_dev, (cpx<double>*)y_dev, 1.0,
+                     (hipStream_t)stream);
+  });
+}
+
+int pf_op_apply_GH(pf_op* op, const void* y_dev, void* grid_dev, pf_stream stream) {
+  return guarded([&] {
+    ProfInstall pi(op);
+    if (op->prec == PF_C64)
+      grid_tiles<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)y_dev, (cpx<float>*)grid_dev, 1.0f,
+                        (hipStream_t)stream);
+    else
+      grid_tiles<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)y_dev, (cpx<double>*)grid_dev, 1.0,
+                         (hipStream_t)stream);
+  });
+}
+
+int pf_fft2d(int32_t Nv, int32_t Nu, int32_t precision, int32_t sign, const void* in_dev, void* out_dev,
+             void* workspace_dev, pf_stream stream) {
+  return guarded([&] {
+    if (sign != -1 && sign != 1) throw Error(PF_ERR_ARG, "sign must be -1 or +1");
+    if (!is_pow2(Nv) || !is_pow2(Nu)) throw Error(PF_ERR_UNSUPPORTED, "FFT axes must be powers of two >= 2");
+    Fft2dPlan p;
+    fft_plan_init(p, Nv, Nu, precision);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t esz = precision ? 16 : 8;
+    void* ws = workspace_dev;
+    bool own = false;
+    if (!ws) {
+      PF_HIP(hipMalloc(&ws, esz * (size_t)Nv * Nu));
+      own = true;
+    }
+    const bool inv = sign > 0;
+    const double scl = 1.0 / std::sqrt((double)Nv * Nu);
+    auto run = [&](auto tag) {
+      using R = decltype(tag);
+      SZ<R> sz;
+      fft_rows<R>(p, ROW_PLAIN, inv, (const cpx<R>*)in_dev, (cpx<R>*)ws, Nv, sz, s);
+      fft_cols<R>(p, inv, (cpx<R>*)ws, (cpx<R>*)out_dev, RowWindow{0, 0}, false, false, s);
+      scale_vec<R>((cpx<R>*)out_dev, (int64_t)Nv * Nu, scl, s);
+    };
+    if (precision) run(double());
+    else run(float());
+    PF_HIP(hipStreamSynchronize(s));
+    if (own) (void)hipFree(ws);
+    fft_plan_free(p);
+  });
+}
+
+int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ftsize, int32_t precision,
+                   const double* x_host, double* out_host, int64_t n) {
+  return guarded([&] {
+    if (kind < 0 || kind > 5) throw Error(PF_ERR_ARG, "Did not choose valid kernel.");
+    if (which == 1) {
+      // correction functions are evaluated on the host at construction (operators.cc:101-121)
+      for (int64_t i = 0; i < n; ++i) out_host[i] = host_ft_kernel(kind, x_host[i] / ftsize - 0.5, J, os);
+      return;
+    }
+    double *dx = nullptr, *dy = nullptr;
+    PF_HIP(hipMalloc(&dx, sizeof(double) * std::max<int64_t>(n, 1)));
+    PF_HIP(hipMalloc(&dy, sizeof(double) * std::max<int64_t>(n, 1)));
+    PF_HIP(hipMemcpy(dx, x_host, sizeof(double) * n, hipMemcpyHostToDevice));
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks) {
+      if (precision == PF_C128)
+        hipLaunchKernelGGL((k_kernel_eval<double>), dim3(blocks), dim3(256), 0, 0, make_kparams<double>(kind, J, os), dx, dy, n);
+      else
+        hipLaunchKernelGGL((k_kernel_eval<float>), dim3(blocks), dim3(256), 0, 0, make_kparams<float>(kind, J, os), dx, dy, n);
+      PF_HIP(hipGetLastError());
+    }
+    PF_HIP(hipMemcpy(out_host, dy, sizeof(double) * n, hipMemcpyDeviceToHost));
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+  });
+}
+
+// ---- synthetic inputs ------------------------------------------------------------------------
+static double uni53(std::mt19937_64& g) { return (double)(g() >> 11) * (1.0 / 9007199254740992.0); }
+static double normal(std::mt19937_64& g) {
+  // Box-Muller with u1 in (0, 1]
+  const double u1 = 1.0 - uni53(g), u2 = uni53(g);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * PI * u2);
+}
+
+int pf_synth_uv(uint64_t seed, int64_t M, double sigma, double* u, double* v) {
+  if ((M > 0 && (!u || !v)) || M < 0) return fail(PF_ERR_ARG, "bad arguments");
+  std::mt19937_64 g(seed);
+  for (int64_t m = 0; m < M; ++m) {
+    u[m] = std::min(PI, std::max(-PI, sigma * normal(g)));
+    v[m] = std::min(PI, std::max(-PI, sigma * normal(g)));
+  }
+  return PF_OK;
+}
+
+int pf_synth_image(uint64_t seed, int32_t ny, int32_t nx, int32_t nsrc, double* img) {
+  if (ny <= 0 || nx <= 0 || nsrc < 0 || !img) return fail(PF_ERR_ARG, "bad arguments");
+  std::mt19937_64 g(seed);
+  std::vector<double> cx(nsrc), cy(nsrc), amp(nsrc), sg(nsrc);
+  for (int k = 0; k < nsrc; ++k) {
+    cx[k] = uni53(g) * nx;
+    cy[k] = uni53(g) * ny;
+    amp[k] = 0.1 + 0.9 * uni53(g);
+    sg[k] = 1.0 + 7.0 * uni53(g);
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double s = 0;
+      for (int k = 0; k < nsrc; ++k) {
+        const double dx = i - cx[k], dy = j - cy[k];
+        s += amp[k] * std::exp(-(dx * dx + dy * dy) / (2 * sg[k] * sg[k]));
+      }
+      img[2 * ((int64_t)j * nx + i)] = s;
+      img[2 * ((int64_t)j * nx + i) + 1] = 0.0;
+    }
+  return PF_OK;
+}
+
+int pf_synth_cnormal(uint64_t seed, int64_t n, double scl, double* out) {
+  if (n < 0 || (n > 0 && !out)) return fail(PF_ERR_ARG, "bad arguments");
+  std::mt19937_64 g(seed);
+  const double s = scl / std::sqrt(2.0);
+  for (int64_t k = 0; k < n; ++k) {
+    out[2 * k] = s * normal(g);
+    out[2 * k + 1] = s * normal(g);
+  }
+  return PF_OK;
+}
+
+const char* pf_last_error(void) { return g_last_error.c_str(); }
+const char* pf_version(void) { return "purify-amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+"""purify_amd -- MI355X-native PURIFY measurement operator (Phi = W G F Z S) behind PURIFY's interface.
+
+The compute runs in libpurify_amd.so (hand-written HIP for gfx950) through the C-ABI of
+include/purify_amd.h; this package is the host-side mirror of the reference's C++ interface.
+"""
+from . import _lib
+from .kernels import kernel, kernel_from_string
+from .operators import (LinearTransform, MeasurementOperator, distributed_measurement_operator,
+                        measurement_operator_factory, measurementoperator)
+from .utilities import vis_params, vis_units
+from . import utilities
+
+__all__ = ["kernel", "kernel_from_string", "LinearTransform", "MeasurementOperator", "measurementoperator",
+           "measurement_operator_factory", "distributed_measurement_operator", "vis_params", "vis_units",
+           "utilities", "fft2d", "kernel_eval", "version"]
+
+
+def version():
+    return _lib.lib().pf_version().decode()
+
+
+def fft2d(x, Nv, Nu, sign=-1):
+    """Unitary 2-D FFT of a torch CUDA tensor (complex64/complex128) of Nv*Nu entries (operators.h:209-261)."""
+    import ctypes as C
+    import torch
+    x = x.contiguous()
+    prec = _lib.PF_C128 if x.dtype == torch.complex128 else _lib.PF_C64
+    out = torch.empty_like(x)
+    s = torch.cuda.current_stream(x.device).cuda_stream
+    _lib.check(_lib.lib().pf_fft2d(Nv, Nu, prec, sign, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), None,
+                                   C.c_void_p(s)))
+    return out
+
+
+def kernel_eval(kind, x, J, which=0, os_=2.0, ftsize=1, precision="c128"):
+    """Evaluate a gridding kernel on the GPU (which=0) or its correction FT (which=1, host)."""
+    import numpy as np
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    out = np.empty_like(x)
+    prec = _lib.PF_C128 if precision in ("c128", "complex128") else _lib.PF_C64
+    _lib.check(_lib.lib().pf_kernel_eval(int(kernel.coerce(kind)), int(which), int(J), float(os_), int(ftsize), prec,
+                                         x.ctypes.data, out.ctypes.data, x.size))
+    return out

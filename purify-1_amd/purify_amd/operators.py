@@ -1,0 +1,299 @@
+"""Python mirror of PURIFY's measurement-operator interface on top of the HIP C-ABI.
+
+    op = measurementoperator.init_degrid_operator_2d(u, v, w, weights, imsizey, imsizex,
+                                                     oversample_ratio=2, power_iters=100, power_tol=1e-4,
+                                                     kernel=kernels.kernel.kb, Ju=4, Jv=4, w_term=False,
+                                                     Jw=6, cellx=1, celly=1)
+    y = op * x                 # Phi x
+    x = op.adjoint() * y       # Phi^dagger y
+
+mirrors measurementoperator::init_degrid_operator_2d (cpp/purify/operators.h:373-414) and its
+sopt::LinearTransform result.  Vectors may be numpy arrays (host, complex128, as the reference's
+Vector<t_complex>) or torch CUDA tensors in the operator's precision (complex64 / complex128), in
+which case the apply runs on the tensor's memory on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .kernels import kernel as _kernel
+from .utilities import vis_params, vis_units
+
+PRECISIONS = {"c64": _lib.PF_C64, "c128": _lib.PF_C128, "complex64": _lib.PF_C64, "complex128": _lib.PF_C128}
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_device_tensor(x):
+    try:
+        torch = _torch()
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+class MeasurementOperator:
+    """Owns a pf_op handle; the direct (Phi) and adjoint (Phi^dagger) applies of one operator."""
+
+    def __init__(self, desc: _lib.pf_op_desc, keep=()):
+        self._keep = keep
+        h = C.c_void_p()
+        _lib.check(_lib.lib().pf_op_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        info = self.info()
+        self.imsizey, self.imsizex = info.imsizey, info.imsizex
+        self.M = info.M
+        self.precision = info.precision
+        self.grid_shape = (info.Nv, info.Nu)
+
+    # -- lifetime
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.lib().pf_op_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- info / norm
+    def info(self):
+        inf = _lib.pf_op_info()
+        _lib.check(_lib.lib().pf_op_get_info(self._h, C.byref(inf)))
+        return inf
+
+    @property
+    def norm(self):
+        d = C.c_double()
+        _lib.check(_lib.lib().pf_op_get_norm(self._h, C.byref(d)))
+        return d.value
+
+    def set_norm(self, norm):
+        _lib.check(_lib.lib().pf_op_set_norm(self._h, float(norm)))
+
+    def power_method(self, iters, tol, start=None):
+        """||Phi||_2 of the operator as it is now (SOPT power_method, operators.h:390-391)."""
+        out = C.c_double()
+        if start is None:
+            _lib.check(_lib.lib().pf_op_power_method(self._h, int(iters), float(tol), None, C.byref(out)))
+        else:
+            s = _c128(start)
+            _lib.check(_lib.lib().pf_op_power_method(self._h, int(iters), float(tol), s.ctypes.data, C.byref(out)))
+        return out.value
+
+    # -- per-kernel timing
+    def profile(self, on=True):
+        """Enable (and clear) or disable HIP-event timing of every kernel this operator launches."""
+        _lib.check(_lib.lib().pf_op_profile_enable(self._h, int(bool(on))))
+
+    def profile_table(self):
+        """{kernel name: (total ms, launches)} since profiling was enabled."""
+        out = {}
+        name = C.create_string_buffer(64)
+        tot = C.c_double()
+        cnt = C.c_int64()
+        i = 0
+        while _lib.lib().pf_op_profile_get(self._h, i, name, 64, C.byref(tot), C.byref(cnt)) == _lib.PF_OK:
+            out[name.value.decode()] = (tot.value, cnt.value)
+            i += 1
+        return out
+
+    # -- applies
+    def _dtype(self):
+        torch = _torch()
+        return torch.complex64 if self.precision == _lib.PF_C64 else torch.complex128
+
+    def _apply(self, adjoint, x, out=None, stream=None):
+        nin = self.M if adjoint else self.imsizex * self.imsizey
+        nout = self.imsizex * self.imsizey if adjoint else self.M
+        fn = _lib.lib().pf_op_adjoint if adjoint else _lib.lib().pf_op_forward
+        if _is_device_tensor(x):
+            torch = _torch()
+            if x.dtype != self._dtype():
+                raise TypeError(f"device input must be {self._dtype()} for this operator")
+            x = x.contiguous()
+            if x.numel() != nin:
+                raise ValueError(f"input has {x.numel()} entries, expected {nin}")
+            if out is None:
+                out = torch.empty(nout, dtype=x.dtype, device=x.device)
+            s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+            _lib.check(fn(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
+                          _lib.PF_DEVICE, C.c_void_p(s)))
+            return out
+        xh = _c128(x).ravel()
+        if xh.size != nin:
+            raise ValueError(f"input has {xh.size} entries, expected {nin}")
+        yh = np.empty(nout, np.complex128)
+        _lib.check(fn(self._h, xh.ctypes.data, yh.ctypes.data, _lib.PF_HOST, _lib.PF_HOST, None))
+        return yh
+
+    def forward(self, x, out=None, stream=None):
+        return self._apply(False, x, out, stream)
+
+    def adjoint_apply(self, y, out=None, stream=None):
+        return self._apply(True, y, out, stream)
+
+    # -- stage access (device tensors, op precision; no norm)
+    def _stage(self, name, x, nout):
+        torch = _torch()
+        x = x.contiguous()
+        out = torch.empty(nout, dtype=x.dtype, device=x.device)
+        s = torch.cuda.current_stream(x.device).cuda_stream
+        _lib.check(getattr(_lib.lib(), name)(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()),
+                                             C.c_void_p(s)))
+        return out
+
+    def apply_G(self, grid):
+        return self._stage("pf_op_apply_G", grid, self.M)
+
+    def apply_GH(self, y):
+        return self._stage("pf_op_apply_GH", y, self.grid_shape[0] * self.grid_shape[1])
+
+    def apply_FZ(self, x):
+        return self._stage("pf_op_apply_FZ", x, self.grid_shape[0] * self.grid_shape[1])
+
+    def apply_FZ_adjoint(self, grid):
+        return self._stage("pf_op_apply_FZ_adjoint", grid, self.imsizex * self.imsizey)
+
+
+class _Adjoint:
+    def __init__(self, lt):
+        self._lt = lt
+
+    def __mul__(self, y):
+        return self._lt.op.adjoint_apply(y)
+
+    __call__ = __mul__
+
+    def adjoint(self):
+        return self._lt
+
+    @property
+    def sizes(self):
+        return self._lt.sizes[::-1]
+
+
+class LinearTransform:
+    """sopt::LinearTransform-like pair: `lt * x` is Phi x, `lt.adjoint() * y` is Phi^dagger y."""
+
+    def __init__(self, op: MeasurementOperator):
+        self.op = op
+        self.sizes = (op.M, op.imsizex * op.imsizey)  # (rows, cols)
+
+    def __mul__(self, x):
+        return self.op.forward(x)
+
+    __call__ = __mul__
+
+    def adjoint(self):
+        return _Adjoint(self)
+
+    @property
+    def norm(self):
+        return self.op.norm
+
+
+def _make_desc(u, v, w, weights, imsizey, imsizex, oversample_ratio, power_iters, power_tol, kernel, Ju, Jv,
+               w_term, Jw, cellx, celly, units, precision, tile=(0, 0)):
+    u, v = _f64(u).ravel(), _f64(v).ravel()
+    M = u.size
+    if v.size != M:
+        raise RuntimeError("Size of u and v vectors are not the same for creating gridding matrix.")
+    w = np.zeros(M) if w is None else _f64(w).ravel()
+    if w_term and w.size != M:
+        raise RuntimeError("Size of u and w vectors are not the same for creating gridding matrix.")
+    wt = None if weights is None else _c128(weights).ravel()
+    if wt is not None and wt.size != M:
+        raise RuntimeError("Size of u and w vectors are not the same for creating gridding matrix.")
+    d = _lib.pf_op_desc()
+    _lib.lib().pf_op_desc_init(C.byref(d))
+    d.imsizey, d.imsizex = int(imsizey), int(imsizex)
+    d.oversample_ratio = float(oversample_ratio)
+    d.kernel = int(_kernel.coerce(kernel))
+    d.Ju, d.Jv = int(Ju), int(Jv)
+    d.w_term, d.Jw = int(bool(w_term)), int(Jw)
+    d.cellx, d.celly = float(cellx), float(celly)
+    d.units = int(units)
+    d.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    d.M = M
+    d.u, d.v, d.w = u.ctypes.data, v.ctypes.data, w.ctypes.data
+    d.weights = None if wt is None else wt.ctypes.data
+    d.power_iters = int(power_iters)
+    d.power_tol = float(power_tol)
+    d.tile_u, d.tile_v = int(tile[0]), int(tile[1])
+    return d, (u, v, w, wt)
+
+
+class measurementoperator:  # noqa: N801 -- mirrors the reference's namespace
+    @staticmethod
+    def init_degrid_operator_2d(*args, **kw):
+        """Both reference overloads:
+        init_degrid_operator_2d(u, v, w, weights, imsizey, imsizex, oversample_ratio=2, power_iters=100,
+                                power_tol=1e-4, kernel=kb, Ju=4, Jv=4, w_term=False, Jw=6, cellx=1, celly=1)
+            -- u, v in pixels (operators.h:373-396)
+        init_degrid_operator_2d(uv_vis, imsizey, imsizex, cell_x, cell_y, oversample_ratio=2, power_iters=100,
+                                power_tol=1e-4, kernel=kb, Ju=4, Jv=4, w_term=False, Jw=6)
+            -- vis_params in lambda / radians / pixels (operators.h:398-414)
+        Extra keywords: precision ("c64" default, or "c128"), tile=(tile_u, tile_v).
+        """
+        precision = kw.pop("precision", "c64")
+        tile = kw.pop("tile", (0, 0))
+        if args and isinstance(args[0], vis_params):
+            return _init_from_vis(*args, precision=precision, tile=tile, **kw)
+        return _init_raw(*args, precision=precision, tile=tile, **kw)
+
+
+def _init_raw(u, v, w, weights, imsizey, imsizex, oversample_ratio=2.0, power_iters=100, power_tol=1e-4,
+              kernel=_kernel.kb, Ju=4, Jv=4, w_term=False, Jw=6, cellx=1.0, celly=1.0, precision="c64", tile=(0, 0)):
+    d, keep = _make_desc(u, v, w, weights, imsizey, imsizex, oversample_ratio, power_iters, power_tol, kernel, Ju,
+                         Jv, w_term, Jw, cellx, celly, _lib.PF_PIXELS, precision, tile)
+    return LinearTransform(MeasurementOperator(d, keep))
+
+
+def _init_from_vis(uv_vis, imsizey, imsizex, cell_x, cell_y, oversample_ratio=2.0, power_iters=100,
+                   power_tol=1e-4, kernel=_kernel.kb, Ju=4, Jv=4, w_term=False, Jw=6, precision="c64", tile=(0, 0)):
+    units = {vis_units.lambda_: _lib.PF_LAMBDA, vis_units.radians: _lib.PF_RADIANS,
+             vis_units.pixels: _lib.PF_PIXELS}[uv_vis.units]
+    d, keep = _make_desc(uv_vis.u, uv_vis.v, uv_vis.w, uv_vis.weights, imsizey, imsizex, oversample_ratio,
+                         power_iters, power_tol, kernel, Ju, Jv, w_term, Jw, cell_x, cell_y, units, precision, tile)
+    return LinearTransform(MeasurementOperator(d, keep))
+
+
+# factory::measurement_operator_factory (cpp/purify/measurement_operator_factory.h:20-96)
+class distributed_measurement_operator:  # noqa: N801
+    serial = "serial"
+    gpu_serial = "gpu_serial"
+    mpi_distribute_image = "mpi_distribute_image"
+    mpi_distribute_grid = "mpi_distribute_grid"
+    gpu_mpi_distribute_image = "gpu_mpi_distribute_image"
+    gpu_mpi_distribute_grid = "gpu_mpi_distribute_grid"
+
+
+def measurement_operator_factory(dist, *args, **kw):
+    """Builds the operator on the GPU for `serial` and `gpu_serial`; the MPI variants have no
+    single-GPU meaning here and raise, as the reference does when built without MPI."""
+    if dist in (distributed_measurement_operator.serial, distributed_measurement_operator.gpu_serial):
+        return measurementoperator.init_degrid_operator_2d(*args, **kw)
+    raise RuntimeError("Compile with MPI if you want to use MPI distributed operators.")

@@ -1,0 +1,98 @@
+"""vis_params and the synthetic-input generator (cpp/purify/utilities.h:13-55, utilities.cc:63-101,422-458).
+
+The reference's random_sample_density and add_noise reseed from std::random_device on every draw
+(utilities.cc:72-75, 443-446), so its simulations are not reproducible.  The generator here is
+seeded: std::mt19937_64 with Box-Muller normals, implemented in the HIP library's host code
+(pf_synth_*), so the same seed gives the same inputs in C++, Python and the benchmark.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+
+class vis_units(enum.Enum):  # noqa: N801
+    lambda_ = 0
+    radians = 1
+    pixels = 2
+
+
+@dataclass
+class vis_params:  # noqa: N801
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray = None
+    vis: np.ndarray = None
+    weights: np.ndarray = None
+    units: vis_units = vis_units.lambda_
+    ra: float = 0.0
+    dec: float = 0.0
+    average_frequency: float = 0.0
+
+    def __post_init__(self):
+        self.u = np.ascontiguousarray(self.u, np.float64)
+        self.v = np.ascontiguousarray(self.v, np.float64)
+        M = self.u.size
+        self.w = np.zeros(M) if self.w is None else np.ascontiguousarray(self.w, np.float64)
+        self.vis = np.ones(M, np.complex128) if self.vis is None else np.ascontiguousarray(self.vis, np.complex128)
+        self.weights = (np.ones(M, np.complex128) if self.weights is None
+                        else np.ascontiguousarray(self.weights, np.complex128))
+
+    def size(self):
+        return self.vis.size
+
+
+def synth_uv(seed, M, sigma):
+    """u, v ~ N(0, sigma) radians, clipped to [-pi, pi]."""
+    u = np.empty(M)
+    v = np.empty(M)
+    _lib.check(_lib.lib().pf_synth_uv(C.c_uint64(seed), M, float(sigma), u.ctypes.data, v.ctypes.data))
+    return u, v
+
+
+def synth_image(seed, ny, nx, nsrc=20):
+    """Sum of nsrc Gaussian sources (centre U[0,n)^2, amplitude U[0.1,1], sigma U[1,8] px), complex128."""
+    img = np.empty(ny * nx, np.complex128)
+    _lib.check(_lib.lib().pf_synth_image(C.c_uint64(seed), ny, nx, nsrc, img.ctypes.data))
+    return img
+
+
+def synth_cnormal(seed, n, scale=1.0):
+    """Complex normal vector, E|z|^2 = scale^2."""
+    z = np.empty(n, np.complex128)
+    _lib.check(_lib.lib().pf_synth_cnormal(C.c_uint64(seed), n, float(scale), z.ctypes.data))
+    return z
+
+
+def radians_to_pixels(u_rad, n, os_=2.0):
+    """utilities::uv_scale (utilities.cc:258-278): u / (2 pi) * floor(os * n)."""
+    return np.asarray(u_rad) / (2 * np.pi) * np.floor(os_ * n)
+
+
+def SNR_to_standard_deviation(y0, snr=30.0):
+    """utilities.cc:422-432: ||y0||_2 / sqrt(M) * 10^(-snr/20)."""
+    y0 = np.asarray(y0)
+    return np.linalg.norm(y0) / np.sqrt(y0.size) * 10.0 ** (-snr / 20.0)
+
+
+def calculate_l2_radius(y_size, sigma=0.0, n_sigma=2.0, distribution="chi"):
+    """utilities.cc:396-421: the l2-ball radius epsilon."""
+    if distribution == "chi^2":
+        r = np.sqrt(2 * y_size + n_sigma * np.sqrt(4 * y_size))
+        return r if sigma == 0 else r * sigma
+    if distribution == "chi":
+        alpha = 1.0 / (8 * y_size) - 1.0 / (128 * y_size * y_size)
+        mean = np.sqrt(2) * np.sqrt(y_size) * (1 - alpha)
+        sd = np.sqrt(2 * y_size * alpha * (2 - alpha))
+        return mean + n_sigma * sd if sigma == 0 else (mean + n_sigma * sd) * sigma
+    return 1.0
+
+
+def add_noise(y0, sigma, seed):
+    """utilities.cc:434-458 with a seeded generator: re, im ~ N(0, sigma/sqrt(2))."""
+    return np.asarray(y0) + synth_cnormal(seed, np.asarray(y0).size, sigma)
