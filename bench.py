@@ -101,6 +101,9 @@ def main():
                                                          precision=prec)
         setup = time.perf_counter() - t0
         op = lt.op
+        t0 = time.perf_counter()
+        variant = op.autotune(3)
+        tune_s = time.perf_counter() - t0
         dt = torch.complex64 if prec == "c64" else torch.complex128
         x = torch.from_numpy(x_h).to(dt).to(dev)
         yin = torch.from_numpy(y_h).to(dt).to(dev)
@@ -132,8 +135,9 @@ def main():
         op.profile(False)
         outs[prec] = (y.cpu().numpy().astype(np.complex128), xo.cpu().numpy().astype(np.complex128))
         res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
-                         value=M * steps / elapsed, lat=lat, table=table)
-        log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s")
+                         value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s)
+        log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s, "
+            f"grid kernel {variant}")
         del lt, op, x, yin, y, xo
         torch.cuda.empty_cache()
 
@@ -217,6 +221,8 @@ def main():
         "rel_err_vs_oracle": acc_oracle,
         "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "setup_s": {"c64": r64["setup_s"], "c128": res["c128"]["setup_s"]},
+        "autotune": {"grid_kernel": {"c64": r64["grid_kernel"], "c128": res["c128"]["grid_kernel"]},
+                     "seconds": {"c64": r64["tune_s"], "c128": res["c128"]["tune_s"]}},
         "kernels_ms_per_launch": {k: tab[k][0] / tab[k][1] for k in tab},
     }
     print(json.dumps(out), flush=True)

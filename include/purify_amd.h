@@ -125,6 +125,16 @@ int pf_fft2d(int32_t Nv, int32_t Nu, int32_t precision, int32_t sign, const void
 int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ftsize, int32_t precision,
                    const double *x_host, double *out_host, int64_t n);
 
+/* ---- tuning (no counterpart in the reference: FFTW_MEASURE planning, operators.h:215-223, is its
+ *      closest analogue) ---- */
+/* Keys: "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics; both deterministic); read-only
+ * "tile_u", "tile_v".  Results do not depend on the setting beyond floating-point summation order. */
+int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
+int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
+/* Times every gridding-kernel variant on this operator (reps adjoint applies each, HIP events on the
+ * null stream) and keeps the fastest.  Writes the chosen variant to *chosen if non-NULL. */
+int pf_op_autotune(pf_op *op, int32_t reps, int64_t *chosen);
+
 /* ---- per-kernel timing (no counterpart in the reference, whose operators log at construction only,
  *      operators.h:274-290) ---- */
 /* on = 1 enables (and clears) HIP-event timing of every kernel the operator launches, recorded on the

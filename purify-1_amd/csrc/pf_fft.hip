@@ -63,13 +63,19 @@ __device__ inline void stage_bfly(cpx<R> (&v)[E], int t, int T, int N, int Ns, c
   }
 }
 
+// LDS position of line element i.  es > 0: element stride es (column tiles, lines interleaved, which
+// is conflict-free because consecutive lanes take consecutive lines).  es == 0: a contiguous line
+// padded by one element every 16, so the strided Stockham writes of the early stages spread over the
+// banks instead of hitting the same ones.
+__device__ inline int lidx(int i, int es) { return es ? i * es : i + (i >> 4); }
+
 template <typename R, int E, int RAD>
 __device__ inline void stage_read_lds(cpx<R> (&v)[E], int t, int T, int N, const cpx<R>* lds, int es) {
 #pragma unroll
   for (int b = 0; b < E / RAD; ++b) {
     const int j = t + b * T;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[(j + r * (N / RAD)) * es];
+    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[lidx(j + r * (N / RAD), es)];
   }
 }
 
@@ -81,7 +87,7 @@ __device__ inline void stage_write_lds(const cpx<R> (&v)[E], int t, int T, int N
     const int k = j & (Ns - 1);
     const int o = (j - k) * RAD + k;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) lds[(o + r * Ns) * es] = v[b * RAD + r];
+    for (int r = 0; r < RAD; ++r) lds[lidx(o + r * Ns, es)] = v[b * RAD + r];
   }
 }
 
@@ -160,6 +166,11 @@ __device__ inline void fft_line(const LinePlan& pl, int t, cpx<R>* lds, int es, 
   }
 }
 
+// Dynamic LDS above 64 KiB (gfx950 has 160 KiB per CU) must be opted into per kernel.
+static void allow_lds(const void* fn, size_t shm) {
+  if (shm > 65536) PF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+}
+
 // ---------------------------------------------------------------------------------------------
 // Row pass
 // ---------------------------------------------------------------------------------------------
@@ -181,7 +192,7 @@ __global__ void __launch_bounds__(512) k_fft_rows(RowArgs<R> a) {
   const int l = threadIdx.x / T, t = threadIdx.x - l * T;
   const int line = blockIdx.x * a.lpb + l;
   const bool valid = line < a.nlines;
-  cpx<R>* lds = smem + l * N;
+  cpx<R>* lds = smem + l * (N + N / 16);
   const SZ<R>& sz = a.sz;
   auto load = [&](int idx) -> cpx<R> {
     if (!valid) return mk<R>(R(0), R(0));
@@ -209,7 +220,7 @@ __global__ void __launch_bounds__(512) k_fft_rows(RowArgs<R> a) {
       a.out[(int64_t)line * N + idx] = val;
     }
   };
-  fft_line<R, E, INV>(a.pl, t, lds, 1, a.tw, a.twm, load, store);
+  fft_line<R, E, INV>(a.pl, t, lds, 0, a.tw, a.twm, load, store);
 }
 
 template <typename R, int E, bool INV>
@@ -218,12 +229,15 @@ static void launch_rows_e(const Fft2dPlan& p, int mode, const RowArgs<R>& a, int
   ProfScope prof(mode == ROW_FWD_SZ ? "fft_rows_fwd_sz" : mode == ROW_ADJ_CROP ? "fft_rows_adj_crop" : "fft_rows", s);
   switch (mode) {
     case ROW_FWD_SZ:
+      allow_lds(reinterpret_cast<const void*>(&k_fft_rows<R, E, INV, ROW_FWD_SZ>), shm);
       hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_FWD_SZ>), dim3(blocks), dim3(threads), shm, s, a);
       break;
     case ROW_ADJ_CROP:
+      allow_lds(reinterpret_cast<const void*>(&k_fft_rows<R, E, INV, ROW_ADJ_CROP>), shm);
       hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_ADJ_CROP>), dim3(blocks), dim3(threads), shm, s, a);
       break;
     default:
+      allow_lds(reinterpret_cast<const void*>(&k_fft_rows<R, E, INV, ROW_PLAIN>), shm);
       hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_PLAIN>), dim3(blocks), dim3(threads), shm, s, a);
   }
 }
@@ -248,7 +262,7 @@ void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* 
   if (lpb > nlines) lpb = nlines;
   a.lpb = lpb;
   const int threads = lpb * T;
-  const size_t shm = (size_t)lpb * N * sizeof(cpx<R>);
+  const size_t shm = (size_t)lpb * (N + N / 16) * sizeof(cpx<R>);
   const int blocks = (nlines + lpb - 1) / lpb;
   if (blocks == 0) return;
 #define PF_ROWS(EE)                                                      \

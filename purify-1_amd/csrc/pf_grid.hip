@@ -57,47 +57,178 @@ __device__ inline void taps(const KParams<R>& p, int k, R f, int Jrt, R (&w)[J =
   }
 }
 
+// Tile kernels keep the hardware's round-robin spread of workgroups over the 8 XCDs: the visibility
+// density is concentrated in the centre of the uv-plane, and giving each XCD a contiguous band of
+// tiles (to share halos in one L2) would put all heavy tiles on two XCDs.
+
 // ---------------------------------------------------------------------------------------------
-// Degridding  y = G grid
+// Degridding  y = G grid: one block per tile stages the tile plus its (J-1)-wide halo in LDS, then
+// every visibility whose first tap lies in the tile reads its J x J taps from LDS.
 // ---------------------------------------------------------------------------------------------
 template <typename R, int JU, int JV>
-__global__ void __launch_bounds__(256) k_degrid(GridGeom<R> g, VisData<R> v, const cpx<R>* __restrict__ grid,
-                                                cpx<R>* __restrict__ y, R scl) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= v.M) return;
+__global__ void __launch_bounds__(256) k_degrid_tiles(GridGeom<R> g, VisData<R> v, TileLists tl,
+                                                      const cpx<R>* __restrict__ grid, cpx<R>* __restrict__ y, R scl) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int tile = tl.order ? (int)tl.order[blockIdx.x] : (int)blockIdx.x;
+  const uint32_t beg = tl.owner[tile], end = tl.owner[tile + 1];
+  if (beg == end) return;
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
-  const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
-  R wu[JU == 0 ? JMAX : JU], wv[JV == 0 ? JMAX : JV];
-  taps<R, JU>(g.ku, c.ku, c.fu, Ju, wu);
-  taps<R, JV>(g.kv, c.kv, c.fv, Jv, wv);
-  const int q0 = wrap(c.ku + 1, g.Nu), p0 = wrap(c.kv + 1, g.Nv);
-  cpx<R> acc = mk<R>(R(0), R(0));
-  constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
-#pragma unroll
-  for (int b = 0; b < JJV; ++b) {
-    if (JV == 0 && b >= Jv) break;
-    int p = p0 + b;
-    if (p >= g.Nv) p -= g.Nv;
-    const cpx<R>* row = grid + (int64_t)p * g.Nu;
-    cpx<R> racc = mk<R>(R(0), R(0));
-#pragma unroll
-    for (int a = 0; a < JJU; ++a) {
-      if (JU == 0 && a >= Ju) break;
-      int q = q0 + a;
-      if (q >= g.Nu) q -= g.Nu;
-      const cpx<R> x = row[q];
-      racc.x += wu[a] * x.x;
-      racc.y += wu[a] * x.y;
-    }
-    acc.x += wv[b] * racc.x;
-    acc.y += wv[b] * racc.y;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * tl.TU, ty0 = tv * tl.TV;
+  const int W = tl.TU + Ju - 1, H = tl.TV + Jv - 1;
+  for (int e = threadIdx.x; e < W * H; e += blockDim.x) {
+    const int ly = e / W, lx = e - ly * W;
+    int gy = ty0 + ly, gx = tx0 + lx;
+    if (gy >= g.Nv) gy -= g.Nv;
+    if (gx >= g.Nu) gx -= g.Nu;
+    lds[e] = grid[(int64_t)gy * g.Nu + gx];
   }
-  if (v.weights) acc = cmul(acc, v.weights[i]);
-  y[v.perm[i]] = scale(acc, scl);
+  __syncthreads();
+  constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
+  for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+    const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
+    R wu[JJU], wv[JJV];
+    taps<R, JU>(g.ku, c.ku, c.fu, Ju, wu);
+    taps<R, JV>(g.kv, c.kv, c.fv, Jv, wv);
+    const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
+    const cpx<R>* base = lds + ly0 * W + lx0;
+    cpx<R> acc = mk<R>(R(0), R(0));
+#pragma unroll
+    for (int b = 0; b < JJV; ++b) {
+      if (JV == 0 && b >= Jv) break;
+      cpx<R> racc = mk<R>(R(0), R(0));
+#pragma unroll
+      for (int a = 0; a < JJU; ++a) {
+        if (JU == 0 && a >= Ju) break;
+        const cpx<R> x = base[b * W + a];
+        racc.x += wu[a] * x.x;
+        racc.y += wu[a] * x.y;
+      }
+      acc.x += wv[b] * racc.x;
+      acc.y += wv[b] * racc.y;
+    }
+    if (v.weights) acc = cmul(acc, v.weights[i]);
+    y[v.perm[i]] = scale(acc, scl);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Gridding  grid = G^H y   (one 64-lane wave per TU x TV tile)
+// Gridding  grid = G^H y, one 64-lane wave per TU x TV tile, no atomics.
+// Lane (a, b) = (lane & 7, lane >> 3) owns every tile cell (x, y) with x = a, y = b (mod 8).  For a
+// footprint [x0, x0+Ju) x [y0, y0+Jv) with Ju, Jv <= 8 each tap therefore belongs to a different
+// lane, and a cell is only ever touched by its owning lane.  Each lane sums into a register while
+// consecutive visibilities hit its current cell and folds the sum into the LDS tile (a plain
+// read-modify-write) when its cell changes; visibilities are sorted by first-tap cell inside a tile,
+// so that is rare.  The order of every cell's sum is the list order: reproducible bit for bit.
+// Requires Nu >= TU + 8 and Nv >= TV + 8 (a footprint meets a tile from one side only).
+// ---------------------------------------------------------------------------------------------
+// One (weight, coordinate) entry of a visibility's residue-rotated tap table.  Inactive taps carry
+// weight 0 and coordinate RC_OFF, so the sum of the u and v coordinates is negative.
+template <typename R>
+struct alignas(2 * sizeof(R)) RCTap {
+  R w;
+  int c;
+};
+constexpr int RC_OFF = -(1 << 20);
+
+template <typename R, int JU, int JV>
+__global__ void __launch_bounds__(64) k_grid_rc(GridGeom<R> g, VisData<R> v, TileLists tl,
+                                                const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
+  extern __shared__ unsigned char smem_raw[];
+  const int TU = tl.TU, TV = tl.TV, P = TU + 1;  // padded row stride of the LDS tile
+  cpx<R>* acc = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* sv = acc + TV * P;                                    // 64 scaled visibility values
+  RCTap<R>* tu_tab = reinterpret_cast<RCTap<R>*>(sv + 64);     // [64][8]: entry a = tap of residue a
+  RCTap<R>* tv_tab = tu_tab + 64 * 8;                           // [64][8]: coordinates pre-multiplied by P
+  const int tile = tl.order ? (int)tl.order[blockIdx.x] : (int)blockIdx.x;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * TU, ty0 = tv * TV;
+  const int lane = threadIdx.x, la = lane & 7, lb = lane >> 3;
+  for (int e = lane; e < TV * P; e += 64) acc[e] = mk<R>(R(0), R(0));
+  int cur = -1;
+  cpx<R> accv = mk<R>(R(0), R(0));
+  const uint32_t end = tl.ptr[tile + 1];
+  for (uint32_t base = tl.ptr[tile]; base < end; base += 64) {
+    const int nb = (int)min((uint32_t)64, end - base);
+    if (lane < nb) {
+      // visibility `lane` of the batch: value, tap weights and their residue-rotated placement
+      const uint32_t i = tl.list[base + lane];
+      const VisCoord<R> c = vis_coord(v, i, JU, JV);
+      cpx<R> s = yin[v.perm[i]];
+      if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
+      sv[lane] = scale(s, scl);
+      R wu[JU], wv[JV];
+      taps<R, JU>(g.ku, c.ku, c.fu, JU, wu);
+      taps<R, JV>(g.kv, c.kv, c.fv, JV, wv);
+      int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
+      if (dx >= g.Nu - 8) dx -= g.Nu;
+      if (dy >= g.Nv - 8) dy -= g.Nv;
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const int x = dx + d;
+        const bool ok = d < JU && (unsigned)x < (unsigned)TU;
+        RCTap<R> e;
+        e.w = ok ? wu[d < JU ? d : 0] : R(0);
+        e.c = ok ? x : RC_OFF;
+        tu_tab[lane * 8 + (x & 7)] = e;
+      }
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const int y = dy + d;
+        const bool ok = d < JV && (unsigned)y < (unsigned)TV;
+        RCTap<R> e;
+        e.w = ok ? wv[d < JV ? d : 0] : R(0);
+        e.c = ok ? y * P : RC_OFF;
+        tv_tab[lane * 8 + (y & 7)] = e;
+      }
+    }
+    __syncthreads();
+    // every lane reads its own residue's entries: three independent LDS reads per visibility
+    RCTap<R> eu = tu_tab[la], ev = tv_tab[lb];
+    cpx<R> s = sv[0];
+    for (int t = 0; t < nb; ++t) {
+      RCTap<R> eu_n = eu, ev_n = ev;
+      cpx<R> s_n = s;
+      if (t + 1 < nb) {
+        eu_n = tu_tab[(t + 1) * 8 + la];
+        ev_n = tv_tab[(t + 1) * 8 + lb];
+        s_n = sv[t + 1];
+      }
+      const int cell = eu.c + ev.c;
+      if (cell >= 0) {
+        if (cell != cur) {
+          if (cur >= 0) {
+            const cpx<R> a0 = acc[cur];
+            acc[cur] = mk<R>(a0.x + accv.x, a0.y + accv.y);
+          }
+          cur = cell;
+          accv = mk<R>(R(0), R(0));
+        }
+        const R w = eu.w * ev.w;
+        accv.x += s.x * w;
+        accv.y += s.y * w;
+      }
+      eu = eu_n;
+      ev = ev_n;
+      s = s_n;
+    }
+    __syncthreads();
+  }
+  if (cur >= 0) {
+    const cpx<R> a0 = acc[cur];
+    acc[cur] = mk<R>(a0.x + accv.x, a0.y + accv.y);
+  }
+  __syncthreads();
+  for (int e = lane; e < TU * TV; e += 64) {
+    const int ly = e / TU, lx = e - ly * TU;
+    grid[(int64_t)(ty0 + ly) * g.Nu + tx0 + lx] = acc[ly * P + lx];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gridding fallback for tiny grids or supports above 8: the tile wave adds with LDS atomics.  Only
+// this wave touches its tile, so inter-wave order never matters.
 // ---------------------------------------------------------------------------------------------
 template <typename R, int JU, int JV>
 __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, TileLists tl,
@@ -123,7 +254,7 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
       const uint32_t i = tl.list[idx];
       const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
       cpx<R> s = yin[v.perm[i]];
-      if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
+      if (v.weights) s = cmulc(s, v.weights[i]);
       s = scale(s, scl);
       R wu[JJU], wv[JJV];
       taps<R, JU>(g.ku, c.ku, c.fu, Ju, wu);
@@ -158,12 +289,13 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
 }
 
 template <typename R>
-void degrid(const GridGeom<R>& g, const VisData<R>& v, const cpx<R>* grid, cpx<R>* y, R scale, hipStream_t s) {
+void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
+            hipStream_t s) {
   if (v.M == 0) return;
-  const int threads = 256;
-  const dim3 blocks((unsigned)((v.M + threads - 1) / threads));
+  const dim3 blocks((unsigned)(t.ntu * t.ntv));
+  const size_t shm = sizeof(cpx<R>) * (size_t)(t.TU + g.Ju - 1) * (t.TV + g.Jv - 1);
   ProfScope prof("degrid", s);
-#define PF_DG(A, B) hipLaunchKernelGGL((k_degrid<R, A, B>), blocks, dim3(threads), 0, s, g, v, grid, y, scale)
+#define PF_DG(A, B) hipLaunchKernelGGL((k_degrid_tiles<R, A, B>), blocks, dim3(256), shm, s, g, v, t, grid, y, scale)
   if (g.Ju == g.Jv) {
     switch (g.Ju) {
       case 2: PF_DG(2, 2); break;
@@ -182,12 +314,46 @@ void degrid(const GridGeom<R>& g, const VisData<R>& v, const cpx<R>* grid, cpx<R
   PF_HIP(hipGetLastError());
 }
 
+bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv) {
+  return Ju <= 8 && Jv <= 8 && Nu >= TU + 8 && Nv >= TV + 8;
+}
+
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* y, cpx<R>* grid, R scale,
-                hipStream_t s) {
+                hipStream_t s, int variant) {
   const dim3 blocks((unsigned)(t.ntu * t.ntv));
+  if (variant == GRID_RESIDUE && grid_rc_ok(g.Nu, g.Nv, t.TU, t.TV, g.Ju, g.Jv)) {
+    const size_t shm = sizeof(cpx<R>) * ((size_t)(t.TU + 1) * t.TV + 64) + 2 * sizeof(R) * 2 * 64 * 8;
+    ProfScope prof("grid_tiles", s);
+#define PF_RC(A, B) hipLaunchKernelGGL((k_grid_rc<R, A, B>), blocks, dim3(64), shm, s, g, v, t, y, grid, scale)
+#define PF_RC_V(A)             \
+  switch (g.Jv) {              \
+    case 1: PF_RC(A, 1); break; \
+    case 2: PF_RC(A, 2); break; \
+    case 3: PF_RC(A, 3); break; \
+    case 4: PF_RC(A, 4); break; \
+    case 5: PF_RC(A, 5); break; \
+    case 6: PF_RC(A, 6); break; \
+    case 7: PF_RC(A, 7); break; \
+    default: PF_RC(A, 8);       \
+  }
+    switch (g.Ju) {
+      case 1: PF_RC_V(1) break;
+      case 2: PF_RC_V(2) break;
+      case 3: PF_RC_V(3) break;
+      case 4: PF_RC_V(4) break;
+      case 5: PF_RC_V(5) break;
+      case 6: PF_RC_V(6) break;
+      case 7: PF_RC_V(7) break;
+      default: PF_RC_V(8)
+    }
+#undef PF_RC_V
+#undef PF_RC
+    PF_HIP(hipGetLastError());
+    return;
+  }
   const size_t shm = 2 * sizeof(R) * (size_t)t.TU * t.TV;
-  ProfScope prof("grid_tiles", s);
+  ProfScope prof("grid_tiles_atomic", s);
 #define PF_GT(A, B) hipLaunchKernelGGL((k_grid_tiles<R, A, B>), blocks, dim3(64), shm, s, g, v, t, y, grid, scale)
   if (g.Ju == g.Jv) {
     switch (g.Ju) {
@@ -207,14 +373,14 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
   PF_HIP(hipGetLastError());
 }
 
-template void degrid<float>(const GridGeom<float>&, const VisData<float>&, const cpx<float>*, cpx<float>*, float,
-                            hipStream_t);
-template void degrid<double>(const GridGeom<double>&, const VisData<double>&, const cpx<double>*, cpx<double>*,
-                             double, hipStream_t);
+template void degrid<float>(const GridGeom<float>&, const VisData<float>&, const TileLists&, const cpx<float>*,
+                            cpx<float>*, float, hipStream_t);
+template void degrid<double>(const GridGeom<double>&, const VisData<double>&, const TileLists&, const cpx<double>*,
+                             cpx<double>*, double, hipStream_t);
 template void grid_tiles<float>(const GridGeom<float>&, const VisData<float>&, const TileLists&, const cpx<float>*,
-                                cpx<float>*, float, hipStream_t);
+                                cpx<float>*, float, hipStream_t, int);
 template void grid_tiles<double>(const GridGeom<double>&, const VisData<double>&, const TileLists&,
-                                 const cpx<double>*, cpx<double>*, double, hipStream_t);
+                                 const cpx<double>*, cpx<double>*, double, hipStream_t, int);
 
 // ---------------------------------------------------------------------------------------------
 // Vector helpers: FP64 norms (fixed-order two-level reduction), scaling, precision conversion

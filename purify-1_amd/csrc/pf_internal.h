@@ -122,15 +122,21 @@ struct GridGeom {
 
 struct TileLists {
   int TU = 32, TV = 32, ntu = 0, ntv = 0;
-  const uint32_t* ptr = nullptr;   // n_tiles + 1
-  const uint32_t* list = nullptr;  // sorted vis indices per tile
+  const uint32_t* ptr = nullptr;    // n_tiles + 1
+  const uint32_t* list = nullptr;   // sorted vis indices per tile (every visibility with a tap in it)
+  const uint32_t* owner = nullptr;  // n_tiles + 1: sorted range of the visibilities whose first tap is in the tile
+  const uint32_t* order = nullptr;  // n_tiles: launch order, heaviest tile first (nullptr: natural order)
 };
 
 template <typename R>
-void degrid(const GridGeom<R>& g, const VisData<R>& v, const cpx<R>* grid, cpx<R>* y, R scale, hipStream_t s);
+void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
+            hipStream_t s);
+bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv);
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* y, cpx<R>* grid,
-                R scale, hipStream_t s);
+                R scale, hipStream_t s, int variant);
+// gridding kernel variants: residue-class lanes (needs J <= 8 and N >= T + 8) or LDS atomics
+enum GridVariant { GRID_RESIDUE = 0, GRID_ATOMIC = 1 };
 
 // ---- projection rows (w_term) -------------------------------------------------------------
 template <typename R>

@@ -198,21 +198,40 @@ KParams<R> make_kparams(int kind, int J, double os) {
   p.box = (R)(1.0 / J);
   p.ncoef = 1;
   if (kind == K_KB || kind == K_KBMIN) {
+    // Chebyshev coefficients of P(t) = I0(alpha sqrt t) / I0(alpha), t = (s + 1) / 2, s in [-1, 1],
+    // from the values at 64 Chebyshev nodes (discrete cosine transform), in long double.
     const long double al = kb_alpha(kind, J, os);
-    const long double q = al * al / 4.0L;
-    const long double i0 = (long double)host_bessel_i0((double)al);
-    const long double cut = sizeof(R) == 8 ? 1e-19L : 1e-10L;
-    long double c = 1.0L;  // (q^k / (k!)^2)
-    int k = 0;
-    for (; k < KB_MAX_COEF; ++k) {
-      p.coef[k] = (R)(c / i0);
-      if (k > 2 && c / i0 < cut) {
-        ++k;
-        break;
+    auto i0l = [](long double z2) {  // I0(sqrt(z2)) by its power series in z2 / 4
+      const long double q = z2 / 4.0L;
+      long double term = 1.0L, sum = 1.0L;
+      for (int k = 1; k < 400; ++k) {
+        term *= q / ((long double)k * k);
+        sum += term;
+        if (term < 1e-22L * sum) break;
       }
-      c *= q / ((long double)(k + 1) * (k + 1));
+      return sum;
+    };
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const int NN = 64;
+    const long double norm = i0l(al * al);
+    long double fv[NN], th[NN];
+    for (int j = 0; j < NN; ++j) {
+      th[j] = pi * (j + 0.5L) / NN;
+      const long double t = (cosl(th[j]) + 1.0L) / 2.0L;
+      fv[j] = i0l(al * al * t) / norm;
     }
-    p.ncoef = k;
+    long double a[KB_MAX_COEF];
+    for (int k = 0; k < KB_MAX_COEF; ++k) {
+      long double sacc = 0;
+      for (int j = 0; j < NN; ++j) sacc += fv[j] * cosl(k * th[j]);
+      a[k] = sacc * 2.0L / NN;
+    }
+    a[0] /= 2;
+    const long double cut = sizeof(R) == 8 ? 1e-18L : 1e-10L;
+    int n = KB_MAX_COEF;
+    while (n > 1 && fabsl(a[n - 1]) < cut) --n;
+    p.ncoef = n;
+    for (int k = 0; k < KB_MAX_COEF; ++k) p.coef[k] = (R)(k < n ? a[k] : 0.0L);
   }
   return p;
 }
@@ -245,6 +264,8 @@ struct pf_op {
   int32_t* d_perm = nullptr;
   uint32_t* d_tptr = nullptr;
   uint32_t* d_tlist = nullptr;
+  uint32_t* d_towner = nullptr;
+  uint32_t* d_torder = nullptr;
   void* d_sx = nullptr;
   void* d_sy = nullptr;
   void* d_gridA = nullptr;
@@ -256,6 +277,7 @@ struct pf_op {
   GridGeom<float> gf;
   GridGeom<double> gd;
   std::unique_ptr<Profiler> prof;  // per-kernel timing when enabled
+  int grid_variant = GRID_RESIDUE;  // tuning knob: gridding kernel
   int64_t device_bytes = 0;
   std::vector<void*> allocs;
 
@@ -387,29 +409,42 @@ void build(pf_op* op, const pf_op_desc& d) {
   std::vector<double> u, v;
   to_pixels(d, op->Nu, op->Nv, u, v);
   const int64_t M = d.M;
-  const int TU = std::min(d.tile_u > 0 ? d.tile_u : 32, op->Nu);
-  const int TV = std::min(d.tile_v > 0 ? d.tile_v : 32, op->Nv);
+  // default tile: 32 x 32, halved on small grids so that N >= T + 8 (the residue-class gridding
+  // kernel's condition) whenever the grid allows it
+  auto pick_tile = [](int req, int N) {
+    if (req > 0) return std::min(req, N);
+    int T = 32;
+    while (T > 8 && N < T + 8) T /= 2;
+    return std::min(T, N);
+  };
+  const int TU = pick_tile(d.tile_u, op->Nu);
+  const int TV = pick_tile(d.tile_v, op->Nv);
   if (!is_pow2(TU) && TU != op->Nu) throw Error(PF_ERR_ARG, "tile_u must be a power of two");
   if (!is_pow2(TV) && TV != op->Nv) throw Error(PF_ERR_ARG, "tile_v must be a power of two");
   const int ntu = op->Nu / TU, ntv = op->Nv / TV;
   op->n_tiles = ntu * ntv;
+  // Counting sort by (owner tile, first-tap cell inside the tile): a tile's visibilities form one
+  // contiguous range, ordered row by row, which keeps the gridding lanes on the same cell.
   std::vector<int64_t> ku(M), kv(M);
-  std::vector<uint32_t> owner(M);
-  std::vector<uint32_t> count(op->n_tiles + 1, 0);
+  std::vector<uint32_t> key(M);
+  const int64_t nkeys = (int64_t)op->n_tiles * TU * TV;
+  std::vector<uint32_t> count(nkeys + 1, 0);
   for (int64_t m = 0; m < M; ++m) {
     if (!std::isfinite(u[m]) || !std::isfinite(v[m])) throw Error(PF_ERR_ARG, "non-finite u/v coordinate");
     ku[m] = (int64_t)std::floor(u[m] - op->Ju * 0.5);  // operators.cc:34-35
     kv[m] = (int64_t)std::floor(v[m] - op->Jv * 0.5);
     const int q0 = wrapi(ku[m] + 1, op->Nu), p0 = wrapi(kv[m] + 1, op->Nv);
-    owner[m] = (uint32_t)((p0 / TV) * ntu + q0 / TU);
-    count[owner[m] + 1]++;
+    const uint32_t tile = (uint32_t)((p0 / TV) * ntu + q0 / TU);
+    key[m] = tile * (uint32_t)(TU * TV) + (uint32_t)((p0 % TV) * TU + q0 % TU);
+    count[key[m] + 1]++;
   }
-  for (int t = 0; t < op->n_tiles; ++t) count[t + 1] += count[t];
+  for (int64_t k = 0; k < nkeys; ++k) count[k + 1] += count[k];
+  std::vector<uint32_t> owner(op->n_tiles + 1);
+  for (int t = 0; t <= op->n_tiles; ++t) owner[t] = count[(int64_t)t * TU * TV];
   std::vector<int32_t> perm(M);
-  {
-    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
-    for (int64_t m = 0; m < M; ++m) perm[fill[owner[m]]++] = (int32_t)m;
-  }
+  for (int64_t m = 0; m < M; ++m) perm[count[key[m]]++] = (int32_t)m;
+  std::vector<uint32_t>().swap(count);
+  std::vector<uint32_t>().swap(key);
   // ---- per-tile gridding lists (every visibility with a tap in the tile) ----
   std::vector<uint32_t> tcount(op->n_tiles + 1, 0);
   auto tile_sets = [&](int64_t m, int* tcols, int& nc, int* trows, int& nr) {
@@ -456,6 +491,20 @@ void build(pf_op* op, const pf_op_desc& d) {
   PF_HIP(hipMemcpy(op->d_perm, perm.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
   op->d_tptr = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
   PF_HIP(hipMemcpy(op->d_tptr, tcount.data(), sizeof(uint32_t) * (op->n_tiles + 1), hipMemcpyHostToDevice));
+  op->d_towner = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
+  PF_HIP(hipMemcpy(op->d_towner, owner.data(), sizeof(uint32_t) * (op->n_tiles + 1), hipMemcpyHostToDevice));
+  op->tl.owner = op->d_towner;
+  {
+    // launch the heaviest tiles first so that the dense centre of the uv-plane does not form the tail
+    std::vector<uint32_t> order(op->n_tiles);
+    for (int t = 0; t < op->n_tiles; ++t) order[t] = (uint32_t)t;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return tcount[a + 1] - tcount[a] > tcount[b + 1] - tcount[b];
+    });
+    op->d_torder = (uint32_t*)op->dalloc(sizeof(uint32_t) * op->n_tiles);
+    PF_HIP(hipMemcpy(op->d_torder, order.data(), sizeof(uint32_t) * op->n_tiles, hipMemcpyHostToDevice));
+    op->tl.order = op->d_torder;
+  }
   op->d_tlist = (uint32_t*)op->dalloc(sizeof(uint32_t) * op->tile_list_len);
   if (op->tile_list_len)
     PF_HIP(hipMemcpy(op->d_tlist, tlist.data(), sizeof(uint32_t) * op->tile_list_len, hipMemcpyHostToDevice));
@@ -576,14 +625,14 @@ void forward_dev(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s) {
   const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
   cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
   fz_forward<R>(op, x, B, scl, s);
-  degrid<R>(geom<R>(op), visdata<R>(op), B, y, R(1), s);
+  degrid<R>(geom<R>(op), visdata<R>(op), op->tl, B, y, R(1), s);
 }
 
 template <typename R>
 void adjoint_dev(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
   const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
   cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
-  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, y, A, R(1), s);
+  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, y, A, R(1), s, op->grid_variant);
   fz_adjoint<R>(op, A, x, scl, s);
 }
 
@@ -748,6 +797,77 @@ int pf_op_adjoint(pf_op* op, const void* y, void* x, int y_loc, int x_loc, pf_st
   });
 }
 
+int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
+  if (!op || !key) return fail(PF_ERR_ARG, "null argument");
+  const std::string k(key);
+  if (k == "grid_kernel") {
+    if (value != GRID_RESIDUE && value != GRID_ATOMIC) return fail(PF_ERR_ARG, "grid_kernel must be 0 or 1");
+    op->grid_variant = (int)value;
+    return PF_OK;
+  }
+  return fail(PF_ERR_ARG, "unknown tuning key " + k);
+}
+
+int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
+  if (!op || !key || !value) return fail(PF_ERR_ARG, "null argument");
+  const std::string k(key);
+  if (k == "grid_kernel") {
+    *value = op->grid_variant;
+    return PF_OK;
+  }
+  if (k == "tile_u") {
+    *value = op->tl.TU;
+    return PF_OK;
+  }
+  if (k == "tile_v") {
+    *value = op->tl.TV;
+    return PF_OK;
+  }
+  return fail(PF_ERR_ARG, "unknown tuning key " + k);
+}
+
+int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
+  if (!op) return fail(PF_ERR_ARG, "null operator");
+  return guarded([&] {
+    ensure_stage(op);
+    hipStream_t s = nullptr;
+    PF_HIP(hipMemsetAsync(op->d_vecy, 0, op->esz() * (size_t)std::max<int64_t>(op->M, 1), s));
+    hipEvent_t a, b;
+    PF_HIP(hipEventCreate(&a));
+    PF_HIP(hipEventCreate(&b));
+    const int n = std::max(1, (int)reps);
+    float best = 1e30f;
+    int best_v = op->grid_variant;
+    for (int variant : {GRID_RESIDUE, GRID_ATOMIC}) {
+      if (variant == GRID_RESIDUE && !grid_rc_ok(op->Nu, op->Nv, op->tl.TU, op->tl.TV, op->Ju, op->Jv)) continue;
+      op->grid_variant = variant;
+      auto run = [&] {
+        if (op->prec == PF_C64)
+          grid_tiles<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)op->d_vecy,
+                            (cpx<float>*)op->d_gridA, 1.0f, s, variant);
+        else
+          grid_tiles<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)op->d_vecy,
+                             (cpx<double>*)op->d_gridA, 1.0, s, variant);
+      };
+      run();  // warm-up
+      PF_HIP(hipEventRecord(a, s));
+      for (int k = 0; k < n; ++k) run();
+      PF_HIP(hipEventRecord(b, s));
+      PF_HIP(hipEventSynchronize(b));
+      float ms = 0;
+      PF_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best) {
+        best = ms;
+        best_v = variant;
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    op->grid_variant = best_v;
+    if (chosen) *chosen = best_v;
+  });
+}
+
 int pf_op_profile_enable(pf_op* op, int on) {
   if (!op) return fail(PF_ERR_ARG, "null operator");
   return guarded([&] {
@@ -846,9 +966,9 @@ int pf_op_apply_G(pf_op* op, const void* grid_dev, void* y_dev, pf_stream stream
   return guarded([&] {
     ProfInstall pi(op);
     if (op->prec == PF_C64)
-      degrid<float>(op->gf, visdata<float>(op), (const cpx<float>*)grid_dev, (cpx<float>*)y_dev, 1.0f, (hipStream_t)stream);
+      degrid<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)grid_dev, (cpx<float>*)y_dev, 1.0f, (hipStream_t)stream);
     else
-      degrid<double>(op->gd, visdata<double>(op), (const cpx<double>*)grid_dev, (cpx<double>*)y_dev, 1.0,
+      degrid<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)grid_dev, (cpx<double>*)y_dev, 1.0,
                      (hipStream_t)stream);
   });
 }
@@ -858,10 +978,10 @@ int pf_op_apply_GH(pf_op* op, const void* y_dev, void* grid_dev, pf_stream strea
     ProfInstall pi(op);
     if (op->prec == PF_C64)
       grid_tiles<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)y_dev, (cpx<float>*)grid_dev, 1.0f,
-                        (hipStream_t)stream);
+                        (hipStream_t)stream, op->grid_variant);
     else
       grid_tiles<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)y_dev, (cpx<double>*)grid_dev, 1.0,
-                         (hipStream_t)stream);
+                         (hipStream_t)stream, op->grid_variant);
   });
 }
 

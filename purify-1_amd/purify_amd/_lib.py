@@ -65,6 +65,9 @@ SIGNATURES = {
                            C.c_void_p]),
     "pf_kernel_eval": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_void_p,
                                  C.c_void_p, C.c_int64]),
+    "pf_op_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "pf_op_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
+    "pf_op_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]),
     "pf_op_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "pf_op_profile_get": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64)]),

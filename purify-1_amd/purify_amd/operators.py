@@ -103,6 +103,21 @@ class MeasurementOperator:
             _lib.check(_lib.lib().pf_op_power_method(self._h, int(iters), float(tol), s.ctypes.data, C.byref(out)))
         return out.value
 
+    # -- tuning
+    def set_tuning(self, key, value):
+        _lib.check(_lib.lib().pf_op_set_tuning(self._h, key.encode(), int(value)))
+
+    def get_tuning(self, key):
+        v = C.c_int64()
+        _lib.check(_lib.lib().pf_op_get_tuning(self._h, key.encode(), C.byref(v)))
+        return v.value
+
+    def autotune(self, reps=3):
+        """Time the gridding-kernel variants on this operator and keep the fastest; returns it."""
+        v = C.c_int64()
+        _lib.check(_lib.lib().pf_op_autotune(self._h, int(reps), C.byref(v)))
+        return v.value
+
     # -- per-kernel timing
     def profile(self, on=True):
         """Enable (and clear) or disable HIP-event timing of every kernel this operator launches."""

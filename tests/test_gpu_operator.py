@@ -111,6 +111,24 @@ def test_gridding_is_deterministic(precision):
     assert torch.equal(c, d)
 
 
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_gridding_variants_agree_and_are_deterministic(precision, variant):
+    n, M = 256, 60_000
+    u, v = make_uv(13, M, n, sigma=0.5)
+    w = utilities.synth_cnormal(14, M)
+    gpu, orc = pair(u, v, n, n, weights=w, Ju=6, Jv=6, precision=precision)
+    gpu.op.set_tuning("grid_kernel", variant)
+    assert gpu.op.get_tuning("grid_kernel") == variant
+    y = utilities.synth_cnormal(15, M)
+    a = gpu.adjoint() * y
+    assert rel(a, orc.adjoint(y)) < TOL[precision]
+    b = gpu.adjoint() * y
+    assert np.array_equal(a, b)
+    chosen = gpu.op.autotune(2)
+    assert chosen in (0, 1) and gpu.op.get_tuning("grid_kernel") == chosen
+
+
 @pytest.mark.parametrize("kern,J", [("gauss", 6), ("pswf", 6), ("box", 3), ("kbmin", 5), ("gauss_alt", 4),
                                     ("kb", 2), ("kb", 3), ("kb", 5), ("kb", 7), ("kb", 8), ("kb", 12)])
 def test_kernels_and_supports_vs_oracle(kern, J):
