@@ -127,8 +127,10 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
 
 /* ---- tuning (no counterpart in the reference: FFTW_MEASURE planning, operators.h:215-223, is its
  *      closest analogue) ---- */
-/* Keys: "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics; both deterministic); read-only
- * "tile_u", "tile_v".  Results do not depend on the setting beyond floating-point summation order. */
+/* Keys: "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics; both deterministic);
+ * "kernel_table" (complex64, J <= 8: 1 = interpolated tap table, about 1e-7 relative error, the default;
+ * 0 = evaluate the kernel for every tap); read-only "tile_u", "tile_v".  Results do not depend on
+ * grid_kernel beyond floating-point summation order. */
 int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
 int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
 /* Times every gridding-kernel variant on this operator (reps adjoint applies each, HIP events on the

@@ -5,7 +5,8 @@
 // length Ns reads x[j + r N/R], twiddles by W^(r (j mod Ns) N / (Ns R)), does an R-point DFT in
 // registers and writes x[(j - j mod Ns) R + j mod Ns + r Ns].  Every thread holds 16 elements; the
 // first stage loads straight from HBM and the last stores straight to HBM, the stages in between
-// exchange through LDS.  Radices are 16 (plus one 2/4/8 stage for odd log2 N).
+// exchange through LDS.  The stage plan is fixed at compile time by log2 N (radix 16, plus one
+// 2/4/8 stage first for odd log2 N), so every stage unrolls completely.
 //
 // Row pass: one or more grid rows per block (contiguous lines).  The forward operator's row pass
 // reads an image row, multiplies the separable correction S, zero-pads (Z) in registers and writes
@@ -38,21 +39,37 @@ LinePlan make_line_plan(int N) {
   return p;
 }
 
-// ---------------------------------------------------------------------------------------------
-// Line transform core.
-//   v: the thread's E elements; t: thread index within the line; T = N/E threads per line.
-//   lds: this line's LDS base, es: LDS element stride.  tw: exp(-2 pi i m / Lmax), twm = Lmax/N.
-//   load(idx) / store(idx, value) access the line's element idx in HBM.
-// ---------------------------------------------------------------------------------------------
-template <typename R, int E, int RAD, bool INV>
-__device__ inline void stage_bfly(cpx<R> (&v)[E], int t, int T, int N, int Ns, const cpx<R>* __restrict__ tw,
-                                  int twm) {
+// Compile-time stage plan of a 2^L-point line (the same plan make_line_plan describes).
+template <int L>
+struct CPlan {
+  static constexpr int N = 1 << L;
+  static constexpr int E = N < FFT_E ? N : FFT_E;  // elements per thread
+  static constexpr int T = N / E;                   // threads per line
+  static constexpr int REM = N <= FFT_E ? 0 : L % 4;
+  static constexpr int NST = N <= FFT_E ? 1 : L / 4 + (REM ? 1 : 0);
+  static constexpr int rad(int s) { return N <= FFT_E ? N : ((REM && s == 0) ? (1 << REM) : 16); }
+  static constexpr int ns(int s) {
+    int r = 1;
+    for (int k = 0; k < s; ++k) r *= rad(k);
+    return r;
+  }
+};
+
+// LDS position of line element i.  es > 0: element stride es (column tiles, lines interleaved, which
+// is conflict-free because consecutive lanes take consecutive lines).  es == 0: a contiguous line
+// padded by one element every 16, so the strided Stockham writes of the early stages spread over the
+// banks instead of hitting the same ones.
+__device__ inline int lidx(int i, int es) { return es ? i * es : i + (i >> 4); }
+
+template <typename R, int E, int RAD, int NS, int N, bool INV>
+__device__ inline void stage_bfly(cpx<R> (&v)[E], int t, const cpx<R>* __restrict__ tw, int twm) {
+  constexpr int T = N / E;
 #pragma unroll
   for (int b = 0; b < E / RAD; ++b) {
-    const int j = t + b * T;
-    const int k = j & (Ns - 1);
-    if (Ns > 1) {
-      const int step = k * (N / (Ns * RAD)) * twm;
+    if constexpr (NS > 1) {
+      const int j = t + b * T;
+      const int k = j & (NS - 1);
+      const int step = k * (N / (NS * RAD)) * twm;
 #pragma unroll
       for (int r = 1; r < RAD; ++r) {
         const cpx<R> w = tw[step * r];
@@ -63,107 +80,49 @@ __device__ inline void stage_bfly(cpx<R> (&v)[E], int t, int T, int N, int Ns, c
   }
 }
 
-// LDS position of line element i.  es > 0: element stride es (column tiles, lines interleaved, which
-// is conflict-free because consecutive lanes take consecutive lines).  es == 0: a contiguous line
-// padded by one element every 16, so the strided Stockham writes of the early stages spread over the
-// banks instead of hitting the same ones.
-__device__ inline int lidx(int i, int es) { return es ? i * es : i + (i >> 4); }
-
-template <typename R, int E, int RAD>
-__device__ inline void stage_read_lds(cpx<R> (&v)[E], int t, int T, int N, const cpx<R>* lds, int es) {
+// One stage of a 2^L-point line: (load from HBM | read LDS) -> butterflies -> (store | write LDS),
+// then the next stage.
+template <typename R, int L, int S, bool INV, class Load, class Store>
+__device__ inline void stages(cpx<R> (&v)[CPlan<L>::E], int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm,
+                              Load& load, Store& store) {
+  using P = CPlan<L>;
+  constexpr int N = P::N, E = P::E, T = P::T, RAD = P::rad(S), NS = P::ns(S);
+  if constexpr (S == 0) {
 #pragma unroll
-  for (int b = 0; b < E / RAD; ++b) {
-    const int j = t + b * T;
+    for (int b = 0; b < E / RAD; ++b)
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[lidx(j + r * (N / RAD), es)];
-  }
-}
-
-template <typename R, int E, int RAD>
-__device__ inline void stage_write_lds(const cpx<R> (&v)[E], int t, int T, int Ns, cpx<R>* lds, int es) {
-#pragma unroll
-  for (int b = 0; b < E / RAD; ++b) {
-    const int j = t + b * T;
-    const int k = j & (Ns - 1);
-    const int o = (j - k) * RAD + k;
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) lds[lidx(o + r * Ns, es)] = v[b * RAD + r];
-  }
-}
-
-template <typename R, int E, int RAD, class Load>
-__device__ inline void stage_load(cpx<R> (&v)[E], int t, int T, int N, Load& load) {
-#pragma unroll
-  for (int b = 0; b < E / RAD; ++b) {
-    const int j = t + b * T;
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = load(j + r * (N / RAD));
-  }
-}
-
-template <typename R, int E, int RAD, class Store>
-__device__ inline void stage_store(const cpx<R> (&v)[E], int t, int T, int Ns, Store& store) {
-#pragma unroll
-  for (int b = 0; b < E / RAD; ++b) {
-    const int j = t + b * T;
-    const int k = j & (Ns - 1);
-    const int o = (j - k) * RAD + k;
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) store(o + r * Ns, v[b * RAD + r]);
-  }
-}
-
-// One stage: (load from HBM | read LDS) -> butterflies -> (store to HBM | write LDS).
-template <typename R, int E, int RAD, bool INV, class Load, class Store>
-__device__ inline void run_stage(cpx<R> (&v)[E], int s, int nst, int t, int T, int N, int Ns, cpx<R>* lds, int es,
-                                 const cpx<R>* tw, int twm, Load& load, Store& store) {
-  if (s == 0) {
-    stage_load<R, E, RAD>(v, t, T, N, load);
+      for (int r = 0; r < RAD; ++r) v[b * RAD + r] = load(t + b * T + r * (N / RAD));
   } else {
-    __syncthreads();  // previous stage's LDS writes are visible
-    stage_read_lds<R, E, RAD>(v, t, T, N, lds, es);
+    __syncthreads();  // the previous stage's LDS writes are visible
+#pragma unroll
+    for (int b = 0; b < E / RAD; ++b)
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[lidx(t + b * T + r * (N / RAD), es)];
   }
-  stage_bfly<R, E, RAD, INV>(v, t, T, N, Ns, tw, twm);
-  if (s == nst - 1) {
-    stage_store<R, E, RAD>(v, t, T, Ns, store);
+  stage_bfly<R, E, RAD, NS, N, INV>(v, t, tw, twm);
+  if constexpr (S == P::NST - 1) {
+#pragma unroll
+    for (int b = 0; b < E / RAD; ++b) {
+      const int j = t + b * T, k = j & (NS - 1), o = (j - k) * RAD + k;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) store(o + r * NS, v[b * RAD + r]);
+    }
   } else {
     __syncthreads();  // every thread has read this stage's inputs before anyone overwrites them
-    stage_write_lds<R, E, RAD>(v, t, T, Ns, lds, es);
+#pragma unroll
+    for (int b = 0; b < E / RAD; ++b) {
+      const int j = t + b * T, k = j & (NS - 1), o = (j - k) * RAD + k;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) lds[lidx(o + r * NS, es)] = v[b * RAD + r];
+    }
+    stages<R, L, S + 1, INV>(v, t, lds, es, tw, twm, load, store);
   }
 }
 
-template <typename R, int E, bool INV, class Load, class Store>
-__device__ inline void fft_line(const LinePlan& pl, int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm, Load& load,
-                                Store& store) {
-  const int N = pl.N, T = N / E;
-  cpx<R> v[E];
-  int Ns = 1;
-  for (int s = 0; s < pl.nst; ++s) {
-    const int rad = pl.rad[s];
-    if constexpr (E >= 16) {
-      if (rad == 16) {
-        run_stage<R, E, 16, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
-        Ns *= 16;
-        continue;
-      }
-    }
-    if constexpr (E >= 8) {
-      if (rad == 8) {
-        run_stage<R, E, 8, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
-        Ns *= 8;
-        continue;
-      }
-    }
-    if constexpr (E >= 4) {
-      if (rad == 4) {
-        run_stage<R, E, 4, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
-        Ns *= 4;
-        continue;
-      }
-    }
-    run_stage<R, E, 2, INV>(v, s, pl.nst, t, T, N, Ns, lds, es, tw, twm, load, store);
-    Ns *= 2;
-  }
+template <typename R, int L, bool INV, class Load, class Store>
+__device__ inline void fft_line(int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm, Load& load, Store& store) {
+  cpx<R> v[CPlan<L>::E];
+  stages<R, L, 0, INV>(v, t, lds, es, tw, twm, load, store);
 }
 
 // Dynamic LDS above 64 KiB (gfx950 has 160 KiB per CU) must be opted into per kernel.
@@ -171,12 +130,32 @@ static void allow_lds(const void* fn, size_t shm) {
   if (shm > 65536) PF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
 }
 
+// Calls F.template operator()<L>() for the runtime log2 length L in [1, 13].
+template <class F>
+static void dispatch_log2(int L, F&& f) {
+  switch (L) {
+    case 1: f.template operator()<1>(); break;
+    case 2: f.template operator()<2>(); break;
+    case 3: f.template operator()<3>(); break;
+    case 4: f.template operator()<4>(); break;
+    case 5: f.template operator()<5>(); break;
+    case 6: f.template operator()<6>(); break;
+    case 7: f.template operator()<7>(); break;
+    case 8: f.template operator()<8>(); break;
+    case 9: f.template operator()<9>(); break;
+    case 10: f.template operator()<10>(); break;
+    case 11: f.template operator()<11>(); break;
+    case 12: f.template operator()<12>(); break;
+    case 13: f.template operator()<13>(); break;
+    default: throw Error(-3, "FFT length out of range");
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Row pass
 // ---------------------------------------------------------------------------------------------
 template <typename R>
 struct RowArgs {
-  LinePlan pl;
   const cpx<R>* in;
   cpx<R>* out;
   const cpx<R>* tw;
@@ -184,11 +163,16 @@ struct RowArgs {
   SZ<R> sz;
 };
 
-template <typename R, int E, bool INV, int MODE>
-__global__ void __launch_bounds__(512) k_fft_rows(RowArgs<R> a) {
+template <int L>
+constexpr int row_threads() {
+  return CPlan<L>::T > 256 ? CPlan<L>::T : 256;
+}
+
+template <typename R, int L, bool INV, int MODE>
+__global__ void __launch_bounds__(row_threads<L>()) k_fft_rows(RowArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
-  const int N = a.pl.N, T = N / E;
+  constexpr int N = CPlan<L>::N, T = CPlan<L>::T;
   const int l = threadIdx.x / T, t = threadIdx.x - l * T;
   const int line = blockIdx.x * a.lpb + l;
   const bool valid = line < a.nlines;
@@ -220,33 +204,34 @@ __global__ void __launch_bounds__(512) k_fft_rows(RowArgs<R> a) {
       a.out[(int64_t)line * N + idx] = val;
     }
   };
-  fft_line<R, E, INV>(a.pl, t, lds, 0, a.tw, a.twm, load, store);
+  fft_line<R, L, INV>(t, lds, 0, a.tw, a.twm, load, store);
 }
 
-template <typename R, int E, bool INV>
-static void launch_rows_e(const Fft2dPlan& p, int mode, const RowArgs<R>& a, int threads, size_t shm, int blocks,
-                          hipStream_t s) {
-  ProfScope prof(mode == ROW_FWD_SZ ? "fft_rows_fwd_sz" : mode == ROW_ADJ_CROP ? "fft_rows_adj_crop" : "fft_rows", s);
-  switch (mode) {
-    case ROW_FWD_SZ:
-      allow_lds(reinterpret_cast<const void*>(&k_fft_rows<R, E, INV, ROW_FWD_SZ>), shm);
-      hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_FWD_SZ>), dim3(blocks), dim3(threads), shm, s, a);
-      break;
-    case ROW_ADJ_CROP:
-      allow_lds(reinterpret_cast<const void*>(&k_fft_rows<R, E, INV, ROW_ADJ_CROP>), shm);
-      hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_ADJ_CROP>), dim3(blocks), dim3(threads), shm, s, a);
-      break;
-    default:
-      allow_lds(reinterpret_cast<const void*>(&k_fft_rows<R, E, INV, ROW_PLAIN>), shm);
-      hipLaunchKernelGGL((k_fft_rows<R, E, INV, ROW_PLAIN>), dim3(blocks), dim3(threads), shm, s, a);
+template <typename R>
+struct RowLaunch {
+  const RowArgs<R>& a;
+  int mode;
+  bool inv;
+  int threads, blocks;
+  size_t shm;
+  hipStream_t s;
+  void go(void (*k)(RowArgs<R>)) {
+    allow_lds((const void*)k, shm);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), shm, s, a);
   }
-}
+  template <int L>
+  void operator()() {
+    if (mode == ROW_FWD_SZ) go(k_fft_rows<R, L, false, ROW_FWD_SZ>);
+    else if (mode == ROW_ADJ_CROP) go(k_fft_rows<R, L, true, ROW_ADJ_CROP>);
+    else if (inv) go(k_fft_rows<R, L, true, ROW_PLAIN>);
+    else go(k_fft_rows<R, L, false, ROW_PLAIN>);
+  }
+};
 
 template <typename R>
 void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* out, int nlines, const SZ<R>& sz,
               hipStream_t s) {
   RowArgs<R> a;
-  a.pl = p.row;
   a.in = in;
   a.out = out;
   a.tw = static_cast<const cpx<R>*>(p.tw);
@@ -265,16 +250,11 @@ void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* 
   const size_t shm = (size_t)lpb * (N + N / 16) * sizeof(cpx<R>);
   const int blocks = (nlines + lpb - 1) / lpb;
   if (blocks == 0) return;
-#define PF_ROWS(EE)                                                      \
-  if (inv) launch_rows_e<R, EE, true>(p, mode, a, threads, shm, blocks, s); \
-  else launch_rows_e<R, EE, false>(p, mode, a, threads, shm, blocks, s);
-  switch (E) {
-    case 2: PF_ROWS(2) break;
-    case 4: PF_ROWS(4) break;
-    case 8: PF_ROWS(8) break;
-    default: PF_ROWS(16)
-  }
-#undef PF_ROWS
+  ProfScope prof(mode == ROW_FWD_SZ ? "fft_rows_fwd_sz" : mode == ROW_ADJ_CROP ? "fft_rows_adj_crop" : "fft_rows", s);
+  if ((mode == ROW_FWD_SZ && inv) || (mode == ROW_ADJ_CROP && !inv))
+    throw Error(-1, "fused row pass direction mismatch");
+  RowLaunch<R> launch{a, mode, inv, threads, blocks, shm, s};
+  dispatch_log2(p.row.log2N, launch);
   PF_HIP(hipGetLastError());
 }
 
@@ -285,7 +265,6 @@ enum ColKind { COL_SINGLE = 0, COL_STEP_A = 1, COL_STEP_B = 2 };
 
 template <typename R>
 struct ColArgs {
-  LinePlan pl;
   const cpx<R>* in;
   cpx<R>* out;
   const cpx<R>* tw;
@@ -296,7 +275,7 @@ struct ColArgs {
   int rin, rout; // restrict input / output rows to the window
 };
 
-template <typename R, int E, bool INV, int KIND>
+template <typename R, int L, bool INV, int KIND>
 __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -328,40 +307,46 @@ __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a) {
     if (a.rout && !in_window(row)) return;
     a.out[row * Nu + col] = val;
   };
-  fft_line<R, E, INV>(a.pl, t, smem + c, C, a.tw, a.twm, load, store);
-}
-
-template <typename R, int E, bool INV>
-static void launch_cols_e(int kind, const ColArgs<R>& a, dim3 grid, int threads, size_t shm, hipStream_t s) {
-  ProfScope prof(kind == COL_STEP_A ? "fft_cols_stepA" : kind == COL_STEP_B ? "fft_cols_stepB" : "fft_cols", s);
-  switch (kind) {
-    case COL_STEP_A:
-      hipLaunchKernelGGL((k_fft_cols<R, E, INV, COL_STEP_A>), grid, dim3(threads), shm, s, a);
-      break;
-    case COL_STEP_B:
-      hipLaunchKernelGGL((k_fft_cols<R, E, INV, COL_STEP_B>), grid, dim3(threads), shm, s, a);
-      break;
-    default:
-      hipLaunchKernelGGL((k_fft_cols<R, E, INV, COL_SINGLE>), grid, dim3(threads), shm, s, a);
-  }
+  fft_line<R, L, INV>(t, smem + c, C, a.tw, a.twm, load, store);
 }
 
 template <typename R>
-static void launch_cols(int kind, bool inv, const ColArgs<R>& a, dim3 grid, hipStream_t s) {
-  const int N = a.pl.N;
+struct ColLaunch {
+  const ColArgs<R>& a;
+  int kind;
+  bool inv;
+  dim3 grid;
+  int threads;
+  size_t shm;
+  hipStream_t s;
+  template <int L, bool I>
+  void go() {
+    void (*k)(ColArgs<R>) = kind == COL_STEP_A   ? k_fft_cols<R, L, I, COL_STEP_A>
+                            : kind == COL_STEP_B ? k_fft_cols<R, L, I, COL_STEP_B>
+                                                 : k_fft_cols<R, L, I, COL_SINGLE>;
+    allow_lds((const void*)k, shm);
+    hipLaunchKernelGGL(k, grid, dim3(threads), shm, s, a);
+  }
+  template <int L>
+  void operator()() {
+    if constexpr (L <= 9) {
+      if (inv) go<L, true>();
+      else go<L, false>();
+    } else {
+      throw Error(-3, "column line length above 512");
+    }
+  }
+};
+
+template <typename R>
+static void launch_cols(int kind, bool inv, int log2n, const ColArgs<R>& a, dim3 grid, hipStream_t s) {
+  const int N = 1 << log2n;
   const int E = N < FFT_E ? N : FFT_E;
   const int threads = a.C * (N / E);
   const size_t shm = (size_t)N * a.C * sizeof(cpx<R>);
-#define PF_COLS(EE)                                                    \
-  if (inv) launch_cols_e<R, EE, true>(kind, a, grid, threads, shm, s); \
-  else launch_cols_e<R, EE, false>(kind, a, grid, threads, shm, s);
-  switch (E) {
-    case 2: PF_COLS(2) break;
-    case 4: PF_COLS(4) break;
-    case 8: PF_COLS(8) break;
-    default: PF_COLS(16)
-  }
-#undef PF_COLS
+  ProfScope prof(kind == COL_STEP_A ? "fft_cols_stepA" : kind == COL_STEP_B ? "fft_cols_stepB" : "fft_cols", s);
+  ColLaunch<R> launch{a, kind, inv, grid, threads, shm, s};
+  dispatch_log2(log2n, launch);
   PF_HIP(hipGetLastError());
 }
 
@@ -380,30 +365,27 @@ void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow w
   a.N1 = p.N1;
   a.N2 = p.N2;
   if (!p.col_two_step) {
-    a.pl = p.col;
     a.twm = p.Lmax / p.Nv;
     a.C = p.col_C;
     a.rin = restrict_in;
     a.rout = restrict_out;
-    launch_cols<R>(COL_SINGLE, inv, a, dim3(p.Nu / a.C), s);
+    launch_cols<R>(COL_SINGLE, inv, p.col.log2N, a, dim3(p.Nu / a.C), s);
     return;
   }
   a.C = p.two_C;
   // step A in place on `in`: N1-point transforms over rows of stride N2
-  a.pl = p.colA;
   a.twm = p.Lmax / p.N1;
   a.out = in;
   a.rin = restrict_in;
   a.rout = 0;
-  launch_cols<R>(COL_STEP_A, inv, a, dim3(p.Nu / a.C, p.N2), s);
+  launch_cols<R>(COL_STEP_A, inv, p.colA.log2N, a, dim3(p.Nu / a.C, p.N2), s);
   // step B `in` -> `out`: N2-point transforms over contiguous row blocks
-  a.pl = p.colB;
   a.twm = p.Lmax / p.N2;
   a.in = in;
   a.out = out;
   a.rin = 0;
   a.rout = restrict_out;
-  launch_cols<R>(COL_STEP_B, inv, a, dim3(p.Nu / a.C, p.N1), s);
+  launch_cols<R>(COL_STEP_B, inv, p.colB.log2N, a, dim3(p.Nu / a.C, p.N1), s);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -416,7 +398,6 @@ static void twiddle_table(int L, std::vector<double>& re, std::vector<double>& i
   im.resize(L);
   const long double pi = 3.141592653589793238462643383279502884L;
   for (int m = 0; m < L; ++m) {
-    // angle = 2 pi m / L = (pi/4) * (8m / L): octant o and residual
     const int64_t num = 8LL * m;  // angle in units of pi/4 is num / L
     const int o = (int)(num / L);
     const int64_t rmd = num - (int64_t)o * L;  // in [0, L)
@@ -434,7 +415,6 @@ static void twiddle_table(int L, std::vector<double>& re, std::vector<double>& i
       c = sn;
       sn = tmp;
     }
-    // rotate the first-quadrant value (c, sn) of angle (o/2)*pi/2 + ... into place
     long double cr, sr;
     switch ((o / 2) % 4) {
       case 0: cr = c; sr = sn; break;

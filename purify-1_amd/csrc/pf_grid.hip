@@ -38,10 +38,9 @@ __device__ inline VisCoord<double> vis_coord(const VisData<double>& v, int64_t i
   return VisCoord<double>{(int)ku, (int)kv, uv.x - ku, uv.y - kv};
 }
 
-__device__ inline int wrap(int a, int n) {
-  int r = a % n;
-  return r < 0 ? r + n : r;
-}
+// a mod n for the power-of-two grid sizes the operator accepts (two's complement makes it exact for
+// negative a as well)
+__device__ inline int wrap(int a, int n) { return a & (n - 1); }
 
 constexpr int JMAX = 16;
 
@@ -55,6 +54,38 @@ __device__ inline void taps(const KParams<R>& p, int k, R f, int Jrt, R (&w)[J =
     R val = kern1d<R>(p, f - R(1 + a));
     w[a] = ((k + a) & 1) ? -val : val;
   }
+}
+
+// Same weights from the complex64 tap table (J <= 8): one row read and J interpolations.
+template <int J>
+__device__ inline void taps_tab(const float* __restrict__ tab, int k, float f, float (&w)[J]) {
+  const float d = f - 0.5f * (float)J;  // in (0, 1]
+  const float qf = d * (float)TAB_Q;
+  int q = (int)qf;
+  q = q < 0 ? 0 : (q > TAB_Q - 1 ? TAB_Q - 1 : q);
+  const float r = qf - (float)q;
+  const float4* row = reinterpret_cast<const float4*>(tab + (size_t)q * TAB_ROW);
+  const float4 t0 = row[0], t1 = row[1], d0 = row[2], d1 = row[3];
+  const float t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+  const float dd[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+  for (int a = 0; a < J; ++a) {
+    const float val = t[a] + r * dd[a];
+    w[a] = ((k + a) & 1) ? -val : val;
+  }
+}
+
+// Tap weights of one axis: the table when there is one (complex64, J <= 8), else the kernel itself.
+template <typename R, int J>
+__device__ inline void taps_axis(const KParams<R>& p, const float* tab, int k, R f, int Jrt,
+                                 R (&w)[J == 0 ? JMAX : J]) {
+  if constexpr (sizeof(R) == 4 && J > 0 && J <= 8) {
+    if (tab) {
+      taps_tab<J>(tab, k, f, w);
+      return;
+    }
+  }
+  taps<R, J>(p, k, f, Jrt, w);
 }
 
 // Tile kernels keep the hardware's round-robin spread of workgroups over the 8 XCDs: the visibility
@@ -89,8 +120,8 @@ __global__ void __launch_bounds__(256) k_degrid_tiles(GridGeom<R> g, VisData<R> 
   for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
     const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
     R wu[JJU], wv[JJV];
-    taps<R, JU>(g.ku, c.ku, c.fu, Ju, wu);
-    taps<R, JV>(g.kv, c.kv, c.fv, Jv, wv);
+    taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
+    taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, Jv, wv);
     const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
     const cpx<R>* base = lds + ly0 * W + lx0;
     cpx<R> acc = mk<R>(R(0), R(0));
@@ -155,12 +186,12 @@ __global__ void __launch_bounds__(64) k_grid_rc(GridGeom<R> g, VisData<R> v, Til
       // visibility `lane` of the batch: value, tap weights and their residue-rotated placement
       const uint32_t i = tl.list[base + lane];
       const VisCoord<R> c = vis_coord(v, i, JU, JV);
-      cpx<R> s = yin[v.perm[i]];
+      cpx<R> s = yin[i];  // visibilities in sorted order
       if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
       sv[lane] = scale(s, scl);
       R wu[JU], wv[JV];
-      taps<R, JU>(g.ku, c.ku, c.fu, JU, wu);
-      taps<R, JV>(g.kv, c.kv, c.fv, JV, wv);
+      taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, JU, wu);
+      taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, JV, wv);
       int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
       if (dx >= g.Nu - 8) dx -= g.Nu;
       if (dy >= g.Nv - 8) dy -= g.Nv;
@@ -253,12 +284,12 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
     if (idx < end) {
       const uint32_t i = tl.list[idx];
       const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
-      cpx<R> s = yin[v.perm[i]];
+      cpx<R> s = yin[i];  // visibilities in sorted order
       if (v.weights) s = cmulc(s, v.weights[i]);
       s = scale(s, scl);
       R wu[JJU], wv[JJV];
-      taps<R, JU>(g.ku, c.ku, c.fu, Ju, wu);
-      taps<R, JV>(g.kv, c.kv, c.fv, Jv, wv);
+      taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
+      taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, Jv, wv);
       const int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
 #pragma unroll
       for (int b = 0; b < JJV; ++b) {
@@ -372,6 +403,25 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
 #undef PF_GT
   PF_HIP(hipGetLastError());
 }
+
+// Caller's order -> sorted order.  A random 8-byte write costs a 32-byte memory transaction while a
+// random 8-byte read costs a 128-byte cache line, so the permutation reads y in order and scatters.
+template <typename R>
+__global__ void __launch_bounds__(256) k_permute_to_sorted(const cpx<R>* __restrict__ y, const int32_t* __restrict__ inv,
+                                                           cpx<R>* __restrict__ ys, int64_t M) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < M) ys[inv[m]] = y[m];
+}
+
+template <typename R>
+void permute_to_sorted(const VisData<R>& v, const cpx<R>* y, cpx<R>* ys, hipStream_t s) {
+  if (v.M == 0) return;
+  ProfScope prof("permute_to_sorted", s);
+  hipLaunchKernelGGL((k_permute_to_sorted<R>), dim3((unsigned)((v.M + 255) / 256)), dim3(256), 0, s, y, v.inv, ys, v.M);
+  PF_HIP(hipGetLastError());
+}
+template void permute_to_sorted<float>(const VisData<float>&, const cpx<float>*, cpx<float>*, hipStream_t);
+template void permute_to_sorted<double>(const VisData<double>&, const cpx<double>*, cpx<double>*, hipStream_t);
 
 template void degrid<float>(const GridGeom<float>&, const VisData<float>&, const TileLists&, const cpx<float>*,
                             cpx<float>*, float, hipStream_t);

@@ -111,14 +111,26 @@ struct VisData {
   const cpx<double>* uv = nullptr;
   const cpx<R>* weights = nullptr;  // nullptr: unit weights
   const int32_t* perm = nullptr;    // sorted position -> caller's index
+  const int32_t* inv = nullptr;     // caller's index -> sorted position
   int64_t M = 0;
 };
+
+// complex64 tap tables: all J taps of a visibility sit at offsets delta + J/2 - 1 - a of one fractional
+// delta = (u - k_u) - J/2 in (0, 1], so row q of the table holds the J kernel values at delta = q / TAB_Q
+// (T[0..7]) and their forward differences to row q + 1 (D[0..7]); a visibility reads one 64-byte row
+// and interpolates linearly.  Interpolation error about 1e-7 relative; used for J <= 8.
+constexpr int TAB_Q = 4096;
+constexpr int TAB_ROW = 16;
 
 template <typename R>
 struct GridGeom {
   int Nu = 0, Nv = 0, Ju = 0, Jv = 0;
-  KParams<R> ku, kv;  // u-axis and v-axis kernels
+  KParams<R> ku, kv;               // u-axis and v-axis kernels
+  const float* tab_u = nullptr;    // complex64 path only: tap tables (nullptr: evaluate the kernels)
+  const float* tab_v = nullptr;
 };
+// Host: builds the TAB_Q x TAB_ROW table of a kernel evaluated with support Jeval over J taps.
+std::vector<float> make_tap_table(int kind, int Jeval, int J, double os);
 
 struct TileLists {
   int TU = 32, TV = 32, ntu = 0, ntv = 0;
@@ -132,9 +144,13 @@ template <typename R>
 void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
             hipStream_t s);
 bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv);
+// grid = G^H y with y in the operator's sorted order (see permute_to_sorted)
 template <typename R>
-void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* y, cpx<R>* grid,
+void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* y_sorted, cpx<R>* grid,
                 R scale, hipStream_t s, int variant);
+// ys[inv[m]] = y[m]: caller's order to sorted order (coalesced reads, scattered 8-byte writes)
+template <typename R>
+void permute_to_sorted(const VisData<R>& v, const cpx<R>* y, cpx<R>* ys, hipStream_t s);
 // gridding kernel variants: residue-class lanes (needs J <= 8 and N >= T + 8) or LDS atomics
 enum GridVariant { GRID_RESIDUE = 0, GRID_ATOMIC = 1 };
 

@@ -238,6 +238,20 @@ KParams<R> make_kparams(int kind, int J, double os) {
 template KParams<float> make_kparams<float>(int, int, double);
 template KParams<double> make_kparams<double>(int, int, double);
 
+std::vector<float> make_tap_table(int kind, int Jeval, int J, double os) {
+  // row q: K(q/Q + J/2 - 1 - a) for a < J in [0, 8), forward differences to row q + 1 in [8, 16)
+  std::vector<double> v((size_t)(TAB_Q + 1) * 8, 0.0);
+  for (int q = 0; q <= TAB_Q; ++q)
+    for (int a = 0; a < J && a < 8; ++a) v[(size_t)q * 8 + a] = host_kernel(kind, (double)q / TAB_Q + 0.5 * J - 1 - a, Jeval, os);
+  std::vector<float> t((size_t)TAB_Q * TAB_ROW, 0.0f);
+  for (int q = 0; q < TAB_Q; ++q)
+    for (int a = 0; a < 8; ++a) {
+      t[(size_t)q * TAB_ROW + a] = (float)v[(size_t)q * 8 + a];
+      t[(size_t)q * TAB_ROW + 8 + a] = (float)(v[(size_t)(q + 1) * 8 + a] - v[(size_t)q * 8 + a]);
+    }
+  return t;
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -262,10 +276,14 @@ struct pf_op {
   cpx<double>* d_uv = nullptr;
   void* d_weights = nullptr;
   int32_t* d_perm = nullptr;
+  int32_t* d_inv = nullptr;    // inverse permutation
+  void* d_ysort = nullptr;     // visibilities in sorted order (adjoint input after the permutation)
   uint32_t* d_tptr = nullptr;
   uint32_t* d_tlist = nullptr;
   uint32_t* d_towner = nullptr;
   uint32_t* d_torder = nullptr;
+  float* d_tab_u = nullptr;  // complex64 tap tables
+  float* d_tab_v = nullptr;
   void* d_sx = nullptr;
   void* d_sy = nullptr;
   void* d_gridA = nullptr;
@@ -404,6 +422,16 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->gf.kv = make_kparams<float>(d.kernel, d.Ju, op->os);
   op->gd.ku = make_kparams<double>(d.kernel, d.Jv, op->os);
   op->gd.kv = make_kparams<double>(d.kernel, d.Ju, op->os);
+  if (op->prec == PF_C64 && op->Ju <= 8 && op->Jv <= 8) {
+    const std::vector<float> tu = make_tap_table(d.kernel, d.Jv, op->Ju, op->os);
+    const std::vector<float> tv = make_tap_table(d.kernel, d.Ju, op->Jv, op->os);
+    op->d_tab_u = (float*)op->dalloc(sizeof(float) * tu.size());
+    op->d_tab_v = (float*)op->dalloc(sizeof(float) * tv.size());
+    PF_HIP(hipMemcpy(op->d_tab_u, tu.data(), sizeof(float) * tu.size(), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(op->d_tab_v, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice));
+    op->gf.tab_u = op->d_tab_u;
+    op->gf.tab_v = op->d_tab_v;
+  }
 
   // ---- coordinates in pixels, tap origins, sort by owner tile ----
   std::vector<double> u, v;
@@ -489,6 +517,13 @@ void build(pf_op* op, const pf_op_desc& d) {
   // ---- upload ----
   op->d_perm = (int32_t*)op->dalloc(sizeof(int32_t) * M);
   PF_HIP(hipMemcpy(op->d_perm, perm.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
+  {
+    std::vector<int32_t> inv(M);
+    for (int64_t i = 0; i < M; ++i) inv[perm[i]] = (int32_t)i;
+    op->d_inv = (int32_t*)op->dalloc(sizeof(int32_t) * M);
+    PF_HIP(hipMemcpy(op->d_inv, inv.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
+  }
+  op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
   op->d_tptr = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
   PF_HIP(hipMemcpy(op->d_tptr, tcount.data(), sizeof(uint32_t) * (op->n_tiles + 1), hipMemcpyHostToDevice));
   op->d_towner = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
@@ -575,6 +610,7 @@ VisData<R> visdata(const pf_op* op) {
   v.uv = op->d_uv;
   v.weights = static_cast<const cpx<R>*>(op->d_weights);
   v.perm = op->d_perm;
+  v.inv = op->d_inv;
   v.M = op->M;
   return v;
 }
@@ -632,7 +668,9 @@ template <typename R>
 void adjoint_dev(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
   const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
   cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
-  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, y, A, R(1), s, op->grid_variant);
+  cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
+  permute_to_sorted<R>(visdata<R>(op), y, ys, s);
+  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, ys, A, R(1), s, op->grid_variant);
   fz_adjoint<R>(op, A, x, scl, s);
 }
 
@@ -805,6 +843,13 @@ int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
     op->grid_variant = (int)value;
     return PF_OK;
   }
+  if (k == "kernel_table") {
+    if (value != 0 && value != 1) return fail(PF_ERR_ARG, "kernel_table must be 0 or 1");
+    if (value && !op->d_tab_u) return fail(PF_ERR_UNSUPPORTED, "tap tables exist only for complex64 and J <= 8");
+    op->gf.tab_u = value ? op->d_tab_u : nullptr;
+    op->gf.tab_v = value ? op->d_tab_v : nullptr;
+    return PF_OK;
+  }
   return fail(PF_ERR_ARG, "unknown tuning key " + k);
 }
 
@@ -813,6 +858,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   const std::string k(key);
   if (k == "grid_kernel") {
     *value = op->grid_variant;
+    return PF_OK;
+  }
+  if (k == "kernel_table") {
+    *value = op->gf.tab_u ? 1 : 0;
     return PF_OK;
   }
   if (k == "tile_u") {
@@ -976,12 +1025,16 @@ int pf_op_apply_G(pf_op* op, const void* grid_dev, void* y_dev, pf_stream stream
 int pf_op_apply_GH(pf_op* op, const void* y_dev, void* grid_dev, pf_stream stream) {
   return guarded([&] {
     ProfInstall pi(op);
-    if (op->prec == PF_C64)
-      grid_tiles<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)y_dev, (cpx<float>*)grid_dev, 1.0f,
-                        (hipStream_t)stream, op->grid_variant);
-    else
-      grid_tiles<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)y_dev, (cpx<double>*)grid_dev, 1.0,
-                         (hipStream_t)stream, op->grid_variant);
+    hipStream_t s = (hipStream_t)stream;
+    if (op->prec == PF_C64) {
+      permute_to_sorted<float>(visdata<float>(op), (const cpx<float>*)y_dev, (cpx<float>*)op->d_ysort, s);
+      grid_tiles<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)op->d_ysort, (cpx<float>*)grid_dev,
+                        1.0f, s, op->grid_variant);
+    } else {
+      permute_to_sorted<double>(visdata<double>(op), (const cpx<double>*)y_dev, (cpx<double>*)op->d_ysort, s);
+      grid_tiles<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)op->d_ysort,
+                         (cpx<double>*)grid_dev, 1.0, s, op->grid_variant);
+    }
   });
 }
 

@@ -129,6 +129,20 @@ def test_gridding_variants_agree_and_are_deterministic(precision, variant):
     assert chosen in (0, 1) and gpu.op.get_tuning("grid_kernel") == chosen
 
 
+@pytest.mark.parametrize("kern,J", [("kb", 6), ("kb", 4), ("gauss", 6), ("pswf", 6), ("kbmin", 8)])
+@pytest.mark.parametrize("table", [1, 0])
+def test_complex64_tap_table_and_exact_kernels(kern, J, table):
+    n, M = 512, 100_000
+    u, v = make_uv(16 + J, M, n)
+    gpu, orc = pair(u, v, n, n, kernel=kern, Ju=J, Jv=J, precision="c64")
+    assert gpu.op.get_tuning("kernel_table") == 1
+    gpu.op.set_tuning("kernel_table", table)
+    x = utilities.synth_image(17, n, n, 20)
+    assert rel(gpu * x, orc.forward(x)) < 1e-5
+    y = utilities.synth_cnormal(18, M)
+    assert rel(gpu.adjoint() * y, orc.adjoint(y)) < 1e-5
+
+
 @pytest.mark.parametrize("kern,J", [("gauss", 6), ("pswf", 6), ("box", 3), ("kbmin", 5), ("gauss_alt", 4),
                                     ("kb", 2), ("kb", 3), ("kb", 5), ("kb", 7), ("kb", 8), ("kb", 12)])
 def test_kernels_and_supports_vs_oracle(kern, J):
