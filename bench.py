@@ -59,6 +59,33 @@ def kernel_alg_bytes(name, n, Nu, Nv, M, e, r):
     return table.get(name)
 
 
+def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev):
+    """Workload for rocprofv3 --pmc passes: a copy of known size, then the operator pair."""
+    import torch
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)  # 512 MiB
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    prec = args.counters
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, os_, 0, 1e-4, "kb", J, J,
+                                                     precision=prec)
+    op = lt.op
+    op.autotune(3)
+    dt = torch.complex64 if prec == "c64" else torch.complex128
+    x = torch.from_numpy(x_h).to(dt).to(dev)
+    yin = torch.from_numpy(y_h).to(dt).to(dev)
+    y = torch.empty(len(u), dtype=dt, device=dev)
+    xo = torch.empty(n * n, dtype=dt, device=dev)
+    for _ in range(args.warmup + args.steps):
+        op.forward(x, out=y)
+        op.adjoint_apply(yin, out=xo)
+    torch.cuda.synchronize()
+    print(json.dumps({"counters_run": prec, "pairs": args.warmup + args.steps,
+                      "grid_kernel": op.get_tuning("grid_kernel")}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +95,9 @@ def main():
     ap.add_argument("--workload", default="north_star", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-pairs", type=int, default=2)
+    ap.add_argument("--counters", choices=("c64", "c128"), default=None,
+                    help="hardware-counter run: three 512 MiB plain copies (the calibration kernel for "
+                         "FETCH_SIZE / WRITE_SIZE) then warmup + steps pairs of one operator, nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one operator per GPU, no "
@@ -92,6 +122,10 @@ def main():
     x_h = utilities.synth_image(seed, n, n, 20)
     y_h = utilities.synth_cnormal(seed + 1, M)
     log(f"[bench] workload {args.workload}: n={n} grid={Nu} J={J} M={M}")
+
+    if args.counters:
+        counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev)
+        return
 
     res = {}
     outs = {}
@@ -134,8 +168,16 @@ def main():
         table = op.profile_table()
         op.profile(False)
         outs[prec] = (y.cpu().numpy().astype(np.complex128), xo.cpu().numpy().astype(np.complex128))
+        # PCIe-inclusive: the same pair on host complex128 buffers (copies and conversions included)
+        host_ms = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            op.forward(x_h)
+            op.adjoint_apply(y_h)
+            host_ms.append(1e3 * (time.perf_counter() - t0))
         res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
-                         value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s)
+                         value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s,
+                         host_ms=statistics.median(host_ms))
         log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s, "
             f"grid kernel {variant}")
         del lt, op, x, yin, y, xo
@@ -158,7 +200,8 @@ def main():
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tr = json.load(f)
-        roofline["traffic"] = tr.get(dom)
+        roofline["traffic"] = tr.get("bytes_per_launch", {}).get(dom)
+        roofline["traffic_source"] = os.path.relpath(traffic_file, ROOT)
 
     pair_bytes = model_bytes_pair(n, Ng, M, e, r)
     rel_fp32_fp64 = {
@@ -221,6 +264,8 @@ def main():
         "rel_err_vs_oracle": acc_oracle,
         "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "setup_s": {"c64": r64["setup_s"], "c128": res["c128"]["setup_s"]},
+        "host_pair_ms": {"c64": r64["host_ms"], "c128": res["c128"]["host_ms"],
+                         "note": "pair on host complex128 buffers: PCIe copies and conversions included"},
         "autotune": {"grid_kernel": {"c64": r64["grid_kernel"], "c128": res["c128"]["grid_kernel"]},
                      "seconds": {"c64": r64["tune_s"], "c128": res["c128"]["tune_s"]}},
         "kernels_ms_per_launch": {k: tab[k][0] / tab[k][1] for k in tab},
