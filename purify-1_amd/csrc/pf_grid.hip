@@ -319,6 +319,108 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Gridding on the matrix cores.  Inside a 16 x 16 tile
+//     tile[i][j] = sum_t (s_t wv_t[i - dy_t]) * wu_t[j - dx_t]
+// over the tile's list: the product A B of a 16 x K complex matrix (one column per visibility, its
+// v-taps scaled by the value, zero outside the footprint) with a K x 16 real one (its u-taps).  One
+// wave per tile.  In batches of 64 list entries, lane t writes entry t's column of A (real and
+// imaginary parts) and row of B to LDS; then each MFMA 16x16x4 step adds four entries: lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16], and holds four result rows of column l % 16:
+// 4 (l / 16) + r for the FP32 instruction, l / 16 + 4 r for the FP64 one (r < 4).  The summation order is fixed by the list and the instruction: reproducible bit for
+// bit.  Needs N >= 16 + J - 1 on both axes, so that a footprint meets a tile in one piece.
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return (lane >> 4) * 4 + r; }  // result row of register r
+};
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+constexpr int MB = 64;  // list entries per batch
+
+template <typename R, int JU, int JV>
+__global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, TileLists tl,
+                                                  const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
+  constexpr int T = MFMA_TILE;
+  __shared__ R a_re[MB * T], a_im[MB * T], b_u[MB * T];  // [entry][row] and [entry][column]
+  using acc_t = typename Mfma<R>::acc_t;
+  const int tile = tl.order ? (int)tl.order[blockIdx.x] : (int)blockIdx.x;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * T, ty0 = tv * T;
+  const int lane = threadIdx.x;
+  const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
+  constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
+  acc_t cre = {R(0), R(0), R(0), R(0)}, cim = {R(0), R(0), R(0), R(0)};
+  R* pre = a_re + lane * T;
+  R* pim = a_im + lane * T;
+  R* pb = b_u + lane * T;
+  const uint32_t end = tl.ptr[tile + 1];
+  for (uint32_t base = tl.ptr[tile]; base < end; base += MB) {
+    const int nb = (int)min((uint32_t)MB, end - base);
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+      pre[k] = R(0);
+      pim[k] = R(0);
+      pb[k] = R(0);
+    }
+    if (lane < nb) {
+      const uint32_t i = tl.list[base + lane];
+      const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
+      cpx<R> s = yin[i];  // visibilities in sorted order
+      if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
+      s = scale(s, scl);
+      R wu[JJU], wv[JJV];
+      taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
+      taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, Jv, wv);
+      int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
+      if (dx >= T) dx -= g.Nu;  // footprint starts left of / above the tile
+      if (dy >= T) dy -= g.Nv;
+#pragma unroll
+      for (int e = 0; e < JJV; ++e) {
+        if (JV == 0 && e >= Jv) break;
+        const int y = dy + e;
+        if ((unsigned)y < (unsigned)T) {
+          pre[y] = s.x * wv[e];
+          pim[y] = s.y * wv[e];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < JJU; ++e) {
+        if (JU == 0 && e >= Ju) break;
+        const int x = dx + e;
+        if ((unsigned)x < (unsigned)T) pb[x] = wu[e];
+      }
+    }
+    __syncthreads();
+    const int nk = (nb + 3) >> 2;
+    for (int k = 0; k < nk; ++k) {
+      const int idx = k * 64 + lane;  // entry 4k + lane / 16, row / column lane % 16
+      const R b = b_u[idx];
+      cre = Mfma<R>::step(a_re[idx], b, cre);
+      cim = Mfma<R>::step(a_im[idx], b, cim);
+    }
+    __syncthreads();
+  }
+  const int col = lane & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) grid[(int64_t)(ty0 + Mfma<R>::row(lane, r)) * g.Nu + tx0 + col] = mk<R>(cre[r], cim[r]);
+}
+
+bool grid_mfma_ok(int Nu, int Nv, int Ju, int Jv) {
+  return Nu % MFMA_TILE == 0 && Nv % MFMA_TILE == 0 && Nu >= MFMA_TILE + Ju - 1 && Nv >= MFMA_TILE + Jv - 1;
+}
+
 template <typename R>
 void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
             hipStream_t s) {
@@ -350,8 +452,30 @@ bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv) {
 }
 
 template <typename R>
-void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* y, cpx<R>* grid, R scale,
-                hipStream_t s, int variant) {
+void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm, const cpx<R>* y,
+                cpx<R>* grid, R scale, hipStream_t s, int variant) {
+  if (variant == GRID_MFMA && tm.ptr && grid_mfma_ok(g.Nu, g.Nv, g.Ju, g.Jv)) {
+    const dim3 blocks((unsigned)(tm.ntu * tm.ntv));
+    ProfScope prof("grid_tiles", s);
+#define PF_GM(A, B) hipLaunchKernelGGL((k_grid_mfma<R, A, B>), blocks, dim3(64), 0, s, g, v, tm, y, grid, scale)
+    if (g.Ju == g.Jv) {
+      switch (g.Ju) {
+        case 2: PF_GM(2, 2); break;
+        case 3: PF_GM(3, 3); break;
+        case 4: PF_GM(4, 4); break;
+        case 5: PF_GM(5, 5); break;
+        case 6: PF_GM(6, 6); break;
+        case 7: PF_GM(7, 7); break;
+        case 8: PF_GM(8, 8); break;
+        default: PF_GM(0, 0);
+      }
+    } else {
+      PF_GM(0, 0);
+    }
+#undef PF_GM
+    PF_HIP(hipGetLastError());
+    return;
+  }
   const dim3 blocks((unsigned)(t.ntu * t.ntv));
   if (variant == GRID_RESIDUE && grid_rc_ok(g.Nu, g.Nv, t.TU, t.TV, g.Ju, g.Jv)) {
     const size_t shm = sizeof(cpx<R>) * ((size_t)(t.TU + 1) * t.TV + 64) + 2 * sizeof(R) * 2 * 64 * 8;
@@ -427,9 +551,9 @@ template void degrid<float>(const GridGeom<float>&, const VisData<float>&, const
                             cpx<float>*, float, hipStream_t);
 template void degrid<double>(const GridGeom<double>&, const VisData<double>&, const TileLists&, const cpx<double>*,
                              cpx<double>*, double, hipStream_t);
-template void grid_tiles<float>(const GridGeom<float>&, const VisData<float>&, const TileLists&, const cpx<float>*,
-                                cpx<float>*, float, hipStream_t, int);
-template void grid_tiles<double>(const GridGeom<double>&, const VisData<double>&, const TileLists&,
+template void grid_tiles<float>(const GridGeom<float>&, const VisData<float>&, const TileLists&, const TileLists&,
+                                const cpx<float>*, cpx<float>*, float, hipStream_t, int);
+template void grid_tiles<double>(const GridGeom<double>&, const VisData<double>&, const TileLists&, const TileLists&,
                                  const cpx<double>*, cpx<double>*, double, hipStream_t, int);
 
 // ---------------------------------------------------------------------------------------------

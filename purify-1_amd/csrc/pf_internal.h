@@ -144,15 +144,20 @@ template <typename R>
 void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
             hipStream_t s);
 bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv);
-// grid = G^H y with y in the operator's sorted order (see permute_to_sorted)
+// side of the matrix-core gridding tiles; usable when N >= MFMA_TILE + J - 1 on both axes
+constexpr int MFMA_TILE = 16;
+bool grid_mfma_ok(int Nu, int Nv, int Ju, int Jv);
+// gridding kernel variants: residue-class lanes (needs J <= 8 and N >= T + 8), LDS atomics, or
+// rank-4 updates of 16 x 16 tiles on the matrix cores (lists `tm`)
+enum GridVariant { GRID_RESIDUE = 0, GRID_ATOMIC = 1, GRID_MFMA = 2 };
+// grid = G^H y with y in the operator's sorted order (see permute_to_sorted).  `t` are the lists of
+// the TU x TV tiles, `tm` those of the 16 x 16 tiles (GRID_MFMA).
 template <typename R>
-void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* y_sorted, cpx<R>* grid,
-                R scale, hipStream_t s, int variant);
+void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm,
+                const cpx<R>* y_sorted, cpx<R>* grid, R scale, hipStream_t s, int variant);
 // ys[inv[m]] = y[m]: caller's order to sorted order (coalesced reads, scattered 8-byte writes)
 template <typename R>
 void permute_to_sorted(const VisData<R>& v, const cpx<R>* y, cpx<R>* ys, hipStream_t s);
-// gridding kernel variants: residue-class lanes (needs J <= 8 and N >= T + 8) or LDS atomics
-enum GridVariant { GRID_RESIDUE = 0, GRID_ATOMIC = 1 };
 
 // ---- projection rows (w_term) -------------------------------------------------------------
 template <typename R>

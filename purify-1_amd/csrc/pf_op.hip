@@ -267,9 +267,10 @@ struct pf_op {
   int64_t M = 0;
   bool unit_weights = true;
   Fft2dPlan fft;
-  TileLists tl;
+  TileLists tl;   // TU x TV tiles: degridding, residue-class and atomic gridding
+  TileLists tlm;  // 16 x 16 tiles: matrix-core gridding (empty when the grid is too small)
   int n_tiles = 0;
-  int64_t tile_list_len = 0;
+  int64_t tile_list_len = 0, tile_list_len_m = 0;
   // device buffers
   int2* d_kk = nullptr;
   cpx<float>* d_ff = nullptr;
@@ -278,10 +279,7 @@ struct pf_op {
   int32_t* d_perm = nullptr;
   int32_t* d_inv = nullptr;    // inverse permutation
   void* d_ysort = nullptr;     // visibilities in sorted order (adjoint input after the permutation)
-  uint32_t* d_tptr = nullptr;
-  uint32_t* d_tlist = nullptr;
   uint32_t* d_towner = nullptr;
-  uint32_t* d_torder = nullptr;
   float* d_tab_u = nullptr;  // complex64 tap tables
   float* d_tab_v = nullptr;
   void* d_sx = nullptr;
@@ -342,6 +340,78 @@ bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
 int wrapi(int64_t a, int n) {
   int64_t r = a % n;
   return (int)(r < 0 ? r + n : r);
+}
+
+// Gridding lists of a TU x TV tiling: for each tile, the sorted index of every visibility with a tap
+// in it (ascending), and the tiles in launch order, heaviest first, so that the dense centre of the
+// uv-plane does not form the tail of the launch.
+struct HostLists {
+  std::vector<uint32_t> ptr, list, order;
+};
+
+HostLists tile_lists(const std::vector<int32_t>& perm, const std::vector<int64_t>& ku, const std::vector<int64_t>& kv,
+                     int Nu, int Nv, int Ju, int Jv, int TU, int TV) {
+  const int ntu = Nu / TU, ntv = Nv / TV, nt = ntu * ntv;
+  const int64_t M = (int64_t)perm.size();
+  HostLists h;
+  h.ptr.assign((size_t)nt + 1, 0);
+  // distinct tile columns / rows met by the footprint of visibility m
+  auto sets = [&](int64_t m, int* tcols, int& nc, int* trows, int& nr) {
+    const int q0 = wrapi(ku[m] + 1, Nu), p0 = wrapi(kv[m] + 1, Nv);
+    nc = nr = 0;
+    for (int a = 0; a < Ju; ++a) {
+      const int tc = ((q0 + a) % Nu) / TU;
+      bool seen = false;
+      for (int k = 0; k < nc; ++k) seen |= (tcols[k] == tc);
+      if (!seen) tcols[nc++] = tc;
+    }
+    for (int b = 0; b < Jv; ++b) {
+      const int tr = ((p0 + b) % Nv) / TV;
+      bool seen = false;
+      for (int k = 0; k < nr; ++k) seen |= (trows[k] == tr);
+      if (!seen) trows[nr++] = tr;
+    }
+  };
+  int tc[16], tr[16], nc, nr;
+  int64_t total = 0;
+  for (int64_t i = 0; i < M; ++i) {
+    sets(perm[i], tc, nc, tr, nr);
+    total += (int64_t)nr * nc;
+    for (int r = 0; r < nr; ++r)
+      for (int c = 0; c < nc; ++c) h.ptr[(size_t)tr[r] * ntu + tc[c] + 1]++;
+  }
+  if (total > 0xffffffffLL) throw Error(PF_ERR_UNSUPPORTED, "gridding tile lists exceed 2^32 entries");
+  for (int t = 0; t < nt; ++t) h.ptr[t + 1] += h.ptr[t];
+  h.list.resize(h.ptr[nt]);
+  std::vector<uint32_t> fill(h.ptr.begin(), h.ptr.end() - 1);
+  for (int64_t i = 0; i < M; ++i) {
+    sets(perm[i], tc, nc, tr, nr);
+    for (int r = 0; r < nr; ++r)
+      for (int c = 0; c < nc; ++c) h.list[fill[(size_t)tr[r] * ntu + tc[c]]++] = (uint32_t)i;
+  }
+  h.order.resize(nt);
+  for (int t = 0; t < nt; ++t) h.order[t] = (uint32_t)t;
+  std::stable_sort(h.order.begin(), h.order.end(), [&](uint32_t a, uint32_t b) {
+    return h.ptr[a + 1] - h.ptr[a] > h.ptr[b + 1] - h.ptr[b];
+  });
+  return h;
+}
+
+void upload_lists(pf_op* op, const HostLists& h, int TU, int TV, TileLists& tl) {
+  const size_t nt = h.order.size();
+  uint32_t* ptr = (uint32_t*)op->dalloc(sizeof(uint32_t) * (nt + 1));
+  uint32_t* list = (uint32_t*)op->dalloc(sizeof(uint32_t) * h.list.size());
+  uint32_t* order = (uint32_t*)op->dalloc(sizeof(uint32_t) * nt);
+  PF_HIP(hipMemcpy(ptr, h.ptr.data(), sizeof(uint32_t) * (nt + 1), hipMemcpyHostToDevice));
+  if (!h.list.empty()) PF_HIP(hipMemcpy(list, h.list.data(), sizeof(uint32_t) * h.list.size(), hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(order, h.order.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice));
+  tl.TU = TU;
+  tl.TV = TV;
+  tl.ntu = op->Nu / TU;
+  tl.ntv = op->Nv / TV;
+  tl.ptr = ptr;
+  tl.list = list;
+  tl.order = order;
 }
 
 // set_cell_size + uv_scale (cpp/purify/utilities.cc:208-278, operators.h:405-410)
@@ -474,45 +544,8 @@ void build(pf_op* op, const pf_op_desc& d) {
   std::vector<uint32_t>().swap(count);
   std::vector<uint32_t>().swap(key);
   // ---- per-tile gridding lists (every visibility with a tap in the tile) ----
-  std::vector<uint32_t> tcount(op->n_tiles + 1, 0);
-  auto tile_sets = [&](int64_t m, int* tcols, int& nc, int* trows, int& nr) {
-    const int q0 = wrapi(ku[m] + 1, op->Nu), p0 = wrapi(kv[m] + 1, op->Nv);
-    nc = nr = 0;
-    for (int a = 0; a < op->Ju; ++a) {
-      const int tc = ((q0 + a) % op->Nu) / TU;
-      bool seen = false;
-      for (int k = 0; k < nc; ++k) seen |= (tcols[k] == tc);
-      if (!seen) tcols[nc++] = tc;
-    }
-    for (int b = 0; b < op->Jv; ++b) {
-      const int tr = ((p0 + b) % op->Nv) / TV;
-      bool seen = false;
-      for (int k = 0; k < nr; ++k) seen |= (trows[k] == tr);
-      if (!seen) trows[nr++] = tr;
-    }
-  };
-  int tc[16], tr[16], nc, nr;
-  for (int64_t i = 0; i < M; ++i) {
-    tile_sets(perm[i], tc, nc, tr, nr);
-    for (int r = 0; r < nr; ++r)
-      for (int c = 0; c < nc; ++c) tcount[tr[r] * ntu + tc[c] + 1]++;
-  }
-  for (int t = 0; t < op->n_tiles; ++t) tcount[t + 1] += tcount[t];
-  op->tile_list_len = tcount[op->n_tiles];
-  if (op->tile_list_len > 0xffffffffLL) throw Error(PF_ERR_UNSUPPORTED, "gridding tile lists exceed 2^32 entries");
-  std::vector<uint32_t> tlist(op->tile_list_len);
-  {
-    std::vector<uint32_t> fill(tcount.begin(), tcount.end() - 1);
-    for (int64_t i = 0; i < M; ++i) {
-      tile_sets(perm[i], tc, nc, tr, nr);
-      for (int r = 0; r < nr; ++r)
-        for (int c = 0; c < nc; ++c) tlist[fill[tr[r] * ntu + tc[c]]++] = (uint32_t)i;
-    }
-  }
-  op->tl.TU = TU;
-  op->tl.TV = TV;
-  op->tl.ntu = ntu;
-  op->tl.ntv = ntv;
+  const HostLists lists = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, TU, TV);
+  op->tile_list_len = (int64_t)lists.list.size();
 
   // ---- upload ----
   op->d_perm = (int32_t*)op->dalloc(sizeof(int32_t) * M);
@@ -524,27 +557,16 @@ void build(pf_op* op, const pf_op_desc& d) {
     PF_HIP(hipMemcpy(op->d_inv, inv.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
   }
   op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
-  op->d_tptr = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
-  PF_HIP(hipMemcpy(op->d_tptr, tcount.data(), sizeof(uint32_t) * (op->n_tiles + 1), hipMemcpyHostToDevice));
+  upload_lists(op, lists, TU, TV, op->tl);
   op->d_towner = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
   PF_HIP(hipMemcpy(op->d_towner, owner.data(), sizeof(uint32_t) * (op->n_tiles + 1), hipMemcpyHostToDevice));
   op->tl.owner = op->d_towner;
-  {
-    // launch the heaviest tiles first so that the dense centre of the uv-plane does not form the tail
-    std::vector<uint32_t> order(op->n_tiles);
-    for (int t = 0; t < op->n_tiles; ++t) order[t] = (uint32_t)t;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-      return tcount[a + 1] - tcount[a] > tcount[b + 1] - tcount[b];
-    });
-    op->d_torder = (uint32_t*)op->dalloc(sizeof(uint32_t) * op->n_tiles);
-    PF_HIP(hipMemcpy(op->d_torder, order.data(), sizeof(uint32_t) * op->n_tiles, hipMemcpyHostToDevice));
-    op->tl.order = op->d_torder;
+  if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv)) {
+    const HostLists lm = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE);
+    upload_lists(op, lm, MFMA_TILE, MFMA_TILE, op->tlm);
+    op->tile_list_len_m = (int64_t)lm.list.size();
+    op->grid_variant = GRID_MFMA;
   }
-  op->d_tlist = (uint32_t*)op->dalloc(sizeof(uint32_t) * op->tile_list_len);
-  if (op->tile_list_len)
-    PF_HIP(hipMemcpy(op->d_tlist, tlist.data(), sizeof(uint32_t) * op->tile_list_len, hipMemcpyHostToDevice));
-  op->tl.ptr = op->d_tptr;
-  op->tl.list = op->d_tlist;
   if (op->prec == PF_C64) {
     std::vector<int2> kk(M);
     std::vector<cpx<float>> ff(M);
@@ -670,7 +692,7 @@ void adjoint_dev(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
   cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
   cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
   permute_to_sorted<R>(visdata<R>(op), y, ys, s);
-  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, ys, A, R(1), s, op->grid_variant);
+  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, op->tlm, ys, A, R(1), s, op->grid_variant);
   fz_adjoint<R>(op, A, x, scl, s);
 }
 
@@ -839,7 +861,10 @@ int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
   if (!op || !key) return fail(PF_ERR_ARG, "null argument");
   const std::string k(key);
   if (k == "grid_kernel") {
-    if (value != GRID_RESIDUE && value != GRID_ATOMIC) return fail(PF_ERR_ARG, "grid_kernel must be 0 or 1");
+    if (value != GRID_RESIDUE && value != GRID_ATOMIC && value != GRID_MFMA)
+      return fail(PF_ERR_ARG, "grid_kernel must be 0, 1 or 2");
+    if (value == GRID_MFMA && !op->tlm.ptr)
+      return fail(PF_ERR_UNSUPPORTED, "matrix-core gridding needs grid sides >= 16 + J - 1");
     op->grid_variant = (int)value;
     return PF_OK;
   }
@@ -887,15 +912,16 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     const int n = std::max(1, (int)reps);
     float best = 1e30f;
     int best_v = op->grid_variant;
-    for (int variant : {GRID_RESIDUE, GRID_ATOMIC}) {
+    for (int variant : {GRID_RESIDUE, GRID_ATOMIC, GRID_MFMA}) {
       if (variant == GRID_RESIDUE && !grid_rc_ok(op->Nu, op->Nv, op->tl.TU, op->tl.TV, op->Ju, op->Jv)) continue;
+      if (variant == GRID_MFMA && !op->tlm.ptr) continue;
       op->grid_variant = variant;
       auto run = [&] {
         if (op->prec == PF_C64)
-          grid_tiles<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)op->d_vecy,
+          grid_tiles<float>(op->gf, visdata<float>(op), op->tl, op->tlm, (const cpx<float>*)op->d_vecy,
                             (cpx<float>*)op->d_gridA, 1.0f, s, variant);
         else
-          grid_tiles<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)op->d_vecy,
+          grid_tiles<double>(op->gd, visdata<double>(op), op->tl, op->tlm, (const cpx<double>*)op->d_vecy,
                              (cpx<double>*)op->d_gridA, 1.0, s, variant);
       };
       run();  // warm-up
@@ -1028,11 +1054,11 @@ int pf_op_apply_GH(pf_op* op, const void* y_dev, void* grid_dev, pf_stream strea
     hipStream_t s = (hipStream_t)stream;
     if (op->prec == PF_C64) {
       permute_to_sorted<float>(visdata<float>(op), (const cpx<float>*)y_dev, (cpx<float>*)op->d_ysort, s);
-      grid_tiles<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)op->d_ysort, (cpx<float>*)grid_dev,
+      grid_tiles<float>(op->gf, visdata<float>(op), op->tl, op->tlm, (const cpx<float>*)op->d_ysort, (cpx<float>*)grid_dev,
                         1.0f, s, op->grid_variant);
     } else {
       permute_to_sorted<double>(visdata<double>(op), (const cpx<double>*)y_dev, (cpx<double>*)op->d_ysort, s);
-      grid_tiles<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)op->d_ysort,
+      grid_tiles<double>(op->gd, visdata<double>(op), op->tl, op->tlm, (const cpx<double>*)op->d_ysort,
                          (cpx<double>*)grid_dev, 1.0, s, op->grid_variant);
     }
   });

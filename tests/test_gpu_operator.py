@@ -112,7 +112,7 @@ def test_gridding_is_deterministic(precision):
 
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_gridding_variants_agree_and_are_deterministic(precision, variant):
     n, M = 256, 60_000
     u, v = make_uv(13, M, n, sigma=0.5)
@@ -126,7 +126,17 @@ def test_gridding_variants_agree_and_are_deterministic(precision, variant):
     b = gpu.adjoint() * y
     assert np.array_equal(a, b)
     chosen = gpu.op.autotune(2)
-    assert chosen in (0, 1) and gpu.op.get_tuning("grid_kernel") == chosen
+    assert chosen in (0, 1, 2) and gpu.op.get_tuning("grid_kernel") == chosen
+
+
+def test_matrix_core_gridding_needs_room_for_a_tile():
+    # 8 x 8 image -> 16 x 16 grid: smaller than 16 + J - 1, so only the other variants exist
+    u, v = make_uv(19, 200, 8)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, 8, 8, 2.0, 0, 1e-4, "kb", 4, 4,
+                                                     precision="c128")
+    assert lt.op.get_tuning("grid_kernel") != 2
+    with pytest.raises(PurifyError, match="matrix-core"):
+        lt.op.set_tuning("grid_kernel", 2)
 
 
 @pytest.mark.parametrize("kern,J", [("kb", 6), ("kb", 4), ("gauss", 6), ("pswf", 6), ("kbmin", 8)])
