@@ -129,8 +129,9 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  *      closest analogue) ---- */
 /* Keys: "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics, 2 = rank-4 updates of 16 x 16 tiles on
  * the matrix cores, the default where the grid sides are at least 16 + J - 1; all reproducible run to run);
- * "kernel_table" (complex64, J <= 8: 1 = interpolated tap table, about 1e-7 relative error, the default;
- * 0 = evaluate the kernel for every tap); read-only "tile_u", "tile_v".  Results do not depend on
+ * "kernel_table" (complex64, J <= 8: interpolated tap tables, about 1e-7 relative error: 1 = in gridding,
+ * the default, 2 = in gridding and degridding; 0 = evaluate the kernel for every tap); read-only
+ * "tile_u", "tile_v".  Results do not depend on
  * grid_kernel beyond floating-point summation order. */
 int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
 int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);

@@ -150,8 +150,10 @@ __device__ inline double dexp<double>(double x) {
   return exp(x);
 }
 
-// Value of the gridding kernel at offset x (|x| <= J/2 by construction of the taps).
-template <typename R>
+// Value of the gridding kernel at offset x (|x| <= J/2 by construction of the taps).  LEN: 1 = the
+// fixed-length Kaiser-Bessel sum only (the caller guarantees ncoef <= KBLen<R>::N, which keeps the
+// unused coefficients out of registers), 0 = runtime length, -1 = choose by ncoef.
+template <typename R, int LEN = -1>
 __device__ inline R kern1d(const KParams<R>& p, R x) {
   switch (p.kind) {
     case K_KB:
@@ -162,8 +164,14 @@ __device__ inline R kern1d(const KParams<R>& p, R x) {
       // (kernels.cc:20); that happens when Ju != Jv because of the kernel-slot swap of operators.h:324.
       if (t < R(0)) return R(NAN);
       const R s = t + t - R(1);
-      if (p.ncoef <= KBLen<R>::N) return clenshaw_fixed<R, KBLen<R>::N>(p.coef, s);
-      return clenshaw_rt<R>(p.coef, p.ncoef, s);
+      if constexpr (LEN == 1) {
+        return clenshaw_fixed<R, KBLen<R>::N>(p.coef, s);
+      } else if constexpr (LEN == 0) {
+        return clenshaw_rt<R>(p.coef, p.ncoef, s);
+      } else {
+        if (p.ncoef <= KBLen<R>::N) return clenshaw_fixed<R, KBLen<R>::N>(p.coef, s);
+        return clenshaw_rt<R>(p.coef, p.ncoef, s);
+      }
     }
     case K_GAUSS:
     case K_GAUSS_ALT:

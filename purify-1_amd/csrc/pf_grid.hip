@@ -51,7 +51,7 @@ __device__ inline void taps(const KParams<R>& p, int k, R f, int Jrt, R (&w)[J =
 #pragma unroll
   for (int a = 0; a < JJ; ++a) {
     if (J == 0 && a >= Jrt) break;
-    R val = kern1d<R>(p, f - R(1 + a));
+    R val = kern1d<R, J == 0 ? 0 : 1>(p, f - R(1 + a));
     w[a] = ((k + a) & 1) ? -val : val;
   }
 }
@@ -88,6 +88,28 @@ __device__ inline void taps_axis(const KParams<R>& p, const float* tab, int k, R
   taps<R, J>(p, k, f, Jrt, w);
 }
 
+// With equal compile-time supports the two axis kernels are identical (the reference's kernel-slot swap
+// exchanges equal kernels), so both axes read the u-axis parameters and one coefficient set stays in
+// registers.
+template <int JU, int JV, typename R>
+__device__ inline const KParams<R>& kparams_v(const GridGeom<R>& g) {
+  if constexpr (JU == JV && JU > 0) return g.ku;
+  else return g.kv;
+}
+template <int JU, int JV, typename R>
+__device__ inline const float* table_v(const GridGeom<R>& g) {
+  if constexpr (JU == JV && JU > 0) return g.tab_u;
+  else return g.tab_v;
+}
+
+// Host: the kernels instantiated for a compile-time support sum the Kaiser-Bessel expansion with its
+// fixed length, which must cover the host's coefficient count.
+template <typename R>
+static bool fixed_len_ok(const GridGeom<R>& g) {
+  auto ok = [](const KParams<R>& p) { return (p.kind != K_KB && p.kind != K_KBMIN) || p.ncoef <= KBLen<R>::N; };
+  return ok(g.ku) && ok(g.kv);
+}
+
 // Tile kernels keep the hardware's round-robin spread of workgroups over the 8 XCDs: the visibility
 // density is concentrated in the centre of the uv-plane, and giving each XCD a contiguous band of
 // tiles (to share halos in one L2) would put all heavy tiles on two XCDs.
@@ -96,32 +118,68 @@ __device__ inline void taps_axis(const KParams<R>& p, const float* tab, int k, R
 // Degridding  y = G grid: one block per tile stages the tile plus its (J-1)-wide halo in LDS, then
 // every visibility whose first tap lies in the tile reads its J x J taps from LDS.
 // ---------------------------------------------------------------------------------------------
+constexpr int DG_THREADS = 256;
+// halo rows one wave stages per round: 4 waves cover 40 rows (32 + J - 1, J <= 9) in one round for
+// complex64, in two for complex128 (half the staging registers)
+template <typename R>
+constexpr int dg_rows() { return sizeof(R) == 4 ? 10 : 5; }
+
 template <typename R, int JU, int JV>
-__global__ void __launch_bounds__(256) k_degrid_tiles(GridGeom<R> g, VisData<R> v, TileLists tl,
-                                                      const cpx<R>* __restrict__ grid, cpx<R>* __restrict__ y, R scl) {
+__global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisData<R> v, TileLists tl,
+                                                             const cpx<R>* __restrict__ grid, cpx<R>* __restrict__ y,
+                                                             R scl) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
-  const int tile = tl.order ? (int)tl.order[blockIdx.x] : (int)blockIdx.x;
-  const uint32_t beg = tl.owner[tile], end = tl.owner[tile + 1];
-  if (beg == end) return;
+  const uint4 sl = tl.slot_own[blockIdx.x];
+  const int tile = (int)sl.x;
+  const uint32_t beg = sl.y, end = sl.z;
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
+  constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
   const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
   const int tx0 = tu * tl.TU, ty0 = tv * tl.TV;
   const int W = tl.TU + Ju - 1, H = tl.TV + Jv - 1;
-  for (int e = threadIdx.x; e < W * H; e += blockDim.x) {
-    const int ly = e / W, lx = e - ly * W;
-    int gy = ty0 + ly, gx = tx0 + lx;
-    if (gy >= g.Nv) gy -= g.Nv;
-    if (gx >= g.Nu) gx -= g.Nu;
-    lds[e] = grid[(int64_t)gy * g.Nu + gx];
+  // this thread's first visibility: its loads go out before the tile staging so the latencies overlap
+  uint32_t i = beg + threadIdx.x;
+  VisCoord<R> c{};
+  int32_t dst = 0;
+  if (i < end) {
+    c = vis_coord(v, i, Ju, Jv);
+    dst = v.perm[i];
+  }
+  // stage the tile and its halo: each wave takes DG_ROWS rows per round, every load before any store
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = DG_THREADS / 64;
+  constexpr int DG_ROWS = dg_rows<R>();
+  for (int r0 = wave * DG_ROWS; r0 < H; r0 += nw * DG_ROWS) {
+    for (int lx = lane; lx < W; lx += 64) {
+      int gx = tx0 + lx;
+      if (gx >= g.Nu) gx -= g.Nu;
+      // rows past the halo re-read its last row (always in bounds, never stored): unconditional loads
+      // keep tmp in registers
+      cpx<R> tmp[DG_ROWS];
+#pragma unroll
+      for (int k = 0; k < DG_ROWS; ++k) {
+        int gy = ty0 + min(r0 + k, H - 1);
+        if (gy >= g.Nv) gy -= g.Nv;
+        tmp[k] = grid[(int64_t)gy * g.Nu + gx];
+      }
+#pragma unroll
+      for (int k = 0; k < DG_ROWS; ++k)
+        if (r0 + k < H) lds[(r0 + k) * W + lx] = tmp[k];
+    }
   }
   __syncthreads();
-  constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
-  for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-    const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
+  for (; i < end; i += DG_THREADS) {
+    // the next visibility's loads go out before this one's arithmetic
+    const uint32_t nxt = i + DG_THREADS;
+    VisCoord<R> cn = c;
+    int32_t dn = dst;
+    if (nxt < end) {
+      cn = vis_coord(v, nxt, Ju, Jv);
+      dn = v.perm[nxt];
+    }
     R wu[JJU], wv[JJV];
     taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
-    taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, Jv, wv);
+    taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, Jv, wv);
     const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
     const cpx<R>* base = lds + ly0 * W + lx0;
     cpx<R> acc = mk<R>(R(0), R(0));
@@ -140,7 +198,9 @@ __global__ void __launch_bounds__(256) k_degrid_tiles(GridGeom<R> g, VisData<R> 
       acc.y += wv[b] * racc.y;
     }
     if (v.weights) acc = cmul(acc, v.weights[i]);
-    y[v.perm[i]] = scale(acc, scl);
+    y[dst] = scale(acc, scl);
+    c = cn;
+    dst = dn;
   }
 }
 
@@ -172,15 +232,16 @@ __global__ void __launch_bounds__(64) k_grid_rc(GridGeom<R> g, VisData<R> v, Til
   cpx<R>* sv = acc + TV * P;                                    // 64 scaled visibility values
   RCTap<R>* tu_tab = reinterpret_cast<RCTap<R>*>(sv + 64);     // [64][8]: entry a = tap of residue a
   RCTap<R>* tv_tab = tu_tab + 64 * 8;                           // [64][8]: coordinates pre-multiplied by P
-  const int tile = tl.order ? (int)tl.order[blockIdx.x] : (int)blockIdx.x;
+  const uint4 sl = tl.slot[blockIdx.x];
+  const int tile = (int)sl.x;
   const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
   const int tx0 = tu * TU, ty0 = tv * TV;
   const int lane = threadIdx.x, la = lane & 7, lb = lane >> 3;
   for (int e = lane; e < TV * P; e += 64) acc[e] = mk<R>(R(0), R(0));
   int cur = -1;
   cpx<R> accv = mk<R>(R(0), R(0));
-  const uint32_t end = tl.ptr[tile + 1];
-  for (uint32_t base = tl.ptr[tile]; base < end; base += 64) {
+  const uint32_t end = sl.z;
+  for (uint32_t base = sl.y; base < end; base += 64) {
     const int nb = (int)min((uint32_t)64, end - base);
     if (lane < nb) {
       // visibility `lane` of the batch: value, tap weights and their residue-rotated placement
@@ -191,7 +252,7 @@ __global__ void __launch_bounds__(64) k_grid_rc(GridGeom<R> g, VisData<R> v, Til
       sv[lane] = scale(s, scl);
       R wu[JU], wv[JV];
       taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, JU, wu);
-      taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, JV, wv);
+      taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, JV, wv);
       int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
       if (dx >= g.Nu - 8) dx -= g.Nu;
       if (dy >= g.Nv - 8) dy -= g.Nv;
@@ -268,7 +329,8 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
   const int TU = tl.TU, TV = tl.TV, TT = TU * TV;
   R* acc_re = reinterpret_cast<R*>(smem_raw);
   R* acc_im = acc_re + TT;
-  const int tile = blockIdx.x;
+  const uint4 sl = tl.slot[blockIdx.x];
+  const int tile = (int)sl.x;
   const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
   const int tx0 = tu * TU, ty0 = tv * TV;
   for (int e = threadIdx.x; e < TT; e += blockDim.x) {
@@ -278,8 +340,8 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
   __syncthreads();
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
   constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
-  const uint32_t end = tl.ptr[tile + 1];
-  for (uint32_t base = tl.ptr[tile]; base < end; base += blockDim.x) {
+  const uint32_t end = sl.z;
+  for (uint32_t base = sl.y; base < end; base += blockDim.x) {
     const uint32_t idx = base + threadIdx.x;
     if (idx < end) {
       const uint32_t i = tl.list[idx];
@@ -289,7 +351,7 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
       s = scale(s, scl);
       R wu[JJU], wv[JJV];
       taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
-      taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, Jv, wv);
+      taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, Jv, wv);
       const int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
 #pragma unroll
       for (int b = 0; b < JJV; ++b) {
@@ -322,13 +384,15 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
 // ---------------------------------------------------------------------------------------------
 // Gridding on the matrix cores.  Inside a 16 x 16 tile
 //     tile[i][j] = sum_t (s_t wv_t[i - dy_t]) * wu_t[j - dx_t]
-// over the tile's list: the product A B of a 16 x K complex matrix (one column per visibility, its
-// v-taps scaled by the value, zero outside the footprint) with a K x 16 real one (its u-taps).  One
-// wave per tile.  In batches of 64 list entries, lane t writes entry t's column of A (real and
-// imaginary parts) and row of B to LDS; then each MFMA 16x16x4 step adds four entries: lane l
-// supplies A[l % 16][l / 16] and B[l / 16][l % 16], and holds four result rows of column l % 16:
-// 4 (l / 16) + r for the FP32 instruction, l / 16 + 4 r for the FP64 one (r < 4).  The summation order is fixed by the list and the instruction: reproducible bit for
-// bit.  Needs N >= 16 + J - 1 on both axes, so that a footprint meets a tile in one piece.
+// over the tile's list: the product A B of a 16 x K complex matrix (one column per visibility: its
+// v-taps times its value, zero outside the footprint) with a K x 16 real one (its u-taps).  One wave
+// per tile.  In batches of 64 list entries, lane t evaluates entry t's taps and stores a compact record
+// in LDS: the footprint offset (dx, dy), the J_v products s wv and the J_u taps wu.  Each MFMA 16x16x4
+// step then adds four entries: lane l gathers A[l % 16][l / 16] and B[l / 16][l % 16] from the records
+// of entry 4k + l / 16 (zero outside the footprint) and holds four result rows of column l % 16:
+// 4 (l / 16) + r for the FP32 instruction, l / 16 + 4 r for the FP64 one (r < 4).  The summation
+// order is fixed by the list and the instruction: reproducible bit for bit.  Needs N >= 16 + J - 1 on
+// both axes, so that a footprint meets a tile in one piece.
 // ---------------------------------------------------------------------------------------------
 template <typename R>
 struct Mfma;
@@ -347,68 +411,80 @@ struct Mfma<double> {
   __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
-constexpr int MB = 64;  // list entries per batch
+constexpr int MB = 64;           // list entries per batch
+constexpr int MFMA_NOTAP = 1024; // footprint offset of an unused entry slot: no row or column matches
 
 template <typename R, int JU, int JV>
 __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, TileLists tl,
                                                   const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
   constexpr int T = MFMA_TILE;
-  __shared__ R a_re[MB * T], a_im[MB * T], b_u[MB * T];  // [entry][row] and [entry][column]
+  constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
+  constexpr int SV = JJV | 1, SU = JJU | 1;  // odd record strides: conflict-free stores from 64 lanes
+  __shared__ int2 rd[MB];                    // (dx, dy): footprint origin relative to the tile
+  __shared__ cpx<R> ra[MB * SV];             // s * wv[e]
+  __shared__ R rb[MB * SU];                  // wu[e]
   using acc_t = typename Mfma<R>::acc_t;
-  const int tile = tl.order ? (int)tl.order[blockIdx.x] : (int)blockIdx.x;
+  const uint4 sl = tl.slot[blockIdx.x];
+  const int tile = (int)sl.x;
   const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
   const int tx0 = tu * T, ty0 = tv * T;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, li = lane & 15, lt = lane >> 4;
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
-  constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
   acc_t cre = {R(0), R(0), R(0), R(0)}, cim = {R(0), R(0), R(0), R(0)};
-  R* pre = a_re + lane * T;
-  R* pim = a_im + lane * T;
-  R* pb = b_u + lane * T;
-  const uint32_t end = tl.ptr[tile + 1];
-  for (uint32_t base = tl.ptr[tile]; base < end; base += MB) {
+  const uint32_t end = sl.z;
+  // entry data of the batch being prepared; the next batch's is loaded while this one's MFMAs run
+  VisCoord<R> c{};
+  cpx<R> s = mk<R>(R(0), R(0));
+  if (sl.y + lane < end) {
+    const uint32_t i = tl.list[sl.y + lane];
+    c = vis_coord(v, i, Ju, Jv);
+    s = yin[i];  // visibilities in sorted order
+    if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
+  }
+  for (uint32_t base = sl.y; base < end; base += MB) {
     const int nb = (int)min((uint32_t)MB, end - base);
-#pragma unroll
-    for (int k = 0; k < T; ++k) {
-      pre[k] = R(0);
-      pim[k] = R(0);
-      pb[k] = R(0);
-    }
     if (lane < nb) {
-      const uint32_t i = tl.list[base + lane];
-      const VisCoord<R> c = vis_coord(v, i, Ju, Jv);
-      cpx<R> s = yin[i];  // visibilities in sorted order
-      if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
       s = scale(s, scl);
       R wu[JJU], wv[JJV];
       taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
-      taps_axis<R, JV>(g.kv, g.tab_v, c.kv, c.fv, Jv, wv);
+      taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, Jv, wv);
       int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
       if (dx >= T) dx -= g.Nu;  // footprint starts left of / above the tile
       if (dy >= T) dy -= g.Nv;
+      rd[lane] = make_int2(dx, dy);
 #pragma unroll
       for (int e = 0; e < JJV; ++e) {
         if (JV == 0 && e >= Jv) break;
-        const int y = dy + e;
-        if ((unsigned)y < (unsigned)T) {
-          pre[y] = s.x * wv[e];
-          pim[y] = s.y * wv[e];
-        }
+        ra[lane * SV + e] = mk<R>(s.x * wv[e], s.y * wv[e]);
       }
 #pragma unroll
       for (int e = 0; e < JJU; ++e) {
         if (JU == 0 && e >= Ju) break;
-        const int x = dx + e;
-        if ((unsigned)x < (unsigned)T) pb[x] = wu[e];
+        rb[lane * SU + e] = wu[e];
       }
+    } else {
+      rd[lane] = make_int2(MFMA_NOTAP, MFMA_NOTAP);
     }
     __syncthreads();
+    const uint32_t nxt = base + MB + lane;
+    if (nxt < end) {
+      const uint32_t i = tl.list[nxt];
+      c = vis_coord(v, i, Ju, Jv);
+      s = yin[i];
+      if (v.weights) s = cmulc(s, v.weights[i]);
+    }
     const int nk = (nb + 3) >> 2;
+#pragma unroll 4
     for (int k = 0; k < nk; ++k) {
-      const int idx = k * 64 + lane;  // entry 4k + lane / 16, row / column lane % 16
-      const R b = b_u[idx];
-      cre = Mfma<R>::step(a_re[idx], b, cre);
-      cim = Mfma<R>::step(a_im[idx], b, cim);
+      const int t = 4 * k + lt;  // this lane's entry in step k; li is its row of A and column of B
+      const int2 d = rd[t];
+      const int ev = li - d.y, eu = li - d.x;
+      const bool okv = (unsigned)ev < (unsigned)Jv, oku = (unsigned)eu < (unsigned)Ju;
+      const cpx<R> a = ra[t * SV + (okv ? ev : 0)];
+      const R b = rb[t * SU + (oku ? eu : 0)];
+      const R bb = oku ? b : R(0);
+      cre = Mfma<R>::step(okv ? a.x : R(0), bb, cre);
+      cim = Mfma<R>::step(okv ? a.y : R(0), bb, cim);
     }
     __syncthreads();
   }
@@ -424,12 +500,13 @@ bool grid_mfma_ok(int Nu, int Nv, int Ju, int Jv) {
 template <typename R>
 void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
             hipStream_t s) {
-  if (v.M == 0) return;
-  const dim3 blocks((unsigned)(t.ntu * t.ntv));
+  if (v.M == 0 || t.n_own == 0) return;
+  const dim3 blocks((unsigned)t.n_own);
   const size_t shm = sizeof(cpx<R>) * (size_t)(t.TU + g.Ju - 1) * (t.TV + g.Jv - 1);
   ProfScope prof("degrid", s);
-#define PF_DG(A, B) hipLaunchKernelGGL((k_degrid_tiles<R, A, B>), blocks, dim3(256), shm, s, g, v, t, grid, y, scale)
-  if (g.Ju == g.Jv) {
+#define PF_DG(A, B) \
+  hipLaunchKernelGGL((k_degrid_tiles<R, A, B>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y, scale)
+  if (g.Ju == g.Jv && fixed_len_ok(g)) {
     switch (g.Ju) {
       case 2: PF_DG(2, 2); break;
       case 3: PF_DG(3, 3); break;
@@ -454,11 +531,11 @@ bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv) {
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm, const cpx<R>* y,
                 cpx<R>* grid, R scale, hipStream_t s, int variant) {
-  if (variant == GRID_MFMA && tm.ptr && grid_mfma_ok(g.Nu, g.Nv, g.Ju, g.Jv)) {
+  if (variant == GRID_MFMA && tm.slot && grid_mfma_ok(g.Nu, g.Nv, g.Ju, g.Jv)) {
     const dim3 blocks((unsigned)(tm.ntu * tm.ntv));
     ProfScope prof("grid_tiles", s);
 #define PF_GM(A, B) hipLaunchKernelGGL((k_grid_mfma<R, A, B>), blocks, dim3(64), 0, s, g, v, tm, y, grid, scale)
-    if (g.Ju == g.Jv) {
+    if (g.Ju == g.Jv && fixed_len_ok(g)) {
       switch (g.Ju) {
         case 2: PF_GM(2, 2); break;
         case 3: PF_GM(3, 3); break;
@@ -477,7 +554,7 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
     return;
   }
   const dim3 blocks((unsigned)(t.ntu * t.ntv));
-  if (variant == GRID_RESIDUE && grid_rc_ok(g.Nu, g.Nv, t.TU, t.TV, g.Ju, g.Jv)) {
+  if (variant == GRID_RESIDUE && grid_rc_ok(g.Nu, g.Nv, t.TU, t.TV, g.Ju, g.Jv) && fixed_len_ok(g)) {
     const size_t shm = sizeof(cpx<R>) * ((size_t)(t.TU + 1) * t.TV + 64) + 2 * sizeof(R) * 2 * 64 * 8;
     ProfScope prof("grid_tiles", s);
 #define PF_RC(A, B) hipLaunchKernelGGL((k_grid_rc<R, A, B>), blocks, dim3(64), shm, s, g, v, t, y, grid, scale)
@@ -510,7 +587,7 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
   const size_t shm = 2 * sizeof(R) * (size_t)t.TU * t.TV;
   ProfScope prof("grid_tiles_atomic", s);
 #define PF_GT(A, B) hipLaunchKernelGGL((k_grid_tiles<R, A, B>), blocks, dim3(64), shm, s, g, v, t, y, grid, scale)
-  if (g.Ju == g.Jv) {
+  if (g.Ju == g.Jv && fixed_len_ok(g)) {
     switch (g.Ju) {
       case 2: PF_GT(2, 2); break;
       case 3: PF_GT(3, 3); break;

@@ -134,10 +134,13 @@ std::vector<float> make_tap_table(int kind, int Jeval, int J, double os);
 
 struct TileLists {
   int TU = 32, TV = 32, ntu = 0, ntv = 0;
-  const uint32_t* ptr = nullptr;    // n_tiles + 1
-  const uint32_t* list = nullptr;   // sorted vis indices per tile (every visibility with a tap in it)
-  const uint32_t* owner = nullptr;  // n_tiles + 1: sorted range of the visibilities whose first tap is in the tile
-  const uint32_t* order = nullptr;  // n_tiles: launch order, heaviest tile first (nullptr: natural order)
+  const uint32_t* list = nullptr;  // sorted vis indices per tile (every visibility with a tap in it)
+  // launch slots, heaviest tile first: (tile, begin, end, 0) of the tile's range in `list` (all tiles)
+  const uint4* slot = nullptr;
+  // launch slots of the degridding kernel: (tile, begin, end, 0) of the sorted visibilities whose first
+  // tap lies in the tile, for the n_own tiles that own any, heaviest first
+  const uint4* slot_own = nullptr;
+  int n_own = 0;
 };
 
 template <typename R>

@@ -279,7 +279,6 @@ struct pf_op {
   int32_t* d_perm = nullptr;
   int32_t* d_inv = nullptr;    // inverse permutation
   void* d_ysort = nullptr;     // visibilities in sorted order (adjoint input after the permutation)
-  uint32_t* d_towner = nullptr;
   float* d_tab_u = nullptr;  // complex64 tap tables
   float* d_tab_v = nullptr;
   void* d_sx = nullptr;
@@ -294,6 +293,7 @@ struct pf_op {
   GridGeom<double> gd;
   std::unique_ptr<Profiler> prof;  // per-kernel timing when enabled
   int grid_variant = GRID_RESIDUE;  // tuning knob: gridding kernel
+  int kernel_table = 0;             // tuning knob: 0 no tap tables, 1 in gridding, 2 in gridding and degridding
   int64_t device_bytes = 0;
   std::vector<void*> allocs;
 
@@ -346,7 +346,8 @@ int wrapi(int64_t a, int n) {
 // in it (ascending), and the tiles in launch order, heaviest first, so that the dense centre of the
 // uv-plane does not form the tail of the launch.
 struct HostLists {
-  std::vector<uint32_t> ptr, list, order;
+  std::vector<uint32_t> ptr, list;
+  std::vector<uint4> slot;  // (tile, begin, end, 0), heaviest tile first
 };
 
 HostLists tile_lists(const std::vector<int32_t>& perm, const std::vector<int64_t>& ku, const std::vector<int64_t>& kv,
@@ -389,29 +390,24 @@ HostLists tile_lists(const std::vector<int32_t>& perm, const std::vector<int64_t
     for (int r = 0; r < nr; ++r)
       for (int c = 0; c < nc; ++c) h.list[fill[(size_t)tr[r] * ntu + tc[c]]++] = (uint32_t)i;
   }
-  h.order.resize(nt);
-  for (int t = 0; t < nt; ++t) h.order[t] = (uint32_t)t;
-  std::stable_sort(h.order.begin(), h.order.end(), [&](uint32_t a, uint32_t b) {
-    return h.ptr[a + 1] - h.ptr[a] > h.ptr[b + 1] - h.ptr[b];
-  });
+  h.slot.resize(nt);
+  for (int t = 0; t < nt; ++t) h.slot[t] = make_uint4((uint32_t)t, h.ptr[t], h.ptr[t + 1], 0u);
+  std::stable_sort(h.slot.begin(), h.slot.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
   return h;
 }
 
 void upload_lists(pf_op* op, const HostLists& h, int TU, int TV, TileLists& tl) {
-  const size_t nt = h.order.size();
-  uint32_t* ptr = (uint32_t*)op->dalloc(sizeof(uint32_t) * (nt + 1));
+  const size_t nt = h.slot.size();
   uint32_t* list = (uint32_t*)op->dalloc(sizeof(uint32_t) * h.list.size());
-  uint32_t* order = (uint32_t*)op->dalloc(sizeof(uint32_t) * nt);
-  PF_HIP(hipMemcpy(ptr, h.ptr.data(), sizeof(uint32_t) * (nt + 1), hipMemcpyHostToDevice));
+  uint4* slot = (uint4*)op->dalloc(sizeof(uint4) * nt);
   if (!h.list.empty()) PF_HIP(hipMemcpy(list, h.list.data(), sizeof(uint32_t) * h.list.size(), hipMemcpyHostToDevice));
-  PF_HIP(hipMemcpy(order, h.order.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(slot, h.slot.data(), sizeof(uint4) * nt, hipMemcpyHostToDevice));
   tl.TU = TU;
   tl.TV = TV;
   tl.ntu = op->Nu / TU;
   tl.ntv = op->Nv / TV;
-  tl.ptr = ptr;
   tl.list = list;
-  tl.order = order;
+  tl.slot = slot;
 }
 
 // set_cell_size + uv_scale (cpp/purify/utilities.cc:208-278, operators.h:405-410)
@@ -501,6 +497,7 @@ void build(pf_op* op, const pf_op_desc& d) {
     PF_HIP(hipMemcpy(op->d_tab_v, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice));
     op->gf.tab_u = op->d_tab_u;
     op->gf.tab_v = op->d_tab_v;
+    op->kernel_table = 1;
   }
 
   // ---- coordinates in pixels, tap origins, sort by owner tile ----
@@ -558,9 +555,17 @@ void build(pf_op* op, const pf_op_desc& d) {
   }
   op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
   upload_lists(op, lists, TU, TV, op->tl);
-  op->d_towner = (uint32_t*)op->dalloc(sizeof(uint32_t) * (op->n_tiles + 1));
-  PF_HIP(hipMemcpy(op->d_towner, owner.data(), sizeof(uint32_t) * (op->n_tiles + 1), hipMemcpyHostToDevice));
-  op->tl.owner = op->d_towner;
+  {
+    // degridding launch slots: the tiles that own visibilities, most visibilities first
+    std::vector<uint4> so;
+    for (int t = 0; t < op->n_tiles; ++t)
+      if (owner[t + 1] > owner[t]) so.push_back(make_uint4((uint32_t)t, owner[t], owner[t + 1], 0u));
+    std::stable_sort(so.begin(), so.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
+    uint4* d = (uint4*)op->dalloc(sizeof(uint4) * std::max<size_t>(so.size(), 1));
+    if (!so.empty()) PF_HIP(hipMemcpy(d, so.data(), sizeof(uint4) * so.size(), hipMemcpyHostToDevice));
+    op->tl.slot_own = d;
+    op->tl.n_own = (int)so.size();
+  }
   if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv)) {
     const HostLists lm = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE);
     upload_lists(op, lm, MFMA_TILE, MFMA_TILE, op->tlm);
@@ -646,6 +651,14 @@ template <>
 const GridGeom<double>& geom<double>(const pf_op* op) {
   return op->gd;
 }
+// Degridding evaluates the kernel directly unless kernel_table = 2: one dependent table read fewer per
+// visibility, which is what bounds the degridding kernel's latency.
+template <typename R>
+GridGeom<R> degrid_geom(const pf_op* op) {
+  GridGeom<R> g = geom<R>(op);
+  if (op->kernel_table != 2) g.tab_u = g.tab_v = nullptr;
+  return g;
+}
 
 template <typename R>
 SZ<R> make_sz(const pf_op* op, double scl) {
@@ -683,7 +696,7 @@ void forward_dev(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s) {
   const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
   cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
   fz_forward<R>(op, x, B, scl, s);
-  degrid<R>(geom<R>(op), visdata<R>(op), op->tl, B, y, R(1), s);
+  degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, B, y, R(1), s);
 }
 
 template <typename R>
@@ -863,14 +876,15 @@ int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
   if (k == "grid_kernel") {
     if (value != GRID_RESIDUE && value != GRID_ATOMIC && value != GRID_MFMA)
       return fail(PF_ERR_ARG, "grid_kernel must be 0, 1 or 2");
-    if (value == GRID_MFMA && !op->tlm.ptr)
+    if (value == GRID_MFMA && !op->tlm.slot)
       return fail(PF_ERR_UNSUPPORTED, "matrix-core gridding needs grid sides >= 16 + J - 1");
     op->grid_variant = (int)value;
     return PF_OK;
   }
   if (k == "kernel_table") {
-    if (value != 0 && value != 1) return fail(PF_ERR_ARG, "kernel_table must be 0 or 1");
+    if (value < 0 || value > 2) return fail(PF_ERR_ARG, "kernel_table must be 0, 1 or 2");
     if (value && !op->d_tab_u) return fail(PF_ERR_UNSUPPORTED, "tap tables exist only for complex64 and J <= 8");
+    op->kernel_table = (int)value;
     op->gf.tab_u = value ? op->d_tab_u : nullptr;
     op->gf.tab_v = value ? op->d_tab_v : nullptr;
     return PF_OK;
@@ -886,7 +900,7 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
     return PF_OK;
   }
   if (k == "kernel_table") {
-    *value = op->gf.tab_u ? 1 : 0;
+    *value = op->kernel_table;
     return PF_OK;
   }
   if (k == "tile_u") {
@@ -914,7 +928,7 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     int best_v = op->grid_variant;
     for (int variant : {GRID_RESIDUE, GRID_ATOMIC, GRID_MFMA}) {
       if (variant == GRID_RESIDUE && !grid_rc_ok(op->Nu, op->Nv, op->tl.TU, op->tl.TV, op->Ju, op->Jv)) continue;
-      if (variant == GRID_MFMA && !op->tlm.ptr) continue;
+      if (variant == GRID_MFMA && !op->tlm.slot) continue;
       op->grid_variant = variant;
       auto run = [&] {
         if (op->prec == PF_C64)
@@ -1041,9 +1055,9 @@ int pf_op_apply_G(pf_op* op, const void* grid_dev, void* y_dev, pf_stream stream
   return guarded([&] {
     ProfInstall pi(op);
     if (op->prec == PF_C64)
-      degrid<float>(op->gf, visdata<float>(op), op->tl, (const cpx<float>*)grid_dev, (cpx<float>*)y_dev, 1.0f, (hipStream_t)stream);
+      degrid<float>(degrid_geom<float>(op), visdata<float>(op), op->tl, (const cpx<float>*)grid_dev, (cpx<float>*)y_dev, 1.0f, (hipStream_t)stream);
     else
-      degrid<double>(op->gd, visdata<double>(op), op->tl, (const cpx<double>*)grid_dev, (cpx<double>*)y_dev, 1.0,
+      degrid<double>(degrid_geom<double>(op), visdata<double>(op), op->tl, (const cpx<double>*)grid_dev, (cpx<double>*)y_dev, 1.0,
                      (hipStream_t)stream);
   });
 }

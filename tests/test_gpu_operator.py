@@ -140,7 +140,7 @@ def test_matrix_core_gridding_needs_room_for_a_tile():
 
 
 @pytest.mark.parametrize("kern,J", [("kb", 6), ("kb", 4), ("gauss", 6), ("pswf", 6), ("kbmin", 8)])
-@pytest.mark.parametrize("table", [1, 0])
+@pytest.mark.parametrize("table", [2, 1, 0])
 def test_complex64_tap_table_and_exact_kernels(kern, J, table):
     n, M = 512, 100_000
     u, v = make_uv(16 + J, M, n)
