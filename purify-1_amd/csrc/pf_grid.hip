@@ -16,6 +16,8 @@
 //    exactly one wave, so the grid needs no zero fill and there are no HBM atomics; the sum order
 //    inside a tile is fixed by the list order and the lane order of the LDS adds, which makes the
 //    result run-to-run reproducible.
+#include <algorithm>
+
 #include "pf_internal.h"
 
 namespace pf {
@@ -56,36 +58,47 @@ __device__ inline void taps(const KParams<R>& p, int k, R f, int Jrt, R (&w)[J =
   }
 }
 
-// Same weights from the complex64 tap table (J <= 8): one row read and J interpolations.
+// Row of a tap table for fraction f: four 16-byte loads, interpolated by tab_interp.
+struct TabRow {
+  float4 t0, t1, d0, d1;
+  float r;
+};
 template <int J>
-__device__ inline void taps_tab(const float* __restrict__ tab, int k, float f, float (&w)[J]) {
-  const float d = f - 0.5f * (float)J;  // in (0, 1]
-  const float qf = d * (float)TAB_Q;
+__device__ inline TabRow tab_row(const float* __restrict__ tab, float f) {
+  const float qf = (f - 0.5f * (float)J) * (float)TAB_Q;  // delta in (0, 1] times TAB_Q
   int q = (int)qf;
   q = q < 0 ? 0 : (q > TAB_Q - 1 ? TAB_Q - 1 : q);
-  const float r = qf - (float)q;
   const float4* row = reinterpret_cast<const float4*>(tab + (size_t)q * TAB_ROW);
-  const float4 t0 = row[0], t1 = row[1], d0 = row[2], d1 = row[3];
-  const float t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
-  const float dd[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+  return TabRow{row[0], row[1], row[2], row[3], qf - (float)q};
+}
+template <int J>
+__device__ inline void tab_interp(const TabRow& a, int k, float (&w)[J]) {
+  const float t[8] = {a.t0.x, a.t0.y, a.t0.z, a.t0.w, a.t1.x, a.t1.y, a.t1.z, a.t1.w};
+  const float dd[8] = {a.d0.x, a.d0.y, a.d0.z, a.d0.w, a.d1.x, a.d1.y, a.d1.z, a.d1.w};
 #pragma unroll
-  for (int a = 0; a < J; ++a) {
-    const float val = t[a] + r * dd[a];
-    w[a] = ((k + a) & 1) ? -val : val;
+  for (int e = 0; e < J; ++e) {
+    const float val = t[e] + a.r * dd[e];
+    w[e] = ((k + e) & 1) ? -val : val;
   }
 }
 
-// Tap weights of one axis: the table when there is one (complex64, J <= 8), else the kernel itself.
-template <typename R, int J>
-__device__ inline void taps_axis(const KParams<R>& p, const float* tab, int k, R f, int Jrt,
-                                 R (&w)[J == 0 ? JMAX : J]) {
-  if constexpr (sizeof(R) == 4 && J > 0 && J <= 8) {
-    if (tab) {
-      taps_tab<J>(tab, k, f, w);
+// Tap weights of both axes.  With tables, both rows are requested before either is used, so a
+// visibility waits for one memory round trip, not two.
+template <typename R, int JU, int JV>
+__device__ inline void taps_both(const GridGeom<R>& g, const KParams<R>& kv, const float* tab_v, int ku_, R fu,
+                                 int kv_, R fv, int Ju, int Jv, R (&wu)[JU == 0 ? JMAX : JU],
+                                 R (&wv)[JV == 0 ? JMAX : JV]) {
+  if constexpr (sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8) {
+    if (g.tab_u && tab_v) {
+      const TabRow a = tab_row<JU>(g.tab_u, fu);
+      const TabRow b = tab_row<JV>(tab_v, fv);
+      tab_interp<JU>(a, ku_, wu);
+      tab_interp<JV>(b, kv_, wv);
       return;
     }
   }
-  taps<R, J>(p, k, f, Jrt, w);
+  taps<R, JU>(g.ku, ku_, fu, Ju, wu);
+  taps<R, JV>(kv, kv_, fv, Jv, wv);
 }
 
 // With equal compile-time supports the two axis kernels are identical (the reference's kernel-slot swap
@@ -178,8 +191,7 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
       dn = v.perm[nxt];
     }
     R wu[JJU], wv[JJV];
-    taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
-    taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, Jv, wv);
+    taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
     const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
     const cpx<R>* base = lds + ly0 * W + lx0;
     cpx<R> acc = mk<R>(R(0), R(0));
@@ -251,8 +263,7 @@ __global__ void __launch_bounds__(64) k_grid_rc(GridGeom<R> g, VisData<R> v, Til
       if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
       sv[lane] = scale(s, scl);
       R wu[JU], wv[JV];
-      taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, JU, wu);
-      taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, JV, wv);
+      taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, JU, JV, wu, wv);
       int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
       if (dx >= g.Nu - 8) dx -= g.Nu;
       if (dy >= g.Nv - 8) dy -= g.Nv;
@@ -350,8 +361,7 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
       if (v.weights) s = cmulc(s, v.weights[i]);
       s = scale(s, scl);
       R wu[JJU], wv[JJV];
-      taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
-      taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, Jv, wv);
+      taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
       const int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
 #pragma unroll
       for (int b = 0; b < JJV; ++b) {
@@ -411,7 +421,8 @@ struct Mfma<double> {
   __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
-constexpr int MB = 64;           // list entries per batch
+constexpr int MB = 64;           // list entries per batch (a multiple of 16)
+constexpr int MQ = MB / 4;       // entries per MFMA row group of a batch
 constexpr int MFMA_NOTAP = 1024; // footprint offset of an unused entry slot: no row or column matches
 
 template <typename R, int JU, int JV>
@@ -419,72 +430,92 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
                                                   const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
   constexpr int T = MFMA_TILE;
   constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
-  constexpr int SV = JJV | 1, SU = JJU | 1;  // odd record strides: conflict-free stores from 64 lanes
-  __shared__ int2 rd[MB];                    // (dx, dy): footprint origin relative to the tile
-  __shared__ cpx<R> ra[MB * SV];             // s * wv[e]
-  __shared__ R rb[MB * SU];                  // wu[e]
+  // Record windows [0, tap 0 .. tap J-1, 0 ...]: a row or column index clamped to [-1, J] selects a tap
+  // or a zero without a branch.  Odd strides keep the 64 lanes' stores conflict-free.
+  constexpr int SV = (JJV + 2) | 1, SU = (JJU + 2) | 1;
+  // footprint origin relative to the tile, packed as (dx + 512) | (dy + 512) << 16, stored so that the
+  // offsets lane l needs in steps 4q .. 4q + 3 (entries 4k + l / 16) are one 16-byte word:
+  // entry t at rd[(t & 3) * MQ + (t >> 2)]
+  __shared__ __attribute__((aligned(16))) int rd[MB];
+  __shared__ cpx<R> ra[MB * SV];  // [0, s * wv[0..Jv-1], 0 ...]
+  __shared__ R rb[MB * SU];       // [0, wu[0..Ju-1], 0 ...]
   using acc_t = typename Mfma<R>::acc_t;
+  const int lane = threadIdx.x, li = lane & 15, lt = lane >> 4;
+  for (int q = lane; q < MB * SV; q += 64) ra[q] = mk<R>(R(0), R(0));  // borders stay zero
+  for (int q = lane; q < MB * SU; q += 64) rb[q] = R(0);
   const uint4 sl = tl.slot[blockIdx.x];
   const int tile = (int)sl.x;
   const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
   const int tx0 = tu * T, ty0 = tv * T;
-  const int lane = threadIdx.x, li = lane & 15, lt = lane >> 4;
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
   acc_t cre = {R(0), R(0), R(0), R(0)}, cim = {R(0), R(0), R(0), R(0)};
   const uint32_t end = sl.z;
-  // entry data of the batch being prepared; the next batch's is loaded while this one's MFMAs run
+  // Entry data of the batch being prepared (coordinates, value, weight) and the list index of the next
+  // batch's entry: while a batch's MFMAs run, the next batch's data and the list index two batches
+  // ahead are in flight.
   VisCoord<R> c{};
-  cpx<R> s = mk<R>(R(0), R(0));
-  if (sl.y + lane < end) {
+  cpx<R> s = mk<R>(R(0), R(0)), w = mk<R>(R(1), R(0));
+  uint32_t inext = 0;
+  const bool act = lane < MB;  // lanes that prepare one entry of each batch
+  if (act && sl.y + lane < end) {
     const uint32_t i = tl.list[sl.y + lane];
     c = vis_coord(v, i, Ju, Jv);
     s = yin[i];  // visibilities in sorted order
-    if (v.weights) s = cmulc(s, v.weights[i]);  // conj(G) = phase * K_u * K_v * conj(weight)
+    if (v.weights) w = v.weights[i];
   }
+  if (act && sl.y + MB + lane < end) inext = tl.list[sl.y + MB + lane];
   for (uint32_t base = sl.y; base < end; base += MB) {
     const int nb = (int)min((uint32_t)MB, end - base);
     if (lane < nb) {
+      if (v.weights) s = cmulc(s, w);  // conj(G) = phase * K_u * K_v * conj(weight)
       s = scale(s, scl);
       R wu[JJU], wv[JJV];
-      taps_axis<R, JU>(g.ku, g.tab_u, c.ku, c.fu, Ju, wu);
-      taps_axis<R, JV>(kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.kv, c.fv, Jv, wv);
+      taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
       int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
       if (dx >= T) dx -= g.Nu;  // footprint starts left of / above the tile
       if (dy >= T) dy -= g.Nv;
-      rd[lane] = make_int2(dx, dy);
+      rd[(lane & 3) * MQ + (lane >> 2)] = (dx + 512) | ((dy + 512) << 16);
 #pragma unroll
       for (int e = 0; e < JJV; ++e) {
         if (JV == 0 && e >= Jv) break;
-        ra[lane * SV + e] = mk<R>(s.x * wv[e], s.y * wv[e]);
+        ra[lane * SV + 1 + e] = mk<R>(s.x * wv[e], s.y * wv[e]);
       }
 #pragma unroll
       for (int e = 0; e < JJU; ++e) {
         if (JU == 0 && e >= Ju) break;
-        rb[lane * SU + e] = wu[e];
+        rb[lane * SU + 1 + e] = wu[e];
       }
-    } else {
-      rd[lane] = make_int2(MFMA_NOTAP, MFMA_NOTAP);
+    } else if (act) {
+      rd[(lane & 3) * MQ + (lane >> 2)] = (MFMA_NOTAP + 512) | ((MFMA_NOTAP + 512) << 16);
     }
     __syncthreads();
-    const uint32_t nxt = base + MB + lane;
-    if (nxt < end) {
-      const uint32_t i = tl.list[nxt];
-      c = vis_coord(v, i, Ju, Jv);
-      s = yin[i];
-      if (v.weights) s = cmulc(s, v.weights[i]);
+    if (act && base + MB + lane < end) {
+      c = vis_coord(v, inext, Ju, Jv);
+      s = yin[inext];
+      if (v.weights) w = v.weights[inext];
     }
-    const int nk = (nb + 3) >> 2;
-#pragma unroll 4
-    for (int k = 0; k < nk; ++k) {
-      const int t = 4 * k + lt;  // this lane's entry in step k; li is its row of A and column of B
-      const int2 d = rd[t];
-      const int ev = li - d.y, eu = li - d.x;
-      const bool okv = (unsigned)ev < (unsigned)Jv, oku = (unsigned)eu < (unsigned)Ju;
-      const cpx<R> a = ra[t * SV + (okv ? ev : 0)];
-      const R b = rb[t * SU + (oku ? eu : 0)];
-      const R bb = oku ? b : R(0);
-      cre = Mfma<R>::step(okv ? a.x : R(0), bb, cre);
-      cim = Mfma<R>::step(okv ? a.y : R(0), bb, cim);
+    if (act && base + 2 * MB + lane < end) inext = tl.list[base + 2 * MB + lane];
+    // groups of four steps (16 entries; slots past nb hold no footprint): every LDS read of a group is
+    // issued before its MFMAs
+    const int ng = (nb + 15) >> 4;
+    for (int q = 0; q < ng; ++q) {
+      const int4 dq = *reinterpret_cast<const int4*>(&rd[lt * MQ + 4 * q]);
+      const int dd[4] = {dq.x, dq.y, dq.z, dq.w};
+      cpx<R> a[4];
+      R b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int t = 16 * q + 4 * r + lt;  // entry of step 4q + r; li is its row of A and column of B
+        const int ev = min(max(li + 512 - (dd[r] >> 16), -1), Jv);
+        const int eu = min(max(li + 512 - (dd[r] & 0xffff), -1), Ju);
+        a[r] = ra[t * SV + 1 + ev];
+        b[r] = rb[t * SU + 1 + eu];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cre = Mfma<R>::step(a[r].x, b[r], cre);
+        cim = Mfma<R>::step(a[r].y, b[r], cim);
+      }
     }
     __syncthreads();
   }
