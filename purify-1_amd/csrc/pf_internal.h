@@ -163,18 +163,32 @@ template <typename R>
 void permute_to_sorted(const VisData<R>& v, const cpx<R>* y, cpx<R>* ys, hipStream_t s);
 
 // ---- projection rows (w_term) -------------------------------------------------------------
+// Row m of the w-projection gridding matrix (operators.cc:49-99): Kv x Ku taps, K = Jw + J - 1, at
+// columns (kw_u + ju, kw_v + jv) mod (Nu, Nv), ju, jv = 1..K, kw = floor(u - K/2).  The values (the
+// projection kernel of projection_kernels.cc:59-85, the (-1)^(kw_u+ju+kw_v+jv) phase and the weight)
+// are built once per operator and stored per sorted visibility, row-major [jv][ju].
 template <typename R>
 struct WRows {
-  int Kv = 0, Ku = 0;              // (Jv+Jw-1) x (Ju+Jw-1) taps per row
-  const int2* k0 = nullptr;        // row start (kw_u, kw_v) = floor(u - Ku/2), floor(v - Kv/2)
-  const cpx<R>* vals = nullptr;    // M * Kv * Ku values, row-major [jv][ju], weight and phase folded
+  int Ku = 0, Kv = 0;
+  const int2* kw = nullptr;      // (kw_u, kw_v) per sorted visibility
+  const cpx<R>* vals = nullptr;  // M * Kv * Ku
 };
+// y = G grid: owner tiles staged with a (K - 1)-wide halo, one wave per visibility
 template <typename R>
-void degrid_wrows(const GridGeom<R>& g, const VisData<R>& v, const WRows<R>& w, const cpx<R>* grid, cpx<R>* y,
-                  R scale, hipStream_t s);
+void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const TileLists& t, const cpx<R>* grid,
+                  cpx<R>* y, R scale, hipStream_t s);
+// grid = G^H ys (ys in sorted order): one wave per tile, the taps of one visibility at a time
 template <typename R>
-void grid_wrows_tiles(const GridGeom<R>& g, const VisData<R>& v, const WRows<R>& w, const TileLists& t,
-                      const cpx<R>* y, cpx<R>* grid, R scale, hipStream_t s);
+void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx<R>* ys, cpx<R>* grid, R scale,
+                hipStream_t s);
+// Host: the projection row of one visibility, FP64, into out[2 * Kv * Ku] (interleaved complex),
+// phase and weight included.  kind / Jeval: the kernel in the u slot (the reference evaluates both
+// axes with it, projection_kernels.cc:72-73).
+struct WRowSpec {
+  int kind = 0, Jeval = 4, Ju = 4, Jv = 4, Jw = 6;
+  double os = 2, du_lambda = 1, dv_lambda = 1;  // pixel_to_lambda of each axis
+};
+void wrow_host(const WRowSpec& sp, double u, double v, double w, double wre, double wim, double* out);
 
 // ---- vector helpers -----------------------------------------------------------------------
 // sum |x|^2 in FP64 over n complex entries, deterministic two-level reduction into *out (device)

@@ -15,6 +15,7 @@
 #include <memory>
 #include <mutex>
 #include <random>
+#include <thread>
 
 #include "../../include/purify_amd.h"
 #include "pf_internal.h"
@@ -269,6 +270,9 @@ struct pf_op {
   Fft2dPlan fft;
   TileLists tl;   // TU x TV tiles: degridding, residue-class and atomic gridding
   TileLists tlm;  // 16 x 16 tiles: matrix-core gridding (empty when the grid is too small)
+  int Ku = 0, Kv = 0;         // footprint of one row of G (J, or Jw + J - 1 with w-projection)
+  void* d_wrows = nullptr;    // w-projection rows, M * Kv * Ku in the operator's precision
+  int64_t nnz_w = 0;
   int n_tiles = 0;
   int64_t tile_list_len = 0, tile_list_len_m = 0;
   // device buffers
@@ -439,6 +443,61 @@ void to_pixels(const pf_op_desc& d, int Nu, int Nv, std::vector<double>& u, std:
   }
 }
 
+// w-projection rows (operators.cc:49-99), FP64 on the host threads, in sorted order, streamed to the
+// device in chunks in the operator's precision.  G receives w - mean(w) (operators.h:323) and both axes
+// of the projection use the u-slot kernel, which is the v-support kernel after the slot swap
+// (operators.h:324, projection_kernels.cc:72-73).
+void build_wrows(pf_op* op, const pf_op_desc& d, const std::vector<double>& u, const std::vector<double>& v,
+                 const std::vector<int32_t>& perm, const std::vector<int64_t>& ku, const std::vector<int64_t>& kv) {
+  const int64_t M = d.M;
+  const int KK = op->Ku * op->Kv;
+  op->nnz_w = M * KK;
+  double mean = 0;
+  for (int64_t m = 0; m < M; ++m) mean += d.w[m];
+  if (M > 0) mean /= (double)M;
+  WRowSpec sp;
+  sp.kind = d.kernel;
+  sp.Jeval = d.Jv;
+  sp.Ju = op->Ju;
+  sp.Jv = op->Jv;
+  sp.Jw = d.Jw;
+  sp.os = op->os;
+  // pixel_to_lambda (projection_kernels.cc:28-30)
+  sp.du_lambda = 60. * 60. * 180. / d.cellx / std::floor(op->os * op->nx) / PI;
+  sp.dv_lambda = 60. * 60. * 180. / d.celly / std::floor(op->os * op->ny) / PI;
+  const size_t esz = op->esz();
+  op->d_wrows = op->dalloc(esz * (size_t)std::max<int64_t>(op->nnz_w, 1));
+  std::vector<int2> kw(M);
+  for (int64_t i = 0; i < M; ++i) kw[i] = make_int2((int)ku[perm[i]], (int)kv[perm[i]]);
+  op->d_kk = (int2*)op->dalloc(sizeof(int2) * std::max<int64_t>(M, 1));
+  if (M) PF_HIP(hipMemcpy(op->d_kk, kw.data(), sizeof(int2) * M, hipMemcpyHostToDevice));
+  const int64_t CH = 1 << 15;  // visibilities per chunk
+  std::vector<double> tmp((size_t)std::min<int64_t>(M, CH) * KK * 2 + 2);
+  std::vector<float> tmpf(op->prec == PF_C64 ? tmp.size() : 0);
+  const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  for (int64_t c0 = 0; c0 < M; c0 += CH) {
+    const int64_t n = std::min(CH, M - c0);
+    auto work = [&](int t) {
+      for (int64_t k = t; k < n; k += nth) {
+        const int64_t m = perm[c0 + k];
+        const double wre = d.weights ? d.weights[2 * m] : 1.0, wim = d.weights ? d.weights[2 * m + 1] : 0.0;
+        wrow_host(sp, u[m], v[m], d.w[m] - mean, wre, wim, tmp.data() + (size_t)k * KK * 2);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    char* dst = static_cast<char*>(op->d_wrows) + (size_t)c0 * KK * esz;
+    if (op->prec == PF_C64) {
+      for (size_t q = 0; q < (size_t)n * KK * 2; ++q) tmpf[q] = (float)tmp[q];
+      PF_HIP(hipMemcpy(dst, tmpf.data(), (size_t)n * KK * esz, hipMemcpyHostToDevice));
+    } else {
+      PF_HIP(hipMemcpy(dst, tmp.data(), (size_t)n * KK * esz, hipMemcpyHostToDevice));
+    }
+  }
+}
+
 void build(pf_op* op, const pf_op_desc& d) {
   if (d.imsizex <= 0 || d.imsizey <= 0) throw Error(PF_ERR_ARG, "image size must be positive");
   if (d.M < 0 || (d.M > 0 && (!d.u || !d.v))) throw Error(PF_ERR_ARG, "u and v must be given for M > 0");
@@ -448,7 +507,10 @@ void build(pf_op* op, const pf_op_desc& d) {
     throw Error(PF_ERR_ARG, "Incorrect input: PSWF requires a support of 6");  // kernels.cc:241-244
   if (d.Ju < 1 || d.Jv < 1 || d.Ju > 16 || d.Jv > 16) throw Error(PF_ERR_UNSUPPORTED, "kernel support must be 1..16");
   if (d.precision != PF_C64 && d.precision != PF_C128) throw Error(PF_ERR_ARG, "precision must be PF_C64 or PF_C128");
-  if (d.w_term) throw Error(PF_ERR_UNSUPPORTED, "w_term operators are built by pf_op_create_w");
+  if (d.w_term) {
+    if (d.M > 0 && !d.w) throw Error(PF_ERR_ARG, "w must be given for w_term operators");
+    if (d.Jw < 1 || d.Jw > 16) throw Error(PF_ERR_UNSUPPORTED, "w-kernel support must be 1..16");
+  }
   op->prec = d.precision;
   op->nx = d.imsizex;
   op->ny = d.imsizey;
@@ -464,6 +526,12 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->Ju = std::min(d.Ju, op->Nu);  // operators.cc:26-27
   op->Jv = std::min(d.Jv, op->Nv);
   op->Jw = d.Jw;
+  op->w_term = d.w_term ? 1 : 0;
+  // footprint of one row of G: J x J taps, or (Jw + J - 1)^2 with w-projection (operators.cc:73-74)
+  op->Ku = op->w_term ? d.Jw + op->Ju - 1 : op->Ju;
+  op->Kv = op->w_term ? d.Jw + op->Jv - 1 : op->Jv;
+  if (op->w_term && (op->Ku > op->Nu || op->Kv > op->Nv))
+    throw Error(PF_ERR_UNSUPPORTED, "w-projection rows wider than the grid");
   op->cellx = d.cellx;
   op->celly = d.celly;
   op->M = d.M;
@@ -488,7 +556,7 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->gf.kv = make_kparams<float>(d.kernel, d.Ju, op->os);
   op->gd.ku = make_kparams<double>(d.kernel, d.Jv, op->os);
   op->gd.kv = make_kparams<double>(d.kernel, d.Ju, op->os);
-  if (op->prec == PF_C64 && op->Ju <= 8 && op->Jv <= 8) {
+  if (op->prec == PF_C64 && op->Ju <= 8 && op->Jv <= 8 && !op->w_term) {
     const std::vector<float> tu = make_tap_table(d.kernel, d.Jv, op->Ju, op->os);
     const std::vector<float> tv = make_tap_table(d.kernel, d.Ju, op->Jv, op->os);
     op->d_tab_u = (float*)op->dalloc(sizeof(float) * tu.size());
@@ -526,8 +594,8 @@ void build(pf_op* op, const pf_op_desc& d) {
   std::vector<uint32_t> count(nkeys + 1, 0);
   for (int64_t m = 0; m < M; ++m) {
     if (!std::isfinite(u[m]) || !std::isfinite(v[m])) throw Error(PF_ERR_ARG, "non-finite u/v coordinate");
-    ku[m] = (int64_t)std::floor(u[m] - op->Ju * 0.5);  // operators.cc:34-35
-    kv[m] = (int64_t)std::floor(v[m] - op->Jv * 0.5);
+    ku[m] = (int64_t)std::floor(u[m] - op->Ku * 0.5);  // operators.cc:34-35 (87-88 with w-projection)
+    kv[m] = (int64_t)std::floor(v[m] - op->Kv * 0.5);
     const int q0 = wrapi(ku[m] + 1, op->Nu), p0 = wrapi(kv[m] + 1, op->Nv);
     const uint32_t tile = (uint32_t)((p0 / TV) * ntu + q0 / TU);
     key[m] = tile * (uint32_t)(TU * TV) + (uint32_t)((p0 % TV) * TU + q0 % TU);
@@ -541,7 +609,7 @@ void build(pf_op* op, const pf_op_desc& d) {
   std::vector<uint32_t>().swap(count);
   std::vector<uint32_t>().swap(key);
   // ---- per-tile gridding lists (every visibility with a tap in the tile) ----
-  const HostLists lists = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, TU, TV);
+  const HostLists lists = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ku, op->Kv, TU, TV);
   op->tile_list_len = (int64_t)lists.list.size();
 
   // ---- upload ----
@@ -566,13 +634,17 @@ void build(pf_op* op, const pf_op_desc& d) {
     op->tl.slot_own = d;
     op->tl.n_own = (int)so.size();
   }
-  if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv)) {
+  if (op->w_term) {
+    build_wrows(op, d, u, v, perm, ku, kv);
+  } else if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv)) {
     const HostLists lm = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE);
     upload_lists(op, lm, MFMA_TILE, MFMA_TILE, op->tlm);
     op->tile_list_len_m = (int64_t)lm.list.size();
     op->grid_variant = GRID_MFMA;
   }
-  if (op->prec == PF_C64) {
+  if (op->w_term) {
+    // row origins only: the values hold the kernel, the phase and the weight
+  } else if (op->prec == PF_C64) {
     std::vector<int2> kk(M);
     std::vector<cpx<float>> ff(M);
     for (int64_t i = 0; i < M; ++i) {
@@ -692,20 +764,43 @@ void fz_adjoint(pf_op* op, cpx<R>* grid_in, cpx<R>* x, double scl, hipStream_t s
 }
 
 template <typename R>
+WRows<R> wrows(const pf_op* op) {
+  WRows<R> w;
+  w.Ku = op->Ku;
+  w.Kv = op->Kv;
+  w.kw = op->d_kk;
+  w.vals = static_cast<const cpx<R>*>(op->d_wrows);
+  return w;
+}
+
+// y = G grid (the gridding matrix without the norm): Kaiser-Bessel taps or stored projection rows
+template <typename R>
+void G_dev(pf_op* op, const cpx<R>* grid, cpx<R>* y, hipStream_t s) {
+  if (op->w_term) degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->d_perm, op->tl, grid, y, R(1), s);
+  else degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, grid, y, R(1), s);
+}
+// grid = G^H y, y in the caller's order
+template <typename R>
+void GH_dev(pf_op* op, const cpx<R>* y, cpx<R>* grid, hipStream_t s) {
+  cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
+  permute_to_sorted<R>(visdata<R>(op), y, ys, s);
+  if (op->w_term) grid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, ys, grid, R(1), s);
+  else grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, op->tlm, ys, grid, R(1), s, op->grid_variant);
+}
+
+template <typename R>
 void forward_dev(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s) {
   const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
   cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
   fz_forward<R>(op, x, B, scl, s);
-  degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, B, y, R(1), s);
+  G_dev<R>(op, B, y, s);
 }
 
 template <typename R>
 void adjoint_dev(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
   const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
   cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
-  cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
-  permute_to_sorted<R>(visdata<R>(op), y, ys, s);
-  grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, op->tlm, ys, A, R(1), s, op->grid_variant);
+  GH_dev<R>(op, y, A, s);
   fz_adjoint<R>(op, A, x, scl, s);
 }
 
@@ -917,6 +1012,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
 int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
   if (!op) return fail(PF_ERR_ARG, "null operator");
   return guarded([&] {
+    if (op->w_term) {  // one gridding kernel for projection rows: nothing to choose
+      if (chosen) *chosen = op->grid_variant;
+      return;
+    }
     ensure_stage(op);
     hipStream_t s = nullptr;
     PF_HIP(hipMemsetAsync(op->d_vecy, 0, op->esz() * (size_t)std::max<int64_t>(op->M, 1), s));
@@ -1020,7 +1119,7 @@ int pf_op_get_info(const pf_op* op, pf_op_info* info) {
   info->precision = op->prec;
   info->w_term = op->w_term;
   info->M = op->M;
-  info->nnz_w = 0;
+  info->nnz_w = op->nnz_w;
   info->tile_u = op->tl.TU;
   info->tile_v = op->tl.TV;
   info->n_tiles = op->n_tiles;
@@ -1054,27 +1153,16 @@ int pf_op_apply_FZ_adjoint(pf_op* op, const void* grid_dev, void* x_dev, pf_stre
 int pf_op_apply_G(pf_op* op, const void* grid_dev, void* y_dev, pf_stream stream) {
   return guarded([&] {
     ProfInstall pi(op);
-    if (op->prec == PF_C64)
-      degrid<float>(degrid_geom<float>(op), visdata<float>(op), op->tl, (const cpx<float>*)grid_dev, (cpx<float>*)y_dev, 1.0f, (hipStream_t)stream);
-    else
-      degrid<double>(degrid_geom<double>(op), visdata<double>(op), op->tl, (const cpx<double>*)grid_dev, (cpx<double>*)y_dev, 1.0,
-                     (hipStream_t)stream);
+    if (op->prec == PF_C64) G_dev<float>(op, (const cpx<float>*)grid_dev, (cpx<float>*)y_dev, (hipStream_t)stream);
+    else G_dev<double>(op, (const cpx<double>*)grid_dev, (cpx<double>*)y_dev, (hipStream_t)stream);
   });
 }
 
 int pf_op_apply_GH(pf_op* op, const void* y_dev, void* grid_dev, pf_stream stream) {
   return guarded([&] {
     ProfInstall pi(op);
-    hipStream_t s = (hipStream_t)stream;
-    if (op->prec == PF_C64) {
-      permute_to_sorted<float>(visdata<float>(op), (const cpx<float>*)y_dev, (cpx<float>*)op->d_ysort, s);
-      grid_tiles<float>(op->gf, visdata<float>(op), op->tl, op->tlm, (const cpx<float>*)op->d_ysort, (cpx<float>*)grid_dev,
-                        1.0f, s, op->grid_variant);
-    } else {
-      permute_to_sorted<double>(visdata<double>(op), (const cpx<double>*)y_dev, (cpx<double>*)op->d_ysort, s);
-      grid_tiles<double>(op->gd, visdata<double>(op), op->tl, op->tlm, (const cpx<double>*)op->d_ysort,
-                         (cpx<double>*)grid_dev, 1.0, s, op->grid_variant);
-    }
+    if (op->prec == PF_C64) GH_dev<float>(op, (const cpx<float>*)y_dev, (cpx<float>*)grid_dev, (hipStream_t)stream);
+    else GH_dev<double>(op, (const cpx<double>*)y_dev, (cpx<double>*)grid_dev, (hipStream_t)stream);
   });
 }
 
