@@ -31,22 +31,31 @@ WORKLOADS = {
     "north_star": (2048, 6, 4_000_000, 1002),
     "c2": (1024, 6, 2_000_000, 1001),
     "c1": (256, 4, 10_000, 1000),
+    # w-projection rows (w_term): w ~ U[0, 200] wavelengths, 1 degree field of view, Jw = 6
+    "c3_wproj": (2048, 6, 4_000_000, 1003),
 }
+WPROJ = {"c3_wproj": dict(Jw=6, wmax=200.0, fov_deg=1.0)}
 
 METRIC = "Φ+Φ† vis/s FP32 vs FP64 (rel.err≤1e-5, frac HBM peak); served p50/p99 latency"
 
 
-def model_bytes_pair(n, Ng, M, e, r):
-    """Stage-boundary compulsory traffic of one Phi + Phi^dagger pair."""
+def model_bytes_pair(n, Ng, M, e, r, KK=None):
+    """Stage-boundary compulsory traffic of one Phi + Phi^dagger pair.  With stored projection rows of KK
+    taps (w_term) each direction reads per visibility its row origin (8 B), its value and its row."""
+    if KK:
+        return 3 * e * n * n + 11 * e * Ng + 2 * (8 + e + KK * e) * M
     return 3 * e * n * n + 11 * e * Ng + (4 * r + 4 * e) * M
 
 
-def kernel_alg_bytes(name, n, Nu, Nv, M, e, r):
+def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, KK=None):
     """Algorithmic bytes of one launch of each kernel: its inputs read once and outputs written once."""
     Ng = Nu * Nv
     win = n * Nu  # grid rows that hold the image (forward) or are kept by the crop (adjoint)
     vis = M * (2 * r + 2 * e)  # u, v, weight and value per visibility
+    wvis = M * (8 + e + (KK or 0) * e)  # row origin, value and stored projection row
     table = {
+        "degrid_wrows": Ng * e + wvis,
+        "grid_wrows": wvis + Ng * e,
         "fft_rows_fwd_sz": n * n * e + win * e,
         "fft_rows_adj_crop": win * e + n * n * e,
         "degrid": Ng * e + vis,
@@ -121,7 +130,13 @@ def main():
     v = utilities.radians_to_pixels(v_rad, n, os_)
     x_h = utilities.synth_image(seed, n, n, 20)
     y_h = utilities.synth_cnormal(seed + 1, M)
-    log(f"[bench] workload {args.workload}: n={n} grid={Nu} J={J} M={M}")
+    wp = WPROJ.get(args.workload)
+    w_term, Jw, cell, w = False, 6, 1.0, None
+    if wp:
+        w_term, Jw = True, wp["Jw"]
+        cell = wp["fov_deg"] * 3600.0 / n  # arcsec per pixel
+        w = np.random.default_rng(seed + 2).uniform(0.0, wp["wmax"], M)
+    log(f"[bench] workload {args.workload}: n={n} grid={Nu} J={J} M={M} w_term={w_term}")
 
     if args.counters:
         counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev)
@@ -131,8 +146,8 @@ def main():
     outs = {}
     for prec, steps in (("c64", args.steps), ("c128", args.fp64_steps)):
         t0 = time.perf_counter()
-        lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, os_, 0, 1e-4, "kb", J, J,
-                                                         precision=prec)
+        lt = measurementoperator.init_degrid_operator_2d(u, v, w, None, n, n, os_, 0, 1e-4, "kb", J, J, w_term, Jw,
+                                                         cell, cell, precision=prec)
         setup = time.perf_counter() - t0
         op = lt.op
         t0 = time.perf_counter()
@@ -189,7 +204,8 @@ def main():
     tab = r64["table"]
     dom = max(tab, key=lambda k: tab[k][0])
     dom_ms = tab[dom][0] / tab[dom][1]
-    b = kernel_alg_bytes(dom, n, Nu, Nv, M, e, r)
+    KK = (J + Jw - 1) ** 2 if w_term else None
+    b = kernel_alg_bytes(dom, n, Nu, Nv, M, e, r, KK)
     if b is None:  # step A: forward reads the image window, adjoint the full grid
         b = ((n * Nu * e + Ng * e) + (2 * Ng * e)) / 2
     achieved = b / (dom_ms * 1e-3)
@@ -203,7 +219,7 @@ def main():
         roofline["traffic"] = tr.get("bytes_per_launch", {}).get(dom)
         roofline["traffic_source"] = os.path.relpath(traffic_file, ROOT)
 
-    pair_bytes = model_bytes_pair(n, Ng, M, e, r)
+    pair_bytes = model_bytes_pair(n, Ng, M, e, r, KK)
     rel_fp32_fp64 = {
         "forward": float(np.linalg.norm(outs["c64"][0] - outs["c128"][0]) / np.linalg.norm(outs["c128"][0])),
         "adjoint": float(np.linalg.norm(outs["c64"][1] - outs["c128"][1]) / np.linalg.norm(outs["c128"][1])),
@@ -211,7 +227,7 @@ def main():
 
     cpu = None
     acc_oracle = None
-    if not args.no_cpu:
+    if not args.no_cpu and not w_term:
         import oracle
         threads = oracle.omp_threads()
         log(f"[bench] cpu baseline: FP64 oracle, explicit CSR G/G^H + OpenMP SpMV, {threads} threads")
@@ -252,14 +268,15 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": f"phi_pair_{args.workload}", "image": n, "grid": Nu, "oversample": os_, "kernel": "kb",
-                   "J": J, "M": M, "w_term": False, "uv_sigma_rad": round(np.pi / 3, 6), "seed": seed},
+                   "J": J, "M": M, "w_term": w_term, "uv_sigma_rad": round(np.pi / 3, 6), "seed": seed,
+                   **({"Jw": Jw, "w_max_lambda": wp["wmax"], "cell_arcsec": cell} if wp else {})},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pair_frac_hbm_peak": pair_bytes / (r64["ms"] * 1e-3) / HBM_PEAK,
         "pair_model_bytes": pair_bytes,
         "fp64": {"value": res["c128"]["value"], "ms_per_step": res["c128"]["ms"],
                  "fp32_speedup": res["c128"]["ms"] / r64["ms"],
-                 "pair_frac_hbm_peak": model_bytes_pair(n, Ng, M, 16, 8) / (res["c128"]["ms"] * 1e-3) / HBM_PEAK},
+                 "pair_frac_hbm_peak": model_bytes_pair(n, Ng, M, 16, 8, KK) / (res["c128"]["ms"] * 1e-3) / HBM_PEAK},
         "rel_err_fp32_vs_fp64_gpu": rel_fp32_fp64,
         "rel_err_vs_oracle": acc_oracle,
         "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
