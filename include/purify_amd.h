@@ -156,6 +156,35 @@ int pf_synth_image(uint64_t seed, int32_t ny, int32_t nx, int32_t nsrc, double *
 /* Complex standard normal vector (re, im each N(0, 1/2) * scale). */
 int pf_synth_cnormal(uint64_t seed, int64_t n, double scale, double *out_complex128);
 
+/* ---- sparse gridding matrices and chirp-based w-projection ---- */
+/* A CSR matrix of complex128 values: row r holds entries rowptr[r] .. rowptr[r + 1] - 1, columns
+ * ascending.  For a gridding matrix, rows are visibilities and columns grid cells p * Nu + q. */
+typedef struct pf_csr {
+  int64_t rows, cols, nnz;
+  int64_t *rowptr; /* rows + 1 */
+  int32_t *colidx; /* nnz */
+  double *vals;    /* nnz complex128 (re, im) */
+} pf_csr;
+
+/* wproj_utilities::wprojection_matrix (cpp/purify/wproj_utilities.cc:132-188): every row m of G is
+ * convolved (row_wise_sparse_convolution, wproj_utilities.h:222-287) with the chirp of w[m]
+ * (create_chirp_row / generate_chirp, wproj_utilities.cc:51-130: FFT of the chirp image times sqrt(N),
+ * hard-thresholded to keep energy_chirp of its l2 energy), and the result is hard-thresholded to keep
+ * energy_wproj (sparsify_row_thres, wproj_utilities.h:101-145).  Runs on the GPU in FP64.  The grid
+ * must be square (x_size == y_size), as the reference's chirp generation assumes
+ * (wproj_utilities.cc:67-71).  The library allocates GW's arrays; release them with pf_csr_free. */
+int pf_wprojection_matrix(const pf_csr *G, int32_t x_size, int32_t y_size, const double *w, double cell_x,
+                          double cell_y, double energy_chirp, double energy_wproj, pf_csr *GW);
+void pf_csr_free(pf_csr *m);
+
+/* An operator Phi = G F Z S / norm whose gridding matrix G is given (e.g. the output of
+ * pf_wprojection_matrix): G->rows visibilities, G->cols = Nv * Nu.  The descriptor supplies the image
+ * size, oversampling, kernel (for the correction S), precision and power-method settings; its u, v, w
+ * and weights are not used.  G is applied as a row gather and G^H as a gather over the transposed
+ * matrix: both deterministic.  No counterpart entry point in the reference, whose operators build G
+ * themselves (operators.h:123-145); this one carries a precomputed G into the same operator. */
+int pf_op_create_csr(const pf_op_desc *desc, const pf_csr *G, pf_op **out);
+
 /* ---- errors ---- */
 const char *pf_last_error(void);
 /* Library version string. */

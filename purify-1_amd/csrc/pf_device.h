@@ -50,6 +50,17 @@ __device__ inline cpx<R> scale(cpx<R> a, R s) {
   return mk(a.x * s, a.y * s);
 }
 
+// Sum over the 64 lanes of a wave in a fixed butterfly order; every lane ends with the total.
+template <typename R>
+__device__ inline cpx<R> wave_sum(cpx<R> a) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    a.x += __shfl_xor(a.x, m, 64);
+    a.y += __shfl_xor(a.y, m, 64);
+  }
+  return a;
+}
+
 // ------------------------------------------------------------------------------------------
 // Gridding kernels (kernels.cc).  Kinds follow kernels::kernel (kernels.h:15).
 // ------------------------------------------------------------------------------------------

@@ -190,6 +190,23 @@ struct WRowSpec {
 };
 void wrow_host(const WRowSpec& sp, double u, double v, double w, double wre, double wim, double* out);
 
+// ---- sparse gridding matrices (pf_wmat.hip) -------------------------------------------------
+// Host-side CSR (complex128 values), the layout of pf_csr.
+struct HostCsr {
+  int64_t rows = 0, cols = 0;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> col;
+  std::vector<double> val;  // interleaved complex
+};
+// wproj_utilities::wprojection_matrix on the GPU in FP64 (see pf_wprojection_matrix).
+void wprojection_matrix_dev(int64_t rows, const int64_t* rowptr, const int32_t* col, const double* val, int x_size,
+                            int y_size, const double* w, double cell_x, double cell_y, double e_chirp, double e_wproj,
+                            HostCsr& out);
+// y[r] = sum_k val[k] x[col[k]] over the entries of row r (one wave per row, fixed-order sums)
+template <typename R>
+void csr_gather(int64_t rows, const int64_t* rowptr, const int32_t* col, const cpx<R>* val, const cpx<R>* x, cpx<R>* y,
+                R scale, hipStream_t s, const char* name);
+
 // ---- vector helpers -----------------------------------------------------------------------
 // sum |x|^2 in FP64 over n complex entries, deterministic two-level reduction into *out (device)
 template <typename R>

@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -273,6 +274,14 @@ struct pf_op {
   int Ku = 0, Kv = 0;         // footprint of one row of G (J, or Jw + J - 1 with w-projection)
   void* d_wrows = nullptr;    // w-projection rows, M * Kv * Ku in the operator's precision
   int64_t nnz_w = 0;
+  // a given sparse gridding matrix (pf_op_create_csr): G by rows and conj(G) transposed, by grid cell
+  bool g_csr = false;
+  int64_t* d_csr_ptr = nullptr;
+  int32_t* d_csr_col = nullptr;
+  void* d_csr_val = nullptr;
+  int64_t* d_csrT_ptr = nullptr;
+  int32_t* d_csrT_col = nullptr;
+  void* d_csrT_val = nullptr;
   int n_tiles = 0;
   int64_t tile_list_len = 0, tile_list_len_m = 0;
   // device buffers
@@ -498,6 +507,8 @@ void build_wrows(pf_op* op, const pf_op_desc& d, const std::vector<double>& u, c
   }
 }
 
+void build_tail(pf_op* op, const pf_op_desc& d);
+
 void build(pf_op* op, const pf_op_desc& d) {
   if (d.imsizex <= 0 || d.imsizey <= 0) throw Error(PF_ERR_ARG, "image size must be positive");
   if (d.M < 0 || (d.M > 0 && (!d.u || !d.v))) throw Error(PF_ERR_ARG, "u and v must be given for M > 0");
@@ -676,6 +687,11 @@ void build(pf_op* op, const pf_op_desc& d) {
     }
   }
 
+  build_tail(op, d);
+}
+
+// The separable correction and the grid workspaces, shared by every way of building G.
+void build_tail(pf_op* op, const pf_op_desc& d) {
   // ---- correction S(j, i) = sy[j] * sx[i] (operators.cc:101-121) ----
   std::vector<double> sx(op->nx), sy(op->ny);
   for (int i = 0; i < op->nx; ++i)
@@ -699,6 +715,90 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->d_gridA = op->dalloc(op->esz() * op->ngrid());
   op->d_gridB = op->dalloc(op->esz() * op->ngrid());
   op->d_red = (double*)op->dalloc(sizeof(double) * 1040);
+}
+
+// An operator on a given sparse gridding matrix G (rows = visibilities in the caller's order, columns
+// = grid cells p * Nu + q).  G is stored by rows in the operator's precision; the adjoint uses conj(G)
+// transposed, built by a stable counting sort on the column, so every grid cell sums its entries in
+// row order: deterministic.
+void build_csr(pf_op* op, const pf_op_desc& d, const pf_csr& G) {
+  if (d.imsizex <= 0 || d.imsizey <= 0) throw Error(PF_ERR_ARG, "image size must be positive");
+  if (d.kernel < 0 || d.kernel > 5) throw Error(PF_ERR_ARG, "Did not choose valid kernel.");
+  if (d.kernel == PF_PSWF && (d.Ju != 6 || d.Jv != 6))
+    throw Error(PF_ERR_ARG, "Incorrect input: PSWF requires a support of 6");  // kernels.cc:241-244
+  if (d.Ju < 1 || d.Jv < 1 || d.Ju > 16 || d.Jv > 16) throw Error(PF_ERR_UNSUPPORTED, "kernel support must be 1..16");
+  if (d.precision != PF_C64 && d.precision != PF_C128) throw Error(PF_ERR_ARG, "precision must be PF_C64 or PF_C128");
+  op->prec = d.precision;
+  op->nx = d.imsizex;
+  op->ny = d.imsizey;
+  op->os = d.oversample_ratio;
+  op->Nu = (int)std::floor(d.imsizex * d.oversample_ratio);
+  op->Nv = (int)std::floor(d.imsizey * d.oversample_ratio);
+  if (!is_pow2(op->Nu) || !is_pow2(op->Nv))
+    throw Error(PF_ERR_UNSUPPORTED, "oversampled grid dimensions must be powers of two (got " + std::to_string(op->Nv) +
+                                        " x " + std::to_string(op->Nu) + ")");
+  op->x0 = (int)std::floor(op->Nu * 0.5 - op->nx * 0.5);
+  op->y0 = (int)std::floor(op->Nv * 0.5 - op->ny * 0.5);
+  op->kind = d.kernel;
+  op->Ju = d.Ju;
+  op->Jv = d.Jv;
+  op->Ku = op->Ju;
+  op->Kv = op->Jv;
+  op->cellx = d.cellx;
+  op->celly = d.celly;
+  const int64_t Ng = op->ngrid();
+  if (G.rows < 0 || G.cols != Ng) throw Error(PF_ERR_ARG, "G must have Nv * Nu columns");
+  if (G.rows > 0x3fffffffLL) throw Error(PF_ERR_UNSUPPORTED, "more than 2^30 visibilities in one operator");
+  if (!G.rowptr || (G.rows > 0 && G.rowptr[0] != 0)) throw Error(PF_ERR_ARG, "G row pointers must start at 0");
+  for (int64_t r = 0; r < G.rows; ++r)
+    if (G.rowptr[r + 1] < G.rowptr[r]) throw Error(PF_ERR_ARG, "G row pointers must not decrease");
+  const int64_t nnz = G.rows > 0 ? G.rowptr[G.rows] : 0;
+  for (int64_t k = 0; k < nnz; ++k)
+    if (G.colidx[k] < 0 || G.colidx[k] >= Ng) throw Error(PF_ERR_ARG, "G column index outside the grid");
+  op->M = G.rows;
+  op->nnz_w = nnz;
+  op->g_csr = true;
+  fft_plan_init(op->fft, op->Nv, op->Nu, op->prec);
+  // transpose: cells in order, each cell's entries in row order
+  std::vector<int64_t> tp((size_t)Ng + 1, 0);
+  for (int64_t k = 0; k < nnz; ++k) tp[(size_t)G.colidx[k] + 1]++;
+  for (int64_t c = 0; c < Ng; ++c) tp[c + 1] += tp[c];
+  std::vector<int32_t> trow((size_t)std::max<int64_t>(nnz, 1));
+  std::vector<double> tval((size_t)std::max<int64_t>(nnz, 1) * 2);
+  {
+    std::vector<int64_t> fill(tp.begin(), tp.end() - 1);
+    for (int64_t r = 0; r < G.rows; ++r)
+      for (int64_t k = G.rowptr[r]; k < G.rowptr[r + 1]; ++k) {
+        const int64_t q = fill[G.colidx[k]]++;
+        trow[q] = (int32_t)r;
+        tval[2 * q] = G.vals[2 * k];
+        tval[2 * q + 1] = -G.vals[2 * k + 1];  // conj(G)
+      }
+  }
+  auto upload_vals = [&](const double* v) {
+    const size_t n = (size_t)std::max<int64_t>(nnz, 1);
+    void* dp = op->dalloc(op->esz() * n);
+    if (nnz == 0) return dp;
+    if (op->prec == PF_C64) {
+      std::vector<float> f((size_t)nnz * 2);
+      for (size_t q = 0; q < f.size(); ++q) f[q] = (float)v[q];
+      PF_HIP(hipMemcpy(dp, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+    } else {
+      PF_HIP(hipMemcpy(dp, v, sizeof(double) * 2 * nnz, hipMemcpyHostToDevice));
+    }
+    return dp;
+  };
+  op->d_csr_ptr = (int64_t*)op->dalloc(sizeof(int64_t) * (G.rows + 1));
+  PF_HIP(hipMemcpy(op->d_csr_ptr, G.rowptr, sizeof(int64_t) * (G.rows + 1), hipMemcpyHostToDevice));
+  op->d_csr_col = (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
+  if (nnz) PF_HIP(hipMemcpy(op->d_csr_col, G.colidx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  op->d_csr_val = upload_vals(G.vals);
+  op->d_csrT_ptr = (int64_t*)op->dalloc(sizeof(int64_t) * (Ng + 1));
+  PF_HIP(hipMemcpy(op->d_csrT_ptr, tp.data(), sizeof(int64_t) * (Ng + 1), hipMemcpyHostToDevice));
+  op->d_csrT_col = (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
+  if (nnz) PF_HIP(hipMemcpy(op->d_csrT_col, trow.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  op->d_csrT_val = upload_vals(tval.data());
+  build_tail(op, d);
 }
 
 template <typename R>
@@ -776,12 +876,20 @@ WRows<R> wrows(const pf_op* op) {
 // y = G grid (the gridding matrix without the norm): Kaiser-Bessel taps or stored projection rows
 template <typename R>
 void G_dev(pf_op* op, const cpx<R>* grid, cpx<R>* y, hipStream_t s) {
-  if (op->w_term) degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->d_perm, op->tl, grid, y, R(1), s);
+  if (op->g_csr)
+    csr_gather<R>(op->M, op->d_csr_ptr, op->d_csr_col, static_cast<const cpx<R>*>(op->d_csr_val), grid, y, R(1), s,
+                  "degrid_csr");
+  else if (op->w_term) degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->d_perm, op->tl, grid, y, R(1), s);
   else degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, grid, y, R(1), s);
 }
 // grid = G^H y, y in the caller's order
 template <typename R>
 void GH_dev(pf_op* op, const cpx<R>* y, cpx<R>* grid, hipStream_t s) {
+  if (op->g_csr) {
+    csr_gather<R>(op->ngrid(), op->d_csrT_ptr, op->d_csrT_col, static_cast<const cpx<R>*>(op->d_csrT_val), y, grid,
+                  R(1), s, "grid_csr");
+    return;
+  }
   cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
   permute_to_sorted<R>(visdata<R>(op), y, ys, s);
   if (op->w_term) grid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, ys, grid, R(1), s);
@@ -946,6 +1054,60 @@ int pf_op_create(const pf_op_desc* desc, pf_op** out) {
   return rc;
 }
 
+int pf_op_create_csr(const pf_op_desc* desc, const pf_csr* G, pf_op** out) {
+  if (!desc || !G || !out) return fail(PF_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<pf_op> op(new pf_op());
+  int rc = guarded([&] {
+    build_csr(op.get(), *desc, *G);
+    if (desc->power_iters > 0) {
+      const double nrm = op->prec == PF_C64 ? power_method_t<float>(op.get(), desc->power_iters, desc->power_tol, nullptr)
+                                            : power_method_t<double>(op.get(), desc->power_iters, desc->power_tol, nullptr);
+      if (!(nrm > 0)) throw Error(PF_ERR_ARG, "Operator norm is not greater than zero.");  // operators.h:187
+      op->norm = nrm;
+    }
+  });
+  if (rc == PF_OK) *out = op.release();
+  return rc;
+}
+
+int pf_wprojection_matrix(const pf_csr* G, int32_t x_size, int32_t y_size, const double* w, double cell_x,
+                          double cell_y, double energy_chirp, double energy_wproj, pf_csr* GW) {
+  if (!G || !GW || (G->rows > 0 && (!w || !G->rowptr))) return fail(PF_ERR_ARG, "null argument");
+  *GW = pf_csr{0, 0, 0, nullptr, nullptr, nullptr};
+  return guarded([&] {
+    HostCsr h;
+    wprojection_matrix_dev(G->rows, G->rowptr, G->colidx, G->vals, x_size, y_size, w, cell_x, cell_y, energy_chirp,
+                           energy_wproj, h);
+    const int64_t nnz = h.rowptr.empty() ? 0 : h.rowptr.back();
+    pf_csr r{h.rows, h.cols, nnz, nullptr, nullptr, nullptr};
+    r.rowptr = (int64_t*)std::malloc(sizeof(int64_t) * (size_t)(h.rows + 1));
+    r.colidx = (int32_t*)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1));
+    r.vals = (double*)std::malloc(sizeof(double) * 2 * (size_t)std::max<int64_t>(nnz, 1));
+    if (!r.rowptr || !r.colidx || !r.vals) {
+      pf_csr_free(&r);
+      throw std::bad_alloc();
+    }
+    std::memcpy(r.rowptr, h.rowptr.data(), sizeof(int64_t) * (size_t)(h.rows + 1));
+    if (nnz) {
+      std::memcpy(r.colidx, h.col.data(), sizeof(int32_t) * (size_t)nnz);
+      std::memcpy(r.vals, h.val.data(), sizeof(double) * 2 * (size_t)nnz);
+    }
+    *GW = r;
+  });
+}
+
+void pf_csr_free(pf_csr* m) {
+  if (!m) return;
+  std::free(m->rowptr);
+  std::free(m->colidx);
+  std::free(m->vals);
+  m->rowptr = nullptr;
+  m->colidx = nullptr;
+  m->vals = nullptr;
+  m->rows = m->cols = m->nnz = 0;
+}
+
 int pf_op_destroy(pf_op* op) {
   delete op;
   return PF_OK;
@@ -1012,7 +1174,7 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
 int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
   if (!op) return fail(PF_ERR_ARG, "null operator");
   return guarded([&] {
-    if (op->w_term) {  // one gridding kernel for projection rows: nothing to choose
+    if (op->w_term || op->g_csr) {  // one gridding kernel for projection rows or a given matrix
       if (chosen) *chosen = op->grid_variant;
       return;
     }

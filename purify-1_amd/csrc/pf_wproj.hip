@@ -87,22 +87,6 @@ void wrow_host(const WRowSpec& sp, double u, double v, double w, double wre, dou
 // ---------------------------------------------------------------------------------------------
 __device__ inline int wrapw(int a, int n) { return a & (n - 1); }
 
-template <typename R>
-__device__ inline R shfl_xor_r(R x, int m) {
-  return __shfl_xor(x, m, 64);
-}
-
-// Sum over the 64 lanes in a fixed butterfly order; every lane ends with the total.
-template <typename R>
-__device__ inline cpx<R> wave_sum(cpx<R> a) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    a.x += shfl_xor_r(a.x, m);
-    a.y += shfl_xor_r(a.y, m);
-  }
-  return a;
-}
-
 constexpr int WD_THREADS = 256;
 constexpr int WD_ROWS = 4;  // staged rows per wave per round
 

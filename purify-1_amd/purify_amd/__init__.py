@@ -8,7 +8,7 @@ from .kernels import kernel, kernel_from_string
 from .operators import (LinearTransform, MeasurementOperator, distributed_measurement_operator,
                         measurement_operator_factory, measurementoperator)
 from .utilities import vis_params, vis_units
-from . import utilities
+from . import utilities, wproj_utilities
 
 __all__ = ["kernel", "kernel_from_string", "LinearTransform", "MeasurementOperator", "measurementoperator",
            "measurement_operator_factory", "distributed_measurement_operator", "vis_params", "vis_units",

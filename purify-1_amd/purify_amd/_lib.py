@@ -44,6 +44,13 @@ class pf_op_info(C.Structure):
     ]
 
 
+class pf_csr(C.Structure):
+    _fields_ = [
+        ("rows", C.c_int64), ("cols", C.c_int64), ("nnz", C.c_int64),
+        ("rowptr", C.POINTER(C.c_int64)), ("colidx", C.POINTER(C.c_int32)), ("vals", C.POINTER(C.c_double)),
+    ]
+
+
 _lib = None
 
 # name -> (restype, argtypes); kept in the order of include/purify_amd.h
@@ -74,6 +81,10 @@ SIGNATURES = {
     "pf_synth_uv": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_void_p, C.c_void_p]),
     "pf_synth_image": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "pf_synth_cnormal": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_void_p]),
+    "pf_wprojection_matrix": (C.c_int, [C.POINTER(pf_csr), C.c_int32, C.c_int32, C.c_void_p, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.POINTER(pf_csr)]),
+    "pf_csr_free": (None, [C.POINTER(pf_csr)]),
+    "pf_op_create_csr": (C.c_int, [C.POINTER(pf_op_desc), C.POINTER(pf_csr), C.POINTER(C.c_void_p)]),
     "pf_last_error": (C.c_char_p, []),
     "pf_version": (C.c_char_p, []),
 }

@@ -50,10 +50,13 @@ def _is_device_tensor(x):
 class MeasurementOperator:
     """Owns a pf_op handle; the direct (Phi) and adjoint (Phi^dagger) applies of one operator."""
 
-    def __init__(self, desc: _lib.pf_op_desc, keep=()):
+    def __init__(self, desc: _lib.pf_op_desc, keep=(), csr=None):
         self._keep = keep
         h = C.c_void_p()
-        _lib.check(_lib.lib().pf_op_create(C.byref(desc), C.byref(h)))
+        if csr is None:
+            _lib.check(_lib.lib().pf_op_create(C.byref(desc), C.byref(h)))
+        else:
+            _lib.check(_lib.lib().pf_op_create_csr(C.byref(desc), C.byref(csr), C.byref(h)))
         self._h = h
         info = self.info()
         self.imsizey, self.imsizex = info.imsizey, info.imsizex
@@ -294,6 +297,23 @@ def _init_from_vis(uv_vis, imsizey, imsizex, cell_x, cell_y, oversample_ratio=2.
     d, keep = _make_desc(uv_vis.u, uv_vis.v, uv_vis.w, uv_vis.weights, imsizey, imsizex, oversample_ratio,
                          power_iters, power_tol, kernel, Ju, Jv, w_term, Jw, cell_x, cell_y, units, precision, tile)
     return LinearTransform(MeasurementOperator(d, keep))
+
+
+def init_degrid_operator_2d_from_matrix(G, imsizey, imsizex, oversample_ratio=2.0, power_iters=100, power_tol=1e-4,
+                                        kernel=_kernel.kb, Ju=4, Jv=4, precision="c64"):
+    """Phi = G F Z S / norm with a given sparse gridding matrix G (rows = visibilities, columns = grid
+    cells p * Nu + q), e.g. wproj_utilities.wprojection_matrix's result.  The kernel and supports fix
+    the correction S (operators.cc:101-121); power_iters / power_tol the normalisation
+    (operators.h:390-391)."""
+    from .wproj_utilities import to_pf_csr
+    csr, keep = to_pf_csr(G)
+    z = np.zeros(0)
+    d, keep2 = _make_desc(z, z, None, None, imsizey, imsizex, oversample_ratio, power_iters, power_tol, kernel, Ju,
+                          Jv, False, 6, 1.0, 1.0, _lib.PF_PIXELS, precision)
+    return LinearTransform(MeasurementOperator(d, (keep, keep2), csr=csr))
+
+
+measurementoperator.init_degrid_operator_2d_from_matrix = staticmethod(init_degrid_operator_2d_from_matrix)
 
 
 # factory::measurement_operator_factory (cpp/purify/measurement_operator_factory.h:20-96)
