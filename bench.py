@@ -68,6 +68,30 @@ def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, KK=None):
     return table.get(name)
 
 
+def serving_run(n_requests, seed, log):
+    """Served latency: a Poisson stream (20 requests / s) of imaging jobs, image side uniform over
+    {256, 512, 1024, 2048}, M log-uniform in [1e4, 4e6], 30 % with w-terms; each job builds its operator
+    and applies one Phi + Phi^dagger pair (complex64).  Latency runs from arrival to the result on the
+    host, so it includes the operator build and the host transfers."""
+    from purify_amd import serving
+    reqs = serving.poisson_requests(seed, n_requests, 20.0)
+    M_total = sum(int(r.u.size) for r in reqs)
+    srv = serving.Server(96 << 30, max_inflight=4)
+    t0 = time.perf_counter()
+    futs = [srv.submit(r) for r in reqs]
+    res = [f.result() for f in futs]
+    wall = time.perf_counter() - t0
+    srv.close()
+    st = serving.latency_stats(res)
+    out = {"requests": len(res), "rate_per_s": 20.0, "p50_ms": 1e3 * st["p50_s"], "p99_ms": 1e3 * st["p99_s"],
+           "mean_ms": 1e3 * st["mean_s"], "throughput_req_per_s": len(res) / wall, "vis_pairs_per_s": M_total / wall,
+           "peak_inflight": srv.peak_inflight, "hbm_budget_gib": 96, "w_term_jobs": sum(r.w_term for r in reqs),
+           "sample": f"{len(res)} requests of the serving mix (the full stream is 2000 at 20 / s)"}
+    log(f"[bench] serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
+        f"{out['throughput_req_per_s']:.1f} req/s, peak in flight {srv.peak_inflight}")
+    return out
+
+
 def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev):
     """Workload for rocprofv3 --pmc passes: a copy of known size, then the operator pair."""
     import torch
@@ -104,6 +128,8 @@ def main():
     ap.add_argument("--workload", default="north_star", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-pairs", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=60,
+                    help="requests of the serving sample (Poisson arrivals at 20 / s); 0 skips it")
     ap.add_argument("--counters", choices=("c64", "c128"), default=None,
                     help="hardware-counter run: three 512 MiB plain copies (the calibration kernel for "
                          "FETCH_SIZE / WRITE_SIZE) then warmup + steps pairs of one operator, nothing else")
@@ -253,6 +279,10 @@ def main():
         log(f"[bench] cpu: {t_pair:.3f} s/pair; rel err vs oracle {acc_oracle}")
         del orc
 
+    serve = None
+    if args.serve_requests > 0 and not w_term:
+        serve = serving_run(args.serve_requests, seed + 10, log)
+
     lat = sorted(r64["lat"])
     out = {
         "metric": METRIC,
@@ -280,6 +310,7 @@ def main():
         "rel_err_fp32_vs_fp64_gpu": rel_fp32_fp64,
         "rel_err_vs_oracle": acc_oracle,
         "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
+        "serving": serve,
         "setup_s": {"c64": r64["setup_s"], "c128": res["c128"]["setup_s"]},
         "host_pair_ms": {"c64": r64["host_ms"], "c128": res["c128"]["host_ms"],
                          "note": "pair on host complex128 buffers: PCIe copies and conversions included"},
