@@ -386,23 +386,52 @@ HostLists tile_lists(const std::vector<int32_t>& perm, const std::vector<int64_t
       if (!seen) trows[nr++] = tr;
     }
   };
-  int tc[16], tr[16], nc, nr;
+  // Threads take contiguous ranges of the sorted order; within a tile the entries of thread t follow
+  // those of threads < t, so every list is in ascending sorted order, as a serial pass would leave it.
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
+                                                                    M / 65536 + 1}));
+  const int64_t chunk = (M + nth - 1) / nth;
+  std::vector<std::vector<uint32_t>> cnt(nth, std::vector<uint32_t>((size_t)nt, 0));
+  std::vector<int64_t> tot(nth, 0);
+  auto parallel = [&](auto&& body) {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t) pool.emplace_back(body, t);
+    body(0);
+    for (auto& th : pool) th.join();
+  };
+  parallel([&](int t) {
+    int tc[16], tr[16], nc, nr;
+    for (int64_t i = t * chunk; i < std::min(M, (t + 1) * chunk); ++i) {
+      sets(perm[i], tc, nc, tr, nr);
+      tot[t] += (int64_t)nr * nc;
+      for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nc; ++c) cnt[t][(size_t)tr[r] * ntu + tc[c]]++;
+    }
+  });
   int64_t total = 0;
-  for (int64_t i = 0; i < M; ++i) {
-    sets(perm[i], tc, nc, tr, nr);
-    total += (int64_t)nr * nc;
-    for (int r = 0; r < nr; ++r)
-      for (int c = 0; c < nc; ++c) h.ptr[(size_t)tr[r] * ntu + tc[c] + 1]++;
-  }
+  for (int t = 0; t < nth; ++t) total += tot[t];
   if (total > 0xffffffffLL) throw Error(PF_ERR_UNSUPPORTED, "gridding tile lists exceed 2^32 entries");
-  for (int t = 0; t < nt; ++t) h.ptr[t + 1] += h.ptr[t];
-  h.list.resize(h.ptr[nt]);
-  std::vector<uint32_t> fill(h.ptr.begin(), h.ptr.end() - 1);
-  for (int64_t i = 0; i < M; ++i) {
-    sets(perm[i], tc, nc, tr, nr);
-    for (int r = 0; r < nr; ++r)
-      for (int c = 0; c < nc; ++c) h.list[fill[(size_t)tr[r] * ntu + tc[c]]++] = (uint32_t)i;
+  // start of thread t's entries in tile k: ptr[k] + sum of cnt[t' < t][k]
+  std::vector<std::vector<uint32_t>> start(nth, std::vector<uint32_t>((size_t)nt, 0));
+  uint32_t run = 0;
+  for (int k = 0; k < nt; ++k) {
+    h.ptr[k] = run;
+    for (int t = 0; t < nth; ++t) {
+      start[t][k] = run;
+      run += cnt[t][k];
+    }
   }
+  h.ptr[nt] = run;
+  h.list.resize(h.ptr[nt]);
+  parallel([&](int t) {
+    int tc[16], tr[16], nc, nr;
+    std::vector<uint32_t>& fill = start[t];
+    for (int64_t i = t * chunk; i < std::min(M, (t + 1) * chunk); ++i) {
+      sets(perm[i], tc, nc, tr, nr);
+      for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nc; ++c) h.list[fill[(size_t)tr[r] * ntu + tc[c]]++] = (uint32_t)i;
+    }
+  });
   h.slot.resize(nt);
   for (int t = 0; t < nt; ++t) h.slot[t] = make_uint4((uint32_t)t, h.ptr[t], h.ptr[t + 1], 0u);
   std::stable_sort(h.slot.begin(), h.slot.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
