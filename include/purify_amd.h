@@ -130,13 +130,17 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
 /* Keys: "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics, 2 = rank-4 updates of 16 x 16 tiles on
  * the matrix cores, the default where the grid sides are at least 16 + J - 1; all reproducible run to run);
  * "kernel_table" (complex64, J <= 8: interpolated tap tables, about 1e-7 relative error: 1 = in gridding,
- * the default, 2 = in gridding and degridding; 0 = evaluate the kernel for every tap); read-only
- * "tile_u", "tile_v".  Results do not depend on
- * grid_kernel beyond floating-point summation order. */
+ * the default, 2 = in gridding and degridding; 0 = evaluate the kernel for every tap); "fft_columns"
+ * (0 = the column FFT in one pass up to 8192 rows, the default; 1 = two passes of N1 x N2 beyond 512
+ * rows); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond
+ * floating-point summation order. */
 int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
 int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
-/* Times every gridding-kernel variant on this operator (reps adjoint applies each, HIP events on the
- * null stream) and keeps the fastest.  Writes the chosen variant to *chosen if non-NULL. */
+/* Times every gridding-kernel variant on this operator (reps adjoint gridding passes each, HIP events on
+ * the null stream) and keeps the fastest, then does the same for the two "fft_columns" modes (reps
+ * forward + adjoint column passes each).  Writes the chosen gridding variant to *chosen if non-NULL;
+ * the column mode is read back with pf_op_get_tuning.  Operators with w-projection rows or a given
+ * gridding matrix have one gridding kernel: only the column mode is chosen. */
 int pf_op_autotune(pf_op *op, int32_t reps, int64_t *chosen);
 
 /* ---- per-kernel timing (no counterpart in the reference, whose operators log at construction only,

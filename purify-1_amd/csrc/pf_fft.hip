@@ -12,9 +12,11 @@
 // reads an image row, multiplies the separable correction S, zero-pads (Z) in registers and writes
 // only the ny non-zero grid rows; the adjoint's row pass reads those rows back and writes the
 // cropped, corrected image.
-// Column pass: a block owns C adjacent columns (512 B row segments) and the lines are columns.
-// Up to 512 rows it is one pass; beyond, Nv = N1*N2 is split into an N1-point pass over rows of
-// stride N2 (with the W_Nv^(r2 k1) twiddle) and an N2-point pass over contiguous row blocks.
+// Column pass: a block owns C adjacent columns and the lines are columns.  Up to 512 rows, 16 or more
+// columns per block; up to 8192 rows, still one pass with 2 to 8 columns per block in up to 160 KiB
+// of padded LDS (a single read and write of the grid).  The alternative for long columns
+// (fft_columns = 1) splits Nv = N1*N2 into an N1-point pass over rows of stride N2 (with the
+// W_Nv^(r2 k1) twiddle) and an N2-point pass over contiguous row blocks, 512 B row segments each.
 #include "pf_internal.h"
 
 namespace pf {
@@ -58,8 +60,9 @@ struct CPlan {
 // LDS position of line element i.  es > 0: element stride es (column tiles, lines interleaved, which
 // is conflict-free because consecutive lanes take consecutive lines).  es == 0: a contiguous line
 // padded by one element every 16, so the strided Stockham writes of the early stages spread over the
-// banks instead of hitting the same ones.
-__device__ inline int lidx(int i, int es) { return es ? i * es : i + (i >> 4); }
+// banks instead of hitting the same ones.  es < 0: column tiles of -es lines with the same padding
+// (the first stage's writes, 16 elements apart, would otherwise fall on one bank group).
+__device__ inline int lidx(int i, int es) { return es > 0 ? i * es : es == 0 ? i + (i >> 4) : (i + (i >> 4)) * -es; }
 
 template <typename R, int E, int RAD, int NS, int N, bool INV>
 __device__ inline void stage_bfly(cpx<R> (&v)[E], int t, const cpx<R>* __restrict__ tw, int twm) {
@@ -123,11 +126,6 @@ template <typename R, int L, bool INV, class Load, class Store>
 __device__ inline void fft_line(int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm, Load& load, Store& store) {
   cpx<R> v[CPlan<L>::E];
   stages<R, L, 0, INV>(v, t, lds, es, tw, twm, load, store);
-}
-
-// Dynamic LDS above 64 KiB (gfx950 has 160 KiB per CU) must be opted into per kernel.
-static void allow_lds(const void* fn, size_t shm) {
-  if (shm > 65536) PF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
 }
 
 // Calls F.template operator()<L>() for the runtime log2 length L in [1, 13].
@@ -310,6 +308,55 @@ __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a) {
   fft_line<R, L, INV>(t, smem + c, C, a.tw, a.twm, load, store);
 }
 
+// Single-pass columns up to 8192 rows: C columns per block, 1024 threads at most, padded LDS tiles
+// above 64 KiB.  Adjacent column blocks share 128-byte lines of every row, so the block index is
+// remapped to give each XCD (workgroups go round-robin over the 8 XCDs) a contiguous band of
+// columns and its L2 the shared lines.
+template <typename R>
+constexpr int big_col_threads() { return sizeof(R) == 4 ? 1024 : 512; }  // FP64: room for 256 VGPRs
+
+template <typename R, int L, bool INV>
+__global__ void __launch_bounds__(big_col_threads<R>()) k_fft_cols_big(ColArgs<R> a) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int C = a.C;
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int G = gridDim.x;
+  const int blk = (G & 7) ? (int)blockIdx.x : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+  const int col = blk * C + c;
+  const int64_t Nu = a.Nu;
+  auto in_window = [&](int row) { return (unsigned)(row - a.y0) < (unsigned)a.ny; };
+  auto load = [&](int row) -> cpx<R> {
+    if (a.rin && !in_window(row)) return mk<R>(R(0), R(0));
+    return a.in[row * Nu + col];
+  };
+  auto store = [&](int row, cpx<R> val) {
+    if (a.rout && !in_window(row)) return;
+    a.out[row * Nu + col] = val;
+  };
+  fft_line<R, L, INV>(t, smem + c, -C, a.tw, a.twm, load, store);
+}
+
+template <typename R>
+struct BigColLaunch {
+  const ColArgs<R>& a;
+  bool inv;
+  dim3 grid;
+  int threads;
+  size_t shm;
+  hipStream_t s;
+  template <int L>
+  void operator()() {
+    if constexpr (L >= 10 && L <= 13) {
+      void (*k)(ColArgs<R>) = inv ? k_fft_cols_big<R, L, true> : k_fft_cols_big<R, L, false>;
+      allow_lds((const void*)k, shm);
+      hipLaunchKernelGGL(k, grid, dim3(threads), shm, s, a);
+    } else {
+      throw Error(-3, "single-pass column length must be 1024..8192");
+    }
+  }
+};
+
 template <typename R>
 struct ColLaunch {
   const ColArgs<R>& a;
@@ -364,6 +411,19 @@ void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow w
   a.ny = win.ny ? win.ny : p.Nv;
   a.N1 = p.N1;
   a.N2 = p.N2;
+  if (p.col_big) {
+    a.twm = p.Lmax / p.Nv;
+    a.C = p.big_C;
+    a.rin = restrict_in;
+    a.rout = restrict_out;
+    const int N = p.Nv;
+    const size_t shm = (size_t)(N + N / 16) * a.C * sizeof(cpx<R>);
+    ProfScope prof("fft_cols", s);
+    BigColLaunch<R> launch{a, inv, dim3(p.Nu / a.C), a.C * (N / FFT_E), shm, s};
+    dispatch_log2(p.col.log2N, launch);
+    PF_HIP(hipGetLastError());
+    return;
+  }
   if (!p.col_two_step) {
     a.twm = p.Lmax / p.Nv;
     a.C = p.col_C;
@@ -427,7 +487,7 @@ static void twiddle_table(int L, std::vector<double>& re, std::vector<double>& i
   }
 }
 
-void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec) {
+void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
   if (Nv > MAX_AXIS || Nu > MAX_AXIS)
     throw Error(-3, "grid axis larger than " + std::to_string(MAX_AXIS) + " is not supported");
   p.Nv = Nv;
@@ -448,6 +508,16 @@ void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec) {
     if (C > Nu) C = Nu;
     if (C < 1) C = 1;
     p.col_C = C;
+  } else if (col_mode == 0 && Nv <= 8192 && Nu >= 8) {
+    // one pass: C columns of Nv rows in <= 160 KiB of padded LDS and <= 1024 threads
+    p.col_two_step = false;
+    p.col_big = true;
+    p.col = make_line_plan(Nv);
+    const int max_threads = prec ? 512 : 1024;
+    int C = 8;
+    while (C > 1 && ((size_t)(Nv + Nv / 16) * C * esz > 160 * 1024 - 1024 || C * (Nv / FFT_E) > max_threads)) C /= 2;
+    if (C > Nu) C = Nu;
+    p.big_C = C;
   } else {
     p.col_two_step = true;
     int m = 0;

@@ -8,8 +8,10 @@
 // The 1-D kernel factors are evaluated once per visibility (2J evaluations, not J^2).
 //
 // Visibilities are sorted at construction by the grid tile that holds their first tap.
-//  * degrid: one lane per sorted visibility, so a wave's J x J reads fall in one or two tiles and hit
-//    the cache; the result goes to y[perm[i]] in the caller's order.
+//  * degrid: one block per tile stages the tile and its halo in LDS; one lane per sorted visibility
+//    gathers its J x J taps there and writes y in the sorted order (coalesced).
+//  * permute_to_sorted / permute_to_caller: the two-pass permutation between the caller's order and
+//    the sorted order (chunk and bucket passes, each staged in LDS so both move whole cache lines).
 //  * grid: one wave per grid tile (tile-owner).  The tile's list holds every visibility with a tap
 //    inside the tile, in ascending sorted order.  The wave accumulates into its own LDS copy of the
 //    tile (no other wave touches it), then writes the tile once.  Every grid cell is written by
@@ -154,11 +156,7 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
   // this thread's first visibility: its loads go out before the tile staging so the latencies overlap
   uint32_t i = beg + threadIdx.x;
   VisCoord<R> c{};
-  int32_t dst = 0;
-  if (i < end) {
-    c = vis_coord(v, i, Ju, Jv);
-    dst = v.perm[i];
-  }
+  if (i < end) c = vis_coord(v, i, Ju, Jv);
   // stage the tile and its halo: each wave takes DG_ROWS rows per round, every load before any store
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = DG_THREADS / 64;
   constexpr int DG_ROWS = dg_rows<R>();
@@ -185,11 +183,7 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
     // the next visibility's loads go out before this one's arithmetic
     const uint32_t nxt = i + DG_THREADS;
     VisCoord<R> cn = c;
-    int32_t dn = dst;
-    if (nxt < end) {
-      cn = vis_coord(v, nxt, Ju, Jv);
-      dn = v.perm[nxt];
-    }
+    if (nxt < end) cn = vis_coord(v, nxt, Ju, Jv);
     R wu[JJU], wv[JJV];
     taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
     const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
@@ -210,9 +204,8 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
       acc.y += wv[b] * racc.y;
     }
     if (v.weights) acc = cmul(acc, v.weights[i]);
-    y[dst] = scale(acc, scl);
+    y[i] = scale(acc, scl);
     c = cn;
-    dst = dn;
   }
 }
 
@@ -636,24 +629,161 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
   PF_HIP(hipGetLastError());
 }
 
-// Caller's order -> sorted order.  A random 8-byte write costs a 32-byte memory transaction while a
-// random 8-byte read costs a 128-byte cache line, so the permutation reads y in order and scatters.
+// ---------------------------------------------------------------------------------------------
+// Permutation between the caller's order and the sorted order (VisPerm).  A random 8-byte access
+// costs a whole cache line, so a one-pass scatter moves about 8x the data.  Each pass here moves a
+// block's chunk or bucket through LDS: the chunk pass writes (reads) one contiguous run per bucket,
+// the bucket pass writes (reads) its sorted range contiguously.  Every thread issues its loads
+// before any LDS store.
+// ---------------------------------------------------------------------------------------------
+constexpr int PERM_THREADS = 1024;
 template <typename R>
-__global__ void __launch_bounds__(256) k_permute_to_sorted(const cpx<R>* __restrict__ y, const int32_t* __restrict__ inv,
-                                                           cpx<R>* __restrict__ ys, int64_t M) {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < M) ys[inv[m]] = y[m];
+constexpr int perm_per_thread() { return (sizeof(R) == 4 ? 16384 : 8192) / PERM_THREADS; }
+
+// TO_STAGING: chunk of the caller's vector -> staging runs; else staging runs -> caller's vector
+template <typename R, bool TO_STAGING>
+__global__ void __launch_bounds__(PERM_THREADS) k_perm_chunks(VisPerm p, const cpx<R>* __restrict__ src,
+                                                              cpx<R>* __restrict__ dst) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* sh = reinterpret_cast<cpx<R>*>(smem_raw);
+  constexpr int E = perm_per_thread<R>();
+  const int64_t c0 = (int64_t)blockIdx.x * p.S;
+  const int n = (int)min((int64_t)p.S, p.M - c0);
+  cpx<R> val[E];
+  int idx[E];
+  if (TO_STAGING) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = threadIdx.x + e * PERM_THREADS;
+      if (k < n) {
+        val[e] = src[c0 + k];
+        idx[e] = p.l1[c0 + k];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (threadIdx.x + e * PERM_THREADS < n) sh[idx[e]] = val[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = threadIdx.x + e * PERM_THREADS;
+      if (j < n) idx[e] = p.g1[c0 + j];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = threadIdx.x + e * PERM_THREADS;
+      if (j < n) dst[idx[e]] = sh[j];
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = threadIdx.x + e * PERM_THREADS;
+      if (j < n) idx[e] = p.g1[c0 + j];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = threadIdx.x + e * PERM_THREADS;
+      if (j < n) val[e] = src[idx[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = threadIdx.x + e * PERM_THREADS;
+      if (j < n) sh[j] = val[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = threadIdx.x + e * PERM_THREADS;
+      if (k < n) idx[e] = p.l1[c0 + k];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = threadIdx.x + e * PERM_THREADS;
+      if (k < n) dst[c0 + k] = sh[idx[e]];
+    }
+  }
+}
+
+// TO_SORTED: bucket of the staging vector -> sorted range; else sorted range -> staging bucket
+template <typename R, bool TO_SORTED>
+__global__ void __launch_bounds__(PERM_THREADS) k_perm_buckets(VisPerm p, const cpx<R>* __restrict__ src,
+                                                               cpx<R>* __restrict__ dst) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* sh = reinterpret_cast<cpx<R>*>(smem_raw);
+  constexpr int E = perm_per_thread<R>();
+  const int64_t b0 = (int64_t)blockIdx.x * p.S;  // the bucket's staging range and sorted range start here
+  const int n = (int)min((int64_t)p.S, p.M - b0);
+  cpx<R> val[E];
+  int idx[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = threadIdx.x + e * PERM_THREADS;
+    if (k < n) {
+      val[e] = src[b0 + k];
+      idx[e] = p.p2[b0 + k];
+    }
+  }
+  if (TO_SORTED) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (threadIdx.x + e * PERM_THREADS < n) sh[idx[e]] = val[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = threadIdx.x + e * PERM_THREADS;
+      if (k < n) dst[b0 + k] = sh[k];
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (threadIdx.x + e * PERM_THREADS < n) sh[threadIdx.x + e * PERM_THREADS] = val[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = threadIdx.x + e * PERM_THREADS;
+      if (k < n) dst[b0 + k] = sh[idx[e]];
+    }
+  }
 }
 
 template <typename R>
-void permute_to_sorted(const VisData<R>& v, const cpx<R>* y, cpx<R>* ys, hipStream_t s) {
-  if (v.M == 0) return;
+static void check_perm(const VisPerm& p) {
+  if (p.S <= 0 || p.S > perm_per_thread<R>() * PERM_THREADS || (p.S & (p.S - 1)) ||
+      (int64_t)p.nb * p.S < p.M || (int64_t)(p.nb - 1) * p.S >= p.M)
+    throw Error(-5, "permutation plan does not match the visibilities");  // PF_ERR_STATE
+}
+
+template <typename R>
+void permute_to_sorted(const VisPerm& p, const cpx<R>* y, cpx<R>* tmp, cpx<R>* ys, hipStream_t s) {
+  if (p.M == 0) return;
+  check_perm<R>(p);
   ProfScope prof("permute_to_sorted", s);
-  hipLaunchKernelGGL((k_permute_to_sorted<R>), dim3((unsigned)((v.M + 255) / 256)), dim3(256), 0, s, y, v.inv, ys, v.M);
+  const size_t shm = sizeof(cpx<R>) * (size_t)p.S;
+  allow_lds((const void*)k_perm_chunks<R, true>, shm);
+  allow_lds((const void*)k_perm_buckets<R, true>, shm);
+  hipLaunchKernelGGL((k_perm_chunks<R, true>), dim3((unsigned)p.nb), dim3(PERM_THREADS), shm, s, p, y, tmp);
+  hipLaunchKernelGGL((k_perm_buckets<R, true>), dim3((unsigned)p.nb), dim3(PERM_THREADS), shm, s, p, tmp, ys);
   PF_HIP(hipGetLastError());
 }
-template void permute_to_sorted<float>(const VisData<float>&, const cpx<float>*, cpx<float>*, hipStream_t);
-template void permute_to_sorted<double>(const VisData<double>&, const cpx<double>*, cpx<double>*, hipStream_t);
+
+template <typename R>
+void permute_to_caller(const VisPerm& p, const cpx<R>* ys, cpx<R>* tmp, cpx<R>* y, hipStream_t s) {
+  if (p.M == 0) return;
+  check_perm<R>(p);
+  ProfScope prof("permute_to_caller", s);
+  const size_t shm = sizeof(cpx<R>) * (size_t)p.S;
+  allow_lds((const void*)k_perm_chunks<R, false>, shm);
+  allow_lds((const void*)k_perm_buckets<R, false>, shm);
+  hipLaunchKernelGGL((k_perm_buckets<R, false>), dim3((unsigned)p.nb), dim3(PERM_THREADS), shm, s, p, ys, tmp);
+  hipLaunchKernelGGL((k_perm_chunks<R, false>), dim3((unsigned)p.nb), dim3(PERM_THREADS), shm, s, p, tmp, y);
+  PF_HIP(hipGetLastError());
+}
+template void permute_to_sorted<float>(const VisPerm&, const cpx<float>*, cpx<float>*, cpx<float>*, hipStream_t);
+template void permute_to_sorted<double>(const VisPerm&, const cpx<double>*, cpx<double>*, cpx<double>*,
+                                        hipStream_t);
+template void permute_to_caller<float>(const VisPerm&, const cpx<float>*, cpx<float>*, cpx<float>*, hipStream_t);
+template void permute_to_caller<double>(const VisPerm&, const cpx<double>*, cpx<double>*, cpx<double>*,
+                                        hipStream_t);
 
 template void degrid<float>(const GridGeom<float>&, const VisData<float>&, const TileLists&, const cpx<float>*,
                             cpx<float>*, float, hipStream_t);

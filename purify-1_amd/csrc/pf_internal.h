@@ -65,13 +65,17 @@ struct Fft2dPlan {
   LinePlan row;        // N = Nu
   int row_lpb = 1;     // lines per block of the row pass
   bool col_two_step = false;
+  bool col_big = false;  // single pass over columns longer than 512 rows
+  int big_C = 4;         // columns per block of that pass
   LinePlan col;        // single-pass column plan, N = Nv
   int col_C = 16;      // columns per block of the single-pass column kernel
   int N1 = 0, N2 = 0;  // Nv = N1 * N2 for the two-step column pass
   LinePlan colA, colB; // N1- and N2-point plans
   int two_C = 64;      // columns per block of the two-step kernels
 };
-void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec);
+// col_mode: 0 = columns in one pass where the LDS allows (up to 8192 rows), 1 = the two-step split
+// beyond 512 rows
+void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode = 0);
 void fft_plan_free(Fft2dPlan& p);
 
 // Which rows of the grid hold data (forward after the row pass) or are wanted (adjoint crop).
@@ -110,10 +114,31 @@ struct VisData {
   const cpx<float>* ff = nullptr;
   const cpx<double>* uv = nullptr;
   const cpx<R>* weights = nullptr;  // nullptr: unit weights
-  const int32_t* perm = nullptr;    // sorted position -> caller's index
-  const int32_t* inv = nullptr;     // caller's index -> sorted position
   int64_t M = 0;
 };
+
+// The permutation between the caller's order and the sorted order, applied in two passes that each
+// move whole cache lines.  The caller's order is cut into chunks and the sorted order into buckets of
+// S visibilities each (nb of both).  A staging vector holds bucket b in [b*S, b*S + size of b), and
+// inside it the visibilities grouped by chunk, each group in the caller's order: so a chunk sends one
+// contiguous run to each bucket, and a bucket gathers one run from each chunk.
+//   chunk pass:  one block stages its chunk in LDS ordered by bucket, then writes (reads) the runs.
+//   bucket pass: one block stages its bucket in LDS, then writes (reads) the sorted range coalesced.
+struct VisPerm {
+  const uint16_t* l1 = nullptr;  // caller index m -> position inside chunk m / S, ordered by bucket
+  const int32_t* g1 = nullptr;   // (chunk c, position j) at c * S + j -> staging index
+  const uint16_t* p2 = nullptr;  // staging index t -> position inside bucket t / S of the sorted order
+  int S = 0;                     // chunk and bucket size, a power of two <= perm_max_S(precision)
+  int nb = 0;                    // ceil(M / S)
+  int64_t M = 0;
+};
+// largest chunk: 128 KiB of LDS
+inline int perm_max_S(int esz) { return esz == 8 ? 16384 : 8192; }
+
+// Dynamic LDS above 64 KiB (gfx950 has 160 KiB per CU) must be opted into per kernel.
+inline void allow_lds(const void* fn, size_t shm) {
+  if (shm > 65536) PF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+}
 
 // complex64 tap tables: all J taps of a visibility sit at offsets delta + J/2 - 1 - a of one fractional
 // delta = (u - k_u) - J/2 in (0, 1], so row q of the table holds the J kernel values at delta = q / TAB_Q
@@ -143,9 +168,10 @@ struct TileLists {
   int n_own = 0;
 };
 
+// y = G grid with y in the operator's sorted order (permute_to_caller brings it to the caller's)
 template <typename R>
-void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
-            hipStream_t s);
+void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y_sorted,
+            R scale, hipStream_t s);
 bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv);
 // side of the matrix-core gridding tiles; usable when N >= MFMA_TILE + J - 1 on both axes
 constexpr int MFMA_TILE = 16;
@@ -158,9 +184,11 @@ enum GridVariant { GRID_RESIDUE = 0, GRID_ATOMIC = 1, GRID_MFMA = 2 };
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm,
                 const cpx<R>* y_sorted, cpx<R>* grid, R scale, hipStream_t s, int variant);
-// ys[inv[m]] = y[m]: caller's order to sorted order (coalesced reads, scattered 8-byte writes)
+// caller's order -> sorted order (ys) and back, through the staging vector `tmp` (M values)
 template <typename R>
-void permute_to_sorted(const VisData<R>& v, const cpx<R>* y, cpx<R>* ys, hipStream_t s);
+void permute_to_sorted(const VisPerm& p, const cpx<R>* y, cpx<R>* tmp, cpx<R>* ys, hipStream_t s);
+template <typename R>
+void permute_to_caller(const VisPerm& p, const cpx<R>* ys, cpx<R>* tmp, cpx<R>* y, hipStream_t s);
 
 // ---- projection rows (w_term) -------------------------------------------------------------
 // Row m of the w-projection gridding matrix (operators.cc:49-99): Kv x Ku taps, K = Jw + J - 1, at
@@ -173,9 +201,9 @@ struct WRows {
   const int2* kw = nullptr;      // (kw_u, kw_v) per sorted visibility
   const cpx<R>* vals = nullptr;  // M * Kv * Ku
 };
-// y = G grid: owner tiles staged with a (K - 1)-wide halo, one wave per visibility
+// y = G grid in the sorted order: owner tiles staged with a (K - 1)-wide halo, one wave per visibility
 template <typename R>
-void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const TileLists& t, const cpx<R>* grid,
+void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx<R>* grid,
                   cpx<R>* y, R scale, hipStream_t s);
 // grid = G^H ys (ys in sorted order): one wave per tile, the taps of one visibility at a time
 template <typename R>

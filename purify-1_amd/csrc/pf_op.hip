@@ -289,9 +289,12 @@ struct pf_op {
   cpx<float>* d_ff = nullptr;
   cpx<double>* d_uv = nullptr;
   void* d_weights = nullptr;
-  int32_t* d_perm = nullptr;
-  int32_t* d_inv = nullptr;    // inverse permutation
-  void* d_ysort = nullptr;     // visibilities in sorted order (adjoint input after the permutation)
+  VisPerm vperm;                // caller's order <-> sorted order (arrays below)
+  uint16_t* d_perm_l1 = nullptr;
+  int32_t* d_perm_g1 = nullptr;
+  uint16_t* d_perm_p2 = nullptr;
+  void* d_ysort = nullptr;     // visibilities in sorted order (degridding output, gridding input)
+  void* d_ystage = nullptr;    // staging vector of the two-pass permutation
   float* d_tab_u = nullptr;  // complex64 tap tables
   float* d_tab_v = nullptr;
   void* d_sx = nullptr;
@@ -307,6 +310,7 @@ struct pf_op {
   std::unique_ptr<Profiler> prof;  // per-kernel timing when enabled
   int grid_variant = GRID_RESIDUE;  // tuning knob: gridding kernel
   int kernel_table = 0;             // tuning knob: 0 no tap tables, 1 in gridding, 2 in gridding and degridding
+  int fft_col_mode = 0;             // tuning knob: column FFT in one pass (0) or two steps (1)
   int64_t device_bytes = 0;
   std::vector<void*> allocs;
 
@@ -436,6 +440,57 @@ HostLists tile_lists(const std::vector<int32_t>& perm, const std::vector<int64_t
   for (int t = 0; t < nt; ++t) h.slot[t] = make_uint4((uint32_t)t, h.ptr[t], h.ptr[t + 1], 0u);
   std::stable_sort(h.slot.begin(), h.slot.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
   return h;
+}
+
+// The two-pass permutation plan (VisPerm) for perm: sorted position -> caller's index.  Staging layout:
+// bucket b (sorted positions [b*S, b*S + S)) occupies staging [b*S, ...), grouped by the caller's chunk
+// (chunk c = caller indices [c*S, c*S + S)), each group in the caller's order.  Chunk c's LDS order is
+// by bucket, then by caller index.
+void build_perm(pf_op* op, const std::vector<int32_t>& perm) {
+  const int64_t M = op->M;
+  VisPerm& p = op->vperm;
+  p.M = M;
+  const int Smax = perm_max_S((int)op->esz());
+  p.S = 64;
+  while (p.S < M && p.S < Smax) p.S <<= 1;
+  p.nb = (int)((M + p.S - 1) / p.S);
+  const int S = p.S, nb = p.nb;
+  std::vector<int32_t> inv(M);
+  for (int64_t i = 0; i < M; ++i) inv[perm[i]] = (int32_t)i;
+  std::vector<uint16_t> l1(M), p2(M);
+  std::vector<int32_t> g1(M);
+  std::vector<int64_t> run(nb);  // next staging index of each bucket
+  for (int b = 0; b < nb; ++b) run[b] = (int64_t)b * S;
+  std::vector<int32_t> cnt(nb), loc(nb);
+  for (int c = 0; c < nb; ++c) {
+    const int64_t c0 = (int64_t)c * S, c1 = std::min<int64_t>(c0 + S, M);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t m = c0; m < c1; ++m) cnt[inv[m] / S]++;
+    int a = 0;
+    for (int b = 0; b < nb; ++b) {
+      loc[b] = a;
+      a += cnt[b];
+    }
+    for (int64_t m = c0; m < c1; ++m) {
+      const int b = inv[m] / S;
+      const int j = loc[b]++;
+      const int64_t t = run[b]++;
+      l1[m] = (uint16_t)j;
+      g1[c0 + j] = (int32_t)t;
+      p2[t] = (uint16_t)(inv[m] - (int64_t)b * S);
+    }
+  }
+  for (int b = 0; b < nb; ++b)
+    if (run[b] != std::min<int64_t>((int64_t)(b + 1) * S, M)) throw Error(PF_ERR_STATE, "permutation plan is inconsistent");
+  op->d_perm_l1 = (uint16_t*)op->dalloc(sizeof(uint16_t) * M);
+  op->d_perm_g1 = (int32_t*)op->dalloc(sizeof(int32_t) * M);
+  op->d_perm_p2 = (uint16_t*)op->dalloc(sizeof(uint16_t) * M);
+  PF_HIP(hipMemcpy(op->d_perm_l1, l1.data(), sizeof(uint16_t) * M, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(op->d_perm_g1, g1.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(op->d_perm_p2, p2.data(), sizeof(uint16_t) * M, hipMemcpyHostToDevice));
+  p.l1 = op->d_perm_l1;
+  p.g1 = op->d_perm_g1;
+  p.p2 = op->d_perm_p2;
 }
 
 void upload_lists(pf_op* op, const HostLists& h, int TU, int TV, TileLists& tl) {
@@ -653,15 +708,9 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->tile_list_len = (int64_t)lists.list.size();
 
   // ---- upload ----
-  op->d_perm = (int32_t*)op->dalloc(sizeof(int32_t) * M);
-  PF_HIP(hipMemcpy(op->d_perm, perm.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
-  {
-    std::vector<int32_t> inv(M);
-    for (int64_t i = 0; i < M; ++i) inv[perm[i]] = (int32_t)i;
-    op->d_inv = (int32_t*)op->dalloc(sizeof(int32_t) * M);
-    PF_HIP(hipMemcpy(op->d_inv, inv.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
-  }
+  build_perm(op, perm);
   op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
+  op->d_ystage = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
   upload_lists(op, lists, TU, TV, op->tl);
   {
     // degridding launch slots: the tiles that own visibilities, most visibilities first
@@ -837,8 +886,6 @@ VisData<R> visdata(const pf_op* op) {
   v.ff = op->d_ff;
   v.uv = op->d_uv;
   v.weights = static_cast<const cpx<R>*>(op->d_weights);
-  v.perm = op->d_perm;
-  v.inv = op->d_inv;
   v.M = op->M;
   return v;
 }
@@ -908,8 +955,12 @@ void G_dev(pf_op* op, const cpx<R>* grid, cpx<R>* y, hipStream_t s) {
   if (op->g_csr)
     csr_gather<R>(op->M, op->d_csr_ptr, op->d_csr_col, static_cast<const cpx<R>*>(op->d_csr_val), grid, y, R(1), s,
                   "degrid_csr");
-  else if (op->w_term) degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->d_perm, op->tl, grid, y, R(1), s);
-  else degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, grid, y, R(1), s);
+  else {
+    cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
+    if (op->w_term) degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, grid, ys, R(1), s);
+    else degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, grid, ys, R(1), s);
+    permute_to_caller<R>(op->vperm, ys, static_cast<cpx<R>*>(op->d_ystage), y, s);
+  }
 }
 // grid = G^H y, y in the caller's order
 template <typename R>
@@ -920,7 +971,7 @@ void GH_dev(pf_op* op, const cpx<R>* y, cpx<R>* grid, hipStream_t s) {
     return;
   }
   cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
-  permute_to_sorted<R>(visdata<R>(op), y, ys, s);
+  permute_to_sorted<R>(op->vperm, y, static_cast<cpx<R>*>(op->d_ystage), ys, s);
   if (op->w_term) grid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, ys, grid, R(1), s);
   else grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, op->tlm, ys, grid, R(1), s, op->grid_variant);
 }
@@ -1167,6 +1218,14 @@ int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
     op->grid_variant = (int)value;
     return PF_OK;
   }
+  if (k == "fft_columns") {
+    if (value != 0 && value != 1) return fail(PF_ERR_ARG, "fft_columns must be 0 or 1");
+    return guarded([&] {
+      fft_plan_free(op->fft);
+      fft_plan_init(op->fft, op->Nv, op->Nu, op->prec, (int)value);
+      op->fft_col_mode = (int)value;
+    });
+  }
   if (k == "kernel_table") {
     if (value < 0 || value > 2) return fail(PF_ERR_ARG, "kernel_table must be 0, 1 or 2");
     if (value && !op->d_tab_u) return fail(PF_ERR_UNSUPPORTED, "tap tables exist only for complex64 and J <= 8");
@@ -1183,6 +1242,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   const std::string k(key);
   if (k == "grid_kernel") {
     *value = op->grid_variant;
+    return PF_OK;
+  }
+  if (k == "fft_columns") {
+    *value = op->fft_col_mode;
     return PF_OK;
   }
   if (k == "kernel_table") {
@@ -1203,10 +1266,8 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
 int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
   if (!op) return fail(PF_ERR_ARG, "null operator");
   return guarded([&] {
-    if (op->w_term || op->g_csr) {  // one gridding kernel for projection rows or a given matrix
-      if (chosen) *chosen = op->grid_variant;
-      return;
-    }
+    // projection rows and a given matrix have one gridding kernel each: only the column FFT is chosen
+    const bool one_grid_kernel = op->w_term || op->g_csr;
     ensure_stage(op);
     hipStream_t s = nullptr;
     PF_HIP(hipMemsetAsync(op->d_vecy, 0, op->esz() * (size_t)std::max<int64_t>(op->M, 1), s));
@@ -1217,6 +1278,7 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     float best = 1e30f;
     int best_v = op->grid_variant;
     for (int variant : {GRID_RESIDUE, GRID_ATOMIC, GRID_MFMA}) {
+      if (one_grid_kernel) break;
       if (variant == GRID_RESIDUE && !grid_rc_ok(op->Nu, op->Nv, op->tl.TU, op->tl.TV, op->Ju, op->Jv)) continue;
       if (variant == GRID_MFMA && !op->tlm.slot) continue;
       op->grid_variant = variant;
@@ -1240,9 +1302,46 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
         best_v = variant;
       }
     }
+    op->grid_variant = best_v;
+    // column FFT: one pass or two steps, timed over the forward and the adjoint column passes
+    if (one_grid_kernel) PF_HIP(hipMemsetAsync(op->d_gridA, 0, op->esz() * op->ngrid(), s));
+    float best_c = 1e30f;
+    int best_m = op->fft_col_mode;
+    for (int mode : {0, 1}) {
+      fft_plan_free(op->fft);
+      fft_plan_init(op->fft, op->Nv, op->Nu, op->prec, mode);
+      auto run = [&] {
+        if (op->prec == PF_C64) {
+          fft_cols<float>(op->fft, false, (cpx<float>*)op->d_gridA, (cpx<float>*)op->d_gridB,
+                          RowWindow{op->y0, op->ny}, true, false, s);
+          fft_cols<float>(op->fft, true, (cpx<float>*)op->d_gridB, (cpx<float>*)op->d_gridA,
+                          RowWindow{op->y0, op->ny}, false, true, s);
+        } else {
+          fft_cols<double>(op->fft, false, (cpx<double>*)op->d_gridA, (cpx<double>*)op->d_gridB,
+                           RowWindow{op->y0, op->ny}, true, false, s);
+          fft_cols<double>(op->fft, true, (cpx<double>*)op->d_gridB, (cpx<double>*)op->d_gridA,
+                           RowWindow{op->y0, op->ny}, false, true, s);
+        }
+      };
+      run();
+      PF_HIP(hipEventRecord(a, s));
+      for (int k = 0; k < n; ++k) run();
+      PF_HIP(hipEventRecord(b, s));
+      PF_HIP(hipEventSynchronize(b));
+      float ms = 0;
+      PF_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best_c) {
+        best_c = ms;
+        best_m = mode;
+      }
+    }
+    if (best_m != 1) {
+      fft_plan_free(op->fft);
+      fft_plan_init(op->fft, op->Nv, op->Nu, op->prec, best_m);
+    }
+    op->fft_col_mode = best_m;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    op->grid_variant = best_v;
     if (chosen) *chosen = best_v;
   });
 }

@@ -78,7 +78,7 @@ def estimate_bytes(req: ImagingRequest) -> int:
     M = int(np.asarray(req.u).size)
     n2 = req.imsizex * req.imsizey
     grid = 2 * e * Nu * Nv
-    vis = M * (16 + 8 + 4 + 4 + e + 4 * 4)  # coordinates, weights, permutations, sorted values, 2 list sets
+    vis = M * (16 + 8 + 8 + 2 * e + 4 * 4)  # coordinates, weights, permutation plan, sorted and staged values, 2 list sets
     rows = M * (req.J + req.Jw - 1) ** 2 * e if req.w_term else 0
     stage = 16 * max(n2, M) + e * (n2 + M)
     io = 2 * e * (n2 + M)

@@ -127,6 +127,8 @@ def test_gridding_variants_agree_and_are_deterministic(precision, variant):
     assert np.array_equal(a, b)
     chosen = gpu.op.autotune(2)
     assert chosen in (0, 1, 2) and gpu.op.get_tuning("grid_kernel") == chosen
+    assert gpu.op.get_tuning("fft_columns") in (0, 1)
+    assert rel(gpu.adjoint() * y, orc.adjoint(y)) < TOL[precision]
 
 
 def test_matrix_core_gridding_needs_room_for_a_tile():
@@ -260,3 +262,38 @@ def test_construction_errors_carry_reference_messages():
         measurementoperator.init_degrid_operator_2d(u, u, None, None, 10, 10, 2.0, 0, 1e-4)
     with pytest.raises(RuntimeError, match="Size of u and v"):
         measurementoperator.init_degrid_operator_2d(u, np.zeros(3), None, None, 16, 16)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_fft_column_modes_agree(precision):
+    # 1024^2 image -> 2048^2 grid: one-pass columns (default) against the N1 x N2 two-step split
+    n, M = 1024, 50_000
+    u, v = make_uv(71, M, n)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", 6, 6,
+                                                     precision=precision)
+    assert lt.op.get_tuning("fft_columns") == 0
+    x = utilities.synth_image(72, n, n, 20)
+    y = utilities.synth_cnormal(73, M)
+    a, b = lt * x, lt.adjoint() * y
+    lt.op.set_tuning("fft_columns", 1)
+    assert lt.op.get_tuning("fft_columns") == 1
+    c, d = lt * x, lt.adjoint() * y
+    tol = 1e-13 if precision == "c128" else 2e-6
+    assert rel(a, c) < tol and rel(b, d) < tol
+    with pytest.raises(PurifyError):
+        lt.op.set_tuning("fft_columns", 2)
+
+
+@pytest.mark.parametrize("precision,M", [("c64", 16384), ("c64", 16385), ("c64", 49153), ("c64", 63),
+                                         ("c128", 8192), ("c128", 8193), ("c128", 24577), ("c128", 1)])
+def test_permutation_chunk_and_bucket_boundaries(precision, M):
+    # the caller <-> sorted permutation runs in chunks and buckets of 16384 (c64) / 8192 (c128)
+    # visibilities: exact multiples, one past, a ragged last chunk and a single partial chunk
+    n = 64
+    u, v = make_uv(81 + M, M, n)
+    w = utilities.synth_cnormal(82, M)
+    gpu, orc = pair(u, v, n, n, weights=w, precision=precision)
+    x = utilities.synth_image(83, n, n, 10)
+    assert rel(gpu * x, orc.forward(x)) < TOL[precision]
+    y = utilities.synth_cnormal(84, M)
+    assert rel(gpu.adjoint() * y, orc.adjoint(y)) < TOL[precision]

@@ -98,6 +98,12 @@ def test_wprojection_adjoint_dot_product_and_determinism(precision):
     b = lt.op.adjoint_apply(yd)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+    # autotuning keeps the one gridding kernel of projection rows and picks a column FFT mode
+    g0 = lt.op.get_tuning("grid_kernel")
+    assert lt.op.autotune(2) == g0
+    c = lt.op.adjoint_apply(yd)
+    torch.cuda.synchronize()
+    assert (c - a).abs().max().item() <= TOL[precision] * a.abs().max().item()
 
 
 def test_wprojection_power_method_and_vis_params_units():
