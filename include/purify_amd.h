@@ -132,7 +132,9 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * "kernel_table" (complex64, J <= 8: interpolated tap tables, about 1e-7 relative error: 1 = in gridding,
  * the default, 2 = in gridding and degridding; 0 = evaluate the kernel for every tap); "fft_columns"
  * (0 = the column FFT in one pass up to 8192 rows, the default; 1 = two passes of N1 x N2 beyond 512
- * rows); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond
+ * rows); "fft_row_blocks" (row FFT grid: that many resident blocks per compute unit, each looping over
+ * lines with the next line prefetched; 0 = one line per block; default 4 for complex64, 2 for
+ * complex128); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond
  * floating-point summation order. */
 int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
 int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);

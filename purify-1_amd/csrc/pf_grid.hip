@@ -9,9 +9,10 @@
 //
 // Visibilities are sorted at construction by the grid tile that holds their first tap.
 //  * degrid: one block per tile stages the tile and its halo in LDS; one lane per sorted visibility
-//    gathers its J x J taps there and writes y in the sorted order (coalesced).
-//  * permute_to_sorted / permute_to_caller: the two-pass permutation between the caller's order and
-//    the sorted order (chunk and bucket passes, each staged in LDS so both move whole cache lines).
+//    gathers its J x J taps there and writes y in the sorted order (coalesced; complex64) or straight
+//    to the caller's order (complex128).
+//  * permute_to_sorted / permute_to_caller: the permutation between the caller's order and the sorted
+//    order: two LDS-staged passes that move whole cache lines (complex64), or one scatter (complex128).
 //  * grid: one wave per grid tile (tile-owner).  The tile's list holds every visibility with a tap
 //    inside the tile, in ascending sorted order.  The wave accumulates into its own LDS copy of the
 //    tile (no other wave touches it), then writes the tile once.  Every grid cell is written by
@@ -156,7 +157,11 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
   // this thread's first visibility: its loads go out before the tile staging so the latencies overlap
   uint32_t i = beg + threadIdx.x;
   VisCoord<R> c{};
-  if (i < end) c = vis_coord(v, i, Ju, Jv);
+  uint32_t dst = i;
+  if (i < end) {
+    c = vis_coord(v, i, Ju, Jv);
+    if (v.perm) dst = (uint32_t)v.perm[i];
+  }
   // stage the tile and its halo: each wave takes DG_ROWS rows per round, every load before any store
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = DG_THREADS / 64;
   constexpr int DG_ROWS = dg_rows<R>();
@@ -183,7 +188,11 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
     // the next visibility's loads go out before this one's arithmetic
     const uint32_t nxt = i + DG_THREADS;
     VisCoord<R> cn = c;
-    if (nxt < end) cn = vis_coord(v, nxt, Ju, Jv);
+    uint32_t dn = nxt;
+    if (nxt < end) {
+      cn = vis_coord(v, nxt, Ju, Jv);
+      if (v.perm) dn = (uint32_t)v.perm[nxt];
+    }
     R wu[JJU], wv[JJV];
     taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
     const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
@@ -204,8 +213,9 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
       acc.y += wv[b] * racc.y;
     }
     if (v.weights) acc = cmul(acc, v.weights[i]);
-    y[i] = scale(acc, scl);
+    y[dst] = scale(acc, scl);
     c = cn;
+    dst = dn;
   }
 }
 
@@ -418,9 +428,37 @@ constexpr int MB = 64;           // list entries per batch (a multiple of 16)
 constexpr int MQ = MB / 4;       // entries per MFMA row group of a batch
 constexpr int MFMA_NOTAP = 1024; // footprint offset of an unused entry slot: no row or column matches
 
-template <typename R, int JU, int JV>
+// One list entry of the gridding pipeline: coordinates, value and weight.
+template <typename R>
+struct GridEntry {
+  VisCoord<R> c{};
+  cpx<R> s{}, w{};
+};
+template <typename R>
+__device__ inline void load_entry(GridEntry<R>& e, const VisData<R>& v, const cpx<R>* __restrict__ yin, uint32_t i,
+                                  int Ju, int Jv) {
+  e.c = vis_coord(v, i, Ju, Jv);
+  e.s = yin[i];  // visibilities in sorted order
+  e.w = v.weights ? v.weights[i] : mk<R>(R(1), R(0));
+}
+// complex64: from an entry record (TileLists::ent); the wrapped first-tap cell keeps the tap phase
+// (-1)^(k + a), as the grid sides are even
+__device__ inline void load_entry(GridEntry<float>& e, const VisData<float>& v, const cpx<float>* __restrict__ yin,
+                                  const int4 r) {
+  e.c = VisCoord<float>{r.y & 0xffff, (int)((unsigned)r.y >> 16), __int_as_float(r.z), __int_as_float(r.w)};
+  e.s = yin[r.x];
+  e.w = v.weights ? v.weights[r.x] : mk<float>(1.0f, 0.0f);
+}
+
+// The entries of a batch are requested two batches ahead (complex64: one 16-byte record each, so a
+// lane's first batch waits for two dependent round trips, not three) and their values one batch
+// ahead.  TAB (complex64, J <= 8, tap tables present): the table rows of a batch are requested one
+// batch ahead too, while the previous batch's MFMAs run, so their latency (the rows' addresses depend
+// on the coordinates) is hidden.
+template <typename R, int JU, int JV, bool TAB_>
 __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, TileLists tl,
                                                   const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
+  constexpr bool TAB = TAB_ && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
   constexpr int T = MFMA_TILE;
   constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
   // Record windows [0, tap 0 .. tap J-1, 0 ...]: a row or column index clamped to [-1, J] selects a tap
@@ -443,28 +481,43 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
   acc_t cre = {R(0), R(0), R(0), R(0)}, cim = {R(0), R(0), R(0), R(0)};
   const uint32_t end = sl.z;
-  // Entry data of the batch being prepared (coordinates, value, weight) and the list index of the next
-  // batch's entry: while a batch's MFMAs run, the next batch's data and the list index two batches
-  // ahead are in flight.
-  VisCoord<R> c{};
-  cpx<R> s = mk<R>(R(0), R(0)), w = mk<R>(R(1), R(0));
+  // lane's entry of the batch being prepared (cur), of the next batch (nxt), and the list entry two
+  // batches ahead (complex64: its record; complex128: its index)
+  constexpr bool ENT = sizeof(R) == 4;
+  GridEntry<R> cur, nxt;
+  TabRow ta{}, tb{};  // TAB: table rows of cur
   uint32_t inext = 0;
-  const bool act = lane < MB;  // lanes that prepare one entry of each batch
-  if (act && sl.y + lane < end) {
-    const uint32_t i = tl.list[sl.y + lane];
-    c = vis_coord(v, i, Ju, Jv);
-    s = yin[i];  // visibilities in sorted order
-    if (v.weights) w = v.weights[i];
+  int4 enext{};
+  if constexpr (ENT) {
+    if (sl.y + lane < end) load_entry(cur, v, yin, tl.ent[sl.y + lane]);
+    if (sl.y + MB + lane < end) load_entry(nxt, v, yin, tl.ent[sl.y + MB + lane]);
+    if (sl.y + 2 * MB + lane < end) enext = tl.ent[sl.y + 2 * MB + lane];
+  } else {
+    if (sl.y + lane < end) load_entry(cur, v, yin, tl.list[sl.y + lane], Ju, Jv);
+    if (sl.y + MB + lane < end) load_entry(nxt, v, yin, tl.list[sl.y + MB + lane], Ju, Jv);
+    if (sl.y + 2 * MB + lane < end) inext = tl.list[sl.y + 2 * MB + lane];
   }
-  if (act && sl.y + MB + lane < end) inext = tl.list[sl.y + MB + lane];
+  if constexpr (TAB) {
+    if (sl.y + lane < end) {
+      ta = tab_row<JU>(g.tab_u, cur.c.fu);
+      tb = tab_row<JV>(table_v<JU, JV>(g), cur.c.fv);
+    }
+  }
   for (uint32_t base = sl.y; base < end; base += MB) {
     const int nb = (int)min((uint32_t)MB, end - base);
     if (lane < nb) {
-      if (v.weights) s = cmulc(s, w);  // conj(G) = phase * K_u * K_v * conj(weight)
+      cpx<R> s = cur.s;
+      if (v.weights) s = cmulc(s, cur.w);  // conj(G) = phase * K_u * K_v * conj(weight)
       s = scale(s, scl);
       R wu[JJU], wv[JJV];
-      taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
-      int dx = wrap(c.ku + 1 - tx0, g.Nu), dy = wrap(c.kv + 1 - ty0, g.Nv);
+      if constexpr (TAB) {
+        tab_interp<JU>(ta, cur.c.ku, wu);
+        tab_interp<JV>(tb, cur.c.kv, wv);
+      } else {
+        taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), nullptr, cur.c.ku, cur.c.fu, cur.c.kv, cur.c.fv, Ju, Jv, wu,
+                             wv);
+      }
+      int dx = wrap(cur.c.ku + 1 - tx0, g.Nu), dy = wrap(cur.c.kv + 1 - ty0, g.Nv);
       if (dx >= T) dx -= g.Nu;  // footprint starts left of / above the tile
       if (dy >= T) dy -= g.Nv;
       rd[(lane & 3) * MQ + (lane >> 2)] = (dx + 512) | ((dy + 512) << 16);
@@ -478,16 +531,25 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
         if (JU == 0 && e >= Ju) break;
         rb[lane * SU + 1 + e] = wu[e];
       }
-    } else if (act) {
+    } else {
       rd[(lane & 3) * MQ + (lane >> 2)] = (MFMA_NOTAP + 512) | ((MFMA_NOTAP + 512) << 16);
     }
     __syncthreads();
-    if (act && base + MB + lane < end) {
-      c = vis_coord(v, inext, Ju, Jv);
-      s = yin[inext];
-      if (v.weights) w = v.weights[inext];
+    // the next batch's table rows and the entries two batches ahead go out before this batch's MFMAs
+    if (base + MB + lane < end) {
+      if constexpr (TAB) {
+        ta = tab_row<JU>(g.tab_u, nxt.c.fu);
+        tb = tab_row<JV>(table_v<JU, JV>(g), nxt.c.fv);
+      }
+      cur = nxt;
     }
-    if (act && base + 2 * MB + lane < end) inext = tl.list[base + 2 * MB + lane];
+    if constexpr (ENT) {
+      if (base + 2 * MB + lane < end) load_entry(nxt, v, yin, enext);
+      if (base + 3 * MB + lane < end) enext = tl.ent[base + 3 * MB + lane];
+    } else {
+      if (base + 2 * MB + lane < end) load_entry(nxt, v, yin, inext, Ju, Jv);
+      if (base + 3 * MB + lane < end) inext = tl.list[base + 3 * MB + lane];
+    }
     // groups of four steps (16 entries; slots past nb hold no footprint): every LDS read of a group is
     // issued before its MFMAs
     const int ng = (nb + 15) >> 4;
@@ -556,9 +618,16 @@ template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm, const cpx<R>* y,
                 cpx<R>* grid, R scale, hipStream_t s, int variant) {
   if (variant == GRID_MFMA && tm.slot && grid_mfma_ok(g.Nu, g.Nv, g.Ju, g.Jv)) {
+    if (sizeof(R) == 4 && !tm.ent) throw Error(-5, "complex64 matrix-core gridding needs the entry records");
     const dim3 blocks((unsigned)(tm.ntu * tm.ntv));
     ProfScope prof("grid_tiles", s);
-#define PF_GM(A, B) hipLaunchKernelGGL((k_grid_mfma<R, A, B>), blocks, dim3(64), 0, s, g, v, tm, y, grid, scale)
+    // tap tables (complex64, equal supports <= 8): the pipelined table variant
+    const bool tab = sizeof(R) == 4 && g.tab_u && g.Ju == g.Jv && g.Ju <= 8;
+#define PF_GM(A, B)                                                                                            \
+  do {                                                                                                         \
+    if (tab && A > 0) hipLaunchKernelGGL((k_grid_mfma<R, A, B, true>), blocks, dim3(64), 0, s, g, v, tm, y, grid, scale); \
+    else hipLaunchKernelGGL((k_grid_mfma<R, A, B, false>), blocks, dim3(64), 0, s, g, v, tm, y, grid, scale);    \
+  } while (0)
     if (g.Ju == g.Jv && fixed_len_ok(g)) {
       switch (g.Ju) {
         case 2: PF_GM(2, 2); break;
@@ -630,12 +699,21 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
 }
 
 // ---------------------------------------------------------------------------------------------
-// Permutation between the caller's order and the sorted order (VisPerm).  A random 8-byte access
-// costs a whole cache line, so a one-pass scatter moves about 8x the data.  Each pass here moves a
-// block's chunk or bucket through LDS: the chunk pass writes (reads) one contiguous run per bucket,
-// the bucket pass writes (reads) its sorted range contiguously.  Every thread issues its loads
-// before any LDS store.
+// Permutation between the caller's order and the sorted order (VisPerm).  A random access costs one
+// memory transaction whatever its size up to a line.  The two-pass plan moves each block's chunk or
+// bucket through LDS: the chunk pass writes (reads) one contiguous run per bucket, the bucket pass
+// writes (reads) its sorted range contiguously.  Every thread issues its loads before any LDS store.
+// At M = 4e6: 8-byte values, scatter 52 us, two passes 28 us; 16-byte values, scatter 55 us, two
+// passes 82 us (twice the bytes, and 489 blocks of 128 KiB LDS take two rounds of the 256 CUs).
 // ---------------------------------------------------------------------------------------------
+// one-pass: ys[inv[m]] = y[m] (coalesced reads, scattered writes)
+template <typename R>
+__global__ void __launch_bounds__(256) k_permute_scatter(const cpx<R>* __restrict__ y, const int32_t* __restrict__ inv,
+                                                         cpx<R>* __restrict__ ys, int64_t M) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < M) ys[inv[m]] = y[m];
+}
+
 constexpr int PERM_THREADS = 1024;
 template <typename R>
 constexpr int perm_per_thread() { return (sizeof(R) == 4 ? 16384 : 8192) / PERM_THREADS; }
@@ -756,8 +834,14 @@ static void check_perm(const VisPerm& p) {
 template <typename R>
 void permute_to_sorted(const VisPerm& p, const cpx<R>* y, cpx<R>* tmp, cpx<R>* ys, hipStream_t s) {
   if (p.M == 0) return;
-  check_perm<R>(p);
   ProfScope prof("permute_to_sorted", s);
+  if (!p.two_pass) {
+    if (!p.inv) throw Error(-5, "one-pass permutation without its index");  // PF_ERR_STATE
+    hipLaunchKernelGGL((k_permute_scatter<R>), dim3((unsigned)((p.M + 255) / 256)), dim3(256), 0, s, y, p.inv, ys, p.M);
+    PF_HIP(hipGetLastError());
+    return;
+  }
+  check_perm<R>(p);
   const size_t shm = sizeof(cpx<R>) * (size_t)p.S;
   allow_lds((const void*)k_perm_chunks<R, true>, shm);
   allow_lds((const void*)k_perm_buckets<R, true>, shm);
@@ -769,6 +853,7 @@ void permute_to_sorted(const VisPerm& p, const cpx<R>* y, cpx<R>* tmp, cpx<R>* y
 template <typename R>
 void permute_to_caller(const VisPerm& p, const cpx<R>* ys, cpx<R>* tmp, cpx<R>* y, hipStream_t s) {
   if (p.M == 0) return;
+  if (!p.two_pass) throw Error(-5, "permute_to_caller needs the two-pass plan");  // PF_ERR_STATE
   check_perm<R>(p);
   ProfScope prof("permute_to_caller", s);
   const size_t shm = sizeof(cpx<R>) * (size_t)p.S;

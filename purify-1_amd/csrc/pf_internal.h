@@ -64,6 +64,8 @@ struct Fft2dPlan {
   void* tw = nullptr;  // device table exp(-2 pi i m / Lmax), m < Lmax, in the plan's precision
   LinePlan row;        // N = Nu
   int row_lpb = 1;     // lines per block of the row pass
+  int row_bpc = 0;     // row pass grid: row_bpc blocks per CU, each looping over lines (0: one pass)
+  int n_cu = 256;      // compute units of the device
   bool col_two_step = false;
   bool col_big = false;  // single pass over columns longer than 512 rows
   int big_C = 4;         // columns per block of that pass
@@ -114,22 +116,31 @@ struct VisData {
   const cpx<float>* ff = nullptr;
   const cpx<double>* uv = nullptr;
   const cpx<R>* weights = nullptr;  // nullptr: unit weights
+  const int32_t* perm = nullptr;    // degridding writes y[perm[i]] (caller's order); nullptr: y[i] (sorted)
   int64_t M = 0;
 };
 
-// The permutation between the caller's order and the sorted order, applied in two passes that each
-// move whole cache lines.  The caller's order is cut into chunks and the sorted order into buckets of
-// S visibilities each (nb of both).  A staging vector holds bucket b in [b*S, b*S + size of b), and
-// inside it the visibilities grouped by chunk, each group in the caller's order: so a chunk sends one
-// contiguous run to each bucket, and a bucket gathers one run from each chunk.
+// The permutation between the caller's order and the sorted order.  A random access costs one memory
+// transaction whatever its size up to a cache line, so a one-pass scatter of 8-byte values is bound by
+// the transaction count; complex64 therefore goes through two passes that each move whole lines, and
+// complex128 (16-byte values, half the transactions per byte) uses the one-pass scatter.
+//
+// Two-pass plan (two_pass): the caller's order is cut into chunks and the sorted order into buckets
+// of S visibilities each (nb of both).  A staging vector holds bucket b in [b*S, b*S + size of b),
+// and inside it the visibilities grouped by chunk, each group in the caller's order: so a chunk sends
+// one contiguous run to each bucket, and a bucket gathers one run from each chunk.
 //   chunk pass:  one block stages its chunk in LDS ordered by bucket, then writes (reads) the runs.
 //   bucket pass: one block stages its bucket in LDS, then writes (reads) the sorted range coalesced.
+// One-pass plan: ys[inv[m]] = y[m], and degridding writes y[perm[i]] itself.
 struct VisPerm {
+  bool two_pass = false;
   const uint16_t* l1 = nullptr;  // caller index m -> position inside chunk m / S, ordered by bucket
   const int32_t* g1 = nullptr;   // (chunk c, position j) at c * S + j -> staging index
   const uint16_t* p2 = nullptr;  // staging index t -> position inside bucket t / S of the sorted order
   int S = 0;                     // chunk and bucket size, a power of two <= perm_max_S(precision)
   int nb = 0;                    // ceil(M / S)
+  const int32_t* perm = nullptr;  // one-pass: sorted position -> caller's index
+  const int32_t* inv = nullptr;   // one-pass: caller's index -> sorted position
   int64_t M = 0;
 };
 // largest chunk: 128 KiB of LDS
@@ -166,9 +177,13 @@ struct TileLists {
   // tap lies in the tile, for the n_own tiles that own any, heaviest first
   const uint4* slot_own = nullptr;
   int n_own = 0;
+  // complex64 matrix-core lists: each list entry as one record (sorted index, first-tap cell
+  // (k_u mod Nu) | (k_v mod Nv) << 16, the fractions u - k_u and v - k_v as float bits)
+  const int4* ent = nullptr;
 };
 
-// y = G grid with y in the operator's sorted order (permute_to_caller brings it to the caller's)
+// y = G grid, written y[v.perm[i]] in the caller's order, or y[i] in the sorted order when v.perm is null
+// (permute_to_caller then brings it to the caller's order)
 template <typename R>
 void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y_sorted,
             R scale, hipStream_t s);
@@ -184,7 +199,8 @@ enum GridVariant { GRID_RESIDUE = 0, GRID_ATOMIC = 1, GRID_MFMA = 2 };
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm,
                 const cpx<R>* y_sorted, cpx<R>* grid, R scale, hipStream_t s, int variant);
-// caller's order -> sorted order (ys) and back, through the staging vector `tmp` (M values)
+// caller's order -> sorted order (ys) and back; the two-pass plan stages through `tmp` (M values).
+// permute_to_caller needs the two-pass plan (the one-pass plan's degridding writes the caller's order).
 template <typename R>
 void permute_to_sorted(const VisPerm& p, const cpx<R>* y, cpx<R>* tmp, cpx<R>* ys, hipStream_t s);
 template <typename R>
@@ -201,9 +217,10 @@ struct WRows {
   const int2* kw = nullptr;      // (kw_u, kw_v) per sorted visibility
   const cpx<R>* vals = nullptr;  // M * Kv * Ku
 };
-// y = G grid in the sorted order: owner tiles staged with a (K - 1)-wide halo, one wave per visibility
+// y = G grid: owner tiles staged with a (K - 1)-wide halo, one wave per visibility; y[perm[i]] in the
+// caller's order, or y[i] in the sorted order when perm is null
 template <typename R>
-void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx<R>* grid,
+void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const TileLists& t, const cpx<R>* grid,
                   cpx<R>* y, R scale, hipStream_t s);
 // grid = G^H ys (ys in sorted order): one wave per tile, the taps of one visibility at a time
 template <typename R>

@@ -293,6 +293,8 @@ struct pf_op {
   uint16_t* d_perm_l1 = nullptr;
   int32_t* d_perm_g1 = nullptr;
   uint16_t* d_perm_p2 = nullptr;
+  int32_t* d_perm = nullptr;   // one-pass plan: sorted -> caller, caller -> sorted
+  int32_t* d_inv = nullptr;
   void* d_ysort = nullptr;     // visibilities in sorted order (degridding output, gridding input)
   void* d_ystage = nullptr;    // staging vector of the two-pass permutation
   float* d_tab_u = nullptr;  // complex64 tap tables
@@ -311,6 +313,7 @@ struct pf_op {
   int grid_variant = GRID_RESIDUE;  // tuning knob: gridding kernel
   int kernel_table = 0;             // tuning knob: 0 no tap tables, 1 in gridding, 2 in gridding and degridding
   int fft_col_mode = 0;             // tuning knob: column FFT in one pass (0) or two steps (1)
+  int fft_row_bpc = -1;             // tuning knob: row pass blocks per CU (0: one line per block; -1: plan default)
   int64_t device_bytes = 0;
   std::vector<void*> allocs;
 
@@ -357,6 +360,20 @@ bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
 int wrapi(int64_t a, int n) {
   int64_t r = a % n;
   return (int)(r < 0 ? r + n : r);
+}
+
+// f(begin, end) over [0, n) in contiguous slices on up to 16 host threads (each index in one slice)
+template <class F>
+void parallel_for(int64_t n, F f) {
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
+                                                                (n + 65535) / 65536}));
+  if (nth == 1) {
+    f(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int k = 0; k < nth; ++k) pool.emplace_back([&, k] { f(n * k / nth, n * (k + 1) / nth); });
+  for (auto& th : pool) th.join();
 }
 
 // Gridding lists of a TU x TV tiling: for each tile, the sorted index of every visibility with a tap
@@ -442,21 +459,32 @@ HostLists tile_lists(const std::vector<int32_t>& perm, const std::vector<int64_t
   return h;
 }
 
-// The two-pass permutation plan (VisPerm) for perm: sorted position -> caller's index.  Staging layout:
-// bucket b (sorted positions [b*S, b*S + S)) occupies staging [b*S, ...), grouped by the caller's chunk
-// (chunk c = caller indices [c*S, c*S + S)), each group in the caller's order.  Chunk c's LDS order is
+// The permutation plan (VisPerm) for perm: sorted position -> caller's index.  complex128: the
+// one-pass plan (perm and its inverse).  complex64: the two-pass plan, whose staging layout puts
+// bucket b (sorted positions [b*S, b*S + S)) at staging [b*S, ...), grouped by the caller's chunk
+// (chunk c = caller indices [c*S, c*S + S)), each group in the caller's order; chunk c's LDS order is
 // by bucket, then by caller index.
 void build_perm(pf_op* op, const std::vector<int32_t>& perm) {
   const int64_t M = op->M;
   VisPerm& p = op->vperm;
   p.M = M;
+  std::vector<int32_t> inv(M);
+  for (int64_t i = 0; i < M; ++i) inv[perm[i]] = (int32_t)i;
+  p.two_pass = op->prec == PF_C64;
+  if (!p.two_pass) {
+    op->d_perm = (int32_t*)op->dalloc(sizeof(int32_t) * M);
+    op->d_inv = (int32_t*)op->dalloc(sizeof(int32_t) * M);
+    PF_HIP(hipMemcpy(op->d_perm, perm.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(op->d_inv, inv.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
+    p.perm = op->d_perm;
+    p.inv = op->d_inv;
+    return;
+  }
   const int Smax = perm_max_S((int)op->esz());
   p.S = 64;
   while (p.S < M && p.S < Smax) p.S <<= 1;
   p.nb = (int)((M + p.S - 1) / p.S);
   const int S = p.S, nb = p.nb;
-  std::vector<int32_t> inv(M);
-  for (int64_t i = 0; i < M; ++i) inv[perm[i]] = (int32_t)i;
   std::vector<uint16_t> l1(M), p2(M);
   std::vector<int32_t> g1(M);
   std::vector<int64_t> run(nb);  // next staging index of each bucket
@@ -710,7 +738,7 @@ void build(pf_op* op, const pf_op_desc& d) {
   // ---- upload ----
   build_perm(op, perm);
   op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
-  op->d_ystage = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
+  if (op->vperm.two_pass) op->d_ystage = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
   upload_lists(op, lists, TU, TV, op->tl);
   {
     // degridding launch slots: the tiles that own visibilities, most visibilities first
@@ -730,6 +758,26 @@ void build(pf_op* op, const pf_op_desc& d) {
     upload_lists(op, lm, MFMA_TILE, MFMA_TILE, op->tlm);
     op->tile_list_len_m = (int64_t)lm.list.size();
     op->grid_variant = GRID_MFMA;
+    if (op->prec == PF_C64) {
+      // entry records: sorted index, wrapped first-tap cell, fractions (as the kk / ff arrays hold them)
+      const size_t ne = lm.list.size();
+      std::vector<int4> ent(std::max<size_t>(ne, 1));
+      parallel_for((int64_t)ne, [&](int64_t a, int64_t b) {
+        for (int64_t p = a; p < b; ++p) {
+          const int32_t i = (int32_t)lm.list[p];
+          const int64_t m = perm[i];
+          const float fu = (float)(u[m] - (double)ku[m]), fv = (float)(v[m] - (double)kv[m]);
+          const int cu = wrapi(ku[m], op->Nu), cv = wrapi(kv[m], op->Nv);
+          int fub, fvb;
+          std::memcpy(&fub, &fu, 4);
+          std::memcpy(&fvb, &fv, 4);
+          ent[p] = make_int4(i, cu | (cv << 16), fub, fvb);
+        }
+      });
+      int4* d_ent = (int4*)op->dalloc(sizeof(int4) * ent.size());
+      PF_HIP(hipMemcpy(d_ent, ent.data(), sizeof(int4) * ent.size(), hipMemcpyHostToDevice));
+      op->tlm.ent = d_ent;
+    }
   }
   if (op->w_term) {
     // row origins only: the values hold the kernel, the phase and the weight
@@ -886,6 +934,7 @@ VisData<R> visdata(const pf_op* op) {
   v.ff = op->d_ff;
   v.uv = op->d_uv;
   v.weights = static_cast<const cpx<R>*>(op->d_weights);
+  v.perm = op->vperm.two_pass ? nullptr : op->vperm.perm;
   v.M = op->M;
   return v;
 }
@@ -956,10 +1005,12 @@ void G_dev(pf_op* op, const cpx<R>* grid, cpx<R>* y, hipStream_t s) {
     csr_gather<R>(op->M, op->d_csr_ptr, op->d_csr_col, static_cast<const cpx<R>*>(op->d_csr_val), grid, y, R(1), s,
                   "degrid_csr");
   else {
-    cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
-    if (op->w_term) degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, grid, ys, R(1), s);
-    else degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, grid, ys, R(1), s);
-    permute_to_caller<R>(op->vperm, ys, static_cast<cpx<R>*>(op->d_ystage), y, s);
+    // complex64: written in the sorted order, then permuted; complex128: written in the caller's order
+    const bool two = op->vperm.two_pass;
+    cpx<R>* out = two ? static_cast<cpx<R>*>(op->d_ysort) : y;
+    if (op->w_term) degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), two ? nullptr : op->vperm.perm, op->tl, grid, out, R(1), s);
+    else degrid<R>(degrid_geom<R>(op), visdata<R>(op), op->tl, grid, out, R(1), s);
+    if (two) permute_to_caller<R>(op->vperm, out, static_cast<cpx<R>*>(op->d_ystage), y, s);
   }
 }
 // grid = G^H y, y in the caller's order
@@ -1223,8 +1274,15 @@ int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
     return guarded([&] {
       fft_plan_free(op->fft);
       fft_plan_init(op->fft, op->Nv, op->Nu, op->prec, (int)value);
+      if (op->fft_row_bpc >= 0) op->fft.row_bpc = op->fft_row_bpc;
       op->fft_col_mode = (int)value;
     });
+  }
+  if (k == "fft_row_blocks") {
+    if (value < 0 || value > 64) return fail(PF_ERR_ARG, "fft_row_blocks must be 0..64");
+    op->fft_row_bpc = (int)value;
+    op->fft.row_bpc = (int)value;
+    return PF_OK;
   }
   if (k == "kernel_table") {
     if (value < 0 || value > 2) return fail(PF_ERR_ARG, "kernel_table must be 0, 1 or 2");
@@ -1242,6 +1300,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   const std::string k(key);
   if (k == "grid_kernel") {
     *value = op->grid_variant;
+    return PF_OK;
+  }
+  if (k == "fft_row_blocks") {
+    *value = op->fft.row_bpc;
     return PF_OK;
   }
   if (k == "fft_columns") {
@@ -1310,6 +1372,7 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     for (int mode : {0, 1}) {
       fft_plan_free(op->fft);
       fft_plan_init(op->fft, op->Nv, op->Nu, op->prec, mode);
+      if (op->fft_row_bpc >= 0) op->fft.row_bpc = op->fft_row_bpc;
       auto run = [&] {
         if (op->prec == PF_C64) {
           fft_cols<float>(op->fft, false, (cpx<float>*)op->d_gridA, (cpx<float>*)op->d_gridB,
@@ -1338,6 +1401,7 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     if (best_m != 1) {
       fft_plan_free(op->fft);
       fft_plan_init(op->fft, op->Nv, op->Nu, op->prec, best_m);
+      if (op->fft_row_bpc >= 0) op->fft.row_bpc = op->fft_row_bpc;
     }
     op->fft_col_mode = best_m;
     (void)hipEventDestroy(a);

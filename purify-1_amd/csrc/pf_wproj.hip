@@ -91,7 +91,8 @@ constexpr int WD_THREADS = 256;
 constexpr int WD_ROWS = 4;  // staged rows per wave per round
 
 template <typename R>
-__global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRows<R> w, TileLists tl, const cpx<R>* __restrict__ grid,
+__global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRows<R> w, const int32_t* __restrict__ perm,
+                                                             TileLists tl, const cpx<R>* __restrict__ grid,
                                                              cpx<R>* __restrict__ y, R scl) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -131,7 +132,7 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRo
       acc.y += a.x * g.y + a.y * g.x;
     }
     acc = wave_sum(acc);
-    if (lane == 0) y[i] = scale(acc, scl);
+    if (lane == 0) y[perm ? perm[i] : i] = scale(acc, scl);
   }
 }
 
@@ -174,14 +175,14 @@ __global__ void __launch_bounds__(64) k_grid_wrows(int Nu, int Nv, WRows<R> w, T
 }
 
 template <typename R>
-void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx<R>* grid,
+void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const TileLists& t, const cpx<R>* grid,
                   cpx<R>* y, R scale, hipStream_t s) {
   if (t.n_own == 0) return;
   const size_t shm = sizeof(cpx<R>) * (size_t)(t.TU + w.Ku - 1) * (t.TV + w.Kv - 1);
   if (shm > 65536) throw Error(PF_ERR_UNSUPPORTED, "w-projection degridding tile and halo exceed 64 KiB of LDS");
   ProfScope prof("degrid_wrows", s);
-  hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, t, grid, y,
-                     scale);
+  hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, grid,
+                     y, scale);
   PF_HIP(hipGetLastError());
 }
 
@@ -195,10 +196,10 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
   PF_HIP(hipGetLastError());
 }
 
-template void degrid_wrows<float>(int, int, const WRows<float>&, const TileLists&, const cpx<float>*, cpx<float>*,
-                                  float, hipStream_t);
-template void degrid_wrows<double>(int, int, const WRows<double>&, const TileLists&, const cpx<double>*,
-                                   cpx<double>*, double, hipStream_t);
+template void degrid_wrows<float>(int, int, const WRows<float>&, const int32_t*, const TileLists&, const cpx<float>*,
+                                  cpx<float>*, float, hipStream_t);
+template void degrid_wrows<double>(int, int, const WRows<double>&, const int32_t*, const TileLists&,
+                                   const cpx<double>*, cpx<double>*, double, hipStream_t);
 template void grid_wrows<float>(int, int, const WRows<float>&, const TileLists&, const cpx<float>*, cpx<float>*, float,
                                 hipStream_t);
 template void grid_wrows<double>(int, int, const WRows<double>&, const TileLists&, const cpx<double>*, cpx<double>*,

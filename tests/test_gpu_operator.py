@@ -297,3 +297,22 @@ def test_permutation_chunk_and_bucket_boundaries(precision, M):
     assert rel(gpu * x, orc.forward(x)) < TOL[precision]
     y = utilities.synth_cnormal(84, M)
     assert rel(gpu.adjoint() * y, orc.adjoint(y)) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_row_pass_grid_sizes_give_identical_results(precision):
+    # each line's arithmetic does not depend on which block transforms it or on the prefetch
+    n, M = 512, 20_000
+    u, v = make_uv(91, M, n)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", 4, 4,
+                                                     precision=precision)
+    assert lt.op.get_tuning("fft_row_blocks") == (4 if precision == "c64" else 2)
+    x = utilities.synth_image(92, n, n, 20)
+    y = utilities.synth_cnormal(93, M)
+    ref = (lt * x, lt.adjoint() * y)
+    for rb in (0, 1, 3, 64):
+        lt.op.set_tuning("fft_row_blocks", rb)
+        assert lt.op.get_tuning("fft_row_blocks") == rb
+        assert np.array_equal(lt * x, ref[0]) and np.array_equal(lt.adjoint() * y, ref[1])
+    with pytest.raises(PurifyError):
+        lt.op.set_tuning("fft_row_blocks", 65)
