@@ -61,7 +61,9 @@ def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, KK=None):
         "degrid": Ng * e + vis,
         "grid_tiles": vis + Ng * e,
         "fft_cols_stepB": 2 * Ng * e,
-        "fft_cols": 2 * Ng * e,
+        "fft_cols": (Ng + win) * e,  # either mode: the window rows one way, the full grid the other
+        "permute_to_sorted": 2 * M * e,
+        "permute_to_caller": 2 * M * e,
     }
     if name == "fft_cols_stepA":
         return None  # depends on direction (window read forward, full read adjoint): averaged below

@@ -86,6 +86,27 @@ int pf_op_forward(pf_op *op, const void *x, void *y, int x_loc, int y_loc, pf_st
 /* x = Phi^dagger y (operators.h:394, the adjoint operator). */
 int pf_op_adjoint(pf_op *op, const void *y, void *x, int y_loc, int x_loc, pf_stream stream);
 
+/* out = Phi^dagger Phi x: operators::init_grid_degrid_operator_2d (compact_operators.h:150-171, built
+ * by base_grid_degrid_operator_2d, 118-147), the self-adjoint grid-degrid operator divided by the
+ * square of the measurement operator's norm (that function runs the same power iteration on
+ * Phi^dagger Phi).  The reference applies an explicit G^H G pruned of entries with |value| <= 1e-7
+ * (compact_operators.h:109-111); this applies G then G^H exactly.  x and out: imsizey*imsizex
+ * (complex128 on the host, or the operator's precision on the device); out may alias x. */
+int pf_op_apply_normal(pf_op *op, const void *x, void *out, int x_loc, int out_loc, pf_stream stream);
+
+/* ---- PSF convolution (operators::init_psf_convolve_2d, compact_operators.h:279-306) ----
+ * psf = Phi^dagger Phi delta (the given measurement operator, delta at row floor(ny/2) - 1, column
+ * floor(nx/2) - 1); the operator is x -> IFFT(FFT(psf) . FFT(x)) with unitary n x n FFTs (a circular
+ * convolution on the unpadded image), divided by the square of its own power-method norm.  start:
+ * host complex128 power-method start vector, or NULL for a fixed pseudo-random one (uniform in
+ * [-1, 1] per component, mt19937_64 seed 0x5eed; the reference draws T::Random).  Image sides must be
+ * powers of two.  op must outlive the handle; applies share op's host staging buffers. */
+typedef struct pf_psf pf_psf;
+int pf_psf_create(pf_op *op, int32_t power_iters, double power_tol, const double *start, pf_psf **out);
+int pf_psf_apply(pf_psf *psf, const void *x, void *out, int x_loc, int out_loc, pf_stream stream);
+int pf_psf_norm(const pf_psf *psf, double *norm);
+int pf_psf_destroy(pf_psf *psf);
+
 /* Power method on Phi^dagger Phi (SOPT power_method as called at operators.h:390-391) over the
  * operator as it currently is (including its norm).  start: host complex128 of imsizey*imsizex,
  * or NULL for all ones.  Iterates in the operator's precision, reduces norms in FP64. */
@@ -134,7 +155,9 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * (0 = the column FFT in one pass up to 8192 rows, the default; 1 = two passes of N1 x N2 beyond 512
  * rows); "fft_row_blocks" (row FFT grid: that many resident blocks per compute unit, each looping over
  * lines with the next line prefetched; 0 = one line per block; default 4 for complex64, 2 for
- * complex128); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond
+ * complex128); "grid_waves" (matrix-core gridding with that many waves, each taking every
+ * grid_waves-th 16 x 16 tile with its batches pipelined across tiles; 0 = one wave per tile, the
+ * default); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond
  * floating-point summation order. */
 int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
 int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);

@@ -370,6 +370,69 @@ class Operator:
         return r
 
 
+def power_method_fn(apply, start, iters, tol):
+    """SOPT power_method on a LinearTransform {A, identity} (as the compact operators call it,
+    compact_operators.h:164-165 and 298-299): x <- A x / ||A x||, stopping when
+    |lambda_k - lambda_(k-1)| / lambda_(k-1) < tol^2; returns sqrt(lambda).  The same iteration as
+    orc_op_power_method; its stopping rule is parity unpinned (SOPT is not in the reference tree)."""
+    if iters <= 0:
+        return 1.0
+    x = np.asarray(start, np.complex128).copy()
+    x /= np.linalg.norm(x)
+    old = 0.0
+    for _ in range(iters):
+        x = apply(x)
+        val = float(np.linalg.norm(x))
+        if not val > 0:
+            old = val
+            break
+        x /= val
+        if old > 0 and tol * tol > abs(old - val) / old:
+            old = val
+            break
+        old = val
+    return float(np.sqrt(old))
+
+
+def gtg_matrix(G: CSR, prune=1e-7):
+    """G^H G with entries of |value| <= prune removed (compact_operators.h:104-113, Eigen's
+    prune with the reference's 1e-7 threshold), as a scipy CSR matrix."""
+    import scipy.sparse as sp
+    Gs = sp.csr_matrix((G.vals, G.colidx, G.rowptr), shape=(G.rows, G.cols))
+    GTG = (Gs.conj().T @ Gs).tocsr()
+    if prune is not None and prune > 0:
+        GTG.data[np.abs(GTG.data) <= prune] = 0
+        GTG.eliminate_zeros()
+    return GTG
+
+
+def grid_degrid(u_px, v_px, weights, ny, nx, os_, x, kind=KB, Ju=4, Jv=4, norm=1.0, prune=1e-7):
+    """init_grid_degrid_operator_2d's operator (compact_operators.h:118-171) on x:
+    Z^dagger F^dagger (G^H G) F Z S x / norm^2 with the explicit, pruned G^H G; G built with the
+    kernel-slot swap of compact_operators.h:141-142 (the u-axis evaluates the Jv kernel)."""
+    M = len(u_px)
+    weights = np.ones(M, np.complex128) if weights is None else np.asarray(weights, np.complex128)
+    G = gridding_matrix(u_px, v_px, weights, ny, nx, os_, kind, Jv, Ju, Ju, Jv)
+    GTG = gtg_matrix(G, prune)
+    S = correction2d(os_, ny, nx, kind, Ju, Jv)
+    Nv, Nu = grid_dims(ny, nx, os_)
+    g = fft2d(Nv, Nu, zero_pad(S, ny, nx, os_, x))
+    g = GTG @ g
+    return zero_pad_adjoint(S, ny, nx, os_, fft2d(Nv, Nu, g, sign=+1)) / (norm * norm)
+
+
+def psf_convolve(op: "Operator", x, psf_norm=1.0):
+    """init_psf_convolve_2d's operator (compact_operators.h:279-306) on x: psf = Phi^dagger Phi delta
+    with delta at sub2ind(floor(ny/2) - 1, floor(nx/2) - 1) (utilities.cc:321-333: row * cols + col),
+    then IFFT(FFT(psf) . FFT(x)) with unitary n x n FFTs, divided by psf_norm^2."""
+    ny, nx = op.ny, op.nx
+    delta = np.zeros(ny * nx, np.complex128)
+    delta[(int(np.floor(ny * 0.5)) - 1) * nx + int(np.floor(nx * 0.5)) - 1] = 1.0
+    psf = op.adjoint(op.forward(delta))
+    ft = fft2d(ny, nx, psf)
+    return fft2d(ny, nx, ft * fft2d(ny, nx, x), sign=+1) / (psf_norm * psf_norm)
+
+
 def omp_threads():
     return lib().orc_omp_threads()
 

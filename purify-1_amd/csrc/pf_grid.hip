@@ -140,10 +140,14 @@ constexpr int DG_THREADS = 256;
 template <typename R>
 constexpr int dg_rows() { return sizeof(R) == 4 ? 10 : 5; }
 
-template <typename R, int JU, int JV>
+// TAB (complex64, J <= 8, tap tables present): the taps come from the tables, whose rows (addressed by
+// the coordinates) are requested one visibility ahead and the coordinates two ahead, so neither round
+// trip is waited for in the loop; otherwise the kernel is evaluated directly.
+template <typename R, int JU, int JV, bool TAB_>
 __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisData<R> v, TileLists tl,
                                                              const cpx<R>* __restrict__ grid, cpx<R>* __restrict__ y,
                                                              R scl) {
+  constexpr bool TAB = TAB_ && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const uint4 sl = tl.slot_own[blockIdx.x];
@@ -156,12 +160,19 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
   const int W = tl.TU + Ju - 1, H = tl.TV + Jv - 1;
   // this thread's first visibility: its loads go out before the tile staging so the latencies overlap
   uint32_t i = beg + threadIdx.x;
-  VisCoord<R> c{};
+  VisCoord<R> c{}, c2{};  // this visibility's and (TAB) the one after next's coordinates
+  TabRow ta{}, tb{};      // TAB: table rows of the visibility being computed
   uint32_t dst = i;
   if (i < end) {
     c = vis_coord(v, i, Ju, Jv);
     if (v.perm) dst = (uint32_t)v.perm[i];
+    if constexpr (TAB) {
+      ta = tab_row<JU>(g.tab_u, c.fu);
+      tb = tab_row<JV>(table_v<JU, JV>(g), c.fv);
+    }
   }
+  if constexpr (TAB)
+    if (i + DG_THREADS < end) c2 = vis_coord(v, i + DG_THREADS, Ju, Jv);
   // stage the tile and its halo: each wave takes DG_ROWS rows per round, every load before any store
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = DG_THREADS / 64;
   constexpr int DG_ROWS = dg_rows<R>();
@@ -189,12 +200,24 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
     const uint32_t nxt = i + DG_THREADS;
     VisCoord<R> cn = c;
     uint32_t dn = nxt;
-    if (nxt < end) {
-      cn = vis_coord(v, nxt, Ju, Jv);
-      if (v.perm) dn = (uint32_t)v.perm[nxt];
-    }
     R wu[JJU], wv[JJV];
-    taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
+    if constexpr (TAB) {
+      tab_interp<JU>(ta, c.ku, wu);
+      tab_interp<JV>(tb, c.kv, wv);
+      if (nxt < end) {
+        cn = c2;
+        ta = tab_row<JU>(g.tab_u, cn.fu);
+        tb = tab_row<JV>(table_v<JU, JV>(g), cn.fv);
+        if (v.perm) dn = (uint32_t)v.perm[nxt];
+      }
+      if (nxt + DG_THREADS < end) c2 = vis_coord(v, nxt + DG_THREADS, Ju, Jv);
+    } else {
+      if (nxt < end) {
+        cn = vis_coord(v, nxt, Ju, Jv);
+        if (v.perm) dn = (uint32_t)v.perm[nxt];
+      }
+      taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
+    }
     const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
     const cpx<R>* base = lds + ly0 * W + lx0;
     cpx<R> acc = mk<R>(R(0), R(0));
@@ -442,23 +465,42 @@ __device__ inline void load_entry(GridEntry<R>& e, const VisData<R>& v, const cp
   e.w = v.weights ? v.weights[i] : mk<R>(R(1), R(0));
 }
 // complex64: from an entry record (TileLists::ent); the wrapped first-tap cell keeps the tap phase
-// (-1)^(k + a), as the grid sides are even
+// (-1)^(k + a), as the grid sides are even.  The record holds the byte offset of the visibility's
+// value (8 * sorted index, below 2^32): a 32-bit offset register, so the loop-carried record needs no
+// 64-bit address pair (whose register copy would make the wave wait for the record's load).
 __device__ inline void load_entry(GridEntry<float>& e, const VisData<float>& v, const cpx<float>* __restrict__ yin,
                                   const int4 r) {
+  const uint32_t off = (uint32_t)r.x;
   e.c = VisCoord<float>{r.y & 0xffff, (int)((unsigned)r.y >> 16), __int_as_float(r.z), __int_as_float(r.w)};
-  e.s = yin[r.x];
-  e.w = v.weights ? v.weights[r.x] : mk<float>(1.0f, 0.0f);
+  e.s = *reinterpret_cast<const cpx<float>*>(reinterpret_cast<const char*>(yin) + off);
+  e.w = v.weights ? *reinterpret_cast<const cpx<float>*>(reinterpret_cast<const char*>(v.weights) + off)
+                  : mk<float>(1.0f, 0.0f);
 }
 
-// The entries of a batch are requested two batches ahead (complex64: one 16-byte record each, so a
-// lane's first batch waits for two dependent round trips, not three) and their values one batch
-// ahead.  TAB (complex64, J <= 8, tap tables present): the table rows of a batch are requested one
-// batch ahead too, while the previous batch's MFMAs run, so their latency (the rows' addresses depend
-// on the coordinates) is hidden.
+// Position of a batch in a wave's stream: the tile's launch slot and the batch's first list entry.
+// Wave w takes the launch slots w, w + W, w + 2W, ... (W waves; heaviest tiles first), each as
+// ceil(n / MB) batches of its list, or one empty batch for a tile without entries (its cells are
+// written as zeros).  tile < 0: past the end of the stream.
+struct BatchPos {
+  int tile;
+  uint32_t base, end;
+};
+__device__ inline BatchPos slot_pos(const TileLists& tl, int j, int nslots) {
+  if (j >= nslots) return BatchPos{-1, 0u, 0u};
+  const uint4 sl = tl.slot[j];
+  return BatchPos{(int)sl.x, sl.y, sl.z};
+}
+
+// The wave's stream is software-pipelined across batches and tiles: the list entries of a batch are
+// requested three batches ahead (complex64: one 16-byte record each), their values two ahead, and
+// (TAB: complex64, J <= 8, tap tables present) the tap-table rows one ahead, while the previous
+// batch's MFMAs run; the rows' addresses depend on the coordinates, so this hides a dependent round
+// trip per batch.  A tile's 16 x 16 sums stay in the MFMA accumulators until its last batch.
 template <typename R, int JU, int JV, bool TAB_>
-__global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, TileLists tl,
+__global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, TileLists tl, int nslots,
                                                   const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
   constexpr bool TAB = TAB_ && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
+  constexpr bool ENT = sizeof(R) == 4;
   constexpr int T = MFMA_TILE;
   constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
   // Record windows [0, tap 0 .. tap J-1, 0 ...]: a row or column index clamped to [-1, J] selects a tap
@@ -474,37 +516,47 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
   const int lane = threadIdx.x, li = lane & 15, lt = lane >> 4;
   for (int q = lane; q < MB * SV; q += 64) ra[q] = mk<R>(R(0), R(0));  // borders stay zero
   for (int q = lane; q < MB * SU; q += 64) rb[q] = R(0);
-  const uint4 sl = tl.slot[blockIdx.x];
-  const int tile = (int)sl.x;
-  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
-  const int tx0 = tu * T, ty0 = tv * T;
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
-  acc_t cre = {R(0), R(0), R(0), R(0)}, cim = {R(0), R(0), R(0), R(0)};
-  const uint32_t end = sl.z;
-  // lane's entry of the batch being prepared (cur), of the next batch (nxt), and the list entry two
-  // batches ahead (complex64: its record; complex128: its index)
-  constexpr bool ENT = sizeof(R) == 4;
+  const int W = (int)gridDim.x;
+  // batch positions: p0 being prepared and multiplied, p1 / p2 / p3 the next three; j3 is p3's slot
+  int j3 = (int)blockIdx.x;
+  auto advance = [&](BatchPos p) -> BatchPos {
+    if (p.tile < 0) return p;
+    if (p.base + MB < p.end) return BatchPos{p.tile, p.base + MB, p.end};
+    j3 += W;
+    return slot_pos(tl, j3, nslots);
+  };
+  const BatchPos p0i = slot_pos(tl, j3, nslots);
+  BatchPos p0 = p0i;
+  BatchPos p1 = advance(p0);
+  BatchPos p2 = advance(p1);
+  BatchPos p3 = advance(p2);
+  auto has = [&](const BatchPos& p) { return p.tile >= 0 && p.base + lane < p.end; };
+  // lane's entry of batch p0 (cur) and of p1 (nxt), and the list entry of p2 (complex64: its record;
+  // complex128: its index)
   GridEntry<R> cur, nxt;
   TabRow ta{}, tb{};  // TAB: table rows of cur
   uint32_t inext = 0;
   int4 enext{};
   if constexpr (ENT) {
-    if (sl.y + lane < end) load_entry(cur, v, yin, tl.ent[sl.y + lane]);
-    if (sl.y + MB + lane < end) load_entry(nxt, v, yin, tl.ent[sl.y + MB + lane]);
-    if (sl.y + 2 * MB + lane < end) enext = tl.ent[sl.y + 2 * MB + lane];
+    if (has(p0)) load_entry(cur, v, yin, tl.ent[p0.base + lane]);
+    if (has(p1)) load_entry(nxt, v, yin, tl.ent[p1.base + lane]);
+    if (has(p2)) enext = tl.ent[p2.base + lane];
   } else {
-    if (sl.y + lane < end) load_entry(cur, v, yin, tl.list[sl.y + lane], Ju, Jv);
-    if (sl.y + MB + lane < end) load_entry(nxt, v, yin, tl.list[sl.y + MB + lane], Ju, Jv);
-    if (sl.y + 2 * MB + lane < end) inext = tl.list[sl.y + 2 * MB + lane];
+    if (has(p0)) load_entry(cur, v, yin, tl.list[p0.base + lane], Ju, Jv);
+    if (has(p1)) load_entry(nxt, v, yin, tl.list[p1.base + lane], Ju, Jv);
+    if (has(p2)) inext = tl.list[p2.base + lane];
   }
   if constexpr (TAB) {
-    if (sl.y + lane < end) {
+    if (has(p0)) {
       ta = tab_row<JU>(g.tab_u, cur.c.fu);
       tb = tab_row<JV>(table_v<JU, JV>(g), cur.c.fv);
     }
   }
-  for (uint32_t base = sl.y; base < end; base += MB) {
-    const int nb = (int)min((uint32_t)MB, end - base);
+  acc_t cre = {R(0), R(0), R(0), R(0)}, cim = {R(0), R(0), R(0), R(0)};
+  while (p0.tile >= 0) {
+    const int tx0 = (p0.tile % tl.ntu) * T, ty0 = (p0.tile / tl.ntu) * T;
+    const int nb = (int)min((uint32_t)MB, p0.end - p0.base);  // 0 for an empty tile
     if (lane < nb) {
       cpx<R> s = cur.s;
       if (v.weights) s = cmulc(s, cur.w);  // conj(G) = phase * K_u * K_v * conj(weight)
@@ -535,8 +587,9 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
       rd[(lane & 3) * MQ + (lane >> 2)] = (MFMA_NOTAP + 512) | ((MFMA_NOTAP + 512) << 16);
     }
     __syncthreads();
-    // the next batch's table rows and the entries two batches ahead go out before this batch's MFMAs
-    if (base + MB + lane < end) {
+    // the next batch's table rows, the values two batches ahead and the entries three ahead go out
+    // before this batch's MFMAs
+    if (has(p1)) {
       if constexpr (TAB) {
         ta = tab_row<JU>(g.tab_u, nxt.c.fu);
         tb = tab_row<JV>(table_v<JU, JV>(g), nxt.c.fv);
@@ -544,11 +597,11 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
       cur = nxt;
     }
     if constexpr (ENT) {
-      if (base + 2 * MB + lane < end) load_entry(nxt, v, yin, enext);
-      if (base + 3 * MB + lane < end) enext = tl.ent[base + 3 * MB + lane];
+      if (has(p2)) load_entry(nxt, v, yin, enext);
+      if (has(p3)) enext = tl.ent[p3.base + lane];
     } else {
-      if (base + 2 * MB + lane < end) load_entry(nxt, v, yin, inext, Ju, Jv);
-      if (base + 3 * MB + lane < end) inext = tl.list[base + 3 * MB + lane];
+      if (has(p2)) load_entry(nxt, v, yin, inext, Ju, Jv);
+      if (has(p3)) inext = tl.list[p3.base + lane];
     }
     // groups of four steps (16 entries; slots past nb hold no footprint): every LDS read of a group is
     // issued before its MFMAs
@@ -573,10 +626,19 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
       }
     }
     __syncthreads();
-  }
-  const int col = lane & 15;
+    if (p1.tile != p0.tile) {  // the tile's last batch: write its cells, start the next tile at zero
+      const int col = lane & 15;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) grid[(int64_t)(ty0 + Mfma<R>::row(lane, r)) * g.Nu + tx0 + col] = mk<R>(cre[r], cim[r]);
+      for (int r = 0; r < 4; ++r)
+        grid[(int64_t)(ty0 + Mfma<R>::row(lane, r)) * g.Nu + tx0 + col] = mk<R>(cre[r], cim[r]);
+      cre = acc_t{R(0), R(0), R(0), R(0)};
+      cim = cre;
+    }
+    p0 = p1;
+    p1 = p2;
+    p2 = p3;
+    p3 = advance(p3);
+  }
 }
 
 bool grid_mfma_ok(int Nu, int Nv, int Ju, int Jv) {
@@ -590,8 +652,15 @@ void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const
   const dim3 blocks((unsigned)t.n_own);
   const size_t shm = sizeof(cpx<R>) * (size_t)(t.TU + g.Ju - 1) * (t.TV + g.Jv - 1);
   ProfScope prof("degrid", s);
-#define PF_DG(A, B) \
-  hipLaunchKernelGGL((k_degrid_tiles<R, A, B>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y, scale)
+  const bool tab = sizeof(R) == 4 && g.tab_u && g.Ju == g.Jv && g.Ju <= 8;
+#define PF_DG(A, B)                                                                                             \
+  do {                                                                                                          \
+    if (tab && A > 0)                                                                                           \
+      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, true>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y, scale); \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, false>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y,  \
+                         scale);                                                                                \
+  } while (0)
   if (g.Ju == g.Jv && fixed_len_ok(g)) {
     switch (g.Ju) {
       case 2: PF_DG(2, 2); break;
@@ -616,17 +685,20 @@ bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv) {
 
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm, const cpx<R>* y,
-                cpx<R>* grid, R scale, hipStream_t s, int variant) {
+                cpx<R>* grid, R scale, hipStream_t s, int variant, int mfma_waves) {
   if (variant == GRID_MFMA && tm.slot && grid_mfma_ok(g.Nu, g.Nv, g.Ju, g.Jv)) {
     if (sizeof(R) == 4 && !tm.ent) throw Error(-5, "complex64 matrix-core gridding needs the entry records");
-    const dim3 blocks((unsigned)(tm.ntu * tm.ntv));
+    const int nslots = tm.ntu * tm.ntv;
+    const dim3 blocks((unsigned)(mfma_waves > 0 ? std::min(nslots, mfma_waves) : nslots));
     ProfScope prof("grid_tiles", s);
     // tap tables (complex64, equal supports <= 8): the pipelined table variant
     const bool tab = sizeof(R) == 4 && g.tab_u && g.Ju == g.Jv && g.Ju <= 8;
 #define PF_GM(A, B)                                                                                            \
   do {                                                                                                         \
-    if (tab && A > 0) hipLaunchKernelGGL((k_grid_mfma<R, A, B, true>), blocks, dim3(64), 0, s, g, v, tm, y, grid, scale); \
-    else hipLaunchKernelGGL((k_grid_mfma<R, A, B, false>), blocks, dim3(64), 0, s, g, v, tm, y, grid, scale);    \
+    if (tab && A > 0)                                                                                          \
+      hipLaunchKernelGGL((k_grid_mfma<R, A, B, true>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale);  \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_grid_mfma<R, A, B, false>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale); \
   } while (0)
     if (g.Ju == g.Jv && fixed_len_ok(g)) {
       switch (g.Ju) {
@@ -875,9 +947,9 @@ template void degrid<float>(const GridGeom<float>&, const VisData<float>&, const
 template void degrid<double>(const GridGeom<double>&, const VisData<double>&, const TileLists&, const cpx<double>*,
                              cpx<double>*, double, hipStream_t);
 template void grid_tiles<float>(const GridGeom<float>&, const VisData<float>&, const TileLists&, const TileLists&,
-                                const cpx<float>*, cpx<float>*, float, hipStream_t, int);
+                                const cpx<float>*, cpx<float>*, float, hipStream_t, int, int);
 template void grid_tiles<double>(const GridGeom<double>&, const VisData<double>&, const TileLists&, const TileLists&,
-                                 const cpx<double>*, cpx<double>*, double, hipStream_t, int);
+                                 const cpx<double>*, cpx<double>*, double, hipStream_t, int, int);
 
 // ---------------------------------------------------------------------------------------------
 // Vector helpers: FP64 norms (fixed-order two-level reduction), scaling, precision conversion
@@ -955,6 +1027,22 @@ void c64_to_c128(const cpx<float>* in, cpx<double>* out, int64_t n, hipStream_t 
   hipLaunchKernelGGL(k_c64_to_c128, dim3(1024), dim3(256), 0, s, in, out, n);
   PF_HIP(hipGetLastError());
 }
+
+// x[k] = x[k] * h[k] * sc (the pointwise product of a convolution in Fourier space)
+template <typename R>
+__global__ void k_cmul_vec(cpx<R>* x, const cpx<R>* __restrict__ h, int64_t n, double sc) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const cpx<R> a = cmul(x[k], h[k]);
+    x[k] = mk<R>((R)((double)a.x * sc), (R)((double)a.y * sc));
+  }
+}
+template <typename R>
+void cmul_vec(cpx<R>* x, const cpx<R>* h, int64_t n, double sc, hipStream_t s) {
+  hipLaunchKernelGGL((k_cmul_vec<R>), dim3(1024), dim3(256), 0, s, x, h, n, sc);
+  PF_HIP(hipGetLastError());
+}
+template void cmul_vec<float>(cpx<float>*, const cpx<float>*, int64_t, double, hipStream_t);
+template void cmul_vec<double>(cpx<double>*, const cpx<double>*, int64_t, double, hipStream_t);
 
 template void norm2_fp64<float>(const cpx<float>*, int64_t, double*, double*, hipStream_t);
 template void norm2_fp64<double>(const cpx<double>*, int64_t, double*, double*, hipStream_t);

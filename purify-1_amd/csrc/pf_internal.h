@@ -177,8 +177,9 @@ struct TileLists {
   // tap lies in the tile, for the n_own tiles that own any, heaviest first
   const uint4* slot_own = nullptr;
   int n_own = 0;
-  // complex64 matrix-core lists: each list entry as one record (sorted index, first-tap cell
-  // (k_u mod Nu) | (k_v mod Nv) << 16, the fractions u - k_u and v - k_v as float bits)
+  // complex64 matrix-core lists: each list entry as one record (8 * sorted index, the byte offset of
+  // its value; first-tap cell (k_u mod Nu) | (k_v mod Nv) << 16; the fractions u - k_u and v - k_v
+  // as float bits)
   const int4* ent = nullptr;
 };
 
@@ -195,10 +196,11 @@ bool grid_mfma_ok(int Nu, int Nv, int Ju, int Jv);
 // rank-4 updates of 16 x 16 tiles on the matrix cores (lists `tm`)
 enum GridVariant { GRID_RESIDUE = 0, GRID_ATOMIC = 1, GRID_MFMA = 2 };
 // grid = G^H y with y in the operator's sorted order (see permute_to_sorted).  `t` are the lists of
-// the TU x TV tiles, `tm` those of the 16 x 16 tiles (GRID_MFMA).
+// the TU x TV tiles, `tm` those of the 16 x 16 tiles (GRID_MFMA), gridded by mfma_waves waves that
+// each take every mfma_waves-th tile (0: one wave per tile).
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm,
-                const cpx<R>* y_sorted, cpx<R>* grid, R scale, hipStream_t s, int variant);
+                const cpx<R>* y_sorted, cpx<R>* grid, R scale, hipStream_t s, int variant, int mfma_waves = 0);
 // caller's order -> sorted order (ys) and back; the two-pass plan stages through `tmp` (M values).
 // permute_to_caller needs the two-pass plan (the one-pass plan's degridding writes the caller's order).
 template <typename R>
@@ -258,6 +260,8 @@ template <typename R>
 void norm2_fp64(const cpx<R>* x, int64_t n, double* partials, double* out, hipStream_t s);
 template <typename R>
 void scale_vec(cpx<R>* x, int64_t n, double s, hipStream_t st);
+template <typename R>
+void cmul_vec(cpx<R>* x, const cpx<R>* h, int64_t n, double sc, hipStream_t s);
 void c128_to_c64(const cpx<double>* in, cpx<float>* out, int64_t n, hipStream_t s);
 void c64_to_c128(const cpx<float>* in, cpx<double>* out, int64_t n, hipStream_t s);
 

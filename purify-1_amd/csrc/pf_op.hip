@@ -313,6 +313,7 @@ struct pf_op {
   int grid_variant = GRID_RESIDUE;  // tuning knob: gridding kernel
   int kernel_table = 0;             // tuning knob: 0 no tap tables, 1 in gridding, 2 in gridding and degridding
   int fft_col_mode = 0;             // tuning knob: column FFT in one pass (0) or two steps (1)
+  int grid_waves = 0;               // tuning knob: matrix-core gridding waves (0: one per 16 x 16 tile)
   int fft_row_bpc = -1;             // tuning knob: row pass blocks per CU (0: one line per block; -1: plan default)
   int64_t device_bytes = 0;
   std::vector<void*> allocs;
@@ -753,7 +754,7 @@ void build(pf_op* op, const pf_op_desc& d) {
   }
   if (op->w_term) {
     build_wrows(op, d, u, v, perm, ku, kv);
-  } else if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv)) {
+  } else if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv) && (op->prec != PF_C64 || M < ((int64_t)1 << 29))) {
     const HostLists lm = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE);
     upload_lists(op, lm, MFMA_TILE, MFMA_TILE, op->tlm);
     op->tile_list_len_m = (int64_t)lm.list.size();
@@ -771,7 +772,7 @@ void build(pf_op* op, const pf_op_desc& d) {
           int fub, fvb;
           std::memcpy(&fub, &fu, 4);
           std::memcpy(&fvb, &fv, 4);
-          ent[p] = make_int4(i, cu | (cv << 16), fub, fvb);
+          ent[p] = make_int4((int32_t)((uint32_t)i * 8u), cu | (cv << 16), fub, fvb);
         }
       });
       int4* d_ent = (int4*)op->dalloc(sizeof(int4) * ent.size());
@@ -1024,7 +1025,8 @@ void GH_dev(pf_op* op, const cpx<R>* y, cpx<R>* grid, hipStream_t s) {
   cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
   permute_to_sorted<R>(op->vperm, y, static_cast<cpx<R>*>(op->d_ystage), ys, s);
   if (op->w_term) grid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, ys, grid, R(1), s);
-  else grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, op->tlm, ys, grid, R(1), s, op->grid_variant);
+  else grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, op->tlm, ys, grid, R(1), s, op->grid_variant,
+                     op->grid_waves);
 }
 
 template <typename R>
@@ -1041,6 +1043,37 @@ void adjoint_dev(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
   cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
   GH_dev<R>(op, y, A, s);
   fz_adjoint<R>(op, A, x, scl, s);
+}
+
+void ensure_stage(pf_op* op);
+
+// Phi^dagger Phi x (compact_operators.h:118-147 chains Z^dagger F^dagger, G^H G and F Z): degridding
+// writes the sorted order that gridding reads, so neither permutation runs; the weights enter as
+// |w|^2 through the weight in G and its conjugate in G^H.  Exact: no entry of G^H G is pruned.
+template <typename R>
+void normal_dev(pf_op* op, const cpx<R>* x, cpx<R>* out, hipStream_t s) {
+  const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
+  cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
+  cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
+  fz_forward<R>(op, x, B, scl, s);
+  if (op->g_csr) {
+    ensure_stage(op);
+    cpx<R>* t = static_cast<cpx<R>*>(op->d_vecy);
+    G_dev<R>(op, B, t, s);
+    GH_dev<R>(op, t, A, s);
+  } else {
+    cpx<R>* ys = static_cast<cpx<R>*>(op->d_ysort);
+    if (op->w_term) {
+      degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), nullptr, op->tl, B, ys, R(1), s);
+      grid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, ys, A, R(1), s);
+    } else {
+      VisData<R> v = visdata<R>(op);
+      v.perm = nullptr;  // sorted order out
+      degrid<R>(degrid_geom<R>(op), v, op->tl, B, ys, R(1), s);
+      grid_tiles<R>(geom<R>(op), v, op->tl, op->tlm, ys, A, R(1), s, op->grid_variant, op->grid_waves);
+    }
+  }
+  fz_adjoint<R>(op, A, out, scl, s);
 }
 
 void ensure_stage(pf_op* op) {
@@ -1258,6 +1291,190 @@ int pf_op_adjoint(pf_op* op, const void* y, void* x, int y_loc, int x_loc, pf_st
   });
 }
 
+int pf_op_apply_normal(pf_op* op, const void* x, void* out, int x_loc, int out_loc, pf_stream stream) {
+  return guarded([&] {
+    if (!op) throw Error(PF_ERR_ARG, "null operator");
+    if ((out_loc != PF_HOST && out_loc != PF_DEVICE) || !x || !out) throw Error(PF_ERR_ARG, "bad buffer arguments");
+    ProfInstall pi(op);
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = op->nimg();
+    if (x_loc == PF_HOST || out_loc == PF_HOST) ensure_stage(op);
+    const void* din = stage_in(op, x, x_loc, n, op->d_vecx, s);
+    // in place is fine: x is consumed into the grid before the output is written
+    void* dout = out_loc == PF_DEVICE ? out : op->d_vecx;
+    if (op->prec == PF_C64) normal_dev<float>(op, (const cpx<float>*)din, (cpx<float>*)dout, s);
+    else normal_dev<double>(op, (const cpx<double>*)din, (cpx<double>*)dout, s);
+    if (out_loc == PF_HOST) stage_out(op, dout, out, n, s);
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+// PSF convolution (init_psf_convolve_2d, compact_operators.h:279-306): psf = Phi^dagger Phi delta
+// with delta at (floor(ny/2) - 1, floor(nx/2) - 1); x -> IFFT(FFT(psf) . FFT(x)) on the n x n image
+// (unitary FFTs, no padding: a circular convolution), divided by the square of its own power-method
+// norm.
+// ---------------------------------------------------------------------------------------------
+struct pf_psf {
+  pf_op* op = nullptr;  // not owned: the measurement operator must outlive this handle
+  int ny = 0, nx = 0, prec = 0;
+  Fft2dPlan fft;
+  void* d_ft = nullptr;  // FFT(psf), n x n in the operator's precision
+  void* d_w1 = nullptr;  // work images
+  void* d_w2 = nullptr;
+  void* d_x = nullptr;   // staged input
+  double* d_red = nullptr;
+  double norm = 1.0;
+  std::vector<void*> allocs;
+  void* dalloc(size_t bytes) {
+    void* p = nullptr;
+    PF_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    allocs.push_back(p);
+    return p;
+  }
+  size_t esz() const { return prec == PF_C64 ? 8 : 16; }
+  int64_t n() const { return (int64_t)ny * nx; }
+  ~pf_psf() {
+    for (void* p : allocs) (void)hipFree(p);
+    fft_plan_free(fft);
+  }
+};
+
+extern "C++" {
+namespace {
+// out = IFFT(ft . FFT(x)) * sc; x is not modified (x and out may alias)
+template <typename R>
+void psf_apply_dev(pf_psf* h, const cpx<R>* x, cpx<R>* out, double sc, hipStream_t s) {
+  SZ<R> sz;
+  cpx<R>* w1 = static_cast<cpx<R>*>(h->d_w1);
+  cpx<R>* w2 = static_cast<cpx<R>*>(h->d_w2);
+  fft_rows<R>(h->fft, ROW_PLAIN, false, x, w1, h->ny, sz, s);
+  fft_cols<R>(h->fft, false, w1, w2, RowWindow{0, 0}, false, false, s);
+  // both unitary scalings (1/sqrt(N) each way) and the normalisation in the pointwise product
+  cmul_vec<R>(w2, static_cast<const cpx<R>*>(h->d_ft), h->n(), sc / (double)h->n(), s);
+  fft_rows<R>(h->fft, ROW_PLAIN, true, w2, w1, h->ny, sz, s);
+  fft_cols<R>(h->fft, true, w1, out, RowWindow{0, 0}, false, false, s);
+}
+
+template <typename R>
+void psf_build(pf_psf* h, int iters, double tol, const double* start) {
+  pf_op* op = h->op;
+  hipStream_t s = nullptr;
+  const int64_t n = h->n();
+  // psf = Phi^dagger Phi delta with the measurement operator (its norm included)
+  std::vector<cpx<R>> delta((size_t)n, mk<R>(R(0), R(0)));
+  const int64_t idx = (int64_t)((int)std::floor(h->ny * 0.5) - 1) * h->nx + ((int)std::floor(h->nx * 0.5) - 1);
+  if (idx < 0 || idx >= n) throw Error(PF_ERR_ARG, "image too small for the PSF's delta");
+  delta[(size_t)idx] = mk<R>(R(1), R(0));
+  cpx<R>* x = static_cast<cpx<R>*>(h->d_x);
+  cpx<R>* ft = static_cast<cpx<R>*>(h->d_ft);
+  PF_HIP(hipMemcpyAsync(x, delta.data(), sizeof(cpx<R>) * n, hipMemcpyHostToDevice, s));
+  normal_dev<R>(op, x, x, s);
+  // FFT(psf), unitary (the reference's fftop divides by sqrt(N))
+  SZ<R> sz;
+  fft_rows<R>(h->fft, ROW_PLAIN, false, x, static_cast<cpx<R>*>(h->d_w1), h->ny, sz, s);
+  fft_cols<R>(h->fft, false, static_cast<cpx<R>*>(h->d_w1), ft, RowWindow{0, 0}, false, false, s);
+  scale_vec<R>(ft, n, 1.0 / std::sqrt((double)n), s);
+  PF_HIP(hipStreamSynchronize(s));
+  h->norm = 1.0;
+  if (iters <= 0) return;
+  // power method on x -> psf_convolve(x) (adjoint: the identity), the iteration of SOPT's
+  // power_method; the start vector is the caller's or uniform in [-1, 1] (re, im) from
+  // mt19937_64(0x5eed): a constant start is an eigenvector of every circular convolution
+  std::vector<cpx<double>> st((size_t)n);
+  if (start) {
+    for (int64_t k = 0; k < n; ++k) st[(size_t)k] = mk<double>(start[2 * k], start[2 * k + 1]);
+  } else {
+    std::mt19937_64 rng(0x5eed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (auto& c : st) {
+      const double re = U(rng);
+      c = mk<double>(re, U(rng));
+    }
+  }
+  std::vector<cpx<R>> stR((size_t)n);
+  for (int64_t k = 0; k < n; ++k) stR[(size_t)k] = mk<R>((R)st[(size_t)k].x, (R)st[(size_t)k].y);
+  PF_HIP(hipMemcpyAsync(x, stR.data(), sizeof(cpx<R>) * n, hipMemcpyHostToDevice, s));
+  auto nrm = [&]() {
+    norm2_fp64<R>(x, n, h->d_red, h->d_red + 1024, s);
+    double v = 0;
+    PF_HIP(hipMemcpyAsync(&v, h->d_red + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+    return std::sqrt(v);
+  };
+  const double n0 = nrm();
+  if (!(n0 > 0)) throw Error(PF_ERR_ARG, "power method start vector is zero");
+  scale_vec<R>(x, n, 1.0 / n0, s);
+  double old = 0, val = 1;
+  for (int it = 0; it < iters; ++it) {
+    psf_apply_dev<R>(h, x, x, 1.0, s);
+    val = nrm();
+    if (!(val > 0)) throw Error(PF_ERR_STATE, "Error in operator.");
+    scale_vec<R>(x, n, 1.0 / val, s);
+    if (tol * tol > std::fabs(old - val) / old) {
+      old = val;
+      break;
+    }
+    old = val;
+  }
+  // init_normalise(op_norm^2) with op_norm = sqrt(old)
+  h->norm = std::sqrt(old);
+}
+}  // namespace
+}  // extern "C++"
+
+int pf_psf_create(pf_op* op, int32_t power_iters, double power_tol, const double* start_host, pf_psf** out) {
+  if (!op || !out) return fail(PF_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<pf_psf> h(new pf_psf());
+  int rc = guarded([&] {
+    if (!is_pow2(op->ny) || !is_pow2(op->nx))
+      throw Error(PF_ERR_UNSUPPORTED, "PSF convolution needs power-of-two image sides");
+    h->op = op;
+    h->ny = op->ny;
+    h->nx = op->nx;
+    h->prec = op->prec;
+    fft_plan_init(h->fft, h->ny, h->nx, h->prec);
+    const size_t b = h->esz() * (size_t)h->n();
+    h->d_ft = h->dalloc(b);
+    h->d_w1 = h->dalloc(b);
+    h->d_w2 = h->dalloc(b);
+    h->d_x = h->dalloc(b);
+    h->d_red = (double*)h->dalloc(sizeof(double) * 1025);
+    if (h->prec == PF_C64) psf_build<float>(h.get(), power_iters, power_tol, start_host);
+    else psf_build<double>(h.get(), power_iters, power_tol, start_host);
+  });
+  if (rc == PF_OK) *out = h.release();
+  return rc;
+}
+
+int pf_psf_apply(pf_psf* h, const void* x, void* out, int x_loc, int out_loc, pf_stream stream) {
+  return guarded([&] {
+    if (!h) throw Error(PF_ERR_ARG, "null handle");
+    if ((out_loc != PF_HOST && out_loc != PF_DEVICE) || !x || !out) throw Error(PF_ERR_ARG, "bad buffer arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = h->n();
+    pf_op* op = h->op;
+    if (x_loc == PF_HOST || out_loc == PF_HOST) ensure_stage(op);
+    const void* din = stage_in(op, x, x_loc, n, h->d_x, s);
+    void* dout = out_loc == PF_DEVICE ? out : h->d_x;
+    const double sc = 1.0 / (h->norm * h->norm);
+    if (h->prec == PF_C64) psf_apply_dev<float>(h, (const cpx<float>*)din, (cpx<float>*)dout, sc, s);
+    else psf_apply_dev<double>(h, (const cpx<double>*)din, (cpx<double>*)dout, sc, s);
+    if (out_loc == PF_HOST) stage_out(op, dout, out, n, s);
+  });
+}
+
+int pf_psf_norm(const pf_psf* h, double* norm) {
+  if (!h || !norm) return fail(PF_ERR_ARG, "null argument");
+  *norm = h->norm;
+  return PF_OK;
+}
+
+int pf_psf_destroy(pf_psf* h) {
+  delete h;
+  return PF_OK;
+}
+
 int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
   if (!op || !key) return fail(PF_ERR_ARG, "null argument");
   const std::string k(key);
@@ -1277,6 +1494,11 @@ int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
       if (op->fft_row_bpc >= 0) op->fft.row_bpc = op->fft_row_bpc;
       op->fft_col_mode = (int)value;
     });
+  }
+  if (k == "grid_waves") {
+    if (value < 0 || value > (int64_t)1 << 24) return fail(PF_ERR_ARG, "grid_waves must be 0..2^24");
+    op->grid_waves = (int)value;
+    return PF_OK;
   }
   if (k == "fft_row_blocks") {
     if (value < 0 || value > 64) return fail(PF_ERR_ARG, "fft_row_blocks must be 0..64");
@@ -1300,6 +1522,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   const std::string k(key);
   if (k == "grid_kernel") {
     *value = op->grid_variant;
+    return PF_OK;
+  }
+  if (k == "grid_waves") {
+    *value = op->grid_waves;
     return PF_OK;
   }
   if (k == "fft_row_blocks") {
@@ -1347,10 +1573,10 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
       auto run = [&] {
         if (op->prec == PF_C64)
           grid_tiles<float>(op->gf, visdata<float>(op), op->tl, op->tlm, (const cpx<float>*)op->d_vecy,
-                            (cpx<float>*)op->d_gridA, 1.0f, s, variant);
+                            (cpx<float>*)op->d_gridA, 1.0f, s, variant, op->grid_waves);
         else
           grid_tiles<double>(op->gd, visdata<double>(op), op->tl, op->tlm, (const cpx<double>*)op->d_vecy,
-                             (cpx<double>*)op->d_gridA, 1.0, s, variant);
+                             (cpx<double>*)op->d_gridA, 1.0, s, variant, op->grid_waves);
       };
       run();  // warm-up
       PF_HIP(hipEventRecord(a, s));

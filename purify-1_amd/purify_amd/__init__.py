@@ -5,12 +5,14 @@ include/purify_amd.h; this package is the host-side mirror of the reference's C+
 """
 from . import _lib
 from .kernels import kernel, kernel_from_string
-from .operators import (LinearTransform, MeasurementOperator, distributed_measurement_operator,
-                        measurement_operator_factory, measurementoperator)
+from .operators import (GridDegridOperator, LinearTransform, MeasurementOperator, PsfOperator,
+                        distributed_measurement_operator, measurement_operator_factory, measurementoperator,
+                        operators)
 from .utilities import vis_params, vis_units
 from . import utilities, wproj_utilities
 
 __all__ = ["kernel", "kernel_from_string", "LinearTransform", "MeasurementOperator", "measurementoperator",
+           "operators", "GridDegridOperator", "PsfOperator",
            "measurement_operator_factory", "distributed_measurement_operator", "vis_params", "vis_units",
            "utilities", "fft2d", "kernel_eval", "version"]
 

@@ -60,6 +60,11 @@ SIGNATURES = {
     "pf_op_destroy": (C.c_int, [C.c_void_p]),
     "pf_op_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "pf_op_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "pf_op_apply_normal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "pf_psf_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pf_psf_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "pf_psf_norm": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pf_psf_destroy": (C.c_int, [C.c_void_p]),
     "pf_op_power_method": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.POINTER(C.c_double)]),
     "pf_op_set_norm": (C.c_int, [C.c_void_p, C.c_double]),
     "pf_op_get_norm": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

@@ -167,6 +167,38 @@ class MeasurementOperator:
         _lib.check(fn(self._h, xh.ctypes.data, yh.ctypes.data, _lib.PF_HOST, _lib.PF_HOST, None))
         return yh
 
+    def _call(self, fn, handle, x, nin, nout, out=None, stream=None):
+        if _is_device_tensor(x):
+            torch = _torch()
+            if x.dtype != self._dtype():
+                raise TypeError(f"device input must be {self._dtype()} for this operator")
+            x = x.contiguous()
+            if x.numel() != nin:
+                raise ValueError(f"input has {x.numel()} entries, expected {nin}")
+            if out is None:
+                out = torch.empty(nout, dtype=x.dtype, device=x.device)
+            s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+            _lib.check(fn(handle, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
+                          _lib.PF_DEVICE, C.c_void_p(s)))
+            return out
+        xh = _c128(x).ravel()
+        if xh.size != nin:
+            raise ValueError(f"input has {xh.size} entries, expected {nin}")
+        if out is not None:  # sopt::OperatorFunction's output-first form: fill the caller's array
+            if not (isinstance(out, np.ndarray) and out.dtype == np.complex128 and out.size == nout
+                    and out.flags.c_contiguous):
+                raise TypeError(f"output must be a contiguous complex128 array of {nout} entries")
+            yh = out.reshape(-1)
+        else:
+            yh = np.empty(nout, np.complex128)
+        _lib.check(fn(handle, xh.ctypes.data, yh.ctypes.data, _lib.PF_HOST, _lib.PF_HOST, None))
+        return out if out is not None else yh
+
+    def normal(self, x, out=None, stream=None):
+        """Phi^dagger Phi x (the compact grid-degrid operator; see init_grid_degrid_operator_2d)."""
+        n = self.imsizex * self.imsizey
+        return self._call(_lib.lib().pf_op_apply_normal, self._h, x, n, n, out, stream)
+
     def forward(self, x, out=None, stream=None):
         return self._apply(False, x, out, stream)
 
@@ -314,6 +346,90 @@ def init_degrid_operator_2d_from_matrix(G, imsizey, imsizex, oversample_ratio=2.
 
 
 measurementoperator.init_degrid_operator_2d_from_matrix = staticmethod(init_degrid_operator_2d_from_matrix)
+
+
+class GridDegridOperator:
+    """operators::init_grid_degrid_operator_2d (compact_operators.h:150-171): the self-adjoint operator
+    x -> Phi^dagger Phi x, normalised by the square of the measurement operator's norm.  The reference
+    returns a sopt::OperatorFunction (output first, input second); here ``op(x)`` returns the output and
+    ``op(out, x)`` fills ``out``.  Wrap it as ``LinearTransform``-like pair with the identity adjoint
+    exactly as the reference's tests do."""
+
+    def __init__(self, mop: MeasurementOperator):
+        self.op = mop
+
+    def __call__(self, *args, stream=None):
+        if len(args) == 2:
+            out, x = args
+            return self.op.normal(x, out=out, stream=stream)
+        (x,) = args
+        return self.op.normal(x, stream=stream)
+
+    @property
+    def norm(self):
+        return self.op.norm
+
+
+class PsfOperator:
+    """operators::init_psf_convolve_2d (compact_operators.h:279-306): x -> IFFT(FFT(psf) . FFT(x)) / norm^2
+    with psf = Phi^dagger Phi delta and norm from its own power method."""
+
+    def __init__(self, mop: MeasurementOperator, power_iters=100, power_tol=1e-4, start=None):
+        self.op = mop  # keeps the measurement operator alive
+        self._h = C.c_void_p()
+        st = None if start is None else _c128(start).ravel()
+        if st is not None and st.size != mop.imsizex * mop.imsizey:
+            raise ValueError("start vector has the wrong size")
+        self._keep = st
+        _lib.check(_lib.lib().pf_psf_create(mop.handle, int(power_iters), float(power_tol),
+                                            None if st is None else st.ctypes.data, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _lib.lib().pf_psf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def norm(self):
+        v = C.c_double()
+        _lib.check(_lib.lib().pf_psf_norm(self._h, C.byref(v)))
+        return v.value
+
+    def __call__(self, *args, stream=None):
+        n = self.op.imsizex * self.op.imsizey
+        if len(args) == 2:
+            out, x = args
+            return self.op._call(_lib.lib().pf_psf_apply, self._h, x, n, n, out, stream)
+        (x,) = args
+        return self.op._call(_lib.lib().pf_psf_apply, self._h, x, n, n, None, stream)
+
+
+class operators:  # noqa: N801 -- mirrors the reference's purify::operators namespace (compact operators)
+    @staticmethod
+    def init_grid_degrid_operator_2d(*args, **kw):
+        """Both reference overloads (compact_operators.h:150-189):
+        init_grid_degrid_operator_2d(u, v, w, weights, imsizey, imsizex, oversample_ratio=2, power_iters=100,
+                                     power_tol=1e-4, kernel=kb, Ju=4, Jv=4, w_term=False, Jw=6, cellx=1, celly=1)
+        init_grid_degrid_operator_2d(uv_vis, imsizey, imsizex, cell_x, cell_y, oversample_ratio=2, power_iters=100,
+                                     power_tol=1e-4, kernel=kb, Ju=4, Jv=4, w_term=False, Jw=6)
+        Extra keyword: precision ("c64" default, or "c128")."""
+        lt = measurementoperator.init_degrid_operator_2d(*args, **kw)
+        return GridDegridOperator(lt.op)
+
+    @staticmethod
+    def init_psf_convolve_2d(measure_op, imsizey=None, imsizex=None, power_iters=100, power_tol=1e-4, start=None):
+        """init_psf_convolve_2d(degrid_grid, imsizey, imsizex, power_iters, power_tol)
+        (compact_operators.h:279-306), from a measurement operator's LinearTransform."""
+        mop = measure_op.op if isinstance(measure_op, LinearTransform) else measure_op
+        if imsizey is not None and (imsizey, imsizex) != (mop.imsizey, mop.imsizex):
+            raise ValueError("image size differs from the measurement operator's")
+        return PsfOperator(mop, power_iters, power_tol, start)
 
 
 # factory::measurement_operator_factory (cpp/purify/measurement_operator_factory.h:20-96)

@@ -316,3 +316,22 @@ def test_row_pass_grid_sizes_give_identical_results(precision):
         assert np.array_equal(lt * x, ref[0]) and np.array_equal(lt.adjoint() * y, ref[1])
     with pytest.raises(PurifyError):
         lt.op.set_tuning("fft_row_blocks", 65)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_matrix_core_wave_counts_give_identical_results(precision):
+    # grid_waves W: wave w grids tiles w, w + W, ... with its batches pipelined across tile boundaries;
+    # every tile's sum runs in the same order whatever W is
+    n, M = 128, 30_000
+    u, v = make_uv(95, M, n, sigma=0.5)
+    w = utilities.synth_cnormal(96, M)
+    gpu, orc = pair(u, v, n, n, weights=w, Ju=6, Jv=6, precision=precision)
+    assert gpu.op.get_tuning("grid_kernel") == 2 and gpu.op.get_tuning("grid_waves") == 0
+    y = utilities.synth_cnormal(97, M)
+    ref = gpu.adjoint() * y
+    assert rel(ref, orc.adjoint(y)) < TOL[precision]
+    for gw in (1, 7, 64, 10**6):
+        gpu.op.set_tuning("grid_waves", gw)
+        assert np.array_equal(gpu.adjoint() * y, ref)
+    with pytest.raises(PurifyError):
+        gpu.op.set_tuning("grid_waves", -1)
