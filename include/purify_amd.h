@@ -107,6 +107,25 @@ int pf_psf_apply(pf_psf *psf, const void *x, void *out, int x_loc, int out_loc, 
 int pf_psf_norm(const pf_psf *psf, double *norm);
 int pf_psf_destroy(pf_psf *psf);
 
+/* ---- Sparsity dictionary Psi (factory::wavelet_operator_factory, wavelet_operator_factory.h:20-45:
+ * sopt::wavelets::SARA wrapped by sopt::linear_transform(sara, imsizey, imsizex)) ----
+ * nwav entries: db[i] = 0 for the Dirac basis or the Daubechies order 1..PF_DB_MAX ("DB1".."DB10"),
+ * levels[i] levels of the periodic 2-D transform (image sides divisible by 2^levels).  As a
+ * LinearTransform: the direct operator is the synthesis Psi (nwav stacked coefficient images ->
+ * image), the adjoint the analysis Psi^dagger; SARA divides each entry by sqrt(nwav).  SOPT is not
+ * in the reference tree: the filters are Daubechies' extremal-phase filters (pf_wavelet_filter), the
+ * transform the periodic down-convolution lo[i] = sum_k h[k] x[(2i + k) mod n], low-pass half first,
+ * rows then columns per level. */
+#define PF_DB_MAX 10
+typedef struct pf_wavelet pf_wavelet;
+/* the Daubechies scaling filter with db vanishing moments (2 * db taps, sum sqrt(2)) into lo */
+int pf_wavelet_filter(int32_t db, double *lo, int32_t *len);
+int pf_wavelet_create(int32_t imsizey, int32_t imsizex, int32_t nwav, const int32_t *db, const int32_t *levels,
+                      int32_t precision, pf_wavelet **out);
+int pf_wavelet_forward(pf_wavelet *wv, const void *coeffs, void *image, int c_loc, int x_loc, pf_stream stream);
+int pf_wavelet_adjoint(pf_wavelet *wv, const void *image, void *coeffs, int x_loc, int c_loc, pf_stream stream);
+int pf_wavelet_destroy(pf_wavelet *wv);
+
 /* Power method on Phi^dagger Phi (SOPT power_method as called at operators.h:390-391) over the
  * operator as it currently is (including its norm).  start: host complex128 of imsizey*imsizex,
  * or NULL for all ones.  Iterates in the operator's precision, reduces norms in FP64. */

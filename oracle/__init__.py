@@ -433,6 +433,90 @@ def psf_convolve(op: "Operator", x, psf_norm=1.0):
     return fft2d(ny, nx, ft * fft2d(ny, nx, x), sign=+1) / (psf_norm * psf_norm)
 
 
+# ---- wavelets (sopt::wavelets::SARA as used by wavelet_operator_factory.h:20-45) ----------------
+# SOPT is not in the reference tree, so this restates the published construction: Daubechies'
+# extremal-phase scaling filters (Ten Lectures on Wavelets, 1992, section 6.1) and the periodic
+# down-convolution lo[i] = sum_k h[k] x[(2i + k) mod n], hi with g[k] = (-1)^k h[L-1-k], low-pass
+# half first, rows then columns per level, the next level on the low-low block.  Parity against SOPT
+# itself is unpinned; the tests pin the filters by their defining identities and closed forms
+# (Haar, DB2), and the transform by orthonormality, perfect reconstruction and a Haar known answer.
+def daubechies(N):
+    """Scaling filter with N vanishing moments (2N taps, sum sqrt(2)), extremal phase."""
+    from math import comb
+    if N == 1:
+        return np.array([1.0, 1.0]) / np.sqrt(2.0)
+    P = [comb(N - 1 + k, k) for k in range(N)][::-1]  # highest degree first
+    y = np.roots(P)
+    z = []
+    for yy in y:
+        b = 4 * yy - 2
+        s_ = np.sqrt(b * b - 4 + 0j)
+        z1, z2 = (-b + s_) / 2, (-b - s_) / 2
+        z.append(z1 if abs(z1) < abs(z2) else z2)
+    z += [-1.0] * N
+    c = np.real(np.poly(z))  # highest degree first
+    return c * np.sqrt(2.0) / c.sum()
+
+
+def _dwt_line(x, h):
+    n, L = x.shape[-1], len(h)
+    g = np.array([(-1) ** k * h[L - 1 - k] for k in range(L)])
+    half = n // 2
+    idx = (2 * np.arange(half)[:, None] + np.arange(L)[None, :]) % n
+    xs = x[..., idx]  # (..., half, L)
+    return np.concatenate([xs @ h, xs @ g], axis=-1)
+
+
+def _idwt_line(c, h):
+    n, L = c.shape[-1], len(h)
+    g = np.array([(-1) ** k * h[L - 1 - k] for k in range(L)])
+    half = n // 2
+    x = np.zeros_like(c)
+    for i in range(half):
+        for k in range(L):
+            j = (2 * i + k) % n
+            x[..., j] += h[k] * c[..., i] + g[k] * c[..., half + i]
+    return x
+
+
+def dwt2(img, ny, nx, h, levels):
+    c = np.asarray(img, np.complex128).reshape(ny, nx).copy()
+    for l in range(levels):
+        hh, ww = ny >> l, nx >> l
+        blk = _dwt_line(c[:hh, :ww], h)
+        c[:hh, :ww] = _dwt_line(blk.T, h).T
+    return c.ravel()
+
+
+def idwt2(coef, ny, nx, h, levels):
+    c = np.asarray(coef, np.complex128).reshape(ny, nx).copy()
+    for l in range(levels - 1, -1, -1):
+        hh, ww = ny >> l, nx >> l
+        blk = _idwt_line(c[:hh, :ww].T, h).T
+        c[:hh, :ww] = _idwt_line(blk, h)
+    return c.ravel()
+
+
+def sara_analysis(img, ny, nx, entries):
+    """Psi^dagger: entries = [(order, levels)], order 0 = Dirac; stacked, each / sqrt(len(entries))."""
+    f = 1.0 / np.sqrt(len(entries))
+    out = []
+    for order, lev in entries:
+        out.append(np.asarray(img, np.complex128).ravel() * f if order == 0 else dwt2(img, ny, nx, daubechies(order), lev) * f)
+    return np.concatenate(out)
+
+
+def sara_synthesis(coef, ny, nx, entries):
+    """Psi: the sum of the entries' reconstructions, each / sqrt(len(entries))."""
+    f = 1.0 / np.sqrt(len(entries))
+    N = ny * nx
+    x = np.zeros(N, np.complex128)
+    for i, (order, lev) in enumerate(entries):
+        ci = np.asarray(coef[i * N:(i + 1) * N], np.complex128)
+        x += (ci if order == 0 else idwt2(ci, ny, nx, daubechies(order), lev)) * f
+    return x
+
+
 def omp_threads():
     return lib().orc_omp_threads()
 

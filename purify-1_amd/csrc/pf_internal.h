@@ -20,6 +20,26 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char* what);
 #define PF_HIP(call) ::pf::hip_check((call), #call)
 
+// The calling thread's last error message (pf_last_error) and the C-ABI wrappers that set it.
+std::string& last_error();
+inline int fail(int code, const std::string& msg) {
+  last_error() = msg;
+  return code;
+}
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(-4, "host allocation failed");  // PF_ERR_NOMEM
+  } catch (const std::exception& e) {
+    return fail(-1, e.what());  // PF_ERR_ARG
+  }
+}
+
 // ---- per-kernel timing --------------------------------------------------------------------
 // When an operator has profiling enabled it installs its Profiler as the calling thread's current
 // one; every launch site then brackets its kernel with HIP events on the launching stream.

@@ -23,7 +23,11 @@
 
 namespace pf {
 
-static thread_local std::string g_last_error;
+std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw Error(PF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -336,25 +340,6 @@ struct pf_op {
 };
 
 namespace {
-
-int fail(int code, const std::string& msg) {
-  g_last_error = msg;
-  return code;
-}
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    return PF_OK;
-  } catch (const Error& e) {
-    return fail(e.code, e.what());
-  } catch (const std::bad_alloc&) {
-    return fail(PF_ERR_NOMEM, "host allocation failed");
-  } catch (const std::exception& e) {
-    return fail(PF_ERR_ARG, e.what());
-  }
-}
 
 bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
 
@@ -1857,7 +1842,7 @@ int pf_synth_cnormal(uint64_t seed, int64_t n, double scl, double* out) {
   return PF_OK;
 }
 
-const char* pf_last_error(void) { return g_last_error.c_str(); }
+const char* pf_last_error(void) { return last_error().c_str(); }
 const char* pf_version(void) { return "purify-amd 0.1 (gfx950)"; }
 
 }  // extern "C"

@@ -1,0 +1,437 @@
+// pf_wavelet.hip -- the sparsity dictionary Psi of PURIFY's solvers on the device: SOPT's
+// sopt::wavelets::SARA wrapped by sopt::linear_transform(sara, imsizey, imsizex), as
+// factory::wavelet_operator_factory builds it (cpp/purify/wavelet_operator_factory.h:20-45).
+//
+// Each dictionary entry is the Dirac basis or a Daubechies wavelet with a number of levels.  One level
+// of the 2-D analysis transform works on the top-left (h, w) block of the coefficient image: every
+// row of the block, then every column, goes through a periodic down-convolution with the low-pass
+// and the high-pass filter, low-pass results first:
+//     lo[i] = sum_k h[k] x[(2i + k) mod w],  hi[i] = sum_k g[k] x[(2i + k) mod w],  i < w / 2,
+// and the next level works on the (h/2, w/2) low-low block.  The filters form an orthonormal pair
+// (g[k] = (-1)^k h[L-1-k]), so the synthesis transform (the adjoint, coarsest level first, columns
+// then rows) is also the inverse.  SARA stacks the entries' analysis coefficients, each divided by
+// sqrt(number of entries); its synthesis sums the entries' reconstructions with the same factor.
+//
+// The Daubechies scaling filters are computed on the host from their spectral factorisation
+// (Daubechies, "Ten Lectures on Wavelets", 1992, section 6.1): the roots y of
+// P(y) = sum_{k<N} C(N-1+k, k) y^k map to z through y = -(z - 1)^2 / (4 z); the roots with |z| < 1,
+// with N roots at z = -1, give the extremal-phase filter, normalised to sum sqrt(2).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <memory>
+#include <vector>
+
+#include "../../include/purify_amd.h"
+#include "pf_internal.h"
+
+namespace pf {
+
+constexpr int WAV_MAXL = 2 * PF_DB_MAX;  // longest filter
+
+// Daubechies scaling filter with N vanishing moments (length 2N), in double, sum sqrt(2).
+std::vector<double> daubechies_filter(int N) {
+  if (N < 1 || N > PF_DB_MAX) throw Error(PF_ERR_ARG, "Daubechies order must be 1.." + std::to_string(PF_DB_MAX));
+  using C = std::complex<long double>;
+  if (N == 1) return {1.0 / std::sqrt(2.0), 1.0 / std::sqrt(2.0)};
+  // P(y) coefficients, highest degree first
+  std::vector<long double> P(N);
+  for (int k = 0; k < N; ++k) {
+    long double c = 1;  // C(N-1+k, k)
+    for (int j = 1; j <= k; ++j) c = c * (N - 1 + j) / j;
+    P[N - 1 - k] = c;
+  }
+  // roots of P (degree N-1) by Durand-Kerner with Newton polishing, in long double
+  const int deg = N - 1;
+  auto evalP = [&](C y) {
+    C r = 0;
+    for (int k = 0; k <= deg; ++k) r = r * y + P[k];
+    return r;
+  };
+  auto evaldP = [&](C y) {
+    C r = 0;
+    for (int k = 0; k < deg; ++k) r = r * y + P[k] * (long double)(deg - k);
+    return r;
+  };
+  std::vector<C> r(deg);
+  for (int i = 0; i < deg; ++i) r[i] = std::pow(C(0.4L, 0.9L), i + 1);
+  for (int it = 0; it < 2000; ++it) {
+    long double delta = 0;
+    for (int i = 0; i < deg; ++i) {
+      C den = P[0];
+      for (int j = 0; j < deg; ++j)
+        if (j != i) den *= (r[i] - r[j]);
+      const C step = evalP(r[i]) / den;
+      r[i] -= step;
+      delta = std::max(delta, std::abs(step));
+    }
+    if (delta < 1e-30L) break;
+  }
+  for (int i = 0; i < deg; ++i)
+    for (int it = 0; it < 5; ++it) {
+      const C d = evaldP(r[i]);
+      if (std::abs(d) == 0) break;
+      r[i] -= evalP(r[i]) / d;
+    }
+  // y = -(z - 1)^2 / (4 z)  <=>  z^2 + (4y - 2) z + 1 = 0: keep the root inside the unit circle
+  std::vector<C> z;
+  for (const C& y : r) {
+    const C b = 4.0L * y - 2.0L;
+    const C s = std::sqrt(b * b - 4.0L);
+    C z1 = (-b + s) / 2.0L, z2 = (-b - s) / 2.0L;
+    z.push_back(std::abs(z1) < std::abs(z2) ? z1 : z2);
+  }
+  for (int k = 0; k < N; ++k) z.push_back(C(-1.0L, 0.0L));
+  // polynomial with those roots (lowest degree first), real up to rounding
+  std::vector<C> poly{C(1.0L)};
+  for (const C& root : z) {
+    std::vector<C> nxt(poly.size() + 1, C(0));
+    for (size_t k = 0; k < poly.size(); ++k) {
+      nxt[k + 1] += poly[k];
+      nxt[k] -= root * poly[k];
+    }
+    poly.swap(nxt);
+  }
+  long double sum = 0;
+  for (const C& c : poly) sum += c.real();
+  std::vector<double> h(poly.size());
+  // highest power first gives the extremal-phase ordering (energy at the start of the filter)
+  for (size_t k = 0; k < poly.size(); ++k)
+    h[k] = (double)(poly[poly.size() - 1 - k].real() * std::sqrt(2.0L) / sum);
+  return h;
+}
+
+template <typename R>
+struct WavFilt {
+  R lo[WAV_MAXL], hi[WAV_MAXL];
+  int L;
+};
+
+// One analysis pass over the (h, w) block: rows (ROWS) or columns.  Thread t takes output index i of
+// line `line` and writes lo[i] and hi[i].  ld: row stride of both images.
+template <typename R, bool ROWS>
+__global__ void __launch_bounds__(256) k_dwt(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int h, int w,
+                                             int ld, WavFilt<R> f, R scl) {
+  const int n = ROWS ? w : h;  // line length
+  const int nl = ROWS ? h : w; // lines
+  const int half = n / 2;
+  const int64_t total = (int64_t)nl * half;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    int line, i;
+    if (ROWS) {
+      line = (int)(t / half);
+      i = (int)(t - (int64_t)line * half);
+    } else {  // consecutive threads take consecutive columns: coalesced
+      i = (int)(t / nl);
+      line = (int)(t - (int64_t)i * nl);
+    }
+    auto at = [&](int k) -> int64_t { return ROWS ? (int64_t)line * ld + k : (int64_t)k * ld + line; };
+    cpx<R> a = mk<R>(R(0), R(0)), b = mk<R>(R(0), R(0));
+    for (int k = 0; k < f.L; ++k) {
+      int p = 2 * i + k;
+      while (p >= n) p -= n;
+      const cpx<R> x = in[at(p)];
+      a.x += f.lo[k] * x.x;
+      a.y += f.lo[k] * x.y;
+      b.x += f.hi[k] * x.x;
+      b.y += f.hi[k] * x.y;
+    }
+    out[at(i)] = scale(a, scl);
+    out[at(half + i)] = scale(b, scl);
+  }
+}
+
+// One synthesis pass (the adjoint of k_dwt): x[j] = sum over k with (j - k) even of
+// lo[k] c[(j - k) mod n / 2] + hi[k] c[half + (j - k) mod n / 2].  ACC: add into out.
+template <typename R, bool ROWS, bool ACC>
+__global__ void __launch_bounds__(256) k_idwt(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int h, int w,
+                                              int ld, WavFilt<R> f, R scl) {
+  const int n = ROWS ? w : h;
+  const int nl = ROWS ? h : w;
+  const int half = n / 2;
+  const int64_t total = (int64_t)nl * n;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    int line, j;
+    if (ROWS) {
+      line = (int)(t / n);
+      j = (int)(t - (int64_t)line * n);
+    } else {
+      j = (int)(t / nl);
+      line = (int)(t - (int64_t)j * nl);
+    }
+    auto at = [&](int k) -> int64_t { return ROWS ? (int64_t)line * ld + k : (int64_t)k * ld + line; };
+    cpx<R> acc = mk<R>(R(0), R(0));
+    for (int k = j & 1 ? 1 : 0; k < f.L; k += 2) {  // (j - k) even
+      int d = j - k;
+      while (d < 0) d += n;
+      const int i = d >> 1;
+      const cpx<R> cl = in[at(i)], ch = in[at(half + i)];
+      acc.x += f.lo[k] * cl.x + f.hi[k] * ch.x;
+      acc.y += f.lo[k] * cl.y + f.hi[k] * ch.y;
+    }
+    acc = scale(acc, scl);
+    if (ACC) {
+      const cpx<R> o = out[at(j)];
+      acc.x += o.x;
+      acc.y += o.y;
+    }
+    out[at(j)] = acc;
+  }
+}
+
+// out[k] = in[k] * scl (ACC: out[k] += in[k] * scl) over the whole image
+template <typename R, bool ACC>
+__global__ void __launch_bounds__(256) k_copy_scaled(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int64_t n,
+                                                     R scl) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    cpx<R> v = scale(in[k], scl);
+    if (ACC) {
+      const cpx<R> o = out[k];
+      v.x += o.x;
+      v.y += o.y;
+    }
+    out[k] = v;
+  }
+}
+
+static dim3 grid_for(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))); }
+
+template <typename R>
+WavFilt<R> make_filt(const std::vector<double>& h) {
+  WavFilt<R> f{};
+  f.L = (int)h.size();
+  for (int k = 0; k < f.L; ++k) {
+    f.lo[k] = (R)h[k];
+    f.hi[k] = (R)(((k & 1) ? -1.0 : 1.0) * h[f.L - 1 - k]);
+  }
+  return f;
+}
+
+// Analysis of one entry: image x -> coefficients c (both ny x nx); T: work image.
+template <typename R>
+void dwt2(const cpx<R>* x, cpx<R>* c, cpx<R>* T, int ny, int nx, int levels, const WavFilt<R>& f, hipStream_t s) {
+  for (int l = 0; l < levels; ++l) {
+    const int h = ny >> l, w = nx >> l;
+    const cpx<R>* src = l == 0 ? x : c;
+    hipLaunchKernelGGL((k_dwt<R, true>), grid_for((int64_t)h * (w / 2)), dim3(256), 0, s, src, T, h, w, nx, f, R(1));
+    hipLaunchKernelGGL((k_dwt<R, false>), grid_for((int64_t)w * (h / 2)), dim3(256), 0, s, T, c, h, w, nx, f, R(1));
+  }
+}
+
+// Synthesis of one entry: coefficients c -> image x (ACC: x += ...); W, T: work images.
+template <typename R>
+void idwt2(const cpx<R>* c, cpx<R>* x, cpx<R>* W, cpx<R>* T, int ny, int nx, int levels, const WavFilt<R>& f, R scl,
+           bool acc, hipStream_t s) {
+  const int64_t N = (int64_t)ny * nx;
+  if (levels == 0) {
+    if (acc) hipLaunchKernelGGL((k_copy_scaled<R, true>), grid_for(N), dim3(256), 0, s, c, x, N, scl);
+    else hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, c, x, N, scl);
+    return;
+  }
+  // W holds the coefficients being reconstructed in place, with the SARA factor applied on the way in
+  hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, c, W, N, scl);
+  for (int l = levels - 1; l >= 0; --l) {
+    const int h = ny >> l, w = nx >> l;
+    hipLaunchKernelGGL((k_idwt<R, false, false>), grid_for((int64_t)w * h), dim3(256), 0, s, W, T, h, w, nx, f, R(1));
+    if (l == 0) {
+      if (acc) hipLaunchKernelGGL((k_idwt<R, true, true>), grid_for((int64_t)h * w), dim3(256), 0, s, T, x, h, w, nx, f, R(1));
+      else hipLaunchKernelGGL((k_idwt<R, true, false>), grid_for((int64_t)h * w), dim3(256), 0, s, T, x, h, w, nx, f, R(1));
+    } else {
+      hipLaunchKernelGGL((k_idwt<R, true, false>), grid_for((int64_t)h * w), dim3(256), 0, s, T, W, h, w, nx, f, R(1));
+    }
+  }
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+struct pf_wavelet {
+  int ny = 0, nx = 0, prec = 0, nwav = 0;
+  std::vector<int> db, levels;
+  std::vector<std::vector<double>> filters;
+  void* d_T = nullptr;  // work images
+  void* d_W = nullptr;
+  void* d_stage = nullptr;  // complex128 host staging (imsize * nwav)
+  void* d_in = nullptr;     // staged input in the operator's precision
+  void* d_out = nullptr;
+  std::vector<void*> allocs;
+  void* dalloc(size_t bytes) {
+    void* p = nullptr;
+    PF_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    allocs.push_back(p);
+    return p;
+  }
+  size_t esz() const { return prec == PF_C64 ? 8 : 16; }
+  int64_t n() const { return (int64_t)ny * nx; }
+  ~pf_wavelet() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace pf {
+// Psi^dagger: image -> nwav stacked coefficient images
+template <typename R>
+void wavelet_analysis(pf_wavelet* wv, const cpx<R>* x, cpx<R>* c, hipStream_t s) {
+  const R scl = (R)(1.0 / std::sqrt((double)wv->nwav));
+  const int64_t N = wv->n();
+  for (int i = 0; i < wv->nwav; ++i) {
+    cpx<R>* ci = c + (int64_t)i * N;
+    if (wv->db[i] == 0) {
+      hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, x, ci, N, scl);
+      continue;
+    }
+    const WavFilt<R> f = make_filt<R>(wv->filters[i]);
+    if (wv->levels[i] == 0) {
+      hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, x, ci, N, scl);
+      continue;
+    }
+    dwt2<R>(x, ci, static_cast<cpx<R>*>(wv->d_T), wv->ny, wv->nx, wv->levels[i], f, s);
+    // the SARA factor (a pass over the stacked entry; the coefficients of the finer levels are not
+    // touched again by the coarser ones)
+    if (scl != R(1)) scale_vec<R>(ci, N, (double)scl, s);
+  }
+  PF_HIP(hipGetLastError());
+}
+
+// Psi: nwav stacked coefficient images -> image
+template <typename R>
+void wavelet_synthesis(pf_wavelet* wv, const cpx<R>* c, cpx<R>* x, hipStream_t s) {
+  const R scl = (R)(1.0 / std::sqrt((double)wv->nwav));
+  const int64_t N = wv->n();
+  for (int i = 0; i < wv->nwav; ++i) {
+    const cpx<R>* ci = c + (int64_t)i * N;
+    const WavFilt<R> f = wv->db[i] ? make_filt<R>(wv->filters[i]) : WavFilt<R>{};
+    idwt2<R>(ci, x, static_cast<cpx<R>*>(wv->d_W), static_cast<cpx<R>*>(wv->d_T), wv->ny, wv->nx,
+             wv->db[i] ? wv->levels[i] : 0, f, scl, i > 0, s);
+  }
+  PF_HIP(hipGetLastError());
+}
+template void wavelet_analysis<float>(pf_wavelet*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void wavelet_analysis<double>(pf_wavelet*, const cpx<double>*, cpx<double>*, hipStream_t);
+template void wavelet_synthesis<float>(pf_wavelet*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void wavelet_synthesis<double>(pf_wavelet*, const cpx<double>*, cpx<double>*, hipStream_t);
+
+}  // namespace pf
+
+extern "C" {
+
+int pf_wavelet_filter(int32_t db, double* lo, int32_t* len) {
+  return guarded([&] {
+    const std::vector<double> h = daubechies_filter(db);
+    if (len) *len = (int32_t)h.size();
+    if (lo) std::copy(h.begin(), h.end(), lo);
+  });
+}
+
+int pf_wavelet_create(int32_t imsizey, int32_t imsizex, int32_t nwav, const int32_t* db, const int32_t* levels,
+                      int32_t precision, pf_wavelet** out) {
+  if (!out || !db || !levels) return fail(PF_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<pf_wavelet> wv(new pf_wavelet());
+  int rc = guarded([&] {
+    if (imsizey < 1 || imsizex < 1 || nwav < 1) throw Error(PF_ERR_ARG, "image size and dictionary size must be positive");
+    if (precision != PF_C64 && precision != PF_C128) throw Error(PF_ERR_ARG, "precision must be PF_C64 or PF_C128");
+    wv->ny = imsizey;
+    wv->nx = imsizex;
+    wv->prec = precision;
+    wv->nwav = nwav;
+    for (int i = 0; i < nwav; ++i) {
+      if (db[i] < 0 || db[i] > PF_DB_MAX) throw Error(PF_ERR_ARG, "wavelet must be Dirac (0) or DB1..DB" + std::to_string(PF_DB_MAX));
+      if (levels[i] < 0 || (db[i] && ((imsizey % (1 << levels[i])) || (imsizex % (1 << levels[i])))))
+        throw Error(PF_ERR_ARG, "image sides must be divisible by 2^levels");
+      wv->db.push_back(db[i]);
+      wv->levels.push_back(db[i] ? levels[i] : 0);
+      wv->filters.push_back(db[i] ? daubechies_filter(db[i]) : std::vector<double>());
+      // the periodic filters need at least as many samples per line at the coarsest level as taps / 2
+    }
+    const size_t b = wv->esz() * (size_t)wv->n();
+    wv->d_T = wv->dalloc(b);
+    wv->d_W = wv->dalloc(b);
+  });
+  if (rc == PF_OK) *out = wv.release();
+  return rc;
+}
+
+// coefficients (nwav * imsizey * imsizex) -> image: Psi, the LinearTransform's direct operator
+int pf_wavelet_forward(pf_wavelet* wv, const void* coeffs, void* image, int c_loc, int x_loc, pf_stream stream) {
+  return guarded([&] {
+    if (!wv || !coeffs || !image) throw Error(PF_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t N = wv->n(), NC = N * wv->nwav;
+    const void* din = coeffs;
+    if (c_loc == PF_HOST || x_loc == PF_HOST) {
+      if (!wv->d_stage) {
+        wv->d_stage = wv->dalloc(16 * (size_t)NC);
+        wv->d_in = wv->dalloc(wv->esz() * (size_t)NC);
+        wv->d_out = wv->dalloc(wv->esz() * (size_t)NC);
+      }
+    }
+    if (c_loc == PF_HOST) {
+      PF_HIP(hipMemcpyAsync(wv->prec == PF_C64 ? wv->d_stage : wv->d_in, coeffs, 16 * NC, hipMemcpyHostToDevice, s));
+      if (wv->prec == PF_C64) c128_to_c64((const cpx<double>*)wv->d_stage, (cpx<float>*)wv->d_in, NC, s);
+      din = wv->d_in;
+    } else if (c_loc != PF_DEVICE) {
+      throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+    }
+    void* dout = x_loc == PF_DEVICE ? image : wv->d_out;
+    if (wv->prec == PF_C64) wavelet_synthesis<float>(wv, (const cpx<float>*)din, (cpx<float>*)dout, s);
+    else wavelet_synthesis<double>(wv, (const cpx<double>*)din, (cpx<double>*)dout, s);
+    if (x_loc == PF_HOST) {
+      if (wv->prec == PF_C64) {
+        c64_to_c128((const cpx<float>*)dout, (cpx<double>*)wv->d_stage, N, s);
+        PF_HIP(hipMemcpyAsync(image, wv->d_stage, 16 * N, hipMemcpyDeviceToHost, s));
+      } else {
+        PF_HIP(hipMemcpyAsync(image, dout, 16 * N, hipMemcpyDeviceToHost, s));
+      }
+      PF_HIP(hipStreamSynchronize(s));
+    } else if (x_loc != PF_DEVICE) {
+      throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+    }
+  });
+}
+
+// image -> coefficients: Psi^dagger, the LinearTransform's adjoint
+int pf_wavelet_adjoint(pf_wavelet* wv, const void* image, void* coeffs, int x_loc, int c_loc, pf_stream stream) {
+  return guarded([&] {
+    if (!wv || !coeffs || !image) throw Error(PF_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t N = wv->n(), NC = N * wv->nwav;
+    const void* din = image;
+    if (c_loc == PF_HOST || x_loc == PF_HOST) {
+      if (!wv->d_stage) {
+        wv->d_stage = wv->dalloc(16 * (size_t)NC);
+        wv->d_in = wv->dalloc(wv->esz() * (size_t)NC);
+        wv->d_out = wv->dalloc(wv->esz() * (size_t)NC);
+      }
+    }
+    if (x_loc == PF_HOST) {
+      PF_HIP(hipMemcpyAsync(wv->prec == PF_C64 ? wv->d_stage : wv->d_in, image, 16 * N, hipMemcpyHostToDevice, s));
+      if (wv->prec == PF_C64) c128_to_c64((const cpx<double>*)wv->d_stage, (cpx<float>*)wv->d_in, N, s);
+      din = wv->d_in;
+    } else if (x_loc != PF_DEVICE) {
+      throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+    }
+    void* dout = c_loc == PF_DEVICE ? coeffs : wv->d_out;
+    if (wv->prec == PF_C64) wavelet_analysis<float>(wv, (const cpx<float>*)din, (cpx<float>*)dout, s);
+    else wavelet_analysis<double>(wv, (const cpx<double>*)din, (cpx<double>*)dout, s);
+    if (c_loc == PF_HOST) {
+      if (wv->prec == PF_C64) {
+        c64_to_c128((const cpx<float>*)dout, (cpx<double>*)wv->d_stage, NC, s);
+        PF_HIP(hipMemcpyAsync(coeffs, wv->d_stage, 16 * NC, hipMemcpyDeviceToHost, s));
+      } else {
+        PF_HIP(hipMemcpyAsync(coeffs, dout, 16 * NC, hipMemcpyDeviceToHost, s));
+      }
+      PF_HIP(hipStreamSynchronize(s));
+    } else if (c_loc != PF_DEVICE) {
+      throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+    }
+  });
+}
+
+int pf_wavelet_destroy(pf_wavelet* wv) {
+  delete wv;
+  return PF_OK;
+}
+
+}  // extern "C"

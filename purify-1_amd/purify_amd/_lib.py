@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libpurify_amd.so")
 PF_OK, PF_ERR_ARG, PF_ERR_HIP, PF_ERR_UNSUPPORTED, PF_ERR_NOMEM, PF_ERR_STATE = 0, -1, -2, -3, -4, -5
 PF_C64, PF_C128 = 0, 1
 PF_HOST, PF_DEVICE = 0, 1
+PF_DB_MAX = 10  # purify_amd.h: Daubechies orders DB1..DB10
 PF_LAMBDA, PF_RADIANS, PF_PIXELS = 0, 1, 2
 
 
@@ -61,6 +62,12 @@ SIGNATURES = {
     "pf_op_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "pf_op_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "pf_op_apply_normal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "pf_wavelet_filter": (C.c_int, [C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
+    "pf_wavelet_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                    C.POINTER(C.c_void_p)]),
+    "pf_wavelet_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "pf_wavelet_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "pf_wavelet_destroy": (C.c_int, [C.c_void_p]),
     "pf_psf_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.POINTER(C.c_void_p)]),
     "pf_psf_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "pf_psf_norm": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
