@@ -126,6 +126,48 @@ int pf_wavelet_forward(pf_wavelet *wv, const void *coeffs, void *image, int c_lo
 int pf_wavelet_adjoint(pf_wavelet *wv, const void *image, void *coeffs, int x_loc, int c_loc, pf_stream stream);
 int pf_wavelet_destroy(pf_wavelet *wv);
 
+/* ---- Imaging PADMM on the device (sopt::algorithm::ImagingProximalADMM with the parameters of
+ * padmm_factory, algorithm_factory.h:36-81) for a tight-frame dictionary: one orthonormal wavelet,
+ * as main.cc:121-126 selects when the dictionary has one entry ----
+ * Iteration (SOPT variable names): z = P_eps(-lambda - r); x~ = x - Phi^dagger(r + lambda + z) / nu;
+ * x = x~ + Psi(soft(Psi^dagger x~, gamma nu) - Psi^dagger x~) / nu; r = Phi x - y;
+ * lambda += lagrange_update_scale (r + z); from x = Phi^dagger y / nu, r = Phi x - y, lambda = 0.
+ * SOPT is not in the reference tree: this is a restatement of the published algorithm and its
+ * agreement with SOPT's iterates is unpinned.  Norms are FP64 reductions on the device; an iteration
+ * runs without host synchronisation and can be captured once into a hipGraph and replayed.  phi and
+ * psi must outlive the solver, and no other call may use them while it runs (their work buffers are
+ * shared). */
+typedef struct pf_padmm pf_padmm;
+typedef struct pf_padmm_params {
+  double gamma;                 /* l1 weight; <= 0: max |Psi^dagger Phi^dagger y| * 1e-3 (algorithm_factory.h:70-73) */
+  double epsilon;               /* l2-ball radius (utilities::calculate_l2_radius), > 0 */
+  double nu;                    /* 1 (algorithm_factory.h:62; the l1 proximal's nu is also 1) */
+  double lagrange_update_scale; /* 0.9 */
+  double relative_variation;    /* 1e-3 */
+  double residual_tolerance;    /* <= 0: epsilon (algorithm_factory.h:74) */
+  int32_t itermax;              /* the most iterations the solver will run (sizes its statistics) */
+} pf_padmm_params;
+typedef struct pf_padmm_result {
+  int32_t niters, converged;
+  double relative_variation, residual_norm, gamma;
+} pf_padmm_result;
+enum { PF_PADMM_IMAGE = 0, PF_PADMM_RESIDUAL = 1 };
+int pf_padmm_params_init(pf_padmm_params *p);
+/* y: the visibilities (host complex128, or device in the operator's precision), M of them. */
+int pf_padmm_create(pf_op *phi, pf_wavelet *psi, const void *y, int y_loc, const pf_padmm_params *params,
+                    pf_padmm **out);
+/* Runs iters more iterations (use_graph: replay one captured iteration).  check_every > 0: every that
+ * many iterations, stop once ||x - x_prev|| / ||x|| < relative_variation and ||r|| <
+ * residual_tolerance (check_every = 1 stops exactly at the first converged iteration, as SOPT does);
+ * 0: run all iters. */
+int pf_padmm_run(pf_padmm *h, int32_t iters, int32_t use_graph, int32_t check_every, pf_padmm_result *res);
+int pf_padmm_reset(pf_padmm *h);
+/* what: PF_PADMM_IMAGE (x, imsizey*imsizex) or PF_PADMM_RESIDUAL (r = Phi x - y, M) */
+int pf_padmm_get(pf_padmm *h, int32_t what, void *out, int out_loc);
+/* per-iteration FP64 statistics (||x - x_prev||^2, ||x||^2, ||r||^2) of up to cap iterations */
+int pf_padmm_stats(pf_padmm *h, double *out, int32_t cap, int32_t *count);
+int pf_padmm_destroy(pf_padmm *h);
+
 /* Power method on Phi^dagger Phi (SOPT power_method as called at operators.h:390-391) over the
  * operator as it currently is (including its norm).  start: host complex128 of imsizey*imsizex,
  * or NULL for all ones.  Iterates in the operator's precision, reduces norms in FP64. */

@@ -517,6 +517,58 @@ def sara_synthesis(coef, ny, nx, entries):
     return x
 
 
+def soft_threshold(c, t):
+    """soft(c, t) = c max(0, 1 - t / |c|) (complex magnitude shrinkage)."""
+    m = np.abs(c)
+    f = np.where(m > t, 1.0 - t / np.where(m > 0, m, 1.0), 0.0)
+    return c * f
+
+
+def padmm(op: "Operator", entries, y, epsilon, iters, gamma=None, nu=1.0, rho=0.9, relative_variation=1e-3,
+          residual_tolerance=None, stop=False):
+    """Imaging proximal ADMM with a tight-frame l1 proximal (the iteration SOPT's ImagingProximalADMM runs
+    with padmm_factory's parameters, algorithm_factory.h:36-81), in FP64:
+        z = P_eps(-lambda - r);  x~ = x - Phi^dagger(r + lambda + z) / nu
+        x = x~ + Psi(soft(Psi^dagger x~, gamma nu) - Psi^dagger x~) / nu
+        r = Phi x - y;  lambda += rho (r + z)
+    from x = Phi^dagger y / nu, r = Phi x - y, lambda = 0; gamma = max |Psi^dagger Phi^dagger y| * 1e-3 if
+    not given.  Restated from the published algorithm (SOPT is not in the reference tree): parity with
+    SOPT's iterates is unpinned.  Returns (x, stats, gamma, niters): stats[k] = (||x_k - x_(k-1)||^2,
+    ||x_k||^2, ||r_k||^2)."""
+    ny, nx = op.ny, op.nx
+    y = np.asarray(y, np.complex128)
+    psit = lambda v: sara_analysis(v, ny, nx, entries)  # noqa: E731
+    psi = lambda c: sara_synthesis(c, ny, nx, entries)  # noqa: E731
+    if gamma is None or gamma <= 0:
+        gamma = float(np.abs(psit(op.adjoint(y))).max()) * 1e-3
+    res_tol = epsilon if not residual_tolerance else residual_tolerance
+    x = op.adjoint(y) / nu
+    r = op.forward(x) - y
+    lam = np.zeros_like(y)
+    stats = []
+    it = 0
+    for it in range(1, iters + 1):
+        v = -lam - r
+        nv = np.linalg.norm(v)
+        z = v * (epsilon / nv if nv > epsilon else 1.0)
+        xu = x - op.adjoint(r + lam + z) / nu
+        c = psit(xu)
+        xn = xu + psi(soft_threshold(c, gamma * nu) - c) / nu
+        r = op.forward(xn) - y
+        lam = lam + rho * (r + z)
+        stats.append((float(np.vdot(xn - x, xn - x).real), float(np.vdot(xn, xn).real), float(np.vdot(r, r).real)))
+        x = xn
+        if stop and np.sqrt(stats[-1][0] / stats[-1][1]) < relative_variation and np.sqrt(stats[-1][2]) < res_tol:
+            break
+    return x, np.array(stats), gamma, it
+
+
+def snr_db(model, solution):
+    """20 log10(||model|| / ||model - solution||)."""
+    model, solution = np.asarray(model), np.asarray(solution)
+    return float(20 * np.log10(np.linalg.norm(model) / np.linalg.norm(model - solution)))
+
+
 def omp_threads():
     return lib().orc_omp_threads()
 

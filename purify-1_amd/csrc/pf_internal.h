@@ -294,3 +294,28 @@ template <typename R>
 KParams<R> make_kparams(int kind, int J, double os);
 
 }  // namespace pf
+
+// ---- access across translation units (the solver drives the operator and the dictionary) ----
+struct pf_op;
+struct pf_wavelet;
+namespace pf {
+// y = Phi x, x = Phi^dagger y on device vectors in the operator's precision, on stream s
+template <typename R>
+void op_forward(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s);
+template <typename R>
+void op_adjoint(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s);
+struct OpShape {
+  int ny, nx, prec;
+  int64_t M;
+};
+OpShape op_shape(const pf_op* op);
+// Psi^dagger (image -> stacked coefficients) and Psi on device vectors
+template <typename R>
+void wavelet_analysis(pf_wavelet* wv, const cpx<R>* x, cpx<R>* c, hipStream_t s);
+template <typename R>
+void wavelet_synthesis(pf_wavelet* wv, const cpx<R>* c, cpx<R>* x, hipStream_t s);
+struct WaveletShape {
+  int ny, nx, prec, nwav;
+};
+WaveletShape wavelet_shape(const pf_wavelet* wv);
+}  // namespace pf

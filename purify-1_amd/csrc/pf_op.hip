@@ -1171,6 +1171,22 @@ __global__ void k_kernel_eval(KParams<R> p, const double* x, double* out, int64_
 // ---------------------------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------------------------
+namespace pf {
+template <typename R>
+void op_forward(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s) {
+  forward_dev<R>(op, x, y, s);
+}
+template <typename R>
+void op_adjoint(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
+  adjoint_dev<R>(op, y, x, s);
+}
+template void op_forward<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void op_forward<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
+template void op_adjoint<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void op_adjoint<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
+OpShape op_shape(const pf_op* op) { return OpShape{op->ny, op->nx, op->prec, op->M}; }
+}  // namespace pf
+
 extern "C" {
 
 void pf_op_desc_init(pf_op_desc* d) {

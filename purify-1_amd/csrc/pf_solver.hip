@@ -1,0 +1,473 @@
+// pf_solver.hip -- the imaging PADMM of PURIFY's pipeline on the device: SOPT's
+// sopt::algorithm::ImagingProximalADMM with the parameters padmm_factory sets
+// (cpp/purify/algorithm_factory.h:36-81), for a tight-frame dictionary (one orthonormal wavelet:
+// main.cc:121-126 sets tight_frame = sara.size() < 2).
+//
+// SOPT is not in the reference tree, so the iteration is restated from the published proximal ADMM
+// (Carrillo, McEwen & Wiaux 2014, "PURIFY: a new approach to radio-interferometric imaging",
+// section 3; Onose et al. 2016, algorithm 1) in SOPT's variable names:
+//   z      = P_eps(-lambda - r)                         l2-ball projection, radius eps
+//   x~     = x - Phi^dagger(r + lambda + z) / nu
+//   x      = x~ + Psi(soft(Psi^dagger x~, gamma nu) - Psi^dagger x~) / nu   tight-frame l1 proximal
+//   r      = Phi x - y
+//   lambda = lambda + rho (r + z)                        rho = lagrange_update_scale
+// from x = Phi^dagger y / nu, r = Phi x - y, lambda = 0.  Convergence: ||x - x_prev|| / ||x|| <
+// relative_variation and ||r|| < residual_tolerance.  soft(c, t) = c max(0, 1 - t / |c|).
+//
+// Vectors stay in the operator's precision; every norm is reduced in FP64 on the device in a fixed
+// order, and the projection reads its norm from device memory, so an iteration has no host
+// synchronisation and is captured once into a hipGraph that is replayed.  Per-iteration FP64
+// statistics (||x - x_prev||^2, ||x||^2, ||r||^2) are appended to a device array through a device
+// counter, so a replayed graph records every iteration.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "../../include/purify_amd.h"
+#include "pf_internal.h"
+
+namespace pf {
+namespace {
+
+constexpr int SR_BLOCKS = 512, SR_THREADS = 256;
+
+template <typename R>
+__device__ inline double norm2d(cpx<R> a) {
+  return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+}
+
+// partials[b] = sum |a - b|^2 (B given) or |a|^2 over a grid-stride slice; fixed order
+template <typename R, bool DIFF>
+__global__ void __launch_bounds__(SR_THREADS) k_sq_partial(const cpx<R>* __restrict__ a, const cpx<R>* __restrict__ b,
+                                                           int64_t n, double* __restrict__ part) {
+  __shared__ double sh[SR_THREADS];
+  double acc = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * SR_THREADS + threadIdx.x; k < n; k += (int64_t)SR_BLOCKS * SR_THREADS) {
+    cpx<R> v = a[k];
+    if (DIFF) {
+      const cpx<R> w = b[k];
+      v.x -= w.x;
+      v.y -= w.y;
+    }
+    acc += norm2d(v);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = SR_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void __launch_bounds__(SR_THREADS) k_sq_final(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[SR_THREADS];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < SR_BLOCKS; k += SR_THREADS) acc += part[k];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = SR_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+template <typename R>
+void sq_norm(const cpx<R>* a, const cpx<R>* b, int64_t n, double* part, double* out, hipStream_t s) {
+  if (b) hipLaunchKernelGGL((k_sq_partial<R, true>), dim3(SR_BLOCKS), dim3(SR_THREADS), 0, s, a, b, n, part);
+  else hipLaunchKernelGGL((k_sq_partial<R, false>), dim3(SR_BLOCKS), dim3(SR_THREADS), 0, s, a, b, n, part);
+  hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(SR_THREADS), 0, s, part, out);
+}
+
+inline dim3 vgrid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096))); }
+#define GRID_STRIDE(k, n) for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (n); k += (int64_t)gridDim.x * blockDim.x)
+
+// v = -lambda - r
+template <typename R>
+__global__ void k_ball_in(const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ r, cpx<R>* __restrict__ v, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = lam[k], b = r[k];
+    v[k] = mk<R>(-a.x - b.x, -a.y - b.y);
+  }
+}
+// z = P_eps(v) (v scaled by min(1, eps / ||v||), ||v||^2 read from *nrm2); w = r + lambda + z
+template <typename R>
+__global__ void k_ball_out(const cpx<R>* __restrict__ v, const double* __restrict__ nrm2, double eps,
+                           const cpx<R>* __restrict__ r, const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ z,
+                           cpx<R>* __restrict__ w, int64_t n) {
+  const double nv = sqrt(*nrm2);
+  const R f = (R)(nv > eps ? eps / nv : 1.0);
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = v[k], b = r[k], c = lam[k];
+    const cpx<R> zz = mk<R>(a.x * f, a.y * f);
+    z[k] = zz;
+    w[k] = mk<R>(b.x + c.x + zz.x, b.y + c.y + zz.y);
+  }
+}
+// xprev = x;  xu = x - g / nu
+template <typename R>
+__global__ void k_grad_step(cpx<R>* __restrict__ x, const cpx<R>* __restrict__ g, cpx<R>* __restrict__ xprev,
+                            cpx<R>* __restrict__ xu, R inv_nu, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = x[k], b = g[k];
+    xprev[k] = a;
+    xu[k] = mk<R>(a.x - b.x * inv_nu, a.y - b.y * inv_nu);
+  }
+}
+// c <- soft(c, t) - c
+template <typename R>
+__global__ void k_soft_diff(cpx<R>* __restrict__ c, R t, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = c[k];
+    const R m = sqrt(a.x * a.x + a.y * a.y);
+    const R f = m > t ? (R(1) - t / m) : R(0);
+    c[k] = mk<R>(a.x * f - a.x, a.y * f - a.y);
+  }
+}
+// x = xu + d / nu
+template <typename R>
+__global__ void k_prox_out(const cpx<R>* __restrict__ xu, const cpx<R>* __restrict__ d, cpx<R>* __restrict__ x, R inv_nu,
+                           int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = xu[k], b = d[k];
+    x[k] = mk<R>(a.x + b.x * inv_nu, a.y + b.y * inv_nu);
+  }
+}
+// r = fx - y; lambda += rho (r + z)
+template <typename R>
+__global__ void k_residual_dual(const cpx<R>* __restrict__ fx, const cpx<R>* __restrict__ y, const cpx<R>* __restrict__ z,
+                                cpx<R>* __restrict__ r, cpx<R>* __restrict__ lam, R rho, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = fx[k], b = y[k], c = z[k];
+    const cpx<R> rr = mk<R>(a.x - b.x, a.y - b.y);
+    r[k] = rr;
+    const cpx<R> l = lam[k];
+    lam[k] = mk<R>(l.x + rho * (rr.x + c.x), l.y + rho * (rr.y + c.y));
+  }
+}
+// r = fx - y
+template <typename R>
+__global__ void k_sub(const cpx<R>* __restrict__ a, const cpx<R>* __restrict__ b, cpx<R>* __restrict__ r, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> u = a[k], v = b[k];
+    r[k] = mk<R>(u.x - v.x, u.y - v.y);
+  }
+}
+template <typename R>
+__global__ void k_scale_copy(const cpx<R>* __restrict__ a, cpx<R>* __restrict__ b, R f, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> u = a[k];
+    b[k] = mk<R>(u.x * f, u.y * f);
+  }
+}
+template <typename R>
+__global__ void k_zero(cpx<R>* __restrict__ a, int64_t n) {
+  GRID_STRIDE(k, n) a[k] = mk<R>(R(0), R(0));
+}
+// stats[*counter] = cur (3 doubles); ++*counter (capacity checked: the run never exceeds it)
+__global__ void k_record(const double* __restrict__ cur, double* __restrict__ stats, int* __restrict__ counter, int cap) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int c = *counter;
+    if (c < cap) {
+      stats[3 * c] = cur[0];
+      stats[3 * c + 1] = cur[1];
+      stats[3 * c + 2] = cur[2];
+    }
+    *counter = c + 1;
+  }
+}
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+}  // namespace
+}  // namespace pf
+
+using namespace pf;
+
+struct pf_padmm {
+  pf_op* phi = nullptr;       // not owned
+  pf_wavelet* psi = nullptr;  // not owned
+  int prec = PF_C64;
+  int64_t N = 0, M = 0, NC = 0;
+  pf_padmm_params par{};
+  double gamma = 0;
+  // device state (operator precision)
+  void *x = nullptr, *xprev = nullptr, *xu = nullptr, *g = nullptr, *c = nullptr, *d = nullptr;
+  void *y = nullptr, *r = nullptr, *lam = nullptr, *z = nullptr, *v = nullptr, *fx = nullptr;
+  double* part = nullptr;  // reduction partials
+  double* cur = nullptr;   // [0] ||x - xprev||^2, [1] ||x||^2, [2] ||r||^2, [3] ||v||^2
+  double* stats = nullptr;
+  int* counter = nullptr;
+  int cap = 0;
+  int iters_done = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void*> allocs;
+  void* dalloc(size_t bytes) {
+    void* p = nullptr;
+    PF_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    allocs.push_back(p);
+    return p;
+  }
+  ~pf_padmm() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace pf {
+namespace {
+
+template <typename R>
+cpx<R>* V(void* p) {
+  return static_cast<cpx<R>*>(p);
+}
+
+// one PADMM iteration on h->stream (no host synchronisation)
+template <typename R>
+void padmm_iteration(pf_padmm* h) {
+  hipStream_t s = h->stream;
+  const int64_t N = h->N, M = h->M, NC = h->NC;
+  const R inv_nu = (R)(1.0 / h->par.nu);
+  hipLaunchKernelGGL((k_ball_in<R>), vgrid(M), dim3(256), 0, s, V<R>(h->lam), V<R>(h->r), V<R>(h->v), M);
+  sq_norm<R>(V<R>(h->v), nullptr, M, h->part, h->cur + 3, s);
+  // fx holds r + lambda + z until the forward apply below overwrites it
+  hipLaunchKernelGGL((k_ball_out<R>), vgrid(M), dim3(256), 0, s, V<R>(h->v), h->cur + 3, h->par.epsilon, V<R>(h->r),
+                     V<R>(h->lam), V<R>(h->z), V<R>(h->fx), M);
+  op_adjoint<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
+  hipLaunchKernelGGL((k_grad_step<R>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev), V<R>(h->xu),
+                     inv_nu, N);
+  wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
+  hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.nu), NC);
+  wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
+  hipLaunchKernelGGL((k_prox_out<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x), inv_nu, N);
+  op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
+  hipLaunchKernelGGL((k_residual_dual<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->z), V<R>(h->r),
+                     V<R>(h->lam), (R)h->par.lagrange_update_scale, M);
+  sq_norm<R>(V<R>(h->x), V<R>(h->xprev), N, h->part, h->cur + 0, s);
+  sq_norm<R>(V<R>(h->x), nullptr, N, h->part, h->cur + 1, s);
+  sq_norm<R>(V<R>(h->r), nullptr, M, h->part, h->cur + 2, s);
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, h->cur, h->stats, h->counter, h->cap);
+  PF_HIP(hipGetLastError());
+}
+
+// x = Phi^dagger y / nu, r = Phi x - y, lambda = 0; gamma from |Psi^dagger Phi^dagger y| if not given
+template <typename R>
+void padmm_init(pf_padmm* h) {
+  hipStream_t s = h->stream;
+  const int64_t N = h->N, M = h->M;
+  op_adjoint<R>(h->phi, V<R>(h->y), V<R>(h->g), s);
+  if (!(h->par.gamma > 0)) {
+    // algorithm_factory.h:70-73: gamma = max |Psi^dagger Phi^dagger y| * 1e-3
+    wavelet_analysis<R>(h->psi, V<R>(h->g), V<R>(h->c), s);
+    std::vector<cpx<R>> hc((size_t)h->NC);
+    PF_HIP(hipMemcpyAsync(hc.data(), h->c, sizeof(cpx<R>) * h->NC, hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+    double mx = 0;
+    for (const auto& a : hc) mx = std::max(mx, std::sqrt((double)a.x * a.x + (double)a.y * a.y));
+    h->gamma = mx * 1e-3;
+  } else {
+    h->gamma = h->par.gamma;
+  }
+  hipLaunchKernelGGL((k_scale_copy<R>), vgrid(N), dim3(256), 0, s, V<R>(h->g), V<R>(h->x), (R)(1.0 / h->par.nu), N);
+  op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
+  hipLaunchKernelGGL((k_sub<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->r), M);
+  hipLaunchKernelGGL((k_zero<R>), vgrid(M), dim3(256), 0, s, V<R>(h->lam), M);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, h->counter, 0);
+  PF_HIP(hipStreamSynchronize(s));
+  h->iters_done = 0;
+}
+
+template <typename R>
+void padmm_run(pf_padmm* h, int iters, bool use_graph, int check_every, pf_padmm_result* res) {
+  hipStream_t s = h->stream;
+  if (h->iters_done + iters > h->cap) throw Error(PF_ERR_ARG, "more iterations than the solver was created for");
+  if (use_graph && !h->exec) {
+    if (h->iters_done == 0) {
+      // one eager iteration first, so every kernel attribute is set before capture; then
+      // re-initialise so the captured run starts from the initial state
+      padmm_iteration<R>(h);
+      PF_HIP(hipStreamSynchronize(s));
+      padmm_init<R>(h);
+    }
+    // capture records the iteration without running it: the state is unchanged
+    PF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    padmm_iteration<R>(h);
+    PF_HIP(hipStreamEndCapture(s, &h->graph));
+    PF_HIP(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  }
+  const int chunk = check_every > 0 ? check_every : iters;
+  int done = 0;
+  bool converged = false;
+  std::vector<double> st;
+  while (done < iters && !converged) {
+    const int k = std::min(chunk, iters - done);
+    for (int i = 0; i < k; ++i) {
+      if (use_graph) PF_HIP(hipGraphLaunch(h->exec, s));
+      else padmm_iteration<R>(h);
+    }
+    done += k;
+    h->iters_done += k;
+    if (check_every > 0) {
+      // convergence of the chunk's last iteration (SOPT checks after every iteration: chunk 1)
+      st.resize(3);
+      PF_HIP(hipMemcpyAsync(st.data(), h->stats + 3 * (h->iters_done - 1), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+      PF_HIP(hipStreamSynchronize(s));
+      const double relvar = st[1] > 0 ? std::sqrt(st[0] / st[1]) : 0.0;
+      converged = relvar < h->par.relative_variation && std::sqrt(st[2]) < h->par.residual_tolerance;
+    }
+  }
+  PF_HIP(hipStreamSynchronize(s));
+  if (res) {
+    st.resize(3);
+    PF_HIP(hipMemcpy(st.data(), h->stats + 3 * (h->iters_done - 1), 3 * sizeof(double), hipMemcpyDeviceToHost));
+    res->niters = h->iters_done;
+    res->converged = converged ? 1 : 0;
+    res->relative_variation = st[1] > 0 ? std::sqrt(st[0] / st[1]) : 0.0;
+    res->residual_norm = std::sqrt(st[2]);
+    res->gamma = h->gamma;
+  }
+}
+
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+int pf_padmm_params_init(pf_padmm_params* p) {
+  if (!p) return fail(PF_ERR_ARG, "null argument");
+  *p = pf_padmm_params{};
+  p->gamma = 0;  // computed from the data (algorithm_factory.h:70-73)
+  p->epsilon = 0;
+  p->nu = 1.0;
+  p->lagrange_update_scale = 0.9;
+  p->relative_variation = 1e-3;
+  p->residual_tolerance = 0;  // 0: epsilon (algorithm_factory.h:74)
+  p->itermax = 500;
+  return PF_OK;
+}
+
+int pf_padmm_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const pf_padmm_params* params,
+                    pf_padmm** out) {
+  if (!phi || !psi || !y || !params || !out) return fail(PF_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<pf_padmm> h(new pf_padmm());
+  int rc = guarded([&] {
+    const OpShape os = op_shape(phi);
+    const WaveletShape ws = wavelet_shape(psi);
+    if (os.ny != ws.ny || os.nx != ws.nx) throw Error(PF_ERR_ARG, "operator and dictionary image sizes differ");
+    if (os.prec != ws.prec) throw Error(PF_ERR_ARG, "operator and dictionary precisions differ");
+    if (ws.nwav != 1)
+      throw Error(PF_ERR_UNSUPPORTED,
+                  "the device PADMM needs a tight frame (one wavelet basis); SARA's iterative l1 proximal is not built");
+    if (!(params->epsilon > 0)) throw Error(PF_ERR_ARG, "epsilon (the l2-ball radius) must be positive");
+    if (!(params->nu > 0)) throw Error(PF_ERR_ARG, "nu must be positive");
+    if (params->itermax < 1) throw Error(PF_ERR_ARG, "itermax must be positive");
+    h->phi = phi;
+    h->psi = psi;
+    h->prec = os.prec;
+    h->N = (int64_t)os.ny * os.nx;
+    h->M = os.M;
+    h->NC = h->N * ws.nwav;
+    h->par = *params;
+    if (!(h->par.residual_tolerance > 0)) h->par.residual_tolerance = h->par.epsilon;
+    const size_t e = os.prec == PF_C64 ? 8 : 16;
+    for (void** p : {&h->x, &h->xprev, &h->xu, &h->g, &h->d}) *p = h->dalloc(e * h->N);
+    h->c = h->dalloc(e * h->NC);
+    for (void** p : {&h->y, &h->r, &h->lam, &h->z, &h->v, &h->fx}) *p = h->dalloc(e * std::max<int64_t>(h->M, 1));
+    h->part = (double*)h->dalloc(sizeof(double) * SR_BLOCKS);
+    h->cur = (double*)h->dalloc(sizeof(double) * 4);
+    h->cap = params->itermax;
+    h->stats = (double*)h->dalloc(sizeof(double) * 3 * h->cap);
+    h->counter = (int*)h->dalloc(sizeof(int));
+    PF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    // the visibilities: host complex128 or device in the operator's precision
+    if (y_loc == PF_HOST) {
+      if (h->prec == PF_C64) {
+        void* tmp = nullptr;
+        PF_HIP(hipMalloc(&tmp, 16 * std::max<int64_t>(h->M, 1)));
+        PF_HIP(hipMemcpyAsync(tmp, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
+        c128_to_c64((const cpx<double>*)tmp, (cpx<float>*)h->y, h->M, h->stream);
+        PF_HIP(hipStreamSynchronize(h->stream));
+        PF_HIP(hipFree(tmp));
+      } else {
+        PF_HIP(hipMemcpyAsync(h->y, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
+      }
+    } else if (y_loc == PF_DEVICE) {
+      PF_HIP(hipMemcpyAsync(h->y, y, e * h->M, hipMemcpyDeviceToDevice, h->stream));
+    } else {
+      throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+    }
+    if (h->prec == PF_C64) padmm_init<float>(h.get());
+    else padmm_init<double>(h.get());
+  });
+  if (rc == PF_OK) *out = h.release();
+  return rc;
+}
+
+int pf_padmm_run(pf_padmm* h, int32_t iters, int32_t use_graph, int32_t check_every, pf_padmm_result* res) {
+  return guarded([&] {
+    if (!h) throw Error(PF_ERR_ARG, "null handle");
+    if (iters < 0) throw Error(PF_ERR_ARG, "iterations must be >= 0");
+    if (iters == 0) return;
+    if (h->prec == PF_C64) padmm_run<float>(h, iters, use_graph != 0, check_every, res);
+    else padmm_run<double>(h, iters, use_graph != 0, check_every, res);
+  });
+}
+
+int pf_padmm_reset(pf_padmm* h) {
+  return guarded([&] {
+    if (!h) throw Error(PF_ERR_ARG, "null handle");
+    if (h->prec == PF_C64) padmm_init<float>(h);
+    else padmm_init<double>(h);
+  });
+}
+
+int pf_padmm_get(pf_padmm* h, int32_t what, void* out, int out_loc) {
+  return guarded([&] {
+    if (!h || !out) throw Error(PF_ERR_ARG, "null argument");
+    PF_HIP(hipStreamSynchronize(h->stream));
+    const void* src = what == PF_PADMM_IMAGE ? h->x : what == PF_PADMM_RESIDUAL ? h->r : nullptr;
+    if (!src) throw Error(PF_ERR_ARG, "what must be PF_PADMM_IMAGE or PF_PADMM_RESIDUAL");
+    const int64_t n = what == PF_PADMM_IMAGE ? h->N : h->M;
+    const size_t e = h->prec == PF_C64 ? 8 : 16;
+    if (out_loc == PF_DEVICE) {
+      PF_HIP(hipMemcpy(out, src, e * n, hipMemcpyDeviceToDevice));
+    } else if (out_loc == PF_HOST) {
+      if (h->prec == PF_C64) {
+        void* tmp = nullptr;
+        PF_HIP(hipMalloc(&tmp, 16 * std::max<int64_t>(n, 1)));
+        c64_to_c128((const cpx<float>*)src, (cpx<double>*)tmp, n, h->stream);
+        PF_HIP(hipMemcpyAsync(out, tmp, 16 * n, hipMemcpyDeviceToHost, h->stream));
+        PF_HIP(hipStreamSynchronize(h->stream));
+        PF_HIP(hipFree(tmp));
+      } else {
+        PF_HIP(hipMemcpy(out, src, 16 * n, hipMemcpyDeviceToHost));
+      }
+    } else {
+      throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+    }
+  });
+}
+
+int pf_padmm_stats(pf_padmm* h, double* out, int32_t cap, int32_t* count) {
+  return guarded([&] {
+    if (!h) throw Error(PF_ERR_ARG, "null handle");
+    PF_HIP(hipStreamSynchronize(h->stream));
+    const int n = std::min(h->iters_done, cap);
+    if (n > 0 && out) PF_HIP(hipMemcpy(out, h->stats, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (count) *count = h->iters_done;
+  });
+}
+
+int pf_padmm_destroy(pf_padmm* h) {
+  delete h;
+  return PF_OK;
+}
+
+}  // extern "C"

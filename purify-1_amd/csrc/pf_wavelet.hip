@@ -307,6 +307,7 @@ void wavelet_synthesis(pf_wavelet* wv, const cpx<R>* c, cpx<R>* x, hipStream_t s
   }
   PF_HIP(hipGetLastError());
 }
+WaveletShape wavelet_shape(const pf_wavelet* wv) { return WaveletShape{wv->ny, wv->nx, wv->prec, wv->nwav}; }
 template void wavelet_analysis<float>(pf_wavelet*, const cpx<float>*, cpx<float>*, hipStream_t);
 template void wavelet_analysis<double>(pf_wavelet*, const cpx<double>*, cpx<double>*, hipStream_t);
 template void wavelet_synthesis<float>(pf_wavelet*, const cpx<float>*, cpx<float>*, hipStream_t);

@@ -9,7 +9,7 @@ from .operators import (GridDegridOperator, LinearTransform, MeasurementOperator
                         distributed_measurement_operator, measurement_operator_factory, measurementoperator,
                         operators)
 from .utilities import vis_params, vis_units
-from . import utilities, wavelets, wproj_utilities
+from . import algorithms, utilities, wavelets, wproj_utilities
 
 __all__ = ["kernel", "kernel_from_string", "LinearTransform", "MeasurementOperator", "measurementoperator",
            "operators", "GridDegridOperator", "PsfOperator",

@@ -13,6 +13,18 @@ PF_OK, PF_ERR_ARG, PF_ERR_HIP, PF_ERR_UNSUPPORTED, PF_ERR_NOMEM, PF_ERR_STATE = 
 PF_C64, PF_C128 = 0, 1
 PF_HOST, PF_DEVICE = 0, 1
 PF_DB_MAX = 10  # purify_amd.h: Daubechies orders DB1..DB10
+PF_PADMM_IMAGE, PF_PADMM_RESIDUAL = 0, 1
+
+
+class pf_padmm_params(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("epsilon", C.c_double), ("nu", C.c_double),
+                ("lagrange_update_scale", C.c_double), ("relative_variation", C.c_double),
+                ("residual_tolerance", C.c_double), ("itermax", C.c_int32)]
+
+
+class pf_padmm_result(C.Structure):
+    _fields_ = [("niters", C.c_int32), ("converged", C.c_int32), ("relative_variation", C.c_double),
+                ("residual_norm", C.c_double), ("gamma", C.c_double)]
 PF_LAMBDA, PF_RADIANS, PF_PIXELS = 0, 1, 2
 
 
@@ -62,6 +74,13 @@ SIGNATURES = {
     "pf_op_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "pf_op_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "pf_op_apply_normal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "pf_padmm_params_init": (C.c_int, [C.c_void_p]),
+    "pf_padmm_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pf_padmm_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "pf_padmm_reset": (C.c_int, [C.c_void_p]),
+    "pf_padmm_get": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int]),
+    "pf_padmm_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "pf_padmm_destroy": (C.c_int, [C.c_void_p]),
     "pf_wavelet_filter": (C.c_int, [C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
     "pf_wavelet_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                     C.POINTER(C.c_void_p)]),

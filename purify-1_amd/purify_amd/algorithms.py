@@ -1,0 +1,124 @@
+"""Imaging PADMM on the GPU: factory::algorithm_factory / padmm_factory
+(cpp/purify/algorithm_factory.h:36-146) for a tight-frame dictionary.
+
+``padmm_factory(dist, measurements, wavelets, uv_data, sigma, imsizey, imsizex, sara_size, ...)`` takes
+the reference's arguments and returns an ``ImagingProximalADMM`` whose ``run()`` gives a diagnostic
+(``x``, ``niters``, ``good``).  The device solver implements the tight-frame l1 proximal (one wavelet
+basis, as main.cc:121-126 selects for one dictionary entry); SARA's iterative, constrained l1
+proximal is not built, so ``tight_frame=False`` raises.  SOPT is not in the reference tree: the
+iteration restates the published algorithm and its agreement with SOPT's iterates is unpinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib, utilities
+from .operators import LinearTransform, MeasurementOperator
+from .wavelets import WaveletOperator
+
+
+@dataclass
+class Diagnostic:
+    x: np.ndarray            # the image, complex128 on the host
+    niters: int
+    good: bool               # converged (relative variation and residual tolerance met)
+    residual_norm: float
+    relative_variation: float
+
+
+class ImagingProximalADMM:
+    def __init__(self, phi: MeasurementOperator, psi: WaveletOperator, y, epsilon, gamma=0.0, nu=1.0,
+                 lagrange_update_scale=0.9, relative_variation=1e-3, residual_tolerance=0.0, itermax=500):
+        self.phi, self.psi = phi, psi  # kept alive: the solver uses them
+        p = _lib.pf_padmm_params()
+        _lib.check(_lib.lib().pf_padmm_params_init(C.byref(p)))
+        p.gamma, p.epsilon, p.nu = float(gamma), float(epsilon), float(nu)
+        p.lagrange_update_scale = float(lagrange_update_scale)
+        p.relative_variation = float(relative_variation)
+        p.residual_tolerance = float(residual_tolerance)
+        p.itermax = int(itermax)
+        self.itermax = int(itermax)
+        self._h = C.c_void_p()
+        if hasattr(y, "is_cuda") and y.is_cuda:
+            y = y.contiguous()
+            self._y = y
+            _lib.check(_lib.lib().pf_padmm_create(phi.handle, psi.handle, C.c_void_p(y.data_ptr()), _lib.PF_DEVICE,
+                                                  C.byref(p), C.byref(self._h)))
+        else:
+            yh = np.ascontiguousarray(np.asarray(y), np.complex128).ravel()
+            if yh.size != phi.M:
+                raise ValueError(f"{yh.size} visibilities for an operator of {phi.M}")
+            _lib.check(_lib.lib().pf_padmm_create(phi.handle, psi.handle, yh.ctypes.data, _lib.PF_HOST, C.byref(p),
+                                                  C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _lib.lib().pf_padmm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def iterate(self, iters, use_graph=True, check_every=0):
+        r = _lib.pf_padmm_result()
+        _lib.check(_lib.lib().pf_padmm_run(self._h, int(iters), int(bool(use_graph)), int(check_every), C.byref(r)))
+        return r
+
+    def reset(self):
+        _lib.check(_lib.lib().pf_padmm_reset(self._h))
+
+    def image(self):
+        out = np.empty(self.phi.imsizey * self.phi.imsizex, np.complex128)
+        _lib.check(_lib.lib().pf_padmm_get(self._h, _lib.PF_PADMM_IMAGE, out.ctypes.data, _lib.PF_HOST))
+        return out
+
+    def residual(self):
+        out = np.empty(self.phi.M, np.complex128)
+        _lib.check(_lib.lib().pf_padmm_get(self._h, _lib.PF_PADMM_RESIDUAL, out.ctypes.data, _lib.PF_HOST))
+        return out
+
+    def stats(self):
+        """Per-iteration (||x - x_prev||^2, ||x||^2, ||r||^2), FP64."""
+        cnt = C.c_int32()
+        _lib.check(_lib.lib().pf_padmm_stats(self._h, None, 0, C.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 3))
+        _lib.check(_lib.lib().pf_padmm_stats(self._h, out.ctypes.data, cnt.value, C.byref(cnt)))
+        return out[: cnt.value]
+
+    def run(self, use_graph=True, check_every=1):
+        """SOPT's operator(): iterate until converged (checked every check_every iterations) or itermax."""
+        r = self.iterate(self.itermax, use_graph, check_every)
+        return Diagnostic(self.image(), int(r.niters), bool(r.converged), float(r.residual_norm),
+                          float(r.relative_variation))
+
+
+class algo_distribution:  # noqa: N801 -- algorithm_factory.h:23
+    serial = "serial"
+    mpi_serial = "mpi_serial"
+    mpi_distributed = "mpi_distributed"
+
+
+def padmm_factory(dist, measurements, wavelets, uv_data, sigma, imsizey, imsizex, sara_size, max_iterations=500,
+                  real_constraint=True, positive_constraint=True, tight_frame=False, relative_variation=1e-3,
+                  l1_proximal_tolerance=1e-2, maximum_proximal_iterations=50):
+    """padmm_factory (algorithm_factory.h:36-122): epsilon = calculate_l2_radius(M, sigma), gamma =
+    max |Psi^dagger Phi^dagger y| * 1e-3, nu 1, lagrange_update_scale 0.9, residual tolerance epsilon."""
+    if sara_size > 1 and tight_frame:
+        raise RuntimeError("l1 proximal not consistent: You say you are using a tight frame, but you have more than "
+                           "one wavelet basis.")
+    if not tight_frame:
+        raise NotImplementedError("the device PADMM implements the tight-frame l1 proximal only (one wavelet basis, "
+                                  "tight_frame=True)")
+    if dist != algo_distribution.serial:
+        raise RuntimeError("Type of distributed proximal ADMM algorithm not recognised. You might not have compiled "
+                           "with MPI.")
+    phi = measurements.op if isinstance(measurements, LinearTransform) else measurements
+    eps = utilities.calculate_l2_radius(uv_data.vis.size, sigma)
+    return ImagingProximalADMM(phi, wavelets, uv_data.vis, eps, 0.0, 1.0, 0.9, relative_variation, eps,
+                               max_iterations)
