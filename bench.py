@@ -94,6 +94,62 @@ def serving_run(n_requests, seed, log):
     return out
 
 
+def padmm_run(iters, log, with_cpu=True):
+    """PADMM iterations / s with a DB4 (4 levels) tight-frame prior, 30 dB noise, KB, oversample 2:
+    the plain-method shape (256^2 image, M = 1e4, J = 4; the oracle's FP64 PADMM on the host beside it,
+    and the reconstruction SNR of both against the true image) and the north-star shape (2048^2,
+    M = 4e6, J = 6).  Each leg builds the operator (normalised by its power method), then times `iters`
+    iterations replayed from a captured hipGraph and the same number launched eagerly."""
+    import numpy as np
+    import torch
+    from purify_amd import algorithms, measurementoperator, utilities, wavelets
+    out = {}
+    for name, n, M, J in (("plain_256", 256, 10_000, 4), ("north_star_2048", 2048, 4_000_000, 6)):
+        seed = 3001 + n
+        u, v = utilities.synth_uv(seed, M, np.pi / 3)
+        u, v = utilities.radians_to_pixels(u, n), utilities.radians_to_pixels(v, n)
+        x0 = utilities.synth_image(seed + 1, n, n, 20)
+        leg = {"image": n, "M": M, "J": J, "wavelet": "DB4", "levels": 4, "iterations": iters}
+        for prec in ("c64", "c128"):
+            lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 50, 1e-6, "kb", J, J,
+                                                             precision=prec)
+            if prec == "c64":
+                y0 = lt * x0
+                sigma = utilities.SNR_to_standard_deviation(y0, 30.0)
+                y = utilities.add_noise(y0, sigma, seed + 2)
+                eps = utilities.calculate_l2_radius(M, sigma)
+            W = wavelets.WaveletOperator([("DB4", 4)], n, n, prec)
+            rates = {}
+            for graph in (True, False):
+                sol = algorithms.ImagingProximalADMM(lt.op, W, y, eps, itermax=iters + 1)
+                sol.iterate(1, use_graph=graph)  # capture (graph) / warm-up
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                sol.iterate(iters, use_graph=graph)
+                torch.cuda.synchronize()
+                rates["graph" if graph else "eager"] = iters / (time.perf_counter() - t0)
+                if graph:
+                    leg[f"{prec}_snr_db"] = float(20 * np.log10(np.linalg.norm(x0) / np.linalg.norm(x0 - sol.image())))
+                del sol
+            leg[f"{prec}_it_per_s"] = rates
+            log(f"[bench] padmm {name} {prec}: {rates['graph']:.0f} it/s graph, {rates['eager']:.0f} it/s eager, "
+                f"SNR {leg[prec + '_snr_db']:.2f} dB")
+            del W, lt
+            torch.cuda.empty_cache()
+        if with_cpu and name == "plain_256":
+            import oracle
+            orc = oracle.Operator(u, v, None, None, n, n, 2.0, oracle.KB, J, J, power_iters=50, power_tol=1e-6)
+            t0 = time.perf_counter()
+            xo, _, _, _ = oracle.padmm(orc, [(4, 4)], y, eps, iters)
+            dt = time.perf_counter() - t0
+            leg["cpu_oracle"] = {"it_per_s": iters / dt, "snr_db": oracle.snr_db(x0, xo), "cores": oracle.omp_threads(),
+                                 "kind": "port"}
+            leg["snr_gap_db_c64_vs_oracle"] = leg["c64_snr_db"] - leg["cpu_oracle"]["snr_db"]
+            log(f"[bench] padmm {name} oracle (FP64, host): {iters / dt:.1f} it/s, SNR {leg['cpu_oracle']['snr_db']:.2f} dB")
+        out[name] = leg
+    return out
+
+
 def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev):
     """Workload for rocprofv3 --pmc passes: a copy of known size, then the operator pair."""
     import torch
@@ -130,6 +186,7 @@ def main():
     ap.add_argument("--workload", default="north_star", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-pairs", type=int, default=2)
+    ap.add_argument("--padmm-iters", type=int, default=100, help="PADMM iterations per leg; 0 skips the PADMM legs")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests of the serving sample (Poisson arrivals at 20 / s); 0 skips it")
     ap.add_argument("--counters", choices=("c64", "c128"), default=None,
@@ -284,6 +341,9 @@ def main():
     serve = None
     if args.serve_requests > 0 and not w_term:
         serve = serving_run(args.serve_requests, seed + 10, log)
+    padmm = None
+    if args.padmm_iters > 0 and not w_term:
+        padmm = padmm_run(args.padmm_iters, log, with_cpu=not args.no_cpu)
 
     lat = sorted(r64["lat"])
     out = {
@@ -313,6 +373,7 @@ def main():
         "rel_err_vs_oracle": acc_oracle,
         "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "serving": serve,
+        "padmm": padmm,
         "setup_s": {"c64": r64["setup_s"], "c128": res["c128"]["setup_s"]},
         "host_pair_ms": {"c64": r64["host_ms"], "c128": res["c128"]["host_ms"],
                          "note": "pair on host complex128 buffers: PCIe copies and conversions included"},
