@@ -209,7 +209,9 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
 
 /* ---- tuning (no counterpart in the reference: FFTW_MEASURE planning, operators.h:215-223, is its
  *      closest analogue) ---- */
-/* Keys: "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics, 2 = rank-4 updates of 16 x 16 tiles on
+/* Keys: "kernel_poly" (complex64, Ju = Jv <= 8, KB / KB-min / Gaussian / PSWF: 1 = the taps of both
+ * axes from per-tap degree-9 polynomials in the fractional offset, checked at build to 1e-7 of the
+ * kernel's peak, the default where they apply and ahead of the tables; 0 = off); "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics, 2 = rank-4 updates of 16 x 16 tiles on
  * the matrix cores, the default where the grid sides are at least 16 + J - 1; all reproducible run to run);
  * "kernel_table" (complex64, J <= 8: interpolated tap tables, about 1e-7 relative error: 1 = in gridding,
  * the default, 2 = in gridding and degridding; 0 = evaluate the kernel for every tap); "fft_columns"

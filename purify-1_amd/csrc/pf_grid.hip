@@ -85,8 +85,32 @@ __device__ inline void tab_interp(const TabRow& a, int k, float (&w)[J]) {
   }
 }
 
-// Tap weights of both axes.  With tables, both rows are requested before either is used, so a
-// visibility waits for one memory round trip, not two.
+// How a kernel instance computes its taps: taps_both (tables if present, else direct evaluation),
+// tables requested ahead by the kernel's own pipeline, or the per-tap polynomials.
+enum TapMode { TAPS_EVAL = 0, TAPS_TABLE = 1, TAPS_POLY = 2 };
+
+// Taps from the per-tap polynomials (make_tap_poly): Horner in s = 2 (f - J/2) - 1; the coefficients
+// are uniform across the wave (scalar loads).
+// The coefficients are read through the constant address space: uniform scalar loads of memory the
+// kernel never writes, which the compiler may repeat instead of holding all J (D + 1) values live.
+typedef const __attribute__((address_space(4))) float const_float;
+template <int J>
+__device__ inline void poly_taps(const float* __restrict__ c_, int k, float f, float (&w)[J]) {
+  const const_float* c = (const const_float*)c_;
+  const float sv = 2.0f * (f - 0.5f * (float)J) - 1.0f;
+#pragma unroll
+  for (int a = 0; a < J; ++a) {
+    const const_float* ca = c + a * (TAP_POLY_D + 1);
+    float p = ca[0];
+#pragma unroll
+    for (int d = 1; d <= TAP_POLY_D; ++d) p = fmaf(p, sv, ca[d]);
+    w[a] = ((k + a) & 1) ? -p : p;
+  }
+}
+
+// Tap weights of both axes: tables where present (both rows are requested before either is used, so
+// a visibility waits for one memory round trip, not two), else direct evaluation.  The kernels that
+// use the polynomials call poly_taps instead (a compile-time tap mode, TapMode).
 template <typename R, int JU, int JV>
 __device__ inline void taps_both(const GridGeom<R>& g, const KParams<R>& kv, const float* tab_v, int ku_, R fu,
                                  int kv_, R fv, int Ju, int Jv, R (&wu)[JU == 0 ? JMAX : JU],
@@ -143,11 +167,12 @@ constexpr int dg_rows() { return sizeof(R) == 4 ? 10 : 5; }
 // TAB (complex64, J <= 8, tap tables present): the taps come from the tables, whose rows (addressed by
 // the coordinates) are requested one visibility ahead and the coordinates two ahead, so neither round
 // trip is waited for in the loop; otherwise the kernel is evaluated directly.
-template <typename R, int JU, int JV, bool TAB_>
+template <typename R, int JU, int JV, int TM>
 __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisData<R> v, TileLists tl,
                                                              const cpx<R>* __restrict__ grid, cpx<R>* __restrict__ y,
                                                              R scl) {
-  constexpr bool TAB = TAB_ && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
+  constexpr bool TAB = TM == TAPS_TABLE && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
+  constexpr bool POLY = TM == TAPS_POLY && sizeof(R) == 4 && JU > 0 && JU == JV && JU <= 8;
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const uint4 sl = tl.slot_own[blockIdx.x];
@@ -216,7 +241,12 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
         cn = vis_coord(v, nxt, Ju, Jv);
         if (v.perm) dn = (uint32_t)v.perm[nxt];
       }
-      taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
+      if constexpr (POLY) {
+        poly_taps<JU>(g.poly, c.ku, c.fu, wu);
+        poly_taps<JV>(g.poly, c.kv, c.fv, wv);
+      } else {
+        taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
+      }
     }
     const int lx0 = wrap(c.ku + 1, g.Nu) - tx0, ly0 = wrap(c.kv + 1, g.Nv) - ty0;
     const cpx<R>* base = lds + ly0 * W + lx0;
@@ -496,10 +526,11 @@ __device__ inline BatchPos slot_pos(const TileLists& tl, int j, int nslots) {
 // (TAB: complex64, J <= 8, tap tables present) the tap-table rows one ahead, while the previous
 // batch's MFMAs run; the rows' addresses depend on the coordinates, so this hides a dependent round
 // trip per batch.  A tile's 16 x 16 sums stay in the MFMA accumulators until its last batch.
-template <typename R, int JU, int JV, bool TAB_>
+template <typename R, int JU, int JV, int TM>
 __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, TileLists tl, int nslots,
                                                   const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
-  constexpr bool TAB = TAB_ && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
+  constexpr bool TAB = TM == TAPS_TABLE && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
+  constexpr bool POLY = TM == TAPS_POLY && sizeof(R) == 4 && JU > 0 && JU == JV && JU <= 8;
   constexpr bool ENT = sizeof(R) == 4;
   constexpr int T = MFMA_TILE;
   constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
@@ -565,6 +596,9 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
       if constexpr (TAB) {
         tab_interp<JU>(ta, cur.c.ku, wu);
         tab_interp<JV>(tb, cur.c.kv, wv);
+      } else if constexpr (POLY) {
+        poly_taps<JU>(g.poly, cur.c.ku, cur.c.fu, wu);
+        poly_taps<JV>(g.poly, cur.c.kv, cur.c.fv, wv);
       } else {
         taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), nullptr, cur.c.ku, cur.c.fu, cur.c.kv, cur.c.fv, Ju, Jv, wu,
                              wv);
@@ -652,14 +686,19 @@ void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const
   const dim3 blocks((unsigned)t.n_own);
   const size_t shm = sizeof(cpx<R>) * (size_t)(t.TU + g.Ju - 1) * (t.TV + g.Jv - 1);
   ProfScope prof("degrid", s);
-  const bool tab = sizeof(R) == 4 && g.tab_u && g.Ju == g.Jv && g.Ju <= 8;
-#define PF_DG(A, B)                                                                                             \
-  do {                                                                                                          \
-    if (tab && A > 0)                                                                                           \
-      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, true>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y, scale); \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, false>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y,  \
-                         scale);                                                                                \
+  const int tm = sizeof(R) == 4 && g.Ju == g.Jv && g.Ju <= 8 ? (g.poly ? TAPS_POLY : g.tab_u ? TAPS_TABLE : TAPS_EVAL)
+                                                            : TAPS_EVAL;
+#define PF_DG(A, B)                                                                                                 \
+  do {                                                                                                              \
+    if (tm == TAPS_POLY && A > 0)                                                                                   \
+      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, TAPS_POLY>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y,  \
+                         scale);                                                                                    \
+    else if (tm == TAPS_TABLE && A > 0)                                                                             \
+      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, TAPS_TABLE>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y, \
+                         scale);                                                                                    \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, TAPS_EVAL>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y,  \
+                         scale);                                                                                    \
   } while (0)
   if (g.Ju == g.Jv && fixed_len_ok(g)) {
     switch (g.Ju) {
@@ -692,13 +731,16 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
     const dim3 blocks((unsigned)(mfma_waves > 0 ? std::min(nslots, mfma_waves) : nslots));
     ProfScope prof("grid_tiles", s);
     // tap tables (complex64, equal supports <= 8): the pipelined table variant
-    const bool tab = sizeof(R) == 4 && g.tab_u && g.Ju == g.Jv && g.Ju <= 8;
-#define PF_GM(A, B)                                                                                            \
-  do {                                                                                                         \
-    if (tab && A > 0)                                                                                          \
-      hipLaunchKernelGGL((k_grid_mfma<R, A, B, true>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale);  \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_grid_mfma<R, A, B, false>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale); \
+    const int tmode = sizeof(R) == 4 && g.Ju == g.Jv && g.Ju <= 8 ? (g.poly ? TAPS_POLY : g.tab_u ? TAPS_TABLE : TAPS_EVAL)
+                                                              : TAPS_EVAL;
+#define PF_GM(A, B)                                                                                                    \
+  do {                                                                                                                 \
+    if (tmode == TAPS_POLY && A > 0)                                                                                      \
+      hipLaunchKernelGGL((k_grid_mfma<R, A, B, TAPS_POLY>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale);  \
+    else if (tmode == TAPS_TABLE && A > 0)                                                                                \
+      hipLaunchKernelGGL((k_grid_mfma<R, A, B, TAPS_TABLE>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale); \
+    else                                                                                                               \
+      hipLaunchKernelGGL((k_grid_mfma<R, A, B, TAPS_EVAL>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale);  \
   } while (0)
     if (g.Ju == g.Jv && fixed_len_ok(g)) {
       switch (g.Ju) {

@@ -184,9 +184,19 @@ struct GridGeom {
   KParams<R> ku, kv;               // u-axis and v-axis kernels
   const float* tab_u = nullptr;    // complex64 path only: tap tables (nullptr: evaluate the kernels)
   const float* tab_v = nullptr;
+  // complex64, Ju = Jv: per-tap polynomials (nullptr: tables or direct evaluation); both axes use them
+  const float* poly = nullptr;
 };
 // Host: builds the TAB_Q x TAB_ROW table of a kernel evaluated with support Jeval over J taps.
 std::vector<float> make_tap_table(int kind, int Jeval, int J, double os);
+// Per-tap polynomials: tap a of a visibility with fractional offset delta = (u - k_u) - J/2 in [0, 1)
+// is K(delta + J/2 - 1 - a) ~ sum_d c[a][d] s^(D - d), s = 2 delta - 1, degree D = TAP_POLY_D
+// (Chebyshev interpolation at D + 1 nodes, in long double, converted to powers of s; highest degree
+// first for Horner).  Returns an empty vector unless the kernel is smooth on every tap interval (KB,
+// KB-min, Gaussian, PSWF with Jeval = J) and the interpolation error, checked on 257 points per tap,
+// is below 1e-7 of the kernel's peak.
+constexpr int TAP_POLY_D = 9;
+std::vector<float> make_tap_poly(int kind, int Jeval, int J, double os, double* max_rel_err = nullptr);
 
 struct TileLists {
   int TU = 32, TV = 32, ntu = 0, ntv = 0;

@@ -258,6 +258,58 @@ std::vector<float> make_tap_table(int kind, int Jeval, int J, double os) {
   return t;
 }
 
+std::vector<float> make_tap_poly(int kind, int Jeval, int J, double os, double* max_rel_err) {
+  if (max_rel_err) *max_rel_err = -1;
+  if (Jeval != J || J < 1 || (kind != K_KB && kind != K_KBMIN && kind != K_GAUSS && kind != K_GAUSS_ALT && kind != K_PSWF))
+    return {};
+  constexpr int D = TAP_POLY_D, NP = D + 1;
+  const long double PI_L = 3.141592653589793238462643383279502884L;
+  std::vector<float> out((size_t)J * NP);
+  double peak = 0, worst = 0;
+  for (int a = 0; a < J; ++a) {
+    auto f = [&](long double sv) {
+      const double delta = (double)((sv + 1) / 2);
+      return (long double)host_kernel(kind, delta + 0.5 * J - 1 - a, Jeval, os);
+    };
+    // Chebyshev coefficients from the values at the D + 1 Chebyshev nodes
+    long double fv[NP], cheb[NP];
+    for (int j = 0; j < NP; ++j) fv[j] = f(std::cos(PI_L * (j + 0.5L) / NP));
+    for (int k = 0; k < NP; ++k) {
+      long double acc = 0;
+      for (int j = 0; j < NP; ++j) acc += fv[j] * std::cos(PI_L * k * (j + 0.5L) / NP);
+      cheb[k] = acc * 2 / NP;
+    }
+    cheb[0] /= 2;
+    // to powers of s: T_0 = 1, T_1 = s, T_(k+1) = 2 s T_k - T_(k-1)
+    long double mono[NP] = {}, tkm1[NP] = {}, tk[NP] = {}, tkp1[NP];
+    tkm1[0] = 1;
+    tk[1] = 1;
+    mono[0] += cheb[0];
+    if (D >= 1) mono[1] += cheb[1];
+    for (int k = 1; k < D; ++k) {
+      for (int d = 0; d < NP; ++d) tkp1[d] = -tkm1[d] + (d > 0 ? 2 * tk[d - 1] : 0);
+      for (int d = 0; d < NP; ++d) mono[d] += cheb[k + 1] * tkp1[d];
+      std::copy(tk, tk + NP, tkm1);
+      std::copy(tkp1, tkp1 + NP, tk);
+    }
+    for (int d = 0; d < NP; ++d) out[(size_t)a * NP + d] = (float)mono[D - d];  // highest degree first
+    // check the float polynomial, evaluated as the device does, against the kernel
+    for (int q = 0; q <= 256; ++q) {
+      const float sv = 2.0f * ((float)q / 256.0f) - 1.0f;
+      double p = out[(size_t)a * NP];
+      for (int d = 1; d < NP; ++d) p = p * sv + out[(size_t)a * NP + d];
+      const double ex = (double)f(sv);
+      if (!std::isfinite(ex)) return {};
+      peak = std::max(peak, std::fabs(ex));
+      worst = std::max(worst, std::fabs(p - ex));
+    }
+  }
+  if (!(peak > 0)) return {};
+  if (max_rel_err) *max_rel_err = worst / peak;
+  if (worst > 1e-7 * peak) return {};
+  return out;
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -316,6 +368,8 @@ struct pf_op {
   std::unique_ptr<Profiler> prof;  // per-kernel timing when enabled
   int grid_variant = GRID_RESIDUE;  // tuning knob: gridding kernel
   int kernel_table = 0;             // tuning knob: 0 no tap tables, 1 in gridding, 2 in gridding and degridding
+  int kernel_poly = 0;              // tuning knob: per-tap polynomials (complex64, Ju = Jv), ahead of tables
+  float* d_poly = nullptr;
   int fft_col_mode = 0;             // tuning knob: column FFT in one pass (0) or two steps (1)
   int grid_waves = 0;               // tuning knob: matrix-core gridding waves (0: one per 16 x 16 tile)
   int fft_row_bpc = -1;             // tuning knob: row pass blocks per CU (0: one line per block; -1: plan default)
@@ -675,6 +729,15 @@ void build(pf_op* op, const pf_op_desc& d) {
     op->gf.tab_u = op->d_tab_u;
     op->gf.tab_v = op->d_tab_v;
     op->kernel_table = 1;
+  }
+  if (op->prec == PF_C64 && op->Ju == op->Jv && op->Ju <= 8 && !op->w_term) {
+    const std::vector<float> pc = make_tap_poly(d.kernel, op->Jv, op->Ju, op->os);
+    if (!pc.empty()) {
+      op->d_poly = (float*)op->dalloc(sizeof(float) * pc.size());
+      PF_HIP(hipMemcpy(op->d_poly, pc.data(), sizeof(float) * pc.size(), hipMemcpyHostToDevice));
+      op->gf.poly = op->d_poly;
+      op->kernel_poly = 1;
+    }
   }
 
   // ---- coordinates in pixels, tap origins, sort by owner tile ----
@@ -1507,6 +1570,14 @@ int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
     op->fft.row_bpc = (int)value;
     return PF_OK;
   }
+  if (k == "kernel_poly") {
+    if (value != 0 && value != 1) return fail(PF_ERR_ARG, "kernel_poly must be 0 or 1");
+    if (value && !op->d_poly)
+      return fail(PF_ERR_UNSUPPORTED, "tap polynomials exist only for complex64, Ju = Jv <= 8 and smooth kernels");
+    op->kernel_poly = (int)value;
+    op->gf.poly = value ? op->d_poly : nullptr;
+    return PF_OK;
+  }
   if (k == "kernel_table") {
     if (value < 0 || value > 2) return fail(PF_ERR_ARG, "kernel_table must be 0, 1 or 2");
     if (value && !op->d_tab_u) return fail(PF_ERR_UNSUPPORTED, "tap tables exist only for complex64 and J <= 8");
@@ -1535,6 +1606,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   }
   if (k == "fft_columns") {
     *value = op->fft_col_mode;
+    return PF_OK;
+  }
+  if (k == "kernel_poly") {
+    *value = op->kernel_poly;
     return PF_OK;
   }
   if (k == "kernel_table") {

@@ -335,3 +335,35 @@ def test_matrix_core_wave_counts_give_identical_results(precision):
         assert np.array_equal(gpu.adjoint() * y, ref)
     with pytest.raises(PurifyError):
         gpu.op.set_tuning("grid_waves", -1)
+
+
+@pytest.mark.parametrize("kernel,J", [("kb", 4), ("kb", 6), ("kb", 8), ("gauss", 6), ("pswf", 6), ("kbmin", 5)])
+def test_tap_polynomials_vs_oracle_and_direct_evaluation(kernel, J):
+    # complex64 with Ju = Jv: per-tap polynomials in the fractional offset (degree 9, checked on the
+    # host against the exact kernel at build) replace the per-tap kernel evaluation
+    n, M = 128, 20_000
+    u, v = make_uv(99 + J, M, n, sigma=0.6)
+    w = utilities.synth_cnormal(98, M)
+    gpu, orc = pair(u, v, n, n, weights=w, kernel=kernel, Ju=J, Jv=J, precision="c64")
+    assert gpu.op.get_tuning("kernel_poly") == 1
+    x = utilities.synth_image(97, n, n, 20)
+    y = utilities.synth_cnormal(96, M)
+    a, b = gpu * x, gpu.adjoint() * y
+    assert rel(a, orc.forward(x)) < 1e-5 and rel(b, orc.adjoint(y)) < 1e-5
+    gpu.op.set_tuning("kernel_poly", 0)
+    assert gpu.op.get_tuning("kernel_poly") == 0
+    assert rel(gpu * x, a) < 2e-6 and rel(gpu.adjoint() * y, b) < 2e-6
+
+
+def test_tap_polynomials_only_where_they_apply():
+    u, v = make_uv(95, 500, 32)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, 32, 32, 2.0, 0, 1e-4, "kb", 4, 4,
+                                                     precision="c128")
+    assert lt.op.get_tuning("kernel_poly") == 0
+    with pytest.raises(PurifyError, match="polynomials"):
+        lt.op.set_tuning("kernel_poly", 1)
+    # unequal supports (the kernel-slot swap puts taps outside the support) and the box kernel: none
+    for kern, Ju, Jv in (("kb", 4, 6), ("box", 4, 4)):
+        lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, 32, 32, 2.0, 0, 1e-4, kern, Ju, Jv,
+                                                         precision="c64")
+        assert lt.op.get_tuning("kernel_poly") == 0
