@@ -73,10 +73,11 @@ def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, KK=None):
 def serving_run(n_requests, seed, log):
     """Served latency: a Poisson stream (20 requests / s) of imaging jobs, image side uniform over
     {256, 512, 1024, 2048}, M log-uniform in [1e4, 4e6], 30 % with w-terms; each job builds its operator
-    and applies one Phi + Phi^dagger pair (complex64).  Latency runs from arrival to the result on the
-    host, so it includes the operator build and the host transfers."""
+    and applies one Phi + Phi^dagger pair, or (25 % of the jobs) normalises it and runs a 100-iteration
+    DB4 PADMM reconstruction (complex64).  Latency runs from arrival to the result on the host, so it
+    includes the operator build and the host transfers."""
     from purify_amd import serving
-    reqs = serving.poisson_requests(seed, n_requests, 20.0)
+    reqs = serving.poisson_requests(seed, n_requests, 20.0, padmm_fraction=0.25, padmm_iters=100)
     M_total = sum(int(r.u.size) for r in reqs)
     srv = serving.Server(96 << 30, max_inflight=4)
     t0 = time.perf_counter()
@@ -88,6 +89,7 @@ def serving_run(n_requests, seed, log):
     out = {"requests": len(res), "rate_per_s": 20.0, "p50_ms": 1e3 * st["p50_s"], "p99_ms": 1e3 * st["p99_s"],
            "mean_ms": 1e3 * st["mean_s"], "throughput_req_per_s": len(res) / wall, "vis_pairs_per_s": M_total / wall,
            "peak_inflight": srv.peak_inflight, "hbm_budget_gib": 96, "w_term_jobs": sum(r.w_term for r in reqs),
+           "padmm_jobs": sum(r.kind == "padmm" for r in reqs),
            "sample": f"{len(res)} requests of the serving mix (the full stream is 2000 at 20 / s)"}
     log(f"[bench] serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
         f"{out['throughput_req_per_s']:.1f} req/s, peak in flight {srv.peak_inflight}")

@@ -156,7 +156,9 @@ int pf_padmm_params_init(pf_padmm_params *p);
 /* y: the visibilities (host complex128, or device in the operator's precision), M of them. */
 int pf_padmm_create(pf_op *phi, pf_wavelet *psi, const void *y, int y_loc, const pf_padmm_params *params,
                     pf_padmm **out);
-/* Runs iters more iterations (use_graph: replay one captured iteration).  check_every > 0: every that
+/* Runs iters more iterations (use_graph: replay one captured iteration; while the capture lasts HIP
+ * refuses legacy-stream work from every thread of the process, so a multi-threaded caller should not
+ * use it while other threads issue such work).  check_every > 0: every that
  * many iterations, stop once ||x - x_prev|| / ||x|| < relative_variation and ||r|| <
  * residual_tolerance (check_every = 1 stops exactly at the first converged iteration, as SOPT does);
  * 0: run all iters. */

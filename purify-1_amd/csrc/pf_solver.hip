@@ -325,7 +325,9 @@ void padmm_run(pf_padmm* h, int iters, bool use_graph, int check_every, pf_padmm
   PF_HIP(hipStreamSynchronize(s));
   if (res) {
     st.resize(3);
-    PF_HIP(hipMemcpy(st.data(), h->stats + 3 * (h->iters_done - 1), 3 * sizeof(double), hipMemcpyDeviceToHost));
+    // stream-ordered copies only: a legacy-stream copy is refused while another thread captures
+    PF_HIP(hipMemcpyAsync(st.data(), h->stats + 3 * (h->iters_done - 1), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
     res->niters = h->iters_done;
     res->converged = converged ? 1 : 0;
     res->relative_variation = st[1] > 0 ? std::sqrt(st[0] / st[1]) : 0.0;
@@ -437,7 +439,8 @@ int pf_padmm_get(pf_padmm* h, int32_t what, void* out, int out_loc) {
     const int64_t n = what == PF_PADMM_IMAGE ? h->N : h->M;
     const size_t e = h->prec == PF_C64 ? 8 : 16;
     if (out_loc == PF_DEVICE) {
-      PF_HIP(hipMemcpy(out, src, e * n, hipMemcpyDeviceToDevice));
+      PF_HIP(hipMemcpyAsync(out, src, e * n, hipMemcpyDeviceToDevice, h->stream));
+      PF_HIP(hipStreamSynchronize(h->stream));
     } else if (out_loc == PF_HOST) {
       if (h->prec == PF_C64) {
         void* tmp = nullptr;
@@ -447,7 +450,8 @@ int pf_padmm_get(pf_padmm* h, int32_t what, void* out, int out_loc) {
         PF_HIP(hipStreamSynchronize(h->stream));
         PF_HIP(hipFree(tmp));
       } else {
-        PF_HIP(hipMemcpy(out, src, 16 * n, hipMemcpyDeviceToHost));
+        PF_HIP(hipMemcpyAsync(out, src, 16 * n, hipMemcpyDeviceToHost, h->stream));
+        PF_HIP(hipStreamSynchronize(h->stream));
       }
     } else {
       throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
@@ -460,7 +464,10 @@ int pf_padmm_stats(pf_padmm* h, double* out, int32_t cap, int32_t* count) {
     if (!h) throw Error(PF_ERR_ARG, "null handle");
     PF_HIP(hipStreamSynchronize(h->stream));
     const int n = std::min(h->iters_done, cap);
-    if (n > 0 && out) PF_HIP(hipMemcpy(out, h->stats, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (n > 0 && out) {
+      PF_HIP(hipMemcpyAsync(out, h->stats, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+      PF_HIP(hipStreamSynchronize(h->stream));
+    }
     if (count) *count = h->iters_done;
   });
 }

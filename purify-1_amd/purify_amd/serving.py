@@ -33,7 +33,9 @@ from .operators import measurementoperator
 
 @dataclass
 class ImagingRequest:
-    """One job.  kind: "forward" (y = Phi x), "adjoint" (x = Phi^dagger y) or "pair" (both)."""
+    """One job.  kind: "forward" (y = Phi x), "adjoint" (x = Phi^dagger y), "pair" (both) or "padmm" (a
+    reconstruction from y: the operator normalised by `power_iters` power iterations, a DB4 prior with
+    `levels` levels, the l2-ball radius from `sigma`, `iters` PADMM iterations)."""
     u: np.ndarray
     v: np.ndarray
     imsizey: int
@@ -52,6 +54,10 @@ class ImagingRequest:
     precision: str = "c64"
     arrival: float = 0.0  # seconds from the server's start; set by submit() when 0
     rid: int = -1
+    iters: int = 100       # padmm
+    sigma: float = 0.0     # padmm: noise standard deviation of y
+    levels: int = 4        # padmm: DB4 levels
+    power_iters: int = 50  # padmm: normalisation of Phi
 
 
 @dataclass
@@ -82,7 +88,8 @@ def estimate_bytes(req: ImagingRequest) -> int:
     rows = M * (req.J + req.Jw - 1) ** 2 * e if req.w_term else 0
     stage = 16 * max(n2, M) + e * (n2 + M)
     io = 2 * e * (n2 + M)
-    return int(1.15 * (grid + vis + rows + stage + io)) + (8 << 20)
+    solver = (e * (8 * n2 + 6 * M) + 16 * M) if req.kind == "padmm" else 0  # PADMM state, DB4 buffers
+    return int(1.15 * (grid + vis + rows + stage + io + solver)) + (8 << 20)
 
 
 class Server:
@@ -158,6 +165,27 @@ class Server:
                     self._lock.wait(timeout=0.05)
             self._pool.submit(self._run, req, fut, slot, need, self.now())
 
+    @staticmethod
+    def _padmm(req: ImagingRequest, op):
+        """The reconstruction: normalise Phi, then `iters` PADMM iterations on the solver's own stream."""
+        from . import algorithms, utilities, wavelets
+        if req.power_iters > 0:
+            op.set_norm(op.power_method(req.power_iters, 1e-4))
+        lv = req.levels
+        while lv > 0 and (req.imsizey % (1 << lv) or req.imsizex % (1 << lv)):
+            lv -= 1
+        W = wavelets.WaveletOperator([("DB4", lv)], req.imsizey, req.imsizex, req.precision)
+        sigma = req.sigma if req.sigma > 0 else 1e-3 * float(np.linalg.norm(req.y)) / np.sqrt(max(req.y.size, 1))
+        eps = utilities.calculate_l2_radius(req.y.size, sigma)
+        sol = algorithms.ImagingProximalADMM(op, W, req.y, eps, itermax=max(req.iters, 1))
+        # launched eagerly on the solver's stream: a graph capture would make every other request's
+        # legacy-stream call (operator builds, power methods) fail while it lasts
+        sol.iterate(req.iters, use_graph=False)
+        x = sol.image()
+        sol.close()
+        W.close()
+        return x
+
     def _run(self, req: ImagingRequest, fut: Future, slot: int, need: int, admitted: float):
         torch = self._torch
         res = RequestResult(rid=req.rid, arrival=req.arrival, admitted=admitted)
@@ -167,6 +195,12 @@ class Server:
                 req.J, req.J, req.w_term, req.Jw, req.cell, req.cell, precision=req.precision)
             op = lt.op
             res.device_bytes = op.info().device_bytes
+            if req.kind == "padmm":
+                res.x = self._padmm(req, op)
+                op.close()
+                res.finished = self.now()
+                fut.set_result(res)
+                return
             st = self._streams[slot]
             dt = torch.complex64 if req.precision == "c64" else torch.complex128
             with torch.cuda.stream(st):
@@ -205,9 +239,11 @@ def latency_stats(results) -> dict:
 
 
 def poisson_requests(seed: int, n_requests: int, rate_per_s: float, sizes=(256, 512, 1024, 2048),
-                     M_range=(1e4, 4e6), w_fraction=0.3, wmax=200.0, precision="c64", kind="pair"):
+                     M_range=(1e4, 4e6), w_fraction=0.3, wmax=200.0, precision="c64", kind="pair",
+                     padmm_fraction=0.0, padmm_iters=100):
     """The serving workload: Poisson arrivals; image side uniform over `sizes`; M log-uniform over
-    M_range; a fraction of the jobs with w-terms (w uniform in [0, wmax], 1 degree field of view)."""
+    M_range; a fraction of the jobs with w-terms (w uniform in [0, wmax], 1 degree field of view); a
+    fraction `padmm_fraction` of the jobs are `padmm_iters`-iteration reconstructions instead of `kind`."""
     from . import utilities
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -223,6 +259,8 @@ def poisson_requests(seed: int, n_requests: int, rate_per_s: float, sizes=(256, 
         w = rng.uniform(0.0, wmax, M) if w_term else None
         x = utilities.synth_image(seed * 1000 + k, n, n, 20)
         y = utilities.synth_cnormal(seed * 1000 + k + 7, M)
-        reqs.append(ImagingRequest(u=u, v=v, imsizey=n, imsizex=n, kind=kind, x=x, y=y, w=w, J=4, w_term=w_term,
-                                   Jw=6, cell=3600.0 / n, precision=precision, arrival=t, rid=k))
+        kd = "padmm" if rng.uniform() < padmm_fraction else kind
+        reqs.append(ImagingRequest(u=u, v=v, imsizey=n, imsizex=n, kind=kd, x=x, y=y, w=w, J=4, w_term=w_term,
+                                   Jw=6, cell=3600.0 / n, precision=precision, arrival=t, rid=k, iters=padmm_iters,
+                                   sigma=0.1 * float(np.linalg.norm(y)) / np.sqrt(M)))
     return reqs

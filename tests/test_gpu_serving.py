@@ -55,3 +55,22 @@ def test_admission_respects_the_hbm_budget():
             srv2.submit(reqs[0])
         finally:
             srv2.close()
+
+
+def test_padmm_requests_alone_and_in_the_mix_are_identical():
+    def mix():
+        return serving.poisson_requests(13, 6, rate_per_s=1e5, sizes=(64, 128), M_range=(2e3, 2e4), w_fraction=0.3,
+                                        padmm_fraction=0.5, padmm_iters=20)
+    reqs = mix()
+    kinds = [r.kind for r in reqs]
+    assert "padmm" in kinds and "pair" in kinds
+    alone = []
+    for r in mix():
+        r.arrival = 0.0
+        (res,), _ = run([r], 1 << 34, 1)
+        alone.append(res)
+    mixed, srv = run(reqs, 1 << 34, 3)
+    for a, b, r in zip(alone, mixed, reqs):
+        assert np.array_equal(a.x, b.x)
+        if r.kind == "padmm":
+            assert a.y is None and np.isfinite(b.x).all() and np.linalg.norm(b.x) > 0
