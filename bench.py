@@ -152,6 +152,63 @@ def padmm_run(iters, log, with_cpu=True):
     return out
 
 
+def multichannel_run(n_channels, iters, seed, log):
+    """Batched multi-channel PADMM: n_channels independent channels, image side drawn from
+    {256, 512, 1024}, M log-uniform in [1e4, 1e6], uv Gaussian with sigma U[0.5, 1.2] rad/px, KB J = 4,
+    oversample 2, complex64 operators normalised by 50 power iterations, DB4 (4 levels) prior, 30 dB
+    noise.  Every channel's iteration is captured as a hipGraph; `iters` iterations per channel are
+    timed once channel after channel (each waits for its own) and once queued together on the
+    channels' streams."""
+    import numpy as np
+    import torch
+    from purify_amd import algorithms, measurementoperator, utilities
+    rng = np.random.default_rng(seed)
+    chans, x0s = [], []
+    t0 = time.perf_counter()
+    for c in range(n_channels):
+        n = int(rng.choice([256, 512, 1024]))
+        M = int(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        sig = float(rng.uniform(0.5, 1.2))
+        u, v = utilities.synth_uv(seed + 7 * c, M, sig)
+        u, v = utilities.radians_to_pixels(u, n), utilities.radians_to_pixels(v, n)
+        x0 = utilities.synth_image(seed + 7 * c + 1, n, n, 20)
+        lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 50, 1e-6, "kb", 4, 4,
+                                                         precision="c64")
+        y0 = lt * x0
+        sigma = utilities.SNR_to_standard_deviation(y0, 30.0)
+        chans.append(dict(u=u, v=v, y=utilities.add_noise(y0, sigma, seed + 7 * c + 2), imsizey=n, imsizex=n,
+                          sigma=sigma, operator=lt))
+        x0s.append(x0)
+    mc = algorithms.MultiChannelPADMM(chans, itermax=iters)
+    mc.capture()
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    M_total = sum(int(ch["u"].size) for ch in chans)
+    t0 = time.perf_counter()
+    for _, _, sol in mc.items:
+        sol.iterate(iters, use_graph=True)
+    t_seq = time.perf_counter() - t0
+    mc.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = mc.run(iters)
+    t_bat = time.perf_counter() - t0
+    snr = [float(20 * np.log10(np.linalg.norm(x0) / np.linalg.norm(x0 - x))) for x0, x in zip(x0s, mc.images())]
+    mc.close()
+    del chans
+    torch.cuda.empty_cache()
+    out = {"channels": n_channels, "iterations": iters, "M_total": M_total,
+           "image_sides": {str(k): sum(x.size == k * k for x in x0s) for k in (256, 512, 1024)},
+           "batched_channel_it_per_s": n_channels * iters / t_bat, "batched_vis_it_per_s": M_total * iters / t_bat,
+           "sequential_channel_it_per_s": n_channels * iters / t_seq, "batched_speedup": t_seq / t_bat,
+           "batched_s": t_bat, "setup_s": setup, "all_ran": all(r.niters == iters for r in res),
+           "snr_db": {"min": min(snr), "median": float(np.median(snr)), "max": max(snr)}}
+    log(f"[bench] multichannel {n_channels} x {iters} it: batched {t_bat:.2f} s "
+        f"({out['batched_channel_it_per_s']:.0f} channel-it/s, {out['batched_vis_it_per_s']:.3e} vis-it/s), "
+        f"sequential {t_seq:.2f} s, setup {setup:.1f} s, SNR median {out['snr_db']['median']:.2f} dB")
+    return out
+
+
 def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev):
     """Workload for rocprofv3 --pmc passes: a copy of known size, then the operator pair."""
     import torch
@@ -189,6 +246,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-pairs", type=int, default=2)
     ap.add_argument("--padmm-iters", type=int, default=100, help="PADMM iterations per leg; 0 skips the PADMM legs")
+    ap.add_argument("--channels", type=int, default=64, help="channels of the batched multi-channel PADMM leg; 0 skips it")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests of the serving sample (Poisson arrivals at 20 / s); 0 skips it")
     ap.add_argument("--counters", choices=("c64", "c128"), default=None,
@@ -346,6 +404,9 @@ def main():
     padmm = None
     if args.padmm_iters > 0 and not w_term:
         padmm = padmm_run(args.padmm_iters, log, with_cpu=not args.no_cpu)
+    multi = None
+    if args.channels > 0 and args.padmm_iters > 0 and not w_term:
+        multi = multichannel_run(args.channels, args.padmm_iters, seed + 20, log)
 
     lat = sorted(r64["lat"])
     out = {
@@ -376,6 +437,7 @@ def main():
         "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "serving": serve,
         "padmm": padmm,
+        "multichannel_padmm": multi,
         "setup_s": {"c64": r64["setup_s"], "c128": res["c128"]["setup_s"]},
         "host_pair_ms": {"c64": r64["host_ms"], "c128": res["c128"]["host_ms"],
                          "note": "pair on host complex128 buffers: PCIe copies and conversions included"},

@@ -163,6 +163,11 @@ int pf_padmm_create(pf_op *phi, pf_wavelet *psi, const void *y, int y_loc, const
  * residual_tolerance (check_every = 1 stops exactly at the first converged iteration, as SOPT does);
  * 0: run all iters. */
 int pf_padmm_run(pf_padmm *h, int32_t iters, int32_t use_graph, int32_t check_every, pf_padmm_result *res);
+/* Enqueues iters iterations on the solver's own stream and returns without waiting (no convergence
+ * checks); pf_padmm_sync waits and reports.  Solvers on different handles then run concurrently: the
+ * batched multi-channel path.  The first graph enqueue captures (see use_graph above). */
+int pf_padmm_enqueue(pf_padmm *h, int32_t iters, int32_t use_graph);
+int pf_padmm_sync(pf_padmm *h, pf_padmm_result *res);
 int pf_padmm_reset(pf_padmm *h);
 /* what: PF_PADMM_IMAGE (x, imsizey*imsizex) or PF_PADMM_RESIDUAL (r = Phi x - y, M) */
 int pf_padmm_get(pf_padmm *h, int32_t what, void *out, int out_loc);

@@ -284,10 +284,9 @@ void padmm_init(pf_padmm* h) {
 }
 
 template <typename R>
-void padmm_run(pf_padmm* h, int iters, bool use_graph, int check_every, pf_padmm_result* res) {
+void padmm_capture(pf_padmm* h) {
   hipStream_t s = h->stream;
-  if (h->iters_done + iters > h->cap) throw Error(PF_ERR_ARG, "more iterations than the solver was created for");
-  if (use_graph && !h->exec) {
+  if (!h->exec) {
     if (h->iters_done == 0) {
       // one eager iteration first, so every kernel attribute is set before capture; then
       // re-initialise so the captured run starts from the initial state
@@ -301,6 +300,42 @@ void padmm_run(pf_padmm* h, int iters, bool use_graph, int check_every, pf_padmm
     PF_HIP(hipStreamEndCapture(s, &h->graph));
     PF_HIP(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   }
+}
+
+template <typename R>
+void padmm_enqueue(pf_padmm* h, int iters, bool use_graph) {
+  if (h->iters_done + iters > h->cap) throw Error(PF_ERR_ARG, "more iterations than the solver was created for");
+  if (use_graph) padmm_capture<R>(h);
+  for (int i = 0; i < iters; ++i) {
+    if (use_graph) PF_HIP(hipGraphLaunch(h->exec, h->stream));
+    else padmm_iteration<R>(h);
+  }
+  h->iters_done += iters;
+}
+
+void padmm_result(pf_padmm* h, bool converged, pf_padmm_result* res) {
+  hipStream_t s = h->stream;
+  PF_HIP(hipStreamSynchronize(s));
+  if (res) {
+    double st[3] = {0, 0, 0};
+    // stream-ordered copies only: a legacy-stream copy is refused while another thread captures
+    if (h->iters_done > 0) {
+      PF_HIP(hipMemcpyAsync(st, h->stats + 3 * (h->iters_done - 1), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+      PF_HIP(hipStreamSynchronize(s));
+    }
+    res->niters = h->iters_done;
+    res->converged = converged ? 1 : 0;
+    res->relative_variation = st[1] > 0 ? std::sqrt(st[0] / st[1]) : 0.0;
+    res->residual_norm = std::sqrt(st[2]);
+    res->gamma = h->gamma;
+  }
+}
+
+template <typename R>
+void padmm_run(pf_padmm* h, int iters, bool use_graph, int check_every, pf_padmm_result* res) {
+  hipStream_t s = h->stream;
+  if (h->iters_done + iters > h->cap) throw Error(PF_ERR_ARG, "more iterations than the solver was created for");
+  if (use_graph) padmm_capture<R>(h);
   const int chunk = check_every > 0 ? check_every : iters;
   int done = 0;
   bool converged = false;
@@ -322,18 +357,7 @@ void padmm_run(pf_padmm* h, int iters, bool use_graph, int check_every, pf_padmm
       converged = relvar < h->par.relative_variation && std::sqrt(st[2]) < h->par.residual_tolerance;
     }
   }
-  PF_HIP(hipStreamSynchronize(s));
-  if (res) {
-    st.resize(3);
-    // stream-ordered copies only: a legacy-stream copy is refused while another thread captures
-    PF_HIP(hipMemcpyAsync(st.data(), h->stats + 3 * (h->iters_done - 1), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-    PF_HIP(hipStreamSynchronize(s));
-    res->niters = h->iters_done;
-    res->converged = converged ? 1 : 0;
-    res->relative_variation = st[1] > 0 ? std::sqrt(st[0] / st[1]) : 0.0;
-    res->residual_norm = std::sqrt(st[2]);
-    res->gamma = h->gamma;
-  }
+  padmm_result(h, converged, res);
 }
 
 }  // namespace
@@ -419,6 +443,22 @@ int pf_padmm_run(pf_padmm* h, int32_t iters, int32_t use_graph, int32_t check_ev
     if (iters == 0) return;
     if (h->prec == PF_C64) padmm_run<float>(h, iters, use_graph != 0, check_every, res);
     else padmm_run<double>(h, iters, use_graph != 0, check_every, res);
+  });
+}
+
+int pf_padmm_enqueue(pf_padmm* h, int32_t iters, int32_t use_graph) {
+  return guarded([&] {
+    if (!h) throw Error(PF_ERR_ARG, "null handle");
+    if (iters < 0) throw Error(PF_ERR_ARG, "iterations must be >= 0");
+    if (h->prec == PF_C64) padmm_enqueue<float>(h, iters, use_graph != 0);
+    else padmm_enqueue<double>(h, iters, use_graph != 0);
+  });
+}
+
+int pf_padmm_sync(pf_padmm* h, pf_padmm_result* res) {
+  return guarded([&] {
+    if (!h) throw Error(PF_ERR_ARG, "null handle");
+    padmm_result(h, false, res);
   });
 }
 

@@ -78,6 +78,8 @@ SIGNATURES = {
     "pf_padmm_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "pf_padmm_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "pf_padmm_reset": (C.c_int, [C.c_void_p]),
+    "pf_padmm_enqueue": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "pf_padmm_sync": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pf_padmm_get": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int]),
     "pf_padmm_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "pf_padmm_destroy": (C.c_int, [C.c_void_p]),

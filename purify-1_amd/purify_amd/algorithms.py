@@ -70,6 +70,15 @@ class ImagingProximalADMM:
         _lib.check(_lib.lib().pf_padmm_run(self._h, int(iters), int(bool(use_graph)), int(check_every), C.byref(r)))
         return r
 
+    def enqueue(self, iters, use_graph=True):
+        """Queue iterations on the solver's stream without waiting (see sync)."""
+        _lib.check(_lib.lib().pf_padmm_enqueue(self._h, int(iters), int(bool(use_graph))))
+
+    def sync(self):
+        r = _lib.pf_padmm_result()
+        _lib.check(_lib.lib().pf_padmm_sync(self._h, C.byref(r)))
+        return r
+
     def reset(self):
         _lib.check(_lib.lib().pf_padmm_reset(self._h))
 
@@ -122,3 +131,58 @@ def padmm_factory(dist, measurements, wavelets, uv_data, sigma, imsizey, imsizex
     eps = utilities.calculate_l2_radius(uv_data.vis.size, sigma)
     return ImagingProximalADMM(phi, wavelets, uv_data.vis, eps, 0.0, 1.0, 0.9, relative_variation, eps,
                                max_iterations)
+
+
+class MultiChannelPADMM:
+    """Many independent channels (frequency channels or snapshots with their own uv-coverage, image size
+    and visibilities) reconstructed together: every channel's operator, dictionary and solver stay
+    resident; every solver's iteration is captured once (in this thread, one after the other), then all
+    channels' iterations are queued at once, each on its solver's stream, so their kernels overlap on
+    the device.  channels: list of dicts with u, v (pixels), y, imsizey, imsizex, sigma and optional
+    J (4), levels (4), precision ("c64"), power_iters (50), or with "operator": a LinearTransform already
+    built (and normalised) for the channel, used in place of building one."""
+
+    def __init__(self, channels, itermax=100):
+        from .operators import measurementoperator
+        self.items = []
+        for ch in channels:
+            ny, nx = int(ch["imsizey"]), int(ch["imsizex"])
+            J, prec = int(ch.get("J", 4)), ch.get("precision", "c64")
+            lt = ch.get("operator")
+            if lt is None:
+                lt = measurementoperator.init_degrid_operator_2d(ch["u"], ch["v"], None, ch.get("weights"), ny, nx,
+                                                                 2.0, int(ch.get("power_iters", 50)), 1e-4, "kb", J, J,
+                                                                 precision=prec)
+            lv = int(ch.get("levels", 4))
+            while lv > 0 and (ny % (1 << lv) or nx % (1 << lv)):
+                lv -= 1
+            W = WaveletOperator([("DB4", lv)], ny, nx, prec)
+            eps = utilities.calculate_l2_radius(np.asarray(ch["y"]).size, float(ch["sigma"]))
+            sol = ImagingProximalADMM(lt.op, W, ch["y"], eps, itermax=itermax)
+            self.items.append((lt, W, sol))
+
+    def capture(self):
+        """Capture every channel's iteration (one graph each), in this thread: one warm-up iteration is
+        run and undone per channel (see pf_padmm_run)."""
+        for _, _, sol in self.items:
+            sol.enqueue(0, use_graph=True)
+            sol.sync()
+
+    def reset(self):
+        for _, _, sol in self.items:
+            sol.reset()
+
+    def run(self, iters, use_graph=True):
+        for _, _, sol in self.items:
+            sol.enqueue(iters, use_graph)
+        return [sol.sync() for _, _, sol in self.items]
+
+    def images(self):
+        return [sol.image() for _, _, sol in self.items]
+
+    def close(self):
+        for lt, W, sol in self.items:
+            sol.close()
+            W.close()
+            lt.op.close()
+        self.items = []

@@ -117,3 +117,28 @@ def test_factory_and_errors():
         algorithms.ImagingProximalADMM(lt.op, W, y, 0.0)
     with pytest.raises(PurifyError, match="more iterations"):
         sol.iterate(100)
+
+
+def test_multichannel_batch_equals_single_channel_runs():
+    """Channels of different sizes and coverages queued together (one graph and stream each) give the
+    same images as each channel solved alone."""
+    specs = [(621, 64, 3000), (622, 128, 5000), (623, 64, 2000)]
+    chans = []
+    for seed, n, M in specs:
+        u, v, x0, y, eps, orc = problem(seed, n, M)
+        sigma = utilities.SNR_to_standard_deviation(orc.forward(x0), 30.0)
+        chans.append(dict(u=u, v=v, y=y, imsizey=n, imsizex=n, sigma=sigma, power_iters=30))
+    K = 12
+    mc = algorithms.MultiChannelPADMM(chans, itermax=K)
+    mc.capture()
+    res = mc.run(K)
+    got = mc.images()
+    assert [r.niters for r in res] == [K] * len(chans)
+    for ch, x in zip(chans, got):
+        one = algorithms.MultiChannelPADMM([ch], itermax=K)
+        _, _, sol = one.items[0]
+        r = sol.iterate(K, use_graph=True)  # the blocking path, alone on the device
+        assert r.niters == K
+        assert np.array_equal(sol.image(), x)
+        one.close()
+    mc.close()
