@@ -3,9 +3,10 @@
 // Each 1-D line is a Stockham auto-sort transform (Govindaraju et al., "High Performance Discrete
 // Fourier Transforms on Graphics Processors", SC 2008): stage s with radix R and sub-transform
 // length Ns reads x[j + r N/R], twiddles by W^(r (j mod Ns) N / (Ns R)), does an R-point DFT in
-// registers and writes x[(j - j mod Ns) R + j mod Ns + r Ns].  Every thread holds 16 elements; the
-// first stage loads straight from HBM and the last stores straight to HBM, the stages in between
-// exchange through LDS.  The stage plan is fixed at compile time by log2 N (radix 16, plus one
+// registers and writes x[(j - j mod Ns) R + j mod Ns + r Ns].  Every thread holds 16 elements (32 in
+// the complex64 single-pass column kernel); the first stage loads straight from HBM and the last
+// stores straight to HBM, the stages in between exchange through LDS.  The row and single-pass
+// column kernels read their twiddles from two small LDS tables, so a stage issues no global loads.  The stage plan is fixed at compile time by log2 N (radix 16, plus one
 // 2/4/8 stage first for odd log2 N), so every stage unrolls completely.
 //
 // Row pass: one or more grid rows per block (contiguous lines).  The forward operator's row pass
@@ -43,12 +44,13 @@ LinePlan make_line_plan(int N) {
   return p;
 }
 
-// Compile-time stage plan of a 2^L-point line (the same plan make_line_plan describes).
-template <int L>
+// Compile-time stage plan of a 2^L-point line (the same plan make_line_plan describes) with EE
+// elements per thread (FFT_E, or 32 for the persistent column pass: E / radix butterflies a stage).
+template <int L, int EE = FFT_E>
 struct CPlan {
   static constexpr int N = 1 << L;
-  static constexpr int E = N < FFT_E ? N : FFT_E;  // elements per thread
-  static constexpr int T = N / E;                   // threads per line
+  static constexpr int E = N < EE ? N : EE;  // elements per thread
+  static constexpr int T = N / E;            // threads per line
   static constexpr int REM = N <= FFT_E ? 0 : L % 4;
   static constexpr int NST = N <= FFT_E ? 1 : L / 4 + (REM ? 1 : 0);
   static constexpr int rad(int s) { return N <= FFT_E ? N : ((REM && s == 0) ? (1 << REM) : 16); }
@@ -66,18 +68,61 @@ struct CPlan {
 // (the first stage's writes, 16 elements apart, would otherwise fall on one bank group).
 __device__ inline int lidx(int i, int es) { return es > 0 ? i * es : es == 0 ? i + (i >> 4) : (i + (i >> 4)) * -es; }
 
-template <typename R, int E, int RAD, int NS, int N, bool INV>
-__device__ inline void stage_bfly(cpx<R> (&v)[E], int t, const cpx<R>* __restrict__ tw, int twm) {
+// Element at a 32-bit byte offset from a uniform base: one VGPR per address (the base stays in SGPRs).
+// Every grid and image here is below 4 GiB (fft_plan_init checks).
+template <typename T>
+__device__ inline T& at_byte(T* base, unsigned off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(const_cast<std::remove_const_t<T>*>(base)) + off);
+}
+
+// Twiddle W_N^m of an N-point line, read from the device table of the longest axis (stride twm).
+template <typename R>
+struct TwGlobal {
+  const cpx<R>* __restrict__ tw;
+  int twm;
+  __device__ cpx<R> operator()(int m) const { return tw[m * twm]; }
+};
+
+// The same from two small LDS tables, W_N^m = W_N^(hi B) * W_N^lo for m = hi B + lo, B = 2^ceil(L/2):
+// B + N/B entries (128 at N = 4096) filled once per block, so the stages issue no global loads and a
+// line's HBM loads in flight are never waited on by a twiddle read.
+template <int L>
+constexpr int tw_lds_shift() { return (L + 1) / 2; }
+template <int L>
+constexpr int tw_lds_entries() { return (1 << tw_lds_shift<L>()) + ((1 << L) >> tw_lds_shift<L>()); }
+inline int tw_lds_entries_rt(int L) { return (1 << ((L + 1) / 2)) + ((1 << L) >> ((L + 1) / 2)); }
+
+template <typename R, int L>
+struct TwLds {
+  const cpx<R>* lo;  // W_N^lo, lo < B
+  const cpx<R>* hi;  // W_N^(hi B), hi < N / B
+  __device__ cpx<R> operator()(int m) const {
+    constexpr int SH = tw_lds_shift<L>();
+    return cmul(hi[m >> SH], lo[m & ((1 << SH) - 1)]);
+  }
+};
+
+// Fills the tables at tab from the device table (stride twm per N-point step); ends with a barrier.
+template <typename R, int L>
+__device__ inline TwLds<R, L> tw_lds_fill(cpx<R>* tab, const cpx<R>* __restrict__ tw, int twm) {
+  constexpr int SH = tw_lds_shift<L>(), B = 1 << SH, H = (1 << L) >> SH;
+  for (int i = threadIdx.x; i < B + H; i += blockDim.x) tab[i] = i < B ? tw[i * twm] : tw[((i - B) << SH) * twm];
+  __syncthreads();
+  return TwLds<R, L>{tab, tab + B};
+}
+
+template <typename R, int E, int RAD, int NS, int N, bool INV, class Tw>
+__device__ inline void stage_bfly(cpx<R> (&v)[E], int t, const Tw& tw) {
   constexpr int T = N / E;
 #pragma unroll
   for (int b = 0; b < E / RAD; ++b) {
     if constexpr (NS > 1) {
       const int j = t + b * T;
       const int k = j & (NS - 1);
-      const int step = k * (N / (NS * RAD)) * twm;
+      const int step = k * (N / (NS * RAD));
 #pragma unroll
       for (int r = 1; r < RAD; ++r) {
-        const cpx<R> w = tw[step * r];
+        const cpx<R> w = tw(step * r);
         v[b * RAD + r] = INV ? cmulc(v[b * RAD + r], w) : cmul(v[b * RAD + r], w);
       }
     }
@@ -86,27 +131,36 @@ __device__ inline void stage_bfly(cpx<R> (&v)[E], int t, const cpx<R>* __restric
 }
 
 // The first stage's inputs of thread t: register b RAD + r holds element t + b T + r N / RAD.
-template <int L>
+template <int L, int EE = FFT_E>
 __device__ inline int line_index(int t, int e) {
-  using P = CPlan<L>;
+  using P = CPlan<L, EE>;
   constexpr int N = P::N, T = P::T, RAD = P::rad(0);
   return t + (e / RAD) * T + (e % RAD) * (N / RAD);
 }
-template <typename R, int L, class Load>
-__device__ inline void line_load(cpx<R> (&v)[CPlan<L>::E], int t, Load& load) {
+// The last stage's output position of register e of thread t.
+template <int L, int EE = FFT_E>
+__device__ inline int store_index(int t, int e) {
+  using P = CPlan<L, EE>;
+  constexpr int S = P::NST - 1, RAD = P::rad(S), NS = P::ns(S), T = P::T;
+  const int b = e / RAD, r = e % RAD;
+  const int j = t + b * T, k = j & (NS - 1);
+  return (j - k) * RAD + k + r * NS;
+}
+template <typename R, int L, int EE, class Load>
+__device__ inline void line_load(cpx<R> (&v)[(CPlan<L, EE>::E)], int t, Load& load) {
 #pragma unroll
-  for (int e = 0; e < CPlan<L>::E; ++e) v[e] = load(line_index<L>(t, e));
+  for (int e = 0; e < CPlan<L, EE>::E; ++e) v[e] = load(line_index<L, EE>(t, e));
 }
 
 // One stage of a 2^L-point line: (load from HBM | read LDS) -> butterflies -> (store | write LDS),
 // then the next stage.  PRE: the first stage's inputs are already in v (line_load).
-template <typename R, int L, int S, bool INV, bool PRE, class Load, class Store>
-__device__ inline void stages(cpx<R> (&v)[CPlan<L>::E], int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm,
-                              Load& load, Store& store) {
-  using P = CPlan<L>;
+template <typename R, int L, int EE, int S, bool INV, bool PRE, class Tw, class Load, class Store>
+__device__ inline void stages(cpx<R> (&v)[(CPlan<L, EE>::E)], int t, cpx<R>* lds, int es, const Tw& tw, Load& load,
+                              Store& store) {
+  using P = CPlan<L, EE>;
   constexpr int N = P::N, E = P::E, T = P::T, RAD = P::rad(S), NS = P::ns(S);
   if constexpr (S == 0) {
-    if constexpr (!PRE) line_load<R, L>(v, t, load);
+    if constexpr (!PRE) line_load<R, L, EE>(v, t, load);
   } else {
     __syncthreads();  // the previous stage's LDS writes are visible
 #pragma unroll
@@ -114,13 +168,13 @@ __device__ inline void stages(cpx<R> (&v)[CPlan<L>::E], int t, cpx<R>* lds, int 
 #pragma unroll
       for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[lidx(t + b * T + r * (N / RAD), es)];
   }
-  stage_bfly<R, E, RAD, NS, N, INV>(v, t, tw, twm);
+  stage_bfly<R, E, RAD, NS, N, INV>(v, t, tw);
   if constexpr (S == P::NST - 1) {
 #pragma unroll
     for (int b = 0; b < E / RAD; ++b) {
       const int j = t + b * T, k = j & (NS - 1), o = (j - k) * RAD + k;
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) store(o + r * NS, v[b * RAD + r]);
+      for (int r = 0; r < RAD; ++r) store(o + r * NS, v[b * RAD + r], b * RAD + r);
     }
   } else {
     __syncthreads();  // every thread has read this stage's inputs before anyone overwrites them
@@ -130,21 +184,21 @@ __device__ inline void stages(cpx<R> (&v)[CPlan<L>::E], int t, cpx<R>* lds, int 
 #pragma unroll
       for (int r = 0; r < RAD; ++r) lds[lidx(o + r * NS, es)] = v[b * RAD + r];
     }
-    stages<R, L, S + 1, INV, PRE>(v, t, lds, es, tw, twm, load, store);
+    stages<R, L, EE, S + 1, INV, PRE>(v, t, lds, es, tw, load, store);
   }
 }
 
-template <typename R, int L, bool INV, class Load, class Store>
-__device__ inline void fft_line(int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm, Load& load, Store& store) {
-  cpx<R> v[CPlan<L>::E];
-  stages<R, L, 0, INV, false>(v, t, lds, es, tw, twm, load, store);
+template <typename R, int L, int EE, bool INV, class Tw, class Load, class Store>
+__device__ inline void fft_line(int t, cpx<R>* lds, int es, const Tw& tw, Load& load, Store& store) {
+  cpx<R> v[(CPlan<L, EE>::E)];
+  stages<R, L, EE, 0, INV, false>(v, t, lds, es, tw, load, store);
 }
 // The same with the first stage's inputs already loaded into v.
-template <typename R, int L, bool INV, class Store>
-__device__ inline void fft_line_pre(cpx<R> (&v)[CPlan<L>::E], int t, cpx<R>* lds, int es, const cpx<R>* tw, int twm,
+template <typename R, int L, int EE, bool INV, class Tw, class Store>
+__device__ inline void fft_line_pre(cpx<R> (&v)[(CPlan<L, EE>::E)], int t, cpx<R>* lds, int es, const Tw& tw,
                                     Store& store) {
   auto none = [](int) { return cpx<R>{}; };
-  stages<R, L, 0, INV, true>(v, t, lds, es, tw, twm, none, store);
+  stages<R, L, EE, 0, INV, true>(v, t, lds, es, tw, none, store);
 }
 
 // Calls F.template operator()<L>() for the runtime log2 length L in [1, 13].
@@ -199,71 +253,87 @@ __global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>()) k_f
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   constexpr int N = CPlan<L>::N, T = CPlan<L>::T, E = CPlan<L>::E;
+  constexpr bool SZM = MODE != ROW_PLAIN;
   const int l = threadIdx.x / T, t = threadIdx.x - l * T;
   cpx<R>* lds = smem + l * (N + N / 16);
+  const TwLds<R, L> tw = tw_lds_fill<R, L>(smem + a.lpb * (N + N / 16), a.tw, a.twm);
   const SZ<R>& sz = a.sz;
+  // S's column factors of this lane's elements, the same on every line: at the first stage's load
+  // positions (forward) or the last stage's store positions (adjoint); 0 outside the image columns
+  R sxr[SZM ? E : 1];
+  if constexpr (SZM) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = (MODE == ROW_FWD_SZ ? line_index<L>(t, e) : store_index<L>(t, e)) - sz.x0;
+      sxr[e] = (unsigned)i < (unsigned)sz.nx ? sz.sx[i] : R(0);
+    }
+  }
+  // row base pointers, then 32-bit unsigned element offsets (one VGPR per address)
   auto loader = [&](int line) {
     const bool valid = line < a.nlines;
-    return [&, line, valid](int idx) -> cpx<R> {
+    const cpx<R>* row = MODE == ROW_FWD_SZ     ? sz.img_in + (int64_t)line * sz.nx
+                        : MODE == ROW_ADJ_CROP ? a.in + (int64_t)(sz.y0 + line) * N
+                                               : a.in + (int64_t)line * N;
+    return [=, &sz](int idx) -> cpx<R> {
       if (!valid) return mk<R>(R(0), R(0));
-      if constexpr (MODE == ROW_FWD_SZ) {  // the image row, zero-padded; S is applied after the prefetch
-        const int i = idx - sz.x0;
-        if ((unsigned)i >= (unsigned)sz.nx) return mk<R>(R(0), R(0));
-        return sz.img_in[(int64_t)line * sz.nx + i];
-      } else if constexpr (MODE == ROW_ADJ_CROP) {
-        return a.in[(int64_t)(sz.y0 + line) * N + idx];
+      if constexpr (MODE == ROW_FWD_SZ) {  // the image row, zero-padded
+        const unsigned i = (unsigned)(idx - sz.x0);
+        if (i >= (unsigned)sz.nx) return mk<R>(R(0), R(0));
+        return row[i];
       } else {
-        return a.in[(int64_t)line * N + idx];
+        return row[(unsigned)idx];
       }
     };
   };
   const int stride = gridDim.x * a.lpb;
   const int first = blockIdx.x * a.lpb;  // the block's first line: the loop bound is block-uniform
+  // the next line's elements and S row factor are loaded together, before the current line's stages;
+  // the stages read only LDS (lines and twiddles), so nothing waits on them until the next iteration
   cpx<R> nxt[E];
+  R nsy = R(0);
   {
     auto ld = loader(first + l);
-    line_load<R, L>(nxt, t, ld);
+    line_load<R, L, FFT_E>(nxt, t, ld);
+    if constexpr (SZM)
+      if (first + l < a.nlines) nsy = sz.sy[first + l];
   }
   for (int base = first; base < a.nlines; base += stride) {
-    // opaque copies of the lane index and the twiddle pointer keep the address arithmetic and the
-    // twiddle loads inside the loop (hoisted, they would hold every stage's addresses and twiddles in
-    // registers across the loop)
-    const cpx<R>* tw = a.tw;
+    // an opaque copy of the lane index keeps the address arithmetic inside the loop (hoisted, every
+    // stage's addresses would stay live in registers across it)
     int tl = t;
-    asm volatile("" : "+s"(tw), "+v"(tl));
+    asm volatile("" : "+v"(tl));
     const int line = base + l;
     const bool valid = line < a.nlines;
     cpx<R> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nxt[e];
+    const R sy = nsy;
     if (base + stride < a.nlines) {
       auto ld = loader(line + stride);
-      line_load<R, L>(nxt, tl, ld);
+      line_load<R, L, FFT_E>(nxt, tl, ld);
+      if constexpr (SZM)
+        if (line + stride < a.nlines) nsy = sz.sy[line + stride];
     }
     if constexpr (MODE == ROW_FWD_SZ) {  // S: sy[line] sx[i] and the folded scale
-      if (valid) {
-        const R sl = sz.sy[line] * sz.scale;
+      const R sl = sy * sz.scale;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int i = line_index<L>(tl, e) - sz.x0;
-          if ((unsigned)i < (unsigned)sz.nx) v[e] = scale(v[e], sl * sz.sx[i]);
-        }
-      }
+      for (int e = 0; e < E; ++e) v[e] = scale(v[e], sl * sxr[e]);
     }
-    auto store = [&](int idx, cpx<R> val) {
+    cpx<R>* orow = MODE == ROW_FWD_SZ     ? a.out + (int64_t)(sz.y0 + line) * N
+                   : MODE == ROW_ADJ_CROP ? sz.img_out + (int64_t)line * sz.nx
+                                          : a.out + (int64_t)line * N;
+    auto store = [&](int idx, cpx<R> val, int e) {
       if (!valid) return;
-      if constexpr (MODE == ROW_FWD_SZ) {
-        a.out[(int64_t)(sz.y0 + line) * N + idx] = val;
-      } else if constexpr (MODE == ROW_ADJ_CROP) {
-        const int i = idx - sz.x0;
-        if ((unsigned)i >= (unsigned)sz.nx) return;
-        const R s = sz.sy[line] * sz.sx[i] * sz.scale;
-        sz.img_out[(int64_t)line * sz.nx + i] = scale(val, s);
+      if constexpr (MODE == ROW_ADJ_CROP) {
+        const unsigned i = (unsigned)(idx - sz.x0);
+        if (i >= (unsigned)sz.nx) return;
+        const R s = sy * sxr[e] * sz.scale;
+        orow[i] = scale(val, s);
       } else {
-        a.out[(int64_t)line * N + idx] = val;
+        orow[(unsigned)idx] = val;
       }
     };
-    fft_line_pre<R, L, INV>(v, tl, lds, 0, tw, a.twm, store);
+    fft_line_pre<R, L, FFT_E, INV>(v, tl, lds, 0, tw, store);
   }
 }
 
@@ -307,7 +377,7 @@ void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* 
   if (lpb > nlines) lpb = nlines;
   a.lpb = lpb;
   const int threads = lpb * T;
-  const size_t shm = (size_t)lpb * (N + N / 16) * sizeof(cpx<R>);
+  const size_t shm = ((size_t)lpb * (N + N / 16) + tw_lds_entries_rt(p.row.log2N)) * sizeof(cpx<R>);
   int blocks = (nlines + lpb - 1) / lpb;
   if (p.row_bpc > 0) blocks = std::min(blocks, p.row_bpc * p.n_cu);
   if (blocks == 0) return;
@@ -354,7 +424,7 @@ __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a) {
     if (a.rin && !in_window(row)) return mk<R>(R(0), R(0));
     return a.in[row * Nu + col];
   };
-  auto store = [&](int idx, cpx<R> val) {
+  auto store = [&](int idx, cpx<R> val, int) {
     int row;
     if constexpr (KIND == COL_SINGLE) {
       row = idx;
@@ -368,52 +438,91 @@ __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a) {
     if (a.rout && !in_window(row)) return;
     a.out[row * Nu + col] = val;
   };
-  fft_line<R, L, INV>(t, smem + c, C, a.tw, a.twm, load, store);
+  const TwGlobal<R> tw{a.tw, a.twm};
+  fft_line<R, L, FFT_E, INV>(t, smem + c, C, tw, load, store);
 }
 
-// Single-pass columns up to 8192 rows: C columns per block, 1024 threads at most, padded LDS tiles
-// above 64 KiB.  Adjacent column blocks share 128-byte lines of every row, so the block index is
-// remapped to give each XCD (workgroups go round-robin over the 8 XCDs) a contiguous band of
-// columns and its L2 the shared lines.
-template <typename R>
-constexpr int big_col_threads() { return sizeof(R) == 4 ? 1024 : 512; }  // FP64: room for 256 VGPRs
+// Single-pass columns of 1024..8192 rows: a block owns C adjacent columns (C fixed by the element size
+// and L: padded LDS tiles and the twiddle tables in 160 KiB, at most 1024 threads, 512 in FP64 for
+// room for 256 VGPRs).  The kernel walks column groups with a grid stride, the next group's
+// first-stage loads issued before the current group's stages; it is launched with one block per
+// group, which measured faster at 4096 x 4096 than a persistent grid (74.6 against 80.6 us in
+// complex64: with one 140 KiB block per CU, the CU's next block overlaps better than the prefetch).
+// With the group count and the grid both multiples of 8, the blocks of one XCD (workgroups go
+// round-robin over the 8 XCDs) take one contiguous band of groups, so that its concurrently running
+// blocks share the rows' 128-byte lines in its L2.
+// Elements per thread of that pass: 32 in complex64 (512 threads hold four 4096-point columns, with
+// registers for the prefetched group beside the current one), 16 in complex128.
+constexpr int big_col_e(int esz) { return esz == 16 ? FFT_E : 2 * FFT_E; }
+constexpr int big_col_c(int esz, int L) {
+  const int N = 1 << L, T = N / big_col_e(esz);
+  const int max_threads = esz == 16 ? 512 : 1024;
+  const int tab = (1 << ((L + 1) / 2)) + (N >> ((L + 1) / 2));
+  int C = 8;
+  while (C > 1 && ((N + N / 16) * C * esz + tab * esz > 160 * 1024 || C * T > max_threads)) C /= 2;
+  return C;
+}
+
+template <typename R, int L>
+constexpr int big_col_threads() { return big_col_c(sizeof(cpx<R>), L) * ((1 << L) / big_col_e(sizeof(cpx<R>))); }
 
 template <typename R, int L, bool INV>
-__global__ void __launch_bounds__(big_col_threads<R>()) k_fft_cols_big(ColArgs<R> a) {
+__global__ void __launch_bounds__((big_col_threads<R, L>())) k_fft_cols_big(ColArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
-  const int C = a.C;
+  constexpr int EE = big_col_e(sizeof(cpx<R>));
+  constexpr int C = big_col_c(sizeof(cpx<R>), L), N = CPlan<L, EE>::N, E = CPlan<L, EE>::E;
   const int c = threadIdx.x % C, t = threadIdx.x / C;
-  const int G = gridDim.x;
-  const int blk = (G & 7) ? (int)blockIdx.x : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
-  const int col = blk * C + c;
-  const int64_t Nu = a.Nu;
+  const TwLds<R, L> tw = tw_lds_fill<R, L>(smem + C * (N + N / 16), a.tw, a.twm);
+  const unsigned Nu = (unsigned)a.Nu;
+  const int G = a.Nu / C, P = gridDim.x;
+  const bool band = !(G & 7) && !(P & 7);
+  const int q = band ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  const int qstep = band ? P >> 3 : P, qend = band ? G >> 3 : G;
+  const int gbase = band ? (int)(blockIdx.x & 7) * (G >> 3) : 0;
   auto in_window = [&](int row) { return (unsigned)(row - a.y0) < (unsigned)a.ny; };
-  auto load = [&](int row) -> cpx<R> {
-    if (a.rin && !in_window(row)) return mk<R>(R(0), R(0));
-    return a.in[row * Nu + col];
+  // 32-bit unsigned element offsets from the column's base (the grid holds < 2^32 cells)
+  auto load_group = [&](cpx<R> (&v)[E], int g, int tl) {
+    const unsigned col = (unsigned)(g * C + c);
+    auto ld = [&](int row) -> cpx<R> {
+      if (a.rin && !in_window(row)) return mk<R>(R(0), R(0));
+      return at_byte(a.in, ((unsigned)row * Nu + col) * (unsigned)sizeof(cpx<R>));
+    };
+    line_load<R, L, EE>(v, tl, ld);
   };
-  auto store = [&](int row, cpx<R> val) {
-    if (a.rout && !in_window(row)) return;
-    a.out[row * Nu + col] = val;
-  };
-  fft_line<R, L, INV>(t, smem + c, -C, a.tw, a.twm, load, store);
+  cpx<R> nxt[E];
+  if (q < qend) load_group(nxt, gbase + q, t);
+  for (int k = q; k < qend; k += qstep) {
+    // an opaque copy of the lane index keeps the address arithmetic inside the loop
+    int tl = t;
+    asm volatile("" : "+v"(tl));
+    cpx<R> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = nxt[e];
+    if (k + qstep < qend) load_group(nxt, gbase + k + qstep, tl);
+    const unsigned col = (unsigned)((gbase + k) * C + c);
+    auto store = [&](int row, cpx<R> val, int) {
+      if (a.rout && !in_window(row)) return;
+      at_byte(a.out, ((unsigned)row * Nu + col) * (unsigned)sizeof(cpx<R>)) = val;
+    };
+    fft_line_pre<R, L, EE, INV>(v, tl, smem + c, -C, tw, store);
+  }
 }
 
 template <typename R>
 struct BigColLaunch {
   const ColArgs<R>& a;
   bool inv;
-  dim3 grid;
-  int threads;
-  size_t shm;
   hipStream_t s;
   template <int L>
   void operator()() {
     if constexpr (L >= 10 && L <= 13) {
+      constexpr int C = big_col_c(sizeof(cpx<R>), L), N = 1 << L;
       void (*k)(ColArgs<R>) = inv ? k_fft_cols_big<R, L, true> : k_fft_cols_big<R, L, false>;
+      const size_t shm = ((size_t)(N + N / 16) * C + tw_lds_entries<L>()) * sizeof(cpx<R>);
       allow_lds((const void*)k, shm);
-      hipLaunchKernelGGL(k, grid, dim3(threads), shm, s, a);
+      const int groups = a.Nu / C;
+      hipLaunchKernelGGL(k, dim3(groups), dim3(big_col_threads<R, L>()), shm, s, a);
     } else {
       throw Error(-3, "single-pass column length must be 1024..8192");
     }
@@ -479,10 +588,8 @@ void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow w
     a.C = p.big_C;
     a.rin = restrict_in;
     a.rout = restrict_out;
-    const int N = p.Nv;
-    const size_t shm = (size_t)(N + N / 16) * a.C * sizeof(cpx<R>);
     ProfScope prof("fft_cols", s);
-    BigColLaunch<R> launch{a, inv, dim3(p.Nu / a.C), a.C * (N / FFT_E), shm, s};
+    BigColLaunch<R> launch{a, inv, s};
     dispatch_log2(p.col.log2N, launch);
     PF_HIP(hipGetLastError());
     return;
@@ -553,6 +660,8 @@ static void twiddle_table(int L, std::vector<double>& re, std::vector<double>& i
 void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
   if (Nv > MAX_AXIS || Nu > MAX_AXIS)
     throw Error(-3, "grid axis larger than " + std::to_string(MAX_AXIS) + " is not supported");
+  if ((uint64_t)Nv * Nu * (prec ? 16 : 8) >= ((uint64_t)1 << 32))
+    throw Error(-3, "grid of 4 GiB or more is not supported (32-bit byte offsets)");
   p.Nv = Nv;
   p.Nu = Nu;
   p.prec = prec;
@@ -585,11 +694,7 @@ void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
     p.col_two_step = false;
     p.col_big = true;
     p.col = make_line_plan(Nv);
-    const int max_threads = prec ? 512 : 1024;
-    int C = 8;
-    while (C > 1 && ((size_t)(Nv + Nv / 16) * C * esz > 160 * 1024 - 1024 || C * (Nv / FFT_E) > max_threads)) C /= 2;
-    if (C > Nu) C = Nu;
-    p.big_C = C;
+    p.big_C = big_col_c((int)esz, p.col.log2N);  // the kernel's own compile-time choice
   } else {
     p.col_two_step = true;
     int m = 0;
