@@ -127,14 +127,19 @@ int pf_wavelet_adjoint(pf_wavelet *wv, const void *image, void *coeffs, int x_lo
 int pf_wavelet_destroy(pf_wavelet *wv);
 
 /* ---- Imaging PADMM on the device (sopt::algorithm::ImagingProximalADMM with the parameters of
- * padmm_factory, algorithm_factory.h:36-81) for a tight-frame dictionary: one orthonormal wavelet,
- * as main.cc:121-126 selects when the dictionary has one entry ----
+ * padmm_factory, algorithm_factory.h:36-81) ----
  * Iteration (SOPT variable names): z = P_eps(-lambda - r); x~ = x - Phi^dagger(r + lambda + z) / nu;
- * x = x~ + Psi(soft(Psi^dagger x~, gamma nu) - Psi^dagger x~) / nu; r = Phi x - y;
- * lambda += lagrange_update_scale (r + z); from x = Phi^dagger y / nu, r = Phi x - y, lambda = 0.
- * SOPT is not in the reference tree: this is a restatement of the published algorithm and its
- * agreement with SOPT's iterates is unpinned.  Norms are FP64 reductions on the device; an iteration
- * runs without host synchronisation and can be captured once into a hipGraph and replayed.  phi and
+ * x = prox(x~); r = Phi x - y; lambda += lagrange_update_scale (r + z); from x = Phi^dagger y / nu,
+ * r = Phi x - y, lambda = 0.  The l1 proximal is the closed form x~ + Psi(soft(Psi^dagger x~, gamma nu)
+ * - Psi^dagger x~) / nu for a tight frame (one orthonormal wavelet: main.cc:121-126), or, for SARA, the
+ * iterative constrained one: argmin 1/2 ||x - x~||^2 + gamma ||Psi^dagger x||_1 over the positivity /
+ * real constraints, by the dual forward-backward iteration (u = P_gamma(u + Psi^dagger x / l1_nu),
+ * x = P_C(x~ - Psi u), from u = 0) until the objective's relative change is below l1_tolerance.
+ * SOPT is not in the reference tree: this is a restatement of the published algorithms and its
+ * agreement with SOPT's iterates is unpinned.  Norms are FP64 reductions on the device; a tight-frame
+ * iteration runs without host synchronisation and can be captured once into a hipGraph and replayed;
+ * the iterative proximal reads its objective on the host every step, so it always runs eagerly
+ * (use_graph is ignored, and pf_padmm_enqueue returns only when its iterations are done).  phi and
  * psi must outlive the solver, and no other call may use them while it runs (their work buffers are
  * shared). */
 typedef struct pf_padmm pf_padmm;
@@ -146,10 +151,19 @@ typedef struct pf_padmm_params {
   double relative_variation;    /* 1e-3 */
   double residual_tolerance;    /* <= 0: epsilon (algorithm_factory.h:74) */
   int32_t itermax;              /* the most iterations the solver will run (sizes its statistics) */
+  /* the l1 proximal (algorithm_factory.h:60-66) */
+  int32_t tight_frame;          /* 1: closed form (one basis); 0: iterative, constrained; -1: 1 iff one basis */
+  int32_t l1_itermax;           /* 50: iterative proximal steps at most */
+  double l1_tolerance;          /* 1e-2: its objective's relative change that stops it */
+  double l1_nu;                 /* 1: its frame bound (step 1 / l1_nu) */
+  int32_t positivity;           /* 1: Re x >= 0 and Im x >= 0 in the iterative proximal */
+  int32_t real_constraint;      /* 1: Im x = 0 in the iterative proximal */
 } pf_padmm_params;
 typedef struct pf_padmm_result {
   int32_t niters, converged;
   double relative_variation, residual_norm, gamma;
+  int32_t l1_iters_last;        /* iterative proximal: steps of the last call, */
+  int64_t l1_iters_total;       /* and of all calls since the last reset */
 } pf_padmm_result;
 enum { PF_PADMM_IMAGE = 0, PF_PADMM_RESIDUAL = 1 };
 int pf_padmm_params_init(pf_padmm_params *p);

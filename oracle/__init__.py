@@ -524,17 +524,58 @@ def soft_threshold(c, t):
     return c * f
 
 
+def project_c(x, positivity=True, real=True):
+    """The constraint set of the iterative l1 proximal (algorithm_factory.h:65-66): real zeroes Im x,
+    positivity clamps Re x and Im x at 0."""
+    x = np.asarray(x, np.complex128)
+    re, im = x.real.copy(), (np.zeros_like(x.real) if real else x.imag.copy())
+    if positivity:
+        re, im = np.maximum(re, 0.0), np.maximum(im, 0.0)
+    return re + 1j * im
+
+
+def l1_prox_constrained(x0, gamma, psit, psi, nu=1.0, itermax=50, tol=1e-2, positivity=True, real=True):
+    """argmin_x 1/2 ||x - x0||^2 + gamma ||Psi^dagger x||_1 + i_C(x) by the dual forward-backward iteration
+    (Combettes & Pesquet 2011, "Proximal splitting methods in signal processing"; Fadili & Starck
+    2009) with step 1 / nu -- the l1 proximal padmm_factory configures for SARA
+    (algorithm_factory.h:60-66: tolerance 1e-2, nu 1, 50 iterations, positivity and real constraints):
+        u_0 = 0, x_0 = P_C(x0);  u_k = P_gamma(u_(k-1) + Psi^dagger x_(k-1) / nu),  x_k = P_C(x0 - Psi u_k)
+    P_gamma clamps each coefficient's modulus to gamma.  Stops after itermax steps or when the objective
+    changes by less than tol relative.  SOPT (not in the reference tree) is not followed line by line:
+    parity with its iterates is unpinned.  Returns (x, steps)."""
+    x0 = np.asarray(x0, np.complex128)
+    x = project_c(x0, positivity, real)
+    c = psit(x)
+    obj = 0.5 * float(np.vdot(x - x0, x - x0).real) + gamma * float(np.abs(c).sum())
+    u = np.zeros_like(c)
+    k = 0
+    while k < itermax:
+        k += 1
+        w = u + c / nu
+        m = np.abs(w)
+        u = w * np.where(m > gamma, gamma / np.where(m > 0, m, 1.0), 1.0)
+        x = project_c(x0 - psi(u), positivity, real)
+        c = psit(x)
+        o = 0.5 * float(np.vdot(x - x0, x - x0).real) + gamma * float(np.abs(c).sum())
+        done = abs(o - obj) < tol * abs(o)
+        obj = o
+        if done:
+            break
+    return x, k
+
+
 def padmm(op: "Operator", entries, y, epsilon, iters, gamma=None, nu=1.0, rho=0.9, relative_variation=1e-3,
-          residual_tolerance=None, stop=False):
+          residual_tolerance=None, stop=False, tight_frame=True, l1_itermax=50, l1_tol=1e-2, l1_nu=1.0,
+          positivity=True, real=True):
     """Imaging proximal ADMM with a tight-frame l1 proximal (the iteration SOPT's ImagingProximalADMM runs
     with padmm_factory's parameters, algorithm_factory.h:36-81), in FP64:
         z = P_eps(-lambda - r);  x~ = x - Phi^dagger(r + lambda + z) / nu
         x = x~ + Psi(soft(Psi^dagger x~, gamma nu) - Psi^dagger x~) / nu
         r = Phi x - y;  lambda += rho (r + z)
     from x = Phi^dagger y / nu, r = Phi x - y, lambda = 0; gamma = max |Psi^dagger Phi^dagger y| * 1e-3 if
-    not given.  Restated from the published algorithm (SOPT is not in the reference tree): parity with
-    SOPT's iterates is unpinned.  Returns (x, stats, gamma, niters): stats[k] = (||x_k - x_(k-1)||^2,
-    ||x_k||^2, ||r_k||^2)."""
+    not given.  tight_frame False: x = l1_prox_constrained(x~, gamma, ...) instead (SARA).  Restated from
+    the published algorithm (SOPT is not in the reference tree): parity with SOPT's iterates is
+    unpinned.  Returns (x, stats, gamma, niters): stats[k] = (||x_k - x_(k-1)||^2, ||x_k||^2, ||r_k||^2)."""
     ny, nx = op.ny, op.nx
     y = np.asarray(y, np.complex128)
     psit = lambda v: sara_analysis(v, ny, nx, entries)  # noqa: E731
@@ -552,8 +593,11 @@ def padmm(op: "Operator", entries, y, epsilon, iters, gamma=None, nu=1.0, rho=0.
         nv = np.linalg.norm(v)
         z = v * (epsilon / nv if nv > epsilon else 1.0)
         xu = x - op.adjoint(r + lam + z) / nu
-        c = psit(xu)
-        xn = xu + psi(soft_threshold(c, gamma * nu) - c) / nu
+        if tight_frame:
+            c = psit(xu)
+            xn = xu + psi(soft_threshold(c, gamma * nu) - c) / nu
+        else:
+            xn, _ = l1_prox_constrained(xu, gamma, psit, psi, l1_nu, l1_itermax, l1_tol, positivity, real)
         r = op.forward(xn) - y
         lam = lam + rho * (r + z)
         stats.append((float(np.vdot(xn - x, xn - x).real), float(np.vdot(xn, xn).real), float(np.vdot(r, r).real)))

@@ -1,7 +1,9 @@
 // pf_solver.hip -- the imaging PADMM of PURIFY's pipeline on the device: SOPT's
 // sopt::algorithm::ImagingProximalADMM with the parameters padmm_factory sets
-// (cpp/purify/algorithm_factory.h:36-81), for a tight-frame dictionary (one orthonormal wavelet:
-// main.cc:121-126 sets tight_frame = sara.size() < 2).
+// (cpp/purify/algorithm_factory.h:36-81): the closed-form l1 proximal for a tight frame (one
+// orthonormal wavelet: main.cc:121-126 sets tight_frame = sara.size() < 2), or the iterative,
+// constrained one (SARA; algorithm_factory.h:60-66: l1_proximal_tolerance 1e-2, nu 1, itermax 50,
+// positivity and real constraints).
 //
 // SOPT is not in the reference tree, so the iteration is restated from the published proximal ADMM
 // (Carrillo, McEwen & Wiaux 2014, "PURIFY: a new approach to radio-interferometric imaging",
@@ -13,6 +15,15 @@
 //   lambda = lambda + rho (r + z)                        rho = lagrange_update_scale
 // from x = Phi^dagger y / nu, r = Phi x - y, lambda = 0.  Convergence: ||x - x_prev|| / ||x|| <
 // relative_variation and ||r|| < residual_tolerance.  soft(c, t) = c max(0, 1 - t / |c|).
+//
+// The iterative proximal, x = argmin 1/2 ||x - x~||^2 + gamma ||Psi^dagger x||_1 over the constraint set
+// C (Re x >= 0 and Im x >= 0 with positivity, Im x = 0 with the real constraint), is the dual
+// forward-backward iteration (Combettes & Pesquet 2011, "Proximal splitting methods in signal
+// processing", the dual of the constrained l1 problem; Fadili & Starck 2009) with step 1 / l1_nu:
+//   u_0 = 0, x_0 = P_C(x~);  u_k = P_gamma(u_(k-1) + Psi^dagger x_(k-1) / l1_nu),  x_k = P_C(x~ - Psi u_k)
+// P_gamma clamps each coefficient's modulus to gamma.  It stops after l1_itermax steps or once the
+// objective 1/2 ||x_k - x~||^2 + gamma ||Psi^dagger x_k||_1 changes by less than l1_tolerance relative:
+// that test reads two FP64 sums on the host, so the iterative mode runs eagerly (no graph).
 //
 // Vectors stay in the operator's precision; every norm is reduced in FP64 on the device in a fixed
 // order, and the projection reads its norm from device memory, so an iteration has no host
@@ -37,8 +48,8 @@ __device__ inline double norm2d(cpx<R> a) {
   return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
 }
 
-// partials[b] = sum |a - b|^2 (B given) or |a|^2 over a grid-stride slice; fixed order
-template <typename R, bool DIFF>
+// partials[b] = sum |a - b|^2 (B given) or |a|^2 (ABS: sum |a|) over a grid-stride slice; fixed order
+template <typename R, bool DIFF, bool ABS = false>
 __global__ void __launch_bounds__(SR_THREADS) k_sq_partial(const cpx<R>* __restrict__ a, const cpx<R>* __restrict__ b,
                                                            int64_t n, double* __restrict__ part) {
   __shared__ double sh[SR_THREADS];
@@ -50,7 +61,7 @@ __global__ void __launch_bounds__(SR_THREADS) k_sq_partial(const cpx<R>* __restr
       v.x -= w.x;
       v.y -= w.y;
     }
-    acc += norm2d(v);
+    acc += ABS ? sqrt(norm2d(v)) : norm2d(v);
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -76,6 +87,13 @@ template <typename R>
 void sq_norm(const cpx<R>* a, const cpx<R>* b, int64_t n, double* part, double* out, hipStream_t s) {
   if (b) hipLaunchKernelGGL((k_sq_partial<R, true>), dim3(SR_BLOCKS), dim3(SR_THREADS), 0, s, a, b, n, part);
   else hipLaunchKernelGGL((k_sq_partial<R, false>), dim3(SR_BLOCKS), dim3(SR_THREADS), 0, s, a, b, n, part);
+  hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(SR_THREADS), 0, s, part, out);
+}
+
+// *out = sum |a| (FP64, fixed order)
+template <typename R>
+void abs_sum(const cpx<R>* a, int64_t n, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_sq_partial<R, false, true>), dim3(SR_BLOCKS), dim3(SR_THREADS), 0, s, a, nullptr, n, part);
   hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(SR_THREADS), 0, s, part, out);
 }
 
@@ -131,6 +149,40 @@ __global__ void k_prox_out(const cpx<R>* __restrict__ xu, const cpx<R>* __restri
   GRID_STRIDE(k, n) {
     const cpx<R> a = xu[k], b = d[k];
     x[k] = mk<R>(a.x + b.x * inv_nu, a.y + b.y * inv_nu);
+  }
+}
+// the constraint set of the iterative proximal: positivity keeps both parts >= 0, real zeroes Im
+template <typename R>
+__device__ inline cpx<R> project_c(cpx<R> a, bool pos, bool real) {
+  if (real) a.y = R(0);
+  if (pos) {
+    a.x = a.x > R(0) ? a.x : R(0);
+    a.y = a.y > R(0) ? a.y : R(0);
+  }
+  return a;
+}
+// x = P_C(x0 - d) (d null: P_C(x0))
+template <typename R>
+__global__ void k_l1_primal(const cpx<R>* __restrict__ x0, const cpx<R>* __restrict__ d, cpx<R>* __restrict__ x, int pos,
+                            int real, int64_t n) {
+  GRID_STRIDE(k, n) {
+    cpx<R> a = x0[k];
+    if (d) {
+      const cpx<R> b = d[k];
+      a = mk<R>(a.x - b.x, a.y - b.y);
+    }
+    x[k] = project_c(a, pos != 0, real != 0);
+  }
+}
+// u = P_gamma(u + c / nu): each coefficient's modulus clamped to gamma
+template <typename R>
+__global__ void k_l1_dual(cpx<R>* __restrict__ u, const cpx<R>* __restrict__ c, R inv_nu, R gamma, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = u[k], b = c[k];
+    const cpx<R> w = mk<R>(a.x + b.x * inv_nu, a.y + b.y * inv_nu);
+    const R m = sqrt(w.x * w.x + w.y * w.y);
+    const R f = m > gamma ? gamma / m : R(1);
+    u[k] = mk<R>(w.x * f, w.y * f);
   }
 }
 // r = fx - y; lambda += rho (r + z)
@@ -196,7 +248,11 @@ struct pf_padmm {
   void *x = nullptr, *xprev = nullptr, *xu = nullptr, *g = nullptr, *c = nullptr, *d = nullptr;
   void *y = nullptr, *r = nullptr, *lam = nullptr, *z = nullptr, *v = nullptr, *fx = nullptr;
   double* part = nullptr;  // reduction partials
-  double* cur = nullptr;   // [0] ||x - xprev||^2, [1] ||x||^2, [2] ||r||^2, [3] ||v||^2
+  double* cur = nullptr;   // [0] ||x - xprev||^2, [1] ||x||^2, [2] ||r||^2, [3] ||v||^2, [4] ||x - x~||^2, [5] ||Psi^dagger x||_1
+  bool tight = true;       // closed-form tight-frame proximal, else the iterative one
+  void* u = nullptr;       // the iterative proximal's dual variable (NC coefficients)
+  int l1_last = 0;         // inner iterations of the last iterative proximal
+  int64_t l1_total = 0;
   double* stats = nullptr;
   int* counter = nullptr;
   int cap = 0;
@@ -228,7 +284,45 @@ cpx<R>* V(void* p) {
   return static_cast<cpx<R>*>(p);
 }
 
-// one PADMM iteration on h->stream (no host synchronisation)
+// x = prox(xu) by the iterative constrained proximal (eager: the stopping test reads the objective)
+template <typename R>
+void l1_prox_iterative(pf_padmm* h) {
+  hipStream_t s = h->stream;
+  const int64_t N = h->N, NC = h->NC;
+  const bool pos = h->par.positivity != 0, real = h->par.real_constraint != 0;
+  const R inv_nu = (R)(1.0 / h->par.l1_nu), gam = (R)h->gamma;
+  auto objective = [&]() {
+    sq_norm<R>(V<R>(h->x), V<R>(h->xu), N, h->part, h->cur + 4, s);
+    abs_sum<R>(V<R>(h->c), NC, h->part, h->cur + 5, s);
+    double o[2];
+    PF_HIP(hipMemcpyAsync(o, h->cur + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+    return 0.5 * o[0] + h->gamma * o[1];
+  };
+  hipLaunchKernelGGL((k_zero<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->u), NC);
+  hipLaunchKernelGGL((k_l1_primal<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), (const cpx<R>*)nullptr, V<R>(h->x),
+                     (int)pos, (int)real, N);
+  wavelet_analysis<R>(h->psi, V<R>(h->x), V<R>(h->c), s);
+  double obj = objective();
+  int k = 0;
+  while (k < h->par.l1_itermax) {
+    ++k;
+    hipLaunchKernelGGL((k_l1_dual<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->u), V<R>(h->c), inv_nu, gam, NC);
+    wavelet_synthesis<R>(h->psi, V<R>(h->u), V<R>(h->d), s);
+    hipLaunchKernelGGL((k_l1_primal<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x), (int)pos,
+                       (int)real, N);
+    wavelet_analysis<R>(h->psi, V<R>(h->x), V<R>(h->c), s);
+    const double o = objective();
+    const bool stop = std::abs(o - obj) < h->par.l1_tolerance * std::abs(o);
+    obj = o;
+    if (stop) break;
+  }
+  h->l1_last = k;
+  h->l1_total += k;
+  PF_HIP(hipGetLastError());
+}
+
+// one PADMM iteration on h->stream (no host synchronisation with the tight-frame proximal)
 template <typename R>
 void padmm_iteration(pf_padmm* h) {
   hipStream_t s = h->stream;
@@ -242,10 +336,14 @@ void padmm_iteration(pf_padmm* h) {
   op_adjoint<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
   hipLaunchKernelGGL((k_grad_step<R>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev), V<R>(h->xu),
                      inv_nu, N);
-  wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
-  hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.nu), NC);
-  wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
-  hipLaunchKernelGGL((k_prox_out<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x), inv_nu, N);
+  if (h->tight) {
+    wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
+    hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.nu), NC);
+    wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
+    hipLaunchKernelGGL((k_prox_out<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x), inv_nu, N);
+  } else {
+    l1_prox_iterative<R>(h);
+  }
   op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
   hipLaunchKernelGGL((k_residual_dual<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->z), V<R>(h->r),
                      V<R>(h->lam), (R)h->par.lagrange_update_scale, M);
@@ -281,6 +379,8 @@ void padmm_init(pf_padmm* h) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, h->counter, 0);
   PF_HIP(hipStreamSynchronize(s));
   h->iters_done = 0;
+  h->l1_last = 0;
+  h->l1_total = 0;
 }
 
 template <typename R>
@@ -305,6 +405,7 @@ void padmm_capture(pf_padmm* h) {
 template <typename R>
 void padmm_enqueue(pf_padmm* h, int iters, bool use_graph) {
   if (h->iters_done + iters > h->cap) throw Error(PF_ERR_ARG, "more iterations than the solver was created for");
+  use_graph = use_graph && h->tight;
   if (use_graph) padmm_capture<R>(h);
   for (int i = 0; i < iters; ++i) {
     if (use_graph) PF_HIP(hipGraphLaunch(h->exec, h->stream));
@@ -328,6 +429,8 @@ void padmm_result(pf_padmm* h, bool converged, pf_padmm_result* res) {
     res->relative_variation = st[1] > 0 ? std::sqrt(st[0] / st[1]) : 0.0;
     res->residual_norm = std::sqrt(st[2]);
     res->gamma = h->gamma;
+    res->l1_iters_last = h->l1_last;
+    res->l1_iters_total = h->l1_total;
   }
 }
 
@@ -335,6 +438,7 @@ template <typename R>
 void padmm_run(pf_padmm* h, int iters, bool use_graph, int check_every, pf_padmm_result* res) {
   hipStream_t s = h->stream;
   if (h->iters_done + iters > h->cap) throw Error(PF_ERR_ARG, "more iterations than the solver was created for");
+  use_graph = use_graph && h->tight;
   if (use_graph) padmm_capture<R>(h);
   const int chunk = check_every > 0 ? check_every : iters;
   int done = 0;
@@ -375,6 +479,13 @@ int pf_padmm_params_init(pf_padmm_params* p) {
   p->relative_variation = 1e-3;
   p->residual_tolerance = 0;  // 0: epsilon (algorithm_factory.h:74)
   p->itermax = 500;
+  // algorithm_factory.h:60-66
+  p->tight_frame = -1;
+  p->l1_itermax = 50;
+  p->l1_tolerance = 1e-2;
+  p->l1_nu = 1.0;
+  p->positivity = 1;
+  p->real_constraint = 1;
   return PF_OK;
 }
 
@@ -388,9 +499,15 @@ int pf_padmm_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const
     const WaveletShape ws = wavelet_shape(psi);
     if (os.ny != ws.ny || os.nx != ws.nx) throw Error(PF_ERR_ARG, "operator and dictionary image sizes differ");
     if (os.prec != ws.prec) throw Error(PF_ERR_ARG, "operator and dictionary precisions differ");
-    if (ws.nwav != 1)
-      throw Error(PF_ERR_UNSUPPORTED,
-                  "the device PADMM needs a tight frame (one wavelet basis); SARA's iterative l1 proximal is not built");
+    if (params->tight_frame > 0 && ws.nwav > 1)
+      throw Error(PF_ERR_ARG,
+                  "l1 proximal not consistent: a tight frame was requested, but the dictionary has more than one basis");
+    if (params->tight_frame == 0 || (params->tight_frame < 0 && ws.nwav > 1)) {
+      if (!(params->l1_nu > 0)) throw Error(PF_ERR_ARG, "l1_nu must be positive");
+      if (params->l1_itermax < 1) throw Error(PF_ERR_ARG, "l1_itermax must be positive");
+      if (!(params->l1_tolerance >= 0)) throw Error(PF_ERR_ARG, "l1_tolerance must be >= 0");
+      h->tight = false;
+    }
     if (!(params->epsilon > 0)) throw Error(PF_ERR_ARG, "epsilon (the l2-ball radius) must be positive");
     if (!(params->nu > 0)) throw Error(PF_ERR_ARG, "nu must be positive");
     if (params->itermax < 1) throw Error(PF_ERR_ARG, "itermax must be positive");
@@ -405,9 +522,10 @@ int pf_padmm_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const
     const size_t e = os.prec == PF_C64 ? 8 : 16;
     for (void** p : {&h->x, &h->xprev, &h->xu, &h->g, &h->d}) *p = h->dalloc(e * h->N);
     h->c = h->dalloc(e * h->NC);
+    if (!h->tight) h->u = h->dalloc(e * h->NC);
     for (void** p : {&h->y, &h->r, &h->lam, &h->z, &h->v, &h->fx}) *p = h->dalloc(e * std::max<int64_t>(h->M, 1));
     h->part = (double*)h->dalloc(sizeof(double) * SR_BLOCKS);
-    h->cur = (double*)h->dalloc(sizeof(double) * 4);
+    h->cur = (double*)h->dalloc(sizeof(double) * 6);
     h->cap = params->itermax;
     h->stats = (double*)h->dalloc(sizeof(double) * 3 * h->cap);
     h->counter = (int*)h->dalloc(sizeof(int));

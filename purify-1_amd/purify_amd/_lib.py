@@ -19,12 +19,15 @@ PF_PADMM_IMAGE, PF_PADMM_RESIDUAL = 0, 1
 class pf_padmm_params(C.Structure):
     _fields_ = [("gamma", C.c_double), ("epsilon", C.c_double), ("nu", C.c_double),
                 ("lagrange_update_scale", C.c_double), ("relative_variation", C.c_double),
-                ("residual_tolerance", C.c_double), ("itermax", C.c_int32)]
+                ("residual_tolerance", C.c_double), ("itermax", C.c_int32),
+                ("tight_frame", C.c_int32), ("l1_itermax", C.c_int32), ("l1_tolerance", C.c_double),
+                ("l1_nu", C.c_double), ("positivity", C.c_int32), ("real_constraint", C.c_int32)]
 
 
 class pf_padmm_result(C.Structure):
     _fields_ = [("niters", C.c_int32), ("converged", C.c_int32), ("relative_variation", C.c_double),
-                ("residual_norm", C.c_double), ("gamma", C.c_double)]
+                ("residual_norm", C.c_double), ("gamma", C.c_double), ("l1_iters_last", C.c_int32),
+                ("l1_iters_total", C.c_int64)]
 PF_LAMBDA, PF_RADIANS, PF_PIXELS = 0, 1, 2
 
 

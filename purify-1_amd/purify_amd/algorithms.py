@@ -1,12 +1,13 @@
 """Imaging PADMM on the GPU: factory::algorithm_factory / padmm_factory
-(cpp/purify/algorithm_factory.h:36-146) for a tight-frame dictionary.
+(cpp/purify/algorithm_factory.h:36-146).
 
 ``padmm_factory(dist, measurements, wavelets, uv_data, sigma, imsizey, imsizex, sara_size, ...)`` takes
 the reference's arguments and returns an ``ImagingProximalADMM`` whose ``run()`` gives a diagnostic
-(``x``, ``niters``, ``good``).  The device solver implements the tight-frame l1 proximal (one wavelet
-basis, as main.cc:121-126 selects for one dictionary entry); SARA's iterative, constrained l1
-proximal is not built, so ``tight_frame=False`` raises.  SOPT is not in the reference tree: the
-iteration restates the published algorithm and its agreement with SOPT's iterates is unpinned.
+(``x``, ``niters``, ``good``).  The l1 proximal is the closed form for a tight frame (one wavelet
+basis, as main.cc:121-126 selects for one dictionary entry) or, with ``tight_frame=False`` (SARA), the
+iterative proximal under the positivity and real constraints (algorithm_factory.h:60-66), which runs
+eagerly (its stopping test reads the objective every step).  SOPT is not in the reference tree: the
+iterations restate the published algorithms and their agreement with SOPT's iterates is unpinned.
 """
 from __future__ import annotations
 
@@ -31,7 +32,10 @@ class Diagnostic:
 
 class ImagingProximalADMM:
     def __init__(self, phi: MeasurementOperator, psi: WaveletOperator, y, epsilon, gamma=0.0, nu=1.0,
-                 lagrange_update_scale=0.9, relative_variation=1e-3, residual_tolerance=0.0, itermax=500):
+                 lagrange_update_scale=0.9, relative_variation=1e-3, residual_tolerance=0.0, itermax=500,
+                 tight_frame=None, l1_proximal_tolerance=1e-2, l1_proximal_itermax=50, l1_proximal_nu=1.0,
+                 positivity_constraint=True, real_constraint=True):
+        """tight_frame None: the closed-form proximal iff the dictionary has one basis."""
         self.phi, self.psi = phi, psi  # kept alive: the solver uses them
         p = _lib.pf_padmm_params()
         _lib.check(_lib.lib().pf_padmm_params_init(C.byref(p)))
@@ -40,6 +44,9 @@ class ImagingProximalADMM:
         p.relative_variation = float(relative_variation)
         p.residual_tolerance = float(residual_tolerance)
         p.itermax = int(itermax)
+        p.tight_frame = -1 if tight_frame is None else int(bool(tight_frame))
+        p.l1_tolerance, p.l1_itermax, p.l1_nu = float(l1_proximal_tolerance), int(l1_proximal_itermax), float(l1_proximal_nu)
+        p.positivity, p.real_constraint = int(bool(positivity_constraint)), int(bool(real_constraint))
         self.itermax = int(itermax)
         self._h = C.c_void_p()
         if hasattr(y, "is_cuda") and y.is_cuda:
@@ -121,16 +128,15 @@ def padmm_factory(dist, measurements, wavelets, uv_data, sigma, imsizey, imsizex
     if sara_size > 1 and tight_frame:
         raise RuntimeError("l1 proximal not consistent: You say you are using a tight frame, but you have more than "
                            "one wavelet basis.")
-    if not tight_frame:
-        raise NotImplementedError("the device PADMM implements the tight-frame l1 proximal only (one wavelet basis, "
-                                  "tight_frame=True)")
     if dist != algo_distribution.serial:
         raise RuntimeError("Type of distributed proximal ADMM algorithm not recognised. You might not have compiled "
                            "with MPI.")
     phi = measurements.op if isinstance(measurements, LinearTransform) else measurements
     eps = utilities.calculate_l2_radius(uv_data.vis.size, sigma)
     return ImagingProximalADMM(phi, wavelets, uv_data.vis, eps, 0.0, 1.0, 0.9, relative_variation, eps,
-                               max_iterations)
+                               max_iterations, tight_frame=tight_frame, l1_proximal_tolerance=l1_proximal_tolerance,
+                               l1_proximal_itermax=maximum_proximal_iterations, l1_proximal_nu=1.0,
+                               positivity_constraint=positive_constraint, real_constraint=real_constraint)
 
 
 class MultiChannelPADMM:

@@ -106,13 +106,16 @@ def test_factory_and_errors():
     sol = algorithms.padmm_factory("serial", lt, W, uvd, sigma, n, n, 1, 20, tight_frame=True)
     r = sol.iterate(5, use_graph=False)
     assert r.niters == 5 and r.gamma > 0
-    with pytest.raises(NotImplementedError):
-        algorithms.padmm_factory("serial", lt, W, uvd, sigma, n, n, 1, 20, tight_frame=False)
+    it = algorithms.padmm_factory("serial", lt, W, uvd, sigma, n, n, 1, 20, tight_frame=False)
+    r = it.iterate(2, use_graph=False)
+    assert r.niters == 2 and r.l1_iters_total >= 2  # the iterative proximal, also for one basis
     with pytest.raises(RuntimeError, match="tight frame"):
         algorithms.padmm_factory("serial", lt, W, uvd, sigma, n, n, 9, 20, tight_frame=True)
     S = wavelets.WaveletOperator([("Dirac", 1), ("DB4", 2)], n, n, "c128")
     with pytest.raises(PurifyError, match="tight frame"):
-        algorithms.ImagingProximalADMM(lt.op, S, y, 1.0)
+        algorithms.ImagingProximalADMM(lt.op, S, y, 1.0, tight_frame=True)
+    with pytest.raises(PurifyError, match="l1_itermax"):
+        algorithms.ImagingProximalADMM(lt.op, S, y, 1.0, l1_proximal_itermax=0)
     with pytest.raises(PurifyError, match="epsilon"):
         algorithms.ImagingProximalADMM(lt.op, W, y, 0.0)
     with pytest.raises(PurifyError, match="more iterations"):
@@ -142,3 +145,53 @@ def test_multichannel_batch_equals_single_channel_runs():
         assert np.array_equal(sol.image(), x)
         one.close()
     mc.close()
+
+
+SARA = [("Dirac", 1), ("DB1", 2), ("DB4", 2)]
+
+
+def sara_solver(u, v, y, n, eps, precision, norm, iters, **kw):
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", 4, 4,
+                                                     precision=precision)
+    lt.op.set_norm(norm)
+    W = wavelets.WaveletOperator(SARA, n, n, precision)
+    return algorithms.ImagingProximalADMM(lt.op, W, y, eps, itermax=iters, **kw), lt, W
+
+
+def _sara_oracle(orc, y, eps, K, **kw):
+    ents = [(wavelets.parse_wavelet(a), b) for a, b in SARA]
+    return oracle.padmm(orc, ents, y, eps, K, tight_frame=False, **kw)
+
+
+def test_sara_iterates_match_the_oracle_with_a_fixed_number_of_proximal_steps():
+    n, M, K = 32, 2500, 6
+    u, v, x0, y, eps, orc = problem(651, n, M)
+    xo, st, gam, _ = _sara_oracle(orc, y, eps, K, l1_itermax=5, l1_tol=0.0)
+    sol, lt, W = sara_solver(u, v, y, n, eps, "c128", orc.norm, K, l1_proximal_itermax=5,
+                             l1_proximal_tolerance=0.0)
+    r = sol.iterate(K, use_graph=True)  # ignored: the iterative proximal runs eagerly
+    assert r.niters == K and r.l1_iters_last == 5 and r.l1_iters_total == 5 * K
+    assert r.gamma == pytest.approx(gam, rel=1e-12)
+    x = sol.image()
+    assert rel(x, xo) < 1e-9
+    assert np.allclose(sol.stats(), st, rtol=1e-8, atol=0)
+    assert np.all(x.real >= 0) and np.all(x.imag == 0)  # positivity and real constraints
+
+
+def test_sara_iterates_match_the_oracle_with_the_default_stopping_rule():
+    n, M, K = 32, 2500, 8
+    u, v, x0, y, eps, orc = problem(652, n, M)
+    xo, st, gam, _ = _sara_oracle(orc, y, eps, K)
+    sol, lt, W = sara_solver(u, v, y, n, eps, "c128", orc.norm, K)
+    r = sol.iterate(K, use_graph=False)
+    assert r.niters == K and 1 <= r.l1_iters_last <= 50
+    assert rel(sol.image(), xo) < 1e-8
+
+
+def test_sara_complex64_reconstruction_snr_within_0p05_db_of_the_oracle():
+    n, M, K = 64, 6000, 25
+    u, v, x0, y, eps, orc = problem(653, n, M)
+    xo, _, _, _ = _sara_oracle(orc, y, eps, K)
+    sol, lt, W = sara_solver(u, v, y, n, eps, "c64", orc.norm, K)
+    sol.iterate(K)
+    assert abs(oracle.snr_db(x0, sol.image()) - oracle.snr_db(x0, xo)) < 0.05

@@ -47,3 +47,76 @@ def test_reconstruction_improves_on_the_dirty_image_and_converges():
     dirty = dirty * np.vdot(dirty, x0).real / np.vdot(dirty, dirty).real  # best scaling of the dirty image
     assert oracle.snr_db(x0, x) > oracle.snr_db(x0, dirty)
     assert it < 300 and np.sqrt(st[-1, 2]) < eps
+
+
+SARA3 = [(0, 1), (1, 2), (4, 2)]  # Dirac, DB1, DB4
+
+
+def _ops(n, ents):
+    return (lambda v: oracle.sara_analysis(v, n, n, ents)), (lambda c: oracle.sara_synthesis(c, n, n, ents))
+
+
+def test_iterative_proximal_without_constraints_reaches_the_tight_frame_closed_form():
+    # one orthonormal basis, no constraints: the minimiser is Psi soft(Psi^dagger x0, gamma)
+    n, ents = 16, [(4, 2)]
+    psit, psi = _ops(n, ents)
+    rng = np.random.default_rng(11)
+    x0 = rng.normal(size=n * n) + 1j * rng.normal(size=n * n)
+    g = 0.4
+    x, k = oracle.l1_prox_constrained(x0, g, psit, psi, 1.0, 500, 0.0, positivity=False, real=False)
+    assert np.allclose(x, psi(oracle.soft_threshold(psit(x0), g)), atol=1e-10)
+
+
+def _objective(x, x0, g, psit):
+    return 0.5 * np.vdot(x - x0, x - x0).real + g * np.abs(psit(x)).sum()
+
+
+def test_iterative_proximal_matches_an_independent_admm_solver_on_sara_with_constraints():
+    # the objective is strongly convex, so its minimiser is unique: a plain ADMM on the split
+    # min 1/2||x - x0||^2 + i_C(x) + gamma ||w||_1 s.t. w = Psi^dagger x (Psi Psi^dagger = I for SARA)
+    # must reach the same point
+    n = 8
+    psit, psi = _ops(n, SARA3)
+    rng = np.random.default_rng(12)
+    x0 = rng.normal(size=n * n) + 0.3 * rng.normal(size=n * n) * 1j
+    g = 0.15
+    x, k = oracle.l1_prox_constrained(x0, g, psit, psi, 1.0, 20000, 0.0)
+    rho = 1.0
+    w = psit(oracle.project_c(x0))
+    lam = np.zeros_like(w)
+    xa = oracle.project_c(x0)
+    for _ in range(20000):
+        # x-step: argmin 1/2||x - x0||^2 + i_C + rho/2 ||Psi^dagger x - w + lam||^2; with Psi Psi^dagger = I
+        # this is P_C((x0 + rho Psi(w - lam)) / (1 + rho))
+        xa = oracle.project_c((x0 + rho * psi(w - lam)) / (1 + rho))
+        cw = psit(xa) + lam
+        w = oracle.soft_threshold(cw, g / rho)
+        lam = lam + psit(xa) - w
+    assert np.allclose(xa.imag, 0) and np.all(xa.real >= 0)
+    assert np.allclose(x, xa, atol=1e-6)
+    # and it is no worse than nearby feasible points
+    f0 = _objective(x, x0, g, psit)
+    for _ in range(20):
+        p = oracle.project_c(x + 1e-3 * rng.normal(size=n * n))
+        assert _objective(p, x0, g, psit) >= f0 - 1e-12
+
+
+def test_iterative_proximal_stops_on_the_objective_change():
+    n = 16
+    psit, psi = _ops(n, SARA3)
+    rng = np.random.default_rng(13)
+    x0 = rng.normal(size=n * n)
+    _, k_loose = oracle.l1_prox_constrained(x0, 0.2, psit, psi, 1.0, 50, 1e-2)
+    _, k_tight = oracle.l1_prox_constrained(x0, 0.2, psit, psi, 1.0, 50, 1e-8)
+    _, k_cap = oracle.l1_prox_constrained(x0, 0.2, psit, psi, 1.0, 3, 0.0)
+    assert 1 <= k_loose < k_tight and k_cap == 3
+
+
+def test_sara_padmm_reconstruction_improves_on_the_dirty_image():
+    op, x0, y, sigma = small(5, 32, 2000)
+    eps = utilities.calculate_l2_radius(y.size, sigma)
+    x, st, g, it = oracle.padmm(op, [(0, 1), (2, 2), (4, 2)], y, eps, 60, tight_frame=False)
+    dirty = op.adjoint(y)
+    dirty = dirty * np.vdot(dirty, x0).real / np.vdot(dirty, dirty).real
+    assert oracle.snr_db(x0, x) > oracle.snr_db(x0, dirty)
+    assert np.allclose(x.imag, 0) and np.all(x.real >= 0)
