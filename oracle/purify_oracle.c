@@ -679,7 +679,9 @@ int orc_wprojection_matrix_dense(int64_t rows, const int64_t *rowptr, const int3
 /* Utilities                                                                                   */
 /* ------------------------------------------------------------------------------------------ */
 
-/* utilities.cc:208-278 with operators.h:405-410 */
+/* operators.h:405-410: lambda -> set_cell_size (utilities.cc:208-250) -> radians -> uv_scale
+ * (utilities.cc:258-278) -> pixels; radians -> uv_scale only (set_cell_size's own radians branch,
+ * utilities.cc:240-244, is not on this path) */
 void orc_uv_to_pixels(const double *u, const double *v, int64_t M, int units, double cell_x, double cell_y,
                       int nx, int ny, double os, double *u_px, double *v_px) {
   double cu = cell_x, cv = cell_y;
@@ -701,14 +703,11 @@ void orc_uv_to_pixels(const double *u, const double *v, int64_t M, int units, do
   if (units == 0) {
     su = 180 * 3600 / cu / PI;
     sv = 180 * 3600 / cv / PI;
-  } else if (units == 1) {
-    su = 180 * 3600 / PI;
-    sv = 180 * 3600 / PI;
   }
   const int Nu = (int)floor(os * nx), Nv = (int)floor(os * ny);
   for (int64_t m = 0; m < M; ++m) {
     double ur = u[m], vr = v[m];
-    if (units == 0 || units == 1) {
+    if (units == 0) {
       ur = u[m] / su * 2 * PI;
       vr = v[m] / sv * 2 * PI;
     }

@@ -575,14 +575,16 @@ void upload_lists(pf_op* op, const HostLists& h, int TU, int TV, TileLists& tl) 
   tl.slot = slot;
 }
 
-// set_cell_size + uv_scale (cpp/purify/utilities.cc:208-278, operators.h:405-410)
+// operators.h:405-410: lambda -> set_cell_size (utilities.cc:208-250) -> radians -> uv_scale
+// (utilities.cc:258-278) -> pixels; radians -> uv_scale only (set_cell_size's radians branch is not on
+// this path)
 void to_pixels(const pf_op_desc& d, int Nu, int Nv, std::vector<double>& u, std::vector<double>& v) {
   const int64_t M = d.M;
   u.assign(d.u, d.u + M);
   v.assign(d.v, d.v + M);
   if (d.units == PF_PIXELS) return;
   double cu = d.cellx, cv = d.celly;
-  double su = 180 * 3600 / PI, sv = 180 * 3600 / PI;
+  double su = 1, sv = 1;  // lambda only
   if (d.units == PF_LAMBDA) {
     if (cu == 0 && cv == 0) {
       double mu = 0, mv = 0;
@@ -597,8 +599,9 @@ void to_pixels(const pf_op_desc& d, int Nu, int Nv, std::vector<double>& u, std:
     su = 180 * 3600 / cu / PI;
     sv = 180 * 3600 / cv / PI;
   }
+  const bool lambda = d.units == PF_LAMBDA;
   for (int64_t m = 0; m < M; ++m) {
-    const double ur = u[m] / su * 2 * PI, vr = v[m] / sv * 2 * PI;
+    const double ur = lambda ? u[m] / su * 2 * PI : u[m], vr = lambda ? v[m] / sv * 2 * PI : v[m];
     u[m] = ur / (2 * PI) * Nu;
     v[m] = vr / (2 * PI) * Nv;
   }

@@ -9,12 +9,12 @@ from .operators import (GridDegridOperator, LinearTransform, MeasurementOperator
                         distributed_measurement_operator, measurement_operator_factory, measurementoperator,
                         operators)
 from .utilities import vis_params, vis_units
-from . import algorithms, utilities, wavelets, wproj_utilities
+from . import algorithms, pfitsio, utilities, wavelets, wproj_utilities
 
 __all__ = ["kernel", "kernel_from_string", "LinearTransform", "MeasurementOperator", "measurementoperator",
            "operators", "GridDegridOperator", "PsfOperator",
            "measurement_operator_factory", "distributed_measurement_operator", "vis_params", "vis_units",
-           "utilities", "fft2d", "kernel_eval", "version"]
+           "utilities", "pfitsio", "fft2d", "kernel_eval", "version"]
 
 
 def version():

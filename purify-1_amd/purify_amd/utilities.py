@@ -96,3 +96,28 @@ def calculate_l2_radius(y_size, sigma=0.0, n_sigma=2.0, distribution="chi"):
 def add_noise(y0, sigma, seed):
     """utilities.cc:434-458 with a seeded generator: re, im ~ N(0, sigma/sqrt(2))."""
     return np.asarray(y0) + synth_cnormal(seed, np.asarray(y0).size, sigma)
+
+
+def read_visibility(vis_name, w_term=False):
+    """utilities.cc:140-183: a whitespace-separated text file of rows u v [w] re(V) im(V) sigma (nan and inf
+    allowed).  Returns vis_params with v negated (the reference's reflection for the image orientation),
+    weights 1 / sigma, units lambda (set ``units`` as the caller's data need)."""
+    cols = 6 if w_term else 5
+    with open(vis_name) as f:
+        vals = np.array([float(t) for t in f.read().split()], np.float64)
+    if vals.size % cols:
+        raise RuntimeError(f"{vis_name}: {vals.size} numbers do not make rows of {cols}")
+    a = vals.reshape(-1, cols)
+    u, v = a[:, 0], a[:, 1]
+    w = a[:, 2] if w_term else np.zeros(a.shape[0])
+    re, im, sig = a[:, cols - 3], a[:, cols - 2], a[:, cols - 1]
+    return vis_params(u=u, v=-v, w=w, vis=re + 1j * im, weights=(1.0 / sig).astype(np.complex128))
+
+
+def write_visibility(uv_vis, file_name, w_term=False):
+    """utilities.cc:185-202: rows u -v [w] re(V) im(V) 1/re(weight), 13 significant digits."""
+    with open(file_name, "w") as f:
+        for i in range(uv_vis.u.size):
+            parts = [uv_vis.u[i], -uv_vis.v[i]] + ([uv_vis.w[i]] if w_term else [])
+            parts += [uv_vis.vis[i].real, uv_vis.vis[i].imag, 1.0 / uv_vis.weights[i].real]
+            f.write(" ".join(f"{x:.13g}" for x in parts) + "\n")

@@ -3,7 +3,7 @@
 Run once in the development container where the reference tree is mounted:
     python tests/golden/import_reference_fixtures.py /root/reference
 The files are the known-answer data that lao19881213/purify-1's own Catch2 tests read
-(cpp/tests/kernels.cc, operators.cc, measurement_operator.cc).  They are GPL-2 data from that
+(cpp/tests/kernels.cc, operators.cc, measurement_operator.cc, algo_factory.cc).  They are GPL-2 data from that
 project; PROVENANCE.md in the output folder records where each came from.  The numeric vectors
 that the reference's wproj_utils test (cpp/tests/wproj_utils.cc:217-289) holds inline are written
 to convolution_known_answers.json.
@@ -19,6 +19,9 @@ FILES = {
     "operators": ["u_data", "v_data", "direct_input_data", "expected_direct_data", "indirect_input_data",
                   "expected_indirect_data", "expected_S_data"],
     "measurement_operator": ["u", "v", "kb_exact_data", "pswf_exact_data", "gauss_exact_data"],
+    # the PADMM factory fixture (cpp/tests/algo_factory.cc:20-67): input visibilities, SOPT's solution
+    # and residual images and the run's parameters
+    "padmm": ["input_data.vis", "solution.fits", "residual.fits", "test_parameters"],
 }
 
 # Inline data of cpp/tests/wproj_utils.cc:236-244 (4x4) and 271-282 (5x5): G row, chirp row and
@@ -74,7 +77,8 @@ def main(ref_root):
                  "`cpp/tests/wproj_utils.cc:236-244,271-282`")
     lines += ["", "Readers: `tests/golden/fixtures.py`.  The reference tests that consume them are "
               "`cpp/tests/kernels.cc:35-73`, `cpp/tests/operators.cc:65-101`, "
-              "`cpp/tests/measurement_operator.cc:14-84` and `cpp/tests/wproj_utils.cc:217-289`."]
+              "`cpp/tests/measurement_operator.cc:14-84`, `cpp/tests/algo_factory.cc:20-67` and "
+              "`cpp/tests/wproj_utils.cc:217-289`."]
     with open(os.path.join(out, "PROVENANCE.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
 

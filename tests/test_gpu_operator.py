@@ -254,6 +254,20 @@ def test_vis_params_overload_units():
     assert rel(lt * x, orc.forward(x)) < 1e-12
 
 
+def test_vis_params_overload_radians_units_take_uv_scale_only():
+    # operators.h:408-410: radians -> uv_scale (utilities.cc:258-278) -> pixels, u / (2 pi) * floor(os n)
+    n, M = 64, 1500
+    u_rad, v_rad = utilities.synth_uv(63, M, 0.8)
+    uvr = vis_params(u_rad, v_rad, units=vis_units.radians)
+    lt = measurementoperator.init_degrid_operator_2d(uvr, n, n, 1.0, 1.0, 2.0, 0, 1e-4, precision="c128")
+    orc = oracle.Operator(utilities.radians_to_pixels(u_rad, n), utilities.radians_to_pixels(v_rad, n), None, None,
+                          n, n, 2.0, oracle.KB, 4, 4)
+    x = utilities.synth_cnormal(64, n * n)
+    assert rel(lt * x, orc.forward(x)) < 1e-12
+    up, vp = oracle.uv_to_pixels(u_rad, v_rad, 1, 1.0, 1.0, n, n, 2.0)
+    assert np.allclose(up, u_rad / (2 * np.pi) * 2 * n) and np.allclose(vp, v_rad / (2 * np.pi) * 2 * n)
+
+
 def test_construction_errors_carry_reference_messages():
     u = np.zeros(4)
     with pytest.raises(PurifyError, match="PSWF requires a support of 6"):

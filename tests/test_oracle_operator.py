@@ -200,6 +200,18 @@ def test_uv_units(cell):
     assert isapprox(u, up * scale, 1e-6)
 
 
+def test_uv_units_radians_take_uv_scale_only():
+    # operators.h:408-410: radians go through uv_scale alone, u / (2 pi) * floor(os n), whatever the cell
+    n, os_ = 100, 2.0
+    u = np.array([-np.pi, -1.0, 0.0, 0.5, np.pi])
+    for cell in (1.0, 7.0):
+        up, vp = oracle.uv_to_pixels(u, -u, 1, cell, cell, n, n, os_)
+        assert isapprox(up, u / (2 * np.pi) * 200, 1e-15) and isapprox(vp, -u / (2 * np.pi) * 200, 1e-15)
+    # pixels pass through unchanged
+    up, _ = oracle.uv_to_pixels(u, u, 2, 1.0, 1.0, n, n, os_)
+    assert np.array_equal(up, u)
+
+
 def test_l2_radius_matches_chi_distribution():
     # utilities.cc:396-421 uses an asymptotic series for the chi mean; compare to the exact
     # chi(2M) distribution's mean + 2 sd.
