@@ -183,6 +183,33 @@ int pf_padmm_run(pf_padmm *h, int32_t iters, int32_t use_graph, int32_t check_ev
 int pf_padmm_enqueue(pf_padmm *h, int32_t iters, int32_t use_graph);
 int pf_padmm_sync(pf_padmm *h, pf_padmm_result *res);
 int pf_padmm_reset(pf_padmm *h);
+
+/* ---- Forward-backward on the device (sopt::algorithm::ImagingForwardBackward as
+ * cpp/example/fb_random_coverage.cc:89-106 configures it: PhiTPhi the compact operator, the target the
+ * dirty image Phi^dagger y) ----
+ * x~ = x - (beta / sigma^2)(Phi^dagger Phi x - Phi^dagger y); x = prox_(mu beta ||Psi^dagger .||_1)(x~) with the
+ * l1 proximal of the PADMM above (closed form for one basis, else iterative and constrained); from
+ * x = 0; converged when ||x - x_prev|| / ||x|| < relative_variation.  Restated from the published
+ * forward-backward splitting (SOPT is not in the reference tree): agreement with SOPT is unpinned.  The
+ * handle is a pf_padmm: pf_padmm_run / enqueue / sync / reset / get / stats / destroy drive it;
+ * PF_PADMM_RESIDUAL returns the image-domain residual Phi^dagger Phi x_prev - Phi^dagger y (imsizey*imsizex),
+ * and the third statistic is its squared norm. */
+typedef struct pf_fb_params {
+  double mu;                    /* <= 0: 1e-5 imsizey imsizex (fb_random_coverage.cc:92) */
+  double beta;                  /* <= 0: 0.5 / (imsizey imsizex) (fb_random_coverage.cc:93) */
+  double sigma;                 /* noise standard deviation, > 0 */
+  double relative_variation;    /* 1e-3 */
+  int32_t itermax;              /* 500 */
+  int32_t tight_frame;          /* as pf_padmm_params; the example passes 0 with SARA */
+  int32_t l1_itermax;           /* 50 */
+  double l1_tolerance;          /* 1e-2 */
+  double l1_nu;                 /* 1 */
+  int32_t positivity;           /* 1 */
+  int32_t real_constraint;      /* 1 */
+} pf_fb_params;
+int pf_fb_params_init(pf_fb_params *p);
+int pf_fb_create(pf_op *phi, pf_wavelet *psi, const void *y, int y_loc, const pf_fb_params *params,
+                 pf_padmm **out);
 /* what: PF_PADMM_IMAGE (x, imsizey*imsizex) or PF_PADMM_RESIDUAL (r = Phi x - y, M) */
 int pf_padmm_get(pf_padmm *h, int32_t what, void *out, int out_loc);
 /* per-iteration FP64 statistics (||x - x_prev||^2, ||x||^2, ||r||^2) of up to cap iterations */

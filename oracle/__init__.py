@@ -607,6 +607,39 @@ def padmm(op: "Operator", entries, y, epsilon, iters, gamma=None, nu=1.0, rho=0.
     return x, np.array(stats), gamma, it
 
 
+def forward_backward(op: "Operator", entries, y, iters, sigma, mu=None, beta=None, tight_frame=True,
+                     l1_itermax=50, l1_tol=1e-2, l1_nu=1.0, positivity=True, real=True):
+    """Forward-backward splitting (Combettes & Wajs 2005) for min ||y - Phi x||^2 / (2 sigma^2) +
+    mu ||Psi^dagger x||_1 [+ i_C], in the PhiTPhi form cpp/example/fb_random_coverage.cc:89-106 sets up
+    (target the dirty image Phi^dagger y), FP64:
+        x~ = x - (beta / sigma^2)(Phi^dagger Phi x - Phi^dagger y);  x = prox_(mu beta ||Psi^dagger .||_1)(x~)
+    from x = 0; the proximal is the tight-frame closed form or l1_prox_constrained.  mu / beta default to
+    the example's 1e-5 N and 0.5 / N.  SOPT (not in the reference tree) is not followed line by line:
+    parity with its iterates is unpinned.  Returns (x, stats): stats[k] = (||x_k - x_(k-1)||^2,
+    ||x_k||^2, ||Phi^dagger Phi x_(k-1) - Phi^dagger y||^2)."""
+    ny, nx = op.ny, op.nx
+    N = ny * nx
+    mu = 1e-5 * N if mu is None or mu <= 0 else mu
+    beta = 0.5 / N if beta is None or beta <= 0 else beta
+    step, gam = beta / sigma ** 2, mu * beta
+    psit = lambda v: sara_analysis(v, ny, nx, entries)  # noqa: E731
+    psi = lambda c: sara_synthesis(c, ny, nx, entries)  # noqa: E731
+    dimage = op.adjoint(np.asarray(y, np.complex128))
+    x = np.zeros(N, np.complex128)
+    stats = []
+    for _ in range(iters):
+        res = op.adjoint(op.forward(x)) - dimage
+        xu = x - step * res
+        if tight_frame:
+            c = psit(xu)
+            xn = xu + psi(soft_threshold(c, gam * l1_nu) - c) / l1_nu
+        else:
+            xn, _ = l1_prox_constrained(xu, gam, psit, psi, l1_nu, l1_itermax, l1_tol, positivity, real)
+        stats.append((float(np.vdot(xn - x, xn - x).real), float(np.vdot(xn, xn).real), float(np.vdot(res, res).real)))
+        x = xn
+    return x, np.array(stats)
+
+
 def snr_db(model, solution):
     """20 log10(||model|| / ||model - solution||)."""
     model, solution = np.asarray(model), np.asarray(solution)

@@ -314,6 +314,9 @@ template <typename R>
 void op_forward(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s);
 template <typename R>
 void op_adjoint(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s);
+// out = Phi^dagger Phi x (both normalised), without the two permutations
+template <typename R>
+void op_normal(pf_op* op, const cpx<R>* x, cpx<R>* out, hipStream_t s);
 struct OpShape {
   int ny, nx, prec;
   int64_t M;

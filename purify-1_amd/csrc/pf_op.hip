@@ -1246,6 +1246,12 @@ template <typename R>
 void op_adjoint(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
   adjoint_dev<R>(op, y, x, s);
 }
+template <typename R>
+void op_normal(pf_op* op, const cpx<R>* x, cpx<R>* out, hipStream_t s) {
+  normal_dev<R>(op, x, out, s);
+}
+template void op_normal<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void op_normal<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
 template void op_forward<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
 template void op_forward<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
 template void op_adjoint<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);

@@ -25,6 +25,13 @@
 // objective 1/2 ||x_k - x~||^2 + gamma ||Psi^dagger x_k||_1 changes by less than l1_tolerance relative:
 // that test reads two FP64 sums on the host, so the iterative mode runs eagerly (no graph).
 //
+// The same handle type also runs SOPT's ImagingForwardBackward as cpp/example/fb_random_coverage.cc:
+// 89-106 configures it (PhiTPhi the compact operator, the target the dirty image Phi^dagger y), restated
+// from the published forward-backward splitting (Combettes & Wajs 2005):
+//   x~ = x - (beta / sigma^2) (Phi^dagger Phi x - Phi^dagger y),   x = prox_(mu beta ||Psi^dagger .||_1)(x~)
+// from x = 0, with either l1 proximal above; convergence on ||x - x_prev|| / ||x|| alone (the example's
+// residual tolerance is 0), the image-domain residual Phi^dagger Phi x - Phi^dagger y recorded beside it.
+//
 // Vectors stay in the operator's precision; every norm is reduced in FP64 on the device in a fixed
 // order, and the projection reads its norm from device memory, so an iteration has no host
 // synchronisation and is captured once into a hipGraph that is replayed.  Per-iteration FP64
@@ -250,6 +257,9 @@ struct pf_padmm {
   double* part = nullptr;  // reduction partials
   double* cur = nullptr;   // [0] ||x - xprev||^2, [1] ||x||^2, [2] ||r||^2, [3] ||v||^2, [4] ||x - x~||^2, [5] ||Psi^dagger x||_1
   bool tight = true;       // closed-form tight-frame proximal, else the iterative one
+  bool fb = false;         // forward-backward instead of PADMM
+  double fb_step = 0;      // beta / sigma^2
+  void* dimage = nullptr;  // forward-backward target Phi^dagger y (N)
   void* u = nullptr;       // the iterative proximal's dual variable (NC coefficients)
   int l1_last = 0;         // inner iterations of the last iterative proximal
   int64_t l1_total = 0;
@@ -286,18 +296,18 @@ cpx<R>* V(void* p) {
 
 // x = prox(xu) by the iterative constrained proximal (eager: the stopping test reads the objective)
 template <typename R>
-void l1_prox_iterative(pf_padmm* h) {
+void l1_prox_iterative(pf_padmm* h, double gamma) {
   hipStream_t s = h->stream;
   const int64_t N = h->N, NC = h->NC;
   const bool pos = h->par.positivity != 0, real = h->par.real_constraint != 0;
-  const R inv_nu = (R)(1.0 / h->par.l1_nu), gam = (R)h->gamma;
+  const R inv_nu = (R)(1.0 / h->par.l1_nu), gam = (R)gamma;
   auto objective = [&]() {
     sq_norm<R>(V<R>(h->x), V<R>(h->xu), N, h->part, h->cur + 4, s);
     abs_sum<R>(V<R>(h->c), NC, h->part, h->cur + 5, s);
     double o[2];
     PF_HIP(hipMemcpyAsync(o, h->cur + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     PF_HIP(hipStreamSynchronize(s));
-    return 0.5 * o[0] + h->gamma * o[1];
+    return 0.5 * o[0] + gamma * o[1];
   };
   hipLaunchKernelGGL((k_zero<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->u), NC);
   hipLaunchKernelGGL((k_l1_primal<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), (const cpx<R>*)nullptr, V<R>(h->x),
@@ -322,9 +332,19 @@ void l1_prox_iterative(pf_padmm* h) {
   PF_HIP(hipGetLastError());
 }
 
-// one PADMM iteration on h->stream (no host synchronisation with the tight-frame proximal)
+template <typename R>
+void fb_iteration(pf_padmm* h);
+template <typename R>
+void padmm_step(pf_padmm* h);
+// one iteration of the handle's solver on h->stream (no host synchronisation with the tight-frame
+// proximal)
 template <typename R>
 void padmm_iteration(pf_padmm* h) {
+  if (h->fb) fb_iteration<R>(h);
+  else padmm_step<R>(h);
+}
+template <typename R>
+void padmm_step(pf_padmm* h) {
   hipStream_t s = h->stream;
   const int64_t N = h->N, M = h->M, NC = h->NC;
   const R inv_nu = (R)(1.0 / h->par.nu);
@@ -342,7 +362,7 @@ void padmm_iteration(pf_padmm* h) {
     wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
     hipLaunchKernelGGL((k_prox_out<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x), inv_nu, N);
   } else {
-    l1_prox_iterative<R>(h);
+    l1_prox_iterative<R>(h, h->gamma);
   }
   op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
   hipLaunchKernelGGL((k_residual_dual<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->z), V<R>(h->r),
@@ -352,6 +372,56 @@ void padmm_iteration(pf_padmm* h) {
   sq_norm<R>(V<R>(h->r), nullptr, M, h->part, h->cur + 2, s);
   hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, h->cur, h->stats, h->counter, h->cap);
   PF_HIP(hipGetLastError());
+}
+
+// x~ = x - step (g - dimage) (g = Phi^dagger Phi x); xprev = x
+template <typename R>
+__global__ void k_fb_grad(const cpx<R>* __restrict__ x, const cpx<R>* __restrict__ g, const cpx<R>* __restrict__ dimg,
+                          cpx<R>* __restrict__ xprev, cpx<R>* __restrict__ xu, cpx<R>* __restrict__ res, R step, int64_t n) {
+  GRID_STRIDE(k, n) {
+    const cpx<R> a = x[k], b = g[k], c = dimg[k];
+    const cpx<R> r = mk<R>(b.x - c.x, b.y - c.y);
+    res[k] = r;
+    xprev[k] = a;
+    xu[k] = mk<R>(a.x - step * r.x, a.y - step * r.y);
+  }
+}
+
+// one forward-backward iteration; stats: ||x - x_prev||^2, ||x||^2 and the squared norm of the image-domain
+// residual Phi^dagger Phi x_prev - Phi^dagger y the step used
+template <typename R>
+void fb_iteration(pf_padmm* h) {
+  hipStream_t s = h->stream;
+  const int64_t N = h->N, NC = h->NC;
+  const double gam = h->gamma;  // mu beta
+  op_normal<R>(h->phi, V<R>(h->x), V<R>(h->g), s);
+  hipLaunchKernelGGL((k_fb_grad<R>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->dimage), V<R>(h->xprev),
+                     V<R>(h->xu), V<R>(h->v), (R)h->fb_step, N);
+  if (h->tight) {
+    wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
+    hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(gam * h->par.l1_nu), NC);
+    wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
+    hipLaunchKernelGGL((k_prox_out<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x),
+                       (R)(1.0 / h->par.l1_nu), N);
+  } else {
+    l1_prox_iterative<R>(h, gam);
+  }
+  sq_norm<R>(V<R>(h->x), V<R>(h->xprev), N, h->part, h->cur + 0, s);
+  sq_norm<R>(V<R>(h->x), nullptr, N, h->part, h->cur + 1, s);
+  sq_norm<R>(V<R>(h->v), nullptr, N, h->part, h->cur + 2, s);
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, h->cur, h->stats, h->counter, h->cap);
+  PF_HIP(hipGetLastError());
+}
+
+template <typename R>
+void fb_init(pf_padmm* h) {
+  hipStream_t s = h->stream;
+  hipLaunchKernelGGL((k_zero<R>), vgrid(h->N), dim3(256), 0, s, V<R>(h->x), h->N);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, h->counter, 0);
+  PF_HIP(hipStreamSynchronize(s));
+  h->iters_done = 0;
+  h->l1_last = 0;
+  h->l1_total = 0;
 }
 
 // x = Phi^dagger y / nu, r = Phi x - y, lambda = 0; gamma from |Psi^dagger Phi^dagger y| if not given
@@ -392,7 +462,8 @@ void padmm_capture(pf_padmm* h) {
       // re-initialise so the captured run starts from the initial state
       padmm_iteration<R>(h);
       PF_HIP(hipStreamSynchronize(s));
-      padmm_init<R>(h);
+      if (h->fb) fb_init<R>(h);
+      else padmm_init<R>(h);
     }
     // capture records the iteration without running it: the state is unchanged
     PF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
@@ -554,6 +625,114 @@ int pf_padmm_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const
   return rc;
 }
 
+int pf_fb_params_init(pf_fb_params* p) {
+  if (!p) return fail(PF_ERR_ARG, "null argument");
+  *p = pf_fb_params{};
+  p->mu = 0;     // 1e-5 N (fb_random_coverage.cc:92)
+  p->beta = 0;   // 0.5 / N (fb_random_coverage.cc:93)
+  p->sigma = 0;  // required
+  p->relative_variation = 1e-3;
+  p->itermax = 500;
+  p->tight_frame = -1;  // fb_random_coverage.cc:98 passes false with SARA
+  p->l1_itermax = 50;
+  p->l1_tolerance = 1e-2;
+  p->l1_nu = 1.0;
+  p->positivity = 1;
+  p->real_constraint = 1;
+  return PF_OK;
+}
+
+int pf_fb_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const pf_fb_params* params, pf_padmm** out) {
+  if (!phi || !psi || !y || !params || !out) return fail(PF_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<pf_padmm> h(new pf_padmm());
+  int rc = guarded([&] {
+    const OpShape os = op_shape(phi);
+    const WaveletShape ws = wavelet_shape(psi);
+    if (os.ny != ws.ny || os.nx != ws.nx) throw Error(PF_ERR_ARG, "operator and dictionary image sizes differ");
+    if (os.prec != ws.prec) throw Error(PF_ERR_ARG, "operator and dictionary precisions differ");
+    if (!(params->sigma > 0)) throw Error(PF_ERR_ARG, "sigma must be positive");
+    if (params->itermax < 1) throw Error(PF_ERR_ARG, "itermax must be positive");
+    if (params->tight_frame > 0 && ws.nwav > 1)
+      throw Error(PF_ERR_ARG,
+                  "l1 proximal not consistent: a tight frame was requested, but the dictionary has more than one basis");
+    h->fb = true;
+    h->phi = phi;
+    h->psi = psi;
+    h->prec = os.prec;
+    h->N = (int64_t)os.ny * os.nx;
+    h->M = os.M;
+    h->NC = h->N * ws.nwav;
+    h->tight = !(params->tight_frame == 0 || (params->tight_frame < 0 && ws.nwav > 1));
+    if (!h->tight) {
+      if (!(params->l1_nu > 0)) throw Error(PF_ERR_ARG, "l1_nu must be positive");
+      if (params->l1_itermax < 1) throw Error(PF_ERR_ARG, "l1_itermax must be positive");
+      if (!(params->l1_tolerance >= 0)) throw Error(PF_ERR_ARG, "l1_tolerance must be >= 0");
+    }
+    const double N = (double)h->N;
+    const double mu = params->mu > 0 ? params->mu : 1e-5 * N;
+    const double beta = params->beta > 0 ? params->beta : 0.5 / N;
+    h->fb_step = beta / (params->sigma * params->sigma);
+    h->gamma = mu * beta;
+    // the shared parameter block: l1 settings, convergence on the relative variation alone
+    h->par = pf_padmm_params{};
+    h->par.nu = 1.0;
+    h->par.relative_variation = params->relative_variation;
+    h->par.residual_tolerance = HUGE_VAL;
+    h->par.itermax = params->itermax;
+    h->par.tight_frame = params->tight_frame;
+    h->par.l1_itermax = params->l1_itermax;
+    h->par.l1_tolerance = params->l1_tolerance;
+    h->par.l1_nu = params->l1_nu > 0 ? params->l1_nu : 1.0;
+    h->par.positivity = params->positivity;
+    h->par.real_constraint = params->real_constraint;
+    const size_t e = os.prec == PF_C64 ? 8 : 16;
+    for (void** p : {&h->x, &h->xprev, &h->xu, &h->g, &h->d, &h->v, &h->dimage}) *p = h->dalloc(e * h->N);
+    h->r = h->v;  // PF_PADMM_RESIDUAL: the image-domain residual
+    h->c = h->dalloc(e * h->NC);
+    if (!h->tight) h->u = h->dalloc(e * h->NC);
+    h->part = (double*)h->dalloc(sizeof(double) * SR_BLOCKS);
+    h->cur = (double*)h->dalloc(sizeof(double) * 6);
+    h->cap = params->itermax;
+    h->stats = (double*)h->dalloc(sizeof(double) * 3 * h->cap);
+    h->counter = (int*)h->dalloc(sizeof(int));
+    PF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    // the target Phi^dagger y
+    void* yd = nullptr;
+    PF_HIP(hipMalloc(&yd, 16 * std::max<int64_t>(h->M, 1)));
+    if (y_loc == PF_HOST) {
+      if (h->prec == PF_C64) {
+        void* tmp = nullptr;
+        PF_HIP(hipMalloc(&tmp, 16 * std::max<int64_t>(h->M, 1)));
+        PF_HIP(hipMemcpyAsync(tmp, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
+        c128_to_c64((const cpx<double>*)tmp, (cpx<float>*)yd, h->M, h->stream);
+        PF_HIP(hipStreamSynchronize(h->stream));
+        PF_HIP(hipFree(tmp));
+      } else {
+        PF_HIP(hipMemcpyAsync(yd, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
+      }
+    } else if (y_loc == PF_DEVICE) {
+      PF_HIP(hipMemcpyAsync(yd, y, e * h->M, hipMemcpyDeviceToDevice, h->stream));
+    } else {
+      PF_HIP(hipFree(yd));
+      throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
+    }
+    if (h->prec == PF_C64) {
+      op_adjoint<float>(phi, (const cpx<float>*)yd, V<float>(h->dimage), h->stream);
+      op_normal<float>(phi, V<float>(h->dimage), V<float>(h->g), h->stream);  // sets up the operator's buffers
+    } else {
+      op_adjoint<double>(phi, (const cpx<double>*)yd, V<double>(h->dimage), h->stream);
+      op_normal<double>(phi, V<double>(h->dimage), V<double>(h->g), h->stream);
+    }
+    PF_HIP(hipStreamSynchronize(h->stream));
+    PF_HIP(hipFree(yd));
+    if (h->prec == PF_C64) fb_init<float>(h.get());
+    else fb_init<double>(h.get());
+  });
+  if (rc == PF_OK) *out = h.release();
+  return rc;
+}
+
 int pf_padmm_run(pf_padmm* h, int32_t iters, int32_t use_graph, int32_t check_every, pf_padmm_result* res) {
   return guarded([&] {
     if (!h) throw Error(PF_ERR_ARG, "null handle");
@@ -583,8 +762,14 @@ int pf_padmm_sync(pf_padmm* h, pf_padmm_result* res) {
 int pf_padmm_reset(pf_padmm* h) {
   return guarded([&] {
     if (!h) throw Error(PF_ERR_ARG, "null handle");
-    if (h->prec == PF_C64) padmm_init<float>(h);
-    else padmm_init<double>(h);
+    if (h->fb) {
+      if (h->prec == PF_C64) fb_init<float>(h);
+      else fb_init<double>(h);
+    } else if (h->prec == PF_C64) {
+      padmm_init<float>(h);
+    } else {
+      padmm_init<double>(h);
+    }
   });
 }
 
@@ -594,7 +779,7 @@ int pf_padmm_get(pf_padmm* h, int32_t what, void* out, int out_loc) {
     PF_HIP(hipStreamSynchronize(h->stream));
     const void* src = what == PF_PADMM_IMAGE ? h->x : what == PF_PADMM_RESIDUAL ? h->r : nullptr;
     if (!src) throw Error(PF_ERR_ARG, "what must be PF_PADMM_IMAGE or PF_PADMM_RESIDUAL");
-    const int64_t n = what == PF_PADMM_IMAGE ? h->N : h->M;
+    const int64_t n = what == PF_PADMM_IMAGE || h->fb ? h->N : h->M;
     const size_t e = h->prec == PF_C64 ? 8 : 16;
     if (out_loc == PF_DEVICE) {
       PF_HIP(hipMemcpyAsync(out, src, e * n, hipMemcpyDeviceToDevice, h->stream));

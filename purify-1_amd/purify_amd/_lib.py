@@ -24,6 +24,13 @@ class pf_padmm_params(C.Structure):
                 ("l1_nu", C.c_double), ("positivity", C.c_int32), ("real_constraint", C.c_int32)]
 
 
+class pf_fb_params(C.Structure):
+    _fields_ = [("mu", C.c_double), ("beta", C.c_double), ("sigma", C.c_double), ("relative_variation", C.c_double),
+                ("itermax", C.c_int32), ("tight_frame", C.c_int32), ("l1_itermax", C.c_int32),
+                ("l1_tolerance", C.c_double), ("l1_nu", C.c_double), ("positivity", C.c_int32),
+                ("real_constraint", C.c_int32)]
+
+
 class pf_padmm_result(C.Structure):
     _fields_ = [("niters", C.c_int32), ("converged", C.c_int32), ("relative_variation", C.c_double),
                 ("residual_norm", C.c_double), ("gamma", C.c_double), ("l1_iters_last", C.c_int32),
@@ -82,6 +89,8 @@ SIGNATURES = {
     "pf_padmm_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "pf_padmm_reset": (C.c_int, [C.c_void_p]),
     "pf_padmm_enqueue": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "pf_fb_params_init": (C.c_int, [C.c_void_p]),
+    "pf_fb_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "pf_padmm_sync": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pf_padmm_get": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int]),
     "pf_padmm_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),

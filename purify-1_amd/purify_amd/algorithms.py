@@ -114,6 +114,44 @@ class ImagingProximalADMM:
                           float(r.relative_variation))
 
 
+class ImagingForwardBackward(ImagingProximalADMM):
+    """sopt::algorithm::ImagingForwardBackward as cpp/example/fb_random_coverage.cc:89-106 configures it, on
+    the device: x~ = x - (beta / sigma^2)(Phi^dagger Phi x - Phi^dagger y), x = prox_(mu beta ||Psi^dagger .||_1)(x~)
+    with the l1 proximal of the PADMM (closed form for one basis, iterative and constrained otherwise),
+    from x = 0; converged on the relative variation alone.  mu <= 0 / beta <= 0 take the example's
+    1e-5 N and 0.5 / N.  ``residual()`` is the image-domain residual Phi^dagger Phi x_prev - Phi^dagger y."""
+
+    def __init__(self, phi: MeasurementOperator, psi: WaveletOperator, y, sigma, mu=0.0, beta=0.0,
+                 relative_variation=1e-3, itermax=500, tight_frame=None, l1_proximal_tolerance=1e-2,
+                 l1_proximal_itermax=50, l1_proximal_nu=1.0, positivity_constraint=True, real_constraint=True):
+        self.phi, self.psi = phi, psi
+        p = _lib.pf_fb_params()
+        _lib.check(_lib.lib().pf_fb_params_init(C.byref(p)))
+        p.mu, p.beta, p.sigma = float(mu), float(beta), float(sigma)
+        p.relative_variation, p.itermax = float(relative_variation), int(itermax)
+        p.tight_frame = -1 if tight_frame is None else int(bool(tight_frame))
+        p.l1_tolerance, p.l1_itermax, p.l1_nu = float(l1_proximal_tolerance), int(l1_proximal_itermax), float(l1_proximal_nu)
+        p.positivity, p.real_constraint = int(bool(positivity_constraint)), int(bool(real_constraint))
+        self.itermax = int(itermax)
+        self._h = C.c_void_p()
+        if hasattr(y, "is_cuda") and y.is_cuda:
+            y = y.contiguous()
+            self._y = y
+            _lib.check(_lib.lib().pf_fb_create(phi.handle, psi.handle, C.c_void_p(y.data_ptr()), _lib.PF_DEVICE,
+                                               C.byref(p), C.byref(self._h)))
+        else:
+            yh = np.ascontiguousarray(np.asarray(y), np.complex128).ravel()
+            if yh.size != phi.M:
+                raise ValueError(f"{yh.size} visibilities for an operator of {phi.M}")
+            _lib.check(_lib.lib().pf_fb_create(phi.handle, psi.handle, yh.ctypes.data, _lib.PF_HOST, C.byref(p),
+                                               C.byref(self._h)))
+
+    def residual(self):
+        out = np.empty(self.phi.imsizey * self.phi.imsizex, np.complex128)
+        _lib.check(_lib.lib().pf_padmm_get(self._h, _lib.PF_PADMM_RESIDUAL, out.ctypes.data, _lib.PF_HOST))
+        return out
+
+
 class algo_distribution:  # noqa: N801 -- algorithm_factory.h:23
     serial = "serial"
     mpi_serial = "mpi_serial"

@@ -223,3 +223,56 @@ def test_sara_padmm_reproduces_the_reference_padmm_fixture(precision):
     assert rel(d.x, sol) < 0.01
     resid = lt.adjoint() * (uv.vis - lt * d.x)
     assert np.linalg.norm(resid.real - res.real) / np.linalg.norm(res.real) < 0.01
+
+
+def fb_solver(u, v, y, n, precision, norm, iters, sigma, dictionary=DB4, **kw):
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", 4, 4,
+                                                     precision=precision)
+    lt.op.set_norm(norm)
+    W = wavelets.WaveletOperator(dictionary, n, n, precision)
+    return algorithms.ImagingForwardBackward(lt.op, W, y, sigma, itermax=iters, **kw), lt, W
+
+
+@pytest.mark.parametrize("dictionary", ["db4", "sara"])
+def test_forward_backward_iterates_match_the_oracle(dictionary):
+    n, M, K = 32, 2500, 12
+    u, v, x0, y, eps, orc = problem(661, n, M)
+    sigma = utilities.SNR_to_standard_deviation(orc.forward(x0), 30.0)
+    d = DB4 if dictionary == "db4" else SARA
+    ents = [(wavelets.parse_wavelet(a), b) for a, b in d]
+    mu, beta = 50.0, sigma ** 2  # step 1 (the normalised Phi^dagger Phi has norm 1)
+    kw = dict(l1_proximal_itermax=4, l1_proximal_tolerance=0.0)
+    xo, st = oracle.forward_backward(orc, ents, y, K, sigma, mu, beta, tight_frame=dictionary == "db4",
+                                     l1_itermax=4, l1_tol=0.0)
+    sol, lt, W = fb_solver(u, v, y, n, "c128", orc.norm, K, sigma, d, mu=mu, beta=beta, **kw)
+    r = sol.iterate(K, use_graph=dictionary == "db4")
+    assert r.niters == K and r.gamma == pytest.approx(mu * beta, rel=1e-14)
+    assert rel(sol.image(), xo) < 1e-9
+    assert np.allclose(sol.stats(), st, rtol=1e-8, atol=1e-300)
+    assert sol.residual().size == n * n
+
+
+def test_forward_backward_graph_replays_equal_eager_and_complex64_snr():
+    n, M, K = 64, 6000, 40
+    u, v, x0, y, eps, orc = problem(662, n, M)
+    sigma = utilities.SNR_to_standard_deviation(orc.forward(x0), 30.0)
+    mu, beta = 20.0, sigma ** 2
+    a, *_ka = fb_solver(u, v, y, n, "c64", orc.norm, K, sigma, mu=mu, beta=beta)
+    b, *_kb = fb_solver(u, v, y, n, "c64", orc.norm, K, sigma, mu=mu, beta=beta)
+    for _ in range(4):
+        a.iterate(K // 4, use_graph=False)
+        b.iterate(K // 4, use_graph=True)
+        assert np.array_equal(a.image(), b.image())
+    ents = [(wavelets.parse_wavelet(x), l) for x, l in DB4]
+    xo, _ = oracle.forward_backward(orc, ents, y, K, sigma, mu, beta)
+    assert abs(oracle.snr_db(x0, a.image()) - oracle.snr_db(x0, xo)) < 0.05
+    assert oracle.snr_db(x0, xo) > 5
+
+
+def test_forward_backward_errors():
+    n, M = 32, 1000
+    u, v, x0, y, eps, orc = problem(663, n, M)
+    with pytest.raises(PurifyError, match="sigma"):
+        fb_solver(u, v, y, n, "c128", orc.norm, 5, 0.0)
+    with pytest.raises(PurifyError, match="tight frame"):
+        fb_solver(u, v, y, n, "c128", orc.norm, 5, 1.0, SARA, tight_frame=True)

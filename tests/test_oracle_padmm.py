@@ -120,3 +120,35 @@ def test_sara_padmm_reconstruction_improves_on_the_dirty_image():
     dirty = dirty * np.vdot(dirty, x0).real / np.vdot(dirty, dirty).real
     assert oracle.snr_db(x0, x) > oracle.snr_db(x0, dirty)
     assert np.allclose(x.imag, 0) and np.all(x.real >= 0)
+
+
+def test_forward_backward_without_regularisation_is_landweber_on_the_dense_matrix():
+    # mu -> 0 with the closed-form proximal: x_(k+1) = x_k - step (A^H A x_k - A^H y), checked against the
+    # dense matrix of the same operator
+    n, M = 8, 40
+    rng = np.random.default_rng(21)
+    u, v = rng.normal(0, 0.3, M) * n, rng.normal(0, 0.3, M) * n
+    op = oracle.Operator(u, v, None, None, n, n, 2.0, oracle.KB, 4, 4, power_iters=50, power_tol=1e-10)
+    A = np.stack([op.forward(np.eye(n * n)[k].astype(np.complex128)) for k in range(n * n)], axis=1)
+    y = rng.normal(size=M) + 1j * rng.normal(size=M)
+    sigma, beta = 0.7, 0.3
+    x, st = oracle.forward_backward(op, [(4, 1)], y, 15, sigma, mu=1e-300, beta=beta)
+    xd = np.zeros(n * n, np.complex128)
+    for _ in range(15):
+        xd = xd - beta / sigma ** 2 * (A.conj().T @ (A @ xd) - A.conj().T @ y)
+    assert np.allclose(x, xd, atol=1e-12)
+
+
+def test_forward_backward_decreases_its_objective_and_reaches_a_fixed_point():
+    op, x0, y, sigma = small(7, 16, 400)
+    ents = [(4, 2)]
+    psit = lambda v: oracle.sara_analysis(v, 16, 16, ents)  # noqa: E731
+    mu, beta = 2.0, sigma ** 2  # step beta / sigma^2 = 1 <= 1 / ||Phi||^2
+    F = lambda x: 0.5 * np.linalg.norm(y - op.forward(x)) ** 2 / sigma ** 2 + mu * np.abs(psit(x)).sum()  # noqa: E731
+    vals = [F(oracle.forward_backward(op, ents, y, k, sigma, mu, beta)[0]) for k in (1, 2, 4, 8, 16)]
+    assert all(b <= a + 1e-9 * abs(a) for a, b in zip(vals, vals[1:]))
+    # the FB map is averaged for step <= 1 / L, so ||x_(k+1) - x_k|| never grows and tends to 0
+    x, st = oracle.forward_backward(op, ents, y, 400, sigma, mu, beta)
+    d = np.sqrt(st[:, 0])
+    assert np.all(d[2:] <= d[1:-1] * (1 + 1e-9))
+    assert d[-1] < 1e-2 * d[1]
