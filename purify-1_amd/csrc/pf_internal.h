@@ -194,7 +194,8 @@ std::vector<float> make_tap_table(int kind, int Jeval, int J, double os);
 // (Chebyshev interpolation at D + 1 nodes, in long double, converted to powers of s; highest degree
 // first for Horner).  Returns an empty vector unless the kernel is smooth on every tap interval (KB,
 // KB-min, Gaussian, PSWF with Jeval = J) and the interpolation error, checked on 257 points per tap,
-// is below 1e-7 of the kernel's peak.
+// is below 1e-7 of the kernel's peak.  The device evaluates taps a < J/2 and mirrors them (the kernels
+// are even: tap J-1-a at s is tap a at -s), so the upper half of the table is not read.
 constexpr int TAP_POLY_D = 9;
 std::vector<float> make_tap_poly(int kind, int Jeval, int J, double os, double* max_rel_err = nullptr);
 
