@@ -93,11 +93,13 @@ enum TapMode { TAPS_EVAL = 0, TAPS_TABLE = 1, TAPS_POLY = 2 };
 // uniform across the wave (scalar loads).  The kernels are even, so tap J-1-a at s is tap a at -s
 // (K(delta + J/2 - 1 - (J-1-a)) = K((1 - delta) + J/2 - 1 - a)): with p_a(s) = E_a(s^2) + s O_a(s^2), the
 // pair is E_a +- s O_a, two Horner chains in s^2 for two taps (11 operations instead of 18 at D = 9).
+// The taps carry (-1)^a only; the rest of the phase, (-1)^(k_u + k_v), is one sign flip of the
+// visibility's value (flip_phase) instead of one per tap.
 // The coefficients are read through the constant address space: uniform scalar loads of memory the
 // kernel never writes, which the compiler may repeat instead of holding all J (D + 1) values live.
 typedef const __attribute__((address_space(4))) float const_float;
 template <int J>
-__device__ inline void poly_taps(const float* __restrict__ c_, int k, float f, float (&w)[J]) {
+__device__ inline void poly_taps(const float* __restrict__ c_, float f, float (&w)[J]) {
   static_assert(TAP_POLY_D % 2 == 1, "the even / odd split assumes an odd degree");
   const const_float* c = (const const_float*)c_;
   const float sv = 2.0f * (f - 0.5f * (float)J) - 1.0f;
@@ -114,12 +116,17 @@ __device__ inline void poly_taps(const float* __restrict__ c_, int k, float f, f
     }
     const float so = sv * o;
     const float pa = e + so;
-    w[a] = ((k + a) & 1) ? -pa : pa;
+    w[a] = (a & 1) ? -pa : pa;  // compile-time sign
     if (J - 1 - a != a) {
       const float pm = e - so;
-      w[J - 1 - a] = ((k + J - 1 - a) & 1) ? -pm : pm;
+      w[J - 1 - a] = ((J - 1 - a) & 1) ? -pm : pm;
     }
   }
+}
+// z (-1)^(k_u + k_v): the sign bits flipped when the parity is odd
+__device__ inline cpx<float> flip_phase(cpx<float> z, int ku, int kv) {
+  const unsigned m = (unsigned)((ku + kv) & 1) << 31;
+  return mk<float>(__uint_as_float(__float_as_uint(z.x) ^ m), __uint_as_float(__float_as_uint(z.y) ^ m));
 }
 
 // Tap weights of both axes: tables where present (both rows are requested before either is used, so
@@ -256,8 +263,8 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
         if (v.perm) dn = (uint32_t)v.perm[nxt];
       }
       if constexpr (POLY) {
-        poly_taps<JU>(g.poly, c.ku, c.fu, wu);
-        poly_taps<JV>(g.poly, c.kv, c.fv, wv);
+        poly_taps<JU>(g.poly, c.fu, wu);
+        poly_taps<JV>(g.poly, c.fv, wv);
       } else {
         taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), table_v<JU, JV>(g), c.ku, c.fu, c.kv, c.fv, Ju, Jv, wu, wv);
       }
@@ -279,6 +286,7 @@ __global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisD
       acc.x += wv[b] * racc.x;
       acc.y += wv[b] * racc.y;
     }
+    if constexpr (POLY) acc = flip_phase(acc, c.ku, c.kv);
     if (v.weights) acc = cmul(acc, v.weights[i]);
     y[dst] = scale(acc, scl);
     c = cn;
@@ -611,8 +619,9 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
         tab_interp<JU>(ta, cur.c.ku, wu);
         tab_interp<JV>(tb, cur.c.kv, wv);
       } else if constexpr (POLY) {
-        poly_taps<JU>(g.poly, cur.c.ku, cur.c.fu, wu);
-        poly_taps<JV>(g.poly, cur.c.kv, cur.c.fv, wv);
+        poly_taps<JU>(g.poly, cur.c.fu, wu);
+        poly_taps<JV>(g.poly, cur.c.fv, wv);
+        s = flip_phase(s, cur.c.ku, cur.c.kv);
       } else {
         taps_both<R, JU, JV>(g, kparams_v<JU, JV>(g), nullptr, cur.c.ku, cur.c.fu, cur.c.kv, cur.c.fv, Ju, Jv, wu,
                              wv);
