@@ -96,12 +96,22 @@ def serving_run(n_requests, seed, log):
     return out
 
 
+def sol_gamma(op, W, y, eps):
+    """The PADMM's gamma for these data (max |Psi^dagger Phi^dagger y| * 1e-3, algorithm_factory.h:70-73)."""
+    from purify_amd import algorithms
+    s = algorithms.ImagingProximalADMM(op, W, y, eps, itermax=1)
+    r = s.sync()
+    s.close()
+    return r.gamma
+
+
 def padmm_run(iters, log, with_cpu=True):
     """PADMM iterations / s with a DB4 (4 levels) tight-frame prior, 30 dB noise, KB, oversample 2:
     the plain-method shape (256^2 image, M = 1e4, J = 4; the oracle's FP64 PADMM on the host beside it,
     and the reconstruction SNR of both against the true image) and the north-star shape (2048^2,
     M = 4e6, J = 6).  Each leg builds the operator (normalised by its power method), then times `iters`
-    iterations replayed from a captured hipGraph and the same number launched eagerly."""
+    iterations replayed from a captured hipGraph and the same number launched eagerly; in complex64 also
+    forward-backward (graph) and, at 256^2, the SARA PADMM (nine bases, iterative proximal, eager)."""
     import numpy as np
     import torch
     from purify_amd import algorithms, measurementoperator, utilities, wavelets
@@ -136,6 +146,35 @@ def padmm_run(iters, log, with_cpu=True):
             leg[f"{prec}_it_per_s"] = rates
             log(f"[bench] padmm {name} {prec}: {rates['graph']:.0f} it/s graph, {rates['eager']:.0f} it/s eager, "
                 f"SNR {leg[prec + '_snr_db']:.2f} dB")
+            if prec == "c64":
+                # forward-backward (PhiTPhi form, DB4, graph replay): lambda = mu sigma^2 equal to the
+                # PADMM's gamma, unit step
+                gam = float(sol_gamma(lt.op, W, y, eps))
+                fb = algorithms.ImagingForwardBackward(lt.op, W, y, sigma, mu=gam / sigma ** 2, beta=sigma ** 2,
+                                                       itermax=iters + 1)
+                fb.iterate(1, use_graph=True)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fb.iterate(iters, use_graph=True)
+                torch.cuda.synchronize()
+                leg["fb_c64_it_per_s"] = iters / (time.perf_counter() - t0)
+                leg["fb_c64_snr_db"] = float(20 * np.log10(np.linalg.norm(x0) / np.linalg.norm(x0 - fb.image())))
+                del fb
+                log(f"[bench] forward-backward {name} c64: {leg['fb_c64_it_per_s']:.0f} it/s graph, "
+                    f"SNR {leg['fb_c64_snr_db']:.2f} dB")
+                if n <= 512:
+                    # SARA (Dirac + DB1..DB8, 3 levels), the iterative constrained proximal (eager)
+                    S = wavelets.WaveletOperator([("Dirac", 3)] + [(f"DB{k}", 3) for k in range(1, 9)], n, n, prec)
+                    sa = algorithms.ImagingProximalADMM(lt.op, S, y, eps, itermax=iters)
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    r = sa.iterate(iters, use_graph=False)
+                    leg["sara_c64_it_per_s"] = iters / (time.perf_counter() - t0)
+                    leg["sara_c64_l1_steps_per_it"] = r.l1_iters_total / iters
+                    leg["sara_c64_snr_db"] = float(20 * np.log10(np.linalg.norm(x0) / np.linalg.norm(x0 - sa.image())))
+                    del sa, S
+                    log(f"[bench] padmm SARA {name} c64: {leg['sara_c64_it_per_s']:.0f} it/s eager, "
+                        f"{leg['sara_c64_l1_steps_per_it']:.1f} proximal steps / it, SNR {leg['sara_c64_snr_db']:.2f} dB")
             del W, lt
             torch.cuda.empty_cache()
         if with_cpu and name == "plain_256":
