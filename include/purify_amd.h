@@ -243,6 +243,19 @@ int pf_op_apply_FZ_adjoint(pf_op *op, const void *grid_dev, void *x_dev, pf_stre
 int pf_op_apply_G(pf_op *op, const void *grid_dev, void *y_dev, pf_stream stream);
 int pf_op_apply_GH(pf_op *op, const void *y_dev, void *grid_dev, pf_stream stream);
 
+/* ---- Super operator over visibility blocks (measurementoperator::init_combine_operators /
+ * init_super_operator, operators.h:513-578): y = [Phi_1 x; Phi_2 x; ...] (blocks stacked in order),
+ * x = sum_b Phi_b^dagger y_b, each Phi_b normalised by its own norm.  The operators must share precision,
+ * image and grid sizes, kernel and supports; F Z S then runs once per apply (the blocks' G_b read one
+ * grid; the G_b^H accumulate into one).  Device pointers in the operators' precision.  The operators must
+ * outlive the handle and are not used by other calls meanwhile (their buffers are shared). ---- */
+typedef struct pf_super pf_super;
+int pf_super_create(pf_op *const *ops, int32_t nblocks, pf_super **out);
+int pf_super_size(const pf_super *sp, int64_t *M);
+int pf_super_forward(pf_super *sp, const void *x_dev, void *y_dev, pf_stream stream);
+int pf_super_adjoint(pf_super *sp, const void *y_dev, void *x_dev, pf_stream stream);
+int pf_super_destroy(pf_super *sp);
+
 /* Unitary 2-D FFT of an Nv x Nu grid (operators.h:209-261): sign -1 is FFTW_FORWARD / sqrt(N),
  * +1 is FFTW_BACKWARD / sqrt(N).  Powers of two from 2 to 16384 per axis.  Device pointers of the
  * given precision; in == out is allowed.  workspace: NULL or a device buffer of Nv*Nu complex. */

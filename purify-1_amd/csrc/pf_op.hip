@@ -1350,6 +1350,119 @@ int pf_op_destroy(pf_op* op) {
   return PF_OK;
 }
 
+}  // extern "C"
+
+// ---- the super operator over visibility blocks (measurementoperator::init_combine_operators,
+// operators.h:513-527): y_b = Phi_b x, x = sum_b Phi_b^dagger y_b.  The blocks share the image, the grid and
+// the correction S (checked at creation), so F Z S runs once per apply: forward, one row and column pass
+// feed every block's G_b; adjoint, every G_b^H (scaled by 1 / ||Phi_b||) accumulates into one grid that
+// one adjoint FFT takes back.  Each block keeps its own norm, as each block's create_operator call
+// normalises it in the reference.
+struct pf_super {
+  std::vector<pf_op*> ops;   // not owned
+  std::vector<int64_t> off;  // first visibility of each block in the stacked vector
+  int64_t M = 0;
+};
+
+namespace pf {
+namespace {
+template <typename R>
+__global__ void k_axpy(const cpx<R>* __restrict__ x, cpx<R>* __restrict__ y, R a, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const cpx<R> u = x[k], v = y[k];
+    y[k] = mk<R>(v.x + a * u.x, v.y + a * u.y);
+  }
+}
+template <typename R>
+void axpy(const cpx<R>* x, cpx<R>* y, double a, int64_t n, hipStream_t s) {
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL((k_axpy<R>), dim3(blocks), dim3(256), 0, s, x, y, (R)a, n);
+}
+template <typename R>
+void super_forward(pf_super* sp, const cpx<R>* x, cpx<R>* y, hipStream_t s) {
+  pf_op* o0 = sp->ops[0];
+  cpx<R>* B = static_cast<cpx<R>*>(o0->d_gridB);
+  fz_forward<R>(o0, x, B, 1.0 / std::sqrt((double)o0->ngrid()), s);
+  for (size_t b = 0; b < sp->ops.size(); ++b) {
+    pf_op* ob = sp->ops[b];
+    if (ob->M == 0) continue;
+    G_dev<R>(ob, B, y + sp->off[b], s);
+    scale_vec<R>(y + sp->off[b], ob->M, 1.0 / ob->norm, s);
+  }
+}
+template <typename R>
+void super_adjoint(pf_super* sp, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
+  pf_op* o0 = sp->ops[0];
+  cpx<R>* A = static_cast<cpx<R>*>(o0->d_gridA);
+  GH_dev<R>(o0, y, A, s);
+  scale_vec<R>(A, o0->ngrid(), 1.0 / o0->norm, s);
+  for (size_t b = 1; b < sp->ops.size(); ++b) {
+    pf_op* ob = sp->ops[b];
+    if (ob->M == 0) continue;
+    cpx<R>* Ab = static_cast<cpx<R>*>(ob->d_gridA);
+    GH_dev<R>(ob, y + sp->off[b], Ab, s);
+    axpy<R>(Ab, A, 1.0 / ob->norm, ob->ngrid(), s);
+  }
+  fz_adjoint<R>(o0, A, x, 1.0 / std::sqrt((double)o0->ngrid()), s);
+}
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+int pf_super_create(pf_op* const* ops, int32_t nblocks, pf_super** out) {
+  if (!ops || !out || nblocks < 1) return fail(PF_ERR_ARG, "need at least one operator");
+  *out = nullptr;
+  return guarded([&] {
+    std::unique_ptr<pf_super> sp(new pf_super());
+    const pf_op* a = ops[0];
+    for (int i = 0; i < nblocks; ++i) {
+      const pf_op* b = ops[i];
+      if (!b) throw Error(PF_ERR_ARG, "null operator");
+      if (b->prec != a->prec || b->ny != a->ny || b->nx != a->nx || b->Nu != a->Nu || b->Nv != a->Nv ||
+          b->kind != a->kind || b->Ju != a->Ju || b->Jv != a->Jv || b->os != a->os)
+        throw Error(PF_ERR_ARG, "the blocks' operators must share precision, image and grid sizes, kernel and supports");
+      sp->ops.push_back(ops[i]);
+      sp->off.push_back(sp->M);
+      sp->M += b->M;
+    }
+    *out = sp.release();
+  });
+}
+
+int pf_super_size(const pf_super* sp, int64_t* M) {
+  if (!sp || !M) return fail(PF_ERR_ARG, "null argument");
+  *M = sp->M;
+  return PF_OK;
+}
+
+int pf_super_forward(pf_super* sp, const void* x_dev, void* y_dev, pf_stream stream) {
+  return guarded([&] {
+    if (!sp || !x_dev || (!y_dev && sp->M)) throw Error(PF_ERR_ARG, "null argument");
+    if (sp->ops[0]->prec == PF_C64)
+      super_forward<float>(sp, (const cpx<float>*)x_dev, (cpx<float>*)y_dev, (hipStream_t)stream);
+    else
+      super_forward<double>(sp, (const cpx<double>*)x_dev, (cpx<double>*)y_dev, (hipStream_t)stream);
+    PF_HIP(hipGetLastError());
+  });
+}
+
+int pf_super_adjoint(pf_super* sp, const void* y_dev, void* x_dev, pf_stream stream) {
+  return guarded([&] {
+    if (!sp || !x_dev || (!y_dev && sp->M)) throw Error(PF_ERR_ARG, "null argument");
+    if (sp->ops[0]->prec == PF_C64)
+      super_adjoint<float>(sp, (const cpx<float>*)y_dev, (cpx<float>*)x_dev, (hipStream_t)stream);
+    else
+      super_adjoint<double>(sp, (const cpx<double>*)y_dev, (cpx<double>*)x_dev, (hipStream_t)stream);
+    PF_HIP(hipGetLastError());
+  });
+}
+
+int pf_super_destroy(pf_super* sp) {
+  delete sp;
+  return PF_OK;
+}
+
 int pf_op_forward(pf_op* op, const void* x, void* y, int x_loc, int y_loc, pf_stream stream) {
   return guarded([&] {
     ProfInstall pi(op);

@@ -90,6 +90,11 @@ SIGNATURES = {
     "pf_padmm_reset": (C.c_int, [C.c_void_p]),
     "pf_padmm_enqueue": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "pf_fb_params_init": (C.c_int, [C.c_void_p]),
+    "pf_super_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pf_super_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "pf_super_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pf_super_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pf_super_destroy": (C.c_int, [C.c_void_p]),
     "pf_fb_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "pf_padmm_sync": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pf_padmm_get": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int]),

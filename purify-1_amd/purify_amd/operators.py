@@ -348,6 +348,135 @@ def init_degrid_operator_2d_from_matrix(G, imsizey, imsizex, oversample_ratio=2.
 measurementoperator.init_degrid_operator_2d_from_matrix = staticmethod(init_degrid_operator_2d_from_matrix)
 
 
+class SuperOperator:
+    """measurementoperator::init_combine_operators (operators.h:513-527) on the device: ``sup * x`` stacks
+    Phi_b x over the blocks, ``sup.adjoint() * y`` is sum_b Phi_b^dagger y_b.  The blocks share one F Z S
+    per apply (pf_super_*).  Inputs: torch CUDA tensors in the operators' precision, or host arrays
+    (copied through the device)."""
+
+    def __init__(self, transforms):
+        self.transforms = list(transforms)  # kept alive: the handle uses their operators
+        ops = [t.op if isinstance(t, LinearTransform) else t for t in self.transforms]
+        arr = (C.c_void_p * len(ops))(*[o.handle.value if isinstance(o.handle, C.c_void_p) else o.handle for o in ops])
+        self._h = C.c_void_p()
+        _lib.check(_lib.lib().pf_super_create(arr, len(ops), C.byref(self._h)))
+        m = C.c_int64()
+        _lib.check(_lib.lib().pf_super_size(self._h, C.byref(m)))
+        self.M = int(m.value)
+        self.segments = []
+        o = 0
+        for op in ops:
+            self.segments.append((o, op.M))
+            o += op.M
+        self.ops = ops
+        self.imsizey, self.imsizex = ops[0].imsizey, ops[0].imsizex
+        self.sizes = (self.M, self.imsizex * self.imsizey)
+
+    def close(self):
+        if self._h:
+            _lib.lib().pf_super_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _run(self, adjoint, x):
+        torch = _torch()
+        dt = self.ops[0]._dtype()
+        nin = self.M if adjoint else self.imsizex * self.imsizey
+        nout = self.imsizex * self.imsizey if adjoint else self.M
+        host = not _is_device_tensor(x)
+        if host:
+            xh = _c128(x).ravel()
+            xd = torch.from_numpy(xh).to(dt).cuda()
+        else:
+            if x.dtype != dt:
+                raise TypeError(f"device input must be {dt} for these operators")
+            xd = x.contiguous()
+        if xd.numel() != nin:
+            raise ValueError(f"input has {xd.numel()} entries, expected {nin}")
+        out = torch.empty(nout, dtype=dt, device=xd.device)
+        s = torch.cuda.current_stream(xd.device).cuda_stream
+        fn = _lib.lib().pf_super_adjoint if adjoint else _lib.lib().pf_super_forward
+        _lib.check(fn(self._h, C.c_void_p(xd.data_ptr()), C.c_void_p(out.data_ptr()), C.c_void_p(s)))
+        if host:
+            return out.to(torch.complex128).cpu().numpy()
+        return out
+
+    def __mul__(self, x):
+        return self._run(False, x)
+
+    __call__ = __mul__
+
+    def adjoint(self):
+        sup = self
+
+        class _A:
+            def __mul__(self, y):
+                return sup._run(True, y)
+
+            __call__ = __mul__
+        return _A()
+
+
+def init_combine_operators(measure_ops, seg=None):
+    """operators.h:513-527.  ``seg`` (the stacked vector's (start, size) per block) is implied by the
+    operators' sizes in block order; given, it must agree."""
+    sup = SuperOperator(measure_ops)
+    if seg is not None and [tuple(map(int, s)) for s in seg] != sup.segments:
+        raise RuntimeError("Numer of groups and measurement operators does not match.")
+    return sup
+
+
+def _segments(total, seg_num):
+    """operators.h:562-576: seg_num blocks of floor(total / seg_num), the last taking the remainder."""
+    if seg_num == 0:
+        raise RuntimeError("Number of segments = 0")
+    size = int(np.floor(total / seg_num))
+    seg, t = [], 0
+    for i in range(seg_num):
+        d = total - t if i == seg_num - 1 else size
+        seg.append((t, d))
+        t += d
+    if t != total:
+        raise RuntimeError("Segments do not sum to the total")
+    return seg
+
+
+def _vis_segment(uv, start, size):
+    from .utilities import vis_params
+    sl = slice(start, start + size)
+    return vis_params(u=uv.u[sl], v=uv.v[sl], w=uv.w[sl], vis=uv.vis[sl], weights=uv.weights[sl], units=uv.units,
+                      ra=uv.ra, dec=uv.dec, average_frequency=uv.average_frequency)
+
+
+def init_super_operator(create_operator, data, *args, **kw):
+    """operators.h:531-578, three overloads: (create_operator, [vis_params, ...], args...),
+    (create_operator, [(start, size), ...], vis_params, args...) and (create_operator, vis_params, seg_num,
+    args...).  create_operator(uv_block, *args, **kw) builds each block's operator (normalised by its own
+    power method, as in the reference)."""
+    from .utilities import vis_params
+    if isinstance(data, vis_params):
+        seg_num, args = int(args[0]), args[1:]
+        blocks = [_vis_segment(data, a, b) for a, b in _segments(data.size(), seg_num)]
+    elif len(data) and isinstance(data[0], vis_params):
+        blocks = list(data)
+    else:
+        uv, args = args[0], args[1:]
+        blocks = [_vis_segment(uv, int(a), int(b)) for a, b in data]
+    ops = [create_operator(b, *args, **kw) for b in blocks]
+    if len(ops) != len(blocks):
+        raise RuntimeError("Numer of groups and measurement operators does not match.")
+    return init_combine_operators(ops)
+
+
+measurementoperator.init_combine_operators = staticmethod(init_combine_operators)
+measurementoperator.init_super_operator = staticmethod(init_super_operator)
+
+
 class GridDegridOperator:
     """operators::init_grid_degrid_operator_2d (compact_operators.h:150-171): the self-adjoint operator
     x -> Phi^dagger Phi x, normalised by the square of the measurement operator's norm.  The reference
