@@ -107,27 +107,104 @@ void abs_sum(const cpx<R>* a, int64_t n, double* part, double* out, hipStream_t 
 inline dim3 vgrid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096))); }
 #define GRID_STRIDE(k, n) for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (n); k += (int64_t)gridDim.x * blockDim.x)
 
-// v = -lambda - r
-template <typename R>
-__global__ void k_ball_in(const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ r, cpx<R>* __restrict__ v, int64_t n) {
-  GRID_STRIDE(k, n) {
-    const cpx<R> a = lam[k], b = r[k];
-    v[k] = mk<R>(-a.x - b.x, -a.y - b.y);
+// ---- fused PADMM vector passes: each also writes its FP64 partial sums (one per block, fixed order);
+// k_fr_final adds them in block order.
+constexpr int FR_BLOCKS = 2048;
+
+template <int NV>
+__device__ inline void reduce_partials(double (&acc)[NV], double* __restrict__ part) {
+  __shared__ double sh[NV][SR_THREADS];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) sh[v][t] = acc[v];
+  __syncthreads();
+  for (int w = SR_THREADS / 2; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sh[v][t] += sh[v][t + w];
+    __syncthreads();
   }
+  if (t == 0)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) part[v * FR_BLOCKS + blockIdx.x] = sh[v][0];
 }
-// z = P_eps(v) (v scaled by min(1, eps / ||v||), ||v||^2 read from *nrm2); w = r + lambda + z
+// out[v] = sum of the FR_BLOCKS partials of quantity v (block v of the launch)
+__global__ void __launch_bounds__(SR_THREADS) k_fr_final(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[SR_THREADS];
+  const double* p = part + blockIdx.x * FR_BLOCKS;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < FR_BLOCKS; k += SR_THREADS) acc += p[k];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = SR_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+#define FR_STRIDE(k, n) for (int64_t k = (int64_t)blockIdx.x * SR_THREADS + threadIdx.x; k < (n); k += (int64_t)FR_BLOCKS * SR_THREADS)
+
+// out = ||lambda + r||^2 (= ||v||^2 for v = -lambda - r)
 template <typename R>
-__global__ void k_ball_out(const cpx<R>* __restrict__ v, const double* __restrict__ nrm2, double eps,
-                           const cpx<R>* __restrict__ r, const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ z,
-                           cpx<R>* __restrict__ w, int64_t n) {
+__global__ void __launch_bounds__(SR_THREADS) k_ball_norm(const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ r,
+                                                          int64_t n, double* part) {
+  double acc[1] = {0.0};
+  FR_STRIDE(k, n) {
+    const cpx<R> a = lam[k], b = r[k];
+    acc[0] += norm2d(mk<R>(-a.x - b.x, -a.y - b.y));
+  }
+  reduce_partials<1>(acc, part);
+}
+// v = -lambda - r; z = P_eps(v) (v scaled by min(1, eps / ||v||), ||v||^2 read from *nrm2); w = r + lambda + z
+template <typename R>
+__global__ void k_ball_out(const double* __restrict__ nrm2, double eps, const cpx<R>* __restrict__ r,
+                           const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ z, cpx<R>* __restrict__ w, int64_t n) {
   const double nv = sqrt(*nrm2);
   const R f = (R)(nv > eps ? eps / nv : 1.0);
   GRID_STRIDE(k, n) {
-    const cpx<R> a = v[k], b = r[k], c = lam[k];
-    const cpx<R> zz = mk<R>(a.x * f, a.y * f);
+    const cpx<R> b = r[k], c = lam[k];
+    const cpx<R> zz = mk<R>((-c.x - b.x) * f, (-c.y - b.y) * f);
     z[k] = zz;
     w[k] = mk<R>(b.x + c.x + zz.x, b.y + c.y + zz.y);
   }
+}
+// x = xu + d / nu (d null: x as it is); out[0] = ||x - xprev||^2, out[1] = ||x||^2
+template <typename R>
+__global__ void __launch_bounds__(SR_THREADS) k_prox_out_norm(const cpx<R>* __restrict__ xu, const cpx<R>* __restrict__ d,
+                                                              cpx<R>* __restrict__ x, const cpx<R>* __restrict__ xprev,
+                                                              R inv_nu, int64_t n, double* part) {
+  double acc[2] = {0.0, 0.0};
+  FR_STRIDE(k, n) {
+    cpx<R> a;
+    if (d) {
+      const cpx<R> u = xu[k], b = d[k];
+      a = mk<R>(u.x + b.x * inv_nu, u.y + b.y * inv_nu);
+      x[k] = a;
+    } else {
+      a = x[k];
+    }
+    const cpx<R> p = xprev[k];
+    acc[0] += norm2d(mk<R>(a.x - p.x, a.y - p.y));
+    acc[1] += norm2d(a);
+  }
+  reduce_partials<2>(acc, part);
+}
+// r = fx - y; lambda += rho (r + z); out = ||r||^2
+template <typename R>
+__global__ void __launch_bounds__(SR_THREADS) k_residual_dual_norm(const cpx<R>* __restrict__ fx, const cpx<R>* __restrict__ y,
+                                                                   const cpx<R>* __restrict__ z, cpx<R>* __restrict__ r,
+                                                                   cpx<R>* __restrict__ lam, R rho, int64_t n,
+                                                                   double* part) {
+  double acc[1] = {0.0};
+  FR_STRIDE(k, n) {
+    const cpx<R> a = fx[k], b = y[k], c = z[k];
+    const cpx<R> rr = mk<R>(a.x - b.x, a.y - b.y);
+    r[k] = rr;
+    const cpx<R> l = lam[k];
+    lam[k] = mk<R>(l.x + rho * (rr.x + c.x), l.y + rho * (rr.y + c.y));
+    acc[0] += norm2d(rr);
+  }
+  reduce_partials<1>(acc, part);
 }
 // xprev = x;  xu = x - g / nu
 template <typename R>
@@ -192,18 +269,6 @@ __global__ void k_l1_dual(cpx<R>* __restrict__ u, const cpx<R>* __restrict__ c, 
     u[k] = mk<R>(w.x * f, w.y * f);
   }
 }
-// r = fx - y; lambda += rho (r + z)
-template <typename R>
-__global__ void k_residual_dual(const cpx<R>* __restrict__ fx, const cpx<R>* __restrict__ y, const cpx<R>* __restrict__ z,
-                                cpx<R>* __restrict__ r, cpx<R>* __restrict__ lam, R rho, int64_t n) {
-  GRID_STRIDE(k, n) {
-    const cpx<R> a = fx[k], b = y[k], c = z[k];
-    const cpx<R> rr = mk<R>(a.x - b.x, a.y - b.y);
-    r[k] = rr;
-    const cpx<R> l = lam[k];
-    lam[k] = mk<R>(l.x + rho * (rr.x + c.x), l.y + rho * (rr.y + c.y));
-  }
-}
 // r = fx - y
 template <typename R>
 __global__ void k_sub(const cpx<R>* __restrict__ a, const cpx<R>* __restrict__ b, cpx<R>* __restrict__ r, int64_t n) {
@@ -255,6 +320,7 @@ struct pf_padmm {
   void *x = nullptr, *xprev = nullptr, *xu = nullptr, *g = nullptr, *c = nullptr, *d = nullptr;
   void *y = nullptr, *r = nullptr, *lam = nullptr, *z = nullptr, *v = nullptr, *fx = nullptr;
   double* part = nullptr;  // reduction partials
+  double* fpart = nullptr;  // fused reductions' partials (2 x FR_BLOCKS)
   double* cur = nullptr;   // [0] ||x - xprev||^2, [1] ||x||^2, [2] ||r||^2, [3] ||v||^2, [4] ||x - x~||^2, [5] ||Psi^dagger x||_1
   bool tight = true;       // closed-form tight-frame proximal, else the iterative one
   bool fb = false;         // forward-backward instead of PADMM
@@ -348,11 +414,11 @@ void padmm_step(pf_padmm* h) {
   hipStream_t s = h->stream;
   const int64_t N = h->N, M = h->M, NC = h->NC;
   const R inv_nu = (R)(1.0 / h->par.nu);
-  hipLaunchKernelGGL((k_ball_in<R>), vgrid(M), dim3(256), 0, s, V<R>(h->lam), V<R>(h->r), V<R>(h->v), M);
-  sq_norm<R>(V<R>(h->v), nullptr, M, h->part, h->cur + 3, s);
+  hipLaunchKernelGGL((k_ball_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->lam), V<R>(h->r), M, h->fpart);
+  hipLaunchKernelGGL(k_fr_final, dim3(1), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 3);
   // fx holds r + lambda + z until the forward apply below overwrites it
-  hipLaunchKernelGGL((k_ball_out<R>), vgrid(M), dim3(256), 0, s, V<R>(h->v), h->cur + 3, h->par.epsilon, V<R>(h->r),
-                     V<R>(h->lam), V<R>(h->z), V<R>(h->fx), M);
+  hipLaunchKernelGGL((k_ball_out<R>), vgrid(M), dim3(256), 0, s, h->cur + 3, h->par.epsilon, V<R>(h->r), V<R>(h->lam),
+                     V<R>(h->z), V<R>(h->fx), M);
   op_adjoint<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
   hipLaunchKernelGGL((k_grad_step<R>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev), V<R>(h->xu),
                      inv_nu, N);
@@ -360,16 +426,18 @@ void padmm_step(pf_padmm* h) {
     wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
     hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.nu), NC);
     wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
-    hipLaunchKernelGGL((k_prox_out<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x), inv_nu, N);
+    hipLaunchKernelGGL((k_prox_out_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x),
+                       V<R>(h->xprev), inv_nu, N, h->fpart);
   } else {
     l1_prox_iterative<R>(h, h->gamma);
+    hipLaunchKernelGGL((k_prox_out_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->xu), (const cpx<R>*)nullptr,
+                       V<R>(h->x), V<R>(h->xprev), inv_nu, N, h->fpart);
   }
+  hipLaunchKernelGGL(k_fr_final, dim3(2), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 0);
   op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
-  hipLaunchKernelGGL((k_residual_dual<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->z), V<R>(h->r),
-                     V<R>(h->lam), (R)h->par.lagrange_update_scale, M);
-  sq_norm<R>(V<R>(h->x), V<R>(h->xprev), N, h->part, h->cur + 0, s);
-  sq_norm<R>(V<R>(h->x), nullptr, N, h->part, h->cur + 1, s);
-  sq_norm<R>(V<R>(h->r), nullptr, M, h->part, h->cur + 2, s);
+  hipLaunchKernelGGL((k_residual_dual_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->fx), V<R>(h->y),
+                     V<R>(h->z), V<R>(h->r), V<R>(h->lam), (R)h->par.lagrange_update_scale, M, h->fpart);
+  hipLaunchKernelGGL(k_fr_final, dim3(1), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 2);
   hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, h->cur, h->stats, h->counter, h->cap);
   PF_HIP(hipGetLastError());
 }
@@ -594,8 +662,9 @@ int pf_padmm_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const
     for (void** p : {&h->x, &h->xprev, &h->xu, &h->g, &h->d}) *p = h->dalloc(e * h->N);
     h->c = h->dalloc(e * h->NC);
     if (!h->tight) h->u = h->dalloc(e * h->NC);
-    for (void** p : {&h->y, &h->r, &h->lam, &h->z, &h->v, &h->fx}) *p = h->dalloc(e * std::max<int64_t>(h->M, 1));
+    for (void** p : {&h->y, &h->r, &h->lam, &h->z, &h->fx}) *p = h->dalloc(e * std::max<int64_t>(h->M, 1));
     h->part = (double*)h->dalloc(sizeof(double) * SR_BLOCKS);
+    h->fpart = (double*)h->dalloc(sizeof(double) * 2 * FR_BLOCKS);
     h->cur = (double*)h->dalloc(sizeof(double) * 6);
     h->cap = params->itermax;
     h->stats = (double*)h->dalloc(sizeof(double) * 3 * h->cap);
