@@ -107,75 +107,86 @@ struct WavFilt {
   int L;
 };
 
-// One analysis pass over the (h, w) block: rows (ROWS) or columns.  Thread t takes output index i of
-// line `line` and writes lo[i] and hi[i].  ld: row stride of both images.
-template <typename R, bool ROWS>
+// One analysis pass over the (h, w) block: rows (ROWS) or columns, for a compile-time filter length L.
+// Rows: block (x, y) takes outputs i = 256 x + t of row y; columns: outputs of row i = y for the columns
+// 256 x + t (consecutive threads, consecutive columns: coalesced).  Thread writes lo[i] and hi[i].
+// ld: row stride of both images.  32-bit indices (an image holds < 2^31 pixels).
+template <typename R, bool ROWS, int L>
 __global__ void __launch_bounds__(256) k_dwt(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int h, int w,
                                              int ld, WavFilt<R> f, R scl) {
   const int n = ROWS ? w : h;  // line length
-  const int nl = ROWS ? h : w; // lines
   const int half = n / 2;
-  const int64_t total = (int64_t)nl * half;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    int line, i;
-    if (ROWS) {
-      line = (int)(t / half);
-      i = (int)(t - (int64_t)line * half);
-    } else {  // consecutive threads take consecutive columns: coalesced
-      i = (int)(t / nl);
-      line = (int)(t - (int64_t)i * nl);
-    }
-    auto at = [&](int k) -> int64_t { return ROWS ? (int64_t)line * ld + k : (int64_t)k * ld + line; };
-    cpx<R> a = mk<R>(R(0), R(0)), b = mk<R>(R(0), R(0));
-    for (int k = 0; k < f.L; ++k) {
-      int p = 2 * i + k;
-      while (p >= n) p -= n;
-      const cpx<R> x = in[at(p)];
-      a.x += f.lo[k] * x.x;
-      a.y += f.lo[k] * x.y;
-      b.x += f.hi[k] * x.x;
-      b.y += f.hi[k] * x.y;
-    }
-    out[at(i)] = scale(a, scl);
-    out[at(half + i)] = scale(b, scl);
+  const int line = ROWS ? (int)blockIdx.y : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = ROWS ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)blockIdx.y;
+  if (ROWS ? i >= half : line >= w) return;
+  auto at = [&](int k) -> int { return ROWS ? line * ld + k : k * ld + line; };
+  cpx<R> a = mk<R>(R(0), R(0)), b = mk<R>(R(0), R(0));
+  int p = 2 * i;
+#pragma unroll
+  for (int k = 0; k < L; ++k, ++p) {
+    int q = p;
+    while (q >= n) q -= n;
+    const cpx<R> x = in[at(q)];
+    a.x += f.lo[k] * x.x;
+    a.y += f.lo[k] * x.y;
+    b.x += f.hi[k] * x.x;
+    b.y += f.hi[k] * x.y;
   }
+  out[at(i)] = scale(a, scl);
+  out[at(half + i)] = scale(b, scl);
 }
 
-// One synthesis pass (the adjoint of k_dwt): x[j] = sum over k with (j - k) even of
-// lo[k] c[(j - k) mod n / 2] + hi[k] c[half + (j - k) mod n / 2].  ACC: add into out.
-template <typename R, bool ROWS, bool ACC>
+// One synthesis pass: output j of each line from its L/2 contributing (low, high) coefficient pairs.
+// Rows: block (x, y) takes outputs j = 256 x + t of row y; columns: output row j = y for the columns
+// 256 x + t.  The taps of output j are k = j mod 2, j mod 2 + 2, ...: both parities' coefficients are
+// compile-time operands and each lane selects its own.
+template <typename R, bool ROWS, bool ACC, int L>
 __global__ void __launch_bounds__(256) k_idwt(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int h, int w,
                                               int ld, WavFilt<R> f, R scl) {
   const int n = ROWS ? w : h;
-  const int nl = ROWS ? h : w;
   const int half = n / 2;
-  const int64_t total = (int64_t)nl * n;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    int line, j;
-    if (ROWS) {
-      line = (int)(t / n);
-      j = (int)(t - (int64_t)line * n);
-    } else {
-      j = (int)(t / nl);
-      line = (int)(t - (int64_t)j * nl);
-    }
-    auto at = [&](int k) -> int64_t { return ROWS ? (int64_t)line * ld + k : (int64_t)k * ld + line; };
-    cpx<R> acc = mk<R>(R(0), R(0));
-    for (int k = j & 1 ? 1 : 0; k < f.L; k += 2) {  // (j - k) even
-      int d = j - k;
-      while (d < 0) d += n;
-      const int i = d >> 1;
-      const cpx<R> cl = in[at(i)], ch = in[at(half + i)];
-      acc.x += f.lo[k] * cl.x + f.hi[k] * ch.x;
-      acc.y += f.lo[k] * cl.y + f.hi[k] * ch.y;
-    }
-    acc = scale(acc, scl);
-    if (ACC) {
-      const cpx<R> o = out[at(j)];
-      acc.x += o.x;
-      acc.y += o.y;
-    }
-    out[at(j)] = acc;
+  const int line = ROWS ? (int)blockIdx.y : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = ROWS ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)blockIdx.y;
+  if (ROWS ? j >= n : line >= w) return;
+  auto at = [&](int k) -> int { return ROWS ? line * ld + k : k * ld + line; };
+  const int par = j & 1;
+  cpx<R> acc = mk<R>(R(0), R(0));
+#pragma unroll
+  for (int kk = 0; kk < L / 2; ++kk) {
+    const int k = 2 * kk + par;  // (j - k) even
+    int d = j - k;
+    while (d < 0) d += n;
+    const int i = d >> 1;
+    const R lo = par ? f.lo[2 * kk + 1] : f.lo[2 * kk];
+    const R hi = par ? f.hi[2 * kk + 1] : f.hi[2 * kk];
+    const cpx<R> cl = in[at(i)], ch = in[at(half + i)];
+    acc.x += lo * cl.x + hi * ch.x;
+    acc.y += lo * cl.y + hi * ch.y;
+  }
+  acc = scale(acc, scl);
+  if (ACC) {
+    const cpx<R> o = out[at(j)];
+    acc.x += o.x;
+    acc.y += o.y;
+  }
+  out[at(j)] = acc;
+}
+
+// Calls F.template operator()<L>() for the runtime filter length L = 2, 4, ..., WAV_MAXL.
+template <class F>
+static void dispatch_len(int L, F&& fn) {
+  switch (L) {
+    case 2: fn.template operator()<2>(); break;
+    case 4: fn.template operator()<4>(); break;
+    case 6: fn.template operator()<6>(); break;
+    case 8: fn.template operator()<8>(); break;
+    case 10: fn.template operator()<10>(); break;
+    case 12: fn.template operator()<12>(); break;
+    case 14: fn.template operator()<14>(); break;
+    case 16: fn.template operator()<16>(); break;
+    case 18: fn.template operator()<18>(); break;
+    case 20: fn.template operator()<20>(); break;
+    default: throw Error(-3, "wavelet filter length must be even and at most " + std::to_string(WAV_MAXL));
   }
 }
 
@@ -207,14 +218,42 @@ WavFilt<R> make_filt(const std::vector<double>& h) {
   return f;
 }
 
+inline unsigned blocks_of(int n) { return (unsigned)std::max(1, (n + 255) / 256); }
+
+template <typename R>
+struct DwtLevel {
+  const cpx<R>* src;
+  cpx<R>* T;
+  cpx<R>* c;
+  int h, w, ld;
+  const WavFilt<R>& f;
+  hipStream_t s;
+  template <int L>
+  void operator()() {
+    hipLaunchKernelGGL((k_dwt<R, true, L>), dim3(blocks_of(w / 2), h), dim3(256), 0, s, src, T, h, w, ld, f, R(1));
+    hipLaunchKernelGGL((k_dwt<R, false, L>), dim3(blocks_of(w), h / 2), dim3(256), 0, s, T, c, h, w, ld, f, R(1));
+  }
+};
+template <typename R, bool ROWS, bool ACC>
+struct IdwtPass {
+  const cpx<R>* in;
+  cpx<R>* out;
+  int h, w, ld;
+  const WavFilt<R>& f;
+  hipStream_t s;
+  template <int L>
+  void operator()() {  // rows: (w outputs, h rows); columns: (w columns, h outputs)
+    hipLaunchKernelGGL((k_idwt<R, ROWS, ACC, L>), dim3(blocks_of(w), h), dim3(256), 0, s, in, out, h, w, ld, f, R(1));
+  }
+};
+
 // Analysis of one entry: image x -> coefficients c (both ny x nx); T: work image.
 template <typename R>
 void dwt2(const cpx<R>* x, cpx<R>* c, cpx<R>* T, int ny, int nx, int levels, const WavFilt<R>& f, hipStream_t s) {
   for (int l = 0; l < levels; ++l) {
     const int h = ny >> l, w = nx >> l;
     const cpx<R>* src = l == 0 ? x : c;
-    hipLaunchKernelGGL((k_dwt<R, true>), grid_for((int64_t)h * (w / 2)), dim3(256), 0, s, src, T, h, w, nx, f, R(1));
-    hipLaunchKernelGGL((k_dwt<R, false>), grid_for((int64_t)w * (h / 2)), dim3(256), 0, s, T, c, h, w, nx, f, R(1));
+    dispatch_len(f.L, DwtLevel<R>{src, T, c, h, w, nx, f, s});
   }
 }
 
@@ -232,12 +271,12 @@ void idwt2(const cpx<R>* c, cpx<R>* x, cpx<R>* W, cpx<R>* T, int ny, int nx, int
   hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, c, W, N, scl);
   for (int l = levels - 1; l >= 0; --l) {
     const int h = ny >> l, w = nx >> l;
-    hipLaunchKernelGGL((k_idwt<R, false, false>), grid_for((int64_t)w * h), dim3(256), 0, s, W, T, h, w, nx, f, R(1));
+    dispatch_len(f.L, IdwtPass<R, false, false>{W, T, h, w, nx, f, s});
     if (l == 0) {
-      if (acc) hipLaunchKernelGGL((k_idwt<R, true, true>), grid_for((int64_t)h * w), dim3(256), 0, s, T, x, h, w, nx, f, R(1));
-      else hipLaunchKernelGGL((k_idwt<R, true, false>), grid_for((int64_t)h * w), dim3(256), 0, s, T, x, h, w, nx, f, R(1));
+      if (acc) dispatch_len(f.L, IdwtPass<R, true, true>{T, x, h, w, nx, f, s});
+      else dispatch_len(f.L, IdwtPass<R, true, false>{T, x, h, w, nx, f, s});
     } else {
-      hipLaunchKernelGGL((k_idwt<R, true, false>), grid_for((int64_t)h * w), dim3(256), 0, s, T, W, h, w, nx, f, R(1));
+      dispatch_len(f.L, IdwtPass<R, true, false>{T, W, h, w, nx, f, s});
     }
   }
 }
