@@ -5,7 +5,7 @@
  * Citations are file:line in lao19881213/purify-1.  Arithmetic is IEEE double; complex
  * numbers use C99 double complex.  OpenMP parallelises the same loops the reference does.
  *
- * Deliberate deviation (documented in DESIGN.md): the gridding phase
+ * Deliberate deviation: the gridding phase
  * exp(-2 pi i ((k_u+ju)/2 + (k_v+jv)/2)) (operators.cc:40,93) is evaluated as the exact
  * parity (-1)^((k_u+ju)+(k_v+jv)) unless literal_phase=1 is requested.
  */

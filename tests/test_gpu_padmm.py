@@ -1,8 +1,9 @@
-"""The device PADMM (tight-frame DB4 prior) against the oracle's FP64 restatement of the same
-iteration: complex128 iterates within 1e-9, complex64 reconstructions within 0.05 dB SNR of the
-oracle's, graph replays identical to eager iterations, convergence stopping,
-and the reference's factory parameters (algorithm_factory.h:57-81).  SOPT's own iterates are not
-available (SOPT is not in the reference tree): parity with SOPT is unpinned."""
+"""The device PADMM and forward-backward against the oracle's FP64 restatements of the same iterations
+(tight-frame DB4 and SARA proximals): complex128 iterates within 1e-9, complex64 reconstructions within
+0.05 dB SNR of the oracle's, graph replays identical to eager iterations, convergence stopping, the
+reference's factory parameters (algorithm_factory.h:57-81), batched channels, and the reference's own
+PADMM fixture end to end (cpp/tests/algo_factory.cc).  SOPT is not in the reference tree: its iterates
+are not available one by one, and the fixture holds the measured distance from its solution."""
 import numpy as np
 import pytest
 import torch
