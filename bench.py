@@ -471,7 +471,7 @@ def main():
         "vs_baseline": None,
         "dtype": "c64",
         "data": "synthetic",
-        "config": {"workload": f"phi_pair_{args.workload}", "image": n, "grid": Nu, "oversample": os_, "kernel": "kb",
+        "config": {"workload": args.workload, "image": n, "grid": Nu, "oversample": os_, "kernel": "kb",
                    "J": J, "M": M, "w_term": w_term, "uv_sigma_rad": round(np.pi / 3, 6), "seed": seed,
                    **({"Jw": Jw, "w_max_lambda": wp["wmax"], "cell_arcsec": cell} if wp else {})},
         "roofline": roofline,
