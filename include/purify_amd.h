@@ -278,7 +278,7 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * the default, 2 = in gridding and degridding; 0 = evaluate the kernel for every tap); "fft_columns"
  * (0 = the column FFT in one pass up to 8192 rows, the default; 1 = two passes of N1 x N2 beyond 512
  * rows); "fft_row_blocks" (row FFT grid: that many resident blocks per compute unit, each looping over
- * lines with the next line prefetched; 0 = one line per block; default 4 for complex64, 2 for
+ * lines with the next line prefetched; 0 = one line per block; default 3 for complex64, 2 for
  * complex128); "grid_waves" (matrix-core gridding with that many waves, each taking every
  * grid_waves-th 16 x 16 tile with its batches pipelined across tiles; 0 = one wave per tile, the
  * default); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond

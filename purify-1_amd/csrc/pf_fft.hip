@@ -242,11 +242,11 @@ constexpr int row_threads() {
 // Lines are taken with a stride of gridDim.x * lpb: a grid smaller than the line count makes each
 // block loop, and the next line's first-stage loads are issued before the current line's stages, so
 // the HBM reads of one line overlap the LDS work of the previous one.
-// Waves per SIMD the row kernel is compiled for: the LDS of a 4096-point line allows 4 blocks of 4
-// waves per CU in complex64 and 2 in complex128, so registers are budgeted for that occupancy (the
-// current and the prefetched line in registers).
+// Waves per SIMD the row kernel is compiled for: 3 blocks of 4 waves per CU in complex64 (at 4 the
+// current and prefetched lines and S's column factors spill a few registers; 3 measured 1-1.5 us faster
+// per 4096 x 4096 pass) and 2 in complex128.
 template <typename R>
-constexpr int row_waves_per_simd() { return sizeof(R) == 4 ? 4 : 2; }
+constexpr int row_waves_per_simd() { return sizeof(R) == 4 ? 3 : 2; }
 
 template <typename R, int L, bool INV, int MODE>
 __global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>()) k_fft_rows(RowArgs<R> a) {
@@ -673,9 +673,9 @@ void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
     PF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     p.n_cu = ncu > 0 ? ncu : 256;
   }
-  // row pass: a grid of 4 (complex64) or 2 (complex128) resident blocks per CU, each looping over lines
-  // with the next line prefetched; at 4096 x 4096 this takes the row passes from 43 to 38 us (c64)
-  p.row_bpc = prec ? 2 : 4;
+  // row pass: a grid of 3 (complex64) or 2 (complex128) resident blocks per CU (row_waves_per_simd), each
+  // looping over lines with the next line prefetched
+  p.row_bpc = prec ? 2 : 3;
   const size_t esz = prec ? 16 : 8;
   // single pass while N * C * esz <= 64 KiB with C >= 16 (fp32) / 8 (fp64) columns
   const int Cmin = prec ? 8 : 16;

@@ -320,7 +320,7 @@ def test_row_pass_grid_sizes_give_identical_results(precision):
     u, v = make_uv(91, M, n)
     lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", 4, 4,
                                                      precision=precision)
-    assert lt.op.get_tuning("fft_row_blocks") == (4 if precision == "c64" else 2)
+    assert lt.op.get_tuning("fft_row_blocks") == (3 if precision == "c64" else 2)
     x = utilities.synth_image(92, n, n, 20)
     y = utilities.synth_cnormal(93, M)
     ref = (lt * x, lt.adjoint() * y)
