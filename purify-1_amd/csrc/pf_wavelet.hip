@@ -136,40 +136,40 @@ __global__ void __launch_bounds__(256) k_dwt(const cpx<R>* __restrict__ in, cpx<
   out[at(half + i)] = scale(b, scl);
 }
 
-// One synthesis pass: output j of each line from its L/2 contributing (low, high) coefficient pairs.
-// Rows: block (x, y) takes outputs j = 256 x + t of row y; columns: output row j = y for the columns
-// 256 x + t.  The taps of output j are k = j mod 2, j mod 2 + 2, ...: both parities' coefficients are
-// compile-time operands and each lane selects its own.
+// One synthesis pass: outputs 2m and 2m + 1 of each line, which read the same L/2 (low, high) coefficient
+// pairs i = m, m - 1, ... (mod n/2): output 2m takes the even taps, 2m + 1 the odd ones, in increasing
+// tap order.  Rows: block (x, y) takes m = 256 x + t of row y; columns: m = y for the columns 256 x + t.
 template <typename R, bool ROWS, bool ACC, int L>
 __global__ void __launch_bounds__(256) k_idwt(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int h, int w,
                                               int ld, WavFilt<R> f, R scl) {
   const int n = ROWS ? w : h;
   const int half = n / 2;
   const int line = ROWS ? (int)blockIdx.y : (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = ROWS ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)blockIdx.y;
-  if (ROWS ? j >= n : line >= w) return;
+  const int m = ROWS ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)blockIdx.y;
+  if (ROWS ? m >= half : line >= w) return;
   auto at = [&](int k) -> int { return ROWS ? line * ld + k : k * ld + line; };
-  const int par = j & 1;
-  cpx<R> acc = mk<R>(R(0), R(0));
+  cpx<R> ae = mk<R>(R(0), R(0)), ao = mk<R>(R(0), R(0));
 #pragma unroll
   for (int kk = 0; kk < L / 2; ++kk) {
-    const int k = 2 * kk + par;  // (j - k) even
-    int d = j - k;
-    while (d < 0) d += n;
-    const int i = d >> 1;
-    const R lo = par ? f.lo[2 * kk + 1] : f.lo[2 * kk];
-    const R hi = par ? f.hi[2 * kk + 1] : f.hi[2 * kk];
+    int i = m - kk;
+    while (i < 0) i += half;
     const cpx<R> cl = in[at(i)], ch = in[at(half + i)];
-    acc.x += lo * cl.x + hi * ch.x;
-    acc.y += lo * cl.y + hi * ch.y;
+    ae.x += f.lo[2 * kk] * cl.x + f.hi[2 * kk] * ch.x;
+    ae.y += f.lo[2 * kk] * cl.y + f.hi[2 * kk] * ch.y;
+    ao.x += f.lo[2 * kk + 1] * cl.x + f.hi[2 * kk + 1] * ch.x;
+    ao.y += f.lo[2 * kk + 1] * cl.y + f.hi[2 * kk + 1] * ch.y;
   }
-  acc = scale(acc, scl);
+  ae = scale(ae, scl);
+  ao = scale(ao, scl);
   if (ACC) {
-    const cpx<R> o = out[at(j)];
-    acc.x += o.x;
-    acc.y += o.y;
+    const cpx<R> oe = out[at(2 * m)], oo = out[at(2 * m + 1)];
+    ae.x += oe.x;
+    ae.y += oe.y;
+    ao.x += oo.x;
+    ao.y += oo.y;
   }
-  out[at(j)] = acc;
+  out[at(2 * m)] = ae;
+  out[at(2 * m + 1)] = ao;
 }
 
 // Calls F.template operator()<L>() for the runtime filter length L = 2, 4, ..., WAV_MAXL.
@@ -242,8 +242,11 @@ struct IdwtPass {
   const WavFilt<R>& f;
   hipStream_t s;
   template <int L>
-  void operator()() {  // rows: (w outputs, h rows); columns: (w columns, h outputs)
-    hipLaunchKernelGGL((k_idwt<R, ROWS, ACC, L>), dim3(blocks_of(w), h), dim3(256), 0, s, in, out, h, w, ld, f, R(1));
+  void operator()() {  // rows: (w / 2 pairs, h rows); columns: (w columns, h / 2 pairs)
+    if (ROWS)
+      hipLaunchKernelGGL((k_idwt<R, true, ACC, L>), dim3(blocks_of(w / 2), h), dim3(256), 0, s, in, out, h, w, ld, f, R(1));
+    else
+      hipLaunchKernelGGL((k_idwt<R, false, ACC, L>), dim3(blocks_of(w), h / 2), dim3(256), 0, s, in, out, h, w, ld, f, R(1));
   }
 };
 
