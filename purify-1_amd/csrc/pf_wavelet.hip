@@ -172,6 +172,35 @@ __global__ void __launch_bounds__(256) k_idwt(const cpx<R>* __restrict__ in, cpx
   out[at(2 * m + 1)] = ao;
 }
 
+// The column synthesis pass of a level read straight from the coefficients: the approximation quadrant
+// (columns < w/2 of the top half) from apx (the previous, coarser level's output; null at the coarsest
+// level, where it is in c too) and everything else from c, scaled by cs on the way in.  This replaces a
+// scaled copy of all coefficients into the work image; the values are the same as the copy's.
+template <typename R, int L>
+__global__ void __launch_bounds__(256) k_idwt_cols_src(const cpx<R>* __restrict__ c, const cpx<R>* __restrict__ apx,
+                                                       cpx<R>* __restrict__ out, int h, int w, int ld, WavFilt<R> f,
+                                                       R cs) {
+  const int half = h / 2;
+  const int line = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int m = (int)blockIdx.y;
+  if (line >= w) return;
+  const cpx<R>* lo_src = apx && line < w / 2 ? apx : c;
+  const R lo_s = lo_src == c ? cs : R(1);
+  cpx<R> ae = mk<R>(R(0), R(0)), ao = mk<R>(R(0), R(0));
+#pragma unroll
+  for (int kk = 0; kk < L / 2; ++kk) {
+    int i = m - kk;
+    while (i < 0) i += half;
+    const cpx<R> cl = scale(lo_src[i * ld + line], lo_s), ch = scale(c[(half + i) * ld + line], cs);
+    ae.x += f.lo[2 * kk] * cl.x + f.hi[2 * kk] * ch.x;
+    ae.y += f.lo[2 * kk] * cl.y + f.hi[2 * kk] * ch.y;
+    ao.x += f.lo[2 * kk + 1] * cl.x + f.hi[2 * kk + 1] * ch.x;
+    ao.y += f.lo[2 * kk + 1] * cl.y + f.hi[2 * kk + 1] * ch.y;
+  }
+  out[(2 * m) * ld + line] = ae;
+  out[(2 * m + 1) * ld + line] = ao;
+}
+
 // Calls F.template operator()<L>() for the runtime filter length L = 2, 4, ..., WAV_MAXL.
 template <class F>
 static void dispatch_len(int L, F&& fn) {
@@ -250,6 +279,21 @@ struct IdwtPass {
   }
 };
 
+template <typename R>
+struct IdwtColsSrc {
+  const cpx<R>* c;
+  const cpx<R>* apx;
+  cpx<R>* out;
+  int h, w, ld;
+  const WavFilt<R>& f;
+  R cs;
+  hipStream_t s;
+  template <int L>
+  void operator()() {
+    hipLaunchKernelGGL((k_idwt_cols_src<R, L>), dim3(blocks_of(w), h / 2), dim3(256), 0, s, c, apx, out, h, w, ld, f, cs);
+  }
+};
+
 // Analysis of one entry: image x -> coefficients c (both ny x nx); T: work image.
 template <typename R>
 void dwt2(const cpx<R>* x, cpx<R>* c, cpx<R>* T, int ny, int nx, int levels, const WavFilt<R>& f, hipStream_t s) {
@@ -270,11 +314,11 @@ void idwt2(const cpx<R>* c, cpx<R>* x, cpx<R>* W, cpx<R>* T, int ny, int nx, int
     else hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, c, x, N, scl);
     return;
   }
-  // W holds the coefficients being reconstructed in place, with the SARA factor applied on the way in
-  hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, c, W, N, scl);
+  // W holds each level's reconstructed approximation; the column passes read it and the coefficients
+  // (with the SARA factor applied on the way in)
   for (int l = levels - 1; l >= 0; --l) {
     const int h = ny >> l, w = nx >> l;
-    dispatch_len(f.L, IdwtPass<R, false, false>{W, T, h, w, nx, f, s});
+    dispatch_len(f.L, IdwtColsSrc<R>{c, l == levels - 1 ? nullptr : W, T, h, w, nx, f, scl, s});
     if (l == 0) {
       if (acc) dispatch_len(f.L, IdwtPass<R, true, true>{T, x, h, w, nx, f, s});
       else dispatch_len(f.L, IdwtPass<R, true, false>{T, x, h, w, nx, f, s});
