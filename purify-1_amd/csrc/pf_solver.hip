@@ -168,13 +168,15 @@ __global__ void k_ball_out(const double* __restrict__ nrm2, double eps, const cp
     w[k] = mk<R>(b.x + c.x + zz.x, b.y + c.y + zz.y);
   }
 }
-// x = xu + d / nu (d null: x as it is); out[0] = ||x - xprev||^2, out[1] = ||x||^2
+// x = xu + d / nu (d null: x as it is); out[0] = ||x - xprev||^2, out[1] = ||x||^2.  xprev may be x itself
+// (its old value is read before the new one is written), so x and xprev are not restrict.
 template <typename R>
 __global__ void __launch_bounds__(SR_THREADS) k_prox_out_norm(const cpx<R>* __restrict__ xu, const cpx<R>* __restrict__ d,
-                                                              cpx<R>* __restrict__ x, const cpx<R>* __restrict__ xprev,
-                                                              R inv_nu, int64_t n, double* part) {
+                                                              cpx<R>* x, const cpx<R>* xprev, R inv_nu, int64_t n,
+                                                              double* part) {
   double acc[2] = {0.0, 0.0};
   FR_STRIDE(k, n) {
+    const cpx<R> p = xprev[k];
     cpx<R> a;
     if (d) {
       const cpx<R> u = xu[k], b = d[k];
@@ -183,36 +185,38 @@ __global__ void __launch_bounds__(SR_THREADS) k_prox_out_norm(const cpx<R>* __re
     } else {
       a = x[k];
     }
-    const cpx<R> p = xprev[k];
     acc[0] += norm2d(mk<R>(a.x - p.x, a.y - p.y));
     acc[1] += norm2d(a);
   }
   reduce_partials<2>(acc, part);
 }
-// r = fx - y; lambda += rho (r + z); out = ||r||^2
+// r = fx - y; lambda += rho (r + z); out[0] = ||r||^2, out[1] = ||lambda + r||^2 (the next iteration's
+// ball norm, ||v||^2 for v = -lambda - r)
 template <typename R>
 __global__ void __launch_bounds__(SR_THREADS) k_residual_dual_norm(const cpx<R>* __restrict__ fx, const cpx<R>* __restrict__ y,
                                                                    const cpx<R>* __restrict__ z, cpx<R>* __restrict__ r,
                                                                    cpx<R>* __restrict__ lam, R rho, int64_t n,
                                                                    double* part) {
-  double acc[1] = {0.0};
+  double acc[2] = {0.0, 0.0};
   FR_STRIDE(k, n) {
     const cpx<R> a = fx[k], b = y[k], c = z[k];
     const cpx<R> rr = mk<R>(a.x - b.x, a.y - b.y);
     r[k] = rr;
     const cpx<R> l = lam[k];
-    lam[k] = mk<R>(l.x + rho * (rr.x + c.x), l.y + rho * (rr.y + c.y));
+    const cpx<R> ln = mk<R>(l.x + rho * (rr.x + c.x), l.y + rho * (rr.y + c.y));
+    lam[k] = ln;
     acc[0] += norm2d(rr);
+    acc[1] += norm2d(mk<R>(-ln.x - rr.x, -ln.y - rr.y));
   }
-  reduce_partials<1>(acc, part);
+  reduce_partials<2>(acc, part);
 }
-// xprev = x;  xu = x - g / nu
-template <typename R>
+// xu = x - g / nu (COPY: and xprev = x)
+template <typename R, bool COPY>
 __global__ void k_grad_step(cpx<R>* __restrict__ x, const cpx<R>* __restrict__ g, cpx<R>* __restrict__ xprev,
                             cpx<R>* __restrict__ xu, R inv_nu, int64_t n) {
   GRID_STRIDE(k, n) {
     const cpx<R> a = x[k], b = g[k];
-    xprev[k] = a;
+    if (COPY) xprev[k] = a;
     xu[k] = mk<R>(a.x - b.x * inv_nu, a.y - b.y * inv_nu);
   }
 }
@@ -414,21 +418,23 @@ void padmm_step(pf_padmm* h) {
   hipStream_t s = h->stream;
   const int64_t N = h->N, M = h->M, NC = h->NC;
   const R inv_nu = (R)(1.0 / h->par.nu);
-  hipLaunchKernelGGL((k_ball_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->lam), V<R>(h->r), M, h->fpart);
-  hipLaunchKernelGGL(k_fr_final, dim3(1), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 3);
+  // cur[3] = ||lambda + r||^2 from the previous iteration's residual pass (or the initialisation)
   // fx holds r + lambda + z until the forward apply below overwrites it
   hipLaunchKernelGGL((k_ball_out<R>), vgrid(M), dim3(256), 0, s, h->cur + 3, h->par.epsilon, V<R>(h->r), V<R>(h->lam),
                      V<R>(h->z), V<R>(h->fx), M);
   op_adjoint<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
-  hipLaunchKernelGGL((k_grad_step<R>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev), V<R>(h->xu),
-                     inv_nu, N);
   if (h->tight) {
+    // x keeps its old value until the proximal output replaces it, so no copy of x_prev is made
+    hipLaunchKernelGGL((k_grad_step<R, false>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev),
+                       V<R>(h->xu), inv_nu, N);
     wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
     hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.nu), NC);
     wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
     hipLaunchKernelGGL((k_prox_out_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x),
-                       V<R>(h->xprev), inv_nu, N, h->fpart);
+                       V<R>(h->x), inv_nu, N, h->fpart);
   } else {
+    hipLaunchKernelGGL((k_grad_step<R, true>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev),
+                       V<R>(h->xu), inv_nu, N);
     l1_prox_iterative<R>(h, h->gamma);
     hipLaunchKernelGGL((k_prox_out_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->xu), (const cpx<R>*)nullptr,
                        V<R>(h->x), V<R>(h->xprev), inv_nu, N, h->fpart);
@@ -437,7 +443,7 @@ void padmm_step(pf_padmm* h) {
   op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
   hipLaunchKernelGGL((k_residual_dual_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->fx), V<R>(h->y),
                      V<R>(h->z), V<R>(h->r), V<R>(h->lam), (R)h->par.lagrange_update_scale, M, h->fpart);
-  hipLaunchKernelGGL(k_fr_final, dim3(1), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 2);
+  hipLaunchKernelGGL(k_fr_final, dim3(2), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 2);
   hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, h->cur, h->stats, h->counter, h->cap);
   PF_HIP(hipGetLastError());
 }
@@ -514,6 +520,8 @@ void padmm_init(pf_padmm* h) {
   op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
   hipLaunchKernelGGL((k_sub<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->r), M);
   hipLaunchKernelGGL((k_zero<R>), vgrid(M), dim3(256), 0, s, V<R>(h->lam), M);
+  hipLaunchKernelGGL((k_ball_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->lam), V<R>(h->r), M, h->fpart);
+  hipLaunchKernelGGL(k_fr_final, dim3(1), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 3);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, h->counter, 0);
   PF_HIP(hipStreamSynchronize(s));
   h->iters_done = 0;
