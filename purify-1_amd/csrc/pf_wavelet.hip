@@ -107,33 +107,71 @@ struct WavFilt {
   int L;
 };
 
-// One analysis pass over the (h, w) block: rows (ROWS) or columns, for a compile-time filter length L.
-// Rows: block (x, y) takes outputs i = 256 x + t of row y; columns: outputs of row i = y for the columns
-// 256 x + t (consecutive threads, consecutive columns: coalesced).  Thread writes lo[i] and hi[i].
-// ld: row stride of both images.  32-bit indices (an image holds < 2^31 pixels).
-template <typename R, bool ROWS, int L>
-__global__ void __launch_bounds__(256) k_dwt(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int h, int w,
-                                             int ld, WavFilt<R> f, R scl) {
-  const int n = ROWS ? w : h;  // line length
-  const int half = n / 2;
-  const int line = ROWS ? (int)blockIdx.y : (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int i = ROWS ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)blockIdx.y;
-  if (ROWS ? i >= half : line >= w) return;
-  auto at = [&](int k) -> int { return ROWS ? line * ld + k : k * ld + line; };
-  cpx<R> a = mk<R>(R(0), R(0)), b = mk<R>(R(0), R(0));
-  int p = 2 * i;
-#pragma unroll
-  for (int k = 0; k < L; ++k, ++p) {
-    int q = p;
-    while (q >= n) q -= n;
-    const cpx<R> x = in[at(q)];
-    a.x += f.lo[k] * x.x;
-    a.y += f.lo[k] * x.y;
-    b.x += f.hi[k] * x.x;
-    b.y += f.hi[k] * x.y;
+// One analysis level over the (h, w) block in one launch, for a compile-time filter length L: each row
+// is filtered into low (lo) and high (hi) halves, out[k] = sum_j f[j] in[(2 k + j) mod n] with j increasing,
+// then each column of that result the same way, giving the four bands.  Block (x, y) produces row-pass
+// outputs p in [16 x, 16 x + 16) (both halves) and column outputs i in [16 y, 16 y + 16): it stages the
+// input rows 2 i .. 2 i + L - 1 and columns 2 p .. 2 p + L - 1 it needs (wrapped periodically) in LDS,
+// filters the rows into LDS, then the columns into dst.  ld: row stride of both images; 32-bit indices
+// (an image holds < 2^31 pixels).  src and dst must not overlap.
+constexpr int DW_T = 16;
+template <typename R, int L>
+__global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, cpx<R>* __restrict__ dst, int h, int w,
+                                               int ld, WavFilt<R> f) {
+  constexpr int RI = 2 * DW_T + L - 2;  // staged rows and columns
+  __shared__ cpx<R> in_t[RI][RI + 1];
+  __shared__ cpx<R> rp_t[RI][2 * DW_T + 1];
+  const int hh = h / 2, hw = w / 2;
+  const int p0 = (int)blockIdx.x * DW_T, i0 = (int)blockIdx.y * DW_T;
+  const int t = (int)threadIdx.x;
+  for (int idx = t; idx < RI * RI; idx += 256) {
+    const int r = idx / RI, cc = idx - r * RI;
+    int row = 2 * i0 + r, col = 2 * p0 + cc;
+    while (row >= h) row -= h;
+    while (col >= w) col -= w;
+    in_t[r][cc] = src[row * ld + col];
   }
-  out[at(i)] = scale(a, scl);
-  out[at(half + i)] = scale(b, scl);
+  __syncthreads();
+  for (int idx = t; idx < RI * DW_T; idx += 256) {
+    const int r = idx / DW_T, q = idx - r * DW_T;
+    cpx<R> a = mk<R>(R(0), R(0)), b = mk<R>(R(0), R(0));
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const cpx<R> x = in_t[r][2 * q + k];
+      a.x += f.lo[k] * x.x;
+      a.y += f.lo[k] * x.y;
+      b.x += f.hi[k] * x.x;
+      b.y += f.hi[k] * x.y;
+    }
+    rp_t[r][q] = a;
+    rp_t[r][DW_T + q] = b;
+  }
+  __syncthreads();
+  for (int idx = t; idx < DW_T * 2 * DW_T; idx += 256) {
+    const int ii = idx / (2 * DW_T), jj = idx - ii * (2 * DW_T);
+    const int q = jj < DW_T ? jj : jj - DW_T;
+    if (i0 + ii >= hh || p0 + q >= hw) continue;
+    cpx<R> a = mk<R>(R(0), R(0)), b = mk<R>(R(0), R(0));
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const cpx<R> x = rp_t[2 * ii + k][jj];
+      a.x += f.lo[k] * x.x;
+      a.y += f.lo[k] * x.y;
+      b.x += f.hi[k] * x.x;
+      b.y += f.hi[k] * x.y;
+    }
+    const int col = (jj < DW_T ? 0 : hw) + p0 + q;
+    dst[(i0 + ii) * ld + col] = a;
+    dst[(hh + i0 + ii) * ld + col] = b;
+  }
+}
+
+// dst[r][c] = src[r][c] over an h x w region (row stride ld)
+template <typename R>
+__global__ void __launch_bounds__(256) k_copy_region(const cpx<R>* __restrict__ src, cpx<R>* __restrict__ dst, int h,
+                                                     int w, int ld) {
+  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x), r = (int)blockIdx.y;
+  if (c < w) dst[r * ld + c] = src[r * ld + c];
 }
 
 // One synthesis pass: outputs 2m and 2m + 1 of each line, which read the same L/2 (low, high) coefficient
@@ -259,8 +297,9 @@ struct DwtLevel {
   hipStream_t s;
   template <int L>
   void operator()() {
-    hipLaunchKernelGGL((k_dwt<R, true, L>), dim3(blocks_of(w / 2), h), dim3(256), 0, s, src, T, h, w, ld, f, R(1));
-    hipLaunchKernelGGL((k_dwt<R, false, L>), dim3(blocks_of(w), h / 2), dim3(256), 0, s, T, c, h, w, ld, f, R(1));
+    // one launch per level through LDS; src is the image (level 0) or a copy of the approximation (T)
+    const unsigned bx = (unsigned)((w / 2 + DW_T - 1) / DW_T), by = (unsigned)((h / 2 + DW_T - 1) / DW_T);
+    hipLaunchKernelGGL((k_dwt2d<R, L>), dim3(bx, by), dim3(256), 0, s, src, c, h, w, ld, f);
   }
 };
 template <typename R, bool ROWS, bool ACC>
@@ -299,8 +338,9 @@ template <typename R>
 void dwt2(const cpx<R>* x, cpx<R>* c, cpx<R>* T, int ny, int nx, int levels, const WavFilt<R>& f, hipStream_t s) {
   for (int l = 0; l < levels; ++l) {
     const int h = ny >> l, w = nx >> l;
-    const cpx<R>* src = l == 0 ? x : c;
-    dispatch_len(f.L, DwtLevel<R>{src, T, c, h, w, nx, f, s});
+    if (l > 0)  // the level reads its approximation band from a copy, as it overwrites that region of c
+      hipLaunchKernelGGL((k_copy_region<R>), dim3(blocks_of(w), h), dim3(256), 0, s, c, T, h, w, nx);
+    dispatch_len(f.L, DwtLevel<R>{l == 0 ? x : T, T, c, h, w, nx, f, s});
   }
 }
 
