@@ -556,18 +556,20 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
   constexpr bool ENT = sizeof(R) == 4;
   constexpr int T = MFMA_TILE;
   constexpr int JJV = JV == 0 ? JMAX : JV, JJU = JU == 0 ? JMAX : JU;
-  // Record windows [0, tap 0 .. tap J-1, 0 ...]: a row or column index clamped to [-1, J] selects a tap
-  // or a zero without a branch.  Odd strides keep the 64 lanes' stores conflict-free.
-  constexpr int SV = (JJV + 2) | 1, SU = (JJU + 2) | 1;
+  // Record windows [tap 0 .. tap J-1, 0 ...]: a row or column index taken as unsigned and capped at J
+  // selects a tap or the zero after the taps without a branch (a negative index wraps to a large value).
+  // Odd strides keep the 64 lanes' stores conflict-free; one zero slot (not one on each side) keeps the
+  // wave's LDS at 5.5 KiB for complex64 and J = 6: 29 resident waves per CU.
+  constexpr int SV = (JJV + 1) | 1, SU = (JJU + 1) | 1;
   // footprint origin relative to the tile, packed as (dx + 512) | (dy + 512) << 16, stored so that the
   // offsets lane l needs in steps 4q .. 4q + 3 (entries 4k + l / 16) are one 16-byte word:
   // entry t at rd[(t & 3) * MQ + (t >> 2)]
   __shared__ __attribute__((aligned(16))) int rd[MB];
-  __shared__ cpx<R> ra[MB * SV];  // [0, s * wv[0..Jv-1], 0 ...]
-  __shared__ R rb[MB * SU];       // [0, wu[0..Ju-1], 0 ...]
+  __shared__ cpx<R> ra[MB * SV];  // [s * wv[0..Jv-1], 0 ...]
+  __shared__ R rb[MB * SU];       // [wu[0..Ju-1], 0 ...]
   using acc_t = typename Mfma<R>::acc_t;
   const int lane = threadIdx.x, li = lane & 15, lt = lane >> 4;
-  for (int q = lane; q < MB * SV; q += 64) ra[q] = mk<R>(R(0), R(0));  // borders stay zero
+  for (int q = lane; q < MB * SV; q += 64) ra[q] = mk<R>(R(0), R(0));  // slots past the taps stay zero
   for (int q = lane; q < MB * SU; q += 64) rb[q] = R(0);
   const int Ju = JU ? JU : g.Ju, Jv = JV ? JV : g.Jv;
   const int W = (int)gridDim.x;
@@ -633,12 +635,12 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
 #pragma unroll
       for (int e = 0; e < JJV; ++e) {
         if (JV == 0 && e >= Jv) break;
-        ra[lane * SV + 1 + e] = mk<R>(s.x * wv[e], s.y * wv[e]);
+        ra[lane * SV + e] = mk<R>(s.x * wv[e], s.y * wv[e]);
       }
 #pragma unroll
       for (int e = 0; e < JJU; ++e) {
         if (JU == 0 && e >= Ju) break;
-        rb[lane * SU + 1 + e] = wu[e];
+        rb[lane * SU + e] = wu[e];
       }
     } else {
       rd[(lane & 3) * MQ + (lane >> 2)] = (MFMA_NOTAP + 512) | ((MFMA_NOTAP + 512) << 16);
@@ -671,10 +673,10 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int t = 16 * q + 4 * r + lt;  // entry of step 4q + r; li is its row of A and column of B
-        const int ev = min(max(li + 512 - (dd[r] >> 16), -1), Jv);
-        const int eu = min(max(li + 512 - (dd[r] & 0xffff), -1), Ju);
-        a[r] = ra[t * SV + 1 + ev];
-        b[r] = rb[t * SU + 1 + eu];
+        const unsigned ev = min((unsigned)(li + 512 - (dd[r] >> 16)), (unsigned)Jv);
+        const unsigned eu = min((unsigned)(li + 512 - (dd[r] & 0xffff)), (unsigned)Ju);
+        a[r] = ra[t * SV + (int)ev];
+        b[r] = rb[t * SU + (int)eu];
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
