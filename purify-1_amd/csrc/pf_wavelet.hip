@@ -112,12 +112,15 @@ struct WavFilt {
 // then each column of that result the same way, giving the four bands.  Block (x, y) produces row-pass
 // outputs p in [16 x, 16 x + 16) (both halves) and column outputs i in [16 y, 16 y + 16): it stages the
 // input rows 2 i .. 2 i + L - 1 and columns 2 p .. 2 p + L - 1 it needs (wrapped periodically) in LDS,
-// filters the rows into LDS, then the columns into dst.  ld: row stride of both images; 32-bit indices
-// (an image holds < 2^31 pixels).  src and dst must not overlap.
+// filters the rows into LDS, then the columns.  The three detail bands go to their places in dst (row
+// stride ld) multiplied by sd, the approximation band (low-low, h/2 x w/2) to apx (row stride la), which
+// may be dst itself, multiplied by sa.  ls: row stride of src.  32-bit indices (an image holds < 2^31
+// pixels).  src must not overlap dst or apx.
 constexpr int DW_T = 16;
 template <typename R, int L>
-__global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, cpx<R>* __restrict__ dst, int h, int w,
-                                               int ld, WavFilt<R> f) {
+__global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, int ls, cpx<R>* __restrict__ dst, int ld,
+                                               cpx<R>* __restrict__ apx, int la, int h, int w, WavFilt<R> f, R sd,
+                                               R sa) {
   constexpr int RI = 2 * DW_T + L - 2;  // staged rows and columns
   __shared__ cpx<R> in_t[RI][RI + 1];
   __shared__ cpx<R> rp_t[RI][2 * DW_T + 1];
@@ -129,7 +132,7 @@ __global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, c
     int row = 2 * i0 + r, col = 2 * p0 + cc;
     while (row >= h) row -= h;
     while (col >= w) col -= w;
-    in_t[r][cc] = src[row * ld + col];
+    in_t[r][cc] = src[row * ls + col];
   }
   __syncthreads();
   for (int idx = t; idx < RI * DW_T; idx += 256) {
@@ -161,17 +164,12 @@ __global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, c
       b.y += f.hi[k] * x.y;
     }
     const int col = (jj < DW_T ? 0 : hw) + p0 + q;
-    dst[(i0 + ii) * ld + col] = a;
-    dst[(hh + i0 + ii) * ld + col] = b;
+    if (jj < DW_T)
+      apx[(i0 + ii) * la + col] = scale(a, sa);
+    else
+      dst[(i0 + ii) * ld + col] = scale(a, sd);
+    dst[(hh + i0 + ii) * ld + col] = scale(b, sd);
   }
-}
-
-// dst[r][c] = src[r][c] over an h x w region (row stride ld)
-template <typename R>
-__global__ void __launch_bounds__(256) k_copy_region(const cpx<R>* __restrict__ src, cpx<R>* __restrict__ dst, int h,
-                                                     int w, int ld) {
-  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x), r = (int)blockIdx.y;
-  if (c < w) dst[r * ld + c] = src[r * ld + c];
 }
 
 // One synthesis pass: outputs 2m and 2m + 1 of each line, which read the same L/2 (low, high) coefficient
@@ -290,16 +288,18 @@ inline unsigned blocks_of(int n) { return (unsigned)std::max(1, (n + 255) / 256)
 template <typename R>
 struct DwtLevel {
   const cpx<R>* src;
-  cpx<R>* T;
+  int ls;
   cpx<R>* c;
-  int h, w, ld;
+  int ld;
+  cpx<R>* apx;
+  int la, h, w;
   const WavFilt<R>& f;
+  R sd, sa;
   hipStream_t s;
   template <int L>
   void operator()() {
-    // one launch per level through LDS; src is the image (level 0) or a copy of the approximation (T)
     const unsigned bx = (unsigned)((w / 2 + DW_T - 1) / DW_T), by = (unsigned)((h / 2 + DW_T - 1) / DW_T);
-    hipLaunchKernelGGL((k_dwt2d<R, L>), dim3(bx, by), dim3(256), 0, s, src, c, h, w, ld, f);
+    hipLaunchKernelGGL((k_dwt2d<R, L>), dim3(bx, by), dim3(256), 0, s, src, ls, c, ld, apx, la, h, w, f, sd, sa);
   }
 };
 template <typename R, bool ROWS, bool ACC>
@@ -333,14 +333,24 @@ struct IdwtColsSrc {
   }
 };
 
-// Analysis of one entry: image x -> coefficients c (both ny x nx); T: work image.
+// Analysis of one entry: image x -> coefficients c (both ny x nx), each coefficient multiplied by scl;
+// T: work image.  Each level writes its detail bands into c and its approximation band, compact (row
+// stride w / 2), into T or T + ny nx / 4 in turn, where the next level reads it: a level never reads
+// what it writes, and the two regions (at most ny nx / 4 and ny nx / 16 elements) are disjoint.  The
+// last level writes its approximation into c.  Only what lands in c is scaled.
 template <typename R>
-void dwt2(const cpx<R>* x, cpx<R>* c, cpx<R>* T, int ny, int nx, int levels, const WavFilt<R>& f, hipStream_t s) {
+void dwt2(const cpx<R>* x, cpx<R>* c, cpx<R>* T, int ny, int nx, int levels, const WavFilt<R>& f, R scl,
+          hipStream_t s) {
+  const cpx<R>* src = x;
+  int ls = nx;
   for (int l = 0; l < levels; ++l) {
     const int h = ny >> l, w = nx >> l;
-    if (l > 0)  // the level reads its approximation band from a copy, as it overwrites that region of c
-      hipLaunchKernelGGL((k_copy_region<R>), dim3(blocks_of(w), h), dim3(256), 0, s, c, T, h, w, nx);
-    dispatch_len(f.L, DwtLevel<R>{l == 0 ? x : T, T, c, h, w, nx, f, s});
+    const bool last = l == levels - 1;
+    cpx<R>* apx = last ? c : T + ((l & 1) ? (size_t)ny * nx / 4 : 0);
+    const int la = last ? nx : w / 2;
+    dispatch_len(f.L, DwtLevel<R>{src, ls, c, nx, apx, la, h, w, f, scl, last ? scl : R(1), s});
+    src = apx;
+    ls = la;
   }
 }
 
@@ -412,10 +422,8 @@ void wavelet_analysis(pf_wavelet* wv, const cpx<R>* x, cpx<R>* c, hipStream_t s)
       hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, x, ci, N, scl);
       continue;
     }
-    dwt2<R>(x, ci, static_cast<cpx<R>*>(wv->d_T), wv->ny, wv->nx, wv->levels[i], f, s);
-    // the SARA factor (a pass over the stacked entry; the coefficients of the finer levels are not
-    // touched again by the coarser ones)
-    if (scl != R(1)) scale_vec<R>(ci, N, (double)scl, s);
+    // the SARA factor is applied as the coefficients are written (a product by 1 for one basis)
+    dwt2<R>(x, ci, static_cast<cpx<R>*>(wv->d_T), wv->ny, wv->nx, wv->levels[i], f, scl, s);
   }
   PF_HIP(hipGetLastError());
 }
