@@ -172,69 +172,77 @@ __global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, i
   }
 }
 
-// One synthesis pass: outputs 2m and 2m + 1 of each line, which read the same L/2 (low, high) coefficient
-// pairs i = m, m - 1, ... (mod n/2): output 2m takes the even taps, 2m + 1 the odd ones, in increasing
-// tap order.  Rows: block (x, y) takes m = 256 x + t of row y; columns: m = y for the columns 256 x + t.
-template <typename R, bool ROWS, bool ACC, int L>
-__global__ void __launch_bounds__(256) k_idwt(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int h, int w,
-                                              int ld, WavFilt<R> f, R scl) {
-  const int n = ROWS ? w : h;
-  const int half = n / 2;
-  const int line = ROWS ? (int)blockIdx.y : (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int m = ROWS ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)blockIdx.y;
-  if (ROWS ? m >= half : line >= w) return;
-  auto at = [&](int k) -> int { return ROWS ? line * ld + k : k * ld + line; };
-  cpx<R> ae = mk<R>(R(0), R(0)), ao = mk<R>(R(0), R(0));
+// One synthesis level in one launch, for a compile-time filter length L: first the columns, then the
+// rows.  Along a line of n samples whose low half holds lo[0 .. n/2) and high half hi[0 .. n/2),
+// outputs 2m and 2m + 1 read the same L/2 pairs i = m, m - 1, ... (mod n/2): output 2m takes the even
+// taps, 2m + 1 the odd ones, in increasing tap order (out[2m + e] = sum_k f.lo[2k + e] lo[m - k] +
+// f.hi[2k + e] hi[m - k]); the column results pass to the row step through LDS.  Block (x, y) produces output rows
+// 2 i .. 2 i + 1 and columns 2 p .. 2 p + 1 for i in [16 y, 16 y + 16), p in [16 x, 16 x + 16): it
+// stages the CI = 16 + L/2 - 1 coefficient rows i - L/2 + 1 .. i + 15 (mod h/2) of the low and of the
+// high half, over the CI columns of the same window in each half of the columns; the approximation
+// quadrant comes from apx (row stride la; the coarser level's output) or, when apx is null, from c
+// scaled by cs, and every other quadrant from c (row stride ld) scaled by cs.  Output to out (row stride
+// lout), added to its values when ACC.  apx and c must not overlap out.
+template <typename R, bool ACC, int L>
+__global__ void __launch_bounds__(256) k_idwt2d(const cpx<R>* __restrict__ c, int ld, const cpx<R>* __restrict__ apx,
+                                                int la, cpx<R>* __restrict__ out, int lout, int h, int w, WavFilt<R> f,
+                                                R cs) {
+  constexpr int K = L / 2, CI = DW_T + K - 1;
+  __shared__ cpx<R> in_t[2 * CI][2 * CI + 1];
+  __shared__ cpx<R> cp_t[2 * DW_T][2 * CI + 1];
+  const int hh = h / 2, hw = w / 2;
+  const int p0 = (int)blockIdx.x * DW_T, i0 = (int)blockIdx.y * DW_T;
+  const int t = (int)threadIdx.x;
+  for (int idx = t; idx < 4 * CI * CI; idx += 256) {
+    const int rr = idx / (2 * CI), cc = idx - rr * (2 * CI);
+    const bool hi_r = rr >= CI, hi_c = cc >= CI;
+    int i = i0 - (K - 1) + (hi_r ? rr - CI : rr), j = p0 - (K - 1) + (hi_c ? cc - CI : cc);
+    while (i < 0) i += hh;
+    while (i >= hh) i -= hh;
+    while (j < 0) j += hw;
+    while (j >= hw) j -= hw;
+    const int row = (hi_r ? hh : 0) + i, col = (hi_c ? hw : 0) + j;
+    in_t[rr][cc] = apx && !hi_r && !hi_c ? apx[i * la + j] : scale(c[row * ld + col], cs);
+  }
+  __syncthreads();
+  for (int idx = t; idx < DW_T * 2 * CI; idx += 256) {
+    const int mi = idx / (2 * CI), cc = idx - mi * (2 * CI);
+    cpx<R> ae = mk<R>(R(0), R(0)), ao = mk<R>(R(0), R(0));
 #pragma unroll
-  for (int kk = 0; kk < L / 2; ++kk) {
-    int i = m - kk;
-    while (i < 0) i += half;
-    const cpx<R> cl = in[at(i)], ch = in[at(half + i)];
-    ae.x += f.lo[2 * kk] * cl.x + f.hi[2 * kk] * ch.x;
-    ae.y += f.lo[2 * kk] * cl.y + f.hi[2 * kk] * ch.y;
-    ao.x += f.lo[2 * kk + 1] * cl.x + f.hi[2 * kk + 1] * ch.x;
-    ao.y += f.lo[2 * kk + 1] * cl.y + f.hi[2 * kk + 1] * ch.y;
+    for (int kk = 0; kk < K; ++kk) {
+      const cpx<R> cl = in_t[K - 1 + mi - kk][cc], ch = in_t[CI + K - 1 + mi - kk][cc];
+      ae.x += f.lo[2 * kk] * cl.x + f.hi[2 * kk] * ch.x;
+      ae.y += f.lo[2 * kk] * cl.y + f.hi[2 * kk] * ch.y;
+      ao.x += f.lo[2 * kk + 1] * cl.x + f.hi[2 * kk + 1] * ch.x;
+      ao.y += f.lo[2 * kk + 1] * cl.y + f.hi[2 * kk + 1] * ch.y;
+    }
+    cp_t[2 * mi][cc] = ae;
+    cp_t[2 * mi + 1][cc] = ao;
   }
-  ae = scale(ae, scl);
-  ao = scale(ao, scl);
-  if (ACC) {
-    const cpx<R> oe = out[at(2 * m)], oo = out[at(2 * m + 1)];
-    ae.x += oe.x;
-    ae.y += oe.y;
-    ao.x += oo.x;
-    ao.y += oo.y;
-  }
-  out[at(2 * m)] = ae;
-  out[at(2 * m + 1)] = ao;
-}
-
-// The column synthesis pass of a level read straight from the coefficients: the approximation quadrant
-// (columns < w/2 of the top half) from apx (the previous, coarser level's output; null at the coarsest
-// level, where it is in c too) and everything else from c, scaled by cs on the way in.  This replaces a
-// scaled copy of all coefficients into the work image; the values are the same as the copy's.
-template <typename R, int L>
-__global__ void __launch_bounds__(256) k_idwt_cols_src(const cpx<R>* __restrict__ c, const cpx<R>* __restrict__ apx,
-                                                       cpx<R>* __restrict__ out, int h, int w, int ld, WavFilt<R> f,
-                                                       R cs) {
-  const int half = h / 2;
-  const int line = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int m = (int)blockIdx.y;
-  if (line >= w) return;
-  const cpx<R>* lo_src = apx && line < w / 2 ? apx : c;
-  const R lo_s = lo_src == c ? cs : R(1);
-  cpx<R> ae = mk<R>(R(0), R(0)), ao = mk<R>(R(0), R(0));
+  __syncthreads();
+  for (int idx = t; idx < 2 * DW_T * DW_T; idx += 256) {
+    const int rl = idx / DW_T, q = idx - rl * DW_T;
+    if (i0 + rl / 2 >= hh || p0 + q >= hw) continue;
+    cpx<R> ae = mk<R>(R(0), R(0)), ao = mk<R>(R(0), R(0));
 #pragma unroll
-  for (int kk = 0; kk < L / 2; ++kk) {
-    int i = m - kk;
-    while (i < 0) i += half;
-    const cpx<R> cl = scale(lo_src[i * ld + line], lo_s), ch = scale(c[(half + i) * ld + line], cs);
-    ae.x += f.lo[2 * kk] * cl.x + f.hi[2 * kk] * ch.x;
-    ae.y += f.lo[2 * kk] * cl.y + f.hi[2 * kk] * ch.y;
-    ao.x += f.lo[2 * kk + 1] * cl.x + f.hi[2 * kk + 1] * ch.x;
-    ao.y += f.lo[2 * kk + 1] * cl.y + f.hi[2 * kk + 1] * ch.y;
+    for (int kk = 0; kk < K; ++kk) {
+      const cpx<R> cl = cp_t[rl][K - 1 + q - kk], ch = cp_t[rl][CI + K - 1 + q - kk];
+      ae.x += f.lo[2 * kk] * cl.x + f.hi[2 * kk] * ch.x;
+      ae.y += f.lo[2 * kk] * cl.y + f.hi[2 * kk] * ch.y;
+      ao.x += f.lo[2 * kk + 1] * cl.x + f.hi[2 * kk + 1] * ch.x;
+      ao.y += f.lo[2 * kk + 1] * cl.y + f.hi[2 * kk + 1] * ch.y;
+    }
+    cpx<R>* o = out + (2 * i0 + rl) * lout + 2 * (p0 + q);
+    if (ACC) {
+      const cpx<R> oe = o[0], oo = o[1];
+      ae.x += oe.x;
+      ae.y += oe.y;
+      ao.x += oo.x;
+      ao.y += oo.y;
+    }
+    o[0] = ae;
+    o[1] = ao;
   }
-  out[(2 * m) * ld + line] = ae;
-  out[(2 * m + 1) * ld + line] = ao;
 }
 
 // Calls F.template operator()<L>() for the runtime filter length L = 2, 4, ..., WAV_MAXL.
@@ -302,36 +310,24 @@ struct DwtLevel {
     hipLaunchKernelGGL((k_dwt2d<R, L>), dim3(bx, by), dim3(256), 0, s, src, ls, c, ld, apx, la, h, w, f, sd, sa);
   }
 };
-template <typename R, bool ROWS, bool ACC>
-struct IdwtPass {
-  const cpx<R>* in;
-  cpx<R>* out;
-  int h, w, ld;
-  const WavFilt<R>& f;
-  hipStream_t s;
-  template <int L>
-  void operator()() {  // rows: (w / 2 pairs, h rows); columns: (w columns, h / 2 pairs)
-    if (ROWS)
-      hipLaunchKernelGGL((k_idwt<R, true, ACC, L>), dim3(blocks_of(w / 2), h), dim3(256), 0, s, in, out, h, w, ld, f, R(1));
-    else
-      hipLaunchKernelGGL((k_idwt<R, false, ACC, L>), dim3(blocks_of(w), h / 2), dim3(256), 0, s, in, out, h, w, ld, f, R(1));
-  }
-};
-
-template <typename R>
-struct IdwtColsSrc {
+template <typename R, bool ACC>
+struct IdwtLevel {
   const cpx<R>* c;
+  int ld;
   const cpx<R>* apx;
+  int la;
   cpx<R>* out;
-  int h, w, ld;
+  int lout, h, w;
   const WavFilt<R>& f;
   R cs;
   hipStream_t s;
   template <int L>
   void operator()() {
-    hipLaunchKernelGGL((k_idwt_cols_src<R, L>), dim3(blocks_of(w), h / 2), dim3(256), 0, s, c, apx, out, h, w, ld, f, cs);
+    const unsigned bx = (unsigned)((w / 2 + DW_T - 1) / DW_T), by = (unsigned)((h / 2 + DW_T - 1) / DW_T);
+    hipLaunchKernelGGL((k_idwt2d<R, ACC, L>), dim3(bx, by), dim3(256), 0, s, c, ld, apx, la, out, lout, h, w, f, cs);
   }
 };
+
 
 // Analysis of one entry: image x -> coefficients c (both ny x nx), each coefficient multiplied by scl;
 // T: work image.  Each level writes its detail bands into c and its approximation band, compact (row
@@ -364,16 +360,21 @@ void idwt2(const cpx<R>* c, cpx<R>* x, cpx<R>* W, cpx<R>* T, int ny, int nx, int
     else hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, c, x, N, scl);
     return;
   }
-  // W holds each level's reconstructed approximation; the column passes read it and the coefficients
-  // (with the SARA factor applied on the way in)
+  // One launch per level.  Each level but the finest writes its reconstructed approximation, compact
+  // (row stride w), into W or T in turn, and the next finer level reads it from there: no level reads
+  // what it writes.  The coefficients enter with the SARA factor applied.
+  const cpx<R>* apx = nullptr;
+  int la = 0;
   for (int l = levels - 1; l >= 0; --l) {
     const int h = ny >> l, w = nx >> l;
-    dispatch_len(f.L, IdwtColsSrc<R>{c, l == levels - 1 ? nullptr : W, T, h, w, nx, f, scl, s});
     if (l == 0) {
-      if (acc) dispatch_len(f.L, IdwtPass<R, true, true>{T, x, h, w, nx, f, s});
-      else dispatch_len(f.L, IdwtPass<R, true, false>{T, x, h, w, nx, f, s});
+      if (acc) dispatch_len(f.L, IdwtLevel<R, true>{c, nx, apx, la, x, nx, h, w, f, scl, s});
+      else dispatch_len(f.L, IdwtLevel<R, false>{c, nx, apx, la, x, nx, h, w, f, scl, s});
     } else {
-      dispatch_len(f.L, IdwtPass<R, true, false>{T, W, h, w, nx, f, s});
+      cpx<R>* o = (l & 1) ? W : T;
+      dispatch_len(f.L, IdwtLevel<R, false>{c, nx, apx, la, o, w, h, w, f, scl, s});
+      apx = o;
+      la = w;
     }
   }
 }
