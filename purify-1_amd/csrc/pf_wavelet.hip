@@ -479,7 +479,7 @@ int pf_wavelet_create(int32_t imsizey, int32_t imsizex, int32_t nwav, const int3
       wv->db.push_back(db[i]);
       wv->levels.push_back(db[i] ? levels[i] : 0);
       wv->filters.push_back(db[i] ? daubechies_filter(db[i]) : std::vector<double>());
-      // the periodic filters need at least as many samples per line at the coarsest level as taps / 2
+      // no minimum line length: the periodic extension wraps a filter around a short line as often as needed
     }
     const size_t b = wv->esz() * (size_t)wv->n();
     wv->d_T = wv->dalloc(b);

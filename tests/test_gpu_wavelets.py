@@ -22,7 +22,8 @@ def entries(ws):
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
 @pytest.mark.parametrize("ws,ny,nx", [([("DB4", 3)], 64, 64), (SARA9, 128, 128), ([("DB2", 2), ("Dirac", 1)], 32, 64),
-                                      ([("DB8", 3), ("DB10", 2)], 16, 16)])
+                                      ([("DB8", 3), ("DB10", 2)], 16, 16), ([("DB1", 5), ("DB2", 4)], 32, 32),
+                                      ([("DB3", 3), ("DB6", 2)], 16, 64)])
 def test_analysis_and_synthesis_vs_oracle(precision, ws, ny, nx):
     W = wavelets.wavelet_operator_factory("serial", ws, ny, nx, precision=precision)
     x = utilities.synth_cnormal(501, ny * nx)
