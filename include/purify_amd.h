@@ -130,8 +130,8 @@ int pf_wavelet_destroy(pf_wavelet *wv);
  * padmm_factory, algorithm_factory.h:36-81) ----
  * Iteration (SOPT variable names): z = P_eps(-lambda - r); x~ = x - Phi^dagger(r + lambda + z) / nu;
  * x = prox(x~); r = Phi x - y; lambda += lagrange_update_scale (r + z); from x = Phi^dagger y / nu,
- * r = Phi x - y, lambda = 0.  The l1 proximal is the closed form x~ + Psi(soft(Psi^dagger x~, gamma nu)
- * - Psi^dagger x~) / nu for a tight frame (one orthonormal wavelet: main.cc:121-126), or, for SARA, the
+ * r = Phi x - y, lambda = 0.  The l1 proximal is the closed form x~ + Psi(soft(Psi^dagger x~, gamma l1_nu)
+ * - Psi^dagger x~) / l1_nu (l1_nu: the proximal's frame bound; nu is the ADMM step) for a tight frame (one orthonormal wavelet: main.cc:121-126), or, for SARA, the
  * iterative constrained one: argmin 1/2 ||x - x~||^2 + gamma ||Psi^dagger x||_1 over the positivity /
  * real constraints, by the dual forward-backward iteration (u = P_gamma(u + Psi^dagger x / l1_nu),
  * x = P_C(x~ - Psi u), from u = 0) until the objective's relative change is below l1_tolerance.
@@ -272,8 +272,10 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  *      closest analogue) ---- */
 /* Keys: "kernel_poly" (complex64, Ju = Jv <= 8, KB / KB-min / Gaussian / PSWF: 1 = the taps of both
  * axes from per-tap degree-9 polynomials in the fractional offset, checked at build to 1e-7 of the
- * kernel's peak, the default where they apply and ahead of the tables; 0 = off); "grid_kernel" (0 = residue-class lanes, 1 = LDS atomics, 2 = rank-4 updates of 16 x 16 tiles on
- * the matrix cores, the default where the grid sides are at least 16 + J - 1; all reproducible run to run);
+ * kernel's peak, the default where they apply and ahead of the tables; 0 = off); "grid_kernel" (0 = residue-class lanes, 2 = rank-4 updates of 16 x 16 tiles on the matrix
+ * cores, the default where the grid sides are at least 16 + J - 1: both sum every grid cell in a fixed
+ * order, so they are bit-identical run to run; 1 = LDS atomics, whose in-wave summation order is not
+ * architecturally fixed: set only on explicit request, never chosen by pf_op_autotune);
  * "kernel_table" (complex64, J <= 8: interpolated tap tables, about 1e-7 relative error: 1 = in gridding,
  * the default, 2 = in gridding and degridding; 0 = evaluate the kernel for every tap); "fft_columns"
  * (0 = the column FFT in one pass up to 8192 rows, the default; 1 = two passes of N1 x N2 beyond 512
@@ -283,10 +285,13 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * grid_waves-th 16 x 16 tile with its batches pipelined across tiles; 0 = one wave per tile, the
  * default); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond
  * floating-point summation order. */
+/* pf_op_set_norm, pf_op_set_tuning and pf_op_autotune change what an apply launches: a solver that
+ * captured the operator's launches into a graph captures them again on its next run. */
 int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
 int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
-/* Times every gridding-kernel variant on this operator (reps adjoint gridding passes each, HIP events on
- * the null stream) and keeps the fastest, then does the same for the two "fft_columns" modes (reps
+/* Times the fixed-order gridding-kernel variants (residue-class lanes and matrix cores, not the LDS-atomic
+ * one) on this operator (reps adjoint gridding passes each, HIP events on the null stream) and keeps the
+ * fastest, then does the same for the two "fft_columns" modes (reps
  * forward + adjoint column passes each).  Writes the chosen gridding variant to *chosen if non-NULL;
  * the column mode is read back with pf_op_get_tuning.  Operators with w-projection rows or a given
  * gridding matrix have one gridding kernel: only the column mode is chosen. */

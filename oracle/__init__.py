@@ -570,8 +570,9 @@ def padmm(op: "Operator", entries, y, epsilon, iters, gamma=None, nu=1.0, rho=0.
     """Imaging proximal ADMM with a tight-frame l1 proximal (the iteration SOPT's ImagingProximalADMM runs
     with padmm_factory's parameters, algorithm_factory.h:36-81), in FP64:
         z = P_eps(-lambda - r);  x~ = x - Phi^dagger(r + lambda + z) / nu
-        x = x~ + Psi(soft(Psi^dagger x~, gamma nu) - Psi^dagger x~) / nu
+        x = x~ + Psi(soft(Psi^dagger x~, gamma l1_nu) - Psi^dagger x~) / l1_nu
         r = Phi x - y;  lambda += rho (r + z)
+    (nu is the ADMM step; l1_nu the l1 proximal's frame bound, SOPT's L1TightFrame nu)
     from x = Phi^dagger y / nu, r = Phi x - y, lambda = 0; gamma = max |Psi^dagger Phi^dagger y| * 1e-3 if
     not given.  tight_frame False: x = l1_prox_constrained(x~, gamma, ...) instead (SARA).  Restated from
     the published algorithm (SOPT is not in the reference tree): parity with SOPT's iterates is
@@ -595,7 +596,7 @@ def padmm(op: "Operator", entries, y, epsilon, iters, gamma=None, nu=1.0, rho=0.
         xu = x - op.adjoint(r + lam + z) / nu
         if tight_frame:
             c = psit(xu)
-            xn = xu + psi(soft_threshold(c, gamma * nu) - c) / nu
+            xn = xu + psi(soft_threshold(c, gamma * l1_nu) - c) / l1_nu
         else:
             xn, _ = l1_prox_constrained(xu, gamma, psit, psi, l1_nu, l1_itermax, l1_tol, positivity, real)
         r = op.forward(xn) - y

@@ -323,6 +323,8 @@ struct OpShape {
   int64_t M;
 };
 OpShape op_shape(const pf_op* op);
+// changes whenever the operator's launches change (norm, tuning, autotune)
+uint64_t op_generation(const pf_op* op);
 // Psi^dagger (image -> stacked coefficients) and Psi on device vectors
 template <typename R>
 void wavelet_analysis(pf_wavelet* wv, const cpx<R>* x, cpx<R>* c, hipStream_t s);

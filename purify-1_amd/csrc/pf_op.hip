@@ -373,6 +373,9 @@ struct pf_op {
   int fft_col_mode = 0;             // tuning knob: column FFT in one pass (0) or two steps (1)
   int grid_waves = 0;               // tuning knob: matrix-core gridding waves (0: one per 16 x 16 tile)
   int fft_row_bpc = -1;             // tuning knob: row pass blocks per CU (0: one line per block; -1: plan default)
+  // bumped by every call that changes what an apply launches (norm, tuning, autotune): a solver that
+  // captured this operator's launches into a graph captures again when it differs from its own copy
+  uint64_t generation = 0;
   int64_t device_bytes = 0;
   std::vector<void*> allocs;
 
@@ -1257,6 +1260,7 @@ template void op_forward<double>(pf_op*, const cpx<double>*, cpx<double>*, hipSt
 template void op_adjoint<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
 template void op_adjoint<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
 OpShape op_shape(const pf_op* op) { return OpShape{op->ny, op->nx, op->prec, op->M}; }
+uint64_t op_generation(const pf_op* op) { return op->generation; }
 }  // namespace pf
 
 extern "C" {
@@ -1661,7 +1665,13 @@ int pf_psf_destroy(pf_psf* h) {
   return PF_OK;
 }
 
+static int set_tuning_impl(pf_op* op, const char* key, int64_t value);
 int pf_op_set_tuning(pf_op* op, const char* key, int64_t value) {
+  const int st = set_tuning_impl(op, key, value);
+  if (st == PF_OK) ++op->generation;
+  return st;
+}
+static int set_tuning_impl(pf_op* op, const char* key, int64_t value) {
   if (!op || !key) return fail(PF_ERR_ARG, "null argument");
   const std::string k(key);
   if (k == "grid_kernel") {
@@ -1763,7 +1773,9 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     const int n = std::max(1, (int)reps);
     float best = 1e30f;
     int best_v = op->grid_variant;
-    for (int variant : {GRID_RESIDUE, GRID_ATOMIC, GRID_MFMA}) {
+    // the LDS-atomic variant's in-wave summation order is not architecturally fixed, so it is never
+    // chosen here: every candidate gives run-to-run bit-identical grids
+    for (int variant : {GRID_RESIDUE, GRID_MFMA}) {
       if (one_grid_kernel) break;
       if (variant == GRID_RESIDUE && !grid_rc_ok(op->Nu, op->Nv, op->tl.TU, op->tl.TV, op->Ju, op->Jv)) continue;
       if (variant == GRID_MFMA && !op->tlm.slot) continue;
@@ -1828,6 +1840,7 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
       if (op->fft_row_bpc >= 0) op->fft.row_bpc = op->fft_row_bpc;
     }
     op->fft_col_mode = best_m;
+    ++op->generation;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (chosen) *chosen = best_v;
@@ -1876,6 +1889,7 @@ int pf_op_set_norm(pf_op* op, double norm) {
   if (!op) return fail(PF_ERR_ARG, "null operator");
   if (!(norm > 0)) return fail(PF_ERR_ARG, "Operator norm is not greater than zero.");
   op->norm = norm;
+  ++op->generation;
   return PF_OK;
 }
 

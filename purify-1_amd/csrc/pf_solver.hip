@@ -340,6 +340,8 @@ struct pf_padmm {
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t captured_gen = 0;  // the operator's generation when the graph was captured
+  double captured_gamma = 0;  // and the l1 weight its threshold kernel was given (reset recomputes it)
   std::vector<void*> allocs;
   void* dalloc(size_t bytes) {
     void* p = nullptr;
@@ -428,10 +430,11 @@ void padmm_step(pf_padmm* h) {
     hipLaunchKernelGGL((k_grad_step<R, false>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev),
                        V<R>(h->xu), inv_nu, N);
     wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
-    hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.nu), NC);
+    // the l1 proximal's own frame bound l1_nu (SOPT's L1TightFrame nu), not the ADMM step nu
+    hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.l1_nu), NC);
     wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
     hipLaunchKernelGGL((k_prox_out_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x),
-                       V<R>(h->x), inv_nu, N, h->fpart);
+                       V<R>(h->x), (R)(1.0 / h->par.l1_nu), N, h->fpart);
   } else {
     hipLaunchKernelGGL((k_grad_step<R, true>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev),
                        V<R>(h->xu), inv_nu, N);
@@ -529,9 +532,29 @@ void padmm_init(pf_padmm* h) {
   h->l1_total = 0;
 }
 
+// A captured graph holds the operator's launch arguments (its norm in the correction scale, the
+// gridding variant, the FFT plan's twiddle table).  pf_op_set_norm, pf_op_set_tuning and
+// pf_op_autotune change them, so the graph is captured again when the operator's generation moved.
 template <typename R>
 void padmm_capture(pf_padmm* h) {
   hipStream_t s = h->stream;
+  const uint64_t gen = op_generation(h->phi);
+  if (h->exec && (h->captured_gen != gen || h->captured_gamma != h->gamma)) {
+    PF_HIP(hipStreamSynchronize(s));
+    (void)hipGraphExecDestroy(h->exec);
+    (void)hipGraphDestroy(h->graph);
+    h->exec = nullptr;
+    h->graph = nullptr;
+    // run the operator's (possibly new) kernels once eagerly, so their attributes are set before capture;
+    // fx and g are scratch between iterations (each iteration writes them before reading them)
+    if (h->fb) {
+      op_normal<R>(h->phi, V<R>(h->x), V<R>(h->g), s);
+    } else {
+      op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
+      op_adjoint<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
+    }
+    PF_HIP(hipStreamSynchronize(s));
+  }
   if (!h->exec) {
     if (h->iters_done == 0) {
       // one eager iteration first, so every kernel attribute is set before capture; then
@@ -546,6 +569,8 @@ void padmm_capture(pf_padmm* h) {
     padmm_iteration<R>(h);
     PF_HIP(hipStreamEndCapture(s, &h->graph));
     PF_HIP(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+    h->captured_gen = gen;
+    h->captured_gamma = h->gamma;
   }
 }
 

@@ -25,8 +25,11 @@ def fft2d(x, Nv, Nu, sign=-1):
     """Unitary 2-D FFT of a torch CUDA tensor (complex64/complex128) of Nv*Nu entries (operators.h:209-261)."""
     import ctypes as C
     import torch
-    x = x.contiguous()
+    if x.dtype not in (torch.complex64, torch.complex128):
+        raise TypeError(f"fft2d takes complex64 or complex128, got {x.dtype}")
     prec = _lib.PF_C128 if x.dtype == torch.complex128 else _lib.PF_C64
+    from .operators import check_device_tensor
+    x = check_device_tensor(x.contiguous(), int(Nv) * int(Nu), x.dtype, None, "input")
     out = torch.empty_like(x)
     s = torch.cuda.current_stream(x.device).cuda_stream
     _lib.check(_lib.lib().pf_fft2d(Nv, Nu, prec, sign, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), None,

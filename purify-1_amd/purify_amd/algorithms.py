@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib, utilities
-from .operators import LinearTransform, MeasurementOperator
+from .operators import LinearTransform, MeasurementOperator, check_device_tensor
 from .wavelets import WaveletOperator
 
 
@@ -50,7 +50,7 @@ class ImagingProximalADMM:
         self.itermax = int(itermax)
         self._h = C.c_void_p()
         if hasattr(y, "is_cuda") and y.is_cuda:
-            y = y.contiguous()
+            y = check_device_tensor(y.contiguous(), phi.M, phi._dtype(), None, "y")
             self._y = y
             _lib.check(_lib.lib().pf_padmm_create(phi.handle, psi.handle, C.c_void_p(y.data_ptr()), _lib.PF_DEVICE,
                                                   C.byref(p), C.byref(self._h)))
@@ -135,7 +135,7 @@ class ImagingForwardBackward(ImagingProximalADMM):
         self.itermax = int(itermax)
         self._h = C.c_void_p()
         if hasattr(y, "is_cuda") and y.is_cuda:
-            y = y.contiguous()
+            y = check_device_tensor(y.contiguous(), phi.M, phi._dtype(), None, "y")
             self._y = y
             _lib.check(_lib.lib().pf_fb_create(phi.handle, psi.handle, C.c_void_p(y.data_ptr()), _lib.PF_DEVICE,
                                                C.byref(p), C.byref(self._h)))

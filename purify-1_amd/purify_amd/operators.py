@@ -47,6 +47,25 @@ def _is_device_tensor(x):
     return isinstance(x, torch.Tensor) and x.is_cuda
 
 
+def check_device_tensor(t, n, dtype, device=None, what="tensor"):
+    """Raise unless t is a contiguous CUDA tensor of n entries of dtype (on device, if given).
+
+    The C library reads and writes device memory through the raw pointer with the operator's sizes, so
+    every device tensor that crosses the boundary is checked here first."""
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError(f"{what} must be a CUDA tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{what} must be {dtype}, got {t.dtype}")
+    if device is not None and t.device != device:
+        raise ValueError(f"{what} is on {t.device}, expected {device}")
+    if t.numel() != n:
+        raise ValueError(f"{what} has {t.numel()} entries, expected {n}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    return t
+
+
 class MeasurementOperator:
     """Owns a pf_op handle; the direct (Phi) and adjoint (Phi^dagger) applies of one operator."""
 
@@ -156,6 +175,8 @@ class MeasurementOperator:
                 raise ValueError(f"input has {x.numel()} entries, expected {nin}")
             if out is None:
                 out = torch.empty(nout, dtype=x.dtype, device=x.device)
+            else:
+                check_device_tensor(out, nout, x.dtype, x.device, "output")
             s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
             _lib.check(fn(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
                           _lib.PF_DEVICE, C.c_void_p(s)))
@@ -177,6 +198,8 @@ class MeasurementOperator:
                 raise ValueError(f"input has {x.numel()} entries, expected {nin}")
             if out is None:
                 out = torch.empty(nout, dtype=x.dtype, device=x.device)
+            else:
+                check_device_tensor(out, nout, x.dtype, x.device, "output")
             s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
             _lib.check(fn(handle, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
                           _lib.PF_DEVICE, C.c_void_p(s)))
@@ -206,9 +229,11 @@ class MeasurementOperator:
         return self._apply(True, y, out, stream)
 
     # -- stage access (device tensors, op precision; no norm)
-    def _stage(self, name, x, nout):
+    def _stage(self, name, x, nin, nout):
         torch = _torch()
-        x = x.contiguous()
+        if not _is_device_tensor(x):
+            raise TypeError("stage access takes a CUDA tensor")
+        x = check_device_tensor(x.contiguous(), nin, self._dtype(), None, "input")
         out = torch.empty(nout, dtype=x.dtype, device=x.device)
         s = torch.cuda.current_stream(x.device).cuda_stream
         _lib.check(getattr(_lib.lib(), name)(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()),
@@ -216,16 +241,18 @@ class MeasurementOperator:
         return out
 
     def apply_G(self, grid):
-        return self._stage("pf_op_apply_G", grid, self.M)
+        ng = self.grid_shape[0] * self.grid_shape[1]
+        return self._stage("pf_op_apply_G", grid, ng, self.M)
 
     def apply_GH(self, y):
-        return self._stage("pf_op_apply_GH", y, self.grid_shape[0] * self.grid_shape[1])
+        return self._stage("pf_op_apply_GH", y, self.M, self.grid_shape[0] * self.grid_shape[1])
 
     def apply_FZ(self, x):
-        return self._stage("pf_op_apply_FZ", x, self.grid_shape[0] * self.grid_shape[1])
+        return self._stage("pf_op_apply_FZ", x, self.imsizex * self.imsizey, self.grid_shape[0] * self.grid_shape[1])
 
     def apply_FZ_adjoint(self, grid):
-        return self._stage("pf_op_apply_FZ_adjoint", grid, self.imsizex * self.imsizey)
+        ng = self.grid_shape[0] * self.grid_shape[1]
+        return self._stage("pf_op_apply_FZ_adjoint", grid, ng, self.imsizex * self.imsizey)
 
 
 class _Adjoint:

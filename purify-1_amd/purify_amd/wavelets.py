@@ -13,6 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
+from .operators import check_device_tensor
 
 
 def parse_wavelet(name):
@@ -83,6 +84,8 @@ class WaveletOperator:
                 raise ValueError(f"input has {x.numel()} entries, expected {nin}")
             if out is None:
                 out = torch.empty(nout, dtype=x.dtype, device=x.device)
+            else:
+                check_device_tensor(out, nout, x.dtype, x.device, "output")
             s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
             _lib.check(fn(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
                           _lib.PF_DEVICE, C.c_void_p(s)))
