@@ -262,6 +262,19 @@ int pf_super_destroy(pf_super *sp);
 int pf_fft2d(int32_t Nv, int32_t Nu, int32_t precision, int32_t sign, const void *in_dev, void *out_dev,
              void *workspace_dev, pf_stream stream);
 
+/* The same transform as pf_fft2d, allocating nothing: no counterpart in the reference.  It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The twiddle table (computed on the device at each call) and the row pass's output
+ * live in a static device array of the library, PF_FFT2D_NOALLOC_MAX_BYTES of workspace plus 128 KiB of
+ * twiddles, which each GPU holds from the first call on it until the process ends; grids larger than
+ * the workspace (Nv * Nu * 8 bytes for PF_C64, * 16 for PF_C128) are rejected with PF_ERR_ARG.  Calls on
+ * one GPU share that array: calls that overlap on one GPU (different streams or threads) race, calls on
+ * one stream are ordered.  Asynchronous on `stream`; in == out is allowed.  Results agree with pf_fft2d
+ * to rounding (the twiddles are FP64 device sincospi values instead of long-double host ones). */
+#define PF_FFT2D_NOALLOC_MAX_BYTES (32u << 20)
+int pf_fft2d_noalloc(int32_t Nv, int32_t Nu, int32_t precision, int32_t sign, const void *in_dev, void *out_dev,
+                     pf_stream stream);
+
 /* Evaluates a gridding kernel (which = 0) or its correction FT (which = 1) on the GPU exactly as the
  * operator's kernels do (kernels.cc:233-323), for n host points; results to host.  For which = 1
  * the argument is the pixel position r and the function is FT(r / ftsize - 0.5). */

@@ -657,7 +657,7 @@ static void twiddle_table(int L, std::vector<double>& re, std::vector<double>& i
   }
 }
 
-void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
+void fft_plan_setup(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
   if (Nv > MAX_AXIS || Nu > MAX_AXIS)
     throw Error(-3, "grid axis larger than " + std::to_string(MAX_AXIS) + " is not supported");
   if ((uint64_t)Nv * Nu * (prec ? 16 : 8) >= ((uint64_t)1 << 32))
@@ -708,6 +708,10 @@ void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
     if (C > Nu) C = Nu;
     p.two_C = C;
   }
+}
+
+void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
+  fft_plan_setup(p, Nv, Nu, prec, col_mode);
   std::vector<double> re, im;
   twiddle_table(p.Lmax, re, im);
   if (prec) {

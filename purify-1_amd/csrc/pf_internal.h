@@ -98,6 +98,8 @@ struct Fft2dPlan {
 // col_mode: 0 = columns in one pass where the LDS allows (up to 8192 rows), 1 = the two-step split
 // beyond 512 rows
 void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode = 0);
+// the plan without its twiddle table (p.tw stays null; the caller provides the table): no allocation
+void fft_plan_setup(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode = 0);
 void fft_plan_free(Fft2dPlan& p);
 
 // Which rows of the grid hold data (forward after the row pass) or are wanted (adjoint crop).

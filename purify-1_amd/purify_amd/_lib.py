@@ -120,6 +120,7 @@ SIGNATURES = {
     "pf_op_apply_GH": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pf_fft2d": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                            C.c_void_p]),
+    "pf_fft2d_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pf_kernel_eval": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_void_p,
                                  C.c_void_p, C.c_int64]),
     "pf_op_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
