@@ -75,6 +75,8 @@ def _declare(L):
         "orc_projection": (None, [i, i, i, i, i, i, d, d, i, i, d, d, d, d, _cp]),
         "orc_csr_adjoint": (None, [i64, i64, _i64p, _i32p, _cp, _i64p, _i32p, _cp]),
         "orc_spmv": (None, [i64, _i64p, _i32p, _cp, _cp, _cp]),
+        "orc_wkernel_eval": (None, [i, d, d, d, d, d, i, i, d, _dp]),
+        "orc_pixel_to_lambda": (d, [d, i, d]),
         "orc_linear_convol_1d": (None, [_cp, i, _cp, i, _cp]),
         "orc_linear_convol_2d": (None, [_cp, i, _cp, i, _cp, i, i, _cp]),
         "orc_generate_chirp": (None, [d, d, d, i, i, _cp]),
@@ -238,6 +240,20 @@ def projection(u, v, w, kind=KB, Jeval_u=4, Ju=4, Jv=4, Jw=6, wkind=W_APPROX, ce
     lib().orc_projection(kind, Jeval_u, Ju, Jv, Jw, wkind, float(cellx), float(celly), nx, ny, float(os_),
                          float(u), float(v), float(w), out)
     return out.reshape(Jv + Jw - 1, Ju + Jw - 1)
+
+
+def wkernel(u, v, w, cellx=1.0, celly=1.0, nx=16, ny=16, os_=2.0, wkind=None):
+    """One w-kernel sample at pixel offsets (u, v) (projection_kernels.cc:8-58); wkind defaults to the
+    approximate w-projection kernel w_projection_kernel_approx (projection_kernels.cc:31-47)."""
+    out = np.zeros(2)
+    lib().orc_wkernel_eval(W_APPROX if wkind is None else wkind, float(u), float(v), float(w), float(cellx),
+                           float(celly), int(nx), int(ny), float(os_), out)
+    return complex(out[0], out[1])
+
+
+def pixel_to_lambda(cell, imsize, os_=2.0):
+    """projection_kernels.cc:28-30: wavelengths per grid pixel."""
+    return float(lib().orc_pixel_to_lambda(float(cell), int(imsize), float(os_)))
 
 
 def linear_convol_1d(f, g_padded):

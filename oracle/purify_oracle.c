@@ -386,6 +386,16 @@ static cplx wkernel(int wkind, double u, double v, double w, double du, double d
   }
 }
 
+/* the w-kernel sample at pixel offsets (u, v) (projection_kernels.cc:8-58) with the pixel -> lambda
+ * scales of an imsizex x imsizey image at oversampling os (pixel_to_lambda, projection_kernels.cc:28-30) */
+void orc_wkernel_eval(int wkind, double u, double v, double w, double cellx, double celly, int nx, int ny,
+                      double os, double *out) {
+  const cplx c = wkernel(wkind, u, v, w, pixel_to_lambda(cellx, nx, os), pixel_to_lambda(celly, ny, os));
+  out[0] = creal(c);
+  out[1] = cimag(c);
+}
+double orc_pixel_to_lambda(double cell, int imsize, double os) { return pixel_to_lambda(cell, imsize, os); }
+
 /* convolution.h:43-52 */
 void orc_linear_convol_1d(const double *f, int Jf, const double *g, int Lg, double *out) {
   const cplx *fc = (const cplx *)f, *gc = (const cplx *)g;

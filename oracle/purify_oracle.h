@@ -81,6 +81,11 @@ void orc_spmv(int64_t rows, const int64_t *rowptr, const int32_t *colidx, const 
               const double *x, double *y);
 
 /* ---- convolution helpers (cpp/purify/convolution.h) ---- */
+/* one w-kernel sample (projection_kernels.cc:8-58) at pixel offsets (u, v), w in wavelengths; the pixel ->
+ * lambda scales are pixel_to_lambda(cell, imsize, os) (projection_kernels.cc:28-30) */
+void orc_wkernel_eval(int wkind, double u, double v, double w, double cellx, double celly, int nx, int ny,
+                      double os, double *out);
+double orc_pixel_to_lambda(double cell, int imsize, double os);
 /* linear_convol_1d: f of length Jf, g already zero padded of length Lg; out length Lg-Jf+1. */
 void orc_linear_convol_1d(const double *f, int Jf, const double *g, int Lg, double *out);
 /* linear_convol_2d(kernelfu, kernelfv, kernelg): g is Jgv x Jgu row-major; out is

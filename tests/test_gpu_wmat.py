@@ -128,3 +128,23 @@ def test_operator_on_the_chirp_matrix(precision):
     assert abs(lhs - rhs) / abs(lhs) < tol
     # the normalised operator has norm 1 within the power method's tolerance (cpp/tests/operators.cc:147-154)
     assert abs(lt.op.power_method(500, 1e-8) - 1.0) < 1e-4
+
+
+def test_chirp_rows_apply_the_exact_w_screen_to_a_point_source():
+    # Independent of the oracle: with both energy fractions 1 and rows of G holding only the zero-frequency
+    # cell, Phi_w delta_b = exp(-2 pi i w (sqrt(1 - X^2 - Y^2) - 1)) Phi delta_b at the direction cosines
+    # X = (b - N/2 - 1/2) L / N, L = 2 sin(N cell / 2), the chirp grid of wproj_utilities.cc:37-75 seen
+    # through the circular row convolution (the same identity pins the oracle in test_oracle_wproj_pins.py).
+    N, cell = 64, 600.0
+    w = np.array([500.0, -300.0, 50.0, 1e3])
+    M = len(w)
+    G = sp.csr_matrix((np.ones(M, np.complex128), (np.arange(M), np.zeros(M, int))), shape=(M, N * N))
+    GW = wproj_utilities.wprojection_matrix(G, N, N, w, cell, cell, 1.0, 1.0).toarray()
+    L = 2 * np.sin(cell * N * np.pi / (180 * 3600) / 2)
+    for bp, bq in ((21, 39), (10, 50), (33, 31), (40, 12), (5, 60)):
+        z = np.zeros((N, N), np.complex128)
+        z[bp, bq] = 1.0
+        g = (np.fft.fft2(z) / N).ravel()  # the unitary forward FFT of the grid
+        X, Y = (bq - N / 2 - 0.5) * L / N, (bp - N / 2 - 0.5) * L / N
+        want = np.exp(-2j * np.pi * w * (np.sqrt(1 - X * X - Y * Y) - 1)) / N
+        np.testing.assert_allclose(GW @ g, want, rtol=1e-12, atol=0)
