@@ -201,21 +201,15 @@ def multichannel_run(n_channels, iters, seed, log):
     import numpy as np
     import torch
     from purify_amd import algorithms, measurementoperator, utilities
-    rng = np.random.default_rng(seed)
     chans, x0s = [], []
     t0 = time.perf_counter()
-    for c in range(n_channels):
-        n = int(rng.choice([256, 512, 1024]))
-        M = int(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
-        sig = float(rng.uniform(0.5, 1.2))
-        u, v = utilities.synth_uv(seed + 7 * c, M, sig)
-        u, v = utilities.radians_to_pixels(u, n), utilities.radians_to_pixels(v, n)
-        x0 = utilities.synth_image(seed + 7 * c + 1, n, n, 20)
+    for ch in utilities.synth_channels(seed, n_channels):
+        n, u, v, x0 = ch["n"], ch["u"], ch["v"], ch["x0"]
         lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 50, 1e-6, "kb", 4, 4,
                                                          precision="c64")
         y0 = lt * x0
         sigma = utilities.SNR_to_standard_deviation(y0, 30.0)
-        chans.append(dict(u=u, v=v, y=utilities.add_noise(y0, sigma, seed + 7 * c + 2), imsizey=n, imsizex=n,
+        chans.append(dict(u=u, v=v, y=utilities.add_noise(y0, sigma, ch["seed"] + 2), imsizey=n, imsizex=n,
                           sigma=sigma, operator=lt))
         x0s.append(x0)
     mc = algorithms.MultiChannelPADMM(chans, itermax=iters)

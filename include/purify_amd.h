@@ -74,8 +74,9 @@ typedef struct pf_op_desc {
 void pf_op_desc_init(pf_op_desc *d);
 
 /* Build Phi: copies and sorts the coordinates, uploads them, builds twiddles and the correction
- * vectors, then (power_iters > 0) normalises by the power-method norm computed on the GPU with
- * FP64 reductions.  Replaces measurementoperator::init_degrid_operator_2d (operators.h:373-414)
+ * vectors, then (power_iters > 0) normalises by the power-method norm computed on the GPU in FP64 (a
+ * complex64 operator takes the norm of its complex128 twin, built from the same inputs for the iteration
+ * and released, so both precisions get the same norm).  Replaces measurementoperator::init_degrid_operator_2d (operators.h:373-414)
  * and factory::measurement_operator_factory's gpu_serial branch (measurement_operator_factory.h:
  * 38-96). */
 int pf_op_create(const pf_op_desc *desc, pf_op **out);

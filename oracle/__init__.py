@@ -76,6 +76,8 @@ def _declare(L):
         "orc_csr_adjoint": (None, [i64, i64, _i64p, _i32p, _cp, _i64p, _i32p, _cp]),
         "orc_spmv": (None, [i64, _i64p, _i32p, _cp, _cp, _cp]),
         "orc_wkernel_eval": (None, [i, d, d, d, d, d, i, i, d, _dp]),
+        "orc_dwt2": (None, [_cp, i, i, _dp, i, i, _cp]),
+        "orc_idwt2": (None, [_cp, i, i, _dp, i, i, _cp]),
         "orc_pixel_to_lambda": (d, [d, i, d]),
         "orc_linear_convol_1d": (None, [_cp, i, _cp, i, _cp]),
         "orc_linear_convol_2d": (None, [_cp, i, _cp, i, _cp, i, i, _cp]),
@@ -475,27 +477,37 @@ def daubechies(N):
 
 
 def _dwt_line(x, h):
+    # lo[i] = sum_k h[k] x[(2i + k) mod n], hi with g; x[(2i + k) mod n] is x2[(i + k // 2) mod half, k % 2]
+    # for the pairs x2 = x viewed as (half, 2)
     n, L = x.shape[-1], len(h)
     g = np.array([(-1) ** k * h[L - 1 - k] for k in range(L)])
     half = n // 2
-    idx = (2 * np.arange(half)[:, None] + np.arange(L)[None, :]) % n
-    xs = x[..., idx]  # (..., half, L)
-    return np.concatenate([xs @ h, xs @ g], axis=-1)
+    x2 = x.reshape(x.shape[:-1] + (half, 2))
+    lo = np.zeros(x.shape[:-1] + (half,), np.result_type(x, h))
+    hi = np.zeros_like(lo)
+    for k in range(L):
+        xs = np.roll(x2[..., k % 2], -(k // 2), axis=-1)
+        lo += h[k] * xs
+        hi += g[k] * xs
+    return np.concatenate([lo, hi], axis=-1)
 
 
 def _idwt_line(c, h):
     n, L = c.shape[-1], len(h)
     g = np.array([(-1) ** k * h[L - 1 - k] for k in range(L)])
     half = n // 2
-    x = np.zeros_like(c)
-    for i in range(half):
-        for k in range(L):
-            j = (2 * i + k) % n
-            x[..., j] += h[k] * c[..., i] + g[k] * c[..., half + i]
+    x = np.zeros(c.shape, np.result_type(c, h))  # C-contiguous, so x2 below is a view of it
+    x2 = x.reshape(x.shape[:-1] + (half, 2))
+    lo, hi = c[..., :half], c[..., half:]
+    for k in range(L):
+        # x[(2i + k) mod n] += h[k] lo[i] + g[k] hi[i], i.e. pair (i + k // 2) mod half, slot k % 2
+        x2[..., k % 2] += np.roll(h[k] * lo + g[k] * hi, k // 2, axis=-1)
     return x
 
 
-def dwt2(img, ny, nx, h, levels):
+def dwt2_numpy(img, ny, nx, h, levels):
+    """The periodic 2-D analysis in numpy (rows, then columns, of the shrinking top-left block per level);
+    dwt2 runs the same in the C restatement (orc_dwt2), and the tests hold the two together."""
     c = np.asarray(img, np.complex128).reshape(ny, nx).copy()
     for l in range(levels):
         hh, ww = ny >> l, nx >> l
@@ -504,13 +516,27 @@ def dwt2(img, ny, nx, h, levels):
     return c.ravel()
 
 
-def idwt2(coef, ny, nx, h, levels):
+def idwt2_numpy(coef, ny, nx, h, levels):
     c = np.asarray(coef, np.complex128).reshape(ny, nx).copy()
     for l in range(levels - 1, -1, -1):
         hh, ww = ny >> l, nx >> l
         blk = _idwt_line(c[:hh, :ww].T, h).T
         c[:hh, :ww] = _idwt_line(blk, h)
     return c.ravel()
+
+
+def dwt2(img, ny, nx, h, levels):
+    h = np.ascontiguousarray(h, np.float64)
+    out = np.zeros(ny * nx, np.complex128)
+    lib().orc_dwt2(_c(img), int(ny), int(nx), h, len(h), int(levels), out)
+    return out
+
+
+def idwt2(coef, ny, nx, h, levels):
+    h = np.ascontiguousarray(h, np.float64)
+    out = np.zeros(ny * nx, np.complex128)
+    lib().orc_idwt2(_c(coef), int(ny), int(nx), h, len(h), int(levels), out)
+    return out
 
 
 def sara_analysis(img, ny, nx, entries):

@@ -685,6 +685,84 @@ int orc_wprojection_matrix_dense(int64_t rows, const int64_t *rowptr, const int3
   return 0;
 }
 
+/* ---- periodic 2-D wavelet transform (the SARA dictionary's Daubechies bases) ----
+ * One line: lo[i] = sum_k h[k] x[(2i + k) mod n], hi[i] = sum_k g[k] x[(2i + k) mod n] with
+ * g[k] = (-1)^k h[L-1-k]; low-pass half first.  The inverse is the transpose.  A level filters the rows
+ * of the top-left (ny >> l) x (nx >> l) block, then its columns; the inverse undoes the columns first. */
+static void dwt_line(const cplx *x, int n, int stride, const double *h, const double *g, int L, cplx *out) {
+  const int half = n / 2;
+  for (int i = 0; i < half; ++i) {
+    cplx lo = 0, hi = 0;
+    for (int k = 0; k < L; ++k) {
+      const cplx a = x[(int64_t)((2 * i + k) % n) * stride];
+      lo += h[k] * a;
+      hi += g[k] * a;
+    }
+    out[i] = lo;
+    out[half + i] = hi;
+  }
+}
+static void idwt_line(const cplx *c, int n, int stride, const double *h, const double *g, int L, cplx *out) {
+  const int half = n / 2;
+  for (int j = 0; j < n; ++j) out[j] = 0;
+  for (int i = 0; i < half; ++i) {
+    const cplx lo = c[(int64_t)i * stride], hi = c[(int64_t)(half + i) * stride];
+    for (int k = 0; k < L; ++k) out[(2 * i + k) % n] += h[k] * lo + g[k] * hi;
+  }
+}
+
+void orc_dwt2(const double *img, int ny, int nx, const double *h, int L, int levels, double *out) {
+  cplx *c = (cplx *)out;
+  const cplx *x = (const cplx *)img;
+  double g[64];
+  for (int k = 0; k < L; ++k) g[k] = ((k % 2) ? -1.0 : 1.0) * h[L - 1 - k];
+  for (int64_t t = 0; t < (int64_t)ny * nx; ++t) c[t] = x[t];
+  for (int l = 0; l < levels; ++l) {
+    const int hh = ny >> l, ww = nx >> l;
+#pragma omp parallel
+    {
+      cplx *tmp = malloc(sizeof(cplx) * (size_t)(hh > ww ? hh : ww));
+#pragma omp for schedule(static)
+      for (int r = 0; r < hh; ++r) {
+        dwt_line(c + (int64_t)r * nx, ww, 1, h, g, L, tmp);
+        for (int i = 0; i < ww; ++i) c[(int64_t)r * nx + i] = tmp[i];
+      }
+#pragma omp for schedule(static)
+      for (int q = 0; q < ww; ++q) {
+        dwt_line(c + q, hh, nx, h, g, L, tmp);
+        for (int i = 0; i < hh; ++i) c[(int64_t)i * nx + q] = tmp[i];
+      }
+      free(tmp);
+    }
+  }
+}
+
+void orc_idwt2(const double *coef, int ny, int nx, const double *h, int L, int levels, double *out) {
+  cplx *c = (cplx *)out;
+  const cplx *x = (const cplx *)coef;
+  double g[64];
+  for (int k = 0; k < L; ++k) g[k] = ((k % 2) ? -1.0 : 1.0) * h[L - 1 - k];
+  for (int64_t t = 0; t < (int64_t)ny * nx; ++t) c[t] = x[t];
+  for (int l = levels - 1; l >= 0; --l) {
+    const int hh = ny >> l, ww = nx >> l;
+#pragma omp parallel
+    {
+      cplx *tmp = malloc(sizeof(cplx) * (size_t)(hh > ww ? hh : ww));
+#pragma omp for schedule(static)
+      for (int q = 0; q < ww; ++q) {
+        idwt_line(c + q, hh, nx, h, g, L, tmp);
+        for (int i = 0; i < hh; ++i) c[(int64_t)i * nx + q] = tmp[i];
+      }
+#pragma omp for schedule(static)
+      for (int r = 0; r < hh; ++r) {
+        idwt_line(c + (int64_t)r * nx, ww, 1, h, g, L, tmp);
+        for (int i = 0; i < ww; ++i) c[(int64_t)r * nx + i] = tmp[i];
+      }
+      free(tmp);
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Utilities                                                                                   */
 /* ------------------------------------------------------------------------------------------ */

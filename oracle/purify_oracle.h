@@ -81,6 +81,10 @@ void orc_spmv(int64_t rows, const int64_t *rowptr, const int32_t *colidx, const 
               const double *x, double *y);
 
 /* ---- convolution helpers (cpp/purify/convolution.h) ---- */
+/* periodic 2-D Daubechies analysis / synthesis of an ny x nx complex image (levels levels, filter h of
+ * length L <= 64): the SARA dictionary's bases (wavelet_operator_factory.h:20-45) */
+void orc_dwt2(const double *img, int ny, int nx, const double *h, int L, int levels, double *out);
+void orc_idwt2(const double *coef, int ny, int nx, const double *h, int L, int levels, double *out);
 /* one w-kernel sample (projection_kernels.cc:8-58) at pixel offsets (u, v), w in wavelengths; the pixel ->
  * lambda scales are pixel_to_lambda(cell, imsize, os) (projection_kernels.cc:28-30) */
 void orc_wkernel_eval(int wkind, double u, double v, double w, double cellx, double celly, int nx, int ny,

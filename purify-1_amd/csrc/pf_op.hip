@@ -1278,6 +1278,21 @@ void pf_op_desc_init(pf_op_desc* d) {
   d->power_tol = 1e-4;
 }
 
+// The normalisation's power method (operators.h:390-391) runs in FP64 whatever the operator's precision:
+// a complex64 operator is normalised by the power iteration of its complex128 twin (the same inputs, built
+// in complex128 for the iteration and released), so both precisions get the same norm bit for bit.
+extern "C++" {
+template <class Build>
+static double fp64_norm(const pf_op_desc& desc, Build&& build_fn) {
+  pf_op_desc d = desc;
+  d.precision = PF_C128;
+  d.power_iters = 0;
+  std::unique_ptr<pf_op> twin(new pf_op());
+  build_fn(twin.get(), d);
+  return power_method_t<double>(twin.get(), desc.power_iters, desc.power_tol, nullptr);
+}
+}
+
 int pf_op_create(const pf_op_desc* desc, pf_op** out) {
   if (!desc || !out) return fail(PF_ERR_ARG, "null argument");
   *out = nullptr;
@@ -1285,7 +1300,7 @@ int pf_op_create(const pf_op_desc* desc, pf_op** out) {
   int rc = guarded([&] {
     build(op.get(), *desc);
     if (desc->power_iters > 0) {
-      const double nrm = op->prec == PF_C64 ? power_method_t<float>(op.get(), desc->power_iters, desc->power_tol, nullptr)
+      const double nrm = op->prec == PF_C64 ? fp64_norm(*desc, [](pf_op* t, const pf_op_desc& d) { build(t, d); })
                                             : power_method_t<double>(op.get(), desc->power_iters, desc->power_tol, nullptr);
       if (!(nrm > 0)) throw Error(PF_ERR_ARG, "Operator norm is not greater than zero.");  // operators.h:187
       op->norm = nrm;
@@ -1302,8 +1317,9 @@ int pf_op_create_csr(const pf_op_desc* desc, const pf_csr* G, pf_op** out) {
   int rc = guarded([&] {
     build_csr(op.get(), *desc, *G);
     if (desc->power_iters > 0) {
-      const double nrm = op->prec == PF_C64 ? power_method_t<float>(op.get(), desc->power_iters, desc->power_tol, nullptr)
-                                            : power_method_t<double>(op.get(), desc->power_iters, desc->power_tol, nullptr);
+      const double nrm =
+          op->prec == PF_C64 ? fp64_norm(*desc, [G](pf_op* t, const pf_op_desc& d) { build_csr(t, d, *G); })
+                             : power_method_t<double>(op.get(), desc->power_iters, desc->power_tol, nullptr);
       if (!(nrm > 0)) throw Error(PF_ERR_ARG, "Operator norm is not greater than zero.");  // operators.h:187
       op->norm = nrm;
     }

@@ -98,6 +98,24 @@ def add_noise(y0, sigma, seed):
     return np.asarray(y0) + synth_cnormal(seed, np.asarray(y0).size, sigma)
 
 
+def synth_channels(seed, n_channels, sizes=(256, 512, 1024), M_range=(1e4, 1e6), sigma_range=(0.5, 1.2)):
+    """The batched multi-channel workload: per channel an image side drawn from `sizes`, M log-uniform over
+    M_range, uv Gaussian with sigma uniform over sigma_range (rad, clipped to [-pi, pi]) in pixels of the
+    oversample-2 grid, and a 20-source image.  Returns dicts with n, M, uv_sigma, u, v, x0 and the
+    channel's own seed (seed + 7 c; +2 is its noise seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(n_channels):
+        n = int(rng.choice(list(sizes)))
+        M = int(np.exp(rng.uniform(np.log(M_range[0]), np.log(M_range[1]))))
+        sig = float(rng.uniform(*sigma_range))
+        cs = seed + 7 * c
+        u, v = synth_uv(cs, M, sig)
+        out.append(dict(n=n, M=M, uv_sigma=sig, u=radians_to_pixels(u, n), v=radians_to_pixels(v, n),
+                        x0=synth_image(cs + 1, n, n, 20), seed=cs))
+    return out
+
+
 def read_visibility(vis_name, w_term=False):
     """utilities.cc:140-183: a whitespace-separated text file of rows u v [w] re(V) im(V) sigma (nan and inf
     allowed).  Returns vis_params with v negated (the reference's reflection for the image orientation),

@@ -65,3 +65,17 @@ def test_factory_errors():
     assert wavelets.parse_wavelet("Dirac") == 0 and wavelets.parse_wavelet("DB8") == 8
     with pytest.raises(RuntimeError, match="MPI"):
         wavelets.wavelet_operator_factory("mpi_sara", [("DB4", 3)], 16, 16)
+
+
+@pytest.mark.parametrize("order,levels,ny,nx", [(1, 1, 8, 8), (4, 3, 64, 32), (8, 2, 16, 64), (10, 4, 128, 128),
+                                                (2, 5, 64, 64)])
+def test_oracle_c_transform_equals_the_numpy_restatement(order, levels, ny, nx):
+    # the C lines (orc_dwt2 / orc_idwt2) and the numpy lines compute the same periodic transform
+    rng = np.random.default_rng(order * 100 + levels)
+    x = rng.normal(size=ny * nx) + 1j * rng.normal(size=ny * nx)
+    h = oracle.daubechies(order)
+    a, b = oracle.dwt2(x, ny, nx, h, levels), oracle.dwt2_numpy(x, ny, nx, h, levels)
+    assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+    c, d = oracle.idwt2(x, ny, nx, h, levels), oracle.idwt2_numpy(x, ny, nx, h, levels)
+    assert np.abs(c - d).max() < 1e-13 * np.abs(d).max()
+    assert np.abs(oracle.idwt2(a, ny, nx, h, levels) - x).max() < 1e-12
