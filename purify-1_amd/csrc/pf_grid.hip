@@ -478,7 +478,8 @@ __global__ void __launch_bounds__(64) k_grid_tiles(GridGeom<R> g, VisData<R> v, 
 // in LDS: the footprint offset (dx, dy), the J_v products s wv and the J_u taps wu.  Each MFMA 16x16x4
 // step then adds four entries: lane l gathers A[l % 16][l / 16] and B[l / 16][l % 16] from the records
 // of entry 4k + l / 16 (zero outside the footprint) and holds four result rows of column l % 16:
-// 4 (l / 16) + r for the FP32 instruction, l / 16 + 4 r for the FP64 one (r < 4).  The summation
+// 4 (l / 16) + r for the FP32 instruction, l / 16 + 4 r for the FP64 one (r < 4).  complex64 sums are
+// compensated per tile (the accumulators fold into a TwoSum running sum every MFMA_COMP batches).  The summation
 // order is fixed by the list and the instruction: reproducible bit for bit.  Needs N >= 16 + J - 1 on
 // both axes, so that a footprint meets a tile in one piece.
 // ---------------------------------------------------------------------------------------------
@@ -499,7 +500,23 @@ struct Mfma<double> {
   __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
+// hi += b by Knuth's TwoSum (no ordering of |hi| and |b| assumed); returns the rounding error of the
+// addition exactly, hi + b = (new hi) + error
+__device__ inline float two_sum(float& hi, float b) {
+  const float s = hi + b;
+  const float bb = s - hi;
+  const float e = (hi - (s - bb)) + (b - bb);
+  hi = s;
+  return e;
+}
+
 constexpr int MB = 64;           // list entries per batch (a multiple of 16)
+// complex64 compensated tile sums: the MFMA accumulators sum up to MFMA_COMP batches (256 entries: at
+// most 64 rounded accumulations per cell); then their partial sum is added to the tile's running sum
+// hi by TwoSum, and the accumulators restart from that addition's rounding error, which the next
+// period's MFMAs carry (Kahan's compensated summation with the accumulator as the compensation).  A
+// cell's error is then bounded by one period's roundings, not by the length of its tile's list.
+constexpr int MFMA_COMP = 4;
 constexpr int MQ = MB / 4;       // entries per MFMA row group of a batch
 constexpr int MFMA_NOTAP = 1024; // footprint offset of an unused entry slot: no row or column matches
 
@@ -609,6 +626,10 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
     }
   }
   acc_t cre = {R(0), R(0), R(0), R(0)}, cim = {R(0), R(0), R(0), R(0)};
+  // complex64: the tile's compensated running sums (see MFMA_COMP)
+  constexpr bool COMP = sizeof(R) == 4;
+  float hre[4] = {0.f, 0.f, 0.f, 0.f}, him[4] = {0.f, 0.f, 0.f, 0.f};
+  int nacc = 0;  // batches summed in cre / cim since the last fold
   while (p0.tile >= 0) {
     const int tx0 = (p0.tile % tl.ntu) * T, ty0 = (p0.tile / tl.ntu) * T;
     const int nb = (int)min((uint32_t)MB, p0.end - p0.base);  // 0 for an empty tile
@@ -685,13 +706,33 @@ __global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, T
       }
     }
     __syncthreads();
-    if (p1.tile != p0.tile) {  // the tile's last batch: write its cells, start the next tile at zero
+    const bool tile_done = p1.tile != p0.tile;
+    if constexpr (COMP) {
+      if (++nacc == MFMA_COMP && !tile_done) {  // fold: hi += partial sum; carry the rounding error
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          cre[r] = two_sum(hre[r], (float)cre[r]);
+          cim[r] = two_sum(him[r], (float)cim[r]);
+        }
+        nacc = 0;
+      }
+    }
+    if (tile_done) {  // the tile's last batch: write its cells, start the next tile at zero
       const int col = lane & 15;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        grid[(int64_t)(ty0 + Mfma<R>::row(lane, r)) * g.Nu + tx0 + col] = mk<R>(cre[r], cim[r]);
+      for (int r = 0; r < 4; ++r) {
+        cpx<R> val;
+        if constexpr (COMP) {
+          val = mk<R>((R)(hre[r] + (float)cre[r]), (R)(him[r] + (float)cim[r]));
+          hre[r] = him[r] = 0.f;
+        } else {
+          val = mk<R>(cre[r], cim[r]);
+        }
+        grid[(int64_t)(ty0 + Mfma<R>::row(lane, r)) * g.Nu + tx0 + col] = val;
+      }
       cre = acc_t{R(0), R(0), R(0), R(0)};
       cim = cre;
+      nacc = 0;
     }
     p0 = p1;
     p1 = p2;

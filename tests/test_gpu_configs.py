@@ -60,6 +60,24 @@ def test_2048_m4e6_both_precisions_and_dot_test():
     full_size_parity(2048, 4_000_000, 6, 1002)
 
 
+def test_dense_centre_adjoint_with_compensated_tile_sums():
+    # the worst case for FP32 accumulation: 4e6 visibilities concentrated at the centre (uv sigma 0.2 rad,
+    # about 38 visibilities per cell at the peak, so the central 16 x 16 tiles sum about 10^4 list
+    # entries each); the complex64 tile sums are compensated, so the adjoint stays at FP32 rounding
+    n, M, J = 2048, 4_000_000, 6
+    u, v = make_uv(1003, M, n, sigma=0.2)
+    y = utilities.synth_cnormal(1004, M)
+    orc = oracle.Operator(u, v, None, None, n, n, 2.0, oracle.KB, J, J)
+    xo = orc.adjoint(y)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", J, J,
+                                                     precision="c64")
+    info = lt.op.info()
+    assert info.tile_list_len > M
+    err = rel(lt.adjoint() * y, xo)
+    print(f"dense centre adjoint, complex64: rel. error {err:.2e}", flush=True)
+    assert err < 1e-6
+
+
 # ---- 64 channels -----------------------------------------------------------------------------------------
 
 CH_SEED, N_CH, ITERS = 1022, 64, 100
