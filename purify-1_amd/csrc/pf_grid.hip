@@ -894,6 +894,14 @@ __global__ void __launch_bounds__(256) k_permute_scatter(const cpx<R>* __restric
   if (m < M) ys[inv[m]] = y[m];
 }
 
+// one-pass inverse: y[m] = ys[inv[m]] (scattered reads, coalesced writes)
+template <typename R>
+__global__ void __launch_bounds__(256) k_permute_gather(const cpx<R>* __restrict__ ys, const int32_t* __restrict__ inv,
+                                                        cpx<R>* __restrict__ y, int64_t M) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < M) y[m] = ys[inv[m]];
+}
+
 constexpr int PERM_THREADS = 1024;
 template <typename R>
 constexpr int perm_per_thread() { return (sizeof(R) == 4 ? 16384 : 8192) / PERM_THREADS; }
@@ -1033,9 +1041,14 @@ void permute_to_sorted(const VisPerm& p, const cpx<R>* y, cpx<R>* tmp, cpx<R>* y
 template <typename R>
 void permute_to_caller(const VisPerm& p, const cpx<R>* ys, cpx<R>* tmp, cpx<R>* y, hipStream_t s) {
   if (p.M == 0) return;
-  if (!p.two_pass) throw Error(-5, "permute_to_caller needs the two-pass plan");  // PF_ERR_STATE
-  check_perm<R>(p);
   ProfScope prof("permute_to_caller", s);
+  if (!p.two_pass) {
+    if (!p.inv) throw Error(-5, "one-pass permutation without its index");  // PF_ERR_STATE
+    hipLaunchKernelGGL((k_permute_gather<R>), dim3((unsigned)((p.M + 255) / 256)), dim3(256), 0, s, ys, p.inv, y, p.M);
+    PF_HIP(hipGetLastError());
+    return;
+  }
+  check_perm<R>(p);
   const size_t shm = sizeof(cpx<R>) * (size_t)p.S;
   allow_lds((const void*)k_perm_chunks<R, false>, shm);
   allow_lds((const void*)k_perm_buckets<R, false>, shm);

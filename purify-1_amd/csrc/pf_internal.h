@@ -235,7 +235,8 @@ template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm,
                 const cpx<R>* y_sorted, cpx<R>* grid, R scale, hipStream_t s, int variant, int mfma_waves = 0);
 // caller's order -> sorted order (ys) and back; the two-pass plan stages through `tmp` (M values).
-// permute_to_caller needs the two-pass plan (the one-pass plan's degridding writes the caller's order).
+// With the one-pass plan permute_to_caller gathers y[m] = ys[inv[m]] (the operator's own applies do not
+// use it: their degridding writes the caller's order directly).
 template <typename R>
 void permute_to_sorted(const VisPerm& p, const cpx<R>* y, cpx<R>* tmp, cpx<R>* ys, hipStream_t s);
 template <typename R>
@@ -320,6 +321,17 @@ void op_adjoint(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s);
 // out = Phi^dagger Phi x (both normalised), without the two permutations
 template <typename R>
 void op_normal(pf_op* op, const cpx<R>* x, cpx<R>* out, hipStream_t s);
+// y = Phi x and x = Phi^dagger y with the visibility vector in the operator's sorted order (no permutation;
+// for an operator on a given matrix the sorted order is the caller's)
+template <typename R>
+void op_forward_sorted(pf_op* op, const cpx<R>* x, cpx<R>* ys, hipStream_t s);
+template <typename R>
+void op_adjoint_sorted(pf_op* op, const cpx<R>* ys, cpx<R>* x, hipStream_t s);
+// a visibility vector from the caller's order to the sorted order and back (y and ys distinct)
+template <typename R>
+void op_to_sorted(pf_op* op, const cpx<R>* y, cpx<R>* ys, hipStream_t s);
+template <typename R>
+void op_to_caller(pf_op* op, const cpx<R>* ys, cpx<R>* y, hipStream_t s);
 struct OpShape {
   int ny, nx, prec;
   int64_t M;

@@ -1083,6 +1083,26 @@ void GH_dev(pf_op* op, const cpx<R>* y, cpx<R>* grid, hipStream_t s) {
                      op->grid_waves);
 }
 
+// y = G grid and grid = G^H y with y in the operator's sorted order (the caller's for a given matrix)
+template <typename R>
+void G_sorted_dev(pf_op* op, const cpx<R>* grid, cpx<R>* ys, hipStream_t s) {
+  if (op->g_csr) {
+    G_dev<R>(op, grid, ys, s);
+  } else if (op->w_term) {
+    degrid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), nullptr, op->tl, grid, ys, R(1), s);
+  } else {
+    VisData<R> v = visdata<R>(op);
+    v.perm = nullptr;  // sorted order out
+    degrid<R>(degrid_geom<R>(op), v, op->tl, grid, ys, R(1), s);
+  }
+}
+template <typename R>
+void GH_sorted_dev(pf_op* op, const cpx<R>* ys, cpx<R>* grid, hipStream_t s) {
+  if (op->g_csr) GH_dev<R>(op, ys, grid, s);
+  else if (op->w_term) grid_wrows<R>(op->Nu, op->Nv, wrows<R>(op), op->tl, ys, grid, R(1), s);
+  else grid_tiles<R>(geom<R>(op), visdata<R>(op), op->tl, op->tlm, ys, grid, R(1), s, op->grid_variant, op->grid_waves);
+}
+
 template <typename R>
 void forward_dev(pf_op* op, const cpx<R>* x, cpx<R>* y, hipStream_t s) {
   const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
@@ -1249,6 +1269,38 @@ template <typename R>
 void op_adjoint(pf_op* op, const cpx<R>* y, cpx<R>* x, hipStream_t s) {
   adjoint_dev<R>(op, y, x, s);
 }
+template <typename R>
+void op_forward_sorted(pf_op* op, const cpx<R>* x, cpx<R>* ys, hipStream_t s) {
+  const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
+  cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
+  fz_forward<R>(op, x, B, scl, s);
+  G_sorted_dev<R>(op, B, ys, s);
+}
+template <typename R>
+void op_adjoint_sorted(pf_op* op, const cpx<R>* ys, cpx<R>* x, hipStream_t s) {
+  const double scl = 1.0 / (std::sqrt((double)op->ngrid()) * op->norm);
+  cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
+  GH_sorted_dev<R>(op, ys, A, s);
+  fz_adjoint<R>(op, A, x, scl, s);
+}
+template <typename R>
+void op_to_sorted(pf_op* op, const cpx<R>* y, cpx<R>* ys, hipStream_t s) {
+  if (op->g_csr) PF_HIP(hipMemcpyAsync(ys, y, sizeof(cpx<R>) * op->M, hipMemcpyDeviceToDevice, s));
+  else permute_to_sorted<R>(op->vperm, y, static_cast<cpx<R>*>(op->d_ystage), ys, s);
+}
+template <typename R>
+void op_to_caller(pf_op* op, const cpx<R>* ys, cpx<R>* y, hipStream_t s) {
+  if (op->g_csr) PF_HIP(hipMemcpyAsync(y, ys, sizeof(cpx<R>) * op->M, hipMemcpyDeviceToDevice, s));
+  else permute_to_caller<R>(op->vperm, ys, static_cast<cpx<R>*>(op->d_ystage), y, s);
+}
+template void op_forward_sorted<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void op_forward_sorted<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
+template void op_adjoint_sorted<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void op_adjoint_sorted<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
+template void op_to_sorted<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void op_to_sorted<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
+template void op_to_caller<float>(pf_op*, const cpx<float>*, cpx<float>*, hipStream_t);
+template void op_to_caller<double>(pf_op*, const cpx<double>*, cpx<double>*, hipStream_t);
 template <typename R>
 void op_normal(pf_op* op, const cpx<R>* x, cpx<R>* out, hipStream_t s) {
   normal_dev<R>(op, x, out, s);

@@ -424,7 +424,7 @@ void padmm_step(pf_padmm* h) {
   // fx holds r + lambda + z until the forward apply below overwrites it
   hipLaunchKernelGGL((k_ball_out<R>), vgrid(M), dim3(256), 0, s, h->cur + 3, h->par.epsilon, V<R>(h->r), V<R>(h->lam),
                      V<R>(h->z), V<R>(h->fx), M);
-  op_adjoint<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
+  op_adjoint_sorted<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
   if (h->tight) {
     // x keeps its old value until the proximal output replaces it, so no copy of x_prev is made
     hipLaunchKernelGGL((k_grad_step<R, false>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev),
@@ -443,7 +443,7 @@ void padmm_step(pf_padmm* h) {
                        V<R>(h->x), V<R>(h->xprev), inv_nu, N, h->fpart);
   }
   hipLaunchKernelGGL(k_fr_final, dim3(2), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 0);
-  op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
+  op_forward_sorted<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
   hipLaunchKernelGGL((k_residual_dual_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->fx), V<R>(h->y),
                      V<R>(h->z), V<R>(h->r), V<R>(h->lam), (R)h->par.lagrange_update_scale, M, h->fpart);
   hipLaunchKernelGGL(k_fr_final, dim3(2), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 2);
@@ -506,7 +506,7 @@ template <typename R>
 void padmm_init(pf_padmm* h) {
   hipStream_t s = h->stream;
   const int64_t N = h->N, M = h->M;
-  op_adjoint<R>(h->phi, V<R>(h->y), V<R>(h->g), s);
+  op_adjoint_sorted<R>(h->phi, V<R>(h->y), V<R>(h->g), s);
   if (!(h->par.gamma > 0)) {
     // algorithm_factory.h:70-73: gamma = max |Psi^dagger Phi^dagger y| * 1e-3
     wavelet_analysis<R>(h->psi, V<R>(h->g), V<R>(h->c), s);
@@ -520,7 +520,7 @@ void padmm_init(pf_padmm* h) {
     h->gamma = h->par.gamma;
   }
   hipLaunchKernelGGL((k_scale_copy<R>), vgrid(N), dim3(256), 0, s, V<R>(h->g), V<R>(h->x), (R)(1.0 / h->par.nu), N);
-  op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
+  op_forward_sorted<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
   hipLaunchKernelGGL((k_sub<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->r), M);
   hipLaunchKernelGGL((k_zero<R>), vgrid(M), dim3(256), 0, s, V<R>(h->lam), M);
   hipLaunchKernelGGL((k_ball_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->lam), V<R>(h->r), M, h->fpart);
@@ -550,8 +550,8 @@ void padmm_capture(pf_padmm* h) {
     if (h->fb) {
       op_normal<R>(h->phi, V<R>(h->x), V<R>(h->g), s);
     } else {
-      op_forward<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
-      op_adjoint<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
+      op_forward_sorted<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
+      op_adjoint_sorted<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
     }
     PF_HIP(hipStreamSynchronize(s));
   }
@@ -720,6 +720,11 @@ int pf_padmm_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const
     } else {
       throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
     }
+    // every visibility-domain vector of the iteration (y, r, lambda, z) lives in the operator's sorted
+    // order, so the iterations apply Phi and Phi^dagger without permutations
+    if (h->prec == PF_C64) op_to_sorted<float>(phi, V<float>(h->y), V<float>(h->fx), h->stream);
+    else op_to_sorted<double>(phi, V<double>(h->y), V<double>(h->fx), h->stream);
+    std::swap(h->y, h->fx);
     if (h->prec == PF_C64) padmm_init<float>(h.get());
     else padmm_init<double>(h.get());
   });
@@ -882,6 +887,12 @@ int pf_padmm_get(pf_padmm* h, int32_t what, void* out, int out_loc) {
     const void* src = what == PF_PADMM_IMAGE ? h->x : what == PF_PADMM_RESIDUAL ? h->r : nullptr;
     if (!src) throw Error(PF_ERR_ARG, "what must be PF_PADMM_IMAGE or PF_PADMM_RESIDUAL");
     const int64_t n = what == PF_PADMM_IMAGE || h->fb ? h->N : h->M;
+    if (what == PF_PADMM_RESIDUAL && !h->fb) {
+      // r is in the operator's sorted order; fx is scratch between iterations
+      if (h->prec == PF_C64) op_to_caller<float>(h->phi, V<float>(h->r), V<float>(h->fx), h->stream);
+      else op_to_caller<double>(h->phi, V<double>(h->r), V<double>(h->fx), h->stream);
+      src = h->fx;
+    }
     const size_t e = h->prec == PF_C64 ? 8 : 16;
     if (out_loc == PF_DEVICE) {
       PF_HIP(hipMemcpyAsync(out, src, e * n, hipMemcpyDeviceToDevice, h->stream));
