@@ -56,7 +56,9 @@ typedef struct pf_op_desc {
   double oversample_ratio; /* grid is floor(os*imsizey) x floor(os*imsizex) */
   int32_t kernel;          /* enum pf_kernel */
   int32_t Ju, Jv;
-  int32_t w_term; /* 0: plain KB gridding (operators.cc:8-46); 1: projection rows (operators.cc:49-99) */
+  int32_t w_term; /* 0: plain KB gridding (operators.cc:8-46); 1: projection rows (operators.cc:49-99);
+                     2: chirp rows, wproj_utilities::wprojection_matrix of the plain G (wproj_utilities.cc:
+                     132-188) with energy_chirp / energy_wproj, built on the device (square grids, Ju = Jv) */
   int32_t Jw;
   double cellx, celly; /* arcsec; used by unit conversion and the w-kernel */
   int32_t units;       /* enum pf_units of u, v (w is always in wavelengths) */
@@ -67,10 +69,13 @@ typedef struct pf_op_desc {
   int32_t power_iters;     /* power-method iterations for the normalisation (operators.h:390) */
   double power_tol;
   int32_t tile_u, tile_v; /* gridding tile shape; 0 = library default */
+  /* w_term = 2: the energy fractions the chirp and the convolved row keep (energy_fraction_chirp,
+   * energy_fraction_wproj of wprojection_matrix); energy_chirp must be below 1 */
+  double energy_chirp, energy_wproj;
 } pf_op_desc;
 
 /* Fills a descriptor with the reference's defaults: os 2, kb, Ju = Jv = 4, w_term 0, Jw 6,
- * cells 1, units pixels, power_iters 100, power_tol 1e-4, complex64. */
+ * cells 1, units pixels, power_iters 100, power_tol 1e-4, complex64; energy_chirp = energy_wproj = 0.999. */
 void pf_op_desc_init(pf_op_desc *d);
 
 /* Build Phi: copies and sorts the coordinates, uploads them, builds twiddles and the correction

@@ -271,6 +271,17 @@ struct WRowSpec {
 };
 void wrow_host(const WRowSpec& sp, double u, double v, double w, double wre, double wim, double* out);
 
+// ---- chirp-truncated w-projection rows for a whole operator (pf_chirp.hip) ----------------------
+struct ChirpBuild;
+ChirpBuild* chirp_build_new();
+void chirp_build_free(ChirpBuild* cb);
+// pass 1 (caller-order inputs, pixels and wavelengths): the largest kept chirp radius R of any row
+int chirp_rows_radius(ChirpBuild& cb, int64_t M, int N, int J, int kind, int Jeval_u, int Jeval_v, double os,
+                      double cellx, double celly, double e_chirp, double e_wproj, const double* u, const double* v,
+                      const double* w, const double* weights);
+// pass 2: rows of (J + 2R)^2 values, row i for the caller's visibility perm[i], into rows (precision prec)
+int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_host, void* rows, int prec);
+
 // ---- sparse gridding matrices (pf_wmat.hip) -------------------------------------------------
 // Host-side CSR (complex128 values), the layout of pf_csr.
 struct HostCsr {

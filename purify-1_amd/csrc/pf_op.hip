@@ -328,6 +328,7 @@ struct pf_op {
   TileLists tl;   // TU x TV tiles: degridding, residue-class and atomic gridding
   TileLists tlm;  // 16 x 16 tiles: matrix-core gridding (empty when the grid is too small)
   int Ku = 0, Kv = 0;         // footprint of one row of G (J, or Jw + J - 1 with w-projection)
+  int w_chirp = 0;            // the stored rows are chirp-truncated rows (w_term = 2), footprint J + 2R
   void* d_wrows = nullptr;    // w-projection rows, M * Kv * Ku in the operator's precision
   int64_t nnz_w = 0;
   // a given sparse gridding matrix (pf_op_create_csr): G by rows and conj(G) transposed, by grid cell
@@ -676,9 +677,10 @@ void build(pf_op* op, const pf_op_desc& d) {
     throw Error(PF_ERR_ARG, "Incorrect input: PSWF requires a support of 6");  // kernels.cc:241-244
   if (d.Ju < 1 || d.Jv < 1 || d.Ju > 16 || d.Jv > 16) throw Error(PF_ERR_UNSUPPORTED, "kernel support must be 1..16");
   if (d.precision != PF_C64 && d.precision != PF_C128) throw Error(PF_ERR_ARG, "precision must be PF_C64 or PF_C128");
+  if (d.w_term < 0 || d.w_term > 2) throw Error(PF_ERR_ARG, "w_term must be 0, 1 or 2");
   if (d.w_term) {
     if (d.M > 0 && !d.w) throw Error(PF_ERR_ARG, "w must be given for w_term operators");
-    if (d.Jw < 1 || d.Jw > 16) throw Error(PF_ERR_UNSUPPORTED, "w-kernel support must be 1..16");
+    if (d.w_term == 1 && (d.Jw < 1 || d.Jw > 16)) throw Error(PF_ERR_UNSUPPORTED, "w-kernel support must be 1..16");
   }
   op->prec = d.precision;
   op->nx = d.imsizex;
@@ -696,9 +698,15 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->Jv = std::min(d.Jv, op->Nv);
   op->Jw = d.Jw;
   op->w_term = d.w_term ? 1 : 0;
-  // footprint of one row of G: J x J taps, or (Jw + J - 1)^2 with w-projection (operators.cc:73-74)
-  op->Ku = op->w_term ? d.Jw + op->Ju - 1 : op->Ju;
-  op->Kv = op->w_term ? d.Jw + op->Jv - 1 : op->Jv;
+  // footprint of one row of G: J x J taps, or (Jw + J - 1)^2 with w-projection (operators.cc:73-74);
+  // chirp rows: J + 2R, R found below
+  op->w_chirp = d.w_term == 2 ? 1 : 0;
+  op->Ku = op->w_term == 1 && !op->w_chirp ? d.Jw + op->Ju - 1 : op->Ju;
+  op->Kv = op->w_term == 1 && !op->w_chirp ? d.Jw + op->Jv - 1 : op->Jv;
+  if (op->w_chirp) {
+    if (op->Nu != op->Nv) throw Error(PF_ERR_UNSUPPORTED, "chirp w-projection rows need a square grid");
+    if (op->Ju != op->Jv) throw Error(PF_ERR_UNSUPPORTED, "chirp w-projection rows need Ju = Jv");
+  }
   if (op->w_term && (op->Ku > op->Nu || op->Kv > op->Nv))
     throw Error(PF_ERR_UNSUPPORTED, "w-projection rows wider than the grid");
   op->cellx = d.cellx;
@@ -750,6 +758,15 @@ void build(pf_op* op, const pf_op_desc& d) {
   std::vector<double> u, v;
   to_pixels(d, op->Nu, op->Nv, u, v);
   const int64_t M = d.M;
+  std::unique_ptr<ChirpBuild, void (*)(ChirpBuild*)> chirp(nullptr, chirp_build_free);
+  if (op->w_chirp) {
+    // the kept chirp radius of every row fixes the footprint of the stored rows
+    chirp.reset(chirp_build_new());
+    const int R = chirp_rows_radius(*chirp, M, op->Nu, op->Ju, d.kernel, d.Jv, d.Ju, op->os, d.cellx, d.celly,
+                                    d.energy_chirp, d.energy_wproj, u.data(), v.data(), d.w, d.weights);
+    op->Ku = op->Kv = op->Ju + 2 * R;
+    if (op->Ku > op->Nu) throw Error(PF_ERR_UNSUPPORTED, "w-projection rows wider than the grid");
+  }
   // default tile: 32 x 32, halved on small grids so that N >= T + 8 (the residue-class gridding
   // kernel's condition) whenever the grid allows it
   auto pick_tile = [](int req, int N) {
@@ -806,7 +823,17 @@ void build(pf_op* op, const pf_op_desc& d) {
     op->tl.slot_own = d;
     op->tl.n_own = (int)so.size();
   }
-  if (op->w_term) {
+  if (op->w_chirp) {
+    // rows of (J + 2R)^2 values at origins floor(u - (J + 2R)/2) = floor(u - J/2) - R
+    const int KK = op->Ku * op->Kv;
+    op->d_wrows = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M * KK, 1));
+    std::vector<int2> kw(M);
+    for (int64_t i = 0; i < M; ++i) kw[i] = make_int2((int)ku[perm[i]], (int)kv[perm[i]]);
+    op->d_kk = (int2*)op->dalloc(sizeof(int2) * std::max<int64_t>(M, 1));
+    if (M) PF_HIP(hipMemcpy(op->d_kk, kw.data(), sizeof(int2) * M, hipMemcpyHostToDevice));
+    op->nnz_w = chirp_rows_fill(*chirp, perm.data(), op->d_wrows, op->prec);
+    chirp.reset();
+  } else if (op->w_term) {
     build_wrows(op, d, u, v, perm, ku, kv);
   } else if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv) && (op->prec != PF_C64 || M < ((int64_t)1 << 29))) {
     const HostLists lm = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE);
@@ -1328,6 +1355,8 @@ void pf_op_desc_init(pf_op_desc* d) {
   d->precision = PF_C64;
   d->power_iters = 100;
   d->power_tol = 1e-4;
+  d->energy_chirp = 0.999;
+  d->energy_wproj = 0.999;
 }
 
 // The normalisation's power method (operators.h:390-391) runs in FP64 whatever the operator's precision:
@@ -1818,6 +1847,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   }
   if (k == "tile_u") {
     *value = op->tl.TU;
+    return PF_OK;
+  }
+  if (k == "w_footprint") {
+    *value = op->w_term ? op->Ku : 0;
     return PF_OK;
   }
   if (k == "tile_v") {

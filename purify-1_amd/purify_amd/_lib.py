@@ -52,6 +52,7 @@ class pf_op_desc(C.Structure):
         ("weights", C.c_void_p),
         ("power_iters", C.c_int32), ("power_tol", C.c_double),
         ("tile_u", C.c_int32), ("tile_v", C.c_int32),
+        ("energy_chirp", C.c_double), ("energy_wproj", C.c_double),
     ]
 
 

@@ -293,7 +293,7 @@ class LinearTransform:
 
 
 def _make_desc(u, v, w, weights, imsizey, imsizex, oversample_ratio, power_iters, power_tol, kernel, Ju, Jv,
-               w_term, Jw, cellx, celly, units, precision, tile=(0, 0)):
+               w_term, Jw, cellx, celly, units, precision, tile=(0, 0), wproj=None):
     u, v = _f64(u).ravel(), _f64(v).ravel()
     M = u.size
     if v.size != M:
@@ -320,6 +320,12 @@ def _make_desc(u, v, w, weights, imsizey, imsizex, oversample_ratio, power_iters
     d.power_iters = int(power_iters)
     d.power_tol = float(power_tol)
     d.tile_u, d.tile_v = int(tile[0]), int(tile[1])
+    if w_term and wproj is not None and wproj.get("w_projection", "approx") == "chirp":
+        d.w_term = 2
+        d.energy_chirp = float(wproj.get("energy_chirp", 0.999))
+        d.energy_wproj = float(wproj.get("energy_wproj", 0.999))
+    elif wproj is not None and wproj.get("w_projection", "approx") not in ("approx", "chirp"):
+        raise ValueError("w_projection must be 'approx' or 'chirp'")
     return d, (u, v, w, wt)
 
 
@@ -333,28 +339,35 @@ class measurementoperator:  # noqa: N801 -- mirrors the reference's namespace
         init_degrid_operator_2d(uv_vis, imsizey, imsizex, cell_x, cell_y, oversample_ratio=2, power_iters=100,
                                 power_tol=1e-4, kernel=kb, Ju=4, Jv=4, w_term=False, Jw=6)
             -- vis_params in lambda / radians / pixels (operators.h:398-414)
-        Extra keywords: precision ("c64" default, or "c128"), tile=(tile_u, tile_v).
+        Extra keywords: precision ("c64" default, or "c128"), tile=(tile_u, tile_v); with w_term,
+        w_projection="approx" (the projection kernels of operators.cc:49-99, the default) or "chirp" (the rows of
+        wproj_utilities::wprojection_matrix applied to the plain gridding matrix, wproj_utilities.cc:132-188,
+        built on the device; square grids) with energy_chirp and energy_wproj (0.999 each by default).
         """
         precision = kw.pop("precision", "c64")
         tile = kw.pop("tile", (0, 0))
+        wproj = {k: kw.pop(k) for k in ("w_projection", "energy_chirp", "energy_wproj") if k in kw}
         if args and isinstance(args[0], vis_params):
-            return _init_from_vis(*args, precision=precision, tile=tile, **kw)
-        return _init_raw(*args, precision=precision, tile=tile, **kw)
+            return _init_from_vis(*args, precision=precision, tile=tile, wproj=wproj, **kw)
+        return _init_raw(*args, precision=precision, tile=tile, wproj=wproj, **kw)
 
 
 def _init_raw(u, v, w, weights, imsizey, imsizex, oversample_ratio=2.0, power_iters=100, power_tol=1e-4,
-              kernel=_kernel.kb, Ju=4, Jv=4, w_term=False, Jw=6, cellx=1.0, celly=1.0, precision="c64", tile=(0, 0)):
+              kernel=_kernel.kb, Ju=4, Jv=4, w_term=False, Jw=6, cellx=1.0, celly=1.0, precision="c64", tile=(0, 0),
+              wproj=None):
     d, keep = _make_desc(u, v, w, weights, imsizey, imsizex, oversample_ratio, power_iters, power_tol, kernel, Ju,
-                         Jv, w_term, Jw, cellx, celly, _lib.PF_PIXELS, precision, tile)
+                         Jv, w_term, Jw, cellx, celly, _lib.PF_PIXELS, precision, tile, wproj)
     return LinearTransform(MeasurementOperator(d, keep))
 
 
 def _init_from_vis(uv_vis, imsizey, imsizex, cell_x, cell_y, oversample_ratio=2.0, power_iters=100,
-                   power_tol=1e-4, kernel=_kernel.kb, Ju=4, Jv=4, w_term=False, Jw=6, precision="c64", tile=(0, 0)):
+                   power_tol=1e-4, kernel=_kernel.kb, Ju=4, Jv=4, w_term=False, Jw=6, precision="c64", tile=(0, 0),
+                   wproj=None):
     units = {vis_units.lambda_: _lib.PF_LAMBDA, vis_units.radians: _lib.PF_RADIANS,
              vis_units.pixels: _lib.PF_PIXELS}[uv_vis.units]
     d, keep = _make_desc(uv_vis.u, uv_vis.v, uv_vis.w, uv_vis.weights, imsizey, imsizex, oversample_ratio,
-                         power_iters, power_tol, kernel, Ju, Jv, w_term, Jw, cell_x, cell_y, units, precision, tile)
+                         power_iters, power_tol, kernel, Ju, Jv, w_term, Jw, cell_x, cell_y, units, precision, tile,
+                         wproj)
     return LinearTransform(MeasurementOperator(d, keep))
 
 

@@ -1,0 +1,115 @@
+"""Chirp-truncated w-projection operators (w_term = 2: the rows of wproj_utilities::wprojection_matrix
+applied to the plain Kaiser-Bessel gridding matrix, cpp/purify/wproj_utilities.cc:132-188), built for the
+whole operator on the device from the power series of the chirp's spectrum, against the oracle's dense
+restatement (one N_g-point FFT per row).
+
+- 64 x 64 grids: every row (through G applied to random grids) and the operator's Phi / Phi^dagger against
+  the oracle, complex128 at 1e-12 and complex64 at 1e-5, several energy fractions and kernel supports;
+  rows whose threshold falls within rounding of a magnitude are skipped, as for the matrix builder;
+- the 2048^2 image, 4e6 visibilities, KB 6 x 6, w in [0, 200], 1 degree field, energy fractions 0.999:
+  five rows against the oracle at the real 4096^2 grid (complex128), the footprint within 24 x 24, and the
+  complex64 operator's adjoint dot-product test.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gpu_helpers import make_uv, rel
+from purify_amd import measurementoperator, utilities
+from test_gpu_wmat import threshold_margin
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c128": 1e-12, "c64": 1e-5}
+
+
+def chirp_op(u, v, w, n, J, cell, e_c, e_w, precision, weights=None):
+    return measurementoperator.init_degrid_operator_2d(u, v, w, weights, n, n, 2.0, 0, 1e-4, "kb", J, J, True, 6,
+                                                       cell, cell, precision=precision, w_projection="chirp",
+                                                       energy_chirp=e_c, energy_wproj=e_w)
+
+
+def oracle_rows(u, v, w, n, J, cell, e_c, e_w, weights=None):
+    M = len(u)
+    wt = np.ones(M, np.complex128) if weights is None else weights
+    G = oracle.gridding_matrix(u, v, wt, n, n, 2.0, oracle.KB, J, J, J, J)
+    dense, _ = oracle.wprojection_matrix_dense(G, 2 * n, 2 * n, w, cell, cell, e_c, e_w)
+    return G, dense
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("J,e_c,e_w", [(4, 0.999, 0.999), (6, 0.99, 1.0), (6, 0.999, 0.99)])
+def test_rows_and_operator_vs_oracle_on_a_64_grid(precision, J, e_c, e_w):
+    n, M, cell = 32, 300, 400.0  # a 7.1 degree chirp field: |w| up to 200 spreads the chirps over cells
+    u, v = make_uv(1201 + J, M, n)
+    w = np.random.default_rng(1202).uniform(-200.0, 200.0, M)
+    wt = utilities.synth_cnormal(1203, M) + 1.0
+    lt = chirp_op(u, v, w, n, J, cell, e_c, e_w, precision, wt)
+    G, dense = oracle_rows(u, v, w, n, J, cell, e_c, e_w, wt)
+    assert lt.op.info().w_term == 1 and lt.op.get_tuning("w_footprint") >= J
+    # Each row against the oracle's, through G applied to two random grids.  A row may differ only where
+    # a threshold lies within rounding of a magnitude, so that the keep decision depends on the last bit:
+    # the chirp's spectrum is symmetric, its mirror entries agree to rounding only, and a bisection that
+    # does not converge ends between two such entries (mostly at the lower energy fractions).
+    tie = np.array([threshold_margin(G, m, 2 * n, w[m], cell, e_c, e_w) < 1e-9 for m in range(M)])
+    dt = torch.complex128 if precision == "c128" else torch.complex64
+    ok = np.ones(M, bool)
+    for seed in (1, 2):
+        g = utilities.synth_cnormal(1210 + seed, (2 * n) ** 2)
+        yg = lt.op.apply_G(torch.from_numpy(g).to(dt).cuda()).cpu().numpy()
+        yo = dense @ g
+        scale = np.abs(dense) @ np.abs(g)
+        ok &= np.abs(yg - yo) <= TOL[precision] * scale
+    assert np.all(ok | tie), np.nonzero(~ok & ~tie)
+    assert (~tie).sum() >= 60 and ok.mean() > 0.5
+    # the operator: Phi = G_w F Z S, Phi^dagger its adjoint (rows with ties replaced by the GPU's own)
+    S = oracle.correction2d(2.0, n, n, oracle.KB, J, J)
+    x = utilities.synth_image(1220, n, n, 5)
+    yx = dense @ oracle.fft2d(2 * n, 2 * n, oracle.zero_pad(S, n, n, 2.0, x), -1)
+    assert rel((lt * x)[ok], yx[ok]) < TOL[precision]
+    y = utilities.synth_cnormal(1221, M) * ok
+    xo = oracle.zero_pad_adjoint(S, n, n, 2.0, oracle.fft2d(2 * n, 2 * n, dense.conj().T @ y, +1))
+    assert rel(lt.adjoint() * y, xo) < TOL[precision]
+
+
+def test_chirp_rows_construction_errors():
+    u = np.zeros(4)
+    from purify_amd._lib import PurifyError
+    with pytest.raises(PurifyError, match="energy_chirp"):
+        chirp_op(u, u, u, 16, 4, 60.0, 1.0, 1.0, "c64")
+    with pytest.raises(PurifyError, match="square grid"):
+        measurementoperator.init_degrid_operator_2d(u, u, u, None, 16, 32, 2.0, 0, 1e-4, "kb", 4, 4, True, 6, 60.0,
+                                                    60.0, w_projection="chirp")
+
+
+def test_2048_m4e6_w_0_200_rows_footprint_and_dot_test():
+    n, M, J = 2048, 4_000_000, 6
+    cell = 3600.0 / n  # 1 degree field of view
+    u, v = make_uv(1003, M, n)
+    w = np.random.default_rng(1004).uniform(0.0, 200.0, M)
+    lt = chirp_op(u, v, w, n, J, cell, 0.999, 0.999, "c128")
+    K = lt.op.get_tuning("w_footprint")
+    info = lt.op.info()
+    print(f"chirp rows at 2048^2 / 4e6: footprint {K} x {K}, {info.nnz_w / M:.1f} kept entries per row", flush=True)
+    assert J <= K <= 24 and 0 < info.nnz_w <= M * K * K
+    # five rows against the oracle's literal construction at the 4096^2 grid
+    rows = np.array([0, 17, 123_456, 2_000_001, M - 1])
+    G = oracle.gridding_matrix(u[rows], v[rows], np.ones(rows.size, np.complex128), n, n, 2.0, oracle.KB, J, J, J, J)
+    dense, _ = oracle.wprojection_matrix_dense(G, 2 * n, 2 * n, w[rows], cell, cell, 0.999, 0.999)
+    g = utilities.synth_cnormal(1005, (2 * n) ** 2)
+    yg = lt.op.apply_G(torch.from_numpy(g).cuda()).cpu().numpy()
+    ok = [threshold_margin(G, k, 2 * n, w[rows[k]], cell, 0.999, 0.999) > 1e-9 for k in range(rows.size)]
+    assert sum(ok) >= 4
+    for k in range(rows.size):
+        if ok[k]:  # to FP64 rounding of the row's sum
+            assert abs(yg[rows[k]] - dense[k] @ g) <= 1e-12 * (np.abs(dense[k]) @ np.abs(g)), k
+    del dense, lt
+    torch.cuda.empty_cache()
+    # the complex64 operator of the workload: its adjoint dot-product test
+    l64 = chirp_op(u, v, w, n, J, cell, 0.999, 0.999, "c64")
+    assert l64.op.get_tuning("w_footprint") == K
+    x = utilities.synth_cnormal(1006, n * n)
+    y = utilities.synth_cnormal(1007, M)
+    lhs, rhs = np.vdot(y, l64 * x), np.vdot(l64.adjoint() * y, x)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
