@@ -33,8 +33,12 @@ WORKLOADS = {
     "pair_256_m1e4": (256, 4, 10_000, 1000),
     # w-projection rows (w_term): w ~ U[0, 200] wavelengths, 1 degree field of view, Jw = 6
     "wproj_2048_m4e6": (2048, 6, 4_000_000, 1003),
+    # chirp-truncated w-projection rows (wprojection_matrix of the plain G, energy fractions 0.999), w ~ U[0, 200]
+    # wavelengths, 1 degree field of view; 200 PADMM iterations beside the pair
+    "wchirp_2048_m4e6": (2048, 6, 4_000_000, 1003),
 }
-WPROJ = {"wproj_2048_m4e6": dict(Jw=6, wmax=200.0, fov_deg=1.0)}
+WPROJ = {"wproj_2048_m4e6": dict(Jw=6, wmax=200.0, fov_deg=1.0),
+         "wchirp_2048_m4e6": dict(Jw=6, wmax=200.0, fov_deg=1.0, chirp=0.999)}
 
 METRIC = "Φ+Φ† vis/s FP32 vs FP64 (rel.err≤1e-5, frac HBM peak); served p50/p99 latency"
 
@@ -61,7 +65,8 @@ def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, KK=None):
         "degrid": Ng * e + vis,
         "grid_tiles": vis + Ng * e,
         "fft_cols_stepB": 2 * Ng * e,
-        "fft_cols": (Ng + win) * e,  # either mode: the window rows one way, the full grid the other
+        "fft_cols_fwd": (win + Ng) * e,  # reads the image window's rows, writes the full grid
+        "fft_cols_adj": (Ng + win) * e,  # reads the full grid, writes the rows the crop keeps
         "permute_to_sorted": 2 * M * e,
         "permute_to_caller": 2 * M * e,
     }
@@ -242,6 +247,40 @@ def multichannel_run(n_channels, iters, seed, log):
     return out
 
 
+def wchirp_padmm_run(u, v, w, n, J, cell, chirp_kw, iters, seed, log):
+    """PADMM on the chirp-row w-projection operator (complex64, normalised by 50 FP64 power iterations):
+    DB4 (4 levels) tight-frame prior, 30 dB noise, `iters` iterations replayed from a captured hipGraph."""
+    import numpy as np
+    import torch
+    from purify_amd import algorithms, measurementoperator, utilities, wavelets
+    t0 = time.perf_counter()
+    lt = measurementoperator.init_degrid_operator_2d(u, v, w, None, n, n, 2.0, 50, 1e-6, "kb", J, J, True, 6, cell,
+                                                     cell, precision="c64", **chirp_kw)
+    setup = time.perf_counter() - t0
+    x0 = utilities.synth_image(seed + 5, n, n, 20)
+    y0 = lt * x0
+    sigma = utilities.SNR_to_standard_deviation(y0, 30.0)
+    y = utilities.add_noise(y0, sigma, seed + 6)
+    eps = utilities.calculate_l2_radius(u.size, sigma)
+    W = wavelets.WaveletOperator([("DB4", 4)], n, n, "c64")
+    sol = algorithms.ImagingProximalADMM(lt.op, W, y, eps, itermax=iters + 1)
+    sol.iterate(1, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = sol.iterate(iters, use_graph=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    snr = float(20 * np.log10(np.linalg.norm(x0) / np.linalg.norm(x0 - sol.image())))
+    out = {"wchirp_2048_m4e6": {"image": n, "M": int(u.size), "J": J, "wavelet": "DB4", "levels": 4,
+                                "iterations": iters, "c64_it_per_s": {"graph": iters / dt}, "c64_snr_db": snr,
+                                "setup_s_with_fp64_norm": setup, "residual_norm": r.residual_norm,
+                                "epsilon": eps}}
+    log(f"[bench] padmm wchirp c64: {iters / dt:.0f} it/s graph, SNR {snr:.2f} dB, setup {setup:.1f} s")
+    del sol, W, lt
+    torch.cuda.empty_cache()
+    return out
+
+
 def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev):
     """Workload for rocprofv3 --pmc passes: a copy of known size, then the operator pair."""
     import torch
@@ -320,14 +359,18 @@ def main():
         counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev)
         return
 
+    chirp_kw = {}
+    if wp and wp.get("chirp"):
+        chirp_kw = dict(w_projection="chirp", energy_chirp=wp["chirp"], energy_wproj=wp["chirp"])
     res = {}
     outs = {}
     for prec, steps in (("c64", args.steps), ("c128", args.fp64_steps)):
         t0 = time.perf_counter()
         lt = measurementoperator.init_degrid_operator_2d(u, v, w, None, n, n, os_, 0, 1e-4, "kb", J, J, w_term, Jw,
-                                                         cell, cell, precision=prec)
+                                                         cell, cell, precision=prec, **chirp_kw)
         setup = time.perf_counter() - t0
         op = lt.op
+        KK_op = op.get_tuning("w_footprint") ** 2 if w_term else None
         t0 = time.perf_counter()
         variant = op.autotune(3)
         tune_s = time.perf_counter() - t0
@@ -370,7 +413,7 @@ def main():
             host_ms.append(1e3 * (time.perf_counter() - t0))
         res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
                          value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s,
-                         host_ms=statistics.median(host_ms))
+                         host_ms=statistics.median(host_ms), KK=KK_op, nnz_w=op.info().nnz_w)
         log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s, "
             f"grid kernel {variant}")
         del lt, op, x, yin, y, xo
@@ -382,7 +425,7 @@ def main():
     tab = r64["table"]
     dom = max(tab, key=lambda k: tab[k][0])
     dom_ms = tab[dom][0] / tab[dom][1]
-    KK = (J + Jw - 1) ** 2 if w_term else None
+    KK = r64["KK"] if w_term else None
     b = kernel_alg_bytes(dom, n, Nu, Nv, M, e, r, KK)
     if b is None:  # step A: forward reads the image window, adjoint the full grid
         b = ((n * Nu * e + Ng * e) + (2 * Ng * e)) / 2
@@ -397,17 +440,25 @@ def main():
         per = tr.get("bytes_per_launch", {})
         # the profiles name kernels by their trace entries: the column FFT per direction (mean of the
         # forward and adjoint launches, as the timing averages them), the gridding by its variant
-        alias = {"fft_cols": ["fft_cols_big_fwd", "fft_cols_big_adj"], "grid_tiles": ["grid_tiles[mfma]"],
+        alias = {"fft_cols_fwd": ["fft_cols_big_fwd"], "fft_cols_adj": ["fft_cols_big_adj"],
+                 "grid_tiles": ["grid_tiles[mfma]"],
                  "permute_to_caller": ["permute_to_caller[chunk]", "permute_to_caller[bucket]"],
                  "permute_to_sorted": ["permute_to_sorted[chunk]", "permute_to_sorted[bucket]"]}
         if dom in per:
             roofline["traffic"] = per[dom]
         elif dom in alias and all(k in per for k in alias[dom]):
             vals = [per[k] for k in alias[dom]]
-            roofline["traffic"] = sum(vals) / len(vals) if dom == "fft_cols" else sum(vals)
+            roofline["traffic"] = sum(vals)
         roofline["traffic_source"] = os.path.relpath(traffic_file, ROOT)
 
     pair_bytes = model_bytes_pair(n, Ng, M, e, r, KK)
+    # the bytes this run moves: unit weights are not stored, so neither direction reads them
+    moved_bytes = pair_bytes - (0 if KK else 2 * e * M)
+    kern_frac = {}
+    for k in tab:
+        kb = kernel_alg_bytes(k, n, Nu, Nv, M, e, r, KK)
+        if kb:
+            kern_frac[k] = kb / (tab[k][0] / tab[k][1] * 1e-3) / HBM_PEAK
     rel_fp32_fp64 = {
         "forward": float(np.linalg.norm(outs["c64"][0] - outs["c128"][0]) / np.linalg.norm(outs["c128"][0])),
         "adjoint": float(np.linalg.norm(outs["c64"][1] - outs["c128"][1]) / np.linalg.norm(outs["c128"][1])),
@@ -450,6 +501,8 @@ def main():
     multi = None
     if args.channels > 0 and args.padmm_iters > 0 and not w_term:
         multi = multichannel_run(args.channels, args.padmm_iters, seed + 20, log)
+    if chirp_kw and args.padmm_iters > 0:
+        padmm = wchirp_padmm_run(u, v, w, n, J, cell, chirp_kw, 2 * args.padmm_iters, seed, log)
 
     lat = sorted(r64["lat"])
     out = {
@@ -467,11 +520,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "image": n, "grid": Nu, "oversample": os_, "kernel": "kb",
                    "J": J, "M": M, "w_term": w_term, "uv_sigma_rad": round(np.pi / 3, 6), "seed": seed,
-                   **({"Jw": Jw, "w_max_lambda": wp["wmax"], "cell_arcsec": cell} if wp else {})},
+                   **({"Jw": Jw, "w_max_lambda": wp["wmax"], "cell_arcsec": cell} if wp else {}),
+                   **({"w_projection": "chirp", "energy_chirp": wp["chirp"], "energy_wproj": wp["chirp"]}
+                      if chirp_kw else {})},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pair_frac_hbm_peak": pair_bytes / (r64["ms"] * 1e-3) / HBM_PEAK,
         "pair_model_bytes": pair_bytes,
+        "pair_frac_hbm_peak_moved": moved_bytes / (r64["ms"] * 1e-3) / HBM_PEAK,
+        "pair_moved_bytes": moved_bytes,
+        "kernels_frac_hbm_peak": kern_frac,
+        **({"w_rows": {"footprint": int(round(KK ** 0.5)), "kept_entries_per_row": r64["nnz_w"] / M,
+                       "stored_bytes_per_row": KK * e}} if KK else {}),
         "fp64": {"value": res["c128"]["value"], "ms_per_step": res["c128"]["ms"],
                  "fp32_speedup": res["c128"]["ms"] / r64["ms"],
                  "pair_frac_hbm_peak": model_bytes_pair(n, Ng, M, 16, 8, KK) / (res["c128"]["ms"] * 1e-3) / HBM_PEAK},

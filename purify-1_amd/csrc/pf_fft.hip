@@ -563,7 +563,10 @@ static void launch_cols(int kind, bool inv, int log2n, const ColArgs<R>& a, dim3
   const int E = N < FFT_E ? N : FFT_E;
   const int threads = a.C * (N / E);
   const size_t shm = (size_t)N * a.C * sizeof(cpx<R>);
-  ProfScope prof(kind == COL_STEP_A ? "fft_cols_stepA" : kind == COL_STEP_B ? "fft_cols_stepB" : "fft_cols", s);
+  ProfScope prof(kind == COL_STEP_A ? "fft_cols_stepA" : kind == COL_STEP_B ? "fft_cols_stepB"
+                 : inv                 ? "fft_cols_adj"
+                                       : "fft_cols_fwd",
+                 s);
   ColLaunch<R> launch{a, kind, inv, grid, threads, shm, s};
   dispatch_log2(log2n, launch);
   PF_HIP(hipGetLastError());
@@ -588,7 +591,7 @@ void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow w
     a.C = p.big_C;
     a.rin = restrict_in;
     a.rout = restrict_out;
-    ProfScope prof("fft_cols", s);
+    ProfScope prof(inv ? "fft_cols_adj" : "fft_cols_fwd", s);
     BigColLaunch<R> launch{a, inv, s};
     dispatch_log2(p.col.log2N, launch);
     PF_HIP(hipGetLastError());
