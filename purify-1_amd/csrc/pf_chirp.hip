@@ -409,19 +409,17 @@ int chirp_rows_radius(ChirpBuild& cb, int64_t M, int N, int J, int kind, int Jev
   }
 }
 
-// Pass 2: the rows, in the sorted order perm (sorted position -> caller's index), into rows
+// Pass 2: the rows, in the sorted order perm (device: sorted position -> caller's index), into rows
 // (M * (J + 2R)^2 values in precision prec); returns the kept entries.
-int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_host, void* rows, int prec) {
+int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_dev, void* rows, int prec) {
   ChirpArgs& a = cb.a;
   a.prec = prec;
   a.rows = rows;
-  int32_t* d_perm = cb.alloc<int32_t>(a.M);
   unsigned long long* d_nnz = cb.alloc<unsigned long long>(1);
   PF_HIP(hipMemset(d_nnz, 0, sizeof(unsigned long long)));
-  a.perm = d_perm;
+  a.perm = perm_dev;
   a.nnz = d_nnz;
   if (a.M) {
-    PF_HIP(hipMemcpy(d_perm, perm_host, sizeof(int32_t) * a.M, hipMemcpyHostToDevice));
     const size_t shm = chirp_lds(a.Rw, a.J, a.R);
     allow_lds((const void*)k_chirp_rows<true>, shm);
     hipLaunchKernelGGL(k_chirp_rows<true>, dim3(cb.blocks), dim3(64), shm, nullptr, a);

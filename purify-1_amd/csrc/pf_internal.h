@@ -279,8 +279,26 @@ void chirp_build_free(ChirpBuild* cb);
 int chirp_rows_radius(ChirpBuild& cb, int64_t M, int N, int J, int kind, int Jeval_u, int Jeval_v, double os,
                       double cellx, double celly, double e_chirp, double e_wproj, const double* u, const double* v,
                       const double* w, const double* weights);
-// pass 2: rows of (J + 2R)^2 values, row i for the caller's visibility perm[i], into rows (precision prec)
-int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_host, void* rows, int prec);
+// pass 2: rows of (J + 2R)^2 values, row i for the caller's visibility perm[i] (device), into rows
+// (precision prec)
+int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_dev, void* rows, int prec);
+
+// ---- operator construction on the device (pf_build.hip) -------------------------------------------
+// the owner-tile sort of the caller's pixel coordinates uv: perm (sorted -> caller) and owner counts
+void dev_sort_visibilities(const cpx<double>* d_uv, int64_t M, int Nu, int Nv, int Ku, int Kv, int TU, int TV,
+                           int32_t* d_perm, std::vector<uint32_t>& owner_count);
+// sorted tap origins floor(u - K/2), complex64 fractions, complex128 pixels, weights (any may be null)
+template <typename R>
+void dev_gather_sorted(const cpx<double>* d_uv, const cpx<double>* d_wt, const int32_t* d_perm, int64_t M, int Ku,
+                       int Kv, int2* kk, cpx<float>* ff, cpx<double>* uvs, cpx<R>* ws);
+// tile lists (hipMalloc'd, owned by the caller), their length and per-tile counts
+uint32_t* dev_tile_lists(const int2* kk, int64_t M, int Nu, int Nv, int Ku, int Kv, int TU, int TV,
+                         std::vector<uint32_t>& count, int64_t& len);
+// complex64 matrix-core entry records of a list
+void dev_entries(const uint32_t* list, int64_t n, const int2* kk, const cpx<float>* ff, int Nu, int Nv, int4* ent);
+void dev_inverse(const int32_t* perm, int64_t M, int32_t* inv);
+// the two-pass permutation plan's index arrays
+void dev_perm_plan(const int32_t* perm, int64_t M, int S, int nb, uint16_t* l1, int32_t* g1, uint16_t* p2);
 
 // ---- sparse gridding matrices (pf_wmat.hip) -------------------------------------------------
 // Host-side CSR (complex128 values), the layout of pf_csr.

@@ -423,103 +423,62 @@ void parallel_for(int64_t n, F f) {
 // Gridding lists of a TU x TV tiling: for each tile, the sorted index of every visibility with a tap
 // in it (ascending), and the tiles in launch order, heaviest first, so that the dense centre of the
 // uv-plane does not form the tail of the launch.
-struct HostLists {
-  std::vector<uint32_t> ptr, list;
-  std::vector<uint4> slot;  // (tile, begin, end, 0), heaviest tile first
+// Device scratch of one construction, freed when the scope ends.
+struct DevTemp {
+  std::vector<void*> bufs;
+  ~DevTemp() {
+    for (void* p : bufs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* alloc(int64_t n) {
+    void* p = nullptr;
+    PF_HIP(hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+    bufs.push_back(p);
+    return static_cast<T*>(p);
+  }
 };
 
-HostLists tile_lists(const std::vector<int32_t>& perm, const std::vector<int64_t>& ku, const std::vector<int64_t>& kv,
-                     int Nu, int Nv, int Ju, int Jv, int TU, int TV) {
-  const int ntu = Nu / TU, ntv = Nv / TV, nt = ntu * ntv;
-  const int64_t M = (int64_t)perm.size();
-  HostLists h;
-  h.ptr.assign((size_t)nt + 1, 0);
-  // distinct tile columns / rows met by the footprint of visibility m
-  auto sets = [&](int64_t m, int* tcols, int& nc, int* trows, int& nr) {
-    const int q0 = wrapi(ku[m] + 1, Nu), p0 = wrapi(kv[m] + 1, Nv);
-    nc = nr = 0;
-    for (int a = 0; a < Ju; ++a) {
-      const int tc = ((q0 + a) % Nu) / TU;
-      bool seen = false;
-      for (int k = 0; k < nc; ++k) seen |= (tcols[k] == tc);
-      if (!seen) tcols[nc++] = tc;
-    }
-    for (int b = 0; b < Jv; ++b) {
-      const int tr = ((p0 + b) % Nv) / TV;
-      bool seen = false;
-      for (int k = 0; k < nr; ++k) seen |= (trows[k] == tr);
-      if (!seen) trows[nr++] = tr;
-    }
-  };
-  // Threads take contiguous ranges of the sorted order; within a tile the entries of thread t follow
-  // those of threads < t, so every list is in ascending sorted order, as a serial pass would leave it.
-  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
-                                                                    M / 65536 + 1}));
-  const int64_t chunk = (M + nth - 1) / nth;
-  std::vector<std::vector<uint32_t>> cnt(nth, std::vector<uint32_t>((size_t)nt, 0));
-  std::vector<int64_t> tot(nth, 0);
-  auto parallel = [&](auto&& body) {
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nth; ++t) pool.emplace_back(body, t);
-    body(0);
-    for (auto& th : pool) th.join();
-  };
-  parallel([&](int t) {
-    int tc[16], tr[16], nc, nr;
-    for (int64_t i = t * chunk; i < std::min(M, (t + 1) * chunk); ++i) {
-      sets(perm[i], tc, nc, tr, nr);
-      tot[t] += (int64_t)nr * nc;
-      for (int r = 0; r < nr; ++r)
-        for (int c = 0; c < nc; ++c) cnt[t][(size_t)tr[r] * ntu + tc[c]]++;
-    }
-  });
-  int64_t total = 0;
-  for (int t = 0; t < nth; ++t) total += tot[t];
-  if (total > 0xffffffffLL) throw Error(PF_ERR_UNSUPPORTED, "gridding tile lists exceed 2^32 entries");
-  // start of thread t's entries in tile k: ptr[k] + sum of cnt[t' < t][k]
-  std::vector<std::vector<uint32_t>> start(nth, std::vector<uint32_t>((size_t)nt, 0));
+// The gridding lists of TU x TV tiles (footprints Ku x Kv at the sorted tap origins kk) into tl: the list
+// built on the device, the launch slots (heaviest tile first) from its per-tile counts.  Returns its length.
+int64_t device_lists(pf_op* op, const int2* kk, int64_t M, int Ku, int Kv, int TU, int TV, TileLists& tl) {
+  std::vector<uint32_t> count;
+  int64_t len = 0;
+  uint32_t* list = dev_tile_lists(kk, M, op->Nu, op->Nv, Ku, Kv, TU, TV, count, len);
+  op->allocs.push_back(list);
+  op->device_bytes += (int64_t)sizeof(uint32_t) * std::max<int64_t>(len, 1);
+  const int nt = (int)count.size();
+  std::vector<uint4> slot((size_t)nt);
   uint32_t run = 0;
-  for (int k = 0; k < nt; ++k) {
-    h.ptr[k] = run;
-    for (int t = 0; t < nth; ++t) {
-      start[t][k] = run;
-      run += cnt[t][k];
-    }
+  for (int t = 0; t < nt; ++t) {
+    slot[t] = make_uint4((uint32_t)t, run, run + count[t], 0u);
+    run += count[t];
   }
-  h.ptr[nt] = run;
-  h.list.resize(h.ptr[nt]);
-  parallel([&](int t) {
-    int tc[16], tr[16], nc, nr;
-    std::vector<uint32_t>& fill = start[t];
-    for (int64_t i = t * chunk; i < std::min(M, (t + 1) * chunk); ++i) {
-      sets(perm[i], tc, nc, tr, nr);
-      for (int r = 0; r < nr; ++r)
-        for (int c = 0; c < nc; ++c) h.list[fill[(size_t)tr[r] * ntu + tc[c]]++] = (uint32_t)i;
-    }
-  });
-  h.slot.resize(nt);
-  for (int t = 0; t < nt; ++t) h.slot[t] = make_uint4((uint32_t)t, h.ptr[t], h.ptr[t + 1], 0u);
-  std::stable_sort(h.slot.begin(), h.slot.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
-  return h;
+  std::stable_sort(slot.begin(), slot.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
+  uint4* d_slot = (uint4*)op->dalloc(sizeof(uint4) * (size_t)std::max(nt, 1));
+  PF_HIP(hipMemcpy(d_slot, slot.data(), sizeof(uint4) * nt, hipMemcpyHostToDevice));
+  tl.TU = TU;
+  tl.TV = TV;
+  tl.ntu = op->Nu / TU;
+  tl.ntv = op->Nv / TV;
+  tl.list = list;
+  tl.slot = d_slot;
+  return len;
 }
 
-// The permutation plan (VisPerm) for perm: sorted position -> caller's index.  complex128: the
-// one-pass plan (perm and its inverse).  complex64: the two-pass plan, whose staging layout puts
-// bucket b (sorted positions [b*S, b*S + S)) at staging [b*S, ...), grouped by the caller's chunk
-// (chunk c = caller indices [c*S, c*S + S)), each group in the caller's order; chunk c's LDS order is
-// by bucket, then by caller index.
-void build_perm(pf_op* op, const std::vector<int32_t>& perm) {
+// The permutation plan (VisPerm) for perm (device: sorted position -> caller's index).  complex128: the
+// one-pass plan (perm and its inverse; d_perm is the operator's).  complex64: the two-pass plan, whose
+// staging layout puts bucket b (sorted positions [b*S, b*S + S)) at staging [b*S, ...), grouped by the
+// caller's chunk (chunk c = caller indices [c*S, c*S + S)), each group in the caller's order; chunk c's
+// LDS order is by bucket, then by caller index.
+void build_perm_dev(pf_op* op, int32_t* d_perm) {
   const int64_t M = op->M;
   VisPerm& p = op->vperm;
   p.M = M;
-  std::vector<int32_t> inv(M);
-  for (int64_t i = 0; i < M; ++i) inv[perm[i]] = (int32_t)i;
   p.two_pass = op->prec == PF_C64;
   if (!p.two_pass) {
-    op->d_perm = (int32_t*)op->dalloc(sizeof(int32_t) * M);
-    op->d_inv = (int32_t*)op->dalloc(sizeof(int32_t) * M);
-    PF_HIP(hipMemcpy(op->d_perm, perm.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
-    PF_HIP(hipMemcpy(op->d_inv, inv.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
+    op->d_perm = d_perm;
+    op->d_inv = (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(M, 1));
+    dev_inverse(d_perm, M, op->d_inv);
     p.perm = op->d_perm;
     p.inv = op->d_inv;
     return;
@@ -528,60 +487,15 @@ void build_perm(pf_op* op, const std::vector<int32_t>& perm) {
   p.S = 64;
   while (p.S < M && p.S < Smax) p.S <<= 1;
   p.nb = (int)((M + p.S - 1) / p.S);
-  const int S = p.S, nb = p.nb;
-  std::vector<uint16_t> l1(M), p2(M);
-  std::vector<int32_t> g1(M);
-  std::vector<int64_t> run(nb);  // next staging index of each bucket
-  for (int b = 0; b < nb; ++b) run[b] = (int64_t)b * S;
-  std::vector<int32_t> cnt(nb), loc(nb);
-  for (int c = 0; c < nb; ++c) {
-    const int64_t c0 = (int64_t)c * S, c1 = std::min<int64_t>(c0 + S, M);
-    std::fill(cnt.begin(), cnt.end(), 0);
-    for (int64_t m = c0; m < c1; ++m) cnt[inv[m] / S]++;
-    int a = 0;
-    for (int b = 0; b < nb; ++b) {
-      loc[b] = a;
-      a += cnt[b];
-    }
-    for (int64_t m = c0; m < c1; ++m) {
-      const int b = inv[m] / S;
-      const int j = loc[b]++;
-      const int64_t t = run[b]++;
-      l1[m] = (uint16_t)j;
-      g1[c0 + j] = (int32_t)t;
-      p2[t] = (uint16_t)(inv[m] - (int64_t)b * S);
-    }
-  }
-  for (int b = 0; b < nb; ++b)
-    if (run[b] != std::min<int64_t>((int64_t)(b + 1) * S, M)) throw Error(PF_ERR_STATE, "permutation plan is inconsistent");
-  op->d_perm_l1 = (uint16_t*)op->dalloc(sizeof(uint16_t) * M);
-  op->d_perm_g1 = (int32_t*)op->dalloc(sizeof(int32_t) * M);
-  op->d_perm_p2 = (uint16_t*)op->dalloc(sizeof(uint16_t) * M);
-  PF_HIP(hipMemcpy(op->d_perm_l1, l1.data(), sizeof(uint16_t) * M, hipMemcpyHostToDevice));
-  PF_HIP(hipMemcpy(op->d_perm_g1, g1.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice));
-  PF_HIP(hipMemcpy(op->d_perm_p2, p2.data(), sizeof(uint16_t) * M, hipMemcpyHostToDevice));
+  op->d_perm_l1 = (uint16_t*)op->dalloc(sizeof(uint16_t) * std::max<int64_t>(M, 1));
+  op->d_perm_g1 = (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(M, 1));
+  op->d_perm_p2 = (uint16_t*)op->dalloc(sizeof(uint16_t) * std::max<int64_t>(M, 1));
+  dev_perm_plan(d_perm, M, p.S, p.nb, op->d_perm_l1, op->d_perm_g1, op->d_perm_p2);
   p.l1 = op->d_perm_l1;
   p.g1 = op->d_perm_g1;
   p.p2 = op->d_perm_p2;
 }
 
-void upload_lists(pf_op* op, const HostLists& h, int TU, int TV, TileLists& tl) {
-  const size_t nt = h.slot.size();
-  uint32_t* list = (uint32_t*)op->dalloc(sizeof(uint32_t) * h.list.size());
-  uint4* slot = (uint4*)op->dalloc(sizeof(uint4) * nt);
-  if (!h.list.empty()) PF_HIP(hipMemcpy(list, h.list.data(), sizeof(uint32_t) * h.list.size(), hipMemcpyHostToDevice));
-  PF_HIP(hipMemcpy(slot, h.slot.data(), sizeof(uint4) * nt, hipMemcpyHostToDevice));
-  tl.TU = TU;
-  tl.TV = TV;
-  tl.ntu = op->Nu / TU;
-  tl.ntv = op->Nv / TV;
-  tl.list = list;
-  tl.slot = slot;
-}
-
-// operators.h:405-410: lambda -> set_cell_size (utilities.cc:208-250) -> radians -> uv_scale
-// (utilities.cc:258-278) -> pixels; radians -> uv_scale only (set_cell_size's radians branch is not on
-// this path)
 void to_pixels(const pf_op_desc& d, int Nu, int Nv, std::vector<double>& u, std::vector<double>& v) {
   const int64_t M = d.M;
   u.assign(d.u, d.u + M);
@@ -616,7 +530,7 @@ void to_pixels(const pf_op_desc& d, int Nu, int Nv, std::vector<double>& u, std:
 // of the projection use the u-slot kernel, which is the v-support kernel after the slot swap
 // (operators.h:324, projection_kernels.cc:72-73).
 void build_wrows(pf_op* op, const pf_op_desc& d, const std::vector<double>& u, const std::vector<double>& v,
-                 const std::vector<int32_t>& perm, const std::vector<int64_t>& ku, const std::vector<int64_t>& kv) {
+                 const std::vector<int32_t>& perm) {
   const int64_t M = d.M;
   const int KK = op->Ku * op->Kv;
   op->nnz_w = M * KK;
@@ -635,10 +549,6 @@ void build_wrows(pf_op* op, const pf_op_desc& d, const std::vector<double>& u, c
   sp.dv_lambda = 60. * 60. * 180. / d.celly / std::floor(op->os * op->ny) / PI;
   const size_t esz = op->esz();
   op->d_wrows = op->dalloc(esz * (size_t)std::max<int64_t>(op->nnz_w, 1));
-  std::vector<int2> kw(M);
-  for (int64_t i = 0; i < M; ++i) kw[i] = make_int2((int)ku[perm[i]], (int)kv[perm[i]]);
-  op->d_kk = (int2*)op->dalloc(sizeof(int2) * std::max<int64_t>(M, 1));
-  if (M) PF_HIP(hipMemcpy(op->d_kk, kw.data(), sizeof(int2) * M, hipMemcpyHostToDevice));
   const int64_t CH = 1 << 15;  // visibilities per chunk
   std::vector<double> tmp((size_t)std::min<int64_t>(M, CH) * KK * 2 + 2);
   std::vector<float> tmpf(op->prec == PF_C64 ? tmp.size() : 0);
@@ -781,119 +691,81 @@ void build(pf_op* op, const pf_op_desc& d) {
   if (!is_pow2(TV) && TV != op->Nv) throw Error(PF_ERR_ARG, "tile_v must be a power of two");
   const int ntu = op->Nu / TU, ntv = op->Nv / TV;
   op->n_tiles = ntu * ntv;
-  // Counting sort by (owner tile, first-tap cell inside the tile): a tile's visibilities form one
-  // contiguous range, ordered row by row, which keeps the gridding lanes on the same cell.
-  std::vector<int64_t> ku(M), kv(M);
-  std::vector<uint32_t> key(M);
-  const int64_t nkeys = (int64_t)op->n_tiles * TU * TV;
-  std::vector<uint32_t> count(nkeys + 1, 0);
-  for (int64_t m = 0; m < M; ++m) {
-    if (!std::isfinite(u[m]) || !std::isfinite(v[m])) throw Error(PF_ERR_ARG, "non-finite u/v coordinate");
-    ku[m] = (int64_t)std::floor(u[m] - op->Ku * 0.5);  // operators.cc:34-35 (87-88 with w-projection)
-    kv[m] = (int64_t)std::floor(v[m] - op->Kv * 0.5);
-    const int q0 = wrapi(ku[m] + 1, op->Nu), p0 = wrapi(kv[m] + 1, op->Nv);
-    const uint32_t tile = (uint32_t)((p0 / TV) * ntu + q0 / TU);
-    key[m] = tile * (uint32_t)(TU * TV) + (uint32_t)((p0 % TV) * TU + q0 % TU);
-    count[key[m] + 1]++;
+  // ---- on the device: the sort by (owner tile, first-tap cell inside the tile), so that a tile's
+  // visibilities form one contiguous range ordered row by row (the gridding lanes stay on one cell), the
+  // tile lists, the permutation plan and the sorted records ----
+  std::vector<cpx<double>> uvh((size_t)M);
+  for (int64_t m = 0; m < M; ++m) uvh[m] = mk<double>(u[m], v[m]);
+  DevTemp tmp;
+  cpx<double>* d_uvc = tmp.alloc<cpx<double>>(M);
+  if (M) PF_HIP(hipMemcpy(d_uvc, uvh.data(), sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
+  std::vector<cpx<double>>().swap(uvh);
+  cpx<double>* d_wtc = nullptr;
+  if (!op->unit_weights) {
+    d_wtc = tmp.alloc<cpx<double>>(M);
+    PF_HIP(hipMemcpy(d_wtc, d.weights, sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
   }
-  for (int64_t k = 0; k < nkeys; ++k) count[k + 1] += count[k];
-  std::vector<uint32_t> owner(op->n_tiles + 1);
-  for (int t = 0; t <= op->n_tiles; ++t) owner[t] = count[(int64_t)t * TU * TV];
-  std::vector<int32_t> perm(M);
-  for (int64_t m = 0; m < M; ++m) perm[count[key[m]]++] = (int32_t)m;
-  std::vector<uint32_t>().swap(count);
-  std::vector<uint32_t>().swap(key);
-  // ---- per-tile gridding lists (every visibility with a tap in the tile) ----
-  const HostLists lists = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ku, op->Kv, TU, TV);
-  op->tile_list_len = (int64_t)lists.list.size();
-
-  // ---- upload ----
-  build_perm(op, perm);
-  op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
-  if (op->vperm.two_pass) op->d_ystage = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
-  upload_lists(op, lists, TU, TV, op->tl);
+  // perm (sorted position -> caller's index): kept as the one-pass plan's for complex128
+  int32_t* d_perm = op->prec == PF_C128 ? (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(M, 1))
+                                        : tmp.alloc<int32_t>(M);
+  std::vector<uint32_t> owner_count;
+  dev_sort_visibilities(d_uvc, M, op->Nu, op->Nv, op->Ku, op->Kv, TU, TV, d_perm, owner_count);
+  // sorted tap origins (footprint origins with w-projection rows)
+  int2* d_kk = (int2*)op->dalloc(sizeof(int2) * std::max<int64_t>(M, 1));
+  cpx<float>* d_ff = nullptr;
+  if (op->prec == PF_C64 && !op->w_term) d_ff = (cpx<float>*)op->dalloc(sizeof(cpx<float>) * std::max<int64_t>(M, 1));
+  if (op->prec == PF_C128 && !op->w_term) op->d_uv = (cpx<double>*)op->dalloc(sizeof(cpx<double>) * std::max<int64_t>(M, 1));
+  void* d_ws = nullptr;
+  if (!op->unit_weights && !op->w_term) d_ws = op->dalloc(op->esz() * std::max<int64_t>(M, 1));
+  if (op->prec == PF_C64)
+    dev_gather_sorted<float>(d_uvc, d_wtc, d_perm, M, op->Ku, op->Kv, d_kk, d_ff, nullptr, (cpx<float>*)d_ws);
+  else
+    dev_gather_sorted<double>(d_uvc, d_wtc, d_perm, M, op->Ku, op->Kv, d_kk, nullptr, op->d_uv, (cpx<double>*)d_ws);
+  op->d_kk = d_kk;
+  op->d_ff = d_ff;
+  op->d_weights = d_ws;
+  // degridding launch slots: the tiles that own visibilities, most visibilities first
   {
-    // degridding launch slots: the tiles that own visibilities, most visibilities first
     std::vector<uint4> so;
-    for (int t = 0; t < op->n_tiles; ++t)
-      if (owner[t + 1] > owner[t]) so.push_back(make_uint4((uint32_t)t, owner[t], owner[t + 1], 0u));
+    uint32_t run = 0;
+    for (int t = 0; t < op->n_tiles; ++t) {
+      if (owner_count[t]) so.push_back(make_uint4((uint32_t)t, run, run + owner_count[t], 0u));
+      run += owner_count[t];
+    }
     std::stable_sort(so.begin(), so.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
-    uint4* d = (uint4*)op->dalloc(sizeof(uint4) * std::max<size_t>(so.size(), 1));
-    if (!so.empty()) PF_HIP(hipMemcpy(d, so.data(), sizeof(uint4) * so.size(), hipMemcpyHostToDevice));
-    op->tl.slot_own = d;
+    uint4* dso = (uint4*)op->dalloc(sizeof(uint4) * std::max<size_t>(so.size(), 1));
+    if (!so.empty()) PF_HIP(hipMemcpy(dso, so.data(), sizeof(uint4) * so.size(), hipMemcpyHostToDevice));
+    op->tl.slot_own = dso;
     op->tl.n_own = (int)so.size();
   }
+  // gridding lists of the TU x TV tiles
+  op->tile_list_len = device_lists(op, d_kk, M, op->Ku, op->Kv, TU, TV, op->tl);
+  // the permutation between the caller's order and the sorted order
+  build_perm_dev(op, d_perm);
+  op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
+  if (op->vperm.two_pass) op->d_ystage = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
   if (op->w_chirp) {
     // rows of (J + 2R)^2 values at origins floor(u - (J + 2R)/2) = floor(u - J/2) - R
     const int KK = op->Ku * op->Kv;
     op->d_wrows = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M * KK, 1));
-    std::vector<int2> kw(M);
-    for (int64_t i = 0; i < M; ++i) kw[i] = make_int2((int)ku[perm[i]], (int)kv[perm[i]]);
-    op->d_kk = (int2*)op->dalloc(sizeof(int2) * std::max<int64_t>(M, 1));
-    if (M) PF_HIP(hipMemcpy(op->d_kk, kw.data(), sizeof(int2) * M, hipMemcpyHostToDevice));
-    op->nnz_w = chirp_rows_fill(*chirp, perm.data(), op->d_wrows, op->prec);
+    op->nnz_w = chirp_rows_fill(*chirp, d_perm, op->d_wrows, op->prec);
     chirp.reset();
   } else if (op->w_term) {
-    build_wrows(op, d, u, v, perm, ku, kv);
+    // projection rows, built on the host threads from the sorted order
+    std::vector<int32_t> perm((size_t)M);
+    if (M) PF_HIP(hipMemcpy(perm.data(), d_perm, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+    build_wrows(op, d, u, v, perm);
   } else if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv) && (op->prec != PF_C64 || M < ((int64_t)1 << 29))) {
-    const HostLists lm = tile_lists(perm, ku, kv, op->Nu, op->Nv, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE);
-    upload_lists(op, lm, MFMA_TILE, MFMA_TILE, op->tlm);
-    op->tile_list_len_m = (int64_t)lm.list.size();
+    op->tile_list_len_m = device_lists(op, d_kk, M, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE, op->tlm);
     op->grid_variant = GRID_MFMA;
     if (op->prec == PF_C64) {
       // entry records: sorted index, wrapped first-tap cell, fractions (as the kk / ff arrays hold them)
-      const size_t ne = lm.list.size();
-      std::vector<int4> ent(std::max<size_t>(ne, 1));
-      parallel_for((int64_t)ne, [&](int64_t a, int64_t b) {
-        for (int64_t p = a; p < b; ++p) {
-          const int32_t i = (int32_t)lm.list[p];
-          const int64_t m = perm[i];
-          const float fu = (float)(u[m] - (double)ku[m]), fv = (float)(v[m] - (double)kv[m]);
-          const int cu = wrapi(ku[m], op->Nu), cv = wrapi(kv[m], op->Nv);
-          int fub, fvb;
-          std::memcpy(&fub, &fu, 4);
-          std::memcpy(&fvb, &fv, 4);
-          ent[p] = make_int4((int32_t)((uint32_t)i * 8u), cu | (cv << 16), fub, fvb);
-        }
-      });
-      int4* d_ent = (int4*)op->dalloc(sizeof(int4) * ent.size());
-      PF_HIP(hipMemcpy(d_ent, ent.data(), sizeof(int4) * ent.size(), hipMemcpyHostToDevice));
+      int4* d_ent = (int4*)op->dalloc(sizeof(int4) * (size_t)std::max<int64_t>(op->tile_list_len_m, 1));
+      dev_entries(op->tlm.list, op->tile_list_len_m, d_kk, d_ff, op->Nu, op->Nv, d_ent);
       op->tlm.ent = d_ent;
     }
   }
-  if (op->w_term) {
-    // row origins only: the values hold the kernel, the phase and the weight
-  } else if (op->prec == PF_C64) {
-    std::vector<int2> kk(M);
-    std::vector<cpx<float>> ff(M);
-    for (int64_t i = 0; i < M; ++i) {
-      const int64_t m = perm[i];
-      kk[i] = make_int2((int)ku[m], (int)kv[m]);
-      ff[i] = mk<float>((float)(u[m] - (double)ku[m]), (float)(v[m] - (double)kv[m]));
-    }
-    op->d_kk = (int2*)op->dalloc(sizeof(int2) * M);
-    op->d_ff = (cpx<float>*)op->dalloc(sizeof(cpx<float>) * M);
-    PF_HIP(hipMemcpy(op->d_kk, kk.data(), sizeof(int2) * M, hipMemcpyHostToDevice));
-    PF_HIP(hipMemcpy(op->d_ff, ff.data(), sizeof(cpx<float>) * M, hipMemcpyHostToDevice));
-    if (!op->unit_weights) {
-      std::vector<cpx<float>> w(M);
-      for (int64_t i = 0; i < M; ++i)
-        w[i] = mk<float>((float)d.weights[2 * perm[i]], (float)d.weights[2 * perm[i] + 1]);
-      op->d_weights = op->dalloc(sizeof(cpx<float>) * M);
-      PF_HIP(hipMemcpy(op->d_weights, w.data(), sizeof(cpx<float>) * M, hipMemcpyHostToDevice));
-    }
-  } else {
-    std::vector<cpx<double>> uv(M);
-    for (int64_t i = 0; i < M; ++i) uv[i] = mk<double>(u[perm[i]], v[perm[i]]);
-    op->d_uv = (cpx<double>*)op->dalloc(sizeof(cpx<double>) * M);
-    PF_HIP(hipMemcpy(op->d_uv, uv.data(), sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
-    if (!op->unit_weights) {
-      std::vector<cpx<double>> w(M);
-      for (int64_t i = 0; i < M; ++i) w[i] = mk<double>(d.weights[2 * perm[i]], d.weights[2 * perm[i] + 1]);
-      op->d_weights = op->dalloc(sizeof(cpx<double>) * M);
-      PF_HIP(hipMemcpy(op->d_weights, w.data(), sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
-    }
-  }
+  PF_HIP(hipDeviceSynchronize());
 
   build_tail(op, d);
 }
