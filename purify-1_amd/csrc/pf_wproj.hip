@@ -4,12 +4,13 @@
 // projection_kernels::projection, cpp/purify/projection_kernels.cc:59-85).
 //
 //  * degrid_wrows: one 256-thread block per owner tile stages the tile and its (K - 1)-wide halo in LDS;
-//    each wave then takes one visibility at a time, its lanes striding over the row's taps (coalesced
-//    reads of the stored row), and sums the products with a fixed butterfly: deterministic.
-//  * grid_wrows: one wave per tile accumulates in LDS; for each visibility of the tile's list (in
-//    sorted order) the lanes add the taps that fall inside the tile.  The taps of one visibility are
-//    distinct grid cells (K <= N), so the lanes' read-modify-writes never collide and every cell's sum
-//    follows the list order: reproducible bit for bit.
+//    each wave then takes eight visibilities at a time, eight lanes each striding over a row's taps, and
+//    each group sums its products with a fixed butterfly: deterministic.
+//  * grid_wrows: one wave per tile accumulates in LDS; the tile's list (in sorted order) is walked in
+//    batches of 64 entries loaded together, one entry at a time with the next entry's row prefetched,
+//    the lanes adding the taps that fall inside the tile.  The taps of one visibility are distinct grid
+//    cells (K <= N), so the lanes' read-modify-writes never collide and every cell's sum follows the
+//    list order: reproducible bit for bit.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -90,6 +91,12 @@ __device__ inline int wrapw(int a, int n) { return a & (n - 1); }
 constexpr int WD_THREADS = 256;
 constexpr int WD_ROWS = 4;  // staged rows per wave per round
 
+constexpr int WG_LANES = 8;  // lanes per visibility in degridding: 8 visibilities per wave step
+
+// Degridding: the block stages its owner tile and the (K - 1)-wide halo; each wave then takes 8
+// visibilities at a time, 8 lanes each: lane r of a group sums the taps k = r, r + 8, ... of its
+// visibility's stored row (8 consecutive taps per step: one 64-byte segment of the row for complex64),
+// and the group adds its 8 partial sums with a fixed butterfly.
 template <typename R>
 __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRows<R> w, const int32_t* __restrict__ perm,
                                                              TileLists tl, const cpx<R>* __restrict__ grid,
@@ -119,24 +126,40 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRo
   }
   __syncthreads();
   const int KK = w.Ku * w.Kv;
-  for (uint32_t i = sl.y + wave; i < sl.z; i += nw) {
-    const int2 k0 = w.kw[i];
-    const int base = (wrapw(k0.y + 1, Nv) - ty0) * W + (wrapw(k0.x + 1, Nu) - tx0);
-    const cpx<R>* row = w.vals + (int64_t)i * KK;
+  const int grp = lane / WG_LANES, r = lane % WG_LANES;
+  constexpr int GPW = 64 / WG_LANES;  // visibilities per wave step
+  for (uint32_t i0 = sl.y + (uint32_t)wave * GPW; i0 < sl.z; i0 += (uint32_t)nw * GPW) {
+    const uint32_t i = i0 + (uint32_t)grp;
     cpx<R> acc = mk<R>(R(0), R(0));
-    for (int k = lane; k < KK; k += 64) {
-      const int jv = k / w.Ku, ju = k - jv * w.Ku;
-      const cpx<R> g = lds[base + jv * W + ju];
-      const cpx<R> a = row[k];
-      acc.x += a.x * g.x - a.y * g.y;
-      acc.y += a.x * g.y + a.y * g.x;
+    if (i < sl.z) {
+      const int2 k0 = w.kw[i];
+      const int base = (wrapw(k0.y + 1, Nv) - ty0) * W + (wrapw(k0.x + 1, Nu) - tx0);
+      const cpx<R>* row = w.vals + (int64_t)i * KK;
+      for (int k = r; k < KK; k += WG_LANES) {
+        const int jv = k / w.Ku, ju = k - jv * w.Ku;
+        const cpx<R> g = lds[base + jv * W + ju];
+        const cpx<R> a = row[k];
+        acc.x += a.x * g.x - a.y * g.y;
+        acc.y += a.x * g.y + a.y * g.x;
+      }
     }
-    acc = wave_sum(acc);
-    if (lane == 0) y[perm ? perm[i] : i] = scale(acc, scl);
+#pragma unroll
+    for (int m = WG_LANES / 2; m >= 1; m >>= 1) {
+      acc.x += __shfl_xor(acc.x, m, 64);
+      acc.y += __shfl_xor(acc.y, m, 64);
+    }
+    if (r == 0 && i < sl.z) y[perm ? perm[i] : i] = scale(acc, scl);
   }
 }
 
-template <typename R>
+// Gridding: one wave per tile accumulates in LDS.  The tile's list is taken in batches of 64 entries:
+// each lane loads one entry's index, tap origin and value, then the wave walks the batch entry by entry
+// (broadcasting the entry from its lane), every lane adding its taps of the entry's stored row that
+// fall inside the tile, with the next D entries' row segments already requested.  The taps of one
+// visibility are distinct cells (K <= N), so the lanes' read-modify-writes never collide and each
+// cell's sum follows the list order: reproducible bit for bit.  TPL: taps per lane (KK <= 64 TPL); D:
+// the prefetch depth in entries (D * TPL row segments in registers).
+template <typename R, int TPL, int D>
 __global__ void __launch_bounds__(64) k_grid_wrows(int Nu, int Nv, WRows<R> w, TileLists tl,
                                                    const cpx<R>* __restrict__ ys, cpx<R>* __restrict__ grid, R scl) {
   extern __shared__ unsigned char smem_raw[];
@@ -150,20 +173,66 @@ __global__ void __launch_bounds__(64) k_grid_wrows(int Nu, int Nv, WRows<R> w, T
   for (int e = lane; e < TU * TV; e += 64) acc[e] = mk<R>(R(0), R(0));
   __syncthreads();
   const int KK = w.Ku * w.Kv;
-  for (uint32_t e = sl.y; e < sl.z; ++e) {
-    const uint32_t i = tl.list[e];
-    const int2 k0 = w.kw[i];
-    const cpx<R> s = scale(ys[i], scl);
-    const cpx<R>* row = w.vals + (int64_t)i * KK;
-    for (int k = lane; k < KK; k += 64) {
-      const int jv = k / w.Ku, ju = k - jv * w.Ku;
-      const int x = wrapw(k0.x + 1 + ju - tx0, Nu), yy = wrapw(k0.y + 1 + jv - ty0, Nv);
-      if (x < TU && yy < TV) {
-        const cpx<R> a = row[k];
-        // conj(G) * y
-        cpx<R>& c = acc[yy * TU + x];
-        c.x += a.x * s.x + a.y * s.y;
-        c.y += a.x * s.y - a.y * s.x;
+  // this lane's taps: k = lane + 64 t, their row and column offsets (lanes past the row have none: they
+  // must not touch any cell, or they would race the lane that holds the same cell's tap)
+  int dv[TPL], du[TPL];
+  bool has[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int k = lane + 64 * t;
+    has[t] = k < KK;
+    dv[t] = has[t] ? k / w.Ku : 0;
+    du[t] = has[t] ? k - (k / w.Ku) * w.Ku : 0;
+  }
+  for (uint32_t b0 = sl.y; b0 < sl.z; b0 += 64) {
+    const int n = (int)min(64u, sl.z - b0);
+    uint32_t my_i = 0;
+    int2 my_k = make_int2(0, 0);
+    cpx<R> my_s = mk<R>(R(0), R(0));
+    if (lane < n) {
+      my_i = tl.list[b0 + lane];
+      my_k = w.kw[my_i];
+      my_s = scale(ys[my_i], scl);
+    }
+    // a ring of the next D entries' row segments, indexed statically (the loop is unrolled by it)
+    cpx<R> ring[D][TPL];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const uint32_t i = __shfl(my_i, d, 64);
+      const cpx<R>* row = w.vals + (int64_t)i * KK;
+#pragma unroll
+      for (int t = 0; t < TPL; ++t)
+        ring[d][t] = (d < n && has[t]) ? row[lane + 64 * t] : mk<R>(R(0), R(0));
+    }
+    for (int j0 = 0; j0 < n; j0 += D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int j = j0 + d;
+        if (j >= n) break;
+        cpx<R> a[TPL];
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) a[t] = ring[d][t];
+        if (j + D < n) {  // entry j + D's row, requested before this entry's updates
+          const uint32_t i1 = __shfl(my_i, j + D, 64);
+          const cpx<R>* row = w.vals + (int64_t)i1 * KK;
+#pragma unroll
+          for (int t = 0; t < TPL; ++t)
+            if (has[t]) ring[d][t] = row[lane + 64 * t];
+        }
+        const int kx = __shfl(my_k.x, j, 64), ky = __shfl(my_k.y, j, 64);
+        cpx<R> s;
+        s.x = __shfl(my_s.x, j, 64);
+        s.y = __shfl(my_s.y, j, 64);
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+          const int x = wrapw(kx + 1 + du[t] - tx0, Nu), yy = wrapw(ky + 1 + dv[t] - ty0, Nv);
+          if (has[t] && x < TU && yy < TV) {
+            // conj(G) * y
+            cpx<R>& c = acc[yy * TU + x];
+            c.x += a[t].x * s.x + a[t].y * s.y;
+            c.y += a[t].x * s.y - a[t].y * s.x;
+          }
+        }
       }
     }
   }
@@ -190,9 +259,21 @@ template <typename R>
 void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx<R>* ys, cpx<R>* grid, R scale,
                 hipStream_t s) {
   const size_t shm = sizeof(cpx<R>) * (size_t)t.TU * t.TV;
+  const int KK = w.Ku * w.Kv;
+  const dim3 blocks((unsigned)(t.ntu * t.ntv));
   ProfScope prof("grid_wrows", s);
-  hipLaunchKernelGGL((k_grid_wrows<R>), dim3((unsigned)(t.ntu * t.ntv)), dim3(64), shm, s, Nu, Nv, w, t, ys, grid,
-                     scale);
+  if (KK <= 64)
+    hipLaunchKernelGGL((k_grid_wrows<R, 1, 8>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+  else if (KK <= 128)
+    hipLaunchKernelGGL((k_grid_wrows<R, 2, 8>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+  else if (KK <= 256)
+    hipLaunchKernelGGL((k_grid_wrows<R, 4, 4>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+  else if (KK <= 576)
+    hipLaunchKernelGGL((k_grid_wrows<R, 9, 2>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+  else if (KK <= 1024)
+    hipLaunchKernelGGL((k_grid_wrows<R, 16, 1>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+  else
+    throw Error(PF_ERR_UNSUPPORTED, "w-projection rows above 1024 taps");
   PF_HIP(hipGetLastError());
 }
 
