@@ -262,14 +262,18 @@ void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const 
 template <typename R>
 void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx<R>* ys, cpx<R>* grid, R scale,
                 hipStream_t s);
-// Host: the projection row of one visibility, FP64, into out[2 * Kv * Ku] (interleaved complex),
-// phase and weight included.  kind / Jeval: the kernel in the u slot (the reference evaluates both
-// axes with it, projection_kernels.cc:72-73).
-struct WRowSpec {
-  int kind = 0, Jeval = 4, Ju = 4, Jv = 4, Jw = 6;
-  double os = 2, du_lambda = 1, dv_lambda = 1;  // pixel_to_lambda of each axis
+// Construction of the projection rows on the device, FP64: every row of Kv x Ku values for the sorted
+// visibility i (the caller's perm[i]), phase and weight included.  kp: the kernel in the u slot (the
+// reference evaluates both axes with it, projection_kernels.cc:72-73); du_l, dv_l: pixel_to_lambda of each
+// axis; wmean: the mean of w (operators.h:323).
+struct WRowArgs {
+  int Ju = 4, Jv = 4, Jw = 6, Ku = 9, Kv = 9;
+  double du_l = 1, dv_l = 1, wmean = 0;
+  KParams<double> kp;
 };
-void wrow_host(const WRowSpec& sp, double u, double v, double w, double wre, double wim, double* out);
+template <typename R>
+void wrows_build(const WRowArgs& g, const cpx<double>* uv, const double* w, const cpx<double>* wt, const int32_t* perm,
+                 int64_t M, cpx<R>* out);
 
 // ---- chirp-truncated w-projection rows for a whole operator (pf_chirp.hip) ----------------------
 struct ChirpBuild;

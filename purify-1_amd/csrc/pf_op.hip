@@ -529,51 +529,32 @@ void to_pixels(const pf_op_desc& d, int Nu, int Nv, std::vector<double>& u, std:
 // device in chunks in the operator's precision.  G receives w - mean(w) (operators.h:323) and both axes
 // of the projection use the u-slot kernel, which is the v-support kernel after the slot swap
 // (operators.h:324, projection_kernels.cc:72-73).
-void build_wrows(pf_op* op, const pf_op_desc& d, const std::vector<double>& u, const std::vector<double>& v,
-                 const std::vector<int32_t>& perm) {
+// The projection rows of operators.cc:49-99 (w_term = 1), built on the device from the caller's pixel
+// coordinates d_uv, w and weights into the sorted order d_perm.
+void build_wrows(pf_op* op, const pf_op_desc& d, const cpx<double>* d_uv, const cpx<double>* d_wt,
+                 const int32_t* d_perm, DevTemp& tmp) {
   const int64_t M = d.M;
   const int KK = op->Ku * op->Kv;
   op->nnz_w = M * KK;
   double mean = 0;
   for (int64_t m = 0; m < M; ++m) mean += d.w[m];
   if (M > 0) mean /= (double)M;
-  WRowSpec sp;
-  sp.kind = d.kernel;
-  sp.Jeval = d.Jv;
-  sp.Ju = op->Ju;
-  sp.Jv = op->Jv;
-  sp.Jw = d.Jw;
-  sp.os = op->os;
+  WRowArgs g;
+  g.Ju = op->Ju;
+  g.Jv = op->Jv;
+  g.Jw = d.Jw;
+  g.Ku = op->Ku;
+  g.Kv = op->Kv;
+  g.kp = make_kparams<double>(d.kernel, d.Jv, op->os);
   // pixel_to_lambda (projection_kernels.cc:28-30)
-  sp.du_lambda = 60. * 60. * 180. / d.cellx / std::floor(op->os * op->nx) / PI;
-  sp.dv_lambda = 60. * 60. * 180. / d.celly / std::floor(op->os * op->ny) / PI;
-  const size_t esz = op->esz();
-  op->d_wrows = op->dalloc(esz * (size_t)std::max<int64_t>(op->nnz_w, 1));
-  const int64_t CH = 1 << 15;  // visibilities per chunk
-  std::vector<double> tmp((size_t)std::min<int64_t>(M, CH) * KK * 2 + 2);
-  std::vector<float> tmpf(op->prec == PF_C64 ? tmp.size() : 0);
-  const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
-  for (int64_t c0 = 0; c0 < M; c0 += CH) {
-    const int64_t n = std::min(CH, M - c0);
-    auto work = [&](int t) {
-      for (int64_t k = t; k < n; k += nth) {
-        const int64_t m = perm[c0 + k];
-        const double wre = d.weights ? d.weights[2 * m] : 1.0, wim = d.weights ? d.weights[2 * m + 1] : 0.0;
-        wrow_host(sp, u[m], v[m], d.w[m] - mean, wre, wim, tmp.data() + (size_t)k * KK * 2);
-      }
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
-    work(0);
-    for (auto& th : pool) th.join();
-    char* dst = static_cast<char*>(op->d_wrows) + (size_t)c0 * KK * esz;
-    if (op->prec == PF_C64) {
-      for (size_t q = 0; q < (size_t)n * KK * 2; ++q) tmpf[q] = (float)tmp[q];
-      PF_HIP(hipMemcpy(dst, tmpf.data(), (size_t)n * KK * esz, hipMemcpyHostToDevice));
-    } else {
-      PF_HIP(hipMemcpy(dst, tmp.data(), (size_t)n * KK * esz, hipMemcpyHostToDevice));
-    }
-  }
+  g.du_l = 60. * 60. * 180. / d.cellx / std::floor(op->os * op->nx) / PI;
+  g.dv_l = 60. * 60. * 180. / d.celly / std::floor(op->os * op->ny) / PI;
+  g.wmean = mean;
+  double* d_w = tmp.alloc<double>(M);
+  if (M) PF_HIP(hipMemcpy(d_w, d.w, sizeof(double) * M, hipMemcpyHostToDevice));
+  op->d_wrows = op->dalloc(op->esz() * (size_t)std::max<int64_t>(op->nnz_w, 1));
+  if (op->prec == PF_C64) wrows_build<float>(g, d_uv, d_w, d_wt, d_perm, M, static_cast<cpx<float>*>(op->d_wrows));
+  else wrows_build<double>(g, d_uv, d_w, d_wt, d_perm, M, static_cast<cpx<double>*>(op->d_wrows));
 }
 
 void build_tail(pf_op* op, const pf_op_desc& d);
@@ -751,10 +732,7 @@ void build(pf_op* op, const pf_op_desc& d) {
     op->nnz_w = chirp_rows_fill(*chirp, d_perm, op->d_wrows, op->prec);
     chirp.reset();
   } else if (op->w_term) {
-    // projection rows, built on the host threads from the sorted order
-    std::vector<int32_t> perm((size_t)M);
-    if (M) PF_HIP(hipMemcpy(perm.data(), d_perm, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
-    build_wrows(op, d, u, v, perm);
+    build_wrows(op, d, d_uvc, d_wtc, d_perm, tmp);
   } else if (grid_mfma_ok(op->Nu, op->Nv, op->Ju, op->Jv) && (op->prec != PF_C64 || M < ((int64_t)1 << 29))) {
     op->tile_list_len_m = device_lists(op, d_kk, M, op->Ju, op->Jv, MFMA_TILE, MFMA_TILE, op->tlm);
     op->grid_variant = GRID_MFMA;

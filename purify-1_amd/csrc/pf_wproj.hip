@@ -1,7 +1,7 @@
 // pf_wproj.hip -- the w-projection gridding matrix (w_term operators): rows of (Jv + Jw - 1) x
 // (Ju + Jw - 1) taps per visibility, built once on the host in FP64 and stored in HBM in the operator's
 // precision (replaces details::init_gridding_matrix_2d with w_term, cpp/purify/operators.cc:49-99, and
-// projection_kernels::projection, cpp/purify/projection_kernels.cc:59-85).
+// projection_kernels::projection, cpp/purify/projection_kernels.cc:59-85); built on the device in FP64.
 //
 //  * degrid_wrows: one 256-thread block per owner tile stages the tile and its (K - 1)-wide halo in LDS;
 //    each wave then takes eight visibilities at a time, eight lanes each striding over a row's taps, and
@@ -21,67 +21,96 @@
 namespace pf {
 
 // ---------------------------------------------------------------------------------------------
-// Host: one projection row in FP64
+// Construction: the projection rows in FP64 on the device, one thread per (visibility, row of taps)
 // ---------------------------------------------------------------------------------------------
 namespace {
-const double PI_W = 3.14159265358979323846;
+constexpr double PI_W = 3.14159265358979323846;
 
 // w_projection_kernel_approx (projection_kernels.cc:31-47) with w_proj_approx (16-22): the chirp
 // exp(-i pi (a^2 + b^2) / w) / (i w) at a = u du, b = v dv inside |a/w| < 1, |b/w| < 1, else 1.
-std::complex<double> wkernel_approx(double u, double v, double w, double du, double dv) {
-  if (std::abs(u * du / w) < 1 && std::abs(v * dv / w) < 1) {
+__device__ inline cpx<double> wkernel_approx(double u, double v, double w, double du, double dv) {
+  if (fabs(u * du / w) < 1 && fabs(v * dv / w) < 1) {
     const double a = u * du, b = v * dv;
-    const std::complex<double> I(0.0, 1.0);
-    return std::exp(-PI_W * I * (a * a + b * b) / w) / w / I;
+    double sn, cs;
+    sincos(-PI_W * (a * a + b * b) / w, &sn, &cs);
+    // (cs + i sn) / w / i
+    return mk<double>(sn / w, -cs / w);
   }
-  return 1.0;
+  return mk<double>(1.0, 0.0);
+}
+
+// Row r (of Kv) of the projection row of sorted visibility i, as projection_kernels.cc:59-85 composes
+// it: 1-D kernel samples on both axes from the u-slot kernel (projection_kernels.cc:66-74), the Jw x Jw
+// w-kernel samples g[jv][ju] (75-80), linear_convol_2d (convolution.h:54-67: columns with fv, then rows
+// with fu; a 1-D pass gives out[j] = sum_t f[t] x[Jx - 1 - j + t] over the valid indices of x,
+// convolution.h:43-52), the exact (-1)^(kw_u + c + 1 + kw_v + r + 1) phase of operators.cc:93 and the
+// weight; w enters as w - mean(w) (operators.h:323).
+template <typename R>
+__global__ void k_wrows_build(WRowArgs g, const cpx<double>* __restrict__ uv, const double* __restrict__ w,
+                              const cpx<double>* __restrict__ wt, const int32_t* __restrict__ perm, int64_t M,
+                              cpx<R>* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / g.Kv;
+  const int r = (int)(t - i * g.Kv);
+  if (i >= M) return;
+  const int64_t m = perm[i];
+  const cpx<double> c = uv[m];
+  const double wv = w[m] - g.wmean;
+  double fu[16], fv[16];
+  const double du = c.x - floor(c.x - g.Ju * 0.5);
+  for (int a = 0; a < g.Ju; ++a) fu[a] = kern1d<double>(g.kp, du - a - 1);
+  const double dv = c.y - floor(c.y - g.Jv * 0.5);
+  for (int b = 0; b < g.Jv; ++b) fv[b] = kern1d<double>(g.kp, dv - b - 1);
+  const double dwu = c.x - floor(c.x - g.Jw * 0.5), dwv = c.y - floor(c.y - g.Jw * 0.5);
+  // buf[q] = row r of the column pass: sum_t fv[t] g[Jw - 1 - r + t][q]
+  cpx<double> buf[16];
+  for (int q = 0; q < g.Jw; ++q) {
+    double sr = 0, si = 0;
+    for (int tt = 0; tt < g.Jv; ++tt) {
+      const int k = g.Jw - 1 - r + tt;
+      if (k >= 0 && k < g.Jw) {
+        const cpx<double> gk = wkernel_approx(dwu - q - 1, dwv - k - 1, wv, g.du_l, g.dv_l);
+        sr += fv[tt] * gk.x;
+        si += fv[tt] * gk.y;
+      }
+    }
+    buf[q] = mk<double>(sr, si);
+  }
+  const cpx<double> wgt = wt ? wt[m] : mk<double>(1.0, 0.0);
+  const double kwu = floor(c.x - g.Ku * 0.5), kwv = floor(c.y - g.Kv * 0.5);
+  cpx<R>* o = out + i * (int64_t)g.Ku * g.Kv + (int64_t)r * g.Ku;
+  for (int col = 0; col < g.Ku; ++col) {
+    double sr = 0, si = 0;
+    for (int tt = 0; tt < g.Ju; ++tt) {
+      const int k = g.Jw - 1 - col + tt;
+      if (k >= 0 && k < g.Jw) {
+        sr += fu[tt] * buf[k].x;
+        si += fu[tt] * buf[k].y;
+      }
+    }
+    const long long par = (long long)kwu + (long long)kwv + col + r + 2;
+    if (par & 1) {
+      sr = -sr;
+      si = -si;
+    }
+    o[col] = mk<R>((R)(sr * wgt.x - si * wgt.y), (R)(sr * wgt.y + si * wgt.x));
+  }
 }
 }  // namespace
 
-void wrow_host(const WRowSpec& sp, double u, double v, double w, double wre, double wim, double* out) {
-  const int Ju = sp.Ju, Jv = sp.Jv, Jw = sp.Jw;
-  const int Ku = Jw + Ju - 1, Kv = Jw + Jv - 1;
-  // 1-D kernel samples; both axes use the u-slot kernel (projection_kernels.cc:66-74)
-  double fu[16], fv[16];
-  const double du = u - std::floor(u - Ju * 0.5);
-  for (int a = 0; a < Ju; ++a) fu[a] = host_kernel(sp.kind, du - a - 1, sp.Jeval, sp.os);
-  const double dv = v - std::floor(v - Jv * 0.5);
-  for (int b = 0; b < Jv; ++b) fv[b] = host_kernel(sp.kind, dv - b - 1, sp.Jeval, sp.os);
-  // Jw x Jw w-kernel samples g[jv][ju] (projection_kernels.cc:75-80)
-  std::complex<double> g[16][16];
-  const double dwu = u - std::floor(u - Jw * 0.5), dwv = v - std::floor(v - Jw * 0.5);
-  for (int jv = 0; jv < Jw; ++jv)
-    for (int ju = 0; ju < Jw; ++ju)
-      g[jv][ju] = wkernel_approx(dwu - ju - 1, dwv - jv - 1, w, sp.du_lambda, sp.dv_lambda);
-  // linear_convol_2d (convolution.h:54-67): columns with fv, then rows with fu.  A 1-D pass gives
-  // out[j] = sum_t f[t] * x[Jx - 1 - j + t] over the valid indices of x (convolution.h:43-52).
-  std::complex<double> buf[32][16];
-  for (int i = 0; i < Jw; ++i)
-    for (int r = 0; r < Kv; ++r) {
-      std::complex<double> acc = 0.0;
-      for (int t = 0; t < Jv; ++t) {
-        const int k = Jw - 1 - r + t;
-        if (k >= 0 && k < Jw) acc += fv[t] * g[k][i];
-      }
-      buf[r][i] = acc;
-    }
-  const std::complex<double> wt(wre, wim);
-  const double ku = std::floor(u - Ku * 0.5), kv = std::floor(v - Kv * 0.5);
-  for (int r = 0; r < Kv; ++r)
-    for (int c = 0; c < Ku; ++c) {
-      std::complex<double> acc = 0.0;
-      for (int t = 0; t < Ju; ++t) {
-        const int k = Jw - 1 - c + t;
-        if (k >= 0 && k < Jw) acc += fu[t] * buf[r][k];
-      }
-      // (-1)^(kw_u + c + 1 + kw_v + r + 1): the exact value of the reference's
-      // exp(-2 pi i ((kw_u + ju) / 2 + (kw_v + jv) / 2)) (operators.cc:93)
-      const long long par = (long long)ku + (long long)kv + c + r + 2;
-      const std::complex<double> val = ((par & 1) ? -acc : acc) * wt;
-      out[2 * (r * Ku + c)] = val.real();
-      out[2 * (r * Ku + c) + 1] = val.imag();
-    }
+template <typename R>
+void wrows_build(const WRowArgs& g, const cpx<double>* uv, const double* w, const cpx<double>* wt, const int32_t* perm,
+                 int64_t M, cpx<R>* out) {
+  if (M == 0) return;
+  const int64_t n = M * g.Kv;
+  hipLaunchKernelGGL((k_wrows_build<R>), dim3((unsigned)((n + 127) / 128)), dim3(128), 0, nullptr, g, uv, w, wt, perm, M,
+                     out);
+  PF_HIP(hipGetLastError());
 }
+template void wrows_build<float>(const WRowArgs&, const cpx<double>*, const double*, const cpx<double>*, const int32_t*,
+                                 int64_t, cpx<float>*);
+template void wrows_build<double>(const WRowArgs&, const cpx<double>*, const double*, const cpx<double>*,
+                                  const int32_t*, int64_t, cpx<double>*);
 
 // ---------------------------------------------------------------------------------------------
 // Device
