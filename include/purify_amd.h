@@ -362,6 +362,18 @@ void pf_csr_free(pf_csr *m);
  * themselves (operators.h:123-145); this one carries a precomputed G into the same operator. */
 int pf_op_create_csr(const pf_op_desc *desc, const pf_csr *G, pf_op **out);
 
+/* ---- device memory ----
+ * Handles (operators, solvers, wavelets, PSFs, FFT plans) and their construction temporaries take device
+ * memory from the library's per-device block cache, and their destroy functions return it there without a
+ * device-wide synchronisation (hipFree would wait for every stream on the device).  Hence: before
+ * destroying a handle, synchronise the streams you passed to it (the library synchronises its own).  The
+ * cache keeps at most 16 GiB per device and is released on allocation failure.  Neither function has a
+ * counterpart in the reference, whose Eigen/FFTW operators live in host memory.
+ * pf_cache_stats: bytes in blocks held by live handles, and bytes cached for reuse.
+ * pf_cache_release: hipFree every cached block (this waits for the device); released_bytes may be NULL. */
+int pf_cache_stats(int64_t *live_bytes, int64_t *cached_bytes);
+int pf_cache_release(int64_t *released_bytes);
+
 /* ---- errors ---- */
 const char *pf_last_error(void);
 /* Library version string. */

@@ -21,7 +21,7 @@ OUT = os.path.join(HERE, "purify_amd", "libpurify_amd.so")
 ARCH = os.environ.get("PURIFY_AMD_ARCH", "gfx950")
 
 SOURCES = ["pf_fft.hip", "pf_grid.hip", "pf_wproj.hip", "pf_wmat.hip", "pf_wavelet.hip", "pf_op.hip", "pf_solver.hip",
-           "pf_noalloc.hip", "pf_chirp.hip", "pf_build.hip"]
+           "pf_noalloc.hip", "pf_chirp.hip", "pf_build.hip", "pf_mem.hip"]
 
 CFLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",

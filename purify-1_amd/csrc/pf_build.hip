@@ -168,14 +168,14 @@ __global__ void k_plan_g1p2(const int32_t* __restrict__ order2, const int32_t* _
 struct Scratch {
   std::vector<void*> bufs;
   ~Scratch() {
-    for (void* p : bufs) (void)hipFree(p);
+    cstream_sync();
+    for (void* p : bufs) dev_free(p);
   }
   template <typename T>
   T* alloc(size_t n) {
-    void* p = nullptr;
-    PF_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)));
+    T* p = dev_alloc_n<T>(n);
     bufs.push_back(p);
-    return static_cast<T*>(p);
+    return p;
   }
 };
 
@@ -196,7 +196,7 @@ void sort_pairs(Scratch& sc, const uint32_t* kin, uint32_t* kout, const V* vin, 
 // caller) and the per-tile owner counts on the host.
 void dev_sort_visibilities(const cpx<double>* d_uv, int64_t M, int Nu, int Nv, int Ku, int Kv, int TU, int TV,
                            int32_t* d_perm, std::vector<uint32_t>& owner_count) {
-  hipStream_t s = nullptr;
+  hipStream_t s = cstream();
   Scratch sc;
   const int ntu = Nu / TU, ntv = Nv / TV, nt = ntu * ntv;
   owner_count.assign((size_t)nt, 0);
@@ -226,7 +226,7 @@ template <typename R>
 void dev_gather_sorted(const cpx<double>* d_uv, const cpx<double>* d_wt, const int32_t* d_perm, int64_t M, int Ku,
                        int Kv, int2* kk, cpx<float>* ff, cpx<double>* uvs, cpx<R>* ws) {
   if (M == 0) return;
-  hipLaunchKernelGGL((k_gather<R>), dim3(nblk(M)), dim3(256), 0, nullptr, d_uv, d_wt, d_perm, M, Ku, Kv, kk, ff, uvs,
+  hipLaunchKernelGGL((k_gather<R>), dim3(nblk(M)), dim3(256), 0, cstream(), d_uv, d_wt, d_perm, M, Ku, Kv, kk, ff, uvs,
                      ws);
   PF_HIP(hipGetLastError());
 }
@@ -236,10 +236,10 @@ template void dev_gather_sorted<double>(const cpx<double>*, const cpx<double>*, 
                                         int2*, cpx<float>*, cpx<double>*, cpx<double>*);
 
 // Tile lists of TU x TV tiles for footprints Ku x Kv at the sorted tap origins kk: the device list
-// (hipMalloc'd; the caller owns it), its length, and the per-tile entry counts on the host.
+// (from dev_alloc; the caller owns it), its length, and the per-tile entry counts on the host.
 uint32_t* dev_tile_lists(const int2* kk, int64_t M, int Nu, int Nv, int Ku, int Kv, int TU, int TV,
                          std::vector<uint32_t>& count, int64_t& len) {
-  hipStream_t s = nullptr;
+  hipStream_t s = cstream();
   Scratch sc;
   const int ntu = Nu / TU, ntv = Nv / TV, nt = ntu * ntv;
   count.assign((size_t)nt, 0);
@@ -248,8 +248,7 @@ uint32_t* dev_tile_lists(const int2* kk, int64_t M, int Nu, int Nv, int Ku, int 
     throw Error(PF_ERR_UNSUPPORTED, "footprint spans too many tiles");
   uint32_t* list = nullptr;
   if (M == 0) {
-    PF_HIP(hipMalloc(&list, sizeof(uint32_t)));
-    return list;
+    return dev_alloc_n<uint32_t>(1);
   }
   uint64_t* nt_i = sc.alloc<uint64_t>(M);
   uint64_t* off = sc.alloc<uint64_t>(M);
@@ -271,7 +270,7 @@ uint32_t* dev_tile_lists(const int2* kk, int64_t M, int Nu, int Nv, int Ku, int 
   uint32_t* tkey = sc.alloc<uint32_t>(total);
   uint32_t* tkey_s = sc.alloc<uint32_t>(total);
   uint32_t* tval = sc.alloc<uint32_t>(total);
-  PF_HIP(hipMalloc(&list, sizeof(uint32_t) * std::max<uint64_t>(total, 1)));
+  list = dev_alloc_n<uint32_t>((size_t)total);
   try {
     hipLaunchKernelGGL(k_touch_fill, dim3(nblk(M)), dim3(256), 0, s, kk, off, M, Nu, Nv, Ku, Kv, TU, TV, ntu, tkey,
                        tval);
@@ -284,7 +283,8 @@ uint32_t* dev_tile_lists(const int2* kk, int64_t M, int Nu, int Nv, int Ku, int 
     PF_HIP(hipMemcpyAsync(count.data(), cnt, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, s));
     PF_HIP(hipStreamSynchronize(s));
   } catch (...) {
-    (void)hipFree(list);
+    cstream_sync();
+    dev_free(list);
     throw;
   }
   return list;
@@ -292,20 +292,20 @@ uint32_t* dev_tile_lists(const int2* kk, int64_t M, int Nu, int Nv, int Ku, int 
 
 void dev_entries(const uint32_t* list, int64_t n, const int2* kk, const cpx<float>* ff, int Nu, int Nv, int4* ent) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_entries, dim3(nblk(n)), dim3(256), 0, nullptr, list, n, kk, ff, Nu, Nv, ent);
+  hipLaunchKernelGGL(k_entries, dim3(nblk(n)), dim3(256), 0, cstream(), list, n, kk, ff, Nu, Nv, ent);
   PF_HIP(hipGetLastError());
 }
 
 void dev_inverse(const int32_t* perm, int64_t M, int32_t* inv) {
   if (M == 0) return;
-  hipLaunchKernelGGL(k_inverse, dim3(nblk(M)), dim3(256), 0, nullptr, perm, M, inv);
+  hipLaunchKernelGGL(k_inverse, dim3(nblk(M)), dim3(256), 0, cstream(), perm, M, inv);
   PF_HIP(hipGetLastError());
 }
 
 // The two-pass permutation plan (VisPerm) of perm with chunks and buckets of S: l1, g1 and p2 (device).
 void dev_perm_plan(const int32_t* perm, int64_t M, int S, int nb, uint16_t* l1, int32_t* g1, uint16_t* p2) {
   if (M == 0) return;
-  hipStream_t s = nullptr;
+  hipStream_t s = cstream();
   Scratch sc;
   int32_t* inv = sc.alloc<int32_t>(M);
   dev_inverse(perm, M, inv);

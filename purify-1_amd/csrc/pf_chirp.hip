@@ -256,9 +256,7 @@ __global__ void __launch_bounds__(64) k_chirp_rows(ChirpArgs g) {
 
 template <typename T>
 T* cr_alloc(size_t n) {
-  void* p = nullptr;
-  PF_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)));
-  return static_cast<T*>(p);
+  return dev_alloc_n<T>(n);
 }
 
 }  // namespace
@@ -269,7 +267,8 @@ struct ChirpBuild {
   int blocks = 0;
   std::vector<void*> bufs;
   ~ChirpBuild() {
-    for (void* p : bufs) (void)hipFree(p);
+    cstream_sync();
+    for (void* p : bufs) dev_free(p);
   }
   template <typename T>
   T* alloc(size_t n) {
@@ -297,7 +296,7 @@ void chirp_basis(ChirpBuild& cb, int N, double cellx, double celly, int Rw, int 
   Fft2dPlan plan;
   try {
     fft_plan_init(plan, N, N, 1);
-    hipStream_t s = nullptr;
+    hipStream_t s = cstream();
     const unsigned nb = (unsigned)(((int64_t)N * N + 255) / 256);
     for (int j = 0; j < JT; ++j) {
       hipLaunchKernelGGL(k_psi_pow, dim3(nb), dim3(256), 0, s, N, dx, dy, j, nz, img);
@@ -308,13 +307,14 @@ void chirp_basis(ChirpBuild& cb, int N, double cellx, double celly, int Rw, int 
     }
     PF_HIP(hipStreamSynchronize(s));
   } catch (...) {
-    (void)hipFree(img);
-    (void)hipFree(ws);
+    cstream_sync();
+    dev_free(img);
+    dev_free(ws);
     fft_plan_free(plan);
     throw;
   }
-  (void)hipFree(img);
-  (void)hipFree(ws);
+  dev_free(img);
+  dev_free(ws);
   fft_plan_free(plan);
   cb.a.basis = basis;
   cb.a.Rw = Rw;
@@ -349,14 +349,14 @@ int chirp_rows_radius(ChirpBuild& cb, int64_t M, int N, int J, int kind, int Jev
   cpx<double>* d_uv = cb.alloc<cpx<double>>(M);
   double* d_w = cb.alloc<double>(M);
   if (M) {
-    PF_HIP(hipMemcpy(d_uv, uv.data(), sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
-    PF_HIP(hipMemcpy(d_w, w, sizeof(double) * M, hipMemcpyHostToDevice));
+    h2d(d_uv, uv.data(), sizeof(cpx<double>) * M);
+    h2d(d_w, w, sizeof(double) * M);
   }
   a.uv = d_uv;
   a.w = d_w;
   if (weights) {
     cpx<double>* d_wt = cb.alloc<cpx<double>>(M);
-    PF_HIP(hipMemcpy(d_wt, weights, sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
+    h2d(d_wt, weights, sizeof(cpx<double>) * M);
     a.wt = d_wt;
   }
   // series length: |a_j Psi_j| <= theta^j / j!, theta = 2 pi max|w| max|psi| (psi is largest at the corner)
@@ -385,23 +385,23 @@ int chirp_rows_radius(ChirpBuild& cb, int64_t M, int N, int J, int kind, int Jev
   for (int Rw = 8;; Rw *= 2) {
     if (Rw > CR_MAXRW || 2 * Rw + 1 > N) throw Error(PF_ERR_UNSUPPORTED, "chirp spectrum wider than the window allows");
     if (a.basis) {
-      (void)hipFree(const_cast<cpx<double>*>(a.basis));
+      dev_free(const_cast<cpx<double>*>(a.basis));
       cb.bufs.erase(std::find(cb.bufs.begin(), cb.bufs.end(), (void*)a.basis));
       a.basis = nullptr;
     }
     chirp_basis(cb, N, cellx, celly, Rw, JT);
     a.R = 0;
-    PF_HIP(hipMemset(d_int, 0, 2 * sizeof(int)));
+    PF_HIP(hipMemsetAsync(d_int, 0, 2 * sizeof(int), cstream()));
     if (chirp_lds(Rw, J, Rw - 1) > 160 * 1024)
       throw Error(PF_ERR_UNSUPPORTED, "chirp spectrum wider than the window allows");
     if (M) {
       const size_t shm = chirp_lds(Rw, J, 0);
       allow_lds((const void*)k_chirp_rows<false>, shm);
-      hipLaunchKernelGGL(k_chirp_rows<false>, dim3(cb.blocks), dim3(64), shm, nullptr, a);
+      hipLaunchKernelGGL(k_chirp_rows<false>, dim3(cb.blocks), dim3(64), shm, cstream(), a);
       PF_HIP(hipGetLastError());
     }
     int h[2] = {0, 0};
-    PF_HIP(hipMemcpy(h, d_int, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    d2h(h, d_int, 2 * sizeof(int));
     if (!h[1]) {
       a.R = h[0];
       return h[0];
@@ -416,17 +416,17 @@ int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_dev, void* rows, int
   a.prec = prec;
   a.rows = rows;
   unsigned long long* d_nnz = cb.alloc<unsigned long long>(1);
-  PF_HIP(hipMemset(d_nnz, 0, sizeof(unsigned long long)));
+  PF_HIP(hipMemsetAsync(d_nnz, 0, sizeof(unsigned long long), cstream()));
   a.perm = perm_dev;
   a.nnz = d_nnz;
   if (a.M) {
     const size_t shm = chirp_lds(a.Rw, a.J, a.R);
     allow_lds((const void*)k_chirp_rows<true>, shm);
-    hipLaunchKernelGGL(k_chirp_rows<true>, dim3(cb.blocks), dim3(64), shm, nullptr, a);
+    hipLaunchKernelGGL(k_chirp_rows<true>, dim3(cb.blocks), dim3(64), shm, cstream(), a);
     PF_HIP(hipGetLastError());
   }
   unsigned long long h = 0;
-  PF_HIP(hipMemcpy(&h, d_nnz, sizeof(h), hipMemcpyDeviceToHost));
+  d2h(&h, d_nnz, sizeof(h));
   return (int64_t)h;
 }
 

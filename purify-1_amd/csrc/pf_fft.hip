@@ -720,18 +720,18 @@ void fft_plan_init(Fft2dPlan& p, int Nv, int Nu, int prec, int col_mode) {
   if (prec) {
     std::vector<cpx<double>> h(p.Lmax);
     for (int m = 0; m < p.Lmax; ++m) h[m] = mk<double>(re[m], im[m]);
-    PF_HIP(hipMalloc(&p.tw, sizeof(cpx<double>) * p.Lmax));
-    PF_HIP(hipMemcpy(p.tw, h.data(), sizeof(cpx<double>) * p.Lmax, hipMemcpyHostToDevice));
+    p.tw = dev_alloc(sizeof(cpx<double>) * p.Lmax);
+    h2d(p.tw, h.data(), sizeof(cpx<double>) * p.Lmax);
   } else {
     std::vector<cpx<float>> h(p.Lmax);
     for (int m = 0; m < p.Lmax; ++m) h[m] = mk<float>((float)re[m], (float)im[m]);
-    PF_HIP(hipMalloc(&p.tw, sizeof(cpx<float>) * p.Lmax));
-    PF_HIP(hipMemcpy(p.tw, h.data(), sizeof(cpx<float>) * p.Lmax, hipMemcpyHostToDevice));
+    p.tw = dev_alloc(sizeof(cpx<float>) * p.Lmax);
+    h2d(p.tw, h.data(), sizeof(cpx<float>) * p.Lmax);
   }
 }
 
 void fft_plan_free(Fft2dPlan& p) {
-  if (p.tw) (void)hipFree(p.tw);
+  dev_free(p.tw);
   p.tw = nullptr;
 }
 

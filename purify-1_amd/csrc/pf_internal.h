@@ -20,6 +20,36 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char* what);
 #define PF_HIP(call) ::pf::hip_check((call), #call)
 
+// ---- device memory and the construction stream ----------------------------------------------
+// Device blocks come from a per-device cache of freed blocks (best fit within a quarter of the request,
+// else hipMalloc): hipFree waits for the whole device, so a handle released while another handle's work
+// runs would stall behind it.  dev_free returns a block to the cache at once: every piece of work that
+// uses it must have completed (the library synchronises its own streams before it frees; a caller
+// synchronises the streams it passed).  The cache holds at most PF_CACHE_MAX bytes per device; above that,
+// and when hipMalloc fails, cached blocks are released with hipFree.
+constexpr size_t PF_CACHE_MAX = size_t(16) << 30;
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+size_t dev_release_cached();
+void dev_cache_stats(int64_t& live_bytes, int64_t& cached_bytes);
+template <typename T>
+T* dev_alloc_n(size_t n) {
+  return static_cast<T*>(dev_alloc(sizeof(T) * (n ? n : 1)));
+}
+// The calling thread's construction stream (non-blocking, default priority; created on first use, on the
+// current device): builds, power methods and other library-internal work run on it rather than on the
+// legacy null stream, which every thread shares and which would order one request's construction behind
+// another's.  h2d / d2h copy on it and wait for the copy.
+hipStream_t cstream();
+void cstream_sync() noexcept;  // waits for the thread's construction stream, if it has one
+void h2d(void* dst, const void* src, size_t bytes);
+void d2h(void* dst, const void* src, size_t bytes);
+// The stream of an apply: the caller's, or, when it passes none and both vectors are on the host (the
+// call is synchronous), the thread's construction stream instead of the shared legacy null stream.
+inline hipStream_t io_stream(void* caller, bool host_io) {
+  return (!caller && host_io) ? cstream() : static_cast<hipStream_t>(caller);
+}
+
 // The calling thread's last error message (pf_last_error) and the C-ABI wrappers that set it.
 std::string& last_error();
 inline int fail(int code, const std::string& msg) {

@@ -381,15 +381,15 @@ struct pf_op {
   std::vector<void*> allocs;
 
   void* dalloc(size_t bytes) {
-    void* p = nullptr;
     if (bytes == 0) bytes = 16;
-    PF_HIP(hipMalloc(&p, bytes));
+    void* p = dev_alloc(bytes);
     allocs.push_back(p);
     device_bytes += (int64_t)bytes;
     return p;
   }
   ~pf_op() {
-    for (void* p : allocs) (void)hipFree(p);
+    cstream_sync();  // a construction that failed part-way may still have work queued
+    for (void* p : allocs) dev_free(p);
     fft_plan_free(fft);
   }
   size_t esz() const { return prec == PF_C128 ? 16 : 8; }
@@ -427,14 +427,14 @@ void parallel_for(int64_t n, F f) {
 struct DevTemp {
   std::vector<void*> bufs;
   ~DevTemp() {
-    for (void* p : bufs) (void)hipFree(p);
+    cstream_sync();
+    for (void* p : bufs) dev_free(p);
   }
   template <typename T>
   T* alloc(int64_t n) {
-    void* p = nullptr;
-    PF_HIP(hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+    T* p = dev_alloc_n<T>((size_t)std::max<int64_t>(n, 1));
     bufs.push_back(p);
-    return static_cast<T*>(p);
+    return p;
   }
 };
 
@@ -455,7 +455,7 @@ int64_t device_lists(pf_op* op, const int2* kk, int64_t M, int Ku, int Kv, int T
   }
   std::stable_sort(slot.begin(), slot.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
   uint4* d_slot = (uint4*)op->dalloc(sizeof(uint4) * (size_t)std::max(nt, 1));
-  PF_HIP(hipMemcpy(d_slot, slot.data(), sizeof(uint4) * nt, hipMemcpyHostToDevice));
+  h2d(d_slot, slot.data(), sizeof(uint4) * nt);
   tl.TU = TU;
   tl.TV = TV;
   tl.ntu = op->Nu / TU;
@@ -551,7 +551,7 @@ void build_wrows(pf_op* op, const pf_op_desc& d, const cpx<double>* d_uv, const 
   g.dv_l = 60. * 60. * 180. / d.celly / std::floor(op->os * op->ny) / PI;
   g.wmean = mean;
   double* d_w = tmp.alloc<double>(M);
-  if (M) PF_HIP(hipMemcpy(d_w, d.w, sizeof(double) * M, hipMemcpyHostToDevice));
+  if (M) h2d(d_w, d.w, sizeof(double) * M);
   op->d_wrows = op->dalloc(op->esz() * (size_t)std::max<int64_t>(op->nnz_w, 1));
   if (op->prec == PF_C64) wrows_build<float>(g, d_uv, d_w, d_wt, d_perm, M, static_cast<cpx<float>*>(op->d_wrows));
   else wrows_build<double>(g, d_uv, d_w, d_wt, d_perm, M, static_cast<cpx<double>*>(op->d_wrows));
@@ -629,8 +629,8 @@ void build(pf_op* op, const pf_op_desc& d) {
     const std::vector<float> tv = make_tap_table(d.kernel, d.Ju, op->Jv, op->os);
     op->d_tab_u = (float*)op->dalloc(sizeof(float) * tu.size());
     op->d_tab_v = (float*)op->dalloc(sizeof(float) * tv.size());
-    PF_HIP(hipMemcpy(op->d_tab_u, tu.data(), sizeof(float) * tu.size(), hipMemcpyHostToDevice));
-    PF_HIP(hipMemcpy(op->d_tab_v, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice));
+    h2d(op->d_tab_u, tu.data(), sizeof(float) * tu.size());
+    h2d(op->d_tab_v, tv.data(), sizeof(float) * tv.size());
     op->gf.tab_u = op->d_tab_u;
     op->gf.tab_v = op->d_tab_v;
     op->kernel_table = 1;
@@ -639,7 +639,7 @@ void build(pf_op* op, const pf_op_desc& d) {
     const std::vector<float> pc = make_tap_poly(d.kernel, op->Jv, op->Ju, op->os);
     if (!pc.empty()) {
       op->d_poly = (float*)op->dalloc(sizeof(float) * pc.size());
-      PF_HIP(hipMemcpy(op->d_poly, pc.data(), sizeof(float) * pc.size(), hipMemcpyHostToDevice));
+      h2d(op->d_poly, pc.data(), sizeof(float) * pc.size());
       op->gf.poly = op->d_poly;
       op->kernel_poly = 1;
     }
@@ -679,12 +679,12 @@ void build(pf_op* op, const pf_op_desc& d) {
   for (int64_t m = 0; m < M; ++m) uvh[m] = mk<double>(u[m], v[m]);
   DevTemp tmp;
   cpx<double>* d_uvc = tmp.alloc<cpx<double>>(M);
-  if (M) PF_HIP(hipMemcpy(d_uvc, uvh.data(), sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
+  if (M) h2d(d_uvc, uvh.data(), sizeof(cpx<double>) * M);
   std::vector<cpx<double>>().swap(uvh);
   cpx<double>* d_wtc = nullptr;
   if (!op->unit_weights) {
     d_wtc = tmp.alloc<cpx<double>>(M);
-    PF_HIP(hipMemcpy(d_wtc, d.weights, sizeof(cpx<double>) * M, hipMemcpyHostToDevice));
+    h2d(d_wtc, d.weights, sizeof(cpx<double>) * M);
   }
   // perm (sorted position -> caller's index): kept as the one-pass plan's for complex128
   int32_t* d_perm = op->prec == PF_C128 ? (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(M, 1))
@@ -715,7 +715,7 @@ void build(pf_op* op, const pf_op_desc& d) {
     }
     std::stable_sort(so.begin(), so.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
     uint4* dso = (uint4*)op->dalloc(sizeof(uint4) * std::max<size_t>(so.size(), 1));
-    if (!so.empty()) PF_HIP(hipMemcpy(dso, so.data(), sizeof(uint4) * so.size(), hipMemcpyHostToDevice));
+    if (!so.empty()) h2d(dso, so.data(), sizeof(uint4) * so.size());
     op->tl.slot_own = dso;
     op->tl.n_own = (int)so.size();
   }
@@ -743,7 +743,7 @@ void build(pf_op* op, const pf_op_desc& d) {
       op->tlm.ent = d_ent;
     }
   }
-  PF_HIP(hipDeviceSynchronize());
+  PF_HIP(hipStreamSynchronize(cstream()));
 
   build_tail(op, d);
 }
@@ -760,13 +760,13 @@ void build_tail(pf_op* op, const pf_op_desc& d) {
     std::vector<float> fx(sx.begin(), sx.end()), fy(sy.begin(), sy.end());
     op->d_sx = op->dalloc(sizeof(float) * op->nx);
     op->d_sy = op->dalloc(sizeof(float) * op->ny);
-    PF_HIP(hipMemcpy(op->d_sx, fx.data(), sizeof(float) * op->nx, hipMemcpyHostToDevice));
-    PF_HIP(hipMemcpy(op->d_sy, fy.data(), sizeof(float) * op->ny, hipMemcpyHostToDevice));
+    h2d(op->d_sx, fx.data(), sizeof(float) * op->nx);
+    h2d(op->d_sy, fy.data(), sizeof(float) * op->ny);
   } else {
     op->d_sx = op->dalloc(sizeof(double) * op->nx);
     op->d_sy = op->dalloc(sizeof(double) * op->ny);
-    PF_HIP(hipMemcpy(op->d_sx, sx.data(), sizeof(double) * op->nx, hipMemcpyHostToDevice));
-    PF_HIP(hipMemcpy(op->d_sy, sy.data(), sizeof(double) * op->ny, hipMemcpyHostToDevice));
+    h2d(op->d_sx, sx.data(), sizeof(double) * op->nx);
+    h2d(op->d_sy, sy.data(), sizeof(double) * op->ny);
   }
 
   // ---- workspaces ----
@@ -840,21 +840,21 @@ void build_csr(pf_op* op, const pf_op_desc& d, const pf_csr& G) {
     if (op->prec == PF_C64) {
       std::vector<float> f((size_t)nnz * 2);
       for (size_t q = 0; q < f.size(); ++q) f[q] = (float)v[q];
-      PF_HIP(hipMemcpy(dp, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+      h2d(dp, f.data(), sizeof(float) * f.size());
     } else {
-      PF_HIP(hipMemcpy(dp, v, sizeof(double) * 2 * nnz, hipMemcpyHostToDevice));
+      h2d(dp, v, sizeof(double) * 2 * nnz);
     }
     return dp;
   };
   op->d_csr_ptr = (int64_t*)op->dalloc(sizeof(int64_t) * (G.rows + 1));
-  PF_HIP(hipMemcpy(op->d_csr_ptr, G.rowptr, sizeof(int64_t) * (G.rows + 1), hipMemcpyHostToDevice));
+  h2d(op->d_csr_ptr, G.rowptr, sizeof(int64_t) * (G.rows + 1));
   op->d_csr_col = (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
-  if (nnz) PF_HIP(hipMemcpy(op->d_csr_col, G.colidx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  if (nnz) h2d(op->d_csr_col, G.colidx, sizeof(int32_t) * nnz);
   op->d_csr_val = upload_vals(G.vals);
   op->d_csrT_ptr = (int64_t*)op->dalloc(sizeof(int64_t) * (Ng + 1));
-  PF_HIP(hipMemcpy(op->d_csrT_ptr, tp.data(), sizeof(int64_t) * (Ng + 1), hipMemcpyHostToDevice));
+  h2d(op->d_csrT_ptr, tp.data(), sizeof(int64_t) * (Ng + 1));
   op->d_csrT_col = (int32_t*)op->dalloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
-  if (nnz) PF_HIP(hipMemcpy(op->d_csrT_col, trow.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  if (nnz) h2d(op->d_csrT_col, trow.data(), sizeof(int32_t) * nnz);
   op->d_csrT_val = upload_vals(tval.data());
   build_tail(op, d);
 }
@@ -1085,7 +1085,7 @@ double read_scalar(pf_op* op, hipStream_t s) {
 
 template <typename R>
 double power_method_t(pf_op* op, int iters, double tol, const double* start) {
-  hipStream_t s = nullptr;
+  hipStream_t s = cstream();
   ensure_stage(op);
   const int64_t n = op->nimg();
   cpx<R>* x = static_cast<cpx<R>*>(op->d_vecx);
@@ -1417,14 +1417,14 @@ int pf_super_destroy(pf_super* sp) {
 int pf_op_forward(pf_op* op, const void* x, void* y, int x_loc, int y_loc, pf_stream stream) {
   return guarded([&] {
     ProfInstall pi(op);
-    apply(op, false, x, y, x_loc, y_loc, (hipStream_t)stream);
+    apply(op, false, x, y, x_loc, y_loc, io_stream(stream, x_loc == PF_HOST && y_loc == PF_HOST));
   });
 }
 
 int pf_op_adjoint(pf_op* op, const void* y, void* x, int y_loc, int x_loc, pf_stream stream) {
   return guarded([&] {
     ProfInstall pi(op);
-    apply(op, true, y, x, y_loc, x_loc, (hipStream_t)stream);
+    apply(op, true, y, x, y_loc, x_loc, io_stream(stream, x_loc == PF_HOST && y_loc == PF_HOST));
   });
 }
 
@@ -1433,7 +1433,7 @@ int pf_op_apply_normal(pf_op* op, const void* x, void* out, int x_loc, int out_l
     if (!op) throw Error(PF_ERR_ARG, "null operator");
     if ((out_loc != PF_HOST && out_loc != PF_DEVICE) || !x || !out) throw Error(PF_ERR_ARG, "bad buffer arguments");
     ProfInstall pi(op);
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = io_stream(stream, x_loc == PF_HOST && out_loc == PF_HOST);
     const int64_t n = op->nimg();
     if (x_loc == PF_HOST || out_loc == PF_HOST) ensure_stage(op);
     const void* din = stage_in(op, x, x_loc, n, op->d_vecx, s);
@@ -1463,15 +1463,15 @@ struct pf_psf {
   double norm = 1.0;
   std::vector<void*> allocs;
   void* dalloc(size_t bytes) {
-    void* p = nullptr;
-    PF_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    void* p = dev_alloc(std::max<size_t>(bytes, 16));
     allocs.push_back(p);
     return p;
   }
   size_t esz() const { return prec == PF_C64 ? 8 : 16; }
   int64_t n() const { return (int64_t)ny * nx; }
   ~pf_psf() {
-    for (void* p : allocs) (void)hipFree(p);
+    cstream_sync();
+    for (void* p : allocs) dev_free(p);
     fft_plan_free(fft);
   }
 };
@@ -1495,7 +1495,7 @@ void psf_apply_dev(pf_psf* h, const cpx<R>* x, cpx<R>* out, double sc, hipStream
 template <typename R>
 void psf_build(pf_psf* h, int iters, double tol, const double* start) {
   pf_op* op = h->op;
-  hipStream_t s = nullptr;
+  hipStream_t s = cstream();
   const int64_t n = h->n();
   // psf = Phi^dagger Phi delta with the measurement operator (its norm included)
   std::vector<cpx<R>> delta((size_t)n, mk<R>(R(0), R(0)));
@@ -1588,7 +1588,7 @@ int pf_psf_apply(pf_psf* h, const void* x, void* out, int x_loc, int out_loc, pf
   return guarded([&] {
     if (!h) throw Error(PF_ERR_ARG, "null handle");
     if ((out_loc != PF_HOST && out_loc != PF_DEVICE) || !x || !out) throw Error(PF_ERR_ARG, "bad buffer arguments");
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = io_stream(stream, x_loc == PF_HOST && out_loc == PF_HOST);
     const int64_t n = h->n();
     pf_op* op = h->op;
     if (x_loc == PF_HOST || out_loc == PF_HOST) ensure_stage(op);
@@ -1716,7 +1716,7 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     // projection rows and a given matrix have one gridding kernel each: only the column FFT is chosen
     const bool one_grid_kernel = op->w_term || op->g_csr;
     ensure_stage(op);
-    hipStream_t s = nullptr;
+    hipStream_t s = cstream();
     PF_HIP(hipMemsetAsync(op->d_vecy, 0, op->esz() * (size_t)std::max<int64_t>(op->M, 1), s));
     hipEvent_t a, b;
     PF_HIP(hipEventCreate(&a));
@@ -1921,7 +1921,7 @@ int pf_fft2d(int32_t Nv, int32_t Nu, int32_t precision, int32_t sign, const void
     void* ws = workspace_dev;
     bool own = false;
     if (!ws) {
-      PF_HIP(hipMalloc(&ws, esz * (size_t)Nv * Nu));
+      ws = dev_alloc(esz * (size_t)Nv * Nu);
       own = true;
     }
     const bool inv = sign > 0;
@@ -1936,7 +1936,7 @@ int pf_fft2d(int32_t Nv, int32_t Nu, int32_t precision, int32_t sign, const void
     if (precision) run(double());
     else run(float());
     PF_HIP(hipStreamSynchronize(s));
-    if (own) (void)hipFree(ws);
+    if (own) dev_free(ws);
     fft_plan_free(p);
   });
 }
@@ -1950,21 +1950,20 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
       for (int64_t i = 0; i < n; ++i) out_host[i] = host_ft_kernel(kind, x_host[i] / ftsize - 0.5, J, os);
       return;
     }
-    double *dx = nullptr, *dy = nullptr;
-    PF_HIP(hipMalloc(&dx, sizeof(double) * std::max<int64_t>(n, 1)));
-    PF_HIP(hipMalloc(&dy, sizeof(double) * std::max<int64_t>(n, 1)));
-    PF_HIP(hipMemcpy(dx, x_host, sizeof(double) * n, hipMemcpyHostToDevice));
+    DevTemp tmp;
+    double* dx = tmp.alloc<double>(n);
+    double* dy = tmp.alloc<double>(n);
+    hipStream_t s = cstream();
+    h2d(dx, x_host, sizeof(double) * n);
     const unsigned blocks = (unsigned)((n + 255) / 256);
     if (blocks) {
       if (precision == PF_C128)
-        hipLaunchKernelGGL((k_kernel_eval<double>), dim3(blocks), dim3(256), 0, 0, make_kparams<double>(kind, J, os), dx, dy, n);
+        hipLaunchKernelGGL((k_kernel_eval<double>), dim3(blocks), dim3(256), 0, s, make_kparams<double>(kind, J, os), dx, dy, n);
       else
-        hipLaunchKernelGGL((k_kernel_eval<float>), dim3(blocks), dim3(256), 0, 0, make_kparams<float>(kind, J, os), dx, dy, n);
+        hipLaunchKernelGGL((k_kernel_eval<float>), dim3(blocks), dim3(256), 0, s, make_kparams<float>(kind, J, os), dx, dy, n);
       PF_HIP(hipGetLastError());
     }
-    PF_HIP(hipMemcpy(out_host, dy, sizeof(double) * n, hipMemcpyDeviceToHost));
-    (void)hipFree(dx);
-    (void)hipFree(dy);
+    d2h(out_host, dy, sizeof(double) * n);
   });
 }
 

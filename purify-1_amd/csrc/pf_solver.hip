@@ -344,8 +344,7 @@ struct pf_padmm {
   double captured_gamma = 0;  // and the l1 weight its threshold kernel was given (reset recomputes it)
   std::vector<void*> allocs;
   void* dalloc(size_t bytes) {
-    void* p = nullptr;
-    PF_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    void* p = dev_alloc(std::max<size_t>(bytes, 16));
     allocs.push_back(p);
     return p;
   }
@@ -353,7 +352,7 @@ struct pf_padmm {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamSynchronize(stream);
-    for (void* p : allocs) (void)hipFree(p);
+    for (void* p : allocs) dev_free(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -706,12 +705,11 @@ int pf_padmm_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const
     // the visibilities: host complex128 or device in the operator's precision
     if (y_loc == PF_HOST) {
       if (h->prec == PF_C64) {
-        void* tmp = nullptr;
-        PF_HIP(hipMalloc(&tmp, 16 * std::max<int64_t>(h->M, 1)));
+        void* tmp = dev_alloc(16 * std::max<int64_t>(h->M, 1));
         PF_HIP(hipMemcpyAsync(tmp, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
         c128_to_c64((const cpx<double>*)tmp, (cpx<float>*)h->y, h->M, h->stream);
         PF_HIP(hipStreamSynchronize(h->stream));
-        PF_HIP(hipFree(tmp));
+        dev_free(tmp);
       } else {
         PF_HIP(hipMemcpyAsync(h->y, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
       }
@@ -805,23 +803,21 @@ int pf_fb_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const pf
     h->counter = (int*)h->dalloc(sizeof(int));
     PF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // the target Phi^dagger y
-    void* yd = nullptr;
-    PF_HIP(hipMalloc(&yd, 16 * std::max<int64_t>(h->M, 1)));
+    void* yd = dev_alloc(16 * std::max<int64_t>(h->M, 1));
     if (y_loc == PF_HOST) {
       if (h->prec == PF_C64) {
-        void* tmp = nullptr;
-        PF_HIP(hipMalloc(&tmp, 16 * std::max<int64_t>(h->M, 1)));
+        void* tmp = dev_alloc(16 * std::max<int64_t>(h->M, 1));
         PF_HIP(hipMemcpyAsync(tmp, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
         c128_to_c64((const cpx<double>*)tmp, (cpx<float>*)yd, h->M, h->stream);
         PF_HIP(hipStreamSynchronize(h->stream));
-        PF_HIP(hipFree(tmp));
+        dev_free(tmp);
       } else {
         PF_HIP(hipMemcpyAsync(yd, y, 16 * h->M, hipMemcpyHostToDevice, h->stream));
       }
     } else if (y_loc == PF_DEVICE) {
       PF_HIP(hipMemcpyAsync(yd, y, e * h->M, hipMemcpyDeviceToDevice, h->stream));
     } else {
-      PF_HIP(hipFree(yd));
+      dev_free(yd);
       throw Error(PF_ERR_ARG, "location must be PF_HOST or PF_DEVICE");
     }
     if (h->prec == PF_C64) {
@@ -832,7 +828,7 @@ int pf_fb_create(pf_op* phi, pf_wavelet* psi, const void* y, int y_loc, const pf
       op_normal<double>(phi, V<double>(h->dimage), V<double>(h->g), h->stream);
     }
     PF_HIP(hipStreamSynchronize(h->stream));
-    PF_HIP(hipFree(yd));
+    dev_free(yd);
     if (h->prec == PF_C64) fb_init<float>(h.get());
     else fb_init<double>(h.get());
   });
@@ -899,12 +895,11 @@ int pf_padmm_get(pf_padmm* h, int32_t what, void* out, int out_loc) {
       PF_HIP(hipStreamSynchronize(h->stream));
     } else if (out_loc == PF_HOST) {
       if (h->prec == PF_C64) {
-        void* tmp = nullptr;
-        PF_HIP(hipMalloc(&tmp, 16 * std::max<int64_t>(n, 1)));
+        void* tmp = dev_alloc(16 * std::max<int64_t>(n, 1));
         c64_to_c128((const cpx<float>*)src, (cpx<double>*)tmp, n, h->stream);
         PF_HIP(hipMemcpyAsync(out, tmp, 16 * n, hipMemcpyDeviceToHost, h->stream));
         PF_HIP(hipStreamSynchronize(h->stream));
-        PF_HIP(hipFree(tmp));
+        dev_free(tmp);
       } else {
         PF_HIP(hipMemcpyAsync(out, src, 16 * n, hipMemcpyDeviceToHost, h->stream));
         PF_HIP(hipStreamSynchronize(h->stream));

@@ -394,15 +394,15 @@ struct pf_wavelet {
   void* d_out = nullptr;
   std::vector<void*> allocs;
   void* dalloc(size_t bytes) {
-    void* p = nullptr;
-    PF_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    void* p = dev_alloc(std::max<size_t>(bytes, 16));
     allocs.push_back(p);
     return p;
   }
   size_t esz() const { return prec == PF_C64 ? 8 : 16; }
   int64_t n() const { return (int64_t)ny * nx; }
   ~pf_wavelet() {
-    for (void* p : allocs) (void)hipFree(p);
+    cstream_sync();
+    for (void* p : allocs) dev_free(p);
   }
 };
 
@@ -493,7 +493,7 @@ int pf_wavelet_create(int32_t imsizey, int32_t imsizex, int32_t nwav, const int3
 int pf_wavelet_forward(pf_wavelet* wv, const void* coeffs, void* image, int c_loc, int x_loc, pf_stream stream) {
   return guarded([&] {
     if (!wv || !coeffs || !image) throw Error(PF_ERR_ARG, "null argument");
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = io_stream(stream, c_loc == PF_HOST && x_loc == PF_HOST);
     const int64_t N = wv->n(), NC = N * wv->nwav;
     const void* din = coeffs;
     if (c_loc == PF_HOST || x_loc == PF_HOST) {
@@ -531,7 +531,7 @@ int pf_wavelet_forward(pf_wavelet* wv, const void* coeffs, void* image, int c_lo
 int pf_wavelet_adjoint(pf_wavelet* wv, const void* image, void* coeffs, int x_loc, int c_loc, pf_stream stream) {
   return guarded([&] {
     if (!wv || !coeffs || !image) throw Error(PF_ERR_ARG, "null argument");
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = io_stream(stream, c_loc == PF_HOST && x_loc == PF_HOST);
     const int64_t N = wv->n(), NC = N * wv->nwav;
     const void* din = image;
     if (c_loc == PF_HOST || x_loc == PF_HOST) {

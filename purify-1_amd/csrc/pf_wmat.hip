@@ -168,13 +168,6 @@ __global__ void __launch_bounds__(256) k_csr_gather(int64_t rows, const int64_t*
   if (lane == 0) y[r] = scale(acc, scl);
 }
 
-template <typename T>
-T* dev_alloc(size_t n) {
-  void* p = nullptr;
-  PF_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)));
-  return static_cast<T*>(p);
-}
-
 unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -202,25 +195,25 @@ void wprojection_matrix_dev(int64_t rows, const int64_t* rowptr, const int32_t* 
   const int64_t B = std::max<int64_t>(1, std::min<int64_t>(rows, (int64_t(1) << 22) / N));
   Fft2dPlan plan;
   fft_plan_init(plan, y_size, x_size, 1);
-  cpx<double>* d_ch = dev_alloc<cpx<double>>(B * N);
-  cpx<double>* d_ws = dev_alloc<cpx<double>>(B * N);
-  cpx<double>* d_out = dev_alloc<cpx<double>>(B * N);
-  double* d_abs = dev_alloc<double>(B * N);
-  double* d_w = dev_alloc<double>(B);
-  double *d_thr = dev_alloc<double>(B), *d_mx = dev_alloc<double>(B);
-  int *d_cnt = dev_alloc<int>(B), *d_hit = dev_alloc<int>(B);
+  cpx<double>* d_ch = dev_alloc_n<cpx<double>>(B * N);
+  cpx<double>* d_ws = dev_alloc_n<cpx<double>>(B * N);
+  cpx<double>* d_out = dev_alloc_n<cpx<double>>(B * N);
+  double* d_abs = dev_alloc_n<double>(B * N);
+  double* d_w = dev_alloc_n<double>(B);
+  double *d_thr = dev_alloc_n<double>(B), *d_mx = dev_alloc_n<double>(B);
+  int *d_cnt = dev_alloc_n<int>(B), *d_hit = dev_alloc_n<int>(B);
   const int64_t maxg = [&] {
     int64_t m = 0;
     for (int64_t r = 0; r < rows; ++r) m = std::max(m, rowptr[r + 1] - rowptr[r]);
     return m;
   }();
-  int64_t* d_gptr = dev_alloc<int64_t>(B + 1);
-  int32_t* d_gcol = dev_alloc<int32_t>(B * maxg);
-  cpx<double>* d_gval = dev_alloc<cpx<double>>(B * maxg);
+  int64_t* d_gptr = dev_alloc_n<int64_t>(B + 1);
+  int32_t* d_gcol = dev_alloc_n<int32_t>(B * maxg);
+  cpx<double>* d_gval = dev_alloc_n<cpx<double>>(B * maxg);
   std::vector<double> h_out((size_t)(B * N) * 2), h_abs((size_t)(B * N)), h_thr(B);
   std::vector<int> h_cnt(B), h_hit(B);
   std::vector<int64_t> gp(B + 1);
-  hipStream_t s = nullptr;
+  hipStream_t s = cstream();
   try {
     for (int64_t r0 = 0; r0 < rows; r0 += B) {
       const int nb = (int)std::min(B, rows - r0);
@@ -228,12 +221,12 @@ void wprojection_matrix_dev(int64_t rows, const int64_t* rowptr, const int32_t* 
       // the batch's rows of G and their w
       for (int b = 0; b <= nb; ++b) gp[b] = rowptr[r0 + b] - rowptr[r0];
       const int64_t ge = gp[nb];
-      PF_HIP(hipMemcpy(d_gptr, gp.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice));
+      h2d(d_gptr, gp.data(), sizeof(int64_t) * (nb + 1));
       if (ge) {
-        PF_HIP(hipMemcpy(d_gcol, col + rowptr[r0], sizeof(int32_t) * ge, hipMemcpyHostToDevice));
-        PF_HIP(hipMemcpy(d_gval, val + 2 * rowptr[r0], sizeof(cpx<double>) * ge, hipMemcpyHostToDevice));
+        h2d(d_gcol, col + rowptr[r0], sizeof(int32_t) * ge);
+        h2d(d_gval, val + 2 * rowptr[r0], sizeof(cpx<double>) * ge);
       }
-      PF_HIP(hipMemcpy(d_w, w + r0, sizeof(double) * nb, hipMemcpyHostToDevice));
+      h2d(d_w, w + r0, sizeof(double) * nb);
       PF_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int) * nb, s));
       PF_HIP(hipMemsetAsync(d_hit, 0, sizeof(int) * nb, s));
       // 1-2: chirp images and their unnormalised DFT (row-major y_size x x_size, as the FFT operator sees them)
@@ -253,11 +246,11 @@ void wprojection_matrix_dev(int64_t rows, const int64_t* rowptr, const int32_t* 
       hipLaunchKernelGGL(k_wm_abs, dim3(nblocks(NB, 256)), dim3(256), 0, s, NB, d_ch, d_abs);
       hipLaunchKernelGGL(k_wm_thres, dim3(nb), dim3(WM_THREADS), 0, s, d_abs, N, e_wproj, d_thr, d_mx);
       PF_HIP(hipGetLastError());
-      PF_HIP(hipMemcpy(h_out.data(), d_ch, sizeof(cpx<double>) * NB, hipMemcpyDeviceToHost));
-      PF_HIP(hipMemcpy(h_abs.data(), d_abs, sizeof(double) * NB, hipMemcpyDeviceToHost));
-      PF_HIP(hipMemcpy(h_thr.data(), d_thr, sizeof(double) * nb, hipMemcpyDeviceToHost));
-      PF_HIP(hipMemcpy(h_cnt.data(), d_cnt, sizeof(int) * nb, hipMemcpyDeviceToHost));
-      PF_HIP(hipMemcpy(h_hit.data(), d_hit, sizeof(int) * nb, hipMemcpyDeviceToHost));
+      d2h(h_out.data(), d_ch, sizeof(cpx<double>) * NB);
+      d2h(h_abs.data(), d_abs, sizeof(double) * NB);
+      d2h(h_thr.data(), d_thr, sizeof(double) * nb);
+      d2h(h_cnt.data(), d_cnt, sizeof(int) * nb);
+      d2h(h_hit.data(), d_hit, sizeof(int) * nb);
       for (int b = 0; b < nb; ++b) {
         // wproj_utilities.h:246-247
         if ((int64_t)h_hit[b] < std::min<int64_t>(gp[b + 1] - gp[b], h_cnt[b]))
@@ -275,15 +268,16 @@ void wprojection_matrix_dev(int64_t rows, const int64_t* rowptr, const int32_t* 
       }
     }
   } catch (...) {
+    cstream_sync();
     for (void* p : {(void*)d_ch, (void*)d_ws, (void*)d_out, (void*)d_abs, (void*)d_w, (void*)d_thr, (void*)d_mx,
                     (void*)d_cnt, (void*)d_hit, (void*)d_gptr, (void*)d_gcol, (void*)d_gval})
-      (void)hipFree(p);
+      dev_free(p);
     fft_plan_free(plan);
     throw;
   }
   for (void* p : {(void*)d_ch, (void*)d_ws, (void*)d_out, (void*)d_abs, (void*)d_w, (void*)d_thr, (void*)d_mx,
                   (void*)d_cnt, (void*)d_hit, (void*)d_gptr, (void*)d_gcol, (void*)d_gval})
-    (void)hipFree(p);
+    dev_free(p);
   fft_plan_free(plan);
 }
 

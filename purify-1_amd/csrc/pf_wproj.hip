@@ -103,7 +103,7 @@ void wrows_build(const WRowArgs& g, const cpx<double>* uv, const double* w, cons
                  int64_t M, cpx<R>* out) {
   if (M == 0) return;
   const int64_t n = M * g.Kv;
-  hipLaunchKernelGGL((k_wrows_build<R>), dim3((unsigned)((n + 127) / 128)), dim3(128), 0, nullptr, g, uv, w, wt, perm, M,
+  hipLaunchKernelGGL((k_wrows_build<R>), dim3((unsigned)((n + 127) / 128)), dim3(128), 0, cstream(), g, uv, w, wt, perm, M,
                      out);
   PF_HIP(hipGetLastError());
 }

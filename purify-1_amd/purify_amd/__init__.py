@@ -14,11 +14,28 @@ from . import algorithms, pfitsio, utilities, wavelets, wproj_utilities
 __all__ = ["kernel", "kernel_from_string", "LinearTransform", "MeasurementOperator", "measurementoperator",
            "operators", "GridDegridOperator", "PsfOperator",
            "measurement_operator_factory", "distributed_measurement_operator", "vis_params", "vis_units",
-           "utilities", "pfitsio", "fft2d", "kernel_eval", "version"]
+           "utilities", "pfitsio", "fft2d", "kernel_eval", "version", "cache_stats", "empty_cache"]
 
 
 def version():
     return _lib.lib().pf_version().decode()
+
+
+def cache_stats():
+    """(bytes held by live handles, bytes cached for reuse) in the library's device block cache."""
+    import ctypes as C
+    live, cached = C.c_int64(), C.c_int64()
+    _lib.check(_lib.lib().pf_cache_stats(C.byref(live), C.byref(cached)))
+    return live.value, cached.value
+
+
+def empty_cache():
+    """Release the library's cached device blocks (like torch.cuda.empty_cache for torch's); returns the bytes
+    released."""
+    import ctypes as C
+    n = C.c_int64()
+    _lib.check(_lib.lib().pf_cache_release(C.byref(n)))
+    return n.value
 
 
 def fft2d(x, Nv, Nu, sign=-1):

@@ -137,6 +137,8 @@ SIGNATURES = {
                                         C.c_double, C.c_double, C.POINTER(pf_csr)]),
     "pf_csr_free": (None, [C.POINTER(pf_csr)]),
     "pf_op_create_csr": (C.c_int, [C.POINTER(pf_op_desc), C.POINTER(pf_csr), C.POINTER(C.c_void_p)]),
+    "pf_cache_stats": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "pf_cache_release": (C.c_int, [C.POINTER(C.c_int64)]),
     "pf_last_error": (C.c_char_p, []),
     "pf_version": (C.c_char_p, []),
 }
@@ -167,3 +169,26 @@ def check(rc):
         msg = lib().pf_last_error().decode(errors="replace")
         raise PurifyError(rc, msg)
     return rc
+
+
+class StreamLog:
+    """The streams a handle's device calls were queued on.  Destroying a handle returns its device memory to
+    the library's block cache without waiting for the device, so the handle's close() first synchronises
+    every stream logged here (each one only: other streams' work goes on)."""
+
+    def __init__(self):
+        self._s = set()
+
+    def add(self, stream, device):
+        self._s.add((int(stream or 0), 0 if device.index is None else device.index))
+
+    def sync(self):
+        if not self._s:
+            return
+        import torch
+        for s, d in self._s:
+            if s == 0:
+                torch.cuda.default_stream(d).synchronize()
+            else:
+                torch.cuda.ExternalStream(s, device=torch.device("cuda", d)).synchronize()
+        self._s.clear()

@@ -71,6 +71,7 @@ class MeasurementOperator:
 
     def __init__(self, desc: _lib.pf_op_desc, keep=(), csr=None):
         self._keep = keep
+        self._streams = _lib.StreamLog()
         h = C.c_void_p()
         if csr is None:
             _lib.check(_lib.lib().pf_op_create(C.byref(desc), C.byref(h)))
@@ -87,6 +88,7 @@ class MeasurementOperator:
     def close(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
+            self._streams.sync()
             _lib.lib().pf_op_destroy(h)
             self._h = None
 
@@ -178,6 +180,7 @@ class MeasurementOperator:
             else:
                 check_device_tensor(out, nout, x.dtype, x.device, "output")
             s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+            self._streams.add(s, x.device)
             _lib.check(fn(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
                           _lib.PF_DEVICE, C.c_void_p(s)))
             return out
@@ -201,6 +204,7 @@ class MeasurementOperator:
             else:
                 check_device_tensor(out, nout, x.dtype, x.device, "output")
             s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+            self._streams.add(s, x.device)
             _lib.check(fn(handle, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
                           _lib.PF_DEVICE, C.c_void_p(s)))
             return out
@@ -236,6 +240,7 @@ class MeasurementOperator:
         x = check_device_tensor(x.contiguous(), nin, self._dtype(), None, "input")
         out = torch.empty(nout, dtype=x.dtype, device=x.device)
         s = torch.cuda.current_stream(x.device).cuda_stream
+        self._streams.add(s, x.device)
         _lib.check(getattr(_lib.lib(), name)(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()),
                                              C.c_void_p(s)))
         return out
@@ -411,9 +416,11 @@ class SuperOperator:
         self.ops = ops
         self.imsizey, self.imsizex = ops[0].imsizey, ops[0].imsizex
         self.sizes = (self.M, self.imsizex * self.imsizey)
+        self._streams = _lib.StreamLog()
 
     def close(self):
         if self._h:
+            self._streams.sync()
             _lib.lib().pf_super_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -440,6 +447,7 @@ class SuperOperator:
             raise ValueError(f"input has {xd.numel()} entries, expected {nin}")
         out = torch.empty(nout, dtype=dt, device=xd.device)
         s = torch.cuda.current_stream(xd.device).cuda_stream
+        self._streams.add(s, xd.device)
         fn = _lib.lib().pf_super_adjoint if adjoint else _lib.lib().pf_super_forward
         _lib.check(fn(self._h, C.c_void_p(xd.data_ptr()), C.c_void_p(out.data_ptr()), C.c_void_p(s)))
         if host:
@@ -555,6 +563,7 @@ class PsfOperator:
 
     def close(self):
         if self._h:
+            self.op._streams.sync()  # the PSF's applies log their streams with the measurement operator
             _lib.lib().pf_psf_destroy(self._h)
             self._h = C.c_void_p()
 

@@ -48,6 +48,7 @@ class WaveletOperator:
         db = np.array([e[0] for e in self.entries], np.int32)
         lv = np.array([e[1] for e in self.entries], np.int32)
         self._h = C.c_void_p()
+        self._streams = _lib.StreamLog()
         _lib.check(_lib.lib().pf_wavelet_create(self.imsizey, self.imsizex, len(self.entries), db.ctypes.data,
                                                 lv.ctypes.data, self.precision, C.byref(self._h)))
 
@@ -61,6 +62,7 @@ class WaveletOperator:
 
     def close(self):
         if self._h:
+            self._streams.sync()
             _lib.lib().pf_wavelet_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -87,6 +89,7 @@ class WaveletOperator:
             else:
                 check_device_tensor(out, nout, x.dtype, x.device, "output")
             s = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+            self._streams.add(s, x.device)
             _lib.check(fn(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), _lib.PF_DEVICE,
                           _lib.PF_DEVICE, C.c_void_p(s)))
             return out
