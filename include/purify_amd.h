@@ -28,7 +28,9 @@ extern "C" {
 #endif
 
 typedef struct pf_op pf_op;
-typedef void *pf_stream; /* a hipStream_t; NULL = the null stream */
+typedef void *pf_stream; /* a hipStream_t; NULL = the null stream, except for an apply whose input and output
+                            are both on the host (a synchronous call): that runs on the calling thread's
+                            own non-blocking stream, as do constructions and power methods */
 
 enum pf_status {
   PF_OK = 0,
@@ -302,14 +304,16 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * lines with the next line prefetched; 0 = one line per block; default 3 for complex64, 2 for
  * complex128); "grid_waves" (matrix-core gridding with that many waves, each taking every
  * grid_waves-th 16 x 16 tile with its batches pipelined across tiles; 0 = one wave per tile, the
- * default); read-only "tile_u", "tile_v".  Results do not depend on grid_kernel or fft_columns beyond
- * floating-point summation order. */
+ * default); read-only "tile_u", "tile_v", "w_footprint" (side of the stored rows' footprint with
+ * w-projection rows, else 0) and "split_tiles" (tiles whose stored-row gridding list is cut into parts
+ * that are summed in a fixed order afterwards, so that a dense tile does not form the launch's tail).
+ * Results do not depend on grid_kernel or fft_columns beyond floating-point summation order. */
 /* pf_op_set_norm, pf_op_set_tuning and pf_op_autotune change what an apply launches: a solver that
  * captured the operator's launches into a graph captures them again on its next run. */
 int pf_op_set_tuning(pf_op *op, const char *key, int64_t value);
 int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
 /* Times the fixed-order gridding-kernel variants (residue-class lanes and matrix cores, not the LDS-atomic
- * one) on this operator (reps adjoint gridding passes each, HIP events on the null stream) and keeps the
+ * one) on this operator (reps adjoint gridding passes each, HIP events on the calling thread's stream) and keeps the
  * fastest, then does the same for the two "fft_columns" modes (reps
  * forward + adjoint column passes each).  Writes the chosen gridding variant to *chosen if non-NULL;
  * the column mode is read back with pf_op_get_tuning.  Operators with w-projection rows or a given

@@ -53,9 +53,16 @@ __global__ void k_keys(const cpx<double>* __restrict__ uv, int64_t M, int Nu, in
   idx[m] = (int32_t)m;
 }
 
-__global__ void k_tile_count(const uint32_t* __restrict__ key, int64_t M, uint32_t TT, uint32_t* __restrict__ cnt) {
+// Run bounds of a sorted key array: for the tile t = key / TT of each run of equal tiles, beg[t] = the
+// run's first index and fin[t] = one past its last (both stay zero for a tile with no entries), so the
+// tile's count is fin[t] - beg[t].  One writer per value: no atomics, which a dense tile would serialise.
+__global__ void k_tile_runs(const uint32_t* __restrict__ key, int64_t M, uint32_t TT, uint32_t* __restrict__ beg,
+                            uint32_t* __restrict__ fin) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < M) atomicAdd(&cnt[key[i] / TT], 1u);
+  if (i >= M) return;
+  const uint32_t t = key[i] / TT;
+  if (i == 0 || key[i - 1] / TT != t) beg[t] = (uint32_t)i;
+  if (i + 1 == M || key[i + 1] / TT != t) fin[t] = (uint32_t)(i + 1);
 }
 
 // the sorted visibilities' tap origins k = floor(u - K/2) and (complex64) fractions, (complex128) pixels,
@@ -179,6 +186,24 @@ struct Scratch {
   }
 };
 
+// per-tile entry counts (host, nt tiles) of a sorted key array of n entries (tile = key / TT); synchronises s
+void tile_counts(Scratch& sc, const uint32_t* key_s, int64_t n, uint32_t TT, int nt, std::vector<uint32_t>& count,
+                 hipStream_t s) {
+  count.assign((size_t)nt, 0);
+  if (n == 0) return;
+  uint32_t* beg = sc.alloc<uint32_t>((size_t)nt);
+  uint32_t* fin = sc.alloc<uint32_t>((size_t)nt);
+  PF_HIP(hipMemsetAsync(beg, 0, sizeof(uint32_t) * nt, s));
+  PF_HIP(hipMemsetAsync(fin, 0, sizeof(uint32_t) * nt, s));
+  hipLaunchKernelGGL(k_tile_runs, dim3(nblk(n)), dim3(256), 0, s, key_s, n, TT, beg, fin);
+  PF_HIP(hipGetLastError());
+  std::vector<uint32_t> hb((size_t)nt);
+  PF_HIP(hipMemcpyAsync(hb.data(), beg, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, s));
+  PF_HIP(hipMemcpyAsync(count.data(), fin, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, s));
+  PF_HIP(hipStreamSynchronize(s));
+  for (int t = 0; t < nt; ++t) count[t] -= hb[t];
+}
+
 // stable sort of (key, value) pairs on the device; keys below 2^bits
 template <typename V>
 void sort_pairs(Scratch& sc, const uint32_t* kin, uint32_t* kout, const V* vin, V* vout, int64_t n, int bits,
@@ -205,18 +230,13 @@ void dev_sort_visibilities(const cpx<double>* d_uv, int64_t M, int Nu, int Nv, i
   uint32_t* key_s = sc.alloc<uint32_t>(M);
   int32_t* idx = sc.alloc<int32_t>(M);
   int* bad = sc.alloc<int>(1);
-  uint32_t* cnt = sc.alloc<uint32_t>(nt);
   PF_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
-  PF_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nt, s));
   hipLaunchKernelGGL(k_keys, dim3(nblk(M)), dim3(256), 0, s, d_uv, M, Nu, Nv, Ku, Kv, TU, TV, ntu, key, idx, bad);
   PF_HIP(hipGetLastError());
   sort_pairs<int32_t>(sc, key, key_s, idx, d_perm, M, bits_for((uint64_t)nt * TU * TV), s);
-  hipLaunchKernelGGL(k_tile_count, dim3(nblk(M)), dim3(256), 0, s, key_s, M, (uint32_t)(TU * TV), cnt);
-  PF_HIP(hipGetLastError());
   int hbad = 0;
   PF_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-  PF_HIP(hipMemcpyAsync(owner_count.data(), cnt, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, s));
-  PF_HIP(hipStreamSynchronize(s));
+  tile_counts(sc, key_s, M, (uint32_t)(TU * TV), nt, owner_count, s);
   if (hbad) throw Error(PF_ERR_ARG, "non-finite u/v coordinate");
 }
 
@@ -276,12 +296,7 @@ uint32_t* dev_tile_lists(const int2* kk, int64_t M, int Nu, int Nv, int Ku, int 
                        tval);
     PF_HIP(hipGetLastError());
     sort_pairs<uint32_t>(sc, tkey, tkey_s, tval, list, (int64_t)total, bits_for((uint64_t)nt), s);
-    uint32_t* cnt = sc.alloc<uint32_t>(nt);
-    PF_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nt, s));
-    hipLaunchKernelGGL(k_tile_count, dim3(nblk((int64_t)total)), dim3(256), 0, s, tkey_s, (int64_t)total, 1u, cnt);
-    PF_HIP(hipGetLastError());
-    PF_HIP(hipMemcpyAsync(count.data(), cnt, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, s));
-    PF_HIP(hipStreamSynchronize(s));
+    tile_counts(sc, tkey_s, (int64_t)total, 1u, nt, count, s);
   } catch (...) {
     cstream_sync();
     dev_free(list);

@@ -231,13 +231,23 @@ std::vector<float> make_tap_table(int kind, int Jeval, int J, double os);
 constexpr int TAP_POLY_D = 9;
 std::vector<float> make_tap_poly(int kind, int Jeval, int J, double os, double* max_rel_err = nullptr);
 
+constexpr uint32_t PART_NONE = 0xffffffffu;
 struct TileLists {
   int TU = 32, TV = 32, ntu = 0, ntv = 0;
   const uint32_t* list = nullptr;  // sorted vis indices per tile (every visibility with a tap in it)
-  // launch slots, heaviest tile first: (tile, begin, end, 0) of the tile's range in `list` (all tiles)
+  // launch slots, heaviest first: (tile, begin, end, part) of a range of the tile's entries in `list`.
+  // Without splitting, one slot per tile (all tiles) with part = PART_NONE.  Split lists (stored-row
+  // gridding) cut a tile with more than a chunk of entries into parts: each part's slot accumulates a
+  // partial tile into `part` (TU * TV values at part * TU * TV), and the combine pass sums a split tile's
+  // parts in part order into the grid; comb holds (tile, first part, parts, 0) per split tile.
   const uint4* slot = nullptr;
-  // launch slots of the degridding kernel: (tile, begin, end, 0) of the sorted visibilities whose first
-  // tap lies in the tile, for the n_own tiles that own any, heaviest first
+  int n_slot = 0;
+  const uint4* comb = nullptr;
+  int n_comb = 0;
+  void* part = nullptr;
+  // launch slots of the degridding kernel: (tile, begin, end, 0) of a chunk of the sorted visibilities
+  // whose first tap lies in the tile (chunks of at most max(512, M / (8 CUs)) visibilities, so that a
+  // dense tile does not form the launch's tail), for the tiles that own any, heaviest first
   const uint4* slot_own = nullptr;
   int n_own = 0;
   // complex64 matrix-core lists: each list entry as one record (8 * sorted index, the byte offset of

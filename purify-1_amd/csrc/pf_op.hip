@@ -440,28 +440,60 @@ struct DevTemp {
 
 // The gridding lists of TU x TV tiles (footprints Ku x Kv at the sorted tap origins kk) into tl: the list
 // built on the device, the launch slots (heaviest tile first) from its per-tile counts.  Returns its length.
-int64_t device_lists(pf_op* op, const int2* kk, int64_t M, int Ku, int Kv, int TU, int TV, TileLists& tl) {
+int device_cus() {
+  int dev = 0, ncu = 0;
+  PF_HIP(hipGetDevice(&dev));
+  PF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(ncu, 1);
+}
+
+// split: a tile with more than max(1024, len / (4 CUs)) entries is cut into equal parts of at most that
+// many, each a launch slot of its own (see TileLists), so that a dense tile does not form the launch's tail.
+int64_t device_lists(pf_op* op, const int2* kk, int64_t M, int Ku, int Kv, int TU, int TV, TileLists& tl,
+                     bool split = false) {
   std::vector<uint32_t> count;
   int64_t len = 0;
   uint32_t* list = dev_tile_lists(kk, M, op->Nu, op->Nv, Ku, Kv, TU, TV, count, len);
   op->allocs.push_back(list);
   op->device_bytes += (int64_t)sizeof(uint32_t) * std::max<int64_t>(len, 1);
   const int nt = (int)count.size();
-  std::vector<uint4> slot((size_t)nt);
-  uint32_t run = 0;
+  uint32_t ch = 0xffffffffu;
+  if (split) {
+    const int64_t per_slot = 4 * (int64_t)device_cus();
+    ch = (uint32_t)std::max<int64_t>(1024, (len + per_slot - 1) / per_slot);
+  }
+  std::vector<uint4> slot, comb;
+  slot.reserve((size_t)nt);
+  uint32_t run = 0, nparts = 0;
   for (int t = 0; t < nt; ++t) {
-    slot[t] = make_uint4((uint32_t)t, run, run + count[t], 0u);
-    run += count[t];
+    const uint32_t c = count[t];
+    if (c <= ch) {
+      slot.push_back(make_uint4((uint32_t)t, run, run + c, PART_NONE));
+    } else {
+      const uint32_t np = (c + ch - 1) / ch, per = (c + np - 1) / np;
+      comb.push_back(make_uint4((uint32_t)t, nparts, np, 0u));
+      for (uint32_t p = 0; p < np; ++p)
+        slot.push_back(make_uint4((uint32_t)t, run + p * per, std::min(run + (p + 1) * per, run + c), nparts++));
+    }
+    run += c;
   }
   std::stable_sort(slot.begin(), slot.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
-  uint4* d_slot = (uint4*)op->dalloc(sizeof(uint4) * (size_t)std::max(nt, 1));
-  h2d(d_slot, slot.data(), sizeof(uint4) * nt);
+  uint4* d_slot = (uint4*)op->dalloc(sizeof(uint4) * std::max<size_t>(slot.size(), 1));
+  h2d(d_slot, slot.data(), sizeof(uint4) * slot.size());
   tl.TU = TU;
   tl.TV = TV;
   tl.ntu = op->Nu / TU;
   tl.ntv = op->Nv / TV;
   tl.list = list;
   tl.slot = d_slot;
+  tl.n_slot = (int)slot.size();
+  if (!comb.empty()) {
+    uint4* d_comb = (uint4*)op->dalloc(sizeof(uint4) * comb.size());
+    h2d(d_comb, comb.data(), sizeof(uint4) * comb.size());
+    tl.comb = d_comb;
+    tl.n_comb = (int)comb.size();
+    tl.part = op->dalloc(op->esz() * (size_t)nparts * TU * TV);
+  }
   return len;
 }
 
@@ -705,12 +737,14 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->d_kk = d_kk;
   op->d_ff = d_ff;
   op->d_weights = d_ws;
-  // degridding launch slots: the tiles that own visibilities, most visibilities first
+  // degridding launch slots: the tiles that own visibilities in chunks, most visibilities first
   {
     std::vector<uint4> so;
+    const uint32_t ch = (uint32_t)std::max<int64_t>(512, (M + 8 * device_cus() - 1) / (8 * device_cus()));
     uint32_t run = 0;
     for (int t = 0; t < op->n_tiles; ++t) {
-      if (owner_count[t]) so.push_back(make_uint4((uint32_t)t, run, run + owner_count[t], 0u));
+      for (uint32_t b = 0; b < owner_count[t]; b += ch)
+        so.push_back(make_uint4((uint32_t)t, run + b, run + std::min(owner_count[t], b + ch), 0u));
       run += owner_count[t];
     }
     std::stable_sort(so.begin(), so.end(), [](const uint4& a, const uint4& b) { return a.z - a.y > b.z - b.y; });
@@ -719,8 +753,8 @@ void build(pf_op* op, const pf_op_desc& d) {
     op->tl.slot_own = dso;
     op->tl.n_own = (int)so.size();
   }
-  // gridding lists of the TU x TV tiles
-  op->tile_list_len = device_lists(op, d_kk, M, op->Ku, op->Kv, TU, TV, op->tl);
+  // gridding lists of the TU x TV tiles (split for the stored-row gridding)
+  op->tile_list_len = device_lists(op, d_kk, M, op->Ku, op->Kv, TU, TV, op->tl, op->w_term != 0);
   // the permutation between the caller's order and the sorted order
   build_perm_dev(op, d_perm);
   op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
@@ -1701,6 +1735,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   }
   if (k == "w_footprint") {
     *value = op->w_term ? op->Ku : 0;
+    return PF_OK;
+  }
+  if (k == "split_tiles") {
+    *value = op->tl.n_comb;
     return PF_OK;
   }
   if (k == "tile_v") {

@@ -266,9 +266,34 @@ __global__ void __launch_bounds__(64) k_grid_wrows(int Nu, int Nv, WRows<R> w, T
     }
   }
   __syncthreads();
-  for (int e = lane; e < TU * TV; e += 64) {
+  if (sl.w == PART_NONE) {
+    for (int e = lane; e < TU * TV; e += 64) {
+      const int ly = e / TU, lx = e - ly * TU;
+      grid[(int64_t)(ty0 + ly) * Nu + tx0 + lx] = acc[e];
+    }
+  } else {  // one part of a split tile: its partial sums, combined below
+    cpx<R>* part = static_cast<cpx<R>*>(tl.part) + (int64_t)sl.w * TU * TV;
+    for (int e = lane; e < TU * TV; e += 64) part[e] = acc[e];
+  }
+}
+
+// The split tiles: each cell sums its parts' partial sums in part order (the list order of the parts'
+// ranges), so the result is the same on every run.
+template <typename R>
+__global__ void __launch_bounds__(256) k_combine_parts(int Nu, TileLists tl, cpx<R>* __restrict__ grid) {
+  const uint4 c = tl.comb[blockIdx.x];
+  const int TU = tl.TU, TT = tl.TU * tl.TV;
+  const int tv = (int)c.x / tl.ntu, tu = (int)c.x - tv * tl.ntu;
+  const cpx<R>* part = static_cast<const cpx<R>*>(tl.part) + (int64_t)c.y * TT;
+  for (int e = threadIdx.x; e < TT; e += 256) {
+    cpx<R> sum = part[e];
+    for (uint32_t p = 1; p < c.z; ++p) {
+      const cpx<R> q = part[(int64_t)p * TT + e];
+      sum.x += q.x;
+      sum.y += q.y;
+    }
     const int ly = e / TU, lx = e - ly * TU;
-    grid[(int64_t)(ty0 + ly) * Nu + tx0 + lx] = acc[e];
+    grid[(int64_t)(tv * tl.TV + ly) * Nu + tu * TU + lx] = sum;
   }
 }
 
@@ -289,7 +314,7 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
                 hipStream_t s) {
   const size_t shm = sizeof(cpx<R>) * (size_t)t.TU * t.TV;
   const int KK = w.Ku * w.Kv;
-  const dim3 blocks((unsigned)(t.ntu * t.ntv));
+  const dim3 blocks((unsigned)t.n_slot);
   ProfScope prof("grid_wrows", s);
   if (KK <= 64)
     hipLaunchKernelGGL((k_grid_wrows<R, 1, 8>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
@@ -304,6 +329,10 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
   else
     throw Error(PF_ERR_UNSUPPORTED, "w-projection rows above 1024 taps");
   PF_HIP(hipGetLastError());
+  if (t.n_comb > 0) {
+    hipLaunchKernelGGL((k_combine_parts<R>), dim3((unsigned)t.n_comb), dim3(256), 0, s, Nu, t, grid);
+    PF_HIP(hipGetLastError());
+  }
 }
 
 template void degrid_wrows<float>(int, int, const WRows<float>&, const int32_t*, const TileLists&, const cpx<float>*,
