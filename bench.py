@@ -75,29 +75,40 @@ def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, KK=None):
     return table.get(name)
 
 
-def serving_run(n_requests, seed, log):
+def serving_run(base, log, precision="c64"):
     """Served latency: a Poisson stream (20 requests / s) of imaging jobs, image side uniform over
     {256, 512, 1024, 2048}, M log-uniform in [1e4, 4e6], 30 % with w-terms; each job builds its operator
     and applies one Phi + Phi^dagger pair, or (25 % of the jobs) normalises it and runs a 100-iteration
-    DB4 PADMM reconstruction (complex64).  Latency runs from arrival to the result on the host, so it
-    includes the operator build and the host transfers."""
+    DB4 PADMM reconstruction.  Latency runs from arrival to the result on the host, so it includes the
+    operator build and the host transfers.  The same requests then arrive all at once: the served rate of
+    that burst is the server's capacity (the Poisson leg's rate is bounded by the arrival rate).  `base`:
+    the stream's requests; both precisions serve the same ones."""
+    import dataclasses
     from purify_amd import serving
-    reqs = serving.poisson_requests(seed, n_requests, 20.0, padmm_fraction=0.25, padmm_iters=100)
+
+    def serve(reqs):
+        srv = serving.Server(96 << 30, max_inflight=4)
+        t0 = time.perf_counter()
+        futs = [srv.submit(r) for r in reqs]
+        res = [f.result() for f in futs]
+        wall = time.perf_counter() - t0
+        srv.close()
+        return serving.latency_stats(res), len(res), wall, srv.peak_inflight
+
+    reqs = [dataclasses.replace(r, precision=precision) for r in base]
     M_total = sum(int(r.u.size) for r in reqs)
-    srv = serving.Server(96 << 30, max_inflight=4)
-    t0 = time.perf_counter()
-    futs = [srv.submit(r) for r in reqs]
-    res = [f.result() for f in futs]
-    wall = time.perf_counter() - t0
-    srv.close()
-    st = serving.latency_stats(res)
-    out = {"requests": len(res), "rate_per_s": 20.0, "p50_ms": 1e3 * st["p50_s"], "p99_ms": 1e3 * st["p99_s"],
-           "mean_ms": 1e3 * st["mean_s"], "throughput_req_per_s": len(res) / wall, "vis_pairs_per_s": M_total / wall,
-           "peak_inflight": srv.peak_inflight, "hbm_budget_gib": 96, "w_term_jobs": sum(r.w_term for r in reqs),
-           "padmm_jobs": sum(r.kind == "padmm" for r in reqs),
-           "sample": f"{len(res)} requests of the serving mix (the full stream is 2000 at 20 / s)"}
-    log(f"[bench] serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
-        f"{out['throughput_req_per_s']:.1f} req/s, peak in flight {srv.peak_inflight}")
+    span = reqs[-1].arrival if reqs else 0.0
+    st, n, wall, peak = serve(reqs)
+    log(f"[bench] serving {precision}: p50 {1e3 * st['p50_s']:.1f} ms, p99 {1e3 * st['p99_s']:.1f} ms, "
+        f"{n / wall:.1f} req/s over a {span:.1f} s arrival span")
+    bst, bn, bwall, _ = serve([dataclasses.replace(r, arrival=1e-9) for r in reqs])  # all pending at the start
+    out = {"requests": n, "rate_per_s": 20.0, "p50_ms": 1e3 * st["p50_s"], "p99_ms": 1e3 * st["p99_s"],
+           "mean_ms": 1e3 * st["mean_s"], "throughput_req_per_s": n / wall, "vis_pairs_per_s": M_total / wall,
+           "capacity_req_per_s": bn / bwall, "burst_p50_ms": 1e3 * bst["p50_s"], "burst_p99_ms": 1e3 * bst["p99_s"],
+           "peak_inflight": peak, "hbm_budget_gib": 96, "w_term_jobs": sum(r.w_term for r in reqs),
+           "padmm_jobs": sum(r.kind == "padmm" for r in reqs), "precision": precision, "arrival_span_s": span,
+           "sample": f"{n} requests of the serving mix (the full stream is 2000 at 20 / s)"}
+    log(f"[bench] serving {precision}: capacity {out['capacity_req_per_s']:.1f} req/s (all {bn} requests at once)")
     return out
 
 
@@ -231,7 +242,18 @@ def multichannel_run(n_channels, iters, seed, log):
     t0 = time.perf_counter()
     res = mc.run(iters)
     t_bat = time.perf_counter() - t0
-    snr = [float(20 * np.log10(np.linalg.norm(x0) / np.linalg.norm(x0 - x))) for x0, x in zip(x0s, mc.images())]
+    imgs = mc.images()
+    snr = [float(20 * np.log10(np.linalg.norm(x0) / np.linalg.norm(x0 - x))) for x0, x in zip(x0s, imgs)]
+    # the channel batch: recorded, its first iteration eager and one iteration captured, then timed anew
+    mc.reset()
+    mc.run_batched(2)
+    mc.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_cb = mc.run_batched(iters)
+    t_cb = time.perf_counter() - t0
+    groups, launches = mc.batch_info()
+    cb_same = all(np.array_equal(a, b) for a, b in zip(mc.images(), imgs))
     mc.close()
     del chans
     torch.cuda.empty_cache()
@@ -239,11 +261,16 @@ def multichannel_run(n_channels, iters, seed, log):
            "image_sides": {str(k): sum(x.size == k * k for x in x0s) for k in (256, 512, 1024)},
            "batched_channel_it_per_s": n_channels * iters / t_bat, "batched_vis_it_per_s": M_total * iters / t_bat,
            "sequential_channel_it_per_s": n_channels * iters / t_seq, "batched_speedup": t_seq / t_bat,
-           "batched_s": t_bat, "setup_s": setup, "all_ran": all(r.niters == iters for r in res),
+           "batched_s": t_bat, "setup_s": setup, "all_ran": all(r.niters == iters for r in res + res_cb),
+           "channel_batch": {"s": t_cb, "channel_it_per_s": n_channels * iters / t_cb,
+                             "vis_it_per_s": M_total * iters / t_cb, "speedup_vs_sequential": t_seq / t_cb,
+                             "speedup_vs_stream_batch": t_bat / t_cb, "groups": groups,
+                             "launches_per_iteration": launches, "equals_stream_batch": cb_same},
            "snr_db": {"min": min(snr), "median": float(np.median(snr)), "max": max(snr)}}
-    log(f"[bench] multichannel {n_channels} x {iters} it: batched {t_bat:.2f} s "
-        f"({out['batched_channel_it_per_s']:.0f} channel-it/s, {out['batched_vis_it_per_s']:.3e} vis-it/s), "
-        f"sequential {t_seq:.2f} s, setup {setup:.1f} s, SNR median {out['snr_db']['median']:.2f} dB")
+    log(f"[bench] multichannel {n_channels} x {iters} it: channel batch {t_cb:.2f} s ({groups} groups, {launches} "
+        f"launches / iteration, {n_channels * iters / t_cb:.0f} channel-it/s), stream batch {t_bat:.2f} s "
+        f"({out['batched_channel_it_per_s']:.0f} channel-it/s), sequential {t_seq:.2f} s, setup {setup:.1f} s, "
+        f"SNR median {out['snr_db']['median']:.2f} dB, channel batch equals stream batch: {cb_same}")
     return out
 
 
@@ -319,7 +346,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=2)
     ap.add_argument("--padmm-iters", type=int, default=100, help="PADMM iterations per leg; 0 skips the PADMM legs")
     ap.add_argument("--channels", type=int, default=64, help="channels of the batched multi-channel PADMM leg; 0 skips it")
-    ap.add_argument("--serve-requests", type=int, default=60,
+    ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests of the serving sample (Poisson arrivals at 20 / s); 0 skips it")
     ap.add_argument("--counters", choices=("c64", "c128"), default=None,
                     help="hardware-counter run: three 512 MiB plain copies (the calibration kernel for "
@@ -494,7 +521,12 @@ def main():
 
     serve = None
     if args.serve_requests > 0 and not w_term:
-        serve = serving_run(args.serve_requests, seed + 10, log)
+        from purify_amd import serving
+        t0 = time.perf_counter()
+        base = serving.poisson_requests(seed + 10, args.serve_requests, 20.0, padmm_fraction=0.25, padmm_iters=100)
+        log(f"[bench] serving: {len(base)} requests generated in {time.perf_counter() - t0:.1f} s")
+        serve = {p: serving_run(base, log, p) for p in ("c64", "c128")}
+        del base
     padmm = None
     if args.padmm_iters > 0 and not w_term:
         padmm = padmm_run(args.padmm_iters, log, with_cpu=not args.no_cpu)

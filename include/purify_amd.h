@@ -366,6 +366,23 @@ void pf_csr_free(pf_csr *m);
  * themselves (operators.h:123-145); this one carries a precomputed G into the same operator. */
 int pf_op_create_csr(const pf_op_desc *desc, const pf_csr *G, pf_op **out);
 
+/* ---- channel batches (no counterpart in the reference, whose channels are separate runs; the closest
+ *      precedent is the stacked visibility blocks of operators.h:513-578) ----
+ * n tight-frame PADMM solvers (pf_padmm_create; not forward-backward, not the iterative proximal) run in
+ * lock-step: every kernel of the iteration is launched once per group of alike channels, its grid's z
+ * index selecting the channel.  Channels form a group when their iterations launch the same kernels with
+ * the same shapes (the same image and grid sizes, kernel support, wavelet, precision and tuning); each group
+ * runs on its own stream.  A channel computes exactly what it computes alone, bit for bit.  The solvers
+ * must outlive the batch; each run synchronises the members' own streams first, adds iters to each
+ * member's iteration count (its pf_padmm_get / stats / sync then report as after pf_padmm_run), and
+ * returns when the iterations are done.  A member whose operator norm, tuning or l1 weight changed is
+ * recorded again.  pf_padmm_batch_info: the groups and the launches one batched iteration makes. */
+typedef struct pf_padmm_batch pf_padmm_batch;
+int pf_padmm_batch_create(pf_padmm *const *solvers, int32_t n, pf_padmm_batch **out);
+int pf_padmm_batch_run(pf_padmm_batch *b, int32_t iters, int32_t use_graph);
+int pf_padmm_batch_info(const pf_padmm_batch *b, int32_t *groups, int32_t *launches_per_iteration);
+int pf_padmm_batch_destroy(pf_padmm_batch *b);
+
 /* ---- device memory ----
  * Handles (operators, solvers, wavelets, PSFs, FFT plans) and their construction temporaries take device
  * memory from the library's per-device block cache, and their destroy functions return it there without a

@@ -225,14 +225,6 @@ static void dispatch_log2(int L, F&& f) {
 // ---------------------------------------------------------------------------------------------
 // Row pass
 // ---------------------------------------------------------------------------------------------
-template <typename R>
-struct RowArgs {
-  const cpx<R>* in;
-  cpx<R>* out;
-  const cpx<R>* tw;
-  int twm, Nu, nlines, lpb;
-  SZ<R> sz;
-};
 
 template <int L>
 constexpr int row_threads() {
@@ -248,8 +240,11 @@ constexpr int row_threads() {
 template <typename R>
 constexpr int row_waves_per_simd() { return sizeof(R) == 4 ? 3 : 2; }
 
-template <typename R, int L, bool INV, int MODE>
-__global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>()) k_fft_rows(RowArgs<R> a) {
+// BT: a batch launch, whose blockIdx.z selects the arguments from tab (one entry per grid of the batch)
+template <typename R, int L, bool INV, int MODE, bool BT = false>
+__global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>())
+    k_fft_rows(RowArgs<R> a0, const RowArgs<R>* __restrict__ tab) {
+  const RowArgs<R> a = BT ? tab[blockIdx.z] : a0;
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   constexpr int N = CPlan<L>::N, T = CPlan<L>::T, E = CPlan<L>::E;
@@ -345,16 +340,19 @@ struct RowLaunch {
   int threads, blocks;
   size_t shm;
   hipStream_t s;
-  void go(void (*k)(RowArgs<R>)) {
+  template <int L, bool I, int MODE>
+  void go() {
+    if (record_launch<RowArgs<R>, k_fft_rows<R, L, I, MODE, true>>(dim3(blocks), dim3(threads), shm, false, a)) return;
+    void (*k)(RowArgs<R>, const RowArgs<R>*) = k_fft_rows<R, L, I, MODE>;
     allow_lds((const void*)k, shm);
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), shm, s, a);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), shm, s, a, nullptr);
   }
   template <int L>
   void operator()() {
-    if (mode == ROW_FWD_SZ) go(k_fft_rows<R, L, false, ROW_FWD_SZ>);
-    else if (mode == ROW_ADJ_CROP) go(k_fft_rows<R, L, true, ROW_ADJ_CROP>);
-    else if (inv) go(k_fft_rows<R, L, true, ROW_PLAIN>);
-    else go(k_fft_rows<R, L, false, ROW_PLAIN>);
+    if (mode == ROW_FWD_SZ) go<L, false, ROW_FWD_SZ>();
+    else if (mode == ROW_ADJ_CROP) go<L, true, ROW_ADJ_CROP>();
+    else if (inv) go<L, true, ROW_PLAIN>();
+    else go<L, false, ROW_PLAIN>();
   }
 };
 
@@ -394,20 +392,10 @@ void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* 
 // ---------------------------------------------------------------------------------------------
 enum ColKind { COL_SINGLE = 0, COL_STEP_A = 1, COL_STEP_B = 2 };
 
-template <typename R>
-struct ColArgs {
-  const cpx<R>* in;
-  cpx<R>* out;
-  const cpx<R>* tw;
-  int twm;       // Lmax / line length
-  int twm_full;  // Lmax / Nv (inter-step twiddle)
-  int Nu, Nv, C, N1, N2;
-  int y0, ny;    // row window
-  int rin, rout; // restrict input / output rows to the window
-};
 
-template <typename R, int L, bool INV, int KIND>
-__global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a) {
+template <typename R, int L, bool INV, int KIND, bool BT = false>
+__global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a0, const ColArgs<R>* __restrict__ tab) {
+  const ColArgs<R> a = BT ? tab[blockIdx.z] : a0;
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   const int C = a.C;
@@ -466,8 +454,10 @@ constexpr int big_col_c(int esz, int L) {
 template <typename R, int L>
 constexpr int big_col_threads() { return big_col_c(sizeof(cpx<R>), L) * ((1 << L) / big_col_e(sizeof(cpx<R>))); }
 
-template <typename R, int L, bool INV>
-__global__ void __launch_bounds__((big_col_threads<R, L>())) k_fft_cols_big(ColArgs<R> a) {
+template <typename R, int L, bool INV, bool BT = false>
+__global__ void __launch_bounds__((big_col_threads<R, L>()))
+    k_fft_cols_big(ColArgs<R> a0, const ColArgs<R>* __restrict__ tab) {
+  const ColArgs<R> a = BT ? tab[blockIdx.z] : a0;
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   constexpr int EE = big_col_e(sizeof(cpx<R>));
@@ -514,15 +504,21 @@ struct BigColLaunch {
   const ColArgs<R>& a;
   bool inv;
   hipStream_t s;
+  template <int L, bool I>
+  void go() {
+    constexpr int C = big_col_c(sizeof(cpx<R>), L), N = 1 << L;
+    const size_t shm = ((size_t)(N + N / 16) * C + tw_lds_entries<L>()) * sizeof(cpx<R>);
+    const dim3 grid((unsigned)(a.Nu / C)), block(big_col_threads<R, L>());
+    if (record_launch<ColArgs<R>, k_fft_cols_big<R, L, I, true>>(grid, block, shm, false, a)) return;
+    void (*k)(ColArgs<R>, const ColArgs<R>*) = k_fft_cols_big<R, L, I>;
+    allow_lds((const void*)k, shm);
+    hipLaunchKernelGGL(k, grid, block, shm, s, a, nullptr);
+  }
   template <int L>
   void operator()() {
     if constexpr (L >= 10 && L <= 13) {
-      constexpr int C = big_col_c(sizeof(cpx<R>), L), N = 1 << L;
-      void (*k)(ColArgs<R>) = inv ? k_fft_cols_big<R, L, true> : k_fft_cols_big<R, L, false>;
-      const size_t shm = ((size_t)(N + N / 16) * C + tw_lds_entries<L>()) * sizeof(cpx<R>);
-      allow_lds((const void*)k, shm);
-      const int groups = a.Nu / C;
-      hipLaunchKernelGGL(k, dim3(groups), dim3(big_col_threads<R, L>()), shm, s, a);
+      if (inv) go<L, true>();
+      else go<L, false>();
     } else {
       throw Error(-3, "single-pass column length must be 1024..8192");
     }
@@ -538,13 +534,20 @@ struct ColLaunch {
   int threads;
   size_t shm;
   hipStream_t s;
+  template <int L, bool I, int KIND>
+  bool record() {
+    return record_launch<ColArgs<R>, k_fft_cols<R, L, I, KIND, true>>(grid, dim3(threads), shm, false, a);
+  }
   template <int L, bool I>
   void go() {
-    void (*k)(ColArgs<R>) = kind == COL_STEP_A   ? k_fft_cols<R, L, I, COL_STEP_A>
-                            : kind == COL_STEP_B ? k_fft_cols<R, L, I, COL_STEP_B>
-                                                 : k_fft_cols<R, L, I, COL_SINGLE>;
+    if (kind == COL_STEP_A ? record<L, I, COL_STEP_A>() : kind == COL_STEP_B ? record<L, I, COL_STEP_B>()
+                                                                             : record<L, I, COL_SINGLE>())
+      return;
+    void (*k)(ColArgs<R>, const ColArgs<R>*) = kind == COL_STEP_A   ? k_fft_cols<R, L, I, COL_STEP_A>
+                                               : kind == COL_STEP_B ? k_fft_cols<R, L, I, COL_STEP_B>
+                                                                    : k_fft_cols<R, L, I, COL_SINGLE>;
     allow_lds((const void*)k, shm);
-    hipLaunchKernelGGL(k, grid, dim3(threads), shm, s, a);
+    hipLaunchKernelGGL(k, grid, dim3(threads), shm, s, a, nullptr);
   }
   template <int L>
   void operator()() {

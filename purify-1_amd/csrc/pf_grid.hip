@@ -188,10 +188,27 @@ constexpr int dg_rows() { return sizeof(R) == 4 ? 10 : 5; }
 // TAB (complex64, J <= 8, tap tables present): the taps come from the tables, whose rows (addressed by
 // the coordinates) are requested one visibility ahead and the coordinates two ahead, so neither round
 // trip is waited for in the loop; otherwise the kernel is evaluated directly.
-template <typename R, int JU, int JV, int TM>
-__global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(GridGeom<R> g, VisData<R> v, TileLists tl,
-                                                             const cpx<R>* __restrict__ grid, cpx<R>* __restrict__ y,
-                                                             R scl) {
+template <typename R>
+struct DegridArgs {
+  GridGeom<R> g;
+  VisData<R> v;
+  TileLists tl;
+  const cpx<R>* grid;
+  cpx<R>* y;
+  R scl;
+};
+// BT: a batch launch, whose blockIdx.z selects the arguments from tab; its grid covers the largest slot
+// count of the batch, so blocks past a grid's own slots return
+template <typename R, int JU, int JV, int TM, bool BT = false>
+__global__ void __launch_bounds__(DG_THREADS) k_degrid_tiles(DegridArgs<R> a0, const DegridArgs<R>* __restrict__ tab) {
+  const DegridArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const GridGeom<R>& g = q.g;
+  const VisData<R>& v = q.v;
+  const TileLists& tl = q.tl;
+  const cpx<R>* __restrict__ grid = q.grid;
+  cpx<R>* __restrict__ y = q.y;
+  const R scl = q.scl;
+  if (BT && blockIdx.x >= (unsigned)tl.n_own) return;
   constexpr bool TAB = TM == TAPS_TABLE && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
   constexpr bool POLY = TM == TAPS_POLY && sizeof(R) == 4 && JU > 0 && JU == JV && JU <= 8;
   extern __shared__ unsigned char smem_raw[];
@@ -565,9 +582,28 @@ __device__ inline BatchPos slot_pos(const TileLists& tl, int j, int nslots) {
 // (TAB: complex64, J <= 8, tap tables present) the tap-table rows one ahead, while the previous
 // batch's MFMAs run; the rows' addresses depend on the coordinates, so this hides a dependent round
 // trip per batch.  A tile's 16 x 16 sums stay in the MFMA accumulators until its last batch.
-template <typename R, int JU, int JV, int TM>
-__global__ void __launch_bounds__(64) k_grid_mfma(GridGeom<R> g, VisData<R> v, TileLists tl, int nslots,
-                                                  const cpx<R>* __restrict__ yin, cpx<R>* __restrict__ grid, R scl) {
+template <typename R>
+struct GridMfmaArgs {
+  GridGeom<R> g;
+  VisData<R> v;
+  TileLists tl;
+  int nslots;
+  const cpx<R>* yin;
+  cpx<R>* grid;
+  R scl;
+};
+// BT: a batch launch, whose blockIdx.z selects the arguments from tab; its grid (the wave count W) covers
+// the largest slot count of the batch, and each tile is still summed by one wave in list order
+template <typename R, int JU, int JV, int TM, bool BT = false>
+__global__ void __launch_bounds__(64) k_grid_mfma(GridMfmaArgs<R> a0, const GridMfmaArgs<R>* __restrict__ tab) {
+  const GridMfmaArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const GridGeom<R>& g = q.g;
+  const VisData<R>& v = q.v;
+  const TileLists& tl = q.tl;
+  const int nslots = q.nslots;
+  const cpx<R>* __restrict__ yin = q.yin;
+  cpx<R>* __restrict__ grid = q.grid;
+  const R scl = q.scl;
   constexpr bool TAB = TM == TAPS_TABLE && sizeof(R) == 4 && JU > 0 && JU <= 8 && JV > 0 && JV <= 8;
   constexpr bool POLY = TM == TAPS_POLY && sizeof(R) == 4 && JU > 0 && JU == JV && JU <= 8;
   constexpr bool ENT = sizeof(R) == 4;
@@ -745,6 +781,12 @@ bool grid_mfma_ok(int Nu, int Nv, int Ju, int Jv) {
   return Nu % MFMA_TILE == 0 && Nv % MFMA_TILE == 0 && Nu >= MFMA_TILE + Ju - 1 && Nv >= MFMA_TILE + Jv - 1;
 }
 
+template <typename R, int A, int B, int TM>
+static void degrid_launch(const DegridArgs<R>& a, dim3 blocks, size_t shm, hipStream_t s) {
+  if (record_launch<DegridArgs<R>, k_degrid_tiles<R, A, B, TM, true>>(blocks, dim3(DG_THREADS), shm, true, a)) return;
+  hipLaunchKernelGGL((k_degrid_tiles<R, A, B, TM>), blocks, dim3(DG_THREADS), shm, s, a, nullptr);
+}
+
 template <typename R>
 void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const cpx<R>* grid, cpx<R>* y, R scale,
             hipStream_t s) {
@@ -754,17 +796,12 @@ void degrid(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const
   ProfScope prof("degrid", s);
   const int tm = sizeof(R) == 4 && g.Ju == g.Jv && g.Ju <= 8 ? (g.poly ? TAPS_POLY : g.tab_u ? TAPS_TABLE : TAPS_EVAL)
                                                             : TAPS_EVAL;
-#define PF_DG(A, B)                                                                                                 \
-  do {                                                                                                              \
-    if (tm == TAPS_POLY && A > 0)                                                                                   \
-      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, TAPS_POLY>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y,  \
-                         scale);                                                                                    \
-    else if (tm == TAPS_TABLE && A > 0)                                                                             \
-      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, TAPS_TABLE>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y, \
-                         scale);                                                                                    \
-    else                                                                                                            \
-      hipLaunchKernelGGL((k_degrid_tiles<R, A, B, TAPS_EVAL>), blocks, dim3(DG_THREADS), shm, s, g, v, t, grid, y,  \
-                         scale);                                                                                    \
+  const DegridArgs<R> a{g, v, t, grid, y, scale};
+#define PF_DG(A, B)                                                                          \
+  do {                                                                                       \
+    if (tm == TAPS_POLY && A > 0) degrid_launch<R, A, B, TAPS_POLY>(a, blocks, shm, s);      \
+    else if (tm == TAPS_TABLE && A > 0) degrid_launch<R, A, B, TAPS_TABLE>(a, blocks, shm, s); \
+    else degrid_launch<R, A, B, TAPS_EVAL>(a, blocks, shm, s);                               \
   } while (0)
   if (g.Ju == g.Jv && fixed_len_ok(g)) {
     switch (g.Ju) {
@@ -788,6 +825,12 @@ bool grid_rc_ok(int Nu, int Nv, int TU, int TV, int Ju, int Jv) {
   return Ju <= 8 && Jv <= 8 && Nu >= TU + 8 && Nv >= TV + 8;
 }
 
+template <typename R, int A, int B, int TM>
+static void grid_mfma_launch(const GridMfmaArgs<R>& a, dim3 blocks, hipStream_t s) {
+  if (record_launch<GridMfmaArgs<R>, k_grid_mfma<R, A, B, TM, true>>(blocks, dim3(64), 0, true, a)) return;
+  hipLaunchKernelGGL((k_grid_mfma<R, A, B, TM>), blocks, dim3(64), 0, s, a, nullptr);
+}
+
 template <typename R>
 void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, const TileLists& tm, const cpx<R>* y,
                 cpx<R>* grid, R scale, hipStream_t s, int variant, int mfma_waves) {
@@ -799,14 +842,12 @@ void grid_tiles(const GridGeom<R>& g, const VisData<R>& v, const TileLists& t, c
     // tap tables (complex64, equal supports <= 8): the pipelined table variant
     const int tmode = sizeof(R) == 4 && g.Ju == g.Jv && g.Ju <= 8 ? (g.poly ? TAPS_POLY : g.tab_u ? TAPS_TABLE : TAPS_EVAL)
                                                               : TAPS_EVAL;
-#define PF_GM(A, B)                                                                                                    \
-  do {                                                                                                                 \
-    if (tmode == TAPS_POLY && A > 0)                                                                                      \
-      hipLaunchKernelGGL((k_grid_mfma<R, A, B, TAPS_POLY>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale);  \
-    else if (tmode == TAPS_TABLE && A > 0)                                                                                \
-      hipLaunchKernelGGL((k_grid_mfma<R, A, B, TAPS_TABLE>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale); \
-    else                                                                                                               \
-      hipLaunchKernelGGL((k_grid_mfma<R, A, B, TAPS_EVAL>), blocks, dim3(64), 0, s, g, v, tm, nslots, y, grid, scale);  \
+    const GridMfmaArgs<R> a{g, v, tm, nslots, y, grid, scale};
+#define PF_GM(A, B)                                                                             \
+  do {                                                                                          \
+    if (tmode == TAPS_POLY && A > 0) grid_mfma_launch<R, A, B, TAPS_POLY>(a, blocks, s);        \
+    else if (tmode == TAPS_TABLE && A > 0) grid_mfma_launch<R, A, B, TAPS_TABLE>(a, blocks, s); \
+    else grid_mfma_launch<R, A, B, TAPS_EVAL>(a, blocks, s);                                    \
   } while (0)
     if (g.Ju == g.Jv && fixed_len_ok(g)) {
       switch (g.Ju) {

@@ -50,6 +50,55 @@ inline hipStream_t io_stream(void* caller, bool host_io) {
   return (!caller && host_io) ? cstream() : static_cast<hipStream_t>(caller);
 }
 
+// ---- launch recording (channel batches) ----------------------------------------------------
+// While a Recorder is installed on the calling thread, the launch sites of the solver iteration's kernels
+// record their argument struct and launch geometry instead of launching.  A channel batch records one
+// iteration per channel, checks that every channel recorded the same sequence of kernels and shapes, and
+// then launches each position once for all channels: the kernel's batch twin (template flag BT) reads
+// its arguments from a device table at blockIdx.z.  The per-channel arithmetic is the single launch's, so
+// a channel's results in the batch equal its results alone, bit for bit.
+struct LaunchRec {
+  const void* key;  // the batch twin
+  dim3 grid, block;
+  size_t shm;
+  bool grid_x_max;  // the kernel strides over gridDim.x: channels may differ there, the batch takes the max
+  std::vector<unsigned char> args;
+  void (*launch)(dim3 grid, dim3 block, size_t shm, hipStream_t s, const void* tab);
+};
+struct Recorder {
+  std::vector<LaunchRec> recs;
+};
+Recorder*& recorder();  // the calling thread's, or null
+
+// (a batch sets the twin's LDS limit when it records, before any launch or capture)
+template <class A, void (*K)(A, const A*)>
+void launch_batch_twin(dim3 grid, dim3 block, size_t shm, hipStream_t s, const void* tab) {
+  hipLaunchKernelGGL(K, grid, block, shm, s, A{}, static_cast<const A*>(tab));
+}
+// Records the launch of K's batch twin KB with arguments a when a Recorder is installed (and returns true).
+template <class A, void (*KB)(A, const A*)>
+bool record_launch(dim3 grid, dim3 block, size_t shm, bool grid_x_max, const A& a) {
+  Recorder* r = recorder();
+  if (!r) return false;
+  LaunchRec rec;
+  rec.key = reinterpret_cast<const void*>(KB);
+  rec.grid = grid;
+  rec.block = block;
+  rec.shm = shm;
+  rec.grid_x_max = grid_x_max;
+  rec.args.assign(reinterpret_cast<const unsigned char*>(&a), reinterpret_cast<const unsigned char*>(&a) + sizeof(A));
+  rec.launch = &launch_batch_twin<A, KB>;
+  r->recs.push_back(std::move(rec));
+  return true;
+}
+
+// Launches K(a) on s, or records its batch twin KB when a Recorder is installed.
+template <class A, void (*K)(A, const A*), void (*KB)(A, const A*)>
+void launch_or_record(dim3 grid, dim3 block, size_t shm, bool grid_x_max, const A& a, hipStream_t s) {
+  if (record_launch<A, KB>(grid, block, shm, grid_x_max, a)) return;
+  hipLaunchKernelGGL(K, grid, block, shm, s, a, nullptr);
+}
+
 // The calling thread's last error message (pf_last_error) and the C-ABI wrappers that set it.
 std::string& last_error();
 inline int fail(int code, const std::string& msg) {
@@ -153,6 +202,28 @@ struct SZ {
 template <typename R>
 void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* out, int nlines, const SZ<R>& sz,
               hipStream_t s);
+
+// Kernel arguments of the row and column passes (recorded by a batch: see Recorder).
+template <typename R>
+struct RowArgs {
+  const cpx<R>* in;
+  cpx<R>* out;
+  const cpx<R>* tw;
+  int twm, Nu, nlines, lpb;
+  SZ<R> sz;
+};
+template <typename R>
+struct ColArgs {
+  const cpx<R>* in;
+  cpx<R>* out;
+  const cpx<R>* tw;
+  int twm;       // Lmax / line length
+  int twm_full;  // Lmax / Nv (inter-step twiddle)
+  int Nu, Nv, C, N1, N2;
+  int y0, ny;    // row window
+  int rin, rout; // restrict input / output rows to the window
+};
+
 // Column pass.  restrict_in: rows outside win are zero and not read; restrict_out: only rows in
 // win are written.  Two-step plans overwrite `in` with the intermediate result.
 template <typename R>

@@ -169,6 +169,11 @@ void dev_cache_stats(int64_t& live_bytes, int64_t& cached_bytes) {
   cached_bytes = (int64_t)n;
 }
 
+Recorder*& recorder() {
+  static thread_local Recorder* r = nullptr;
+  return r;
+}
+
 hipStream_t cstream() {
   const int dev = current_device();
   hipStream_t& s = t_streams.s[dev];

@@ -38,6 +38,7 @@
 // statistics (||x - x_prev||^2, ||x||^2, ||r||^2) are appended to a device array through a device
 // counter, so a replayed graph records every iteration.
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -128,8 +129,18 @@ __device__ inline void reduce_partials(double (&acc)[NV], double* __restrict__ p
 #pragma unroll
     for (int v = 0; v < NV; ++v) part[v * FR_BLOCKS + blockIdx.x] = sh[v][0];
 }
+// The kernels of the PADMM iteration take their arguments as one struct; BT marks a batch launch, whose
+// blockIdx.z selects the arguments from tab (see Recorder).
+struct FrFinalArgs {
+  const double* part;
+  double* out;
+};
 // out[v] = sum of the FR_BLOCKS partials of quantity v (block v of the launch)
-__global__ void __launch_bounds__(SR_THREADS) k_fr_final(const double* __restrict__ part, double* __restrict__ out) {
+template <bool BT = false>
+__global__ void __launch_bounds__(SR_THREADS) k_fr_final(FrFinalArgs a0, const FrFinalArgs* __restrict__ tab) {
+  const FrFinalArgs q = BT ? tab[blockIdx.z] : a0;
+  const double* __restrict__ part = q.part;
+  double* __restrict__ out = q.out;
   __shared__ double sh[SR_THREADS];
   const double* p = part + blockIdx.x * FR_BLOCKS;
   double acc = 0.0;
@@ -157,8 +168,25 @@ __global__ void __launch_bounds__(SR_THREADS) k_ball_norm(const cpx<R>* __restri
 }
 // v = -lambda - r; z = P_eps(v) (v scaled by min(1, eps / ||v||), ||v||^2 read from *nrm2); w = r + lambda + z
 template <typename R>
-__global__ void k_ball_out(const double* __restrict__ nrm2, double eps, const cpx<R>* __restrict__ r,
-                           const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ z, cpx<R>* __restrict__ w, int64_t n) {
+struct BallOutArgs {
+  const double* nrm2;
+  double eps;
+  const cpx<R>* r;
+  const cpx<R>* lam;
+  cpx<R>* z;
+  cpx<R>* w;
+  int64_t n;
+};
+template <typename R, bool BT = false>
+__global__ void k_ball_out(BallOutArgs<R> a0, const BallOutArgs<R>* __restrict__ tab) {
+  const BallOutArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const double* __restrict__ nrm2 = q.nrm2;
+  const double eps = q.eps;
+  const cpx<R>* __restrict__ r = q.r;
+  const cpx<R>* __restrict__ lam = q.lam;
+  cpx<R>* __restrict__ z = q.z;
+  cpx<R>* __restrict__ w = q.w;
+  const int64_t n = q.n;
   const double nv = sqrt(*nrm2);
   const R f = (R)(nv > eps ? eps / nv : 1.0);
   GRID_STRIDE(k, n) {
@@ -171,9 +199,25 @@ __global__ void k_ball_out(const double* __restrict__ nrm2, double eps, const cp
 // x = xu + d / nu (d null: x as it is); out[0] = ||x - xprev||^2, out[1] = ||x||^2.  xprev may be x itself
 // (its old value is read before the new one is written), so x and xprev are not restrict.
 template <typename R>
-__global__ void __launch_bounds__(SR_THREADS) k_prox_out_norm(const cpx<R>* __restrict__ xu, const cpx<R>* __restrict__ d,
-                                                              cpx<R>* x, const cpx<R>* xprev, R inv_nu, int64_t n,
-                                                              double* part) {
+struct ProxOutArgs {
+  const cpx<R>* xu;
+  const cpx<R>* d;
+  cpx<R>* x;
+  const cpx<R>* xprev;
+  R inv_nu;
+  int64_t n;
+  double* part;
+};
+template <typename R, bool BT = false>
+__global__ void __launch_bounds__(SR_THREADS) k_prox_out_norm(ProxOutArgs<R> a0, const ProxOutArgs<R>* __restrict__ tab) {
+  const ProxOutArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const cpx<R>* __restrict__ xu = q.xu;
+  const cpx<R>* __restrict__ d = q.d;
+  cpx<R>* x = q.x;
+  const cpx<R>* xprev = q.xprev;
+  const R inv_nu = q.inv_nu;
+  const int64_t n = q.n;
+  double* part = q.part;
   double acc[2] = {0.0, 0.0};
   FR_STRIDE(k, n) {
     const cpx<R> p = xprev[k];
@@ -193,10 +237,28 @@ __global__ void __launch_bounds__(SR_THREADS) k_prox_out_norm(const cpx<R>* __re
 // r = fx - y; lambda += rho (r + z); out[0] = ||r||^2, out[1] = ||lambda + r||^2 (the next iteration's
 // ball norm, ||v||^2 for v = -lambda - r)
 template <typename R>
-__global__ void __launch_bounds__(SR_THREADS) k_residual_dual_norm(const cpx<R>* __restrict__ fx, const cpx<R>* __restrict__ y,
-                                                                   const cpx<R>* __restrict__ z, cpx<R>* __restrict__ r,
-                                                                   cpx<R>* __restrict__ lam, R rho, int64_t n,
-                                                                   double* part) {
+struct ResidualArgs {
+  const cpx<R>* fx;
+  const cpx<R>* y;
+  const cpx<R>* z;
+  cpx<R>* r;
+  cpx<R>* lam;
+  R rho;
+  int64_t n;
+  double* part;
+};
+template <typename R, bool BT = false>
+__global__ void __launch_bounds__(SR_THREADS) k_residual_dual_norm(ResidualArgs<R> a0,
+                                                                   const ResidualArgs<R>* __restrict__ tab) {
+  const ResidualArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const cpx<R>* __restrict__ fx = q.fx;
+  const cpx<R>* __restrict__ y = q.y;
+  const cpx<R>* __restrict__ z = q.z;
+  cpx<R>* __restrict__ r = q.r;
+  cpx<R>* __restrict__ lam = q.lam;
+  const R rho = q.rho;
+  const int64_t n = q.n;
+  double* part = q.part;
   double acc[2] = {0.0, 0.0};
   FR_STRIDE(k, n) {
     const cpx<R> a = fx[k], b = y[k], c = z[k];
@@ -211,9 +273,24 @@ __global__ void __launch_bounds__(SR_THREADS) k_residual_dual_norm(const cpx<R>*
   reduce_partials<2>(acc, part);
 }
 // xu = x - g / nu (COPY: and xprev = x)
-template <typename R, bool COPY>
-__global__ void k_grad_step(cpx<R>* __restrict__ x, const cpx<R>* __restrict__ g, cpx<R>* __restrict__ xprev,
-                            cpx<R>* __restrict__ xu, R inv_nu, int64_t n) {
+template <typename R>
+struct GradArgs {
+  cpx<R>* x;
+  const cpx<R>* g;
+  cpx<R>* xprev;
+  cpx<R>* xu;
+  R inv_nu;
+  int64_t n;
+};
+template <typename R, bool COPY, bool BT = false>
+__global__ void k_grad_step(GradArgs<R> a0, const GradArgs<R>* __restrict__ tab) {
+  const GradArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  cpx<R>* __restrict__ x = q.x;
+  const cpx<R>* __restrict__ g = q.g;
+  cpx<R>* __restrict__ xprev = q.xprev;
+  cpx<R>* __restrict__ xu = q.xu;
+  const R inv_nu = q.inv_nu;
+  const int64_t n = q.n;
   GRID_STRIDE(k, n) {
     const cpx<R> a = x[k], b = g[k];
     if (COPY) xprev[k] = a;
@@ -222,7 +299,17 @@ __global__ void k_grad_step(cpx<R>* __restrict__ x, const cpx<R>* __restrict__ g
 }
 // c <- soft(c, t) - c
 template <typename R>
-__global__ void k_soft_diff(cpx<R>* __restrict__ c, R t, int64_t n) {
+struct SoftArgs {
+  cpx<R>* c;
+  R t;
+  int64_t n;
+};
+template <typename R, bool BT = false>
+__global__ void k_soft_diff(SoftArgs<R> a0, const SoftArgs<R>* __restrict__ tab) {
+  const SoftArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  cpx<R>* __restrict__ c = q.c;
+  const R t = q.t;
+  const int64_t n = q.n;
   GRID_STRIDE(k, n) {
     const cpx<R> a = c[k];
     const R m = sqrt(a.x * a.x + a.y * a.y);
@@ -293,7 +380,19 @@ __global__ void k_zero(cpx<R>* __restrict__ a, int64_t n) {
   GRID_STRIDE(k, n) a[k] = mk<R>(R(0), R(0));
 }
 // stats[*counter] = cur (3 doubles); ++*counter (capacity checked: the run never exceeds it)
-__global__ void k_record(const double* __restrict__ cur, double* __restrict__ stats, int* __restrict__ counter, int cap) {
+struct RecordArgs {
+  const double* cur;
+  double* stats;
+  int* counter;
+  int cap;
+};
+template <bool BT = false>
+__global__ void k_record(RecordArgs a0, const RecordArgs* __restrict__ tab) {
+  const RecordArgs q = BT ? tab[blockIdx.z] : a0;
+  const double* __restrict__ cur = q.cur;
+  double* __restrict__ stats = q.stats;
+  int* __restrict__ counter = q.counter;
+  const int cap = q.cap;
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     const int c = *counter;
     if (c < cap) {
@@ -306,6 +405,44 @@ __global__ void k_record(const double* __restrict__ cur, double* __restrict__ st
 }
 __global__ void k_set_int(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// launch (or record) helpers of the iteration's kernels
+inline void fr_final(const double* part, double* out, int nq, hipStream_t s) {
+  launch_or_record<FrFinalArgs, k_fr_final<>, k_fr_final<true>>(dim3(nq), dim3(SR_THREADS), 0, false,
+                                                                 FrFinalArgs{part, out}, s);
+}
+template <typename R>
+void ball_out(const double* nrm2, double eps, const cpx<R>* r, const cpx<R>* lam, cpx<R>* z, cpx<R>* w, int64_t n,
+              hipStream_t s) {
+  launch_or_record<BallOutArgs<R>, k_ball_out<R>, k_ball_out<R, true>>(vgrid(n), dim3(256), 0, true,
+                                                                       BallOutArgs<R>{nrm2, eps, r, lam, z, w, n}, s);
+}
+template <typename R>
+void prox_out_norm(const cpx<R>* xu, const cpx<R>* d, cpx<R>* x, const cpx<R>* xprev, R inv_nu, int64_t n, double* part,
+                   hipStream_t s) {
+  launch_or_record<ProxOutArgs<R>, k_prox_out_norm<R>, k_prox_out_norm<R, true>>(
+      dim3(FR_BLOCKS), dim3(SR_THREADS), 0, false, ProxOutArgs<R>{xu, d, x, xprev, inv_nu, n, part}, s);
+}
+template <typename R>
+void residual_dual_norm(const cpx<R>* fx, const cpx<R>* y, const cpx<R>* z, cpx<R>* r, cpx<R>* lam, R rho, int64_t n,
+                        double* part, hipStream_t s) {
+  launch_or_record<ResidualArgs<R>, k_residual_dual_norm<R>, k_residual_dual_norm<R, true>>(
+      dim3(FR_BLOCKS), dim3(SR_THREADS), 0, false, ResidualArgs<R>{fx, y, z, r, lam, rho, n, part}, s);
+}
+template <typename R, bool COPY>
+void grad_step(cpx<R>* x, const cpx<R>* g, cpx<R>* xprev, cpx<R>* xu, R inv_nu, int64_t n, hipStream_t s) {
+  launch_or_record<GradArgs<R>, k_grad_step<R, COPY>, k_grad_step<R, COPY, true>>(
+      vgrid(n), dim3(256), 0, true, GradArgs<R>{x, g, xprev, xu, inv_nu, n}, s);
+}
+template <typename R>
+void soft_diff(cpx<R>* c, R t, int64_t n, hipStream_t s) {
+  launch_or_record<SoftArgs<R>, k_soft_diff<R>, k_soft_diff<R, true>>(vgrid(n), dim3(256), 0, true,
+                                                                      SoftArgs<R>{c, t, n}, s);
+}
+inline void record_stats(const double* cur, double* stats, int* counter, int cap, hipStream_t s) {
+  launch_or_record<RecordArgs, k_record<>, k_record<true>>(dim3(1), dim3(64), 0, false,
+                                                           RecordArgs{cur, stats, counter, cap}, s);
 }
 
 }  // namespace
@@ -421,32 +558,27 @@ void padmm_step(pf_padmm* h) {
   const R inv_nu = (R)(1.0 / h->par.nu);
   // cur[3] = ||lambda + r||^2 from the previous iteration's residual pass (or the initialisation)
   // fx holds r + lambda + z until the forward apply below overwrites it
-  hipLaunchKernelGGL((k_ball_out<R>), vgrid(M), dim3(256), 0, s, h->cur + 3, h->par.epsilon, V<R>(h->r), V<R>(h->lam),
-                     V<R>(h->z), V<R>(h->fx), M);
+  ball_out<R>(h->cur + 3, h->par.epsilon, V<R>(h->r), V<R>(h->lam), V<R>(h->z), V<R>(h->fx), M, s);
   op_adjoint_sorted<R>(h->phi, V<R>(h->fx), V<R>(h->g), s);
   if (h->tight) {
     // x keeps its old value until the proximal output replaces it, so no copy of x_prev is made
-    hipLaunchKernelGGL((k_grad_step<R, false>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev),
-                       V<R>(h->xu), inv_nu, N);
+    grad_step<R, false>(V<R>(h->x), V<R>(h->g), V<R>(h->xprev), V<R>(h->xu), inv_nu, N, s);
     wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
     // the l1 proximal's own frame bound l1_nu (SOPT's L1TightFrame nu), not the ADMM step nu
-    hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(h->gamma * h->par.l1_nu), NC);
+    soft_diff<R>(V<R>(h->c), (R)(h->gamma * h->par.l1_nu), NC, s);
     wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
-    hipLaunchKernelGGL((k_prox_out_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x),
-                       V<R>(h->x), (R)(1.0 / h->par.l1_nu), N, h->fpart);
+    prox_out_norm<R>(V<R>(h->xu), V<R>(h->d), V<R>(h->x), V<R>(h->x), (R)(1.0 / h->par.l1_nu), N, h->fpart, s);
   } else {
-    hipLaunchKernelGGL((k_grad_step<R, true>), vgrid(N), dim3(256), 0, s, V<R>(h->x), V<R>(h->g), V<R>(h->xprev),
-                       V<R>(h->xu), inv_nu, N);
+    grad_step<R, true>(V<R>(h->x), V<R>(h->g), V<R>(h->xprev), V<R>(h->xu), inv_nu, N, s);
     l1_prox_iterative<R>(h, h->gamma);
-    hipLaunchKernelGGL((k_prox_out_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->xu), (const cpx<R>*)nullptr,
-                       V<R>(h->x), V<R>(h->xprev), inv_nu, N, h->fpart);
+    prox_out_norm<R>(V<R>(h->xu), (const cpx<R>*)nullptr, V<R>(h->x), V<R>(h->xprev), inv_nu, N, h->fpart, s);
   }
-  hipLaunchKernelGGL(k_fr_final, dim3(2), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 0);
+  fr_final(h->fpart, h->cur + 0, 2, s);
   op_forward_sorted<R>(h->phi, V<R>(h->x), V<R>(h->fx), s);
-  hipLaunchKernelGGL((k_residual_dual_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->fx), V<R>(h->y),
-                     V<R>(h->z), V<R>(h->r), V<R>(h->lam), (R)h->par.lagrange_update_scale, M, h->fpart);
-  hipLaunchKernelGGL(k_fr_final, dim3(2), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 2);
-  hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, h->cur, h->stats, h->counter, h->cap);
+  residual_dual_norm<R>(V<R>(h->fx), V<R>(h->y), V<R>(h->z), V<R>(h->r), V<R>(h->lam), (R)h->par.lagrange_update_scale,
+                        M, h->fpart, s);
+  fr_final(h->fpart, h->cur + 2, 2, s);
+  record_stats(h->cur, h->stats, h->counter, h->cap, s);
   PF_HIP(hipGetLastError());
 }
 
@@ -475,7 +607,7 @@ void fb_iteration(pf_padmm* h) {
                      V<R>(h->xu), V<R>(h->v), (R)h->fb_step, N);
   if (h->tight) {
     wavelet_analysis<R>(h->psi, V<R>(h->xu), V<R>(h->c), s);
-    hipLaunchKernelGGL((k_soft_diff<R>), vgrid(NC), dim3(256), 0, s, V<R>(h->c), (R)(gam * h->par.l1_nu), NC);
+    soft_diff<R>(V<R>(h->c), (R)(gam * h->par.l1_nu), NC, s);
     wavelet_synthesis<R>(h->psi, V<R>(h->c), V<R>(h->d), s);
     hipLaunchKernelGGL((k_prox_out<R>), vgrid(N), dim3(256), 0, s, V<R>(h->xu), V<R>(h->d), V<R>(h->x),
                        (R)(1.0 / h->par.l1_nu), N);
@@ -485,7 +617,7 @@ void fb_iteration(pf_padmm* h) {
   sq_norm<R>(V<R>(h->x), V<R>(h->xprev), N, h->part, h->cur + 0, s);
   sq_norm<R>(V<R>(h->x), nullptr, N, h->part, h->cur + 1, s);
   sq_norm<R>(V<R>(h->v), nullptr, N, h->part, h->cur + 2, s);
-  hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, s, h->cur, h->stats, h->counter, h->cap);
+  record_stats(h->cur, h->stats, h->counter, h->cap, s);
   PF_HIP(hipGetLastError());
 }
 
@@ -523,7 +655,7 @@ void padmm_init(pf_padmm* h) {
   hipLaunchKernelGGL((k_sub<R>), vgrid(M), dim3(256), 0, s, V<R>(h->fx), V<R>(h->y), V<R>(h->r), M);
   hipLaunchKernelGGL((k_zero<R>), vgrid(M), dim3(256), 0, s, V<R>(h->lam), M);
   hipLaunchKernelGGL((k_ball_norm<R>), dim3(FR_BLOCKS), dim3(SR_THREADS), 0, s, V<R>(h->lam), V<R>(h->r), M, h->fpart);
-  hipLaunchKernelGGL(k_fr_final, dim3(1), dim3(SR_THREADS), 0, s, h->fpart, h->cur + 3);
+  fr_final(h->fpart, h->cur + 3, 1, s);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, h->counter, 0);
   PF_HIP(hipStreamSynchronize(s));
   h->iters_done = 0;
@@ -925,6 +1057,275 @@ int pf_padmm_stats(pf_padmm* h, double* out, int32_t cap, int32_t* count) {
 
 int pf_padmm_destroy(pf_padmm* h) {
   delete h;
+  return PF_OK;
+}
+
+}  // extern "C"
+
+// ---- channel batches: one launch per kernel of the iteration for every group of alike channels ----
+// Each member solver's tight-frame PADMM iteration is recorded (Recorder: the launch sites record their
+// arguments and shapes instead of launching, inside a stream capture that must stay empty, so that a
+// launch without a batch twin is caught).  Channels whose recordings agree position by position (the
+// same kernels, block shapes, LDS and grids, or grids that differ only where the kernel strides over
+// gridDim.x) form a group; each group runs on its own stream, every position launched once with the
+// group's argument table, blockIdx.z selecting the channel.
+struct pf_padmm_batch {
+  struct Pos {
+    void (*launch)(dim3, dim3, size_t, hipStream_t, const void*);
+    dim3 grid, block;
+    size_t shm, off;
+  };
+  struct Group {
+    std::vector<int> members;
+    std::vector<Pos> pos;
+    hipStream_t s = nullptr;
+    hipEvent_t done = nullptr;
+  };
+  std::vector<pf_padmm*> h;  // not owned
+  std::vector<Group> groups;
+  void* d_tab = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> rec_gen;  // each member's operator generation and l1 weight when recorded
+  std::vector<double> rec_gamma;
+  bool ran = false;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void drop_groups() {
+    for (Group& g : groups) {
+      if (g.s) (void)hipStreamDestroy(g.s);
+      if (g.done) (void)hipEventDestroy(g.done);
+    }
+    groups.clear();
+    dev_free(d_tab);
+    d_tab = nullptr;
+  }
+  ~pf_padmm_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
+    drop_groups();
+    if (fork) (void)hipEventDestroy(fork);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace pf {
+namespace {
+
+bool same_shape(const LaunchRec& a, const LaunchRec& b) {
+  return a.key == b.key && a.block.x == b.block.x && a.block.y == b.block.y && a.block.z == b.block.z &&
+         a.shm == b.shm && a.grid.y == b.grid.y && a.grid.z == b.grid.z && a.args.size() == b.args.size() &&
+         a.grid_x_max == b.grid_x_max && (a.grid_x_max || a.grid.x == b.grid.x);
+}
+
+void batch_record(pf_padmm_batch* b) {
+  const int n = (int)b->h.size();
+  std::vector<std::vector<LaunchRec>> recs((size_t)n);
+  hipStream_t cap = nullptr;
+  PF_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  PF_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+  size_t stray = 0;
+  try {
+    for (int c = 0; c < n; ++c) {
+      pf_padmm* h = b->h[(size_t)c];
+      Recorder r;
+      recorder() = &r;
+      const hipStream_t own = h->stream;
+      h->stream = cap;
+      try {
+        if (h->prec == PF_C64) padmm_step<float>(h);
+        else padmm_step<double>(h);
+      } catch (...) {
+        h->stream = own;
+        recorder() = nullptr;
+        throw;
+      }
+      h->stream = own;
+      recorder() = nullptr;
+      recs[(size_t)c] = std::move(r.recs);
+    }
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(cap, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cap);
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  PF_HIP(hipStreamEndCapture(cap, &g));
+  if (g) {
+    (void)hipGraphGetNodes(g, nullptr, &stray);
+    (void)hipGraphDestroy(g);
+  }
+  (void)hipStreamDestroy(cap);
+  if (stray)
+    throw Error(PF_ERR_UNSUPPORTED, "the iteration queues work that a channel batch cannot take (" +
+                                        std::to_string(stray) + " operations; is operator profiling on?)");
+  // groups of channels whose recordings agree position by position
+  b->drop_graph();
+  b->drop_groups();
+  for (int c = 0; c < n; ++c) {
+    int gi = -1;
+    for (size_t k = 0; k < b->groups.size() && gi < 0; ++k) {
+      const auto& rep = recs[(size_t)b->groups[k].members[0]];
+      const auto& mine = recs[(size_t)c];
+      bool ok = rep.size() == mine.size();
+      for (size_t p = 0; ok && p < rep.size(); ++p) ok = same_shape(rep[p], mine[p]);
+      if (ok) gi = (int)k;
+    }
+    if (gi < 0) {
+      b->groups.emplace_back();
+      gi = (int)b->groups.size() - 1;
+    }
+    b->groups[(size_t)gi].members.push_back(c);
+  }
+  if (b->groups.size() > 65535) throw Error(PF_ERR_UNSUPPORTED, "too many channel groups");
+  // the argument tables: per group and position, the members' argument structs side by side
+  std::vector<unsigned char> host;
+  for (auto& gr : b->groups) {
+    const auto& rep = recs[(size_t)gr.members[0]];
+    for (size_t p = 0; p < rep.size(); ++p) {
+      pf_padmm_batch::Pos pos;
+      pos.launch = rep[p].launch;
+      pos.grid = rep[p].grid;
+      pos.block = rep[p].block;
+      pos.shm = rep[p].shm;
+      pos.off = (host.size() + 15) & ~size_t(15);
+      host.resize(pos.off);
+      const size_t sz = rep[p].args.size();  // the kernel's stride: sizeof of its argument struct
+      for (int c : gr.members) {
+        const LaunchRec& lr = recs[(size_t)c][p];
+        if (lr.grid_x_max) pos.grid.x = std::max(pos.grid.x, lr.grid.x);
+        const size_t at = host.size();
+        host.resize(at + sz, 0);
+        std::memcpy(host.data() + at, lr.args.data(), lr.args.size());
+      }
+      pos.grid.z = (unsigned)gr.members.size();
+      if (pos.shm > 65536)
+        PF_HIP(hipFuncSetAttribute(rep[p].key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pos.shm));
+      gr.pos.push_back(pos);
+    }
+    if (b->groups.size() > 1) {
+      PF_HIP(hipStreamCreateWithFlags(&gr.s, hipStreamNonBlocking));
+      PF_HIP(hipEventCreateWithFlags(&gr.done, hipEventDisableTiming));
+    }
+  }
+  b->d_tab = dev_alloc(std::max<size_t>(host.size(), 16));
+  h2d(b->d_tab, host.data(), host.size());
+  b->rec_gen.resize((size_t)n);
+  b->rec_gamma.resize((size_t)n);
+  for (int c = 0; c < n; ++c) {
+    b->rec_gen[(size_t)c] = op_generation(b->h[(size_t)c]->phi);
+    b->rec_gamma[(size_t)c] = b->h[(size_t)c]->gamma;
+  }
+}
+
+// one iteration of every member, queued on the batch's stream (forked over the groups' streams)
+void batch_iteration(pf_padmm_batch* b) {
+  const bool fork = b->groups.size() > 1;
+  if (fork) PF_HIP(hipEventRecord(b->fork, b->stream));
+  for (auto& gr : b->groups) {
+    hipStream_t s = fork ? gr.s : b->stream;
+    if (fork) PF_HIP(hipStreamWaitEvent(s, b->fork, 0));
+    for (const auto& p : gr.pos)
+      p.launch(p.grid, p.block, p.shm, s, static_cast<const unsigned char*>(b->d_tab) + p.off);
+    if (fork) {
+      PF_HIP(hipEventRecord(gr.done, s));
+      PF_HIP(hipStreamWaitEvent(b->stream, gr.done, 0));
+    }
+  }
+  PF_HIP(hipGetLastError());
+}
+
+void batch_run(pf_padmm_batch* b, int iters, bool use_graph) {
+  const int n = (int)b->h.size();
+  bool stale = b->groups.empty();
+  for (int c = 0; c < n; ++c) {
+    pf_padmm* h = b->h[(size_t)c];
+    if (h->iters_done + iters > h->cap) throw Error(PF_ERR_ARG, "more iterations than a member solver was created for");
+    if (!stale && (op_generation(h->phi) != b->rec_gen[(size_t)c] || h->gamma != b->rec_gamma[(size_t)c])) stale = true;
+    PF_HIP(hipStreamSynchronize(h->stream));  // the member's own queued work first
+  }
+  if (stale) batch_record(b);
+  int done = 0;
+  if (use_graph && !b->exec && iters > 0 && !b->ran) {
+    batch_iteration(b);  // the first launches of the batch twins run eagerly, before any capture
+    done = 1;
+  }
+  if (use_graph && !b->exec && iters > done) {
+    PF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    try {
+      batch_iteration(b);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(b->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    PF_HIP(hipStreamEndCapture(b->stream, &b->graph));
+    PF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  for (; done < iters; ++done) {
+    if (use_graph && b->exec) PF_HIP(hipGraphLaunch(b->exec, b->stream));
+    else batch_iteration(b);
+  }
+  if (iters > 0) b->ran = true;
+  PF_HIP(hipStreamSynchronize(b->stream));
+  for (pf_padmm* h : b->h) h->iters_done += iters;
+}
+
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+int pf_padmm_batch_create(pf_padmm* const* solvers, int32_t n, pf_padmm_batch** out) {
+  if (!solvers || !out || n <= 0) return fail(PF_ERR_ARG, "null argument or empty batch");
+  *out = nullptr;
+  std::unique_ptr<pf_padmm_batch> b(new pf_padmm_batch());
+  int rc = guarded([&] {
+    for (int c = 0; c < n; ++c) {
+      pf_padmm* h = solvers[c];
+      if (!h) throw Error(PF_ERR_ARG, "null solver in the batch");
+      if (h->fb || !h->tight) throw Error(PF_ERR_UNSUPPORTED, "channel batches take tight-frame PADMM solvers");
+      for (int k = 0; k < c; ++k)
+        if (solvers[k] == h) throw Error(PF_ERR_ARG, "a solver appears twice in the batch");
+      b->h.push_back(h);
+    }
+    PF_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    PF_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (pf_padmm* h : b->h) PF_HIP(hipStreamSynchronize(h->stream));
+    batch_record(b.get());
+  });
+  if (rc == PF_OK) *out = b.release();
+  return rc;
+}
+
+int pf_padmm_batch_run(pf_padmm_batch* b, int32_t iters, int32_t use_graph) {
+  return guarded([&] {
+    if (!b) throw Error(PF_ERR_ARG, "null batch");
+    if (iters < 0) throw Error(PF_ERR_ARG, "iterations must be >= 0");
+    batch_run(b, iters, use_graph != 0);
+  });
+}
+
+int pf_padmm_batch_info(const pf_padmm_batch* b, int32_t* groups, int32_t* launches_per_iteration) {
+  if (!b) return fail(PF_ERR_ARG, "null batch");
+  int64_t l = 0;
+  for (const auto& g : b->groups) l += (int64_t)g.pos.size();
+  if (groups) *groups = (int32_t)b->groups.size();
+  if (launches_per_iteration) *launches_per_iteration = (int32_t)l;
+  return PF_OK;
+}
+
+int pf_padmm_batch_destroy(pf_padmm_batch* b) {
+  delete b;
   return PF_OK;
 }
 

@@ -117,10 +117,27 @@ struct WavFilt {
 // may be dst itself, multiplied by sa.  ls: row stride of src.  32-bit indices (an image holds < 2^31
 // pixels).  src must not overlap dst or apx.
 constexpr int DW_T = 16;
-template <typename R, int L>
-__global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, int ls, cpx<R>* __restrict__ dst, int ld,
-                                               cpx<R>* __restrict__ apx, int la, int h, int w, WavFilt<R> f, R sd,
-                                               R sa) {
+template <typename R>
+struct DwtArgs {
+  const cpx<R>* src;
+  int ls;
+  cpx<R>* dst;
+  int ld;
+  cpx<R>* apx;
+  int la, h, w;
+  WavFilt<R> f;
+  R sd, sa;
+};
+// BT: a batch launch, whose blockIdx.z selects the arguments from tab
+template <typename R, int L, bool BT = false>
+__global__ void __launch_bounds__(256) k_dwt2d(DwtArgs<R> a0, const DwtArgs<R>* __restrict__ tab) {
+  const DwtArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const cpx<R>* __restrict__ src = q.src;
+  cpx<R>* __restrict__ dst = q.dst;
+  cpx<R>* __restrict__ apx = q.apx;
+  const int ls = q.ls, ld = q.ld, la = q.la, h = q.h, w = q.w;
+  const WavFilt<R>& f = q.f;
+  const R sd = q.sd, sa = q.sa;
   constexpr int RI = 2 * DW_T + L - 2;  // staged rows and columns
   __shared__ cpx<R> in_t[RI][RI + 1];
   __shared__ cpx<R> rp_t[RI][2 * DW_T + 1];
@@ -183,10 +200,26 @@ __global__ void __launch_bounds__(256) k_dwt2d(const cpx<R>* __restrict__ src, i
 // quadrant comes from apx (row stride la; the coarser level's output) or, when apx is null, from c
 // scaled by cs, and every other quadrant from c (row stride ld) scaled by cs.  Output to out (row stride
 // lout), added to its values when ACC.  apx and c must not overlap out.
-template <typename R, bool ACC, int L>
-__global__ void __launch_bounds__(256) k_idwt2d(const cpx<R>* __restrict__ c, int ld, const cpx<R>* __restrict__ apx,
-                                                int la, cpx<R>* __restrict__ out, int lout, int h, int w, WavFilt<R> f,
-                                                R cs) {
+template <typename R>
+struct IdwtArgs {
+  const cpx<R>* c;
+  int ld;
+  const cpx<R>* apx;
+  int la;
+  cpx<R>* out;
+  int lout, h, w;
+  WavFilt<R> f;
+  R cs;
+};
+template <typename R, bool ACC, int L, bool BT = false>
+__global__ void __launch_bounds__(256) k_idwt2d(IdwtArgs<R> a0, const IdwtArgs<R>* __restrict__ tab) {
+  const IdwtArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const cpx<R>* __restrict__ c = q.c;
+  const cpx<R>* __restrict__ apx = q.apx;
+  cpx<R>* __restrict__ out = q.out;
+  const int ld = q.ld, la = q.la, lout = q.lout, h = q.h, w = q.w;
+  const WavFilt<R>& f = q.f;
+  const R cs = q.cs;
   constexpr int K = L / 2, CI = DW_T + K - 1;
   __shared__ cpx<R> in_t[2 * CI][2 * CI + 1];
   __shared__ cpx<R> cp_t[2 * DW_T][2 * CI + 1];
@@ -264,9 +297,20 @@ static void dispatch_len(int L, F&& fn) {
 }
 
 // out[k] = in[k] * scl (ACC: out[k] += in[k] * scl) over the whole image
-template <typename R, bool ACC>
-__global__ void __launch_bounds__(256) k_copy_scaled(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int64_t n,
-                                                     R scl) {
+template <typename R>
+struct CopyArgs {
+  const cpx<R>* in;
+  cpx<R>* out;
+  int64_t n;
+  R scl;
+};
+template <typename R, bool ACC, bool BT = false>
+__global__ void __launch_bounds__(256) k_copy_scaled(CopyArgs<R> a0, const CopyArgs<R>* __restrict__ tab) {
+  const CopyArgs<R> q = BT ? tab[blockIdx.z] : a0;
+  const cpx<R>* __restrict__ in = q.in;
+  cpx<R>* __restrict__ out = q.out;
+  const int64_t n = q.n;
+  const R scl = q.scl;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
     cpx<R> v = scale(in[k], scl);
     if (ACC) {
@@ -279,6 +323,19 @@ __global__ void __launch_bounds__(256) k_copy_scaled(const cpx<R>* __restrict__ 
 }
 
 static dim3 grid_for(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))); }
+
+// out = in * scl (acc: out += in * scl) over n values
+template <typename R>
+void copy_scaled(const cpx<R>* in, cpx<R>* out, int64_t n, R scl, bool acc, hipStream_t s) {
+  const CopyArgs<R> a{in, out, n, scl};
+  if (acc) {
+    if (record_launch<CopyArgs<R>, k_copy_scaled<R, true, true>>(grid_for(n), dim3(256), 0, true, a)) return;
+    hipLaunchKernelGGL((k_copy_scaled<R, true>), grid_for(n), dim3(256), 0, s, a, nullptr);
+  } else {
+    if (record_launch<CopyArgs<R>, k_copy_scaled<R, false, true>>(grid_for(n), dim3(256), 0, true, a)) return;
+    hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(n), dim3(256), 0, s, a, nullptr);
+  }
+}
 
 template <typename R>
 WavFilt<R> make_filt(const std::vector<double>& h) {
@@ -307,7 +364,9 @@ struct DwtLevel {
   template <int L>
   void operator()() {
     const unsigned bx = (unsigned)((w / 2 + DW_T - 1) / DW_T), by = (unsigned)((h / 2 + DW_T - 1) / DW_T);
-    hipLaunchKernelGGL((k_dwt2d<R, L>), dim3(bx, by), dim3(256), 0, s, src, ls, c, ld, apx, la, h, w, f, sd, sa);
+    const DwtArgs<R> a{src, ls, c, ld, apx, la, h, w, f, sd, sa};
+    if (record_launch<DwtArgs<R>, k_dwt2d<R, L, true>>(dim3(bx, by), dim3(256), 0, false, a)) return;
+    hipLaunchKernelGGL((k_dwt2d<R, L>), dim3(bx, by), dim3(256), 0, s, a, nullptr);
   }
 };
 template <typename R, bool ACC>
@@ -324,7 +383,9 @@ struct IdwtLevel {
   template <int L>
   void operator()() {
     const unsigned bx = (unsigned)((w / 2 + DW_T - 1) / DW_T), by = (unsigned)((h / 2 + DW_T - 1) / DW_T);
-    hipLaunchKernelGGL((k_idwt2d<R, ACC, L>), dim3(bx, by), dim3(256), 0, s, c, ld, apx, la, out, lout, h, w, f, cs);
+    const IdwtArgs<R> a{c, ld, apx, la, out, lout, h, w, f, cs};
+    if (record_launch<IdwtArgs<R>, k_idwt2d<R, ACC, L, true>>(dim3(bx, by), dim3(256), 0, false, a)) return;
+    hipLaunchKernelGGL((k_idwt2d<R, ACC, L>), dim3(bx, by), dim3(256), 0, s, a, nullptr);
   }
 };
 
@@ -356,8 +417,7 @@ void idwt2(const cpx<R>* c, cpx<R>* x, cpx<R>* W, cpx<R>* T, int ny, int nx, int
            bool acc, hipStream_t s) {
   const int64_t N = (int64_t)ny * nx;
   if (levels == 0) {
-    if (acc) hipLaunchKernelGGL((k_copy_scaled<R, true>), grid_for(N), dim3(256), 0, s, c, x, N, scl);
-    else hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, c, x, N, scl);
+    copy_scaled<R>(c, x, N, scl, acc, s);
     return;
   }
   // One launch per level.  Each level but the finest writes its reconstructed approximation, compact
@@ -415,12 +475,12 @@ void wavelet_analysis(pf_wavelet* wv, const cpx<R>* x, cpx<R>* c, hipStream_t s)
   for (int i = 0; i < wv->nwav; ++i) {
     cpx<R>* ci = c + (int64_t)i * N;
     if (wv->db[i] == 0) {
-      hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, x, ci, N, scl);
+      copy_scaled<R>(x, ci, N, scl, false, s);
       continue;
     }
     const WavFilt<R> f = make_filt<R>(wv->filters[i]);
     if (wv->levels[i] == 0) {
-      hipLaunchKernelGGL((k_copy_scaled<R, false>), grid_for(N), dim3(256), 0, s, x, ci, N, scl);
+      copy_scaled<R>(x, ci, N, scl, false, s);
       continue;
     }
     // the SARA factor is applied as the coefficients are written (a product by 1 for one basis)

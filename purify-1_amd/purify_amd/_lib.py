@@ -137,6 +137,10 @@ SIGNATURES = {
                                         C.c_double, C.c_double, C.POINTER(pf_csr)]),
     "pf_csr_free": (None, [C.POINTER(pf_csr)]),
     "pf_op_create_csr": (C.c_int, [C.POINTER(pf_op_desc), C.POINTER(pf_csr), C.POINTER(C.c_void_p)]),
+    "pf_padmm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    "pf_padmm_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "pf_padmm_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "pf_padmm_batch_destroy": (C.c_int, [C.c_void_p]),
     "pf_cache_stats": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "pf_cache_release": (C.c_int, [C.POINTER(C.c_int64)]),
     "pf_last_error": (C.c_char_p, []),

@@ -180,11 +180,18 @@ def padmm_factory(dist, measurements, wavelets, uv_data, sigma, imsizey, imsizex
 class MultiChannelPADMM:
     """Many independent channels (frequency channels or snapshots with their own uv-coverage, image size
     and visibilities) reconstructed together: every channel's operator, dictionary and solver stay
-    resident; every solver's iteration is captured once (in this thread, one after the other), then all
-    channels' iterations are queued at once, each on its solver's stream, so their kernels overlap on
-    the device.  channels: list of dicts with u, v (pixels), y, imsizey, imsizex, sigma and optional
-    J (4), levels (4), precision ("c64"), power_iters (50), or with "operator": a LinearTransform already
-    built (and normalised) for the channel, used in place of building one."""
+    resident.  Two ways to run them:
+
+      * run_batched: a channel batch (pf_padmm_batch) launches every kernel of the iteration once per
+        group of alike channels (same image size and configuration), each group on its own stream, the
+        whole lock-step iteration captured as one graph;
+      * run: every solver's iteration captured as its own graph, all channels' graphs queued at once on
+        the solvers' streams.
+
+    Both give each channel exactly its result alone.  channels: list of dicts with u, v (pixels), y,
+    imsizey, imsizex, sigma and optional J (4), levels (4), precision ("c64"), power_iters (50), or with
+    "operator": a LinearTransform already built (and normalised) for the channel, used in place of
+    building one."""
 
     def __init__(self, channels, itermax=100):
         from .operators import measurementoperator
@@ -204,6 +211,7 @@ class MultiChannelPADMM:
             eps = utilities.calculate_l2_radius(np.asarray(ch["y"]).size, float(ch["sigma"]))
             sol = ImagingProximalADMM(lt.op, W, ch["y"], eps, itermax=itermax)
             self.items.append((lt, W, sol))
+        self._batch = C.c_void_p()
 
     def capture(self):
         """Capture every channel's iteration (one graph each), in this thread: one warm-up iteration is
@@ -221,10 +229,30 @@ class MultiChannelPADMM:
             sol.enqueue(iters, use_graph)
         return [sol.sync() for _, _, sol in self.items]
 
+    def _make_batch(self):
+        if not self._batch:
+            hs = (C.c_void_p * len(self.items))(*[sol._h.value for _, _, sol in self.items])
+            _lib.check(_lib.lib().pf_padmm_batch_create(hs, len(self.items), C.byref(self._batch)))
+        return self._batch
+
+    def run_batched(self, iters, use_graph=True):
+        """iters lock-step iterations of every channel through the channel batch (see the class docstring)."""
+        _lib.check(_lib.lib().pf_padmm_batch_run(self._make_batch(), int(iters), int(bool(use_graph))))
+        return [sol.sync() for _, _, sol in self.items]
+
+    def batch_info(self):
+        """(groups, launches per batched iteration) of the channel batch."""
+        g, n = C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().pf_padmm_batch_info(self._make_batch(), C.byref(g), C.byref(n)))
+        return g.value, n.value
+
     def images(self):
         return [sol.image() for _, _, sol in self.items]
 
     def close(self):
+        if self._batch:
+            _lib.lib().pf_padmm_batch_destroy(self._batch)
+            self._batch = C.c_void_p()
         for lt, W, sol in self.items:
             sol.close()
             W.close()

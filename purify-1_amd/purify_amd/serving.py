@@ -243,24 +243,33 @@ def poisson_requests(seed: int, n_requests: int, rate_per_s: float, sizes=(256, 
                      padmm_fraction=0.0, padmm_iters=100):
     """The serving workload: Poisson arrivals; image side uniform over `sizes`; M log-uniform over
     M_range; a fraction of the jobs with w-terms (w uniform in [0, wmax], 1 degree field of view); a
-    fraction `padmm_fraction` of the jobs are `padmm_iters`-iteration reconstructions instead of `kind`."""
+    fraction `padmm_fraction` of the jobs are `padmm_iters`-iteration reconstructions instead of `kind`.
+    The shared generator draws every request's parameters in order; each request's coordinates, image and
+    visibilities come from generators seeded by its index, on a thread pool."""
     from . import utilities
     rng = np.random.default_rng(seed)
     t = 0.0
-    reqs = []
+    params = []
     for k in range(n_requests):
         t += rng.exponential(1.0 / rate_per_s)
         n = int(rng.choice(sizes))
         M = int(round(10 ** rng.uniform(np.log10(M_range[0]), np.log10(M_range[1]))))
+        w_term = bool(rng.uniform() < w_fraction)
+        w = rng.uniform(0.0, wmax, M) if w_term else None
+        kd = "padmm" if rng.uniform() < padmm_fraction else kind
+        params.append((k, t, n, M, w_term, w, kd))
+
+    def make(p):
+        k, t, n, M, w_term, w, kd = p
         u, v = utilities.synth_uv(seed * 1000 + k, M, np.pi / 3)
         u = utilities.radians_to_pixels(u, n)
         v = utilities.radians_to_pixels(v, n)
-        w_term = bool(rng.uniform() < w_fraction)
-        w = rng.uniform(0.0, wmax, M) if w_term else None
         x = utilities.synth_image(seed * 1000 + k, n, n, 20)
         y = utilities.synth_cnormal(seed * 1000 + k + 7, M)
-        kd = "padmm" if rng.uniform() < padmm_fraction else kind
-        reqs.append(ImagingRequest(u=u, v=v, imsizey=n, imsizex=n, kind=kd, x=x, y=y, w=w, J=4, w_term=w_term,
-                                   Jw=6, cell=3600.0 / n, precision=precision, arrival=t, rid=k, iters=padmm_iters,
-                                   sigma=0.1 * float(np.linalg.norm(y)) / np.sqrt(M)))
+        return ImagingRequest(u=u, v=v, imsizey=n, imsizex=n, kind=kd, x=x, y=y, w=w, J=4, w_term=w_term, Jw=6,
+                              cell=3600.0 / n, precision=precision, arrival=t, rid=k, iters=padmm_iters,
+                              sigma=0.1 * float(np.linalg.norm(y)) / np.sqrt(M))
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        reqs = list(ex.map(make, params))
     return reqs

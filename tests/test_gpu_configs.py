@@ -5,7 +5,8 @@
 - 2048^2 image, oversample 2, KB 6 x 6, M = 4e6 (the measured pair): both precisions and the dot test;
 - 64 channels (sides 256-1024, M log-uniform in [1e4, 1e6], uv sigma U[0.5, 1.2], KB 4 x 4, complex64, 100
   PADMM iterations each, captured as graphs and run together): every channel in the batch equals the
-  channel run alone, bit for bit, and the channel with the fewest visibilities, the one with the most and
+  channel run alone, bit for bit, and so does every channel of the channel batch (one launch per kernel of
+  the iteration for each image size); the channel with the fewest visibilities, the one with the most and
   the sparsest one (visibilities per pixel) reconstruct within 0.05 dB SNR of the oracle's FP64 PADMM;
 - 20 requests of the serving mix (sides 256-2048, M log-uniform in [1e4, 4e6], 30 % with w-terms, w in
   [0, 200], 1 degree field): each result is the same alone and in the mix, and each request's Phi and
@@ -113,6 +114,20 @@ def test_64_channels_batch_equals_each_channel_alone(channels):
         sol.reset()
         sol.iterate(ITERS, use_graph=True)  # alone: nothing else queued
         assert np.array_equal(sol.image(), batch[k]), k
+
+
+def test_64_channels_channel_batch_equals_the_stream_batch(channels):
+    # one launch per kernel of the iteration for each image size: every channel's image bit for bit as
+    # the stream batch's (which equals each channel alone, above)
+    chans, x0s, mc, batch = channels
+    mc.reset()
+    res = mc.run_batched(ITERS)
+    assert all(r.niters == ITERS for r in res)
+    groups, launches = mc.batch_info()
+    print(f"channel batch: {groups} groups, {launches} launches per iteration", flush=True)
+    assert groups == 3
+    for k, img in enumerate(mc.images()):
+        assert np.array_equal(img, batch[k]), k
 
 
 def oracle_padmm_snr(ch, x0, gpu_norm):

@@ -4,6 +4,8 @@
 reference's factory parameters (algorithm_factory.h:57-81), batched channels, and the reference's own
 PADMM fixture end to end (cpp/tests/algo_factory.cc).  SOPT is not in the reference tree: its iterates
 are not available one by one, and the fixture holds the measured distance from its solution."""
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -146,6 +148,59 @@ def test_multichannel_batch_equals_single_channel_runs():
         assert np.array_equal(sol.image(), x)
         one.close()
     mc.close()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_channel_batch_equals_each_channel_alone(use_graph):
+    """The channel batch (one launch per kernel of the iteration for each group of alike channels) gives
+    every channel's image, iteration statistics and count bit for bit as the channel solved alone: two
+    image sizes, both precisions (four groups), the batch run in two calls (recorded once)."""
+    specs = [(631, 64, 3000, "c64"), (632, 128, 5000, "c64"), (633, 64, 2000, "c64"), (634, 128, 800, "c64"),
+             (635, 64, 2500, "c128"), (636, 64, 4000, "c128")]
+    chans = []
+    for seed, n, M, prec in specs:
+        u, v, x0, y, eps, orc = problem(seed, n, M)
+        sigma = utilities.SNR_to_standard_deviation(orc.forward(x0), 30.0)
+        chans.append(dict(u=u, v=v, y=y, imsizey=n, imsizex=n, sigma=sigma, power_iters=30, precision=prec))
+    K = 12
+    mc = algorithms.MultiChannelPADMM(chans, itermax=K)
+    groups, launches = mc.batch_info()
+    assert groups == 3  # (64, c64), (128, c64), (64, c128)
+    assert launches < 3 * 40
+    mc.run_batched(5, use_graph)
+    res = mc.run_batched(K - 5, use_graph)
+    assert [r.niters for r in res] == [K] * len(chans)
+    got = mc.images()
+    stats = [sol.stats() for _, _, sol in mc.items]
+    for ch, x, st in zip(chans, got, stats):
+        one = algorithms.MultiChannelPADMM([ch], itermax=K)
+        _, _, sol = one.items[0]
+        r = sol.iterate(K, use_graph=True)
+        assert r.niters == K
+        assert np.array_equal(sol.image(), x)
+        assert np.array_equal(sol.stats(), st)
+        one.close()
+    # a reset channel batch runs again from the initial state
+    mc.reset()
+    mc.run_batched(K, use_graph)
+    assert all(np.array_equal(a, b) for a, b in zip(mc.images(), got))
+    mc.close()
+
+
+def test_channel_batch_rejects_the_iterative_proximal():
+    u, v, x0, y, eps, orc = problem(641, 64, 2000)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, 64, 64, 2.0, 20, 1e-4, "kb", 4, 4,
+                                                     precision="c64")
+    W = wavelets.WaveletOperator(SARA, 64, 64, "c64")
+    sol = algorithms.ImagingProximalADMM(lt.op, W, y, eps, itermax=4)
+    hs = (C.c_void_p * 1)(sol._h.value)
+    b = C.c_void_p()
+    from purify_amd import _lib
+    rc = _lib.lib().pf_padmm_batch_create(hs, 1, C.byref(b))
+    assert rc == _lib.PF_ERR_UNSUPPORTED and "tight-frame" in _lib.lib().pf_last_error().decode()
+    sol.close()
+    W.close()
+    lt.op.close()
 
 
 SARA = [("Dirac", 1), ("DB1", 2), ("DB4", 2)]
