@@ -258,9 +258,12 @@ def test_sara_padmm_reproduces_the_reference_padmm_fixture(precision):
     """cpp/tests/algo_factory.cc:20-67 end to end: input_data.vis (radians), a 256^2 operator (os 2, KB 4x4,
     100 power iterations), the nine-basis SARA dictionary at 3 levels, sigma 0.02378738741225, 500
     iterations at most.  The reference checks SOPT's own run against its solution at 1e-6 and 139
-    iterations; this PADMM restates the published algorithms (SOPT is not in the reference tree), so it is
-    held to its measured distance from SOPT's solution: 0.44 % relative L2 (47 dB), the residual image to
-    0.53 %, 143 iterations."""
+    iterations.  SOPT is not in the reference tree (a floating branch), so this PADMM restates the published
+    algorithm; the fixture discriminates its choices (x from Phi^dagger y, the l1 proximal's dual restarted
+    at zero on every call, the extremal-phase filters at even phase: each alternative lands 1.5-6 % away)
+    and holds what the restatement reaches: the trajectory passes closest to SOPT's solution at SOPT's own
+    iteration count, 139 (0.144 % relative L2); stopped by the relative-variation and residual test it ends
+    at iteration 143, 0.44 % away, its residual image 0.53 % from SOPT's."""
     import os
     from purify_amd import measurement_operator_factory, pfitsio
     D = os.path.join(os.path.dirname(__file__), "golden", "reference_data", "padmm")
@@ -274,11 +277,18 @@ def test_sara_padmm_reproduces_the_reference_padmm_fixture(precision):
     lt = measurement_operator_factory("serial", uv, 256, 256, 1, 1, 2, 100, 1e-4, "kb", 4, 4, precision=precision)
     W = wavelets.wavelet_operator_factory("serial", sara, 256, 256, precision=precision)
     p = algorithms.padmm_factory("serial", lt, W, uv, 0.02378738741225, 256, 256, len(sara), 500)
+    # the trajectory at SOPT's iteration count
+    p.iterate(139, use_graph=False)
+    at139 = rel(p.image(), sol)
+    print(f"{precision}: iterate 139 is {100 * at139:.3f} % from SOPT's solution", flush=True)
+    assert at139 < 0.002
+    # SOPT's operator(): stopped by the convergence test
+    p.reset()
     d = p.run(use_graph=False, check_every=1)
-    assert d.good and abs(d.niters - 139) <= 8
-    assert rel(d.x, sol) < 0.01
+    assert d.good and 139 <= d.niters <= 144
+    assert rel(d.x, sol) < 0.005
     resid = lt.adjoint() * (uv.vis - lt * d.x)
-    assert np.linalg.norm(resid.real - res.real) / np.linalg.norm(res.real) < 0.01
+    assert np.linalg.norm(resid.real - res.real) / np.linalg.norm(res.real) < 0.006
 
 
 def fb_solver(u, v, y, n, precision, norm, iters, sigma, dictionary=DB4, **kw):
