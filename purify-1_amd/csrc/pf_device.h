@@ -36,14 +36,20 @@ template <typename R>
 __device__ inline cpx<R> operator-(cpx<R> a, cpx<R> b) {
   return mk(a.x - b.x, a.y - b.y);
 }
+// The complex products state their fused multiply-adds explicitly: left to the compiler, a * b + c * d
+// may be contracted as fma(a, b, c * d) at one inlined call site and fma(c, d, a * b) at another, and
+// a kernel whose loop body is inlined twice would then round the same entry differently depending on
+// which copy handled it.
+__device__ inline float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <typename R>
 __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b) {
-  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return mk(fmar(a.x, b.x, -(a.y * b.y)), fmar(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
 template <typename R>
 __device__ inline cpx<R> cmulc(cpx<R> a, cpx<R> b) {
-  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return mk(fmar(a.x, b.x, a.y * b.y), fmar(a.y, b.x, -(a.x * b.y)));
 }
 template <typename R>
 __device__ inline cpx<R> scale(cpx<R> a, R s) {
