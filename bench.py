@@ -393,10 +393,13 @@ def main():
     outs = {}
     for prec, steps in (("c64", args.steps), ("c128", args.fp64_steps)):
         t0 = time.perf_counter()
+        # the product's own tuning: tile shape and permutation bucket by timed builds (tile="auto"), then
+        # the gridding kernel and column FFT mode (autotune)
         lt = measurementoperator.init_degrid_operator_2d(u, v, w, None, n, n, os_, 0, 1e-4, "kb", J, J, w_term, Jw,
-                                                         cell, cell, precision=prec, **chirp_kw)
+                                                         cell, cell, precision=prec, tile="auto", **chirp_kw)
         setup = time.perf_counter() - t0
         op = lt.op
+        tiles = {"tile": [op.get_tuning("tile_u"), op.get_tuning("tile_v")], "perm_bucket": op.get_tuning("perm_bucket")}
         KK_op = op.get_tuning("w_footprint") ** 2 if w_term else None
         t0 = time.perf_counter()
         variant = op.autotune(3)
@@ -440,7 +443,8 @@ def main():
             host_ms.append(1e3 * (time.perf_counter() - t0))
         res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
                          value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s,
-                         host_ms=statistics.median(host_ms), KK=KK_op, nnz_w=op.info().nnz_w)
+                         host_ms=statistics.median(host_ms), KK=KK_op, nnz_w=op.info().nnz_w,
+                         fft_columns=op.get_tuning("fft_columns"), **tiles)
         log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s, "
             f"grid kernel {variant}")
         del lt, op, x, yin, y, xo
@@ -577,7 +581,11 @@ def main():
         "host_pair_ms": {"c64": r64["host_ms"], "c128": res["c128"]["host_ms"],
                          "note": "pair on host complex128 buffers: PCIe copies and conversions included"},
         "autotune": {"grid_kernel": {"c64": r64["grid_kernel"], "c128": res["c128"]["grid_kernel"]},
-                     "seconds": {"c64": r64["tune_s"], "c128": res["c128"]["tune_s"]}},
+                     "fft_columns": {"c64": r64["fft_columns"], "c128": res["c128"]["fft_columns"]},
+                     "tile": {"c64": r64["tile"], "c128": res["c128"]["tile"]},
+                     "perm_bucket": {"c64": r64["perm_bucket"], "c128": res["c128"]["perm_bucket"]},
+                     "seconds": {"c64": r64["tune_s"], "c128": res["c128"]["tune_s"]},
+                     "setup_seconds_with_tile_search": {"c64": r64["setup_s"], "c128": res["c128"]["setup_s"]}},
         "kernels_ms_per_launch": {k: tab[k][0] / tab[k][1] for k in tab},
     }
     print(json.dumps(out), flush=True)

@@ -70,7 +70,10 @@ typedef struct pf_op_desc {
   const double *weights;   /* host complex128, length M; NULL = unit weights */
   int32_t power_iters;     /* power-method iterations for the normalisation (operators.h:390) */
   double power_tol;
-  int32_t tile_u, tile_v; /* gridding tile shape; 0 = library default */
+  int32_t tile_u, tile_v; /* gridding tile shape; 0 = library default; either < 0 (pf_op_create only) = chosen by
+                           * timing Phi + Phi^dagger pairs on one build per candidate shape (32 x 32, 64 x 32,
+                           * 32 x 64, 64 x 64) and, complex64, per permutation bucket (read back with
+                           * pf_op_get_tuning "tile_u", "tile_v", "perm_bucket") */
   /* w_term = 2: the energy fractions the chirp and the convolved row keep (energy_fraction_chirp,
    * energy_fraction_wproj of wprojection_matrix); energy_chirp must be below 1 */
   double energy_chirp, energy_wproj;
@@ -304,7 +307,8 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * lines with the next line prefetched; 0 = one line per block; default 3 for complex64, 2 for
  * complex128); "grid_waves" (matrix-core gridding with that many waves, each taking every
  * grid_waves-th 16 x 16 tile with its batches pipelined across tiles; 0 = one wave per tile, the
- * default); read-only "tile_u", "tile_v", "w_footprint" (side of the stored rows' footprint with
+ * default); read-only "tile_u", "tile_v", "perm_bucket" (complex64: the caller-order
+ * permutation's bucket of visibilities, else 0), "w_footprint" (side of the stored rows' footprint with
  * w-projection rows, else 0) and "split_tiles" (tiles whose stored-row gridding list is cut into parts
  * that are summed in a fixed order afterwards, so that a dense tile does not form the launch's tail).
  * Results do not depend on grid_kernel or fft_columns beyond floating-point summation order. */

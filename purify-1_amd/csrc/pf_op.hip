@@ -347,6 +347,7 @@ struct pf_op {
   cpx<double>* d_uv = nullptr;
   void* d_weights = nullptr;
   VisPerm vperm;                // caller's order <-> sorted order (arrays below)
+  int perm_S_req = 0;           // complex64 permutation bucket requested at build (0: the largest that fits)
   uint16_t* d_perm_l1 = nullptr;
   int32_t* d_perm_g1 = nullptr;
   uint16_t* d_perm_p2 = nullptr;
@@ -515,7 +516,7 @@ void build_perm_dev(pf_op* op, int32_t* d_perm) {
     p.inv = op->d_inv;
     return;
   }
-  const int Smax = perm_max_S((int)op->esz());
+  const int Smax = op->perm_S_req > 0 ? std::min(op->perm_S_req, perm_max_S((int)op->esz())) : perm_max_S((int)op->esz());
   p.S = 64;
   while (p.S < M && p.S < Smax) p.S <<= 1;
   p.nb = (int)((M + p.S - 1) / p.S);
@@ -590,6 +591,15 @@ void build_wrows(pf_op* op, const pf_op_desc& d, const cpx<double>* d_uv, const 
 }
 
 void build_tail(pf_op* op, const pf_op_desc& d);
+
+// Tile side for a requested one (req > 0), else the default: 32, halved while the grid is narrower
+// than a tile and its halo.
+static int pick_tile(int req, int N) {
+  if (req > 0) return std::min(req, N);
+  int T = 32;
+  while (T > 8 && N < T + 8) T /= 2;
+  return std::min(T, N);
+}
 
 void build(pf_op* op, const pf_op_desc& d) {
   if (d.imsizex <= 0 || d.imsizey <= 0) throw Error(PF_ERR_ARG, "image size must be positive");
@@ -692,12 +702,6 @@ void build(pf_op* op, const pf_op_desc& d) {
   }
   // default tile: 32 x 32, halved on small grids so that N >= T + 8 (the residue-class gridding
   // kernel's condition) whenever the grid allows it
-  auto pick_tile = [](int req, int N) {
-    if (req > 0) return std::min(req, N);
-    int T = 32;
-    while (T > 8 && N < T + 8) T /= 2;
-    return std::min(T, N);
-  };
   const int TU = pick_tile(d.tile_u, op->Nu);
   const int TV = pick_tile(d.tile_v, op->Nv);
   if (!is_pow2(TU) && TU != op->Nu) throw Error(PF_ERR_ARG, "tile_u must be a power of two");
@@ -1258,14 +1262,109 @@ static double fp64_norm(const pf_op_desc& desc, Build&& build_fn) {
 }
 }
 
+// Seconds per Phi + Phi^dagger pair of a built operator on device vectors (reps pairs after one, HIP
+// events on the thread's construction stream).
+static double time_pairs(pf_op* op, int reps) {
+  hipStream_t s = cstream();
+  DevTemp tmp;
+  const int64_t n = (int64_t)op->nx * op->ny;
+  struct Events {  // released however the timing ends (a candidate's apply may throw)
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  PF_HIP(hipEventCreate(&ev.a));
+  PF_HIP(hipEventCreate(&ev.b));
+  hipEvent_t a = ev.a, b = ev.b;
+  auto run = [&](auto* x, auto* y) {
+    using R = std::remove_pointer_t<decltype(x)>;
+    using F = decltype(R{}.x);
+    PF_HIP(hipMemsetAsync(x, 0, sizeof(R) * (size_t)std::max<int64_t>(n, 1), s));
+    PF_HIP(hipMemsetAsync(y, 0, sizeof(R) * (size_t)std::max<int64_t>(op->M, 1), s));
+    forward_dev<F>(op, x, y, s);
+    adjoint_dev<F>(op, y, x, s);
+    PF_HIP(hipEventRecord(a, s));
+    for (int k = 0; k < reps; ++k) {
+      forward_dev<F>(op, x, y, s);
+      adjoint_dev<F>(op, y, x, s);
+    }
+    PF_HIP(hipEventRecord(b, s));
+  };
+  if (op->prec == PF_C64) run(tmp.alloc<cpx<float>>(n), tmp.alloc<cpx<float>>(op->M));
+  else run(tmp.alloc<cpx<double>>(n), tmp.alloc<cpx<double>>(op->M));
+  PF_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  PF_HIP(hipEventElapsedTime(&ms, a, b));
+  return ms * 1e-3 / std::max(reps, 1);
+}
+
+// tile_u or tile_v < 0: the tile shape (of the degridding and stored-row gridding tiles, which set the
+// visibilities' sort order, so each candidate is a build of its own) is chosen by timing Phi + Phi^dagger
+// pairs on a build per candidate: 32 x 32 (the default), 64 x 32, 32 x 64 and 64 x 64 where the grid holds
+// them; then, for complex64, the caller-order permutation's bucket (the largest that fits, against half
+// of it) on the winning shape.  d gets the winning tile; the winner is returned.
+static std::unique_ptr<pf_op> build_tuned(pf_op_desc& d, int reps) {
+  const int Nu = (int)std::floor(d.imsizex * d.oversample_ratio), Nv = (int)std::floor(d.imsizey * d.oversample_ratio);
+  std::vector<std::pair<int, int>> cand;
+  auto add = [&](int tu, int tv) {
+    tu = std::min(tu, Nu);
+    tv = std::min(tv, Nv);
+    for (const auto& c : cand)
+      if (c.first == tu && c.second == tv) return;
+    cand.emplace_back(tu, tv);
+  };
+  add(pick_tile(0, Nu), pick_tile(0, Nv));  // the default
+  for (const auto& c : {std::make_pair(32, 32), std::make_pair(64, 32), std::make_pair(32, 64), std::make_pair(64, 64)})
+    if (Nu >= c.first + 8 && Nv >= c.second + 8) add(c.first, c.second);
+  std::unique_ptr<pf_op> best;
+  double best_t = 1e30;
+  pf_op_desc bd = d;
+  for (const auto& c : cand) {
+    pf_op_desc dc = d;
+    dc.tile_u = c.first;
+    dc.tile_v = c.second;
+    dc.power_iters = 0;
+    std::unique_ptr<pf_op> op(new pf_op());
+    double t = 0;
+    try {
+      build(op.get(), dc);
+      t = time_pairs(op.get(), reps);
+    } catch (const Error& e) {
+      // a shape this operator cannot use (a stored-row tile and halo beyond the kernel's LDS, say) is
+      // skipped; the default shape comes first, so its failure is the operator's own and propagates
+      if (e.code != PF_ERR_UNSUPPORTED || !best) throw;
+      continue;
+    }
+    if (t < best_t) {
+      best_t = t;
+      best = std::move(op);
+      bd = dc;
+    }
+  }
+  if (best->prec == PF_C64 && best->vperm.two_pass && best->vperm.S > 64) {
+    pf_op_desc dc = bd;
+    std::unique_ptr<pf_op> op(new pf_op());
+    op->perm_S_req = best->vperm.S / 2;
+    build(op.get(), dc);
+    if (time_pairs(op.get(), reps) < best_t) best = std::move(op);
+  }
+  d.tile_u = bd.tile_u;
+  d.tile_v = bd.tile_v;
+  return best;
+}
+
 int pf_op_create(const pf_op_desc* desc, pf_op** out) {
   if (!desc || !out) return fail(PF_ERR_ARG, "null argument");
   *out = nullptr;
   std::unique_ptr<pf_op> op(new pf_op());
   int rc = guarded([&] {
-    build(op.get(), *desc);
+    pf_op_desc dd = *desc;
+    if (dd.tile_u < 0 || dd.tile_v < 0) op = build_tuned(dd, 3);
+    else build(op.get(), dd);
     if (desc->power_iters > 0) {
-      const double nrm = op->prec == PF_C64 ? fp64_norm(*desc, [](pf_op* t, const pf_op_desc& d) { build(t, d); })
+      const double nrm = op->prec == PF_C64 ? fp64_norm(dd, [](pf_op* t, const pf_op_desc& d) { build(t, d); })
                                             : power_method_t<double>(op.get(), desc->power_iters, desc->power_tol, nullptr);
       if (!(nrm > 0)) throw Error(PF_ERR_ARG, "Operator norm is not greater than zero.");  // operators.h:187
       op->norm = nrm;
@@ -1739,6 +1838,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   }
   if (k == "split_tiles") {
     *value = op->tl.n_comb;
+    return PF_OK;
+  }
+  if (k == "perm_bucket") {
+    *value = op->vperm.two_pass ? op->vperm.S : 0;
     return PF_OK;
   }
   if (k == "tile_v") {

@@ -344,13 +344,19 @@ class measurementoperator:  # noqa: N801 -- mirrors the reference's namespace
         init_degrid_operator_2d(uv_vis, imsizey, imsizex, cell_x, cell_y, oversample_ratio=2, power_iters=100,
                                 power_tol=1e-4, kernel=kb, Ju=4, Jv=4, w_term=False, Jw=6)
             -- vis_params in lambda / radians / pixels (operators.h:398-414)
-        Extra keywords: precision ("c64" default, or "c128"), tile=(tile_u, tile_v); with w_term,
+        Extra keywords: precision ("c64" default, or "c128"), tile=(tile_u, tile_v) or tile="auto" (the shape,
+        and for complex64 the permutation bucket, chosen by timing Phi + Phi^dagger pairs on one build per
+        candidate; read back with op.get_tuning("tile_u" / "tile_v" / "perm_bucket")); with w_term,
         w_projection="approx" (the projection kernels of operators.cc:49-99, the default) or "chirp" (the rows of
         wproj_utilities::wprojection_matrix applied to the plain gridding matrix, wproj_utilities.cc:132-188,
         built on the device; square grids) with energy_chirp and energy_wproj (0.999 each by default).
         """
         precision = kw.pop("precision", "c64")
         tile = kw.pop("tile", (0, 0))
+        if isinstance(tile, str):
+            if tile != "auto":
+                raise ValueError('tile must be (tile_u, tile_v) or "auto"')
+            tile = (-1, -1)
         wproj = {k: kw.pop(k) for k in ("w_projection", "energy_chirp", "energy_wproj") if k in kw}
         if args and isinstance(args[0], vis_params):
             return _init_from_vis(*args, precision=precision, tile=tile, wproj=wproj, **kw)

@@ -381,3 +381,35 @@ def test_tap_polynomials_only_where_they_apply():
         lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, 32, 32, 2.0, 0, 1e-4, kern, Ju, Jv,
                                                          precision="c64")
         assert lt.op.get_tuning("kernel_poly") == 0
+
+
+@pytest.mark.parametrize("precision", ["c64", "c128"])
+@pytest.mark.parametrize("w_term", [False, True])
+def test_tile_shape_chosen_by_timing_equals_the_explicit_build(precision, w_term):
+    # tile="auto": one build per candidate tile shape (and complex64 permutation bucket), the fastest
+    # Phi + Phi^dagger pair kept.  The tile shape sets the sort order and the bucket only moves data, so
+    # the tuned operator equals the one built with the chosen shape, bit for bit, normalisation included.
+    n, M = 256, 100_000
+    u, v = make_uv(121, M, n)
+    w = np.random.default_rng(122).uniform(0, 150, M) if w_term else None
+    wt = utilities.synth_cnormal(123, M)
+    kw = dict(precision=precision, w_term=w_term, Jw=4, cellx=3600.0 / n, celly=3600.0 / n)
+    args = (u, v, w, wt, n, n, 2.0, 20, 1e-4, "kb", 6, 6)
+    lta = measurementoperator.init_degrid_operator_2d(*args, tile="auto", **kw)
+    tu, tv = lta.op.get_tuning("tile_u"), lta.op.get_tuning("tile_v")
+    assert (tu, tv) in ((32, 32), (64, 32), (32, 64), (64, 64))
+    bucket = lta.op.get_tuning("perm_bucket")
+    if precision == "c64":
+        assert bucket in (8192, 16384)
+    else:
+        assert bucket == 0
+    lte = measurementoperator.init_degrid_operator_2d(*args, tile=(tu, tv), **kw)
+    x = utilities.synth_image(124, n, n, 20)
+    y = utilities.synth_cnormal(125, M)
+    assert np.array_equal(lta * x, lte * x)
+    assert np.array_equal(lta.adjoint() * y, lte.adjoint() * y)
+    orc = oracle.Operator(u, v, w, wt, n, n, 2.0, oracle.KB, 6, 6, w_term, 4, 3600.0 / n, 3600.0 / n)
+    nrm = lta.op.norm
+    assert rel(lta * x, orc.forward(x) / nrm) < TOL[precision] * (10 if w_term else 1)
+    with pytest.raises(ValueError):
+        measurementoperator.init_degrid_operator_2d(*args, tile="fast", **kw)
