@@ -219,10 +219,25 @@ def multichannel_run(n_channels, iters, seed, log):
     from purify_amd import algorithms, measurementoperator, utilities
     chans, x0s = [], []
     t0 = time.perf_counter()
-    for ch in utilities.synth_channels(seed, n_channels):
+    specs = list(utilities.synth_channels(seed, n_channels))
+    # the tile shape per class of alike channels (here: image side), chosen by the timed builds of
+    # tile="auto" on the class's channel with the most visibilities (as MultiChannelPADMM(tile="auto"))
+    rep = {}
+    for i, ch in enumerate(specs):
+        if ch["n"] not in rep or ch["u"].size > specs[rep[ch["n"]]]["u"].size:
+            rep[ch["n"]] = i
+    tuned, tile_choice = {}, {}
+    for n, i in rep.items():
+        ch = specs[i]
+        tuned[i] = measurementoperator.init_degrid_operator_2d(ch["u"], ch["v"], None, None, n, n, 2.0, 50, 1e-6,
+                                                               "kb", 4, 4, precision="c64", tile="auto")
+        tile_choice[n] = (tuned[i].op.get_tuning("tile_u"), tuned[i].op.get_tuning("tile_v"))
+    for i, ch in enumerate(specs):
         n, u, v, x0 = ch["n"], ch["u"], ch["v"], ch["x0"]
-        lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 50, 1e-6, "kb", 4, 4,
-                                                         precision="c64")
+        lt = tuned.get(i)
+        if lt is None:
+            lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 50, 1e-6, "kb", 4, 4,
+                                                             precision="c64", tile=tile_choice[n])
         y0 = lt * x0
         sigma = utilities.SNR_to_standard_deviation(y0, 30.0)
         chans.append(dict(u=u, v=v, y=utilities.add_noise(y0, sigma, ch["seed"] + 2), imsizey=n, imsizex=n,
@@ -266,7 +281,8 @@ def multichannel_run(n_channels, iters, seed, log):
                              "vis_it_per_s": M_total * iters / t_cb, "speedup_vs_sequential": t_seq / t_cb,
                              "speedup_vs_stream_batch": t_bat / t_cb, "groups": groups,
                              "launches_per_iteration": launches, "equals_stream_batch": cb_same},
-           "snr_db": {"min": min(snr), "median": float(np.median(snr)), "max": max(snr)}}
+           "snr_db": {"min": min(snr), "median": float(np.median(snr)), "max": max(snr)},
+           "tile_per_image_side": {str(k): list(t) for k, t in tile_choice.items()}}
     log(f"[bench] multichannel {n_channels} x {iters} it: channel batch {t_cb:.2f} s ({groups} groups, {launches} "
         f"launches / iteration, {n_channels * iters / t_cb:.0f} channel-it/s), stream batch {t_bat:.2f} s "
         f"({out['batched_channel_it_per_s']:.0f} channel-it/s), sequential {t_seq:.2f} s, setup {setup:.1f} s, "

@@ -191,19 +191,46 @@ class MultiChannelPADMM:
     Both give each channel exactly its result alone.  channels: list of dicts with u, v (pixels), y,
     imsizey, imsizex, sigma and optional J (4), levels (4), precision ("c64"), power_iters (50), or with
     "operator": a LinearTransform already built (and normalised) for the channel, used in place of
-    building one."""
+    building one.  tile: the operators' tile shape, (tile_u, tile_v), or "auto": per class of alike
+    channels (image size, support, precision; a class is what the channel batch launches together) the
+    shape is chosen by the timed builds of init_degrid_operator_2d(tile="auto") on the class's channel
+    with the most visibilities, and every channel of the class is built with it (tile_choice holds the
+    choices)."""
 
-    def __init__(self, channels, itermax=100):
+    def __init__(self, channels, itermax=100, tile=(0, 0)):
         from .operators import measurementoperator
+
+        def key(ch):
+            return (int(ch["imsizey"]), int(ch["imsizex"]), int(ch.get("J", 4)), ch.get("precision", "c64"))
+
+        def build(ch, t):
+            ny, nx, J, prec = key(ch)
+            return measurementoperator.init_degrid_operator_2d(ch["u"], ch["v"], None, ch.get("weights"), ny, nx, 2.0,
+                                                               int(ch.get("power_iters", 50)), 1e-4, "kb", J, J,
+                                                               precision=prec, tile=t)
+
+        self.tile_choice = {}
+        tuned = {}
+        if isinstance(tile, str):
+            if tile != "auto":
+                raise ValueError('tile must be (tile_u, tile_v) or "auto"')
+            rep = {}
+            for i, ch in enumerate(channels):
+                if ch.get("operator") is None:
+                    k = key(ch)
+                    if k not in rep or len(ch["u"]) > len(channels[rep[k]]["u"]):
+                        rep[k] = i
+            for k, i in rep.items():
+                tuned[i] = build(channels[i], "auto")
+                self.tile_choice[k] = (tuned[i].op.get_tuning("tile_u"), tuned[i].op.get_tuning("tile_v"))
         self.items = []
-        for ch in channels:
-            ny, nx = int(ch["imsizey"]), int(ch["imsizex"])
-            J, prec = int(ch.get("J", 4)), ch.get("precision", "c64")
+        for i, ch in enumerate(channels):
+            ny, nx, J, prec = key(ch)
             lt = ch.get("operator")
             if lt is None:
-                lt = measurementoperator.init_degrid_operator_2d(ch["u"], ch["v"], None, ch.get("weights"), ny, nx,
-                                                                 2.0, int(ch.get("power_iters", 50)), 1e-4, "kb", J, J,
-                                                                 precision=prec)
+                lt = tuned.get(i)
+            if lt is None:
+                lt = build(ch, self.tile_choice.get(key(ch), tile if not isinstance(tile, str) else (0, 0)))
             lv = int(ch.get("levels", 4))
             while lv > 0 and (ny % (1 << lv) or nx % (1 << lv)):
                 lv -= 1

@@ -342,3 +342,32 @@ def test_forward_backward_errors():
         fb_solver(u, v, y, n, "c128", orc.norm, 5, 0.0)
     with pytest.raises(PurifyError, match="tight frame"):
         fb_solver(u, v, y, n, "c128", orc.norm, 5, 1.0, SARA, tight_frame=True)
+
+
+def test_channel_batch_with_tiles_chosen_per_class_of_alike_channels():
+    # tile="auto": one timed tile search per class (image size, support, precision) on its largest
+    # channel, the class built with the winner; the batch still equals every channel alone
+    specs = [(651, 64, 3000), (652, 128, 6000), (653, 64, 2000), (654, 128, 900)]
+    chans = []
+    for seed, n, M in specs:
+        u, v, x0, y, eps, orc = problem(seed, n, M)
+        sigma = utilities.SNR_to_standard_deviation(orc.forward(x0), 30.0)
+        chans.append(dict(u=u, v=v, y=y, imsizey=n, imsizex=n, sigma=sigma, power_iters=30, precision="c64"))
+    K = 8
+    mc = algorithms.MultiChannelPADMM(chans, itermax=K, tile="auto")
+    assert set(mc.tile_choice) == {(64, 64, 4, "c64"), (128, 128, 4, "c64")}
+    for (ny, nx, _, _), t in mc.tile_choice.items():
+        assert t[0] <= 2 * nx and t[1] <= 2 * ny
+    for (lt, _, _), ch in zip(mc.items, chans):
+        t = mc.tile_choice[(ch["imsizey"], ch["imsizex"], 4, "c64")]
+        assert (lt.op.get_tuning("tile_u"), lt.op.get_tuning("tile_v")) == t
+    mc.run_batched(K)
+    got = mc.images()
+    for (lt, _, _), ch, x in zip(mc.items, chans, got):
+        t = (lt.op.get_tuning("tile_u"), lt.op.get_tuning("tile_v"))
+        one = algorithms.MultiChannelPADMM([ch], itermax=K, tile=t)
+        _, _, sol = one.items[0]
+        sol.iterate(K, use_graph=True)
+        assert np.array_equal(sol.image(), x)
+        one.close()
+    mc.close()
