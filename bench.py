@@ -107,7 +107,8 @@ def serving_run(base, log, precision="c64"):
            "capacity_req_per_s": bn / bwall, "burst_p50_ms": 1e3 * bst["p50_s"], "burst_p99_ms": 1e3 * bst["p99_s"],
            "peak_inflight": peak, "hbm_budget_gib": 96, "w_term_jobs": sum(r.w_term for r in reqs),
            "padmm_jobs": sum(r.kind == "padmm" for r in reqs), "precision": precision, "arrival_span_s": span,
-           "sample": f"{n} requests of the serving mix (the full stream is 2000 at 20 / s)"}
+           "sample": (f"the full stream: {n} requests at 20 / s" if n >= 2000 else
+                      f"{n} requests of the serving mix (the full stream is 2000 at 20 / s)")}
     log(f"[bench] serving {precision}: capacity {out['capacity_req_per_s']:.1f} req/s (all {bn} requests at once)")
     return out
 
