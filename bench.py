@@ -43,20 +43,26 @@ WPROJ = {"wproj_2048_m4e6": dict(Jw=6, wmax=200.0, fov_deg=1.0),
 METRIC = "Φ+Φ† vis/s FP32 vs FP64 (rel.err≤1e-5, frac HBM peak); served p50/p99 latency"
 
 
-def model_bytes_pair(n, Ng, M, e, r, KK=None):
-    """Stage-boundary compulsory traffic of one Phi + Phi^dagger pair.  With stored projection rows of KK
-    taps (w_term) each direction reads per visibility its row origin (8 B), its value and its row."""
-    if KK:
-        return 3 * e * n * n + 11 * e * Ng + 2 * (8 + e + KK * e) * M
+def row_bytes(KK, nnz, M, e, packed):
+    """Stored bytes per projection row: KK taps dense, or a 16-byte record (keep mask, offset) and the kept
+    entries."""
+    return 16 + e * nnz / M if packed else KK * e
+
+
+def model_bytes_pair(n, Ng, M, e, r, RB=None):
+    """Stage-boundary compulsory traffic of one Phi + Phi^dagger pair.  With stored projection rows of RB
+    bytes (w_term) each direction reads per visibility its row origin (8 B), its value and its row."""
+    if RB:
+        return 3 * e * n * n + 11 * e * Ng + 2 * (8 + e + RB) * M
     return 3 * e * n * n + 11 * e * Ng + (4 * r + 4 * e) * M
 
 
-def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, KK=None):
+def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, RB=None):
     """Algorithmic bytes of one launch of each kernel: its inputs read once and outputs written once."""
     Ng = Nu * Nv
     win = n * Nu  # grid rows that hold the image (forward) or are kept by the crop (adjoint)
     vis = M * (2 * r + 2 * e)  # u, v, weight and value per visibility
-    wvis = M * (8 + e + (KK or 0) * e)  # row origin, value and stored projection row
+    wvis = M * (8 + e + (RB or 0))  # row origin, value and stored projection row
     table = {
         "degrid_wrows": Ng * e + wvis,
         "grid_wrows": wvis + Ng * e,
@@ -325,7 +331,8 @@ def wchirp_padmm_run(u, v, w, n, J, cell, chirp_kw, iters, seed, log):
     return out
 
 
-def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev):
+def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev, w=None, w_term=False, Jw=6, cell=1.0,
+                 chirp_kw=None):
     """Workload for rocprofv3 --pmc passes: a copy of known size, then the operator pair."""
     import torch
     a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)  # 512 MiB
@@ -335,8 +342,8 @@ def counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev):
     torch.cuda.synchronize()
     del a, b
     prec = args.counters
-    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, os_, 0, 1e-4, "kb", J, J,
-                                                     precision=prec)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, w, None, n, n, os_, 0, 1e-4, "kb", J, J, w_term, Jw, cell,
+                                                     cell, precision=prec, **(chirp_kw or {}))
     op = lt.op
     op.autotune(3)
     dt = torch.complex64 if prec == "c64" else torch.complex128
@@ -399,13 +406,12 @@ def main():
         w = np.random.default_rng(seed + 2).uniform(0.0, wp["wmax"], M)
     log(f"[bench] workload {args.workload}: n={n} grid={Nu} J={J} M={M} w_term={w_term}")
 
-    if args.counters:
-        counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev)
-        return
-
     chirp_kw = {}
     if wp and wp.get("chirp"):
         chirp_kw = dict(w_projection="chirp", energy_chirp=wp["chirp"], energy_wproj=wp["chirp"])
+    if args.counters:
+        counters_run(args, measurementoperator, u, v, x_h, y_h, n, J, os_, dev, w, w_term, Jw, cell, chirp_kw)
+        return
     res = {}
     outs = {}
     for prec, steps in (("c64", args.steps), ("c128", args.fp64_steps)):
@@ -418,6 +424,7 @@ def main():
         op = lt.op
         tiles = {"tile": [op.get_tuning("tile_u"), op.get_tuning("tile_v")], "perm_bucket": op.get_tuning("perm_bucket")}
         KK_op = op.get_tuning("w_footprint") ** 2 if w_term else None
+        packed = op.get_tuning("w_rows_packed") if w_term else 0
         t0 = time.perf_counter()
         variant = op.autotune(3)
         tune_s = time.perf_counter() - t0
@@ -460,7 +467,7 @@ def main():
             host_ms.append(1e3 * (time.perf_counter() - t0))
         res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
                          value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s,
-                         host_ms=statistics.median(host_ms), KK=KK_op, nnz_w=op.info().nnz_w,
+                         host_ms=statistics.median(host_ms), KK=KK_op, nnz_w=op.info().nnz_w, packed=packed,
                          fft_columns=op.get_tuning("fft_columns"), **tiles)
         log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s, "
             f"grid kernel {variant}")
@@ -474,7 +481,9 @@ def main():
     dom = max(tab, key=lambda k: tab[k][0])
     dom_ms = tab[dom][0] / tab[dom][1]
     KK = r64["KK"] if w_term else None
-    b = kernel_alg_bytes(dom, n, Nu, Nv, M, e, r, KK)
+    RB = row_bytes(KK, r64["nnz_w"], M, e, r64["packed"]) if KK else None
+    RB128 = row_bytes(KK, res["c128"]["nnz_w"], M, 16, res["c128"]["packed"]) if KK else None
+    b = kernel_alg_bytes(dom, n, Nu, Nv, M, e, r, RB)
     if b is None:  # step A: forward reads the image window, adjoint the full grid
         b = ((n * Nu * e + Ng * e) + (2 * Ng * e)) / 2
     achieved = b / (dom_ms * 1e-3)
@@ -499,12 +508,12 @@ def main():
             roofline["traffic"] = sum(vals)
         roofline["traffic_source"] = os.path.relpath(traffic_file, ROOT)
 
-    pair_bytes = model_bytes_pair(n, Ng, M, e, r, KK)
+    pair_bytes = model_bytes_pair(n, Ng, M, e, r, RB)
     # the bytes this run moves: unit weights are not stored, so neither direction reads them
     moved_bytes = pair_bytes - (0 if KK else 2 * e * M)
     kern_frac = {}
     for k in tab:
-        kb = kernel_alg_bytes(k, n, Nu, Nv, M, e, r, KK)
+        kb = kernel_alg_bytes(k, n, Nu, Nv, M, e, r, RB)
         if kb:
             kern_frac[k] = kb / (tab[k][0] / tab[k][1] * 1e-3) / HBM_PEAK
     rel_fp32_fp64 = {
@@ -584,10 +593,11 @@ def main():
         "pair_moved_bytes": moved_bytes,
         "kernels_frac_hbm_peak": kern_frac,
         **({"w_rows": {"footprint": int(round(KK ** 0.5)), "kept_entries_per_row": r64["nnz_w"] / M,
-                       "stored_bytes_per_row": KK * e}} if KK else {}),
+                       "layout": "kept entries" if r64["packed"] else "dense",
+                       "stored_bytes_per_row": RB}} if KK else {}),
         "fp64": {"value": res["c128"]["value"], "ms_per_step": res["c128"]["ms"],
                  "fp32_speedup": res["c128"]["ms"] / r64["ms"],
-                 "pair_frac_hbm_peak": model_bytes_pair(n, Ng, M, 16, 8, KK) / (res["c128"]["ms"] * 1e-3) / HBM_PEAK},
+                 "pair_frac_hbm_peak": model_bytes_pair(n, Ng, M, 16, 8, RB128) / (res["c128"]["ms"] * 1e-3) / HBM_PEAK},
         "rel_err_fp32_vs_fp64_gpu": rel_fp32_fp64,
         "rel_err_vs_oracle": acc_oracle,
         "step_latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},

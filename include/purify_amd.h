@@ -310,7 +310,11 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * default); read-only "tile_u", "tile_v", "perm_bucket" (complex64: the caller-order
  * permutation's bucket of visibilities, else 0), "w_footprint" (side of the stored rows' footprint with
  * w-projection rows, else 0) and "split_tiles" (tiles whose stored-row gridding list is cut into parts
- * that are summed in a fixed order afterwards, so that a dense tile does not form the launch's tail).
+ * that are summed in a fixed order afterwards, so that a dense tile does not form the launch's tail);
+ * "w_rows_packed" (chirp w-projection rows of at most 8 x 8 taps: 1 = stored as their kept entries, a
+ * 64-bit keep mask and offset per row and the kept values, the default; 0 = dense rows, rebuilt from the
+ * kept entries on first use; results are bit-identical either way); "w_grid_depth" (gridding with kept-entry
+ * rows: the entries whose tap values are requested ahead of their use, 8, 16 (the default) or 32).
  * Results do not depend on grid_kernel or fft_columns beyond floating-point summation order. */
 /* pf_op_set_norm, pf_op_set_tuning and pf_op_autotune change what an apply launches: a solver that
  * captured the operator's launches into a graph captures them again on its next run. */

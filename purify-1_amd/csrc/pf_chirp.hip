@@ -92,6 +92,7 @@ struct ChirpArgs {
   unsigned long long* nnz;   // fill: kept entries
   void* rows;                // fill: M * (J + 2R)^2 values in the operator's precision
   int prec;
+  uint64_t* mask;            // fill with (J + 2R)^2 <= 64: each row's keep mask (bit ov K + ou), or null
 };
 
 // The bisection of sparsify_row_(dense_)thres on the values a[0..n) held by the wave (lane-strided in
@@ -237,6 +238,10 @@ __global__ void __launch_bounds__(64) k_chirp_rows(ChirpArgs g) {
         static_cast<cpx<float>*>(g.rows)[i * KK + o] = mk<float>((float)val.x, (float)val.y);
       else
         static_cast<cpx<double>*>(g.rows)[i * KK + o] = val;
+    }
+    if (g.mask) {  // KK <= 64: entry o is lane o's
+      const unsigned long long bits = __ballot(lane < KK && OA[lane] > thr2);
+      if (lane == 0) g.mask[i] = bits;
     }
     __syncthreads();
   }
@@ -410,11 +415,14 @@ int chirp_rows_radius(ChirpBuild& cb, int64_t M, int N, int J, int kind, int Jev
 }
 
 // Pass 2: the rows, in the sorted order perm (device: sorted position -> caller's index), into rows
-// (M * (J + 2R)^2 values in precision prec); returns the kept entries.
-int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_dev, void* rows, int prec) {
+// (M * (J + 2R)^2 values in precision prec), and with (J + 2R)^2 <= 64 each row's keep mask into mask
+// (if not null); returns the kept entries.
+int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_dev, void* rows, int prec, uint64_t* mask) {
   ChirpArgs& a = cb.a;
+  if (mask && (a.J + 2 * a.R) * (a.J + 2 * a.R) > 64) throw Error(PF_ERR_ARG, "keep masks need (J + 2R)^2 <= 64");
   a.prec = prec;
   a.rows = rows;
+  a.mask = mask;
   unsigned long long* d_nnz = cb.alloc<unsigned long long>(1);
   PF_HIP(hipMemsetAsync(d_nnz, 0, sizeof(unsigned long long), cstream()));
   a.perm = perm_dev;

@@ -362,8 +362,22 @@ template <typename R>
 struct WRows {
   int Ku = 0, Kv = 0;
   const int2* kw = nullptr;      // (kw_u, kw_v) per sorted visibility
-  const cpx<R>* vals = nullptr;  // M * Kv * Ku
+  const cpx<R>* vals = nullptr;  // M * Kv * Ku, or the kept entries only when rec is set
+  // kept-entry rows (Ku Kv <= 64): rec[i] = {mask lo, mask hi, offset lo, offset hi}; bit jv Ku + ju of
+  // the mask is set for a kept entry, and row i's kept values lie at vals[offset ..] in bit order
+  const uint4* rec = nullptr;
+  int depth = 16;      // kept-entry gridding: entries whose taps are requested ahead (8, 16 or 32)
+  bool off64 = false;  // kept-entry offsets reach 2^32 (else their high words are zero)
 };
+// Kept-entry rows from dense rows and their keep masks (Ku Kv <= 64): rows_pack_records writes rec from
+// the masks (the offsets an exclusive sum of the kept counts) and returns the total kept entries;
+// rows_pack_values copies the kept values (packed holds one spare entry past the last, which the gridding
+// kernel may read for a lane whose tap is dropped); rows_unpack rebuilds the dense rows (zeros elsewhere).
+uint64_t rows_pack_records(const uint64_t* mask, int64_t M, uint4* rec, hipStream_t s);
+template <typename R>
+void rows_pack_values(const cpx<R>* dense, const uint4* rec, int64_t M, int KK, cpx<R>* packed, hipStream_t s);
+template <typename R>
+void rows_unpack(const cpx<R>* packed, const uint4* rec, int64_t M, int KK, cpx<R>* dense, hipStream_t s);
 // y = G grid: owner tiles staged with a (K - 1)-wide halo, one wave per visibility; y[perm[i]] in the
 // caller's order, or y[i] in the sorted order when perm is null
 template <typename R>
@@ -396,7 +410,7 @@ int chirp_rows_radius(ChirpBuild& cb, int64_t M, int N, int J, int kind, int Jev
                       const double* w, const double* weights);
 // pass 2: rows of (J + 2R)^2 values, row i for the caller's visibility perm[i] (device), into rows
 // (precision prec)
-int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_dev, void* rows, int prec);
+int64_t chirp_rows_fill(ChirpBuild& cb, const int32_t* perm_dev, void* rows, int prec, uint64_t* mask = nullptr);
 
 // ---- operator construction on the device (pf_build.hip) -------------------------------------------
 // the owner-tile sort of the caller's pixel coordinates uv: perm (sorted -> caller) and owner counts

@@ -330,6 +330,12 @@ struct pf_op {
   int Ku = 0, Kv = 0;         // footprint of one row of G (J, or Jw + J - 1 with w-projection)
   int w_chirp = 0;            // the stored rows are chirp-truncated rows (w_term = 2), footprint J + 2R
   void* d_wrows = nullptr;    // w-projection rows, M * Kv * Ku in the operator's precision
+  // chirp rows with Ku Kv <= 64 as their kept entries: per row {mask, offset} and the kept values; the
+  // dense rows exist only when asked for ("w_rows_packed" = 0)
+  uint4* d_wrec = nullptr;
+  void* d_wpack = nullptr;
+  int w_packed = 0;
+  int w_grid_depth = 16;      // tuning knob: kept-entry gridding's prefetch depth in entries
   int64_t nnz_w = 0;
   // a given sparse gridding matrix (pf_op_create_csr): G by rows and conj(G) transposed, by grid cell
   bool g_csr = false;
@@ -764,10 +770,30 @@ void build(pf_op* op, const pf_op_desc& d) {
   op->d_ysort = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
   if (op->vperm.two_pass) op->d_ystage = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M, 1));
   if (op->w_chirp) {
-    // rows of (J + 2R)^2 values at origins floor(u - (J + 2R)/2) = floor(u - J/2) - R
+    // rows of (J + 2R)^2 values at origins floor(u - (J + 2R)/2) = floor(u - J/2) - R; with (J + 2R)^2
+    // <= 64 written once with their keep masks and stored as their kept entries
     const int KK = op->Ku * op->Kv;
-    op->d_wrows = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M * KK, 1));
-    op->nnz_w = chirp_rows_fill(*chirp, d_perm, op->d_wrows, op->prec);
+    if (KK <= 64) {
+      void* dense = tmp.alloc<unsigned char>((int64_t)op->esz() * std::max<int64_t>(M * KK, 1));
+      uint64_t* mask = tmp.alloc<uint64_t>(M);
+      op->nnz_w = chirp_rows_fill(*chirp, d_perm, dense, op->prec, mask);
+      op->d_wrec = (uint4*)op->dalloc(sizeof(uint4) * (size_t)std::max<int64_t>(M, 1));
+      const uint64_t kept = rows_pack_records(mask, M, op->d_wrec, cstream());
+      if ((int64_t)kept != op->nnz_w) throw Error(PF_ERR_STATE, "kept-entry count differs from the rows' mask bits");
+      // one spare value past the last (zero): read, and not used, for a dropped tap of the last row
+      op->d_wpack = op->dalloc(op->esz() * (size_t)(kept + 1));
+      PF_HIP(hipMemsetAsync(static_cast<unsigned char*>(op->d_wpack) + op->esz() * kept, 0, op->esz(), cstream()));
+      if (op->prec == PF_C64)
+        rows_pack_values<float>(static_cast<const cpx<float>*>(dense), op->d_wrec, M, KK,
+                                static_cast<cpx<float>*>(op->d_wpack), cstream());
+      else
+        rows_pack_values<double>(static_cast<const cpx<double>*>(dense), op->d_wrec, M, KK,
+                                 static_cast<cpx<double>*>(op->d_wpack), cstream());
+      op->w_packed = 1;
+    } else {
+      op->d_wrows = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M * KK, 1));
+      op->nnz_w = chirp_rows_fill(*chirp, d_perm, op->d_wrows, op->prec);
+    }
     chirp.reset();
   } else if (op->w_term) {
     build_wrows(op, d, d_uvc, d_wtc, d_perm, tmp);
@@ -964,7 +990,14 @@ WRows<R> wrows(const pf_op* op) {
   w.Ku = op->Ku;
   w.Kv = op->Kv;
   w.kw = op->d_kk;
-  w.vals = static_cast<const cpx<R>*>(op->d_wrows);
+  if (op->w_packed) {
+    w.vals = static_cast<const cpx<R>*>(op->d_wpack);
+    w.rec = op->d_wrec;
+    w.depth = op->w_grid_depth;
+    w.off64 = (uint64_t)op->nnz_w >= ((uint64_t)1 << 32);
+  } else {
+    w.vals = static_cast<const cpx<R>*>(op->d_wrows);
+  }
   return w;
 }
 
@@ -1790,6 +1823,33 @@ static int set_tuning_impl(pf_op* op, const char* key, int64_t value) {
     op->gf.poly = value ? op->d_poly : nullptr;
     return PF_OK;
   }
+  if (k == "w_grid_depth") {
+    if (value != 8 && value != 16 && value != 32) return fail(PF_ERR_ARG, "w_grid_depth must be 8, 16 or 32");
+    op->w_grid_depth = (int)value;
+    return PF_OK;
+  }
+  if (k == "w_rows_packed") {
+    if (value != 0 && value != 1) return fail(PF_ERR_ARG, "w_rows_packed must be 0 or 1");
+    if (!op->d_wpack) {
+      if (value) return fail(PF_ERR_UNSUPPORTED, "kept-entry rows exist only for chirp rows of at most 8 x 8 taps");
+      return PF_OK;
+    }
+    return guarded([&] {
+      if (!value && !op->d_wrows) {  // the dense rows, rebuilt once from the kept entries
+        const int64_t KK = (int64_t)op->Ku * op->Kv;
+        void* dense = op->dalloc(op->esz() * (size_t)std::max<int64_t>(op->M * KK, 1));
+        if (op->prec == PF_C64)
+          rows_unpack<float>(static_cast<const cpx<float>*>(op->d_wpack), op->d_wrec, op->M, (int)KK,
+                             static_cast<cpx<float>*>(dense), cstream());
+        else
+          rows_unpack<double>(static_cast<const cpx<double>*>(op->d_wpack), op->d_wrec, op->M, (int)KK,
+                              static_cast<cpx<double>*>(dense), cstream());
+        PF_HIP(hipStreamSynchronize(cstream()));
+        op->d_wrows = dense;
+      }
+      op->w_packed = (int)value;
+    });
+  }
   if (k == "kernel_table") {
     if (value < 0 || value > 2) return fail(PF_ERR_ARG, "kernel_table must be 0, 1 or 2");
     if (value && !op->d_tab_u) return fail(PF_ERR_UNSUPPORTED, "tap tables exist only for complex64 and J <= 8");
@@ -1838,6 +1898,14 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   }
   if (k == "split_tiles") {
     *value = op->tl.n_comb;
+    return PF_OK;
+  }
+  if (k == "w_rows_packed") {
+    *value = op->w_packed;
+    return PF_OK;
+  }
+  if (k == "w_grid_depth") {
+    *value = op->w_grid_depth;
     return PF_OK;
   }
   if (k == "perm_bucket") {

@@ -11,6 +11,12 @@
 //    the lanes adding the taps that fall inside the tile.  The taps of one visibility are distinct grid
 //    cells (K <= N), so the lanes' read-modify-writes never collide and every cell's sum follows the
 //    list order: reproducible bit for bit.
+//  * kept-entry rows: the chirp-truncated rows (K^2 <= 64) keep a small share of their K^2 entries (the
+//    reference stores them as a sparse matrix, wproj_utilities.h:89-100), so they are stored as a 64-bit
+//    keep mask and an offset per row followed by the kept values.  Both kernels walk the same taps in the
+//    same order and skip only zero entries: the results equal the dense rows' bit for bit.
+#include <rocprim/device/device_scan.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -113,9 +119,112 @@ template void wrows_build<double>(const WRowArgs&, const cpx<double>*, const dou
                                   const int32_t*, int64_t, cpx<double>*);
 
 // ---------------------------------------------------------------------------------------------
+// Kept-entry rows
+// ---------------------------------------------------------------------------------------------
+namespace {
+__global__ void k_pack_count(const uint64_t* __restrict__ mask, int64_t M, uint64_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) cnt[i] = (uint64_t)__popcll(mask[i]);
+}
+__global__ void k_pack_rec(const uint64_t* __restrict__ mask, const uint64_t* __restrict__ off, int64_t M,
+                           uint4* __restrict__ rec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const uint64_t m = mask[i], o = off[i];
+  rec[i] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), (uint32_t)o, (uint32_t)(o >> 32));
+}
+__device__ inline uint64_t rec_mask(uint4 r) { return ((uint64_t)r.y << 32) | r.x; }
+__device__ inline int64_t rec_off(uint4 r) { return (int64_t)(((uint64_t)r.w << 32) | r.z); }
+// one thread per dense entry (row i, entry o): its position among the row's kept entries is the count of
+// kept entries before it
+template <typename R>
+__global__ void k_pack_vals(const cpx<R>* __restrict__ dense, const uint4* __restrict__ rec, int64_t M, int KK,
+                            cpx<R>* __restrict__ packed) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * KK) return;
+  const int64_t i = t / KK;
+  const int o = (int)(t - i * KK);
+  const uint4 r = rec[i];
+  const uint64_t m = rec_mask(r);
+  if ((m >> o) & 1) packed[rec_off(r) + __popcll(m & ((1ull << o) - 1))] = dense[t];
+}
+template <typename R>
+__global__ void k_unpack(const cpx<R>* __restrict__ packed, const uint4* __restrict__ rec, int64_t M, int KK,
+                         cpx<R>* __restrict__ dense) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * KK) return;
+  const int64_t i = t / KK;
+  const int o = (int)(t - i * KK);
+  const uint4 r = rec[i];
+  const uint64_t m = rec_mask(r);
+  dense[t] = ((m >> o) & 1) ? packed[rec_off(r) + __popcll(m & ((1ull << o) - 1))] : mk<R>(R(0), R(0));
+}
+unsigned pack_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+}  // namespace
+
+uint64_t rows_pack_records(const uint64_t* mask, int64_t M, uint4* rec, hipStream_t s) {
+  if (M == 0) return 0;
+  uint64_t* cnt = dev_alloc_n<uint64_t>((size_t)M);
+  uint64_t* off = nullptr;
+  void* tmp = nullptr;
+  uint64_t last[2] = {0, 0};
+  try {
+    off = dev_alloc_n<uint64_t>((size_t)M);
+    hipLaunchKernelGGL(k_pack_count, dim3(pack_blocks(M)), dim3(256), 0, s, mask, M, cnt);
+    PF_HIP(hipGetLastError());
+    size_t tb = 0;
+    PF_HIP(rocprim::exclusive_scan(nullptr, tb, cnt, off, (uint64_t)0, (size_t)M, rocprim::plus<uint64_t>(), s));
+    tmp = dev_alloc(tb ? tb : 1);
+    PF_HIP(rocprim::exclusive_scan(tmp, tb, cnt, off, (uint64_t)0, (size_t)M, rocprim::plus<uint64_t>(), s));
+    hipLaunchKernelGGL(k_pack_rec, dim3(pack_blocks(M)), dim3(256), 0, s, mask, off, M, rec);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipMemcpyAsync(&last[0], off + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipMemcpyAsync(&last[1], cnt + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    dev_free(cnt);
+    if (off) dev_free(off);
+    if (tmp) dev_free(tmp);
+    throw;
+  }
+  dev_free(cnt);
+  dev_free(off);
+  dev_free(tmp);
+  return last[0] + last[1];
+}
+
+template <typename R>
+void rows_pack_values(const cpx<R>* dense, const uint4* rec, int64_t M, int KK, cpx<R>* packed, hipStream_t s) {
+  if (M == 0) return;
+  hipLaunchKernelGGL((k_pack_vals<R>), dim3(pack_blocks(M * KK)), dim3(256), 0, s, dense, rec, M, KK, packed);
+  PF_HIP(hipGetLastError());
+}
+template <typename R>
+void rows_unpack(const cpx<R>* packed, const uint4* rec, int64_t M, int KK, cpx<R>* dense, hipStream_t s) {
+  if (M == 0) return;
+  hipLaunchKernelGGL((k_unpack<R>), dim3(pack_blocks(M * KK)), dim3(256), 0, s, packed, rec, M, KK, dense);
+  PF_HIP(hipGetLastError());
+}
+template void rows_pack_values<float>(const cpx<float>*, const uint4*, int64_t, int, cpx<float>*, hipStream_t);
+template void rows_pack_values<double>(const cpx<double>*, const uint4*, int64_t, int, cpx<double>*, hipStream_t);
+template void rows_unpack<float>(const cpx<float>*, const uint4*, int64_t, int, cpx<float>*, hipStream_t);
+template void rows_unpack<double>(const cpx<double>*, const uint4*, int64_t, int, cpx<double>*, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------
 // Device
 // ---------------------------------------------------------------------------------------------
 __device__ inline int wrapw(int a, int n) { return a & (n - 1); }
+// c + a g with four fused multiply-adds (both degridding kernels, so that they round alike)
+template <typename R>
+__device__ inline cpx<R> mac(cpx<R> c, cpx<R> a, cpx<R> g) {
+  return mk<R>(fmar(a.x, g.x, fmar(-a.y, g.y, c.x)), fmar(a.x, g.y, fmar(a.y, g.x, c.y)));
+}
+// c + conj(a) s with four fused multiply-adds (both gridding kernels, so that they round alike)
+template <typename R>
+__device__ inline cpx<R> mac_conj(cpx<R> c, cpx<R> a, cpx<R> s) {
+  return mk<R>(fmar(s.x, a.x, fmar(s.y, a.y, c.x)), fmar(s.y, a.x, fmar(-s.x, a.y, c.y)));
+}
 
 constexpr int WD_THREADS = 256;
 constexpr int WD_ROWS = 4;  // staged rows per wave per round
@@ -126,19 +235,11 @@ constexpr int WG_LANES = 8;  // lanes per visibility in degridding: 8 visibiliti
 // visibilities at a time, 8 lanes each: lane r of a group sums the taps k = r, r + 8, ... of its
 // visibility's stored row (8 consecutive taps per step: one 64-byte segment of the row for complex64),
 // and the group adds its 8 partial sums with a fixed butterfly.
+// The block's tile and its halo (wrapping at the grid edges) into LDS, loads before stores.
 template <typename R>
-__global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRows<R> w, const int32_t* __restrict__ perm,
-                                                             TileLists tl, const cpx<R>* __restrict__ grid,
-                                                             cpx<R>* __restrict__ y, R scl) {
-  extern __shared__ unsigned char smem_raw[];
-  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
-  const uint4 sl = tl.slot_own[blockIdx.x];
-  const int tile = (int)sl.x;
-  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
-  const int tx0 = tu * tl.TU, ty0 = tv * tl.TV;
-  const int W = tl.TU + w.Ku - 1, H = tl.TV + w.Kv - 1;
+__device__ inline void stage_halo(cpx<R>* lds, const cpx<R>* __restrict__ grid, int Nu, int Nv, int tx0, int ty0, int W,
+                                  int H) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = WD_THREADS / 64;
-  // stage the tile and its halo (wrapping at the grid edges), loads before stores
   for (int r0 = wave * WD_ROWS; r0 < H; r0 += nw * WD_ROWS) {
     for (int lx = lane; lx < W; lx += 64) {
       const int gx = wrapw(tx0 + lx, Nu);
@@ -153,6 +254,21 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRo
         if (r0 + k < H) lds[(r0 + k) * W + lx] = tmp[k];
     }
   }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRows<R> w, const int32_t* __restrict__ perm,
+                                                             TileLists tl, const cpx<R>* __restrict__ grid,
+                                                             cpx<R>* __restrict__ y, R scl) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const uint4 sl = tl.slot_own[blockIdx.x];
+  const int tile = (int)sl.x;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * tl.TU, ty0 = tv * tl.TV;
+  const int W = tl.TU + w.Ku - 1, H = tl.TV + w.Kv - 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = WD_THREADS / 64;
+  stage_halo<R>(lds, grid, Nu, Nv, tx0, ty0, W, H);
   __syncthreads();
   const int KK = w.Ku * w.Kv;
   const int grp = lane / WG_LANES, r = lane % WG_LANES;
@@ -167,9 +283,7 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRo
       for (int k = r; k < KK; k += WG_LANES) {
         const int jv = k / w.Ku, ju = k - jv * w.Ku;
         const cpx<R> g = lds[base + jv * W + ju];
-        const cpx<R> a = row[k];
-        acc.x += a.x * g.x - a.y * g.y;
-        acc.y += a.x * g.y + a.y * g.x;
+        acc = mac(acc, row[k], g);
       }
     }
 #pragma unroll
@@ -178,6 +292,115 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRo
       acc.y += __shfl_xor(acc.y, m, 64);
     }
     if (r == 0 && i < sl.z) y[perm ? perm[i] : i] = scale(acc, scl);
+  }
+}
+
+// Degridding with kept-entry rows (K^2 <= 64): the groups of k_degrid_wrows, each lane's taps k = r,
+// r + 8, ... in the same order with the same multiply-adds and the zero entries skipped, so the sums
+// equal the dense rows' bit for bit.  A visibility's taps wait on two dependent loads (its record, then
+// its kept values), so the wave walks its steps of 8 visibilities as a pipeline: the records two steps
+// ahead, the values one step ahead, in two register sets taken in turn (every load unconditional: the
+// record clamped to the slot's last visibility, and a dropped tap reading at most one past its row's
+// last kept value, which exists: the kept values carry one spare entry).  The lane's 8 tap cells, bit
+// masks and LDS offsets are constants; the taps' LDS reads are unconditional (a tap past the footprint
+// reads cell 0) so that they issue together, and a dropped tap is a select.  PERM: the output position
+// in the caller's order is loaded with the record (a load issued at the store would wait for every load
+// in flight).
+template <typename R, bool PERM>
+__global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows_packed(int Nu, int Nv, WRows<R> w,
+                                                                    const int32_t* __restrict__ perm, TileLists tl,
+                                                                    const cpx<R>* __restrict__ grid,
+                                                                    cpx<R>* __restrict__ y, R scl) {
+  constexpr int NQ = 64 / WG_LANES;  // taps per lane
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const uint4 sl = tl.slot_own[blockIdx.x];
+  const int tile = (int)sl.x;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * tl.TU, ty0 = tv * tl.TV;
+  const int W = tl.TU + w.Ku - 1, H = tl.TV + w.Kv - 1;
+  stage_halo<R>(lds, grid, Nu, Nv, tx0, ty0, W, H);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr uint32_t STEP = (WD_THREADS / 64) * (64 / WG_LANES);  // visibilities per block step
+  const int grp = lane / WG_LANES, r = lane % WG_LANES;
+  const uint32_t end = sl.z, last = sl.z - 1;
+  uint32_t iw = sl.y + (uint32_t)wave * (64 / WG_LANES);  // the wave's first visibility of the step
+  if (iw >= end) return;
+  // the lane's taps k = r + 8 q: the bits below k within its 32-bit half of the mask (q and q + 4 alike),
+  // and the tap's LDS offset from the footprint origin (cell 0 past the footprint)
+  const int KK = w.Ku * w.Kv;
+  uint32_t below[NQ / 2];
+  int ofs[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ / 2; ++q) below[q] = (1u << (r + WG_LANES * q)) - 1u;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int k = r + WG_LANES * q;
+    ofs[q] = k < KK ? (k / w.Ku) * W + (k - (k / w.Ku) * w.Ku) : 0;
+  }
+  auto records = [&](uint32_t i0, uint4& rc, int2& kw, uint32_t& pos) {
+    const uint32_t i = min(i0 + (uint32_t)grp, last);
+    kw = w.kw[i];
+    rc = w.rec[i];
+    if constexpr (PERM) pos = (uint32_t)perm[i];
+    else pos = i;
+  };
+  auto values = [&](const uint4& rc, cpx<R> (&a)[NQ]) {
+    const cpx<R>* p = w.vals + (int64_t)(((uint64_t)rc.w << 32) | rc.z);
+    const uint32_t nlo = (uint32_t)__popc(rc.x);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const uint32_t idx = q < NQ / 2 ? (uint32_t)__popc(rc.x & below[q]) : (uint32_t)__popc(rc.y & below[q - NQ / 2]) + nlo;
+      a[q] = p[idx];
+    }
+  };
+  auto sum = [&](uint32_t i0, uint32_t mlo, uint32_t mhi, int base, uint32_t pos, const cpx<R> (&a)[NQ]) {
+    cpx<R> g[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) g[q] = lds[base + ofs[q]];
+    cpx<R> acc = mk<R>(R(0), R(0));
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const uint32_t bit = q < NQ / 2 ? (mlo & (below[q] + 1u)) : (mhi & (below[q - NQ / 2] + 1u));
+      const cpx<R> t = mac(acc, a[q], g[q]);
+      acc.x = bit ? t.x : acc.x;  // per component: a select of the pair would go through memory
+      acc.y = bit ? t.y : acc.y;
+    }
+#pragma unroll
+    for (int m = WG_LANES / 2; m >= 1; m >>= 1) {
+      acc.x += __shfl_xor(acc.x, m, 64);
+      acc.y += __shfl_xor(acc.y, m, 64);
+    }
+    if (r == 0 && i0 + (uint32_t)grp < end) y[pos] = scale(acc, scl);
+  };
+  auto base_of = [&](int2 k0) { return (wrapw(k0.y + 1, Nv) - ty0) * W + (wrapw(k0.x + 1, Nu) - tx0); };
+  uint4 rA, rB;
+  int2 kA, kB;
+  uint32_t pA, pB;
+  cpx<R> aA[NQ], aB[NQ];
+  records(iw, rA, kA, pA);
+  records(iw + STEP, rB, kB, pB);
+  values(rA, aA);
+  for (;;) {
+    {  // step in set A: its record, origin and position taken before the set is reloaded two steps ahead
+      const uint32_t mlo = rA.x, mhi = rA.y, pos = pA;
+      const int base = base_of(kA);
+      records(iw + 2 * STEP, rA, kA, pA);
+      values(rB, aB);
+      sum(iw, mlo, mhi, base, pos, aA);
+    }
+    iw += STEP;
+    if (iw >= end) break;
+    {  // the same in set B
+      const uint32_t mlo = rB.x, mhi = rB.y, pos = pB;
+      const int base = base_of(kB);
+      records(iw + 2 * STEP, rB, kB, pB);
+      values(rA, aA);
+      sum(iw, mlo, mhi, base, pos, aB);
+    }
+    iw += STEP;
+    if (iw >= end) break;
   }
 }
 
@@ -256,10 +479,8 @@ __global__ void __launch_bounds__(64) k_grid_wrows(int Nu, int Nv, WRows<R> w, T
         for (int t = 0; t < TPL; ++t) {
           const int x = wrapw(kx + 1 + du[t] - tx0, Nu), yy = wrapw(ky + 1 + dv[t] - ty0, Nv);
           if (has[t] && x < TU && yy < TV) {
-            // conj(G) * y
             cpx<R>& c = acc[yy * TU + x];
-            c.x += a[t].x * s.x + a[t].y * s.y;
-            c.y += a[t].x * s.y - a[t].y * s.x;
+            c = mac_conj(c, a[t], s);
           }
         }
       }
@@ -272,6 +493,120 @@ __global__ void __launch_bounds__(64) k_grid_wrows(int Nu, int Nv, WRows<R> w, T
       grid[(int64_t)(ty0 + ly) * Nu + tx0 + lx] = acc[e];
     }
   } else {  // one part of a split tile: its partial sums, combined below
+    cpx<R>* part = static_cast<cpx<R>*>(tl.part) + (int64_t)sl.w * TU * TV;
+    for (int e = lane; e < TU * TV; e += 64) part[e] = acc[e];
+  }
+}
+
+// Gridding with kept-entry rows (K^2 <= 64): as k_grid_wrows with one tap per lane, the lane's tap of
+// an entry held only where the entry's mask keeps it; its value is the kept entry numbered by the mask
+// bits below the lane (mbcnt).  The kept taps are added in the dense kernel's order with the same
+// multiply-adds and the zeros are skipped, so the grid equals the dense rows' bit for bit.
+// The kernel is bound by vector instruction issue, so the per-entry work is kept small: the mask itself
+// is the lane predicate (inverse ballot: no vector compare), the value is read at a uniform base plus the
+// lane's mbcnt (a non-kept lane reads at most one past the row's last kept value, which exists: the kept
+// values carry one spare entry), the entry's footprint origin relative to the tile (wrapped, clamped to
+// 16 bits) is computed once per batch, and the LDS cell is a uniform offset plus the lane's own.
+// The list is walked in batches of 64 entries (one per lane: index, origin, value and record), each
+// batch fully unrolled so that every load is unconditional (clamped to the tile's last entry) and the
+// waits are counted statically: the D-deep ring of tap values runs across the batch boundary, and the
+// next batch's records are requested a quarter of the way through the current one.  Entries are
+// broadcast from their lane with readlane (the batch position is a compile-time constant).
+__device__ inline int rl(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ inline uint32_t rl(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
+__device__ inline float rl(float v, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j)); }
+__device__ inline double rl(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <typename R>
+struct GEnt {
+  uint32_t xy;  // footprint origin relative to the tile: low 16 bits x, high 16 bits y (signed)
+  cpx<R> s;     // the entry's visibility, scaled
+  uint4 r;      // its record
+};
+
+template <typename R, int D, bool OFF64>
+__global__ void __launch_bounds__(64) k_grid_wrows_packed(int Nu, int Nv, WRows<R> w, TileLists tl,
+                                                          const cpx<R>* __restrict__ ys, cpx<R>* __restrict__ grid,
+                                                          R scl) {
+  static_assert(64 % D == 0 && D <= 32, "the ring refills from the next batch after 64 - D entries");
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* acc = reinterpret_cast<cpx<R>*>(smem_raw);
+  const uint4 sl = tl.slot[blockIdx.x];
+  const int tile = (int)sl.x;
+  const int TU = tl.TU, TV = tl.TV;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * TU, ty0 = tv * TV;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < TU * TV; e += 64) acc[e] = mk<R>(R(0), R(0));
+  __syncthreads();
+  const uint32_t beg = sl.y, end = sl.z;
+  if (beg < end) {
+    // this lane's tap: row and column offsets (lanes past the row have no mask bit) and its LDS offset
+    // from the footprint origin's cell
+    const int dv = lane / w.Ku, du = lane - (lane / w.Ku) * w.Ku;
+    const int lofs = (dv * TU + du) * (int)sizeof(cpx<R>);
+    auto rel16 = [](int a, int n) {  // a wrapped into (-n/2, n/2], clamped to 16 bits
+      const int r = ((a + n / 2) & (n - 1)) - n / 2;
+      return (uint32_t)(uint16_t)(int16_t)max(-32768, min(32767, r));
+    };
+    auto load_ent = [&](uint32_t i) {
+      GEnt<R> e;
+      const int2 k = w.kw[i];
+      e.xy = rel16(k.x + 1 - tx0, Nu) | (rel16(k.y + 1 - ty0, Nv) << 16);
+      e.s = scale(ys[i], scl);
+      e.r = w.rec[i];
+      return e;
+    };
+    // batch entry j's tap of this lane: the entry's mask and this lane's value (read for every lane)
+    auto fetch = [&](const GEnt<R>& e, int j, uint64_t& m, cpx<R>& val) {
+      const uint32_t mlo = rl(e.r.x, j), mhi = rl(e.r.y, j);
+      m = ((uint64_t)mhi << 32) | mlo;
+      uint64_t off = rl(e.r.z, j);
+      if constexpr (OFF64) off |= (uint64_t)rl(e.r.w, j) << 32;
+      const uint32_t idx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
+      val = (w.vals + off)[idx];
+    };
+    GEnt<R> cur = load_ent(tl.list[min(beg + (uint32_t)lane, end - 1)]);
+    uint64_t rm[D];
+    cpx<R> ring[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) fetch(cur, d, rm[d], ring[d]);
+    for (uint32_t b0 = beg; b0 < end; b0 += 64) {
+      // the next batch (or this one again after the last: loaded, never used)
+      const uint32_t nb = b0 + 64 < end ? b0 + 64 : b0;
+      const uint32_t nidx = tl.list[min(nb + (uint32_t)lane, end - 1)];
+      GEnt<R> nxt;
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        if (j == 16) nxt = load_ent(nidx);
+        const cpx<R> a = ring[j % D];
+        const uint64_t ma = b0 + (uint32_t)j < end ? rm[j % D] : 0ull;
+        if (j + D < 64) fetch(cur, j + D, rm[j % D], ring[j % D]);
+        else fetch(nxt, j + D - 64, rm[j % D], ring[j % D]);
+        const uint32_t xy = rl(cur.xy, j);
+        const int xs = (int)(int16_t)(xy & 0xffffu), ysh = (int)(int16_t)(xy >> 16);
+        const cpx<R> sv = mk<R>(rl(cur.s.x, j), rl(cur.s.y, j));
+        if (__builtin_amdgcn_inverse_ballot_w64(ma) && (unsigned)(xs + du) < (unsigned)TU &&
+            (unsigned)(ysh + dv) < (unsigned)TV) {
+          cpx<R>* c = reinterpret_cast<cpx<R>*>(smem_raw + (ysh * TU + xs) * (int)sizeof(cpx<R>) + lofs);
+          *c = mac_conj(*c, a, sv);
+        }
+      }
+      cur = nxt;
+    }
+  }
+  __syncthreads();
+  if (sl.w == PART_NONE) {
+    for (int e = lane; e < TU * TV; e += 64) {
+      const int ly = e / TU, lx = e - ly * TU;
+      grid[(int64_t)(ty0 + ly) * Nu + tx0 + lx] = acc[e];
+    }
+  } else {
     cpx<R>* part = static_cast<cpx<R>*>(tl.part) + (int64_t)sl.w * TU * TV;
     for (int e = lane; e < TU * TV; e += 64) part[e] = acc[e];
   }
@@ -304,8 +639,18 @@ void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const 
   const size_t shm = sizeof(cpx<R>) * (size_t)(t.TU + w.Ku - 1) * (t.TV + w.Kv - 1);
   if (shm > 65536) throw Error(PF_ERR_UNSUPPORTED, "w-projection degridding tile and halo exceed 64 KiB of LDS");
   ProfScope prof("degrid_wrows", s);
-  hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, grid,
-                     y, scale);
+  if (w.rec) {
+    if (w.Ku * w.Kv > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
+    if (perm)
+      hipLaunchKernelGGL((k_degrid_wrows_packed<R, true>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w,
+                         perm, t, grid, y, scale);
+    else
+      hipLaunchKernelGGL((k_degrid_wrows_packed<R, false>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv,
+                         w, perm, t, grid, y, scale);
+  } else {
+    hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, grid,
+                       y, scale);
+  }
   PF_HIP(hipGetLastError());
 }
 
@@ -316,7 +661,21 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
   const int KK = w.Ku * w.Kv;
   const dim3 blocks((unsigned)t.n_slot);
   ProfScope prof("grid_wrows", s);
-  if (KK <= 64)
+  if (w.rec) {
+    if (KK > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
+#define PF_GWP(DD, O64) \
+  hipLaunchKernelGGL((k_grid_wrows_packed<R, DD, O64>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale)
+    if (w.off64) {
+      if (w.depth == 32) PF_GWP(32, true);
+      else if (w.depth == 8) PF_GWP(8, true);
+      else PF_GWP(16, true);
+    } else {
+      if (w.depth == 32) PF_GWP(32, false);
+      else if (w.depth == 8) PF_GWP(8, false);
+      else PF_GWP(16, false);
+    }
+#undef PF_GWP
+  } else if (KK <= 64)
     hipLaunchKernelGGL((k_grid_wrows<R, 1, 8>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else if (KK <= 128)
     hipLaunchKernelGGL((k_grid_wrows<R, 2, 8>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);

@@ -8,7 +8,9 @@ restatement (one N_g-point FFT per row).
   rows whose threshold falls within rounding of a magnitude are skipped, as for the matrix builder;
 - the 2048^2 image, 4e6 visibilities, KB 6 x 6, w in [0, 200], 1 degree field, energy fractions 0.999:
   five rows against the oracle at the real 4096^2 grid (complex128), the footprint within 24 x 24, and the
-  complex64 operator's adjoint dot-product test.
+  complex64 operator's adjoint dot-product test;
+- rows of at most 8 x 8 taps are stored as their kept entries: every application equals the dense rows'
+  bit for bit (rows at the band's edge included), and the rows match the oracle's.
 """
 import numpy as np
 import pytest
@@ -113,3 +115,61 @@ def test_2048_m4e6_w_0_200_rows_footprint_and_dot_test():
     y = utilities.synth_cnormal(1007, M)
     lhs, rhs = np.vdot(y, l64 * x), np.vdot(l64.adjoint() * y, x)
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("J", [4, 6])
+def test_kept_entry_rows_equal_the_dense_rows_bit_for_bit(precision, J):
+    # a 1 degree field with |w| <= 200 keeps the chirp within one cell (footprint J + 2 <= 8): the rows are
+    # stored as their kept entries (keep mask, offset, values); the dense rows, rebuilt from them on
+    # request, must give the same numbers bit for bit in every application, and both match the oracle
+    n, M = 32, 3000
+    cell = 3600.0 / n
+    u, v = make_uv(1301 + J, M, n)
+    # rows at the band's edge (the chirp offsets there cross the centred grid's border) and centre
+    u[:4] = utilities.radians_to_pixels(np.array([np.pi * 0.999, -np.pi, 0.0, np.pi * 0.5]), n, 2.0)
+    v[:4] = utilities.radians_to_pixels(np.array([-np.pi, np.pi * 0.999, np.pi * 0.999, -np.pi * 0.999]), n, 2.0)
+    w = np.random.default_rng(1302).uniform(-200.0, 200.0, M)
+    wt = utilities.synth_cnormal(1303, M) + 1.0
+    lt = chirp_op(u, v, w, n, J, cell, 0.999, 0.999, precision, wt)
+    K = lt.op.get_tuning("w_footprint")
+    assert K <= 8 and lt.op.get_tuning("w_rows_packed") == 1
+    dt = torch.complex128 if precision == "c128" else torch.complex64
+    x = utilities.synth_image(1304, n, n, 5)
+    y = utilities.synth_cnormal(1305, M)
+    g = torch.from_numpy(utilities.synth_cnormal(1306, (2 * n) ** 2)).to(dt).cuda()
+    yd = torch.from_numpy(y).to(dt).cuda()
+
+    def run():
+        return (lt * x, lt.adjoint() * y, lt.op.apply_G(g).cpu(), lt.op.apply_GH(yd).cpu(), lt.op.normal(x))
+
+    packed = run()
+    lt.op.set_tuning("w_rows_packed", 0)
+    assert lt.op.get_tuning("w_rows_packed") == 0
+    dense_run = run()
+    for a, b in zip(packed, dense_run):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    lt.op.set_tuning("w_rows_packed", 1)
+    for a, b in zip(packed, run()):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    # the rows against the oracle's (ties in the keep decision skipped, as above)
+    G, dense = oracle_rows(u, v, w, n, J, cell, 0.999, 0.999, wt)
+    tie = np.array([threshold_margin(G, m, 2 * n, w[m], cell, 0.999, 0.999) < 1e-9 for m in range(M)])
+    gh = g.cpu().numpy().astype(np.complex128)
+    yo = dense @ gh
+    ok = np.abs(packed[2].numpy() - yo) <= TOL[precision] * (np.abs(dense) @ np.abs(gh))
+    assert np.all(ok | tie) and (~tie).sum() > M // 2
+
+
+def test_kept_entry_rows_tuning_errors():
+    from purify_amd._lib import PurifyError
+    n, M = 32, 100
+    u, v = make_uv(1311, M, n)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", 4, 4,
+                                                     precision="c64")
+    assert lt.op.get_tuning("w_rows_packed") == 0
+    lt.op.set_tuning("w_rows_packed", 0)
+    with pytest.raises(PurifyError, match="kept-entry rows"):
+        lt.op.set_tuning("w_rows_packed", 1)
+    with pytest.raises(PurifyError, match="0 or 1"):
+        lt.op.set_tuning("w_rows_packed", 2)
