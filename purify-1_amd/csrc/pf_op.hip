@@ -1798,8 +1798,10 @@ static int set_tuning_impl(pf_op* op, const char* key, int64_t value) {
   if (k == "fft_columns") {
     if (value != 0 && value != 1) return fail(PF_ERR_ARG, "fft_columns must be 0 or 1");
     return guarded([&] {
+      Fft2dPlan np;  // built first: a mode the grid cannot use leaves the current plan in place
+      fft_plan_init(np, op->Nv, op->Nu, op->prec, (int)value);
       fft_plan_free(op->fft);
-      fft_plan_init(op->fft, op->Nv, op->Nu, op->prec, (int)value);
+      op->fft = np;
       if (op->fft_row_bpc >= 0) op->fft.row_bpc = op->fft_row_bpc;
       op->fft_col_mode = (int)value;
     });
