@@ -45,8 +45,8 @@ METRIC = "Φ+Φ† vis/s FP32 vs FP64 (rel.err≤1e-5, frac HBM peak); served p5
 
 def row_bytes(KK, nnz, M, e, packed):
     """Stored bytes per projection row: KK taps dense, or a 16-byte record (keep mask, offset) and the kept
-    entries."""
-    return 16 + e * nnz / M if packed else KK * e
+    entries with one tap byte each."""
+    return 16 + (e + 1) * nnz / M if packed else KK * e
 
 
 def model_bytes_pair(n, Ng, M, e, r, RB=None):

@@ -366,16 +366,22 @@ struct WRows {
   // kept-entry rows (Ku Kv <= 64): rec[i] = {mask lo, mask hi, offset lo, offset hi}; bit jv Ku + ju of
   // the mask is set for a kept entry, and row i's kept values lie at vals[offset ..] in bit order
   const uint4* rec = nullptr;
+  const uint8_t* pos = nullptr;  // kept-entry rows: each kept value's tap, jv << 4 | ju (beside vals)
+  int slots = 8;                 // kept-entry degridding: slots per lane (8 lanes per row): 2, 4 or 8
   int depth = 16;      // kept-entry gridding: entries whose taps are requested ahead (8, 16 or 32)
   bool off64 = false;  // kept-entry offsets reach 2^32 (else their high words are zero)
 };
 // Kept-entry rows from dense rows and their keep masks (Ku Kv <= 64): rows_pack_records writes rec from
 // the masks (the offsets an exclusive sum of the kept counts) and returns the total kept entries;
-// rows_pack_values copies the kept values (packed holds one spare entry past the last, which the gridding
-// kernel may read for a lane whose tap is dropped); rows_unpack rebuilds the dense rows (zeros elsewhere).
-uint64_t rows_pack_records(const uint64_t* mask, int64_t M, uint4* rec, hipStream_t s);
+// (and the largest kept count of a row into max_kept); rows_pack_values copies the kept values and writes
+// each one's tap (jv << 4 | ju, Ku the row width) into
+// pos; both arrays carry ROWS_PACK_PAD spare entries past the last (zero), which the kernels read and do
+// not use for a dropped tap or an unused slot; rows_unpack rebuilds the dense rows (zeros elsewhere).
+constexpr int ROWS_PACK_PAD = 64;
+uint64_t rows_pack_records(const uint64_t* mask, int64_t M, uint4* rec, int* max_kept, hipStream_t s);
 template <typename R>
-void rows_pack_values(const cpx<R>* dense, const uint4* rec, int64_t M, int KK, cpx<R>* packed, hipStream_t s);
+void rows_pack_values(const cpx<R>* dense, const uint4* rec, int64_t M, int KK, int Ku, cpx<R>* packed,
+                      uint8_t* pos, hipStream_t s);
 template <typename R>
 void rows_unpack(const cpx<R>* packed, const uint4* rec, int64_t M, int KK, cpx<R>* dense, hipStream_t s);
 // y = G grid: owner tiles staged with a (K - 1)-wide halo, one wave per visibility; y[perm[i]] in the

@@ -334,6 +334,8 @@ struct pf_op {
   // dense rows exist only when asked for ("w_rows_packed" = 0)
   uint4* d_wrec = nullptr;
   void* d_wpack = nullptr;
+  uint8_t* d_wpos = nullptr;  // each kept value's tap (jv << 4 | ju)
+  int w_slots = 8;            // kept-entry degridding's slots per lane (the largest row's kept taps / 8)
   int w_packed = 0;
   int w_grid_depth = 16;      // tuning knob: kept-entry gridding's prefetch depth in entries
   int64_t nnz_w = 0;
@@ -778,17 +780,23 @@ void build(pf_op* op, const pf_op_desc& d) {
       uint64_t* mask = tmp.alloc<uint64_t>(M);
       op->nnz_w = chirp_rows_fill(*chirp, d_perm, dense, op->prec, mask);
       op->d_wrec = (uint4*)op->dalloc(sizeof(uint4) * (size_t)std::max<int64_t>(M, 1));
-      const uint64_t kept = rows_pack_records(mask, M, op->d_wrec, cstream());
+      int max_kept = 0;
+      const uint64_t kept = rows_pack_records(mask, M, op->d_wrec, &max_kept, cstream());
+      op->w_slots = max_kept <= 16 ? 2 : max_kept <= 32 ? 4 : 8;
       if ((int64_t)kept != op->nnz_w) throw Error(PF_ERR_STATE, "kept-entry count differs from the rows' mask bits");
-      // one spare value past the last (zero): read, and not used, for a dropped tap of the last row
-      op->d_wpack = op->dalloc(op->esz() * (size_t)(kept + 1));
-      PF_HIP(hipMemsetAsync(static_cast<unsigned char*>(op->d_wpack) + op->esz() * kept, 0, op->esz(), cstream()));
+      // the kept values and their taps, with ROWS_PACK_PAD spare (zero) entries past the last: read, and
+      // not used, for the last rows' dropped taps and unused slots
+      op->d_wpack = op->dalloc(op->esz() * (size_t)(kept + ROWS_PACK_PAD));
+      op->d_wpos = (uint8_t*)op->dalloc((size_t)(kept + ROWS_PACK_PAD));
+      PF_HIP(hipMemsetAsync(static_cast<unsigned char*>(op->d_wpack) + op->esz() * kept, 0, op->esz() * ROWS_PACK_PAD,
+                            cstream()));
+      PF_HIP(hipMemsetAsync(op->d_wpos + kept, 0, ROWS_PACK_PAD, cstream()));
       if (op->prec == PF_C64)
-        rows_pack_values<float>(static_cast<const cpx<float>*>(dense), op->d_wrec, M, KK,
-                                static_cast<cpx<float>*>(op->d_wpack), cstream());
+        rows_pack_values<float>(static_cast<const cpx<float>*>(dense), op->d_wrec, M, KK, op->Ku,
+                                static_cast<cpx<float>*>(op->d_wpack), op->d_wpos, cstream());
       else
-        rows_pack_values<double>(static_cast<const cpx<double>*>(dense), op->d_wrec, M, KK,
-                                 static_cast<cpx<double>*>(op->d_wpack), cstream());
+        rows_pack_values<double>(static_cast<const cpx<double>*>(dense), op->d_wrec, M, KK, op->Ku,
+                                 static_cast<cpx<double>*>(op->d_wpack), op->d_wpos, cstream());
       op->w_packed = 1;
     } else {
       op->d_wrows = op->dalloc(op->esz() * (size_t)std::max<int64_t>(M * KK, 1));
@@ -993,6 +1001,8 @@ WRows<R> wrows(const pf_op* op) {
   if (op->w_packed) {
     w.vals = static_cast<const cpx<R>*>(op->d_wpack);
     w.rec = op->d_wrec;
+    w.pos = op->d_wpos;
+    w.slots = op->w_slots;
     w.depth = op->w_grid_depth;
     w.off64 = (uint64_t)op->nnz_w >= ((uint64_t)1 << 32);
   } else {

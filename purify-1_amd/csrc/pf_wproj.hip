@@ -13,8 +13,9 @@
 //    list order: reproducible bit for bit.
 //  * kept-entry rows: the chirp-truncated rows (K^2 <= 64) keep a small share of their K^2 entries (the
 //    reference stores them as a sparse matrix, wproj_utilities.h:89-100), so they are stored as a 64-bit
-//    keep mask and an offset per row followed by the kept values.  Both kernels walk the same taps in the
-//    same order and skip only zero entries: the results equal the dense rows' bit for bit.
+//    keep mask and an offset per row, the kept values and each one's tap.  Gridding walks the same taps
+//    in the same order as with dense rows and skips only zero entries (bit-identical); degridding takes
+//    the kept values in stored order (equal up to the order of its additions).
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -122,9 +123,21 @@ template void wrows_build<double>(const WRowArgs&, const cpx<double>*, const dou
 // Kept-entry rows
 // ---------------------------------------------------------------------------------------------
 namespace {
-__global__ void k_pack_count(const uint64_t* __restrict__ mask, int64_t M, uint64_t* __restrict__ cnt) {
+// kept counts per row, and their maximum (one atomic per block after a block-wide max)
+__global__ void __launch_bounds__(256) k_pack_count(const uint64_t* __restrict__ mask, int64_t M,
+                                                    uint64_t* __restrict__ cnt, unsigned* __restrict__ maxc) {
+  __shared__ unsigned bm;
+  if (threadIdx.x == 0) bm = 0;
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < M) cnt[i] = (uint64_t)__popcll(mask[i]);
+  unsigned c = 0;
+  if (i < M) {
+    c = (unsigned)__popcll(mask[i]);
+    cnt[i] = c;
+  }
+  atomicMax(&bm, c);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(maxc, bm);
 }
 __global__ void k_pack_rec(const uint64_t* __restrict__ mask, const uint64_t* __restrict__ off, int64_t M,
                            uint4* __restrict__ rec) {
@@ -139,14 +152,18 @@ __device__ inline int64_t rec_off(uint4 r) { return (int64_t)(((uint64_t)r.w << 
 // kept entries before it
 template <typename R>
 __global__ void k_pack_vals(const cpx<R>* __restrict__ dense, const uint4* __restrict__ rec, int64_t M, int KK,
-                            cpx<R>* __restrict__ packed) {
+                            int Ku, cpx<R>* __restrict__ packed, uint8_t* __restrict__ pos) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M * KK) return;
   const int64_t i = t / KK;
   const int o = (int)(t - i * KK);
   const uint4 r = rec[i];
   const uint64_t m = rec_mask(r);
-  if ((m >> o) & 1) packed[rec_off(r) + __popcll(m & ((1ull << o) - 1))] = dense[t];
+  if ((m >> o) & 1) {
+    const int64_t d = rec_off(r) + __popcll(m & ((1ull << o) - 1));
+    packed[d] = dense[t];
+    pos[d] = (uint8_t)(((o / Ku) << 4) | (o % Ku));
+  }
 }
 template <typename R>
 __global__ void k_unpack(const cpx<R>* __restrict__ packed, const uint4* __restrict__ rec, int64_t M, int KK,
@@ -162,15 +179,20 @@ __global__ void k_unpack(const cpx<R>* __restrict__ packed, const uint4* __restr
 unsigned pack_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
-uint64_t rows_pack_records(const uint64_t* mask, int64_t M, uint4* rec, hipStream_t s) {
+uint64_t rows_pack_records(const uint64_t* mask, int64_t M, uint4* rec, int* max_kept, hipStream_t s) {
+  *max_kept = 0;
   if (M == 0) return 0;
   uint64_t* cnt = dev_alloc_n<uint64_t>((size_t)M);
   uint64_t* off = nullptr;
+  unsigned* maxc = nullptr;
   void* tmp = nullptr;
   uint64_t last[2] = {0, 0};
+  unsigned mx = 0;
   try {
     off = dev_alloc_n<uint64_t>((size_t)M);
-    hipLaunchKernelGGL(k_pack_count, dim3(pack_blocks(M)), dim3(256), 0, s, mask, M, cnt);
+    maxc = dev_alloc_n<unsigned>(1);
+    PF_HIP(hipMemsetAsync(maxc, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_pack_count, dim3(pack_blocks(M)), dim3(256), 0, s, mask, M, cnt, maxc);
     PF_HIP(hipGetLastError());
     size_t tb = 0;
     PF_HIP(rocprim::exclusive_scan(nullptr, tb, cnt, off, (uint64_t)0, (size_t)M, rocprim::plus<uint64_t>(), s));
@@ -180,24 +202,30 @@ uint64_t rows_pack_records(const uint64_t* mask, int64_t M, uint4* rec, hipStrea
     PF_HIP(hipGetLastError());
     PF_HIP(hipMemcpyAsync(&last[0], off + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     PF_HIP(hipMemcpyAsync(&last[1], cnt + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipMemcpyAsync(&mx, maxc, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     PF_HIP(hipStreamSynchronize(s));
   } catch (...) {
     (void)hipStreamSynchronize(s);
     dev_free(cnt);
     if (off) dev_free(off);
+    if (maxc) dev_free(maxc);
     if (tmp) dev_free(tmp);
     throw;
   }
   dev_free(cnt);
   dev_free(off);
+  dev_free(maxc);
   dev_free(tmp);
+  *max_kept = (int)mx;
   return last[0] + last[1];
 }
 
 template <typename R>
-void rows_pack_values(const cpx<R>* dense, const uint4* rec, int64_t M, int KK, cpx<R>* packed, hipStream_t s) {
+void rows_pack_values(const cpx<R>* dense, const uint4* rec, int64_t M, int KK, int Ku, cpx<R>* packed,
+                      uint8_t* pos, hipStream_t s) {
+  if (Ku > 15 || KK > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
   if (M == 0) return;
-  hipLaunchKernelGGL((k_pack_vals<R>), dim3(pack_blocks(M * KK)), dim3(256), 0, s, dense, rec, M, KK, packed);
+  hipLaunchKernelGGL((k_pack_vals<R>), dim3(pack_blocks(M * KK)), dim3(256), 0, s, dense, rec, M, KK, Ku, packed, pos);
   PF_HIP(hipGetLastError());
 }
 template <typename R>
@@ -206,8 +234,10 @@ void rows_unpack(const cpx<R>* packed, const uint4* rec, int64_t M, int KK, cpx<
   hipLaunchKernelGGL((k_unpack<R>), dim3(pack_blocks(M * KK)), dim3(256), 0, s, packed, rec, M, KK, dense);
   PF_HIP(hipGetLastError());
 }
-template void rows_pack_values<float>(const cpx<float>*, const uint4*, int64_t, int, cpx<float>*, hipStream_t);
-template void rows_pack_values<double>(const cpx<double>*, const uint4*, int64_t, int, cpx<double>*, hipStream_t);
+template void rows_pack_values<float>(const cpx<float>*, const uint4*, int64_t, int, int, cpx<float>*, uint8_t*,
+                                      hipStream_t);
+template void rows_pack_values<double>(const cpx<double>*, const uint4*, int64_t, int, int, cpx<double>*, uint8_t*,
+                                       hipStream_t);
 template void rows_unpack<float>(const cpx<float>*, const uint4*, int64_t, int, cpx<float>*, hipStream_t);
 template void rows_unpack<double>(const cpx<double>*, const uint4*, int64_t, int, cpx<double>*, hipStream_t);
 
@@ -295,23 +325,22 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRo
   }
 }
 
-// Degridding with kept-entry rows (K^2 <= 64): the groups of k_degrid_wrows, each lane's taps k = r,
-// r + 8, ... in the same order with the same multiply-adds and the zero entries skipped, so the sums
-// equal the dense rows' bit for bit.  A visibility's taps wait on two dependent loads (its record, then
-// its kept values), so the wave walks its steps of 8 visibilities as a pipeline: the records two steps
-// ahead, the values one step ahead, in two register sets taken in turn (every load unconditional: the
-// record clamped to the slot's last visibility, and a dropped tap reading at most one past its row's
-// last kept value, which exists: the kept values carry one spare entry).  The lane's 8 tap cells, bit
-// masks and LDS offsets are constants; the taps' LDS reads are unconditional (a tap past the footprint
-// reads cell 0) so that they issue together, and a dropped tap is a select.  PERM: the output position
-// in the caller's order is loaded with the record (a load issued at the store would wait for every load
-// in flight).
-template <typename R, bool PERM>
+// Degridding with kept-entry rows (K^2 <= 64): 8 lanes per visibility as in k_degrid_wrows, but the
+// lanes take the row's kept values in their stored order: lane r the SL consecutive slots from SL r (SL
+// per lane: 2, 4 or 8, the fewest that hold every row's kept taps), each with its tap's position byte, so
+// that each load instruction reads 8 short contiguous runs instead of one tap of 8 rows.  The group's 8
+// partial sums meet in a fixed butterfly: deterministic, and equal to the dense rows' sums up to the
+// order of the additions.  A visibility's slots wait on two dependent loads (its record, then its
+// values), so the wave walks its steps of 8 visibilities as a pipeline: the records two steps ahead, the
+// values one step ahead, in two register sets taken in turn (every load unconditional: the record
+// clamped to the slot's last visibility, and a slot past the row reads the next row's value or a spare
+// entry, and is not used).  PERM: the output position in the caller's order is loaded with the record (a
+// load issued at the store would wait for every load in flight).
+template <typename R, bool PERM, int SL>
 __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows_packed(int Nu, int Nv, WRows<R> w,
                                                                     const int32_t* __restrict__ perm, TileLists tl,
                                                                     const cpx<R>* __restrict__ grid,
                                                                     cpx<R>* __restrict__ y, R scl) {
-  constexpr int NQ = 64 / WG_LANES;  // taps per lane
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const uint4 sl = tl.slot_own[blockIdx.x];
@@ -327,18 +356,10 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows_packed(int Nu, int 
   const uint32_t end = sl.z, last = sl.z - 1;
   uint32_t iw = sl.y + (uint32_t)wave * (64 / WG_LANES);  // the wave's first visibility of the step
   if (iw >= end) return;
-  // the lane's taps k = r + 8 q: the bits below k within its 32-bit half of the mask (q and q + 4 alike),
-  // and the tap's LDS offset from the footprint origin (cell 0 past the footprint)
-  const int KK = w.Ku * w.Kv;
-  uint32_t below[NQ / 2];
-  int ofs[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ / 2; ++q) below[q] = (1u << (r + WG_LANES * q)) - 1u;
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int k = r + WG_LANES * q;
-    ofs[q] = k < KK ? (k / w.Ku) * W + (k - (k / w.Ku) * w.Ku) : 0;
-  }
+  struct Slots {
+    cpx<R> a[SL];
+    uint8_t p[SL];
+  };
   auto records = [&](uint32_t i0, uint4& rc, int2& kw, uint32_t& pos) {
     const uint32_t i = min(i0 + (uint32_t)grp, last);
     kw = w.kw[i];
@@ -346,26 +367,25 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows_packed(int Nu, int 
     if constexpr (PERM) pos = (uint32_t)perm[i];
     else pos = i;
   };
-  auto values = [&](const uint4& rc, cpx<R> (&a)[NQ]) {
-    const cpx<R>* p = w.vals + (int64_t)(((uint64_t)rc.w << 32) | rc.z);
-    const uint32_t nlo = (uint32_t)__popc(rc.x);
+  auto values = [&](const uint4& rc, Slots& v) {
+    const int64_t j = rec_off(rc) + SL * r;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const uint32_t idx = q < NQ / 2 ? (uint32_t)__popc(rc.x & below[q]) : (uint32_t)__popc(rc.y & below[q - NQ / 2]) + nlo;
-      a[q] = p[idx];
-    }
+    for (int k = 0; k < SL; ++k) v.a[k] = w.vals[j + k];
+#pragma unroll
+    for (int k = 0; k < SL; ++k) v.p[k] = w.pos[j + k];
   };
-  auto sum = [&](uint32_t i0, uint32_t mlo, uint32_t mhi, int base, uint32_t pos, const cpx<R> (&a)[NQ]) {
-    cpx<R> g[NQ];
+  auto sum = [&](uint32_t i0, const uint4& rc, int base, uint32_t pos, const Slots& v) {
+    const int nnz = __popc(rc.x) + __popc(rc.y);
+    cpx<R> g[SL];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) g[q] = lds[base + ofs[q]];
+    for (int k = 0; k < SL; ++k) g[k] = lds[base + (int)(v.p[k] >> 4) * W + (int)(v.p[k] & 15u)];
     cpx<R> acc = mk<R>(R(0), R(0));
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const uint32_t bit = q < NQ / 2 ? (mlo & (below[q] + 1u)) : (mhi & (below[q - NQ / 2] + 1u));
-      const cpx<R> t = mac(acc, a[q], g[q]);
-      acc.x = bit ? t.x : acc.x;  // per component: a select of the pair would go through memory
-      acc.y = bit ? t.y : acc.y;
+    for (int k = 0; k < SL; ++k) {
+      const cpx<R> t = mac(acc, v.a[k], g[k]);
+      const bool on = SL * r + k < nnz;
+      acc.x = on ? t.x : acc.x;  // per component: a select of the pair would go through memory
+      acc.y = on ? t.y : acc.y;
     }
 #pragma unroll
     for (int m = WG_LANES / 2; m >= 1; m >>= 1) {
@@ -378,26 +398,28 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows_packed(int Nu, int 
   uint4 rA, rB;
   int2 kA, kB;
   uint32_t pA, pB;
-  cpx<R> aA[NQ], aB[NQ];
+  Slots aA, aB;
   records(iw, rA, kA, pA);
   records(iw + STEP, rB, kB, pB);
   values(rA, aA);
   for (;;) {
     {  // step in set A: its record, origin and position taken before the set is reloaded two steps ahead
-      const uint32_t mlo = rA.x, mhi = rA.y, pos = pA;
+      const uint4 rc = rA;
+      const uint32_t pos = pA;
       const int base = base_of(kA);
       records(iw + 2 * STEP, rA, kA, pA);
       values(rB, aB);
-      sum(iw, mlo, mhi, base, pos, aA);
+      sum(iw, rc, base, pos, aA);
     }
     iw += STEP;
     if (iw >= end) break;
     {  // the same in set B
-      const uint32_t mlo = rB.x, mhi = rB.y, pos = pB;
+      const uint4 rc = rB;
+      const uint32_t pos = pB;
       const int base = base_of(kB);
       records(iw + 2 * STEP, rB, kB, pB);
       values(rA, aA);
-      sum(iw, mlo, mhi, base, pos, aB);
+      sum(iw, rc, base, pos, aB);
     }
     iw += STEP;
     if (iw >= end) break;
@@ -641,12 +663,22 @@ void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const 
   ProfScope prof("degrid_wrows", s);
   if (w.rec) {
     if (w.Ku * w.Kv > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
-    if (perm)
-      hipLaunchKernelGGL((k_degrid_wrows_packed<R, true>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w,
-                         perm, t, grid, y, scale);
-    else
-      hipLaunchKernelGGL((k_degrid_wrows_packed<R, false>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv,
-                         w, perm, t, grid, y, scale);
+#define PF_DWP(PP, SS)                                                                                          \
+  hipLaunchKernelGGL((k_degrid_wrows_packed<R, PP, SS>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, \
+                     w, perm, t, grid, y, scale)
+    if (w.slots == 2) {
+      if (perm) PF_DWP(true, 2);
+      else PF_DWP(false, 2);
+    } else if (w.slots == 4) {
+      if (perm) PF_DWP(true, 4);
+      else PF_DWP(false, 4);
+    } else if (w.slots == 8) {
+      if (perm) PF_DWP(true, 8);
+      else PF_DWP(false, 8);
+    } else {
+      throw Error(PF_ERR_ARG, "kept-entry degridding: 2, 4 or 8 slots per lane");
+    }
+#undef PF_DWP
   } else {
     hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, grid,
                        y, scale);
