@@ -314,7 +314,9 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * "w_rows_packed" (chirp w-projection rows of at most 8 x 8 taps: 1 = stored as their kept entries, a
  * 64-bit keep mask and offset per row and the kept values, the default; 0 = dense rows, rebuilt from the
  * kept entries on first use; results are bit-identical either way); "w_grid_depth" (gridding with kept-entry
- * rows: the entries whose tap values are requested ahead of their use, 8, 16 (the default) or 32).
+ * rows: the entries whose tap values are requested ahead of their use, 8, 16 (the default) or 32); read-only
+ * "w_degrid_slots" (kept-entry rows: the kept values each degridding lane takes, 2, 4 or 8, the fewest that
+ * hold the operator's largest row over 8 lanes; else 0).
  * Results do not depend on grid_kernel or fft_columns beyond floating-point summation order. */
 /* pf_op_set_norm, pf_op_set_tuning and pf_op_autotune change what an apply launches: a solver that
  * captured the operator's launches into a graph captures them again on its next run. */

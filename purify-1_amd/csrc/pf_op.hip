@@ -1920,6 +1920,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
     *value = op->w_grid_depth;
     return PF_OK;
   }
+  if (k == "w_degrid_slots") {
+    *value = op->d_wpack ? op->w_slots : 0;
+    return PF_OK;
+  }
   if (k == "perm_bucket") {
     *value = op->vperm.two_pass ? op->vperm.S : 0;
     return PF_OK;

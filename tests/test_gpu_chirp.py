@@ -182,3 +182,32 @@ def test_kept_entry_rows_tuning_errors():
         lt.op.set_tuning("w_rows_packed", 1)
     with pytest.raises(PurifyError, match="0 or 1"):
         lt.op.set_tuning("w_rows_packed", 2)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_kept_entry_degridding_slot_counts(precision):
+    # the degridding lanes take the kept values in stored order, as many slots each as the largest row
+    # needs over 8 lanes (2, 4 or 8): rows pruned hard keep at most 16 taps, rows at energy 1 every
+    # non-zero tap of their 6 x 6 or 8 x 8 footprint; every variant must equal the dense rows to rounding
+    # and repeat bit for bit
+    n, M = 32, 2000
+    cell = 3600.0 / n
+    u, v = make_uv(1401, M, n)
+    w = np.random.default_rng(1402).uniform(-200.0, 200.0, M)
+    wt = utilities.synth_cnormal(1403, M) + 1.0
+    dt = torch.complex128 if precision == "c128" else torch.complex64
+    tol = {"c128": 1e-14, "c64": 1e-6}[precision]
+    x = utilities.synth_image(1405, n, n, 5)
+    seen = set()
+    for J, e_w in ((4, 0.9), (4, 0.999), (4, 1.0), (6, 0.999), (6, 0.9999), (6, 1.0)):
+        lt = chirp_op(u, v, w, n, J, cell, 0.999, e_w, precision, wt)
+        assert lt.op.get_tuning("w_footprint") <= 8
+        slots = lt.op.get_tuning("w_degrid_slots")
+        seen.add(slots)
+        g = torch.from_numpy(utilities.synth_cnormal(1404, (2 * n) ** 2)).to(dt).cuda()
+        yg, yx = lt.op.apply_G(g).cpu().numpy(), lt * x
+        assert np.array_equal(lt.op.apply_G(g).cpu().numpy(), yg)
+        lt.op.set_tuning("w_rows_packed", 0)
+        assert rel(yg, lt.op.apply_G(g).cpu().numpy()) < tol, (J, e_w, slots)
+        assert rel(yx, lt * x) < tol, (J, e_w, slots)
+    assert seen == {2, 4, 8}, seen
