@@ -325,6 +325,87 @@ __global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows(int Nu, int Nv, WRo
   }
 }
 
+// Degridding with dense rows of up to 8 NT taps: k_degrid_wrows's groups and sums (lane r the taps r,
+// r + 8, ..., the same multiply-adds in the same order: bit-identical), with each lane's NT taps a
+// compile-time count, so that all of a step's row loads are in flight together; the steps of 8
+// visibilities run as a pipeline, the next step's row segments, origins and output positions loaded
+// while this step sums, in two register sets taken in turn.  Every load is unconditional (taps past the
+// row read its last tap, and are not used; visibilities past the slot read the slot's last one).
+template <typename R, bool PERM, int NT>
+__global__ void __launch_bounds__(WD_THREADS) k_degrid_wrows_dense(int Nu, int Nv, WRows<R> w,
+                                                                   const int32_t* __restrict__ perm, TileLists tl,
+                                                                   const cpx<R>* __restrict__ grid,
+                                                                   cpx<R>* __restrict__ y, R scl) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const uint4 sl = tl.slot_own[blockIdx.x];
+  const int tile = (int)sl.x;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * tl.TU, ty0 = tv * tl.TV;
+  const int W = tl.TU + w.Ku - 1, H = tl.TV + w.Kv - 1;
+  stage_halo<R>(lds, grid, Nu, Nv, tx0, ty0, W, H);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr uint32_t STEP = (WD_THREADS / 64) * (64 / WG_LANES);
+  const int grp = lane / WG_LANES, r = lane % WG_LANES;
+  const uint32_t end = sl.z, last = sl.z - 1;
+  uint32_t iw = sl.y + (uint32_t)wave * (64 / WG_LANES);
+  if (iw >= end) return;
+  const int KK = w.Ku * w.Kv;
+  int ofs[NT];  // LDS offsets of this lane's taps from the footprint origin (the last tap's past the row)
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int k = min(r + WG_LANES * t, KK - 1);
+    ofs[t] = (k / w.Ku) * W + (k - (k / w.Ku) * w.Ku);
+  }
+  struct Step {
+    cpx<R> a[NT];
+    int2 kw;
+    uint32_t pos;
+  };
+  auto load = [&](uint32_t i0, Step& st) {
+    const uint32_t i = min(i0 + (uint32_t)grp, last);
+    const cpx<R>* row = w.vals + (int64_t)i * KK;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) st.a[t] = row[min(r + WG_LANES * t, KK - 1)];
+    st.kw = w.kw[i];
+    if constexpr (PERM) st.pos = (uint32_t)perm[i];
+    else st.pos = i;
+  };
+  auto sum = [&](uint32_t i0, const Step& st) {
+    const int base = (wrapw(st.kw.y + 1, Nv) - ty0) * W + (wrapw(st.kw.x + 1, Nu) - tx0);
+    cpx<R> g[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) g[t] = lds[base + ofs[t]];
+    cpx<R> acc = mk<R>(R(0), R(0));
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const cpx<R> m = mac(acc, st.a[t], g[t]);
+      const bool on = r + WG_LANES * t < KK;
+      acc.x = on ? m.x : acc.x;  // per component: a select of the pair would go through memory
+      acc.y = on ? m.y : acc.y;
+    }
+#pragma unroll
+    for (int m = WG_LANES / 2; m >= 1; m >>= 1) {
+      acc.x += __shfl_xor(acc.x, m, 64);
+      acc.y += __shfl_xor(acc.y, m, 64);
+    }
+    if (r == 0 && i0 + (uint32_t)grp < end) y[st.pos] = scale(acc, scl);
+  };
+  Step sA, sB;
+  load(iw, sA);
+  for (;;) {
+    load(iw + STEP, sB);
+    sum(iw, sA);
+    iw += STEP;
+    if (iw >= end) break;
+    load(iw + STEP, sA);
+    sum(iw, sB);
+    iw += STEP;
+    if (iw >= end) break;
+  }
+}
+
 // Degridding with kept-entry rows (K^2 <= 64): 8 lanes per visibility as in k_degrid_wrows, but the
 // lanes take the row's kept values in their stored order: lane r the SL consecutive slots from SL r (SL
 // per lane: 2, 4 or 8, the fewest that hold every row's kept taps), each with its tap's position byte, so
@@ -634,6 +715,107 @@ __global__ void __launch_bounds__(64) k_grid_wrows_packed(int Nu, int Nv, WRows<
   }
 }
 
+// Gridding with dense rows of up to 64 TPL taps: k_grid_wrows's walk (one wave per tile slot, lane l the
+// taps l + 64 t, the same multiply-adds in list order: bit-identical) in k_grid_wrows_packed's form: each
+// 64-entry batch fully unrolled with every load unconditional (taps past the row read its last tap, and
+// are not used), a D-entry ring of row segments running across the batch boundary, the next batch's
+// records requested a quarter of the way in, entries broadcast by readlane, and the footprint origin
+// relative to the tile computed once per batch.
+template <typename R, int D, int TPL>
+__global__ void __launch_bounds__(64) k_grid_wrows_dense(int Nu, int Nv, WRows<R> w, TileLists tl,
+                                                         const cpx<R>* __restrict__ ys, cpx<R>* __restrict__ grid,
+                                                         R scl) {
+  static_assert(64 % D == 0 && D <= 32, "the ring refills from the next batch after 64 - D entries");
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* acc = reinterpret_cast<cpx<R>*>(smem_raw);
+  const uint4 sl = tl.slot[blockIdx.x];
+  const int tile = (int)sl.x;
+  const int TU = tl.TU, TV = tl.TV;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * TU, ty0 = tv * TV;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < TU * TV; e += 64) acc[e] = mk<R>(R(0), R(0));
+  __syncthreads();
+  const uint32_t beg = sl.y, end = sl.z;
+  if (beg < end) {
+    const int KK = w.Ku * w.Kv;
+    // this lane's taps: row and column offsets, LDS offsets from the footprint origin's cell, and the
+    // index it reads (the row's last tap past the row)
+    int dv[TPL], du[TPL], lofs[TPL], kix[TPL];
+    bool has[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      const int k = lane + 64 * t;
+      has[t] = k < KK;
+      kix[t] = min(k, KK - 1);
+      dv[t] = kix[t] / w.Ku;
+      du[t] = kix[t] - dv[t] * w.Ku;
+      lofs[t] = (dv[t] * TU + du[t]) * (int)sizeof(cpx<R>);
+    }
+    auto rel16 = [](int a, int n) {  // a wrapped into (-n/2, n/2], clamped to 16 bits
+      const int r = ((a + n / 2) & (n - 1)) - n / 2;
+      return (uint32_t)(uint16_t)(int16_t)max(-32768, min(32767, r));
+    };
+    struct Ent {
+      uint32_t i, xy;
+      cpx<R> s;
+    };
+    auto load_ent = [&](uint32_t i) {
+      Ent e;
+      const int2 k = w.kw[i];
+      e.i = i;
+      e.xy = rel16(k.x + 1 - tx0, Nu) | (rel16(k.y + 1 - ty0, Nv) << 16);
+      e.s = scale(ys[i], scl);
+      return e;
+    };
+    auto fetch = [&](const Ent& e, int j, cpx<R> (&val)[TPL]) {
+      const cpx<R>* row = w.vals + (int64_t)rl(e.i, j) * KK;
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) val[t] = row[kix[t]];
+    };
+    Ent cur = load_ent(tl.list[min(beg + (uint32_t)lane, end - 1)]);
+    cpx<R> ring[D][TPL];
+#pragma unroll
+    for (int d = 0; d < D; ++d) fetch(cur, d, ring[d]);
+    for (uint32_t b0 = beg; b0 < end; b0 += 64) {
+      const uint32_t nb = b0 + 64 < end ? b0 + 64 : b0;
+      const uint32_t nidx = tl.list[min(nb + (uint32_t)lane, end - 1)];
+      Ent nxt;
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        if (j == 16) nxt = load_ent(nidx);
+        cpx<R> a[TPL];
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) a[t] = ring[j % D][t];
+        if (j + D < 64) fetch(cur, j + D, ring[j % D]);
+        else fetch(nxt, j + D - 64, ring[j % D]);
+        const bool live = b0 + (uint32_t)j < end;
+        const uint32_t xy = rl(cur.xy, j);
+        const int xs = (int)(int16_t)(xy & 0xffffu), ysh = (int)(int16_t)(xy >> 16);
+        const cpx<R> sv = mk<R>(rl(cur.s.x, j), rl(cur.s.y, j));
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+          if (live && has[t] && (unsigned)(xs + du[t]) < (unsigned)TU && (unsigned)(ysh + dv[t]) < (unsigned)TV) {
+            cpx<R>* c = reinterpret_cast<cpx<R>*>(smem_raw + (ysh * TU + xs) * (int)sizeof(cpx<R>) + lofs[t]);
+            *c = mac_conj(*c, a[t], sv);
+          }
+        }
+      }
+      cur = nxt;
+    }
+  }
+  __syncthreads();
+  if (sl.w == PART_NONE) {
+    for (int e = lane; e < TU * TV; e += 64) {
+      const int ly = e / TU, lx = e - ly * TU;
+      grid[(int64_t)(ty0 + ly) * Nu + tx0 + lx] = acc[e];
+    }
+  } else {
+    cpx<R>* part = static_cast<cpx<R>*>(tl.part) + (int64_t)sl.w * TU * TV;
+    for (int e = lane; e < TU * TV; e += 64) part[e] = acc[e];
+  }
+}
+
 // The split tiles: each cell sums its parts' partial sums in part order (the list order of the parts'
 // ranges), so the result is the same on every run.
 template <typename R>
@@ -680,8 +862,24 @@ void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const 
     }
 #undef PF_DWP
   } else {
-    hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, grid,
-                       y, scale);
+    const int KK = w.Ku * w.Kv;
+#define PF_DWD(PP, NN)                                                                                        \
+  hipLaunchKernelGGL((k_degrid_wrows_dense<R, PP, NN>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, \
+                     w, perm, t, grid, y, scale)
+    if (KK <= 64) {
+      if (perm) PF_DWD(true, 8);
+      else PF_DWD(false, 8);
+    } else if (KK <= 128) {
+      if (perm) PF_DWD(true, 16);
+      else PF_DWD(false, 16);
+    } else if (KK <= 192) {
+      if (perm) PF_DWD(true, 24);
+      else PF_DWD(false, 24);
+    } else {
+      hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t,
+                         grid, y, scale);
+    }
+#undef PF_DWD
   }
   PF_HIP(hipGetLastError());
 }
@@ -708,9 +906,10 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
     }
 #undef PF_GWP
   } else if (KK <= 64)
-    hipLaunchKernelGGL((k_grid_wrows<R, 1, 8>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+    hipLaunchKernelGGL((k_grid_wrows_dense<R, 16, 1>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else if (KK <= 128)
-    hipLaunchKernelGGL((k_grid_wrows<R, 2, 8>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+    hipLaunchKernelGGL((k_grid_wrows_dense<R, sizeof(R) == 4 ? 16 : 8, 2>), blocks, dim3(64), shm, s, Nu, Nv, w, t,
+                       ys, grid, scale);
   else if (KK <= 256)
     hipLaunchKernelGGL((k_grid_wrows<R, 4, 4>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else if (KK <= 576)
