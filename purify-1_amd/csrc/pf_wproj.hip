@@ -836,18 +836,26 @@ __global__ void __launch_bounds__(256) k_combine_parts(int Nu, TileLists tl, cpx
   }
 }
 
+// A launch with dynamic LDS beyond the default 64 KiB allowed for the kernel first.
+template <typename... P, typename... A>
+static void launch_lds(void (*kernel)(P...), dim3 g, dim3 b, size_t shm, hipStream_t s, A... args) {
+  allow_lds((const void*)kernel, shm);
+  hipLaunchKernelGGL(kernel, g, b, shm, s, args...);
+}
+
 template <typename R>
 void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const TileLists& t, const cpx<R>* grid,
                   cpx<R>* y, R scale, hipStream_t s) {
   if (t.n_own == 0) return;
+  // the staged tile and halo: up to the CU's 160 KiB of LDS
   const size_t shm = sizeof(cpx<R>) * (size_t)(t.TU + w.Ku - 1) * (t.TV + w.Kv - 1);
-  if (shm > 65536) throw Error(PF_ERR_UNSUPPORTED, "w-projection degridding tile and halo exceed 64 KiB of LDS");
+  if (shm > 160 * 1024) throw Error(PF_ERR_UNSUPPORTED, "w-projection degridding tile and halo exceed 160 KiB of LDS");
   ProfScope prof("degrid_wrows", s);
   if (w.rec) {
     if (w.Ku * w.Kv > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
-#define PF_DWP(PP, SS)                                                                                          \
-  hipLaunchKernelGGL((k_degrid_wrows_packed<R, PP, SS>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, \
-                     w, perm, t, grid, y, scale)
+#define PF_DWP(PP, SS) \
+  launch_lds(k_degrid_wrows_packed<R, PP, SS>, dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, \
+             grid, y, scale)
     if (w.slots == 2) {
       if (perm) PF_DWP(true, 2);
       else PF_DWP(false, 2);
@@ -863,9 +871,9 @@ void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const 
 #undef PF_DWP
   } else {
     const int KK = w.Ku * w.Kv;
-#define PF_DWD(PP, NN)                                                                                        \
-  hipLaunchKernelGGL((k_degrid_wrows_dense<R, PP, NN>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, \
-                     w, perm, t, grid, y, scale)
+#define PF_DWD(PP, NN) \
+  launch_lds(k_degrid_wrows_dense<R, PP, NN>, dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, \
+             grid, y, scale)
     if (KK <= 64) {
       if (perm) PF_DWD(true, 8);
       else PF_DWD(false, 8);
@@ -876,8 +884,8 @@ void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const 
       if (perm) PF_DWD(true, 24);
       else PF_DWD(false, 24);
     } else {
-      hipLaunchKernelGGL((k_degrid_wrows<R>), dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t,
-                         grid, y, scale);
+      launch_lds(k_degrid_wrows<R>, dim3((unsigned)t.n_own), dim3(WD_THREADS), shm, s, Nu, Nv, w, perm, t, grid, y,
+                 scale);
     }
 #undef PF_DWD
   }
@@ -887,14 +895,15 @@ void degrid_wrows(int Nu, int Nv, const WRows<R>& w, const int32_t* perm, const 
 template <typename R>
 void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx<R>* ys, cpx<R>* grid, R scale,
                 hipStream_t s) {
+  // the tile's accumulator: up to the CU's 160 KiB of LDS
   const size_t shm = sizeof(cpx<R>) * (size_t)t.TU * t.TV;
+  if (shm > 160 * 1024) throw Error(PF_ERR_UNSUPPORTED, "w-projection gridding tile exceeds 160 KiB of LDS");
   const int KK = w.Ku * w.Kv;
   const dim3 blocks((unsigned)t.n_slot);
   ProfScope prof("grid_wrows", s);
   if (w.rec) {
     if (KK > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
-#define PF_GWP(DD, O64) \
-  hipLaunchKernelGGL((k_grid_wrows_packed<R, DD, O64>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale)
+#define PF_GWP(DD, O64) launch_lds(k_grid_wrows_packed<R, DD, O64>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale)
     if (w.off64) {
       if (w.depth == 32) PF_GWP(32, true);
       else if (w.depth == 8) PF_GWP(8, true);
@@ -906,16 +915,16 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
     }
 #undef PF_GWP
   } else if (KK <= 64)
-    hipLaunchKernelGGL((k_grid_wrows_dense<R, 16, 1>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+    launch_lds(k_grid_wrows_dense<R, 16, 1>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else if (KK <= 128)
-    hipLaunchKernelGGL((k_grid_wrows_dense<R, sizeof(R) == 4 ? 16 : 8, 2>), blocks, dim3(64), shm, s, Nu, Nv, w, t,
-                       ys, grid, scale);
+    launch_lds(k_grid_wrows_dense<R, sizeof(R) == 4 ? 16 : 8, 2>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid,
+               scale);
   else if (KK <= 256)
-    hipLaunchKernelGGL((k_grid_wrows<R, 4, 4>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+    launch_lds(k_grid_wrows<R, 4, 4>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else if (KK <= 576)
-    hipLaunchKernelGGL((k_grid_wrows<R, 9, 2>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+    launch_lds(k_grid_wrows<R, 9, 2>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else if (KK <= 1024)
-    hipLaunchKernelGGL((k_grid_wrows<R, 16, 1>), blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
+    launch_lds(k_grid_wrows<R, 16, 1>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else
     throw Error(PF_ERR_UNSUPPORTED, "w-projection rows above 1024 taps");
   PF_HIP(hipGetLastError());

@@ -211,3 +211,61 @@ def test_kept_entry_degridding_slot_counts(precision):
         assert rel(yg, lt.op.apply_G(g).cpu().numpy()) < tol, (J, e_w, slots)
         assert rel(yx, lt * x) < tol, (J, e_w, slots)
     assert seen == {2, 4, 8}, seen
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("M", [1, 3, 65])
+def test_kept_entry_rows_with_few_visibilities(precision, M):
+    # a handful of rows: one owner tile, partial wave steps and batches, the last row's spare entries
+    n = 32
+    cell = 3600.0 / n
+    u, v = make_uv(1501 + M, M, n)
+    w = np.random.default_rng(1502 + M).uniform(-200.0, 200.0, M)
+    lt = chirp_op(u, v, w, n, 6, cell, 0.999, 0.999, precision)
+    assert lt.op.get_tuning("w_rows_packed") == 1
+    x = utilities.synth_image(1503, n, n, 3)
+    y = utilities.synth_cnormal(1504, M)
+    f, a = lt * x, lt.adjoint() * y
+    lt.op.set_tuning("w_rows_packed", 0)
+    tol = {"c128": 1e-14, "c64": 1e-6}[precision]
+    assert rel(f, lt * x) < tol
+    assert np.array_equal(a, lt.adjoint() * y)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("tile", [(16, 16), (64, 32), (64, 64)])
+def test_kept_entry_rows_on_other_tile_shapes(precision, tile):
+    # the kept-entry kernels stage and accumulate whole tiles: other tile shapes (owner tiles, halos,
+    # list duplication) give the dense rows' results too
+    n, M = 64, 20_000
+    cell = 3600.0 / n
+    u, v = make_uv(1601, M, n)
+    w = np.random.default_rng(1602).uniform(-200.0, 200.0, M)
+    lt = measurementoperator.init_degrid_operator_2d(u, v, w, None, n, n, 2.0, 0, 1e-4, "kb", 6, 6, True, 6, cell,
+                                                     cell, precision=precision, w_projection="chirp",
+                                                     energy_chirp=0.999, energy_wproj=0.999, tile=tile)
+    assert (lt.op.get_tuning("tile_u"), lt.op.get_tuning("tile_v")) == tile
+    assert lt.op.get_tuning("w_rows_packed") == 1
+    x = utilities.synth_image(1603, n, n, 5)
+    y = utilities.synth_cnormal(1604, M)
+    f, a = lt * x, lt.adjoint() * y
+    lt.op.set_tuning("w_rows_packed", 0)
+    assert rel(f, lt * x) < {"c128": 1e-14, "c64": 1e-6}[precision]
+    assert np.array_equal(a, lt.adjoint() * y)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_kept_entry_gridding_with_split_tiles(precision):
+    # visibilities crowded into the centre: the densest tiles' lists are cut into parts summed in a fixed
+    # order afterwards, by the kept-entry kernel as by the dense one
+    n, M = 32, 30_000
+    cell = 3600.0 / n
+    u, v = make_uv(1701, M, n, sigma=0.3)
+    w = np.random.default_rng(1702).uniform(-200.0, 200.0, M)
+    lt = chirp_op(u, v, w, n, 6, cell, 0.999, 0.999, precision)
+    assert lt.op.get_tuning("split_tiles") > 0
+    y = utilities.synth_cnormal(1703, M)
+    a = lt.adjoint() * y
+    assert np.array_equal(lt.adjoint() * y, a)
+    lt.op.set_tuning("w_rows_packed", 0)
+    assert np.array_equal(lt.adjoint() * y, a)
