@@ -325,7 +325,7 @@ int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
 /* Times the fixed-order gridding-kernel variants (residue-class lanes and matrix cores, not the LDS-atomic
  * one) on this operator (reps adjoint gridding passes each, HIP events on the calling thread's stream) and keeps the
  * fastest, then does the same for the two "fft_columns" modes (reps
- * forward + adjoint column passes each).  Writes the chosen gridding variant to *chosen if non-NULL;
+ * Phi + Phi^dagger pairs each, the best of three trials).  Writes the chosen gridding variant to *chosen if non-NULL;
  * the column mode is read back with pf_op_get_tuning.  Operators with w-projection rows or a given
  * gridding matrix have one gridding kernel: only the column mode is chosen. */
 int pf_op_autotune(pf_op *op, int32_t reps, int64_t *chosen);

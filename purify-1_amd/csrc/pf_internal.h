@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -269,9 +271,20 @@ struct VisPerm {
 // largest chunk: 128 KiB of LDS
 inline int perm_max_S(int esz) { return esz == 8 ? 16384 : 8192; }
 
-// Dynamic LDS above 64 KiB (gfx950 has 160 KiB per CU) must be opted into per kernel.
+// Dynamic LDS above 64 KiB (gfx950 has 160 KiB per CU) must be opted into per kernel.  The attribute is
+// set once per kernel and device (and raised when a launch needs more): a host call per launch would
+// stall a stream of short launches behind the host.
 inline void allow_lds(const void* fn, size_t shm) {
-  if (shm > 65536) PF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  if (shm <= 65536) return;
+  int dev = 0;
+  PF_HIP(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> set;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = set[{fn, dev}];
+  if (cur >= shm) return;
+  PF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  cur = shm;
 }
 
 // complex64 tap tables: all J taps of a visibility sit at offsets delta + J/2 - 1 - a of one fractional
