@@ -313,8 +313,14 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * that are summed in a fixed order afterwards, so that a dense tile does not form the launch's tail);
  * "w_rows_packed" (chirp w-projection rows of at most 8 x 8 taps: 1 = stored as their kept entries, a
  * 64-bit keep mask and offset per row and the kept values, the default; 0 = dense rows, rebuilt from the
- * kept entries on first use; results are bit-identical either way); "w_grid_depth" (gridding with kept-entry
- * rows: the entries whose tap values are requested ahead of their use, 8, 16 (the default) or 32); read-only
+ * kept entries on first use; results are bit-identical either way at one gridding entry per step, and equal
+ * to rounding at more); "w_grid_entries" (gridding with kept-entry rows: the list entries one wave step
+ * takes, 1, 2 or 4, each over 64 / entries lanes that take its kept values in stored order and with its own
+ * accumulator copy of the tile, the copies summed in order at the end; allowed while the operator's largest
+ * row keeps at most 64 / entries taps and the copies fit 160 KiB of LDS; the default is the most allowed
+ * within 16 KiB of copies; 0 without kept-entry rows); "w_grid_depth" (gridding with kept-entry
+ * rows at one entry per step: the entries whose tap values are requested ahead of their use, 8, 16 (the
+ * default) or 32); read-only
  * "w_degrid_slots" (kept-entry rows: the kept values each degridding lane takes, 2, 4 or 8, the fewest that
  * hold the operator's largest row over 8 lanes; else 0).
  * Results do not depend on grid_kernel or fft_columns beyond floating-point summation order. */
@@ -327,7 +333,8 @@ int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
  * fastest, then does the same for the two "fft_columns" modes (reps
  * Phi + Phi^dagger pairs each, the best of three trials).  Writes the chosen gridding variant to *chosen if non-NULL;
  * the column mode is read back with pf_op_get_tuning.  Operators with w-projection rows or a given
- * gridding matrix have one gridding kernel: only the column mode is chosen. */
+ * gridding matrix have one gridding kernel: only the column mode is chosen, and with kept-entry rows the
+ * gridding's "w_grid_entries" (reps gridding passes each). */
 int pf_op_autotune(pf_op *op, int32_t reps, int64_t *chosen);
 
 /* ---- per-kernel timing (no counterpart in the reference, whose operators log at construction only,

@@ -336,6 +336,7 @@ struct pf_op {
   void* d_wpack = nullptr;
   uint8_t* d_wpos = nullptr;  // each kept value's tap (jv << 4 | ju)
   int w_slots = 8;            // kept-entry degridding's slots per lane (the largest row's kept taps / 8)
+  int w_qe = 1, w_qe_max = 1; // kept-entry gridding's entries per step (tuning knob), and the largest allowed
   int w_packed = 0;
   int w_grid_depth = 16;      // tuning knob: kept-entry gridding's prefetch depth in entries
   int64_t nnz_w = 0;
@@ -783,6 +784,12 @@ void build(pf_op* op, const pf_op_desc& d) {
       int max_kept = 0;
       const uint64_t kept = rows_pack_records(mask, M, op->d_wrec, &max_kept, cstream());
       op->w_slots = max_kept <= 16 ? 2 : max_kept <= 32 ? 4 : 8;
+      // gridding entries per step: as many as the largest row allows (64 / QE lanes each), while the QE
+      // accumulator copies of a tile stay within 16 KiB of LDS (ten waves per CU: with 32 x 32 tiles two
+      // entries per step grid complex64 rows 9 % faster than one, and complex128 rows 30 % slower)
+      op->w_qe_max = max_kept <= 16 ? 4 : max_kept <= 32 ? 2 : 1;
+      op->w_qe = op->w_qe_max;
+      while (op->w_qe > 1 && (size_t)op->w_qe * op->tl.TU * op->tl.TV * op->esz() > 16384) op->w_qe /= 2;
       if ((int64_t)kept != op->nnz_w) throw Error(PF_ERR_STATE, "kept-entry count differs from the rows' mask bits");
       // the kept values and their taps, with ROWS_PACK_PAD spare (zero) entries past the last: read, and
       // not used, for the last rows' dropped taps and unused slots
@@ -1003,6 +1010,7 @@ WRows<R> wrows(const pf_op* op) {
     w.rec = op->d_wrec;
     w.pos = op->d_wpos;
     w.slots = op->w_slots;
+    w.qe = op->w_qe;
     w.depth = op->w_grid_depth;
     w.off64 = (uint64_t)op->nnz_w >= ((uint64_t)1 << 32);
   } else {
@@ -1837,6 +1845,15 @@ static int set_tuning_impl(pf_op* op, const char* key, int64_t value) {
     op->gf.poly = value ? op->d_poly : nullptr;
     return PF_OK;
   }
+  if (k == "w_grid_entries") {
+    if (value != 1 && value != 2 && value != 4) return fail(PF_ERR_ARG, "w_grid_entries must be 1, 2 or 4");
+    if (value > 1 && (!op->d_wpack || value > op->w_qe_max))
+      return fail(PF_ERR_UNSUPPORTED, "w_grid_entries above 1 needs kept-entry rows of at most 64 / w_grid_entries taps");
+    if ((size_t)value * op->tl.TU * op->tl.TV * op->esz() > 160 * 1024)
+      return fail(PF_ERR_UNSUPPORTED, "w_grid_entries copies of the tile exceed 160 KiB of LDS");
+    op->w_qe = (int)value;
+    return PF_OK;
+  }
   if (k == "w_grid_depth") {
     if (value != 8 && value != 16 && value != 32) return fail(PF_ERR_ARG, "w_grid_depth must be 8, 16 or 32");
     op->w_grid_depth = (int)value;
@@ -1922,6 +1939,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
     *value = op->w_grid_depth;
     return PF_OK;
   }
+  if (k == "w_grid_entries") {
+    *value = op->d_wpack ? op->w_qe : 0;
+    return PF_OK;
+  }
   if (k == "w_degrid_slots") {
     *value = op->d_wpack ? op->w_slots : 0;
     return PF_OK;
@@ -1940,7 +1961,8 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
 int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
   if (!op) return fail(PF_ERR_ARG, "null operator");
   return guarded([&] {
-    // projection rows and a given matrix have one gridding kernel each: only the column FFT is chosen
+    // projection rows and a given matrix have one gridding kernel each (kept-entry rows: its entries per
+    // step are chosen), then the column FFT
     const bool one_grid_kernel = op->w_term || op->g_csr;
     ensure_stage(op);
     hipStream_t s = cstream();
@@ -1979,6 +2001,36 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
       }
     }
     op->grid_variant = best_v;
+    // kept-entry rows: the gridding's entries per step, from one up to what the largest row and the LDS
+    // allow (every setting is fixed-order)
+    if (op->w_term && op->w_packed && op->w_qe_max > 1) {
+      float best_q = 1e30f;
+      int best_e = op->w_qe;
+      for (int q = 1; q <= op->w_qe_max; q *= 2) {
+        if ((size_t)q * op->tl.TU * op->tl.TV * op->esz() > 160 * 1024) break;
+        op->w_qe = q;
+        auto run = [&] {
+          if (op->prec == PF_C64)
+            grid_wrows<float>(op->Nu, op->Nv, wrows<float>(op), op->tl, (const cpx<float>*)op->d_vecy,
+                              (cpx<float>*)op->d_gridA, 1.0f, s);
+          else
+            grid_wrows<double>(op->Nu, op->Nv, wrows<double>(op), op->tl, (const cpx<double>*)op->d_vecy,
+                               (cpx<double>*)op->d_gridA, 1.0, s);
+        };
+        for (int k = 0; k < n; ++k) run();  // warm-up
+        PF_HIP(hipEventRecord(a, s));
+        for (int k = 0; k < n; ++k) run();
+        PF_HIP(hipEventRecord(b, s));
+        PF_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        PF_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms < best_q) {
+          best_q = ms;
+          best_e = q;
+        }
+      }
+      op->w_qe = best_e;
+    }
     // column FFT: one pass or two steps, timed over whole Phi + Phi^dagger pairs (a timing of the column
     // passes alone is at the mercy of the host's launch rate).  Both plans are built first and swapped in
     // (a rebuild between trials idles the GPU, and the pairs after it run slow); each trial warms up with

@@ -715,6 +715,115 @@ __global__ void __launch_bounds__(64) k_grid_wrows_packed(int Nu, int Nv, WRows<
   }
 }
 
+// Gridding with kept-entry rows, QE entries per step (2 or 4, while no row keeps more than 64 / QE taps):
+// the lanes take the kept values in stored order, 64 / QE lanes per entry (lane k of entry h the slot k,
+// with its tap byte), so that a step spends its lanes on kept taps only.  Entries of one step may share
+// cells, so entry h of each step adds into its own accumulator copy h of the tile; the copies are summed
+// in order at the end: deterministic, and equal to the one-entry kernel's sums up to the order of the
+// additions.  The batch's entries (index, origin, value, record) are loaded one per lane as in
+// k_grid_wrows_packed and handed to the step's lanes by ds_bpermute; the slots' values and tap bytes run
+// DS steps ahead in a ring across the batch boundary, every load unconditional (a slot past the row reads
+// the next rows' values or the spare entries, and is not used).
+template <typename R, int QE, bool OFF64>
+__global__ void __launch_bounds__(64) k_grid_wrows_kept(int Nu, int Nv, WRows<R> w, TileLists tl,
+                                                        const cpx<R>* __restrict__ ys, cpx<R>* __restrict__ grid,
+                                                        R scl) {
+  constexpr int LPE = 64 / QE;  // lanes (slots) per entry
+  constexpr int SPB = 64 / QE;  // steps per 64-entry batch
+  constexpr int DS = 16 / QE;   // steps in flight
+  static_assert(QE == 2 || QE == 4, "two or four entries per step");
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* acc = reinterpret_cast<cpx<R>*>(smem_raw);  // QE copies of the tile
+  const uint4 sl = tl.slot[blockIdx.x];
+  const int tile = (int)sl.x;
+  const int TU = tl.TU, TV = tl.TV, TT = TU * TV;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * TU, ty0 = tv * TV;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < QE * TT; e += 64) acc[e] = mk<R>(R(0), R(0));
+  __syncthreads();
+  const uint32_t beg = sl.y, end = sl.z;
+  if (beg < end) {
+    const int h = lane / LPE, k = lane % LPE;  // this lane's entry of a step and its slot
+    cpx<R>* my_acc = acc + h * TT;
+    auto rel16 = [](int a, int n) {  // a wrapped into (-n/2, n/2], clamped to 16 bits
+      const int r = ((a + n / 2) & (n - 1)) - n / 2;
+      return (uint32_t)(uint16_t)(int16_t)max(-32768, min(32767, r));
+    };
+    struct Ent {
+      uint32_t xy, nnz, olo, ohi;
+      cpx<R> s;
+    };
+    auto load_ent = [&](uint32_t i) {
+      Ent e;
+      const int2 kw = w.kw[i];
+      e.xy = rel16(kw.x + 1 - tx0, Nu) | (rel16(kw.y + 1 - ty0, Nv) << 16);
+      e.s = scale(ys[i], scl);
+      const uint4 r = w.rec[i];
+      e.nnz = (uint32_t)(__popc(r.x) + __popc(r.y));
+      e.olo = r.z;
+      e.ohi = r.w;
+      return e;
+    };
+    auto bp = [](uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); };
+    struct Slot {
+      cpx<R> a;
+      uint32_t p;   // tap byte
+      uint32_t on;  // the slot holds a kept tap
+    };
+    // step t of a batch: this lane's slot of entry QE t + h
+    auto fetch = [&](const Ent& e, int t, Slot& sv) {
+      const int src = QE * t + h;
+      uint64_t off = bp(e.olo, src);
+      if constexpr (OFF64) off |= (uint64_t)bp(e.ohi, src) << 32;
+      sv.on = (uint32_t)k < bp(e.nnz, src);
+      sv.a = w.vals[off + (uint32_t)k];
+      sv.p = w.pos[off + (uint32_t)k];
+    };
+    Ent cur = load_ent(tl.list[min(beg + (uint32_t)lane, end - 1)]);
+    Slot ring[DS];
+#pragma unroll
+    for (int d = 0; d < DS; ++d) fetch(cur, d, ring[d]);
+    for (uint32_t b0 = beg; b0 < end; b0 += 64) {
+      // the next batch (or this one again after the last: loaded, never used)
+      const uint32_t nb = b0 + 64 < end ? b0 + 64 : b0;
+      const uint32_t nidx = tl.list[min(nb + (uint32_t)lane, end - 1)];
+      Ent nxt;
+#pragma unroll
+      for (int t = 0; t < SPB; ++t) {
+        if (t == SPB / 4) nxt = load_ent(nidx);
+        const Slot sv = ring[t % DS];
+        if (t + DS < SPB) fetch(cur, t + DS, ring[t % DS]);
+        else fetch(nxt, t + DS - SPB, ring[t % DS]);
+        const int src = QE * t + h;
+        const uint32_t xy = bp(cur.xy, src);
+        const cpx<R> s = mk<R>(__shfl(cur.s.x, src, 64), __shfl(cur.s.y, src, 64));
+        const int x = (int)(int16_t)(xy & 0xffffu) + (int)(sv.p & 15u);
+        const int y = (int)(int16_t)(xy >> 16) + (int)(sv.p >> 4);
+        if (sv.on && b0 + (uint32_t)src < end && (unsigned)x < (unsigned)TU && (unsigned)y < (unsigned)TV) {
+          cpx<R>& c = my_acc[y * TU + x];
+          c = mac_conj(c, sv.a, s);
+        }
+      }
+      cur = nxt;
+    }
+  }
+  __syncthreads();
+  // the copies summed in order, into the grid or this part's partial sums
+  cpx<R>* out = sl.w == PART_NONE ? nullptr : static_cast<cpx<R>*>(tl.part) + (int64_t)sl.w * TT;
+  for (int e = lane; e < TT; e += 64) {
+    cpx<R> v = acc[e];
+#pragma unroll
+    for (int q = 1; q < QE; ++q) v = v + acc[q * TT + e];
+    if (out) {
+      out[e] = v;
+    } else {
+      const int ly = e / TU, lx = e - ly * TU;
+      grid[(int64_t)(ty0 + ly) * Nu + tx0 + lx] = v;
+    }
+  }
+}
+
 // Gridding with dense rows of up to 64 TPL taps: k_grid_wrows's walk (one wave per tile slot, lane l the
 // taps l + 64 t, the same multiply-adds in list order: bit-identical) in k_grid_wrows_packed's form: each
 // 64-entry batch fully unrolled with every load unconditional (taps past the row read its last tap, and
@@ -901,7 +1010,20 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
   const int KK = w.Ku * w.Kv;
   const dim3 blocks((unsigned)t.n_slot);
   ProfScope prof("grid_wrows", s);
-  if (w.rec) {
+  if (w.rec && w.qe > 1) {
+    if (KK > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
+    const size_t shq = shm * (size_t)w.qe;
+    if (shq > 160 * 1024) throw Error(PF_ERR_UNSUPPORTED, "kept-entry gridding copies exceed 160 KiB of LDS");
+#define PF_GWK(QQ, O64) launch_lds(k_grid_wrows_kept<R, QQ, O64>, blocks, dim3(64), shq, s, Nu, Nv, w, t, ys, grid, scale)
+    if (w.qe == 4) {
+      if (w.off64) PF_GWK(4, true);
+      else PF_GWK(4, false);
+    } else {
+      if (w.off64) PF_GWK(2, true);
+      else PF_GWK(2, false);
+    }
+#undef PF_GWK
+  } else if (w.rec) {
     if (KK > 64) throw Error(PF_ERR_ARG, "kept-entry rows need Ku Kv <= 64");
 #define PF_GWP(DD, O64) launch_lds(k_grid_wrows_packed<R, DD, O64>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale)
     if (w.off64) {

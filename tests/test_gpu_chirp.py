@@ -117,14 +117,48 @@ def test_2048_m4e6_w_0_200_rows_footprint_and_dot_test():
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
 
 
+def grid_entry_counts(op):
+    """The entries-per-step settings of kept-entry gridding that the operator accepts (1, 2 or 4)."""
+    from purify_amd._lib import PurifyError
+    q0 = op.get_tuning("w_grid_entries")
+    ok = []
+    for q in (1, 2, 4):
+        try:
+            op.set_tuning("w_grid_entries", q)
+            ok.append(q)
+        except PurifyError:
+            pass
+    op.set_tuning("w_grid_entries", q0)
+    return ok
+
+
+def check_grid_entries(lt, y, ref, tol):
+    """Kept-entry gridding at every accepted entries-per-step setting: one entry per step gives the dense
+    rows' sums bit for bit (same additions, same order), more entries the same sums to rounding (each entry
+    slot of a step adds into its own copy of the tile, the copies summed in order); every setting repeats
+    bit for bit."""
+    q0 = lt.op.get_tuning("w_grid_entries")
+    counts = grid_entry_counts(lt.op)
+    for q in counts:
+        lt.op.set_tuning("w_grid_entries", q)
+        a = lt.adjoint() * y
+        assert np.array_equal(a, lt.adjoint() * y), q
+        if q == 1:
+            assert np.array_equal(a, ref)
+        else:
+            assert rel(a, ref) < tol, q
+    lt.op.set_tuning("w_grid_entries", q0)
+    return counts
+
+
 @pytest.mark.parametrize("precision", ["c128", "c64"])
 @pytest.mark.parametrize("J", [4, 6])
 def test_kept_entry_rows_against_the_dense_rows(precision, J):
     # a 1 degree field with |w| <= 200 keeps the chirp within one cell (footprint J + 2 <= 8): the rows are
     # stored as their kept entries (keep mask, offset, values and tap bytes); the dense rows, rebuilt from
-    # them on request, must give the same gridding bit for bit (the same sums in the same order) and the
-    # same degridding to rounding (its lanes take the kept values in stored order), every application must
-    # repeat bit for bit, and both match the oracle
+    # them on request, must give the same gridding bit for bit at one entry per step (the same sums in the
+    # same order) and to rounding at more, the same degridding to rounding (its lanes take the kept values
+    # in stored order), every application must repeat bit for bit, and both match the oracle
     n, M = 32, 3000
     cell = 3600.0 / n
     u, v = make_uv(1301 + J, M, n)
@@ -136,6 +170,8 @@ def test_kept_entry_rows_against_the_dense_rows(precision, J):
     lt = chirp_op(u, v, w, n, J, cell, 0.999, 0.999, precision, wt)
     K = lt.op.get_tuning("w_footprint")
     assert K <= 8 and lt.op.get_tuning("w_rows_packed") == 1
+    q0 = lt.op.get_tuning("w_grid_entries")
+    tol = {"c128": 1e-14, "c64": 1e-6}[precision]
     dt = torch.complex128 if precision == "c128" else torch.complex64
     x = utilities.synth_image(1304, n, n, 5)
     y = utilities.synth_cnormal(1305, M)
@@ -154,13 +190,14 @@ def test_kept_entry_rows_against_the_dense_rows(precision, J):
     names = ("forward", "adjoint", "G", "G^H", "normal")
     for name, a, b in zip(names, packed, dense_run):
         a, b = np.asarray(a), np.asarray(b)
-        if name in ("adjoint", "G^H"):
+        if name in ("adjoint", "G^H") and q0 == 1:
             assert np.array_equal(a, b), name
         else:
-            assert rel(a, b) < {"c128": 1e-14, "c64": 1e-6}[precision], name
+            assert rel(a, b) < tol, name
     lt.op.set_tuning("w_rows_packed", 1)
     for a, b in zip(packed, run()):
         assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert q0 in check_grid_entries(lt, y, np.asarray(dense_run[1]), tol)
     # the rows against the oracle's (ties in the keep decision skipped, as above)
     G, dense = oracle_rows(u, v, w, n, J, cell, 0.999, 0.999, wt)
     tie = np.array([threshold_margin(G, m, 2 * n, w[m], cell, 0.999, 0.999) < 1e-9 for m in range(M)])
@@ -182,6 +219,11 @@ def test_kept_entry_rows_tuning_errors():
         lt.op.set_tuning("w_rows_packed", 1)
     with pytest.raises(PurifyError, match="0 or 1"):
         lt.op.set_tuning("w_rows_packed", 2)
+    assert lt.op.get_tuning("w_grid_entries") == 0
+    with pytest.raises(PurifyError, match="1, 2 or 4"):
+        lt.op.set_tuning("w_grid_entries", 3)
+    with pytest.raises(PurifyError, match="kept-entry rows"):
+        lt.op.set_tuning("w_grid_entries", 2)
 
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
@@ -210,6 +252,12 @@ def test_kept_entry_degridding_slot_counts(precision):
         lt.op.set_tuning("w_rows_packed", 0)
         assert rel(yg, lt.op.apply_G(g).cpu().numpy()) < tol, (J, e_w, slots)
         assert rel(yx, lt * x) < tol, (J, e_w, slots)
+        # gridding: as many entries per step as the largest row allows (64 / entries lanes each)
+        y = utilities.synth_cnormal(1406, M)
+        ref = lt.adjoint() * y
+        lt.op.set_tuning("w_rows_packed", 1)
+        counts = check_grid_entries(lt, y, ref, tol)
+        assert counts == {2: [1, 2, 4], 4: [1, 2], 8: [1]}[slots], (J, e_w, slots, counts)
     assert seen == {2, 4, 8}, seen
 
 
@@ -225,11 +273,13 @@ def test_kept_entry_rows_with_few_visibilities(precision, M):
     assert lt.op.get_tuning("w_rows_packed") == 1
     x = utilities.synth_image(1503, n, n, 3)
     y = utilities.synth_cnormal(1504, M)
-    f, a = lt * x, lt.adjoint() * y
+    f = lt * x
     lt.op.set_tuning("w_rows_packed", 0)
     tol = {"c128": 1e-14, "c64": 1e-6}[precision]
     assert rel(f, lt * x) < tol
-    assert np.array_equal(a, lt.adjoint() * y)
+    ref = lt.adjoint() * y
+    lt.op.set_tuning("w_rows_packed", 1)
+    assert len(check_grid_entries(lt, y, ref, tol)) > 1
 
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
@@ -248,16 +298,19 @@ def test_kept_entry_rows_on_other_tile_shapes(precision, tile):
     assert lt.op.get_tuning("w_rows_packed") == 1
     x = utilities.synth_image(1603, n, n, 5)
     y = utilities.synth_cnormal(1604, M)
-    f, a = lt * x, lt.adjoint() * y
+    f = lt * x
     lt.op.set_tuning("w_rows_packed", 0)
-    assert rel(f, lt * x) < {"c128": 1e-14, "c64": 1e-6}[precision]
-    assert np.array_equal(a, lt.adjoint() * y)
+    tol = {"c128": 1e-14, "c64": 1e-6}[precision]
+    assert rel(f, lt * x) < tol
+    ref = lt.adjoint() * y
+    lt.op.set_tuning("w_rows_packed", 1)
+    assert len(check_grid_entries(lt, y, ref, tol)) > 1
 
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
 def test_kept_entry_gridding_with_split_tiles(precision):
     # visibilities crowded into the centre: the densest tiles' lists are cut into parts summed in a fixed
-    # order afterwards, by the kept-entry kernel as by the dense one
+    # order afterwards, by the kept-entry kernels as by the dense one
     n, M = 32, 30_000
     cell = 3600.0 / n
     u, v = make_uv(1701, M, n, sigma=0.3)
@@ -265,7 +318,30 @@ def test_kept_entry_gridding_with_split_tiles(precision):
     lt = chirp_op(u, v, w, n, 6, cell, 0.999, 0.999, precision)
     assert lt.op.get_tuning("split_tiles") > 0
     y = utilities.synth_cnormal(1703, M)
-    a = lt.adjoint() * y
-    assert np.array_equal(lt.adjoint() * y, a)
     lt.op.set_tuning("w_rows_packed", 0)
-    assert np.array_equal(lt.adjoint() * y, a)
+    ref = lt.adjoint() * y
+    lt.op.set_tuning("w_rows_packed", 1)
+    assert len(check_grid_entries(lt, y, ref, {"c128": 1e-14, "c64": 1e-6}[precision])) > 1
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_autotune_chooses_the_kept_entry_gridding_entries(precision):
+    # rows pruned to at most 16 kept taps allow 1, 2 or 4 gridding entries per step: the autotuner times
+    # each and keeps one of them, with the same adjoint to rounding, repeating bit for bit
+    n, M = 64, 20_000
+    cell = 3600.0 / n
+    u, v = make_uv(1801, M, n)
+    w = np.random.default_rng(1802).uniform(-200.0, 200.0, M)
+    lt = chirp_op(u, v, w, n, 4, cell, 0.999, 0.9, precision)
+    assert lt.op.get_tuning("w_degrid_slots") == 2
+    counts = grid_entry_counts(lt.op)
+    assert counts == [1, 2, 4]
+    y = utilities.synth_cnormal(1803, M)
+    lt.op.set_tuning("w_grid_entries", 1)
+    ref = lt.adjoint() * y
+    lt.op.autotune(2)
+    q = lt.op.get_tuning("w_grid_entries")
+    assert q in counts
+    a = lt.adjoint() * y
+    assert np.array_equal(a, lt.adjoint() * y)
+    assert rel(a, ref) < {"c128": 1e-14, "c64": 1e-6}[precision]
