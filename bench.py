@@ -471,7 +471,8 @@ def main():
         res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
                          value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s,
                          host_ms=statistics.median(host_ms), KK=KK_op, nnz_w=op.info().nnz_w, packed=packed,
-                         fft_columns=op.get_tuning("fft_columns"), **tiles)
+                         fft_columns=op.get_tuning("fft_columns"),
+                         grid_entries=op.get_tuning("w_grid_entries") if packed else 0, **tiles)
         log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s, "
             f"grid kernel {variant}")
         del lt, op, x, yin, y, xo
@@ -597,7 +598,9 @@ def main():
         "kernels_frac_hbm_peak": kern_frac,
         **({"w_rows": {"footprint": int(round(KK ** 0.5)), "kept_entries_per_row": r64["nnz_w"] / M,
                        "layout": "kept entries" if r64["packed"] else "dense",
-                       "stored_bytes_per_row": RB}} if KK else {}),
+                       "stored_bytes_per_row": RB,
+                       "grid_entries_per_step": {"c64": r64["grid_entries"],
+                                                 "c128": res["c128"]["grid_entries"]}}} if KK else {}),
         "fp64": {"value": res["c128"]["value"], "ms_per_step": res["c128"]["ms"],
                  "fp32_speedup": res["c128"]["ms"] / r64["ms"],
                  "pair_frac_hbm_peak": model_bytes_pair(n, Ng, M, 16, 8, RB128) / (res["c128"]["ms"] * 1e-3) / HBM_PEAK},

@@ -334,7 +334,8 @@ int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
  * Phi + Phi^dagger pairs each, the best of three trials).  Writes the chosen gridding variant to *chosen if non-NULL;
  * the column mode is read back with pf_op_get_tuning.  Operators with w-projection rows or a given
  * gridding matrix have one gridding kernel: only the column mode is chosen, and with kept-entry rows the
- * gridding's "w_grid_entries" (reps gridding passes each). */
+ * gridding's "w_grid_entries" (reps gridding passes each, after as many warm-up passes, the settings
+ * alternating over three trials and each keeping its best). */
 int pf_op_autotune(pf_op *op, int32_t reps, int64_t *chosen);
 
 /* ---- per-kernel timing (no counterpart in the reference, whose operators log at construction only,

@@ -2002,34 +2002,35 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     }
     op->grid_variant = best_v;
     // kept-entry rows: the gridding's entries per step, from one up to what the largest row and the LDS
-    // allow (every setting is fixed-order)
+    // allow (every setting is fixed-order); the settings alternate over three trials, each keeping its best
     if (op->w_term && op->w_packed && op->w_qe_max > 1) {
-      float best_q = 1e30f;
-      int best_e = op->w_qe;
-      for (int q = 1; q <= op->w_qe_max; q *= 2) {
-        if ((size_t)q * op->tl.TU * op->tl.TV * op->esz() > 160 * 1024) break;
-        op->w_qe = q;
-        auto run = [&] {
-          if (op->prec == PF_C64)
-            grid_wrows<float>(op->Nu, op->Nv, wrows<float>(op), op->tl, (const cpx<float>*)op->d_vecy,
-                              (cpx<float>*)op->d_gridA, 1.0f, s);
-          else
-            grid_wrows<double>(op->Nu, op->Nv, wrows<double>(op), op->tl, (const cpx<double>*)op->d_vecy,
-                               (cpx<double>*)op->d_gridA, 1.0, s);
-        };
-        for (int k = 0; k < n; ++k) run();  // warm-up
-        PF_HIP(hipEventRecord(a, s));
-        for (int k = 0; k < n; ++k) run();
-        PF_HIP(hipEventRecord(b, s));
-        PF_HIP(hipEventSynchronize(b));
-        float ms = 0;
-        PF_HIP(hipEventElapsedTime(&ms, a, b));
-        if (ms < best_q) {
-          best_q = ms;
-          best_e = q;
+      float best_q[3] = {1e30f, 1e30f, 1e30f};  // entries 1, 2, 4
+      for (int trial = 0; trial < 3; ++trial) {
+        for (int qi = 0, q = 1; q <= op->w_qe_max; ++qi, q *= 2) {
+          if ((size_t)q * op->tl.TU * op->tl.TV * op->esz() > 160 * 1024) break;
+          op->w_qe = q;
+          auto run = [&] {
+            if (op->prec == PF_C64)
+              grid_wrows<float>(op->Nu, op->Nv, wrows<float>(op), op->tl, (const cpx<float>*)op->d_vecy,
+                                (cpx<float>*)op->d_gridA, 1.0f, s);
+            else
+              grid_wrows<double>(op->Nu, op->Nv, wrows<double>(op), op->tl, (const cpx<double>*)op->d_vecy,
+                                 (cpx<double>*)op->d_gridA, 1.0, s);
+          };
+          for (int k = 0; k < n; ++k) run();  // warm-up
+          PF_HIP(hipEventRecord(a, s));
+          for (int k = 0; k < n; ++k) run();
+          PF_HIP(hipEventRecord(b, s));
+          PF_HIP(hipEventSynchronize(b));
+          float ms = 0;
+          PF_HIP(hipEventElapsedTime(&ms, a, b));
+          best_q[qi] = std::min(best_q[qi], ms);
         }
       }
-      op->w_qe = best_e;
+      int bi = 0;
+      for (int qi = 1; qi < 3; ++qi)
+        if (best_q[qi] < best_q[bi]) bi = qi;
+      op->w_qe = 1 << bi;
     }
     // column FFT: one pass or two steps, timed over whole Phi + Phi^dagger pairs (a timing of the column
     // passes alone is at the mercy of the host's launch rate).  Both plans are built first and swapped in
