@@ -73,6 +73,12 @@ def kernel_alg_bytes(name, n, Nu, Nv, M, e, r, RB=None):
         "fft_cols_stepB": 2 * Ng * e,
         "fft_cols_fwd": (win + Ng) * e,  # reads the image window's rows, writes the full grid
         "fft_cols_adj": (Ng + win) * e,  # reads the full grid, writes the rows the crop keeps
+        # the image-column-first order (one-pass column plans): the column pass on the n image columns,
+        # the row pass over every grid row
+        "fft_cols_fwd_x": (n * n + Nv * n) * e,  # reads the image, writes Nv x n
+        "fft_rows_fwd_x": (Nv * n + Ng) * e,  # reads Nv x n, writes the full grid
+        "fft_rows_adj_x": (Ng + Nv * n) * e,  # reads the full grid, writes its image columns
+        "fft_cols_adj_x": (Nv * n + n * n) * e,  # reads Nv x n, writes the image
         "permute_to_sorted": 2 * M * e,
         "permute_to_caller": 2 * M * e,
     }

@@ -248,7 +248,10 @@ __global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>())
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
   constexpr int N = CPlan<L>::N, T = CPlan<L>::T, E = CPlan<L>::E;
-  constexpr bool SZM = MODE != ROW_PLAIN;
+  constexpr bool SZM = MODE == ROW_FWD_SZ || MODE == ROW_ADJ_CROP || MODE == ROW_ADJ_X;  // S's column factors
+  constexpr bool SYM = MODE == ROW_FWD_SZ || MODE == ROW_ADJ_CROP;  // S's row factors applied here
+  constexpr bool CROP = MODE == ROW_ADJ_CROP || MODE == ROW_ADJ_X;  // output cropped to the image columns
+  constexpr bool PAD = MODE == ROW_FWD_SZ || MODE == ROW_FWD_X;     // input zero-padded from the image width
   const int l = threadIdx.x / T, t = threadIdx.x - l * T;
   cpx<R>* lds = smem + l * (N + N / 16);
   const TwLds<R, L> tw = tw_lds_fill<R, L>(smem + a.lpb * (N + N / 16), a.tw, a.twm);
@@ -266,12 +269,12 @@ __global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>())
   // row base pointers, then 32-bit unsigned element offsets (one VGPR per address)
   auto loader = [&](int line) {
     const bool valid = line < a.nlines;
-    const cpx<R>* row = MODE == ROW_FWD_SZ     ? sz.img_in + (int64_t)line * sz.nx
+    const cpx<R>* row = PAD                    ? sz.img_in + (int64_t)line * sz.nx
                         : MODE == ROW_ADJ_CROP ? a.in + (int64_t)(sz.y0 + line) * N
-                                               : a.in + (int64_t)line * N;
+                                               : a.in + (int64_t)line * N;  // ROW_PLAIN, ROW_ADJ_X
     return [=, &sz](int idx) -> cpx<R> {
       if (!valid) return mk<R>(R(0), R(0));
-      if constexpr (MODE == ROW_FWD_SZ) {  // the image row, zero-padded
+      if constexpr (PAD) {  // the image-width row, zero-padded
         const unsigned i = (unsigned)(idx - sz.x0);
         if (i >= (unsigned)sz.nx) return mk<R>(R(0), R(0));
         return row[i];
@@ -289,7 +292,7 @@ __global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>())
   {
     auto ld = loader(first + l);
     line_load<R, L, FFT_E>(nxt, t, ld);
-    if constexpr (SZM)
+    if constexpr (SYM)
       if (first + l < a.nlines) nsy = sz.sy[first + l];
   }
   for (int base = first; base < a.nlines; base += stride) {
@@ -306,7 +309,7 @@ __global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>())
     if (base + stride < a.nlines) {
       auto ld = loader(line + stride);
       line_load<R, L, FFT_E>(nxt, tl, ld);
-      if constexpr (SZM)
+      if constexpr (SYM)
         if (line + stride < a.nlines) nsy = sz.sy[line + stride];
     }
     if constexpr (MODE == ROW_FWD_SZ) {  // S: sy[line] sx[i] and the folded scale
@@ -316,13 +319,14 @@ __global__ void __launch_bounds__(row_threads<L>(), row_waves_per_simd<R>())
     }
     cpx<R>* orow = MODE == ROW_FWD_SZ     ? a.out + (int64_t)(sz.y0 + line) * N
                    : MODE == ROW_ADJ_CROP ? sz.img_out + (int64_t)line * sz.nx
+                   : MODE == ROW_ADJ_X    ? a.out + (int64_t)line * sz.nx
                                           : a.out + (int64_t)line * N;
     auto store = [&](int idx, cpx<R> val, int e) {
       if (!valid) return;
-      if constexpr (MODE == ROW_ADJ_CROP) {
+      if constexpr (CROP) {
         const unsigned i = (unsigned)(idx - sz.x0);
         if (i >= (unsigned)sz.nx) return;
-        const R s = sy * sxr[e] * sz.scale;
+        const R s = MODE == ROW_ADJ_X ? sxr[e] * sz.scale : sy * sxr[e] * sz.scale;
         orow[i] = scale(val, s);
       } else {
         orow[(unsigned)idx] = val;
@@ -351,6 +355,8 @@ struct RowLaunch {
   void operator()() {
     if (mode == ROW_FWD_SZ) go<L, false, ROW_FWD_SZ>();
     else if (mode == ROW_ADJ_CROP) go<L, true, ROW_ADJ_CROP>();
+    else if (mode == ROW_ADJ_X) go<L, true, ROW_ADJ_X>();
+    else if (mode == ROW_FWD_X) go<L, false, ROW_FWD_X>();
     else if (inv) go<L, true, ROW_PLAIN>();
     else go<L, false, ROW_PLAIN>();
   }
@@ -379,8 +385,13 @@ void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* 
   int blocks = (nlines + lpb - 1) / lpb;
   if (p.row_bpc > 0) blocks = std::min(blocks, p.row_bpc * p.n_cu);
   if (blocks == 0) return;
-  ProfScope prof(mode == ROW_FWD_SZ ? "fft_rows_fwd_sz" : mode == ROW_ADJ_CROP ? "fft_rows_adj_crop" : "fft_rows", s);
-  if ((mode == ROW_FWD_SZ && inv) || (mode == ROW_ADJ_CROP && !inv))
+  ProfScope prof(mode == ROW_FWD_SZ     ? "fft_rows_fwd_sz"
+                 : mode == ROW_ADJ_CROP ? "fft_rows_adj_crop"
+                 : mode == ROW_ADJ_X    ? "fft_rows_adj_x"
+                 : mode == ROW_FWD_X    ? "fft_rows_fwd_x"
+                                        : "fft_rows",
+                 s);
+  if (((mode == ROW_FWD_SZ || mode == ROW_FWD_X) && inv) || ((mode == ROW_ADJ_CROP || mode == ROW_ADJ_X) && !inv))
     throw Error(-1, "fused row pass direction mismatch");
   RowLaunch<R> launch{a, mode, inv, threads, blocks, shm, s};
   dispatch_log2(p.row.log2N, launch);
@@ -391,9 +402,12 @@ void fft_rows(const Fft2dPlan& p, int mode, bool inv, const cpx<R>* in, cpx<R>* 
 // Column passes
 // ---------------------------------------------------------------------------------------------
 enum ColKind { COL_SINGLE = 0, COL_STEP_A = 1, COL_STEP_B = 2 };
+// Column-pass input and output forms (compile-time, so that no branch sits among the loads): plain;
+// PAD: window row y read from row y - y0 of an image, times isy[y - y0] isx[col] iscale (rin); CROP:
+// window row y written at row y - y0, times osy[y - y0] (rout)
+enum ColIo { COL_IO_PLAIN = 0, COL_IO_PAD = 1, COL_IO_CROP = 2 };
 
-
-template <typename R, int L, bool INV, int KIND, bool BT = false>
+template <typename R, int L, bool INV, int KIND, bool BT = false, int IO = COL_IO_PLAIN>
 __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a0, const ColArgs<R>* __restrict__ tab) {
   const ColArgs<R> a = BT ? tab[blockIdx.z] : a0;
   extern __shared__ unsigned char smem_raw[];
@@ -410,6 +424,10 @@ __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a0, const ColArgs<R
     else if constexpr (KIND == COL_STEP_A) row = a.N2 * idx + by;
     else row = a.N2 * by + idx;
     if (a.rin && !in_window(row)) return mk<R>(R(0), R(0));
+    if constexpr (IO == COL_IO_PAD) {
+      const int r = row - a.y0;
+      return scale(a.in[r * Nu + col], a.isy[r] * (a.isx[col] * a.iscale));
+    }
     return a.in[row * Nu + col];
   };
   auto store = [&](int idx, cpx<R> val, int) {
@@ -424,6 +442,10 @@ __global__ void __launch_bounds__(512) k_fft_cols(ColArgs<R> a0, const ColArgs<R
       row = by + a.N1 * idx;
     }
     if (a.rout && !in_window(row)) return;
+    if constexpr (IO == COL_IO_CROP) {
+      row -= a.y0;
+      val = scale(val, a.osy[row]);
+    }
     a.out[row * Nu + col] = val;
   };
   const TwGlobal<R> tw{a.tw, a.twm};
@@ -454,7 +476,7 @@ constexpr int big_col_c(int esz, int L) {
 template <typename R, int L>
 constexpr int big_col_threads() { return big_col_c(sizeof(cpx<R>), L) * ((1 << L) / big_col_e(sizeof(cpx<R>))); }
 
-template <typename R, int L, bool INV, bool BT = false>
+template <typename R, int L, bool INV, bool BT = false, int IO = COL_IO_PLAIN>
 __global__ void __launch_bounds__((big_col_threads<R, L>()))
     k_fft_cols_big(ColArgs<R> a0, const ColArgs<R>* __restrict__ tab) {
   const ColArgs<R> a = BT ? tab[blockIdx.z] : a0;
@@ -463,6 +485,13 @@ __global__ void __launch_bounds__((big_col_threads<R, L>()))
   constexpr int EE = big_col_e(sizeof(cpx<R>));
   constexpr int C = big_col_c(sizeof(cpx<R>), L), N = CPlan<L, EE>::N, E = CPlan<L, EE>::E;
   const int c = threadIdx.x % C, t = threadIdx.x / C;
+  // PAD / CROP: S's row factors in LDS behind the twiddles (filled before tw_lds_fill's barrier), so
+  // that scaling a loaded line or a stored value reads LDS rather than waiting on further global loads
+  R* lsy = reinterpret_cast<R*>(smem + C * (N + N / 16) + tw_lds_entries<L>());
+  if constexpr (IO != COL_IO_PLAIN) {
+    const R* fac = IO == COL_IO_PAD ? a.isy : a.osy;
+    for (int i = threadIdx.x; i < a.ny; i += blockDim.x) lsy[i] = fac[i];
+  }
   const TwLds<R, L> tw = tw_lds_fill<R, L>(smem + C * (N + N / 16), a.tw, a.twm);
   const unsigned Nu = (unsigned)a.Nu;
   const int G = a.Nu / C, P = gridDim.x;
@@ -476,12 +505,23 @@ __global__ void __launch_bounds__((big_col_threads<R, L>()))
     const unsigned col = (unsigned)(g * C + c);
     auto ld = [&](int row) -> cpx<R> {
       if (a.rin && !in_window(row)) return mk<R>(R(0), R(0));
-      return at_byte(a.in, ((unsigned)row * Nu + col) * (unsigned)sizeof(cpx<R>));
+      // PAD: window row y is the image's row y - y0
+      const unsigned r = IO == COL_IO_PAD ? (unsigned)(row - a.y0) : (unsigned)row;
+      return at_byte(a.in, (r * Nu + col) * (unsigned)sizeof(cpx<R>));
     };
     line_load<R, L, EE>(v, tl, ld);
   };
+  // PAD: the group's column factor isx[col] iscale, loaded with the group
+  auto col_factor = [&](int g) -> R {
+    if constexpr (IO == COL_IO_PAD) return a.isx[g * C + c] * a.iscale;
+    else return R(1);
+  };
   cpx<R> nxt[E];
-  if (q < qend) load_group(nxt, gbase + q, t);
+  R ncs = R(1);
+  if (q < qend) {
+    load_group(nxt, gbase + q, t);
+    ncs = col_factor(gbase + q);
+  }
   for (int k = q; k < qend; k += qstep) {
     // an opaque copy of the lane index keeps the address arithmetic inside the loop
     int tl = t;
@@ -489,10 +529,25 @@ __global__ void __launch_bounds__((big_col_threads<R, L>()))
     cpx<R> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nxt[e];
-    if (k + qstep < qend) load_group(nxt, gbase + k + qstep, tl);
+    const R cs = ncs;
+    if (k + qstep < qend) {
+      load_group(nxt, gbase + k + qstep, tl);
+      ncs = col_factor(gbase + k + qstep);
+    }
+    if constexpr (IO == COL_IO_PAD) {  // S on the loaded line (rows outside the window are zero already)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int r = min(max(line_index<L, EE>(tl, e) - a.y0, 0), a.ny - 1);
+        v[e] = scale(v[e], lsy[r] * cs);
+      }
+    }
     const unsigned col = (unsigned)((gbase + k) * C + c);
     auto store = [&](int row, cpx<R> val, int) {
       if (a.rout && !in_window(row)) return;
+      if constexpr (IO == COL_IO_CROP) {
+        row -= a.y0;
+        val = scale(val, lsy[row]);
+      }
       at_byte(a.out, ((unsigned)row * Nu + col) * (unsigned)sizeof(cpx<R>)) = val;
     };
     fft_line_pre<R, L, EE, INV>(v, tl, smem + c, -C, tw, store);
@@ -504,21 +559,28 @@ struct BigColLaunch {
   const ColArgs<R>& a;
   bool inv;
   hipStream_t s;
-  template <int L, bool I>
+  template <int L, bool I, int IO>
   void go() {
     constexpr int C = big_col_c(sizeof(cpx<R>), L), N = 1 << L;
-    const size_t shm = ((size_t)(N + N / 16) * C + tw_lds_entries<L>()) * sizeof(cpx<R>);
+    const size_t shm = ((size_t)(N + N / 16) * C + tw_lds_entries<L>()) * sizeof(cpx<R>) +
+                       (IO != COL_IO_PLAIN ? (size_t)a.ny * sizeof(R) : 0);
     const dim3 grid((unsigned)(a.Nu / C)), block(big_col_threads<R, L>());
-    if (record_launch<ColArgs<R>, k_fft_cols_big<R, L, I, true>>(grid, block, shm, false, a)) return;
-    void (*k)(ColArgs<R>, const ColArgs<R>*) = k_fft_cols_big<R, L, I>;
+    if (record_launch<ColArgs<R>, k_fft_cols_big<R, L, I, true, IO>>(grid, block, shm, false, a)) return;
+    void (*k)(ColArgs<R>, const ColArgs<R>*) = k_fft_cols_big<R, L, I, false, IO>;
     allow_lds((const void*)k, shm);
     hipLaunchKernelGGL(k, grid, block, shm, s, a, nullptr);
   }
   template <int L>
   void operator()() {
     if constexpr (L >= 10 && L <= 13) {
-      if (inv) go<L, true>();
-      else go<L, false>();
+      // the padded input form is a forward pass's, the cropped output form an inverse pass's
+      if (inv) {
+        if (a.osy) go<L, true, COL_IO_CROP>();
+        else go<L, true, COL_IO_PLAIN>();
+      } else {
+        if (a.isy) go<L, false, COL_IO_PAD>();
+        else go<L, false, COL_IO_PLAIN>();
+      }
     } else {
       throw Error(-3, "single-pass column length must be 1024..8192");
     }
@@ -534,20 +596,27 @@ struct ColLaunch {
   int threads;
   size_t shm;
   hipStream_t s;
-  template <int L, bool I, int KIND>
-  bool record() {
-    return record_launch<ColArgs<R>, k_fft_cols<R, L, I, KIND, true>>(grid, dim3(threads), shm, false, a);
+  template <int L, bool I, int KIND, int IO>
+  void go_kind() {
+    if (record_launch<ColArgs<R>, k_fft_cols<R, L, I, KIND, true, IO>>(grid, dim3(threads), shm, false, a)) return;
+    void (*k)(ColArgs<R>, const ColArgs<R>*) = k_fft_cols<R, L, I, KIND, false, IO>;
+    allow_lds((const void*)k, shm);
+    hipLaunchKernelGGL(k, grid, dim3(threads), shm, s, a, nullptr);
   }
   template <int L, bool I>
   void go() {
-    if (kind == COL_STEP_A ? record<L, I, COL_STEP_A>() : kind == COL_STEP_B ? record<L, I, COL_STEP_B>()
-                                                                             : record<L, I, COL_SINGLE>())
-      return;
-    void (*k)(ColArgs<R>, const ColArgs<R>*) = kind == COL_STEP_A   ? k_fft_cols<R, L, I, COL_STEP_A>
-                                               : kind == COL_STEP_B ? k_fft_cols<R, L, I, COL_STEP_B>
-                                                                    : k_fft_cols<R, L, I, COL_SINGLE>;
-    allow_lds((const void*)k, shm);
-    hipLaunchKernelGGL(k, grid, dim3(threads), shm, s, a, nullptr);
+    // the padded input form only in a forward single pass, the cropped output form only in an inverse
+    // single pass or second step
+    if constexpr (!I) {
+      if (a.isy && kind == COL_SINGLE) return go_kind<L, I, COL_SINGLE, COL_IO_PAD>();
+    } else {
+      if (a.osy && kind == COL_SINGLE) return go_kind<L, I, COL_SINGLE, COL_IO_CROP>();
+      if (a.osy && kind == COL_STEP_B) return go_kind<L, I, COL_STEP_B, COL_IO_CROP>();
+    }
+    if (a.isy || a.osy) throw Error(-1, "column pass form not instantiated");
+    if (kind == COL_STEP_A) go_kind<L, I, COL_STEP_A, COL_IO_PLAIN>();
+    else if (kind == COL_STEP_B) go_kind<L, I, COL_STEP_B, COL_IO_PLAIN>();
+    else go_kind<L, I, COL_SINGLE, COL_IO_PLAIN>();
   }
   template <int L>
   void operator()() {
@@ -561,12 +630,15 @@ struct ColLaunch {
 };
 
 template <typename R>
-static void launch_cols(int kind, bool inv, int log2n, const ColArgs<R>& a, dim3 grid, hipStream_t s) {
+static void launch_cols(int kind, bool inv, int log2n, const ColArgs<R>& a, dim3 grid, hipStream_t s,
+                        const char* name = nullptr) {
   const int N = 1 << log2n;
   const int E = N < FFT_E ? N : FFT_E;
   const int threads = a.C * (N / E);
   const size_t shm = (size_t)N * a.C * sizeof(cpx<R>);
-  ProfScope prof(kind == COL_STEP_A ? "fft_cols_stepA" : kind == COL_STEP_B ? "fft_cols_stepB"
+  ProfScope prof(name                  ? name
+                 : kind == COL_STEP_A ? "fft_cols_stepA"
+                 : kind == COL_STEP_B ? "fft_cols_stepB"
                  : inv                 ? "fft_cols_adj"
                                        : "fft_cols_fwd",
                  s);
@@ -589,6 +661,10 @@ void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow w
   a.ny = win.ny ? win.ny : p.Nv;
   a.N1 = p.N1;
   a.N2 = p.N2;
+  a.osy = nullptr;
+  a.isy = nullptr;
+  a.isx = nullptr;
+  a.iscale = R(1);
   if (p.col_big) {
     a.twm = p.Lmax / p.Nv;
     a.C = p.big_C;
@@ -623,6 +699,124 @@ void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow w
   a.rout = restrict_out;
   launch_cols<R>(COL_STEP_B, inv, p.colB.log2N, a, dim3(p.Nu / a.C, p.N1), s);
 }
+
+bool fft_cols_width_ok(const Fft2dPlan& p, int width) {
+  const int C = p.col_big ? p.big_C : p.col_two_step ? p.two_C : p.col_C;
+  return width > 0 && width <= p.Nu && width % C == 0;
+}
+
+template <typename R>
+void fft_cols_crop(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, int width, RowWindow win, const R* osy,
+                   hipStream_t s) {
+  if (!fft_cols_crop_ok(p, width, win.ny) || !osy || win.ny <= 0)
+    throw Error(-1, "cropped column pass does not fit the plan");
+  Fft2dPlan q = p;  // the same transforms over `width` columns
+  q.Nu = width;
+  ColArgs<R> a;
+  a.in = in;
+  a.out = out;
+  a.tw = static_cast<const cpx<R>*>(q.tw);
+  a.Nu = q.Nu;
+  a.Nv = q.Nv;
+  a.twm_full = q.Lmax / q.Nv;
+  a.y0 = win.y0;
+  a.ny = win.ny;
+  a.N1 = q.N1;
+  a.N2 = q.N2;
+  a.osy = nullptr;
+  a.isy = nullptr;
+  a.isx = nullptr;
+  a.iscale = R(1);
+  if (q.col_big) {
+    a.twm = q.Lmax / q.Nv;
+    a.C = q.big_C;
+    a.rin = 0;
+    a.rout = 1;
+    a.osy = osy;
+    ProfScope prof(inv ? "fft_cols_adj_x" : "fft_cols_fwd_x", s);
+    BigColLaunch<R> launch{a, inv, s};
+    dispatch_log2(q.col.log2N, launch);
+    PF_HIP(hipGetLastError());
+    return;
+  }
+  if (!q.col_two_step) {
+    a.twm = q.Lmax / q.Nv;
+    a.C = q.col_C;
+    a.rin = 0;
+    a.rout = 1;
+    a.osy = osy;
+    launch_cols<R>(COL_SINGLE, inv, q.col.log2N, a, dim3(q.Nu / a.C), s, "fft_cols_adj_x");
+    return;
+  }
+  a.C = q.two_C;
+  a.twm = q.Lmax / q.N1;
+  a.out = in;
+  a.rin = 0;
+  a.rout = 0;
+  launch_cols<R>(COL_STEP_A, inv, q.colA.log2N, a, dim3(q.Nu / a.C, q.N2), s, "fft_cols_stepA_x");
+  a.twm = q.Lmax / q.N2;
+  a.in = in;
+  a.out = out;
+  a.rout = 1;
+  a.osy = osy;
+  launch_cols<R>(COL_STEP_B, inv, q.colB.log2N, a, dim3(q.Nu / a.C, q.N1), s, "fft_cols_stepB_x");
+}
+template void fft_cols_crop<float>(const Fft2dPlan&, bool, cpx<float>*, cpx<float>*, int, RowWindow, const float*,
+                                   hipStream_t);
+template void fft_cols_crop<double>(const Fft2dPlan&, bool, cpx<double>*, cpx<double>*, int, RowWindow,
+                                    const double*, hipStream_t);
+
+// the single-pass kernel's LDS with the ny row factors: padded columns, twiddles and the factors
+static bool big_cols_lds_ok(const Fft2dPlan& p, int ny) {
+  if (!p.col_big) return true;
+  const size_t esz = p.prec ? 16 : 8, N = (size_t)1 << p.col.log2N;
+  return ((N + N / 16) * p.big_C + tw_lds_entries_rt(p.col.log2N)) * esz + (size_t)ny * (esz / 2) <= 160 * 1024;
+}
+bool fft_cols_pad_ok(const Fft2dPlan& p, int width, int ny) {
+  return !p.col_two_step && fft_cols_width_ok(p, width) && big_cols_lds_ok(p, ny);
+}
+bool fft_cols_crop_ok(const Fft2dPlan& p, int width, int ny) {
+  return fft_cols_width_ok(p, width) && big_cols_lds_ok(p, ny);
+}
+
+template <typename R>
+void fft_cols_pad(const Fft2dPlan& p, bool inv, const cpx<R>* in, cpx<R>* out, int width, RowWindow win,
+                  const R* isy, const R* isx, R iscale, hipStream_t s) {
+  if (!fft_cols_pad_ok(p, width, win.ny) || !isy || !isx || win.ny <= 0)
+    throw Error(-1, "padded column pass does not fit the plan");
+  ColArgs<R> a;
+  a.in = const_cast<cpx<R>*>(in);  // read only: a single-pass plan writes `out` alone
+  a.out = out;
+  a.tw = static_cast<const cpx<R>*>(p.tw);
+  a.Nu = width;
+  a.Nv = p.Nv;
+  a.twm_full = p.Lmax / p.Nv;
+  a.twm = p.Lmax / p.Nv;
+  a.y0 = win.y0;
+  a.ny = win.ny;
+  a.N1 = p.N1;
+  a.N2 = p.N2;
+  a.rin = 1;
+  a.rout = 0;
+  a.osy = nullptr;
+  a.isy = isy;
+  a.isx = isx;
+  a.iscale = iscale;
+  if (p.col_big) {
+    a.C = p.big_C;
+    ProfScope prof(inv ? "fft_cols_adj_x" : "fft_cols_fwd_x", s);
+    BigColLaunch<R> launch{a, inv, s};
+    dispatch_log2(p.col.log2N, launch);
+    PF_HIP(hipGetLastError());
+    return;
+  }
+  a.C = p.col_C;
+  launch_cols<R>(COL_SINGLE, inv, p.col.log2N, a, dim3(width / a.C), s, "fft_cols_fwd_x");
+}
+template void fft_cols_pad<float>(const Fft2dPlan&, bool, const cpx<float>*, cpx<float>*, int, RowWindow,
+                                  const float*, const float*, float, hipStream_t);
+template void fft_cols_pad<double>(const Fft2dPlan&, bool, const cpx<double>*, cpx<double>*, int, RowWindow,
+                                   const double*, const double*, double, hipStream_t);
 
 // ---------------------------------------------------------------------------------------------
 // Plans and twiddles

@@ -189,7 +189,11 @@ struct RowWindow {
 };
 
 // Fused row-pass variants.
-enum RowMode { ROW_PLAIN = 0, ROW_FWD_SZ = 1, ROW_ADJ_CROP = 2 };
+// ROW_ADJ_X: inverse transforms of every line, each cropped to the image columns [x0, x0 + nx) and
+// scaled by sx and the folded scale (S's row factors are left to the column pass that follows);
+// ROW_FWD_X: forward transforms of lines of nx values (img_in, line stride nx) zero-padded to the grid
+// width at x0, unscaled (S applied by the column pass before)
+enum RowMode { ROW_PLAIN = 0, ROW_FWD_SZ = 1, ROW_ADJ_CROP = 2, ROW_ADJ_X = 3, ROW_FWD_X = 4 };
 
 template <typename R>
 struct SZ {
@@ -224,6 +228,10 @@ struct ColArgs {
   int Nu, Nv, C, N1, N2;
   int y0, ny;    // row window
   int rin, rout; // restrict input / output rows to the window
+  const R* osy;  // non-null (with rout): window row y is written at row y - y0, times osy[y - y0]
+  const R* isy;  // non-null (with rin): window row y is read at row y - y0, times isy[y - y0] isx[col] iscale
+  const R* isx;
+  R iscale;
 };
 
 // Column pass.  restrict_in: rows outside win are zero and not read; restrict_out: only rows in
@@ -231,6 +239,21 @@ struct ColArgs {
 template <typename R>
 void fft_cols(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, RowWindow win, bool restrict_in,
               bool restrict_out, hipStream_t s);
+// Column pass over the `width` columns of a p.Nv x width array (width < p.Nu), writing only the window's
+// rows, row y at y - win.y0 of `out` (win.ny x width) times osy[y - win.y0].  Usable when
+// fft_cols_crop_ok(p, width, win.ny).
+template <typename R>
+void fft_cols_crop(const Fft2dPlan& p, bool inv, cpx<R>* in, cpx<R>* out, int width, RowWindow win, const R* osy,
+                   hipStream_t s);
+bool fft_cols_width_ok(const Fft2dPlan& p, int width);
+bool fft_cols_crop_ok(const Fft2dPlan& p, int width, int ny);
+// Column pass over `width` columns of a win.ny x width image, S applied as it is read (row y - win.y0
+// times isy[y - win.y0] isx[col] iscale, zero outside the window), into a p.Nv x width array.  Not for
+// two-step plans (fft_cols_pad_ok).
+template <typename R>
+void fft_cols_pad(const Fft2dPlan& p, bool inv, const cpx<R>* in, cpx<R>* out, int width, RowWindow win,
+                  const R* isy, const R* isx, R iscale, hipStream_t s);
+bool fft_cols_pad_ok(const Fft2dPlan& p, int width, int ny);
 
 // ---- gridding -----------------------------------------------------------------------------
 template <typename R>

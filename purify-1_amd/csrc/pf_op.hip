@@ -984,10 +984,19 @@ SZ<R> make_sz(const pf_op* op, double scl) {
 template <typename R>
 void fz_forward(pf_op* op, const cpx<R>* x, cpx<R>* grid_out, double scl, hipStream_t s) {
   SZ<R> sz = make_sz<R>(op, scl);
-  sz.img_in = x;
   cpx<R>* A = static_cast<cpx<R>*>(op->d_gridA);
-  fft_rows<R>(op->fft, ROW_FWD_SZ, false, nullptr, A, op->ny, sz, s);
-  fft_cols<R>(op->fft, false, A, grid_out, RowWindow{op->y0, op->ny}, true, false, s);
+  if (fft_cols_pad_ok(op->fft, op->nx, op->ny)) {
+    // columns first, over the image's columns (S applied as they are read, zero-padded to the grid
+    // height), into an Nv x nx array; then every row of it, zero-padded to the grid width.  The row
+    // pass writes the full grid at its row-contiguous rate; the column pass only the image-wide half
+    fft_cols_pad<R>(op->fft, false, x, A, op->nx, RowWindow{op->y0, op->ny}, sz.sy, sz.sx, sz.scale, s);
+    sz.img_in = A;
+    fft_rows<R>(op->fft, ROW_FWD_X, false, nullptr, grid_out, op->Nv, sz, s);
+  } else {
+    sz.img_in = x;
+    fft_rows<R>(op->fft, ROW_FWD_SZ, false, nullptr, A, op->ny, sz, s);
+    fft_cols<R>(op->fft, false, A, grid_out, RowWindow{op->y0, op->ny}, true, false, s);
+  }
 }
 
 template <typename R>
@@ -995,8 +1004,17 @@ void fz_adjoint(pf_op* op, cpx<R>* grid_in, cpx<R>* x, double scl, hipStream_t s
   SZ<R> sz = make_sz<R>(op, scl);
   sz.img_out = x;
   cpx<R>* B = static_cast<cpx<R>*>(op->d_gridB);
-  fft_cols<R>(op->fft, true, grid_in, B, RowWindow{op->y0, op->ny}, false, true, s);
-  fft_rows<R>(op->fft, ROW_ADJ_CROP, true, B, nullptr, op->ny, sz, s);
+  if (fft_cols_crop_ok(op->fft, op->nx, op->ny)) {
+    // rows first, over every row of the grid, each cropped to the image columns (S's column factors and
+    // the scale applied); then the columns of that Nv x nx array, cropped to the image rows with S's
+    // row factors.  The row pass moves the full grid at its row-contiguous rate, and the column pass,
+    // whose 32-byte row segments make it the slower of the two per byte, only the cropped half
+    fft_rows<R>(op->fft, ROW_ADJ_X, true, grid_in, B, op->Nv, sz, s);
+    fft_cols_crop<R>(op->fft, true, B, x, op->nx, RowWindow{op->y0, op->ny}, sz.sy, s);
+  } else {
+    fft_cols<R>(op->fft, true, grid_in, B, RowWindow{op->y0, op->ny}, false, true, s);
+    fft_rows<R>(op->fft, ROW_ADJ_CROP, true, B, nullptr, op->ny, sz, s);
+  }
 }
 
 template <typename R>
