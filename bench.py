@@ -477,7 +477,7 @@ def main():
         res[prec] = dict(setup_s=setup, steps=steps, elapsed=elapsed, ms=1e3 * elapsed / steps,
                          value=M * steps / elapsed, lat=lat, table=table, grid_kernel=variant, tune_s=tune_s,
                          host_ms=statistics.median(host_ms), KK=KK_op, nnz_w=op.info().nnz_w, packed=packed,
-                         fft_columns=op.get_tuning("fft_columns"),
+                         fft_columns=op.get_tuning("fft_columns"), fft_row_blocks=op.get_tuning("fft_row_blocks"),
                          grid_entries=op.get_tuning("w_grid_entries") if packed else 0, **tiles)
         log(f"[bench] {prec}: {res[prec]['ms']:.3f} ms/pair, {res[prec]['value']:.3e} vis/s, setup {setup:.2f} s, "
             f"grid kernel {variant}")
@@ -621,6 +621,7 @@ def main():
                          "note": "pair on host complex128 buffers: PCIe copies and conversions included"},
         "autotune": {"grid_kernel": {"c64": r64["grid_kernel"], "c128": res["c128"]["grid_kernel"]},
                      "fft_columns": {"c64": r64["fft_columns"], "c128": res["c128"]["fft_columns"]},
+                     "fft_row_blocks": {"c64": r64["fft_row_blocks"], "c128": res["c128"]["fft_row_blocks"]},
                      "tile": {"c64": r64["tile"], "c128": res["c128"]["tile"]},
                      "perm_bucket": {"c64": r64["perm_bucket"], "c128": res["c128"]["perm_bucket"]},
                      "seconds": {"c64": r64["tune_s"], "c128": res["c128"]["tune_s"]},

@@ -335,7 +335,8 @@ int pf_op_get_tuning(const pf_op *op, const char *key, int64_t *value);
  * the column mode is read back with pf_op_get_tuning.  Operators with w-projection rows or a given
  * gridding matrix have one gridding kernel: only the column mode is chosen, and with kept-entry rows the
  * gridding's "w_grid_entries" (reps gridding passes each, after as many warm-up passes, the settings
- * alternating over three trials and each keeping its best). */
+ * alternating over three trials and each keeping its best).  Last, "fft_row_blocks" is chosen among 0, 2, 3
+ * and 4 the same way as the column mode (it replaces a value set before; results do not depend on it). */
 int pf_op_autotune(pf_op *op, int32_t reps, int64_t *chosen);
 
 /* ---- per-kernel timing (no counterpart in the reference, whose operators log at construction only,

@@ -2079,6 +2079,23 @@ int pf_op_autotune(pf_op* op, int32_t reps, int64_t* chosen) {
     op->fft = plans[best_m];
     fft_plan_free(plans[1 - best_m]);
     op->fft_col_mode = best_m;
+    // the row passes' grid (every line computed alike, so results do not depend on it): a few resident
+    // blocks per CU looping over lines with the next line prefetched, or one block per group of lines;
+    // timed over whole pairs like the column mode (at the 4096^2 grid, complex64, one block per group
+    // of lines beat three resident blocks per CU by 4 us per pass once the passes covered all 4096 rows)
+    {
+      const int cand[4] = {0, 2, 3, 4};
+      double bt[4] = {1e30, 1e30, 1e30, 1e30};
+      for (int trial = 0; trial < 3; ++trial)
+        for (int i = 0; i < 4; ++i) {
+          op->fft.row_bpc = cand[i];
+          bt[i] = std::min(bt[i], time_pairs(op, n, n));
+        }
+      int bi = 0;
+      for (int i = 1; i < 4; ++i)
+        if (bt[i] < bt[bi]) bi = i;
+      op->fft.row_bpc = op->fft_row_bpc = cand[bi];
+    }
     ++op->generation;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -330,6 +330,9 @@ def test_row_pass_grid_sizes_give_identical_results(precision):
         assert np.array_equal(lt * x, ref[0]) and np.array_equal(lt.adjoint() * y, ref[1])
     with pytest.raises(PurifyError):
         lt.op.set_tuning("fft_row_blocks", 65)
+    # the autotuner times the row pass's grids on whole pairs and keeps one of its candidates
+    lt.op.autotune(1)
+    assert lt.op.get_tuning("fft_row_blocks") in (0, 2, 3, 4)
 
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
