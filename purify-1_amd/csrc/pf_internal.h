@@ -357,6 +357,15 @@ struct TileLists {
   // dense tile does not form the launch's tail), for the tiles that own any, heaviest first
   const uint4* slot_own = nullptr;
   int n_own = 0;
+  // owner-tile gridding of dense stored rows (footprints of K <= tile side + 1): each degridding slot
+  // sums its visibilities' rows into its tile extended by the footprint's K - 1 columns and rows
+  // (ext_w x ext_h cells at ext + slot * ext_w * ext_h), and a combine pass adds, for each grid cell, the
+  // owner tile's slots, then the left, upper and upper-left neighbours' extensions, in that order.
+  // own_first[t] .. own_first[t + 1] index own_ids, tile t's slots in the order of their visibilities.
+  const int* own_first = nullptr;
+  const int* own_ids = nullptr;
+  void* ext = nullptr;
+  int ext_w = 0, ext_h = 0;
   // complex64 matrix-core lists: each list entry as one record (8 * sorted index, the byte offset of
   // its value; first-tap cell (k_u mod Nu) | (k_v mod Nv) << 16; the fractions u - k_u and v - k_v
   // as float bits)

@@ -765,6 +765,32 @@ void build(pf_op* op, const pf_op_desc& d) {
     if (!so.empty()) h2d(dso, so.data(), sizeof(uint4) * so.size());
     op->tl.slot_own = dso;
     op->tl.n_own = (int)so.size();
+    // owner-tile gridding of dense stored rows beyond 8 x 8 taps (up to 24 x 24), when the footprint's
+    // extension reaches the neighbouring tiles only: each tile's slots in the order of their
+    // visibilities, and the slots' extended partial tiles
+    const int EW = TU + op->Ku - 1, EH = TV + op->Kv - 1;
+    const size_t ext_bytes = op->esz() * (size_t)EW * EH * so.size();
+    if (op->w_term && op->Ku * op->Kv > 64 && op->Ku * op->Kv <= 576 && op->Ku - 1 <= TU && op->Kv - 1 <= TV &&
+        op->Nu % TU == 0 && op->Nv % TV == 0 && op->esz() * (size_t)EW * EH <= 160 * 1024 &&
+        ext_bytes <= ((size_t)8 << 30)) {
+      std::vector<int> first(op->n_tiles + 1, 0), ids(so.size());
+      for (const uint4& q : so) ++first[q.x + 1];
+      for (int t = 0; t < op->n_tiles; ++t) first[t + 1] += first[t];
+      std::vector<int> fill(first.begin(), first.end() - 1);
+      std::vector<int> order(so.size());
+      for (size_t k = 0; k < so.size(); ++k) order[k] = (int)k;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return so[a].y < so[b].y; });
+      for (int k : order) ids[fill[so[k].x]++] = k;
+      int* d_first = (int*)op->dalloc(sizeof(int) * first.size());
+      int* d_ids = (int*)op->dalloc(sizeof(int) * std::max<size_t>(ids.size(), 1));
+      h2d(d_first, first.data(), sizeof(int) * first.size());
+      if (!ids.empty()) h2d(d_ids, ids.data(), sizeof(int) * ids.size());
+      op->tl.own_first = d_first;
+      op->tl.own_ids = d_ids;
+      op->tl.ext = op->dalloc(std::max<size_t>(ext_bytes, 1));
+      op->tl.ext_w = EW;
+      op->tl.ext_h = EH;
+    }
   }
   // gridding lists of the TU x TV tiles (split for the stored-row gridding)
   op->tile_list_len = device_lists(op, d_kk, M, op->Ku, op->Kv, TU, TV, op->tl, op->w_term != 0);

@@ -925,6 +925,122 @@ __global__ void __launch_bounds__(64) k_grid_wrows_dense(int Nu, int Nv, WRows<R
   }
 }
 
+// Gridding with dense rows by owner tile: one wave per degridding slot (a chunk of the visibilities whose
+// first tap lies in its tile, in their sorted order), each row read once and summed into the tile extended
+// by K - 1 columns and rows in LDS, with k_grid_wrows_dense's walk (lane l the taps l + 64 t, the rows of a
+// D-entry ring requested ahead, entries broadcast by readlane); the extended tile goes to the slot's
+// partial sums for k_combine_own.  A list-based walk reads a row once per tile its footprint touches
+// (1.72 times for 11 x 11 rows in 32 x 32 tiles); this reads it once.
+template <typename R, int D, int TPL>
+__global__ void __launch_bounds__(64) k_grid_wrows_own(int Nu, int Nv, WRows<R> w, TileLists tl,
+                                                       const cpx<R>* __restrict__ ys, R scl) {
+  static_assert(64 % D == 0 && D <= 32, "the ring refills from the next batch after 64 - D entries");
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* acc = reinterpret_cast<cpx<R>*>(smem_raw);
+  const uint4 sl = tl.slot_own[blockIdx.x];
+  const int tile = (int)sl.x;
+  const int EW = tl.ext_w, EE = tl.ext_w * tl.ext_h;
+  const int tv = tile / tl.ntu, tu = tile - tv * tl.ntu;
+  const int tx0 = tu * tl.TU, ty0 = tv * tl.TV;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < EE; e += 64) acc[e] = mk<R>(R(0), R(0));
+  __syncthreads();
+  const uint32_t beg = sl.y, end = sl.z;
+  if (beg < end) {
+    const int KK = w.Ku * w.Kv;
+    int lofs[TPL], kix[TPL];
+    bool has[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      const int k = lane + 64 * t;
+      has[t] = k < KK;
+      kix[t] = min(k, KK - 1);
+      const int dv = kix[t] / w.Ku, du = kix[t] - dv * w.Ku;
+      lofs[t] = (dv * EW + du) * (int)sizeof(cpx<R>);
+    }
+    struct Ent {
+      uint32_t i, off;  // sorted index, LDS byte offset of the footprint origin in the extended tile
+      cpx<R> s;
+    };
+    auto load_ent = [&](uint32_t i) {
+      Ent e;
+      const int2 k = w.kw[i];
+      // the first tap lies in the tile: origin offsets in [0, TU) x [0, TV)
+      const int dx = (k.x + 1 - tx0) & (Nu - 1), dy = (k.y + 1 - ty0) & (Nv - 1);
+      e.i = i;
+      e.off = (uint32_t)((dy * EW + dx) * (int)sizeof(cpx<R>));
+      e.s = scale(ys[i], scl);
+      return e;
+    };
+    auto fetch = [&](const Ent& e, int j, cpx<R> (&val)[TPL]) {
+      const cpx<R>* row = w.vals + (int64_t)rl(e.i, j) * KK;
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) val[t] = row[kix[t]];
+    };
+    Ent cur = load_ent(min(beg + (uint32_t)lane, end - 1));
+    cpx<R> ring[D][TPL];
+#pragma unroll
+    for (int d = 0; d < D; ++d) fetch(cur, d, ring[d]);
+    for (uint32_t b0 = beg; b0 < end; b0 += 64) {
+      const uint32_t nb = b0 + 64 < end ? b0 + 64 : b0;
+      const uint32_t nidx = min(nb + (uint32_t)lane, end - 1);
+      Ent nxt;
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        if (j == 16) nxt = load_ent(nidx);
+        cpx<R> a[TPL];
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) a[t] = ring[j % D][t];
+        if (j + D < 64) fetch(cur, j + D, ring[j % D]);
+        else fetch(nxt, j + D - 64, ring[j % D]);
+        const bool live = b0 + (uint32_t)j < end;
+        const int off = (int)rl(cur.off, j);
+        const cpx<R> sv = mk<R>(rl(cur.s.x, j), rl(cur.s.y, j));
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+          if (live && has[t]) {
+            cpx<R>* c = reinterpret_cast<cpx<R>*>(smem_raw + off + lofs[t]);
+            *c = mac_conj(*c, a[t], sv);
+          }
+        }
+      }
+      cur = nxt;
+    }
+  }
+  __syncthreads();
+  cpx<R>* out = static_cast<cpx<R>*>(tl.ext) + (int64_t)blockIdx.x * EE;
+  for (int e = lane; e < EE; e += 64) out[e] = acc[e];
+}
+
+// Each grid cell of the owner-tile gridding: its tile's slots (in the order of their visibilities), then
+// the extensions reaching it from the left, upper and upper-left tiles (periodic grid), in that order,
+// so the result is the same on every run.
+template <typename R>
+__global__ void __launch_bounds__(256) k_combine_own(int Nu, int Nv, int Ku, int Kv, TileLists tl,
+                                                     cpx<R>* __restrict__ grid) {
+  const int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (cell >= (int64_t)Nu * Nv) return;
+  const int y = (int)(cell / Nu), x = (int)(cell - (int64_t)y * Nu);
+  const int TU = tl.TU, TV = tl.TV, EW = tl.ext_w, EE = tl.ext_w * tl.ext_h;
+  const int tu = x / TU, tv = y / TV, lx = x - tu * TU, ly = y - tv * TV;
+  const cpx<R>* ext = static_cast<const cpx<R>*>(tl.ext);
+  cpx<R> sum = mk<R>(R(0), R(0));
+  auto add = [&](int ttu, int ttv, int ex, int ey) {
+    const int t = ttv * tl.ntu + ttu;
+    for (int k = tl.own_first[t]; k < tl.own_first[t + 1]; ++k) {
+      const cpx<R> v = ext[(int64_t)tl.own_ids[k] * EE + ey * EW + ex];
+      sum.x += v.x;
+      sum.y += v.y;
+    }
+  };
+  const int lu = tu > 0 ? tu - 1 : tl.ntu - 1, uv = tv > 0 ? tv - 1 : tl.ntv - 1;
+  add(tu, tv, lx, ly);
+  if (lx < Ku - 1) add(lu, tv, lx + TU, ly);
+  if (ly < Kv - 1) add(tu, uv, lx, ly + TV);
+  if (lx < Ku - 1 && ly < Kv - 1) add(lu, uv, lx + TU, ly + TV);
+  grid[cell] = sum;
+}
+
 // The split tiles: each cell sums its parts' partial sums in part order (the list order of the parts'
 // ranges), so the result is the same on every run.
 template <typename R>
@@ -1036,6 +1152,22 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
       else PF_GWP(16, false);
     }
 #undef PF_GWP
+  } else if (t.ext && KK > 64 && KK <= 576) {
+    // dense rows by owner tile, then the combine pass
+    const size_t she = sizeof(cpx<R>) * (size_t)t.ext_w * t.ext_h;
+    if (she > 160 * 1024) throw Error(PF_ERR_UNSUPPORTED, "owner-tile gridding tile exceeds 160 KiB of LDS");
+    const dim3 ob((unsigned)t.n_own);
+    constexpr bool F = sizeof(R) == 4;
+    if (t.n_own > 0) {
+      if (KK <= 128) launch_lds(k_grid_wrows_own<R, F ? 16 : 8, 2>, ob, dim3(64), she, s, Nu, Nv, w, t, ys, scale);
+      else if (KK <= 256) launch_lds(k_grid_wrows_own<R, F ? 8 : 4, 4>, ob, dim3(64), she, s, Nu, Nv, w, t, ys, scale);
+      else launch_lds(k_grid_wrows_own<R, F ? 4 : 2, 9>, ob, dim3(64), she, s, Nu, Nv, w, t, ys, scale);
+    }
+    const int64_t cells = (int64_t)Nu * Nv;
+    hipLaunchKernelGGL(k_combine_own<R>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, Nu, Nv, w.Ku, w.Kv, t,
+                       grid);
+    PF_HIP(hipGetLastError());
+    return;
   } else if (KK <= 64)
     launch_lds(k_grid_wrows_dense<R, 16, 1>, blocks, dim3(64), shm, s, Nu, Nv, w, t, ys, grid, scale);
   else if (KK <= 128)
