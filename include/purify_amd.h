@@ -318,7 +318,11 @@ int pf_kernel_eval(int32_t kind, int32_t which, int32_t J, double os, int32_t ft
  * takes, 1, 2 or 4, each over 64 / entries lanes that take its kept values in stored order and with its own
  * accumulator copy of the tile, the copies summed in order at the end; allowed while the operator's largest
  * row keeps at most 64 / entries taps and the copies fit 160 KiB of LDS; the default is the most allowed
- * within 16 KiB of copies; 0 without kept-entry rows); "w_grid_depth" (gridding with kept-entry
+ * within 16 KiB of copies; 0 without kept-entry rows); "w_grid_owner" (dense stored rows of 65 to 576
+ * taps whose K - 1 fits in the tile: 1 = each row read once into its owner tile extended by K - 1, then
+ * a combine pass adds each cell's owner slots and the left, upper and upper-left extensions in that
+ * order, the default; 0 = the tile lists, a row read for every tile it touches; equal to rounding; 0 for
+ * other operators); "w_grid_depth" (gridding with kept-entry
  * rows at one entry per step: the entries whose tap values are requested ahead of their use, 8, 16 (the
  * default) or 32); read-only
  * "w_degrid_slots" (kept-entry rows: the kept values each degridding lane takes, 2, 4 or 8, the fewest that

@@ -413,7 +413,8 @@ struct WRows {
   const uint4* rec = nullptr;
   const uint8_t* pos = nullptr;  // kept-entry rows: each kept value's tap, jv << 4 | ju (beside vals)
   int slots = 8;                 // kept-entry degridding: slots per lane (8 lanes per row): 2, 4 or 8
-  int qe = 1;                    // kept-entry gridding: entries per step (1: one lane per tap position)
+  int qe = 1;
+  bool own = true;  // dense rows: owner-tile gridding where the tile lists carry its partial sums                    // kept-entry gridding: entries per step (1: one lane per tap position)
   int depth = 16;      // kept-entry gridding: entries whose taps are requested ahead (8, 16 or 32)
   bool off64 = false;  // kept-entry offsets reach 2^32 (else their high words are zero)
 };

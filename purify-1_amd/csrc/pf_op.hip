@@ -337,6 +337,7 @@ struct pf_op {
   uint8_t* d_wpos = nullptr;  // each kept value's tap (jv << 4 | ju)
   int w_slots = 8;            // kept-entry degridding's slots per lane (the largest row's kept taps / 8)
   int w_qe = 1, w_qe_max = 1; // kept-entry gridding's entries per step (tuning knob), and the largest allowed
+  bool w_owner = true;         // dense rows: owner-tile gridding where built (tuning knob)
   int w_packed = 0;
   int w_grid_depth = 16;      // tuning knob: kept-entry gridding's prefetch depth in entries
   int64_t nnz_w = 0;
@@ -1049,6 +1050,7 @@ WRows<R> wrows(const pf_op* op) {
   w.Ku = op->Ku;
   w.Kv = op->Kv;
   w.kw = op->d_kk;
+  w.own = op->w_owner;
   if (op->w_packed) {
     w.vals = static_cast<const cpx<R>*>(op->d_wpack);
     w.rec = op->d_wrec;
@@ -1889,6 +1891,13 @@ static int set_tuning_impl(pf_op* op, const char* key, int64_t value) {
     op->gf.poly = value ? op->d_poly : nullptr;
     return PF_OK;
   }
+  if (k == "w_grid_owner") {
+    if (value != 0 && value != 1) return fail(PF_ERR_ARG, "w_grid_owner must be 0 or 1");
+    if (value && !op->tl.ext)
+      return fail(PF_ERR_UNSUPPORTED, "owner-tile gridding needs dense stored rows of 65 to 576 taps");
+    op->w_owner = value != 0;
+    return PF_OK;
+  }
   if (k == "w_grid_entries") {
     if (value != 1 && value != 2 && value != 4) return fail(PF_ERR_ARG, "w_grid_entries must be 1, 2 or 4");
     if (value > 1 && (!op->d_wpack || value > op->w_qe_max))
@@ -1981,6 +1990,10 @@ int pf_op_get_tuning(const pf_op* op, const char* key, int64_t* value) {
   }
   if (k == "w_grid_depth") {
     *value = op->w_grid_depth;
+    return PF_OK;
+  }
+  if (k == "w_grid_owner") {
+    *value = op->tl.ext && op->w_owner ? 1 : 0;
     return PF_OK;
   }
   if (k == "w_grid_entries") {

@@ -1152,7 +1152,7 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
       else PF_GWP(16, false);
     }
 #undef PF_GWP
-  } else if (t.ext && KK > 64 && KK <= 576) {
+  } else if (w.own && t.ext && KK > 64 && KK <= 576) {
     // dense rows by owner tile, then the combine pass
     const size_t she = sizeof(cpx<R>) * (size_t)t.ext_w * t.ext_h;
     if (she > 160 * 1024) throw Error(PF_ERR_UNSUPPORTED, "owner-tile gridding tile exceeds 160 KiB of LDS");

@@ -129,3 +129,36 @@ def test_wprojection_construction_errors():
         measurementoperator.init_degrid_operator_2d(u, u, u, None, 4, 4, 2.0, 0, 1e-4, "kb", 6, 6, True, 6)
     with pytest.raises(RuntimeError, match="Size of u and w"):
         measurementoperator.init_degrid_operator_2d(u, u, np.zeros(3), None, 16, 16, 2.0, 0, 1e-4, "kb", 4, 4, True, 6)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_owner_tile_gridding_against_the_tile_lists(precision):
+    # 11 x 11 rows: each row summed once into its owner tile extended by K - 1, the extensions combined
+    # in a fixed order; the tile-list walk sums the same products in another order.  Visibilities crowd
+    # the centre (owner tiles cut into several slots) and reach the grid's edges (extensions wrap)
+    from purify_amd._lib import PurifyError
+    n, M = 64, 20_000
+    u, v = make_uv(341, M, n, sigma=0.6)
+    u[:4] = utilities.radians_to_pixels(np.array([np.pi * 0.999, -np.pi, 0.0, np.pi * 0.5]), n, 2.0)
+    v[:4] = utilities.radians_to_pixels(np.array([-np.pi, np.pi * 0.999, np.pi * 0.999, -np.pi * 0.999]), n, 2.0)
+    w = w_uniform(342, M)
+    wt = utilities.synth_cnormal(343, M) + 2.0
+    lt = measurementoperator.init_degrid_operator_2d(u, v, w, wt, n, n, 2.0, 0, 1e-4, "kb", 6, 6, True, 6,
+                                                     3600.0 / n, 3600.0 / n, precision=precision)
+    assert lt.op.get_tuning("w_footprint") == 11 and lt.op.get_tuning("w_grid_owner") == 1
+    y = utilities.synth_cnormal(344, M)
+    a = lt.adjoint() * y
+    assert np.array_equal(a, lt.adjoint() * y)
+    lt.op.set_tuning("w_grid_owner", 0)
+    assert lt.op.get_tuning("w_grid_owner") == 0
+    b = lt.adjoint() * y
+    assert rel(a, b) < TOL[precision]
+    lt.op.set_tuning("w_grid_owner", 1)
+    assert np.array_equal(a, lt.adjoint() * y)
+    with pytest.raises(PurifyError, match="0 or 1"):
+        lt.op.set_tuning("w_grid_owner", 2)
+    plain = measurementoperator.init_degrid_operator_2d(u, v, None, None, n, n, 2.0, 0, 1e-4, "kb", 4, 4,
+                                                        precision=precision)
+    assert plain.op.get_tuning("w_grid_owner") == 0
+    with pytest.raises(PurifyError, match="owner-tile"):
+        plain.op.set_tuning("w_grid_owner", 1)
