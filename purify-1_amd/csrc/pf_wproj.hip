@@ -1014,31 +1014,32 @@ __global__ void __launch_bounds__(64) k_grid_wrows_own(int Nu, int Nv, WRows<R> 
 
 // Each grid cell of the owner-tile gridding: its tile's slots (in the order of their visibilities), then
 // the extensions reaching it from the left, upper and upper-left tiles (periodic grid), in that order,
-// so the result is the same on every run.
+// so the result is the same on every run.  One block per tile, so that the four tiles' slot ranges are
+// block-uniform and each cell's partial sums are independent loads.
 template <typename R>
-__global__ void __launch_bounds__(256) k_combine_own(int Nu, int Nv, int Ku, int Kv, TileLists tl,
-                                                     cpx<R>* __restrict__ grid) {
-  const int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (cell >= (int64_t)Nu * Nv) return;
-  const int y = (int)(cell / Nu), x = (int)(cell - (int64_t)y * Nu);
+__global__ void __launch_bounds__(256) k_combine_own(int Nu, int Ku, int Kv, TileLists tl, cpx<R>* __restrict__ grid) {
+  const int t = (int)blockIdx.x;
   const int TU = tl.TU, TV = tl.TV, EW = tl.ext_w, EE = tl.ext_w * tl.ext_h;
-  const int tu = x / TU, tv = y / TV, lx = x - tu * TU, ly = y - tv * TV;
+  const int tv = t / tl.ntu, tu = t - tv * tl.ntu;
+  const int lu = tu > 0 ? tu - 1 : tl.ntu - 1, uv = tv > 0 ? tv - 1 : tl.ntv - 1;
+  const int tl_ = tv * tl.ntu + lu, tup = uv * tl.ntu + tu, tul = uv * tl.ntu + lu;
   const cpx<R>* ext = static_cast<const cpx<R>*>(tl.ext);
-  cpx<R> sum = mk<R>(R(0), R(0));
-  auto add = [&](int ttu, int ttv, int ex, int ey) {
-    const int t = ttv * tl.ntu + ttu;
-    for (int k = tl.own_first[t]; k < tl.own_first[t + 1]; ++k) {
-      const cpx<R> v = ext[(int64_t)tl.own_ids[k] * EE + ey * EW + ex];
+  auto add = [&](cpx<R>& sum, int tile, int off) {
+    for (int k = tl.own_first[tile]; k < tl.own_first[tile + 1]; ++k) {
+      const cpx<R> v = ext[(int64_t)tl.own_ids[k] * EE + off];
       sum.x += v.x;
       sum.y += v.y;
     }
   };
-  const int lu = tu > 0 ? tu - 1 : tl.ntu - 1, uv = tv > 0 ? tv - 1 : tl.ntv - 1;
-  add(tu, tv, lx, ly);
-  if (lx < Ku - 1) add(lu, tv, lx + TU, ly);
-  if (ly < Kv - 1) add(tu, uv, lx, ly + TV);
-  if (lx < Ku - 1 && ly < Kv - 1) add(lu, uv, lx + TU, ly + TV);
-  grid[cell] = sum;
+  for (int e = threadIdx.x; e < TU * TV; e += 256) {
+    const int ly = e / TU, lx = e - ly * TU;
+    cpx<R> sum = mk<R>(R(0), R(0));
+    add(sum, t, ly * EW + lx);
+    if (lx < Ku - 1) add(sum, tl_, ly * EW + lx + TU);
+    if (ly < Kv - 1) add(sum, tup, (ly + TV) * EW + lx);
+    if (lx < Ku - 1 && ly < Kv - 1) add(sum, tul, (ly + TV) * EW + lx + TU);
+    grid[(int64_t)(tv * TV + ly) * Nu + tu * TU + lx] = sum;
+  }
 }
 
 // The split tiles: each cell sums its parts' partial sums in part order (the list order of the parts'
@@ -1163,9 +1164,7 @@ void grid_wrows(int Nu, int Nv, const WRows<R>& w, const TileLists& t, const cpx
       else if (KK <= 256) launch_lds(k_grid_wrows_own<R, F ? 8 : 4, 4>, ob, dim3(64), she, s, Nu, Nv, w, t, ys, scale);
       else launch_lds(k_grid_wrows_own<R, F ? 4 : 2, 9>, ob, dim3(64), she, s, Nu, Nv, w, t, ys, scale);
     }
-    const int64_t cells = (int64_t)Nu * Nv;
-    hipLaunchKernelGGL(k_combine_own<R>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, Nu, Nv, w.Ku, w.Kv, t,
-                       grid);
+    hipLaunchKernelGGL(k_combine_own<R>, dim3((unsigned)(t.ntu * t.ntv)), dim3(256), 0, s, Nu, w.Ku, w.Kv, t, grid);
     PF_HIP(hipGetLastError());
     return;
   } else if (KK <= 64)
