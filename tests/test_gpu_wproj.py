@@ -132,7 +132,8 @@ def test_wprojection_construction_errors():
 
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
-def test_owner_tile_gridding_against_the_tile_lists(precision):
+@pytest.mark.parametrize("tile", [None, (64, 32), (32, 16)])
+def test_owner_tile_gridding_against_the_tile_lists(precision, tile):
     # 11 x 11 rows: each row summed once into its owner tile extended by K - 1, the extensions combined
     # in a fixed order; the tile-list walk sums the same products in another order.  Visibilities crowd
     # the centre (owner tiles cut into several slots) and reach the grid's edges (extensions wrap)
@@ -143,8 +144,11 @@ def test_owner_tile_gridding_against_the_tile_lists(precision):
     v[:4] = utilities.radians_to_pixels(np.array([-np.pi, np.pi * 0.999, np.pi * 0.999, -np.pi * 0.999]), n, 2.0)
     w = w_uniform(342, M)
     wt = utilities.synth_cnormal(343, M) + 2.0
+    kw = {"tile": tile} if tile else {}
     lt = measurementoperator.init_degrid_operator_2d(u, v, w, wt, n, n, 2.0, 0, 1e-4, "kb", 6, 6, True, 6,
-                                                     3600.0 / n, 3600.0 / n, precision=precision)
+                                                     3600.0 / n, 3600.0 / n, precision=precision, **kw)
+    if tile:
+        assert (lt.op.get_tuning("tile_u"), lt.op.get_tuning("tile_v")) == tile
     assert lt.op.get_tuning("w_footprint") == 11 and lt.op.get_tuning("w_grid_owner") == 1
     y = utilities.synth_cnormal(344, M)
     a = lt.adjoint() * y
